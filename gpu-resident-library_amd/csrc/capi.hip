@@ -1,0 +1,244 @@
+// extern "C" handle API (include/gpufhe_capi.h) over the engine.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <functional>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/gpufhe_capi.h"
+#include "engine.hpp"
+
+namespace gpufhe {
+void native_primes(std::vector<uint64_t>&, uint32_t, uint32_t, uint32_t, const std::vector<uint64_t>&);
+uint64_t native_root(uint32_t, uint64_t);
+void native_rescale_consts(std::vector<uint64_t>&, std::vector<uint64_t>&, const std::vector<uint64_t>&,
+                           uint32_t);
+}  // namespace gpufhe
+
+using namespace gf;
+
+struct gf_ctx {
+    std::unique_ptr<Ctx> C;
+    std::mutex mu;
+    std::map<std::vector<uint64_t>, std::unique_ptr<Plan>> cache;  // single-op plans
+};
+struct gf_key {
+    gf_ctx* ctx;
+    std::unique_ptr<DevKey> K;
+};
+struct gf_plan {
+    std::unique_ptr<Plan> P;
+};
+
+static thread_local std::string g_err;
+template <class F>
+static int guard(F&& f) {
+    try {
+        f();
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+template <class T, class F>
+static T* guard_ptr(F&& f) {
+    try {
+        return f();
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return nullptr;
+    }
+}
+
+// single-op plans are cached by (op, arguments); building one uploads its job
+// arrays synchronously, which a stream capture cannot contain
+static Plan* cached_plan(gf_ctx* c, std::vector<uint64_t> key, hipStream_t s,
+                         const std::function<void(PlanBuilder&)>& build) {
+    std::lock_guard<std::mutex> lk(c->mu);
+    auto it = c->cache.find(key);
+    if (it != c->cache.end()) return it->second.get();
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    if (s && hipStreamIsCapturing(s, &st) == hipSuccess && st != hipStreamCaptureStatusNone)
+        throw std::runtime_error("gpufhe: first call of an operation with new arguments inside a stream capture; "
+                                 "run it once outside the capture first");
+    if (c->cache.size() > 4096) c->cache.clear();
+    PlanBuilder b(*c->C);
+    build(b);
+    auto P = b.finish();
+    Plan* raw = P.get();
+    c->cache.emplace(std::move(key), std::move(P));
+    return raw;
+}
+
+#define U(x) ((uint64_t)(uintptr_t)(x))
+
+extern "C" {
+
+const char* gf_last_error(void) { return g_err.c_str(); }
+
+int gf_malloc(void** p, size_t bytes) { return guard([&] { hip_check(hipMalloc(p, bytes), "hipMalloc"); }); }
+int gf_free(void* p) { return guard([&] { hip_check(hipFree(p), "hipFree"); }); }
+int gf_h2d(void* d, const void* h, size_t b) {
+    return guard([&] { hip_check(hipMemcpy(d, h, b, hipMemcpyHostToDevice), "H2D"); });
+}
+int gf_d2h(void* h, const void* d, size_t b) {
+    return guard([&] { hip_check(hipMemcpy(h, d, b, hipMemcpyDeviceToHost), "D2H"); });
+}
+int gf_d2d(void* d, const void* s, size_t b) {
+    return guard([&] { hip_check(hipMemcpy(d, s, b, hipMemcpyDeviceToDevice), "D2D"); });
+}
+int gf_device_sync(void) { return guard([&] { hip_check(hipDeviceSynchronize(), "sync"); }); }
+int gf_stream_create(hipStream_t* s) {
+    return guard([&] { hip_check(hipStreamCreateWithFlags(s, hipStreamNonBlocking), "stream"); });
+}
+int gf_stream_destroy(hipStream_t s) { return guard([&] { hip_check(hipStreamDestroy(s), "stream"); }); }
+int gf_stream_sync(hipStream_t s) { return guard([&] { hip_check(hipStreamSynchronize(s), "stream sync"); }); }
+
+int gf_native_primes(uint64_t* out, uint32_t count, uint32_t bits, uint32_t n, const uint64_t* avoid,
+                     uint32_t navoid) {
+    return guard([&] {
+        std::vector<uint64_t> o, a(avoid, avoid + navoid);
+        gpufhe::native_primes(o, count, bits, n, a);
+        std::memcpy(out, o.data(), o.size() * 8);
+    });
+}
+uint64_t gf_native_root(uint32_t n, uint64_t q) {
+    uint64_t r = 0;
+    guard([&] { r = gpufhe::native_root(n, q); });
+    return r;
+}
+void gf_native_rescale_consts(uint64_t* s1, uint64_t* s2, const uint64_t* mod, uint32_t last) {
+    std::vector<uint64_t> a, b, m(mod, mod + last + 1);
+    gpufhe::native_rescale_consts(a, b, m, last);
+    std::memcpy(s1, a.data(), last * 8);
+    std::memcpy(s2, b.data(), last * 8);
+}
+
+gf_ctx* gf_ctx_create(uint32_t n, const uint64_t* q, uint32_t L, const uint64_t* p, uint32_t K,
+                      const uint64_t* roots, uint32_t dnum) {
+    return guard_ptr<gf_ctx>([&] {
+        if (dnum == 0 || dnum > L) throw std::runtime_error("gpufhe: dnum must be in [1, L]");
+        auto c = new gf_ctx;
+        c->C = std::make_unique<Ctx>(n, std::vector<uint64_t>(q, q + L), std::vector<uint64_t>(p, p + K),
+                                     std::vector<uint64_t>(roots, roots + L + K), (L + dnum - 1) / dnum);
+        return c;
+    });
+}
+void gf_ctx_destroy(gf_ctx* c) { delete c; }
+size_t gf_ws_bytes(const gf_ctx* c) { return c->C->ws_words() * 8; }
+
+gf_key* gf_key_create(gf_ctx* c, const uint64_t* av, const uint64_t* bv, uint32_t nparts, hipStream_t s) {
+    return guard_ptr<gf_key>([&] {
+        auto k = new gf_key;
+        k->ctx = c;
+        k->K = upload_key(*c->C, av, bv, nparts, c->C->L + c->C->K, c->C->L, s);
+        return k;
+    });
+}
+void gf_key_destroy(gf_key* k) { delete k; }
+
+int gf_evalmult_rescale(gf_ctx* c, gf_key* k, uint32_t l, const uint64_t* c0a, const uint64_t* c1a,
+                        const uint64_t* c0b, const uint64_t* c1b, uint64_t* o0, uint64_t* o1, uint64_t* ws,
+                        hipStream_t s) {
+    return guard([&] {
+        Plan* P = cached_plan(c, {1, l, U(k), U(c0a), U(c1a), U(c0b), U(c1b), U(o0), U(o1), U(ws)}, s,
+                              [&](PlanBuilder& b) { b.evalmult_rescale(l, c0a, c1a, c0b, c1b, o0, o1, *k->K, ws); });
+        P->run(s);
+    });
+}
+int gf_evalmult(gf_ctx* c, gf_key* k, uint32_t l, const uint64_t* c0a, const uint64_t* c1a, const uint64_t* c0b,
+                const uint64_t* c1b, uint64_t* o0, uint64_t* o1, uint64_t* ws, hipStream_t s) {
+    return guard([&] {
+        Plan* P = cached_plan(c, {2, l, U(k), U(c0a), U(c1a), U(c0b), U(c1b), U(o0), U(o1), U(ws)}, s,
+                              [&](PlanBuilder& b) { b.evalmult(l, c0a, c1a, c0b, c1b, o0, o1, *k->K, ws); });
+        P->run(s);
+    });
+}
+int gf_keyswitch(gf_ctx* c, gf_key* k, uint32_t l, const uint64_t* d, uint64_t* ba0, uint64_t* ba1, uint64_t* ws,
+                 hipStream_t s) {
+    return guard([&] {
+        Plan* P = cached_plan(c, {3, l, U(k), U(d), U(ba0), U(ba1), U(ws)}, s,
+                              [&](PlanBuilder& b) { b.keyswitch(l, d, ba0, ba1, *k->K, ws); });
+        P->run(s);
+    });
+}
+int gf_rescale(gf_ctx* c, uint32_t l, const uint64_t* c0, const uint64_t* c1, uint64_t* o0, uint64_t* o1,
+               uint64_t* ws, hipStream_t s) {
+    return guard([&] {
+        Plan* P = cached_plan(c, {4, l, U(c0), U(c1), U(o0), U(o1), U(ws)}, s,
+                              [&](PlanBuilder& b) { b.rescale(l, c0, c1, o0, o1, ws); });
+        P->run(s);
+    });
+}
+int gf_rotate(gf_ctx* c, gf_key* k, uint32_t l, uint64_t kk, const uint64_t* c0, const uint64_t* c1, uint64_t* o0,
+              uint64_t* o1, uint64_t* ws, hipStream_t s) {
+    return guard([&] {
+        Plan* P = cached_plan(c, {5, l, U(k), kk, U(c0), U(c1), U(o0), U(o1), U(ws)}, s,
+                              [&](PlanBuilder& b) { b.rotate(l, kk, c0, c1, o0, o1, *k->K, ws); });
+        P->run(s);
+    });
+}
+int gf_add(gf_ctx* c, uint32_t l, const uint64_t* a0, const uint64_t* a1, const uint64_t* b0, const uint64_t* b1,
+           uint64_t* o0, uint64_t* o1, hipStream_t s) {
+    return guard([&] {
+        Plan* P = cached_plan(c, {6, l, U(a0), U(a1), U(b0), U(b1), U(o0), U(o1)}, s,
+                              [&](PlanBuilder& b) { b.add(l, a0, a1, b0, b1, o0, o1); });
+        P->run(s);
+    });
+}
+
+gf_plan* gf_plan_evalmult_rescale(gf_ctx* c, gf_key* k, uint32_t count, const uint32_t* levels,
+                                  const uint64_t* const* c0a, const uint64_t* const* c1a,
+                                  const uint64_t* const* c0b, const uint64_t* const* c1b, uint64_t* const* o0,
+                                  uint64_t* const* o1, uint64_t* const* ws) {
+    return guard_ptr<gf_plan>([&] {
+        PlanBuilder b(*c->C);
+        for (uint32_t i = 0; i < count; ++i)
+            b.evalmult_rescale(levels[i], c0a[i], c1a[i], c0b[i], c1b[i], o0[i], o1[i], *k->K, ws[i]);
+        auto p = new gf_plan;
+        p->P = b.finish();
+        return p;
+    });
+}
+gf_plan* gf_plan_rotate(gf_ctx* c, gf_key* const* rks, uint32_t count, const uint32_t* levels, const uint64_t* ks,
+                        const uint64_t* const* c0, const uint64_t* const* c1, uint64_t* const* o0,
+                        uint64_t* const* o1, uint64_t* const* ws) {
+    return guard_ptr<gf_plan>([&] {
+        PlanBuilder b(*c->C);
+        for (uint32_t i = 0; i < count; ++i) b.rotate(levels[i], ks[i], c0[i], c1[i], o0[i], o1[i], *rks[i]->K, ws[i]);
+        auto p = new gf_plan;
+        p->P = b.finish();
+        return p;
+    });
+}
+gf_plan* gf_plan_add(gf_ctx* c, uint32_t count, const uint32_t* levels, const uint64_t* const* a0,
+                     const uint64_t* const* a1, const uint64_t* const* b0, const uint64_t* const* b1,
+                     uint64_t* const* o0, uint64_t* const* o1) {
+    return guard_ptr<gf_plan>([&] {
+        PlanBuilder b(*c->C);
+        for (uint32_t i = 0; i < count; ++i) b.add(levels[i], a0[i], a1[i], b0[i], b1[i], o0[i], o1[i]);
+        auto p = new gf_plan;
+        p->P = b.finish();
+        return p;
+    });
+}
+int gf_plan_run(gf_plan* p, hipStream_t s) { return guard([&] { p->P->run(s); }); }
+int gf_plan_run_graph(gf_plan* p, hipStream_t s) {
+    return guard([&] {
+        if (!s) throw std::runtime_error("gpufhe: graph capture needs a non-null stream");
+        p->P->run_graph(s);
+    });
+}
+uint32_t gf_plan_launches(const gf_plan* p) {
+    uint32_t n = 0;
+    for (const Step& s : p->P->steps)
+        n += (s.kind == Step::NTT_F || s.kind == Step::NTT_I) ? (p->P->geom.single ? 1 : 2) : 1;
+    return n;
+}
+void gf_plan_destroy(gf_plan* p) { delete p; }
+
+}  // extern "C"

@@ -1,0 +1,473 @@
+// Host-side engine implementation (see engine.hpp).
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <map>
+#include <tuple>
+
+#include "engine.hpp"
+
+namespace gf {
+
+// ------------------------------------------------------------ root tables --
+// fr[i] = psi^brv(i), ir[i] = psi^-brv(i) with Shoup precons, n^-1: the table
+// layout of the reference (src/keyswitch.cpp:22-35).  Tables depend only on
+// (n, q, psi); they are built once per process and shared by every context
+// (the reference caches them the same way, src/keyswitch_resident.cpp:47-68).
+RootTable root_table(uint32_t n, uint64_t q, uint64_t root) {
+    static std::mutex mu;
+    static std::map<std::tuple<uint32_t, uint64_t, uint64_t>, RootTable> cache;
+    std::lock_guard<std::mutex> lk(mu);
+    auto key = std::make_tuple(n, q, root);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    uint32_t logn = 0;
+    while ((1u << logn) < n) ++logn;
+    std::vector<uint64_t> h((size_t)4 * n), pw(n);
+    const uint64_t bases[2] = {root, host_invmod(root, q)};
+    for (int s = 0; s < 2; ++s) {
+        pw[0] = 1;
+        for (uint32_t i = 1; i < n; ++i) pw[i] = host_mulmod(pw[i - 1], bases[s], q);
+        uint64_t* tr = h.data() + (size_t)(2 * s) * n;
+        uint64_t* tp = tr + n;
+        for (uint32_t i = 0; i < n; ++i) {
+            uint32_t r = 0, v = i;
+            for (uint32_t k = 0; k < logn; ++k) { r = (r << 1) | (v & 1); v >>= 1; }
+            tr[i] = pw[r];
+            tp[i] = host_precon(pw[r], q);
+        }
+    }
+    void* d = nullptr;
+    hip_check(hipMalloc(&d, h.size() * 8), "hipMalloc root table");
+    hip_check(hipMemcpy(d, h.data(), h.size() * 8, hipMemcpyHostToDevice), "upload root table");
+    const uint64_t* b = (const uint64_t*)d;
+    RootTable t;
+    t.fr = b;
+    t.fp = b + n;
+    t.ir = b + 2 * (size_t)n;
+    t.ip = b + 3 * (size_t)n;
+    t.ninv = host_invmod(n, q);
+    t.ninvp = host_precon(t.ninv, q);
+    cache.emplace(key, t);
+    return t;
+}
+
+static uint64_t prod_mod(const std::vector<uint64_t>& mods, size_t b, size_t e, uint64_t q,
+                         size_t skip) {
+    uint64_t r = 1 % q;
+    for (size_t k = b; k < e; ++k)
+        if (k != skip) r = host_mulmod(r, mods[k] % q, q);
+    return r;
+}
+
+// ------------------------------------------------------------------- Ctx --
+Ctx::Ctx(uint32_t n_, const std::vector<uint64_t>& q, const std::vector<uint64_t>& p,
+         const std::vector<uint64_t>& roots_, uint32_t alpha_)
+    : n(n_), L((uint32_t)q.size()), K((uint32_t)p.size()), alpha(alpha_) {
+    if (n < 2 || (n & (n - 1)) || n > (1u << 17)) throw std::runtime_error("gpufhe: n must be a power of two in [2, 2^17]");
+    if (L == 0 || K == 0) throw std::runtime_error("gpufhe: need at least one Q and one P modulus");
+    if (roots_.size() != L + K) throw std::runtime_error("gpufhe: need one root per modulus");
+    if (alpha == 0 || alpha > L) throw std::runtime_error("gpufhe: bad part width");
+    if (alpha > CONV_MAX_SRC || K > CONV_MAX_SRC)
+        throw std::runtime_error("gpufhe: part width and P tower count must be <= 16");
+    logn = 0;
+    while ((1u << logn) < n) ++logn;
+    geom = make_geom(n);
+    dnum = (L + alpha - 1) / alpha;
+    if (dnum > MAXP) throw std::runtime_error("gpufhe: too many digits");
+    mods = q;
+    mods.insert(mods.end(), p.begin(), p.end());
+    roots = roots_;
+    uint64_t qmax = 0;
+    for (uint64_t m : mods) {
+        if (m < 3 || (m >> 61)) throw std::runtime_error("gpufhe: moduli must be odd and < 2^61");
+        if ((m - 1) % (2ull * n)) throw std::runtime_error("gpufhe: modulus is not 1 mod 2n");
+        qmax = m > qmax ? m : qmax;
+    }
+    // one REDC of a sum of up to max(alpha, K, dnum) products must stay below q 2^64
+    const uint32_t terms = alpha > K ? (alpha > dnum ? alpha : dnum) : (K > dnum ? K : dnum);
+    if ((unsigned __int128)terms * qmax >= ((unsigned __int128)1 << 64))
+        throw std::runtime_error("gpufhe: moduli too large for the single-REDC accumulators");
+    rc.resize(L + K);
+    rt.resize(L + K);
+    for (uint32_t r = 0; r < L + K; ++r) {
+        const uint64_t m = mods[r];
+        rt[r] = root_table(n, m, roots[r]);
+        rc[r] = RowConst{m, host_qneg(m), host_r2(m), host_r1(m), rt[r].ninv, rt[r].ninvp};
+    }
+
+    // per-level constants; device tables go into one buffer
+    std::vector<uint64_t> blob;
+    struct Fix { const uint64_t** dst; size_t off; };
+    std::vector<Fix> fix;
+    auto put = [&](const std::vector<uint64_t>& v, const uint64_t** dst) {
+        fix.push_back({dst, blob.size()});
+        blob.insert(blob.end(), v.begin(), v.end());
+    };
+    lv.resize(L + 1);
+    for (uint32_t l = 1; l <= L; ++l) {
+        LevelConst& c = lv[l];
+        c.l = l;
+        c.np = num_parts(l);
+        c.parts.resize(c.np);
+        for (uint32_t pi = 0; pi < c.np; ++pi) {
+            PartConst& P = c.parts[pi];
+            P.st = alpha * pi;
+            P.sp = (l > P.st + alpha) ? alpha : l - P.st;
+            for (uint32_t i = 0; i < l; ++i)
+                if (i < P.st || i >= P.st + P.sp) P.compl_rows.push_back(row_q(i));
+            for (uint32_t k = 0; k < K; ++k) P.compl_rows.push_back(row_p(k));
+            P.sc = (uint32_t)P.compl_rows.size();
+            std::vector<uint64_t> tab((size_t)P.sp * P.sc), tq(P.sc), tqn(P.sc);
+            for (uint32_t i = 0; i < P.sp; ++i) {
+                const uint64_t qi = mods[P.st + i];
+                const uint64_t qhat = prod_mod(mods, P.st, P.st + P.sp, qi, P.st + i);
+                const uint64_t v = host_mulmod(host_invmod(qhat, qi), rt[P.st + i].ninv, qi);
+                P.post.push_back(v);
+                P.postp.push_back(host_precon(v, qi));
+                for (uint32_t j = 0; j < P.sc; ++j) {
+                    const uint64_t t = mods[P.compl_rows[j]];
+                    tab[(size_t)i * P.sc + j] =
+                        host_mulmod(prod_mod(mods, P.st, P.st + P.sp, t, P.st + i), host_r1(t), t);
+                }
+            }
+            for (uint32_t j = 0; j < P.sc; ++j) {
+                tq[j] = mods[P.compl_rows[j]];
+                tqn[j] = host_qneg(tq[j]);
+            }
+            put(tab, &P.d_tab);
+            put(tq, &P.d_tq);
+            put(tqn, &P.d_tqneg);
+        }
+        // ModDown P -> Q_l
+        std::vector<uint64_t> tab((size_t)K * l), tq(l), tqn(l);
+        for (uint32_t k = 0; k < K; ++k) {
+            const uint64_t pk = mods[L + k];
+            const uint64_t v = host_mulmod(host_invmod(prod_mod(mods, L, L + K, pk, L + k), pk), rt[L + k].ninv, pk);
+            c.pd_post.push_back(v);
+            c.pd_postp.push_back(host_precon(v, pk));
+            for (uint32_t i = 0; i < l; ++i)
+                tab[(size_t)k * l + i] = host_mulmod(prod_mod(mods, L, L + K, mods[i], L + k), host_r1(mods[i]), mods[i]);
+        }
+        for (uint32_t i = 0; i < l; ++i) {
+            tq[i] = mods[i];
+            tqn[i] = host_qneg(mods[i]);
+            const uint64_t pinv = host_invmod(prod_mod(mods, L, L + K, mods[i], (size_t)-1), mods[i]);
+            c.pinv.push_back(pinv);
+            c.pinvp.push_back(host_precon(pinv, mods[i]));
+        }
+        put(tab, &c.d_tabMD);
+        put(tq, &c.d_tqMD);
+        put(tqn, &c.d_tqnegMD);
+        // rescale dropping tower l-1: s1 = q_{l-1}^-1 mod q_i, s2 = -s1 mod q_i
+        // (s2 = floor(Q' [Q'^-1]_{q_l} / q_l) mod q_i is congruent to -s1, src/params.cpp:50-75)
+        if (l >= 2) {
+            const uint64_t ql = mods[l - 1];
+            for (uint32_t i = 0; i + 1 < l; ++i) {
+                const uint64_t s1 = host_invmod(ql % mods[i], mods[i]);
+                const uint64_t s2 = s1 ? mods[i] - s1 : 0;
+                c.s1.push_back(s1);
+                c.s1p.push_back(host_precon(s1, mods[i]));
+                c.s2.push_back(s2);
+                c.s2p.push_back(host_precon(s2, mods[i]));
+            }
+        }
+    }
+    consts.alloc(blob.size() * 8);
+    hip_check(hipMemcpy(consts.p, blob.data(), blob.size() * 8, hipMemcpyHostToDevice), "upload constants");
+    for (auto& f : fix) *f.dst = consts.as<uint64_t>() + f.off;
+}
+
+WsLayout WsLayout::make(const Ctx& C) {
+    WsLayout w{};
+    const size_t L = C.L, K = C.K, W = L + K;
+    w.t0 = 0;
+    w.t1 = L;
+    w.t2 = 2 * L;
+    w.x = 3 * L;                         // 2L: INTT'd part towers, later the ModDown conversions
+    w.dig = 5 * L;
+    size_t digsz = (size_t)C.dnum * W;
+    if (digsz < 2 * (L - 1)) digsz = 2 * (L - 1);   // also holds the rescale's switched towers
+    w.res0 = w.dig + digsz;
+    w.res1 = w.res0 + W;
+    w.xp = w.res1 + W;
+    w.drop = w.xp + 2 * K;
+    w.total = w.drop + 2;
+    return w;
+}
+size_t Ctx::ws_words() const { return WsLayout::make(*this).total * (size_t)n; }
+
+// ------------------------------------------------------------------ keys --
+std::unique_ptr<DevKey> upload_key(const Ctx& C, const uint64_t* av, const uint64_t* bv,
+                                   uint32_t nparts, uint32_t rows, uint32_t fullQ, hipStream_t s) {
+    if (fullQ < C.L || rows != fullQ + C.K) throw std::runtime_error("gpufhe: key row layout does not match the context");
+    if (nparts < C.dnum) throw std::runtime_error("gpufhe: key has fewer parts than the context needs");
+    auto k = std::make_unique<DevKey>();
+    k->nparts = nparts;
+    k->rows = rows;
+    k->fullQ = fullQ;
+    const size_t words = (size_t)nparts * rows * C.n;
+    k->a.alloc(words * 8);
+    k->b.alloc(words * 8);
+    hip_check(hipMemcpyAsync(k->a.p, av, words * 8, hipMemcpyHostToDevice, s), "upload key");
+    hip_check(hipMemcpyAsync(k->b.p, bv, words * 8, hipMemcpyHostToDevice, s), "upload key");
+    // to Montgomery form (x R mod q) on the device, rows this context can address
+    std::vector<FuseJob> jobs;
+    for (uint32_t p = 0; p < nparts; ++p)
+        for (uint32_t r = 0; r < rows; ++r) {
+            uint32_t row;
+            if (r < C.L) row = r;
+            else if (r >= fullQ) row = C.L + (r - fullQ);
+            else continue;
+            const uint64_t q = C.mods[row], R = C.rc[row].r1;
+            for (int ab = 0; ab < 2; ++ab) {
+                uint64_t* t = (ab ? k->b.as<uint64_t>() : k->a.as<uint64_t>()) + ((size_t)p * rows + r) * C.n;
+                jobs.push_back(FuseJob{t, t, t, q, R, host_precon(R, q), 0, 0});
+            }
+        }
+    DevBuf dj(jobs.size() * sizeof(FuseJob));
+    hip_check(hipMemcpyAsync(dj.p, jobs.data(), dj.bytes, hipMemcpyHostToDevice, s), "upload jobs");
+    launch_fuse(dj.as<FuseJob>(), (uint32_t)jobs.size(), C.n, s);
+    hip_check(hipStreamSynchronize(s), "key conversion");
+    return k;
+}
+
+// ------------------------------------------------------------------ Plan --
+Plan::~Plan() {
+    if (graph) (void)hipGraphExecDestroy(graph);
+}
+
+void Plan::run(hipStream_t s) const {
+    const char* base = (const char*)dev.p;
+    for (const Step& st : steps) {
+        const void* p = base + st.off;
+        switch (st.kind) {
+            case Step::NTT_F: launch_ntt((const NttJob*)p, st.count, geom, true, s); break;
+            case Step::NTT_I: launch_ntt((const NttJob*)p, st.count, geom, false, s); break;
+            case Step::TENSOR: launch_tensor((const TensorJob*)p, st.count, n, s); break;
+            case Step::CONV: launch_conv((const ConvJob*)p, st.count, n, s); break;
+            case Step::INNER: launch_inner((const InnerJob*)p, st.count, n, s); break;
+            case Step::COMBINE: launch_combine((const CombineJob*)p, st.count, n, s); break;
+            case Step::SWITCH: launch_switch((const SwitchJob*)p, st.count, n, s); break;
+            case Step::FUSE: launch_fuse((const FuseJob*)p, st.count, n, s); break;
+            case Step::ADD: launch_add((const AddJob*)p, st.count, n, s); break;
+            case Step::PERM: launch_perm((const PermJob*)p, st.count, n, logn, s); break;
+            case Step::COPY: launch_copy((const CopyJob*)p, st.count, n, s); break;
+        }
+    }
+}
+
+void Plan::run_graph(hipStream_t s) {
+    if (!graph) {
+        hipGraph_t g = nullptr;
+        hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
+        run(s);
+        hip_check(hipStreamEndCapture(s, &g), "end capture");
+        hip_check(hipGraphInstantiate(&graph, g, nullptr, nullptr, 0), "graph instantiate");
+        (void)hipGraphDestroy(g);
+    }
+    hip_check(hipGraphLaunch(graph, s), "graph launch");
+}
+
+// --------------------------------------------------------------- builder --
+static const uint32_t CONV_CHUNK = 8;
+
+void PlanBuilder::keyswitch(uint32_t l, const uint64_t* d, uint64_t* ba0, uint64_t* ba1,
+                            const DevKey& key, uint64_t* ws, const uint64_t* t0,
+                            const uint64_t* t1) {
+    const Ctx& C = C_;
+    if (l < 1 || l > C.L) throw std::runtime_error("gpufhe: bad level");
+    const uint32_t n = C.n, K = C.K, W = l + K;
+    const LevelConst& lv = C.lv[l];
+    if (key.nparts < lv.np) throw std::runtime_error("gpufhe: key has too few parts for this level");
+    const WsLayout w = WsLayout::make(C);
+    uint64_t* x = ws + w.x * n;
+    uint64_t* dig = ws + w.dig * n;
+    uint64_t* res[2] = {ws + w.res0 * n, ws + w.res1 * n};
+    uint64_t* xp = ws + w.xp * n;
+    uint64_t* conv = x;  // the part towers are dead once ModUp has converted them
+    // 1. part towers to coefficient form, scaled by n^-1 QHatInv (ModUp's first factor)
+    for (uint32_t i = 0; i < l; ++i) {
+        const uint32_t pi = i / C.alpha, ii = i - pi * C.alpha;
+        intt_up_.push_back(NttJob{d + (size_t)i * n, x + (size_t)i * n, C.rt[i].ir, C.rt[i].ip,
+                                  C.mods[i], lv.parts[pi].post[ii], lv.parts[pi].postp[ii]});
+    }
+    // 2-3. ModUp: convert every part to its complement, back to evaluation form
+    for (uint32_t pi = 0; pi < lv.np; ++pi) {
+        const PartConst& P = lv.parts[pi];
+        uint64_t* dp = dig + (size_t)pi * W * n;
+        for (uint32_t j0 = 0; j0 < P.sc; j0 += CONV_CHUNK)
+            conv_up_.push_back(ConvJob{x + (size_t)P.st * n, dp, P.d_tab, P.d_tq, P.d_tqneg, P.sp, P.sc,
+                                       j0, j0 + CONV_CHUNK < P.sc ? j0 + CONV_CHUNK : P.sc});
+        for (uint32_t j = 0; j < P.sc; ++j) {
+            const uint32_t r = P.compl_rows[j];
+            ntt_dig_.push_back(NttJob{dp + (size_t)j * n, dp + (size_t)j * n, C.rt[r].fr, C.rt[r].fp,
+                                      C.mods[r], 0, 0});
+        }
+    }
+    // 4. inner product with the key over Q_l P (src/keyswitch_resident.cpp:171-200)
+    for (uint32_t i = 0; i < W; ++i) {
+        const uint32_t row = i < l ? C.row_q(i) : C.row_p(i - l);
+        const uint32_t krow = i < l ? i : key.fullQ + (i - l);
+        InnerJob J{};
+        J.np = lv.np;
+        for (uint32_t pi = 0; pi < lv.np; ++pi) {
+            const PartConst& P = lv.parts[pi];
+            const uint64_t* dp = dig + (size_t)pi * W * n;
+            if (i >= P.st && i < P.st + P.sp) J.dig[pi] = d + (size_t)i * n;
+            else J.dig[pi] = dp + (size_t)(i < P.st ? i : i - P.sp) * n;
+            const size_t ko = ((size_t)pi * key.rows + krow) * n;
+            J.kb[pi] = key.b.as<uint64_t>() + ko;
+            J.ka[pi] = key.a.as<uint64_t>() + ko;
+        }
+        J.o0 = res[0] + (size_t)i * n;
+        J.o1 = res[1] + (size_t)i * n;
+        J.q = C.rc[row].q;
+        J.qneg = C.rc[row].qneg;
+        inner_.push_back(J);
+    }
+    // 5-8. ApproxModDown of both accumulators, fused with r = t + ba
+    uint64_t* outs[2] = {ba0, ba1};
+    const uint64_t* ts[2] = {t0, t1};
+    for (int c = 0; c < 2; ++c) {
+        for (uint32_t k = 0; k < K; ++k) {
+            const uint32_t r = C.row_p(k);
+            intt_p_.push_back(NttJob{res[c] + (size_t)(l + k) * n, xp + (size_t)(c * K + k) * n, C.rt[r].ir,
+                                     C.rt[r].ip, C.mods[r], lv.pd_post[k], lv.pd_postp[k]});
+        }
+        uint64_t* cv = conv + (size_t)c * l * n;
+        for (uint32_t j0 = 0; j0 < l; j0 += CONV_CHUNK)
+            conv_dn_.push_back(ConvJob{xp + (size_t)c * K * n, cv, lv.d_tabMD, lv.d_tqMD, lv.d_tqnegMD, K, l,
+                                       j0, j0 + CONV_CHUNK < l ? j0 + CONV_CHUNK : l});
+        for (uint32_t i = 0; i < l; ++i) {
+            ntt_conv_.push_back(NttJob{cv + (size_t)i * n, cv + (size_t)i * n, C.rt[i].fr, C.rt[i].fp,
+                                       C.mods[i], 0, 0});
+            combine_.push_back(CombineJob{res[c] + (size_t)i * n, cv + (size_t)i * n,
+                                          ts[c] ? ts[c] + (size_t)i * n : nullptr, outs[c] + (size_t)i * n,
+                                          C.mods[i], lv.pinv[i], lv.pinvp[i]});
+        }
+    }
+}
+
+void PlanBuilder::evalmult(uint32_t l, const uint64_t* c0a, const uint64_t* c1a, const uint64_t* c0b,
+                           const uint64_t* c1b, uint64_t* r0, uint64_t* r1, const DevKey& key,
+                           uint64_t* ws) {
+    const Ctx& C = C_;
+    const uint32_t n = C.n;
+    const WsLayout w = WsLayout::make(C);
+    uint64_t* t0 = ws + w.t0 * n;
+    uint64_t* t1 = ws + w.t1 * n;
+    uint64_t* t2 = ws + w.t2 * n;
+    for (uint32_t i = 0; i < l; ++i) {
+        const size_t o = (size_t)i * n;
+        tensor_.push_back(TensorJob{c0a + o, c1a + o, c0b + o, c1b + o, t0 + o, t1 + o, t2 + o, C.rc[i].q,
+                                    C.rc[i].qneg, C.rc[i].r2});
+    }
+    keyswitch(l, t2, r0, r1, key, ws, t0, t1);
+}
+
+void PlanBuilder::rescale_comp(uint32_t l, const uint64_t* c, uint64_t* o, uint64_t* drop, uint64_t* sw) {
+    const Ctx& C = C_;
+    const uint32_t n = C.n, last = l - 1;
+    const LevelConst& lv = C.lv[l];
+    intt_drop_.push_back(NttJob{c + (size_t)last * n, drop, C.rt[last].ir, C.rt[last].ip, C.mods[last],
+                                C.rt[last].ninv, C.rt[last].ninvp});
+    for (uint32_t i = 0; i < last; ++i) {
+        uint64_t* si = sw + (size_t)i * n;
+        switch_.push_back(make_switch(drop, si, C.mods[last], C.mods[i]));
+        ntt_sw_.push_back(NttJob{si, si, C.rt[i].fr, C.rt[i].fp, C.mods[i], 0, 0});
+        fuse_.push_back(FuseJob{c + (size_t)i * n, si, o + (size_t)i * n, C.mods[i], lv.s1[i], lv.s1p[i],
+                                lv.s2[i], lv.s2p[i]});
+    }
+}
+
+void PlanBuilder::rescale(uint32_t l, const uint64_t* c0, const uint64_t* c1, uint64_t* o0, uint64_t* o1,
+                          uint64_t* ws) {
+    const Ctx& C = C_;
+    if (l < 2 || l > C.L) throw std::runtime_error("gpufhe: rescale needs at least two towers");
+    const uint32_t n = C.n;
+    const WsLayout w = WsLayout::make(C);
+    uint64_t* drop = ws + w.drop * n;
+    uint64_t* sw = ws + w.dig * n;   // ModUp digits are dead by now
+    rescale_comp(l, c0, o0, drop, sw);
+    rescale_comp(l, c1, o1, drop + n, sw + (size_t)(l - 1) * n);
+}
+
+void PlanBuilder::evalmult_rescale(uint32_t l, const uint64_t* c0a, const uint64_t* c1a,
+                                   const uint64_t* c0b, const uint64_t* c1b, uint64_t* o0, uint64_t* o1,
+                                   const DevKey& key, uint64_t* ws) {
+    const Ctx& C = C_;
+    if (l < 2) throw std::runtime_error("gpufhe: EvalMult+Rescale needs at least two towers");
+    const WsLayout w = WsLayout::make(C);
+    uint64_t* r0 = ws + w.t0 * C.n;
+    uint64_t* r1 = ws + w.t1 * C.n;
+    evalmult(l, c0a, c1a, c0b, c1b, r0, r1, key, ws);
+    rescale(l, r0, r1, o0, o1, ws);
+}
+
+void PlanBuilder::rotate(uint32_t l, uint64_t k, const uint64_t* c0, const uint64_t* c1, uint64_t* o0,
+                         uint64_t* o1, const DevKey& key, uint64_t* ws) {
+    const Ctx& C = C_;
+    const uint32_t n = C.n;
+    if (!(k & 1) || k >= 2ull * n) throw std::runtime_error("gpufhe: automorphism index must be odd and < 2n");
+    const WsLayout w = WsLayout::make(C);
+    uint64_t* a0 = ws + w.t0 * n;
+    uint64_t* a1 = ws + w.t2 * n;
+    for (uint32_t i = 0; i < l; ++i) {
+        const size_t o = (size_t)i * n;
+        perm_.push_back(PermJob{c0 + o, a0 + o, k});
+        perm_.push_back(PermJob{c1 + o, a1 + o, k});
+    }
+    // (sigma(c0) + ba0, ba1) with (ba0, ba1) = KS(sigma(c1)) (src/rotate.cpp:118-132)
+    keyswitch(l, a1, o0, o1, key, ws, a0, nullptr);
+}
+
+void PlanBuilder::add(uint32_t l, const uint64_t* a0, const uint64_t* a1, const uint64_t* b0,
+                      const uint64_t* b1, uint64_t* o0, uint64_t* o1) {
+    const uint32_t n = C_.n;
+    for (uint32_t i = 0; i < l; ++i) {
+        const size_t o = (size_t)i * n;
+        add_.push_back(AddJob{a0 + o, b0 + o, o0 + o, C_.mods[i]});
+        add_.push_back(AddJob{a1 + o, b1 + o, o1 + o, C_.mods[i]});
+    }
+}
+
+std::unique_ptr<Plan> PlanBuilder::finish() {
+    auto P = std::make_unique<Plan>();
+    P->n = C_.n;
+    P->logn = C_.logn;
+    P->geom = C_.geom;
+    std::vector<char> blob;
+    auto stage = [&](Step::Kind kind, const void* data, size_t cnt, size_t sz) {
+        if (!cnt) return;
+        size_t off = (blob.size() + 15) & ~(size_t)15;
+        blob.resize(off + cnt * sz);
+        std::memcpy(blob.data() + off, data, cnt * sz);
+        P->steps.push_back(Step{kind, off, (uint32_t)cnt});
+    };
+#define GF_STAGE(k, v) stage(Step::k, v.data(), v.size(), sizeof(v[0]))
+    GF_STAGE(TENSOR, tensor_);
+    GF_STAGE(PERM, perm_);
+    GF_STAGE(NTT_I, intt_up_);
+    GF_STAGE(CONV, conv_up_);
+    GF_STAGE(NTT_F, ntt_dig_);
+    GF_STAGE(INNER, inner_);
+    GF_STAGE(NTT_I, intt_p_);
+    GF_STAGE(CONV, conv_dn_);
+    GF_STAGE(NTT_F, ntt_conv_);
+    GF_STAGE(COMBINE, combine_);
+    GF_STAGE(ADD, add_rot_);
+    GF_STAGE(COPY, copy_rot_);
+    GF_STAGE(NTT_I, intt_drop_);
+    GF_STAGE(SWITCH, switch_);
+    GF_STAGE(NTT_F, ntt_sw_);
+    GF_STAGE(FUSE, fuse_);
+    GF_STAGE(ADD, add_);
+#undef GF_STAGE
+    P->dev.alloc(blob.size());
+    if (!blob.empty())
+        hip_check(hipMemcpy(P->dev.p, blob.data(), blob.size(), hipMemcpyHostToDevice), "upload plan");
+    return P;
+}
+
+}  // namespace gf

@@ -1,0 +1,168 @@
+// Host-side engine: resident parameter context, resident keys, per-op
+// workspaces and "plans" (precomputed job arrays + the launch sequence of one
+// operation over one batch of ciphertexts).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <memory>
+#include <mutex>
+#include <vector>
+
+#include "internal.hpp"
+
+namespace gf {
+
+// RAII device allocation
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t b) { alloc(b); }
+    ~DevBuf() { release(); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+        return *this;
+    }
+    void alloc(size_t b) {
+        release();
+        if (b) hip_check(hipMalloc(&p, b), "hipMalloc");
+        bytes = b;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T> T* as() const { return (T*)p; }
+};
+
+// device root tables of one modulus (shared process-wide, never freed)
+struct RootTable {
+    const uint64_t *fr, *fp, *ir, *ip;
+    uint64_t ninv, ninvp;
+};
+RootTable root_table(uint32_t n, uint64_t q, uint64_t root);
+
+// constants of one level (sizeQl = l towers of Q)
+struct PartConst {
+    uint32_t st, sp, sc;
+    std::vector<uint32_t> compl_rows;          // context rows of the complement towers
+    std::vector<uint64_t> post, postp;         // n^-1 * QHatInv_i (INTT folding), per source
+    const uint64_t* d_tab = nullptr;           // QHatModp * R mod target, [sp][sc]
+    const uint64_t* d_tq = nullptr;            // target moduli [sc]
+    const uint64_t* d_tqneg = nullptr;
+};
+struct LevelConst {
+    uint32_t l = 0, np = 0;
+    std::vector<PartConst> parts;
+    std::vector<uint64_t> pd_post, pd_postp;   // n^-1 * PHatInv_k, per P tower
+    const uint64_t* d_tabMD = nullptr;         // PHatModq * R mod q_i, [K][l]
+    const uint64_t* d_tqMD = nullptr;          // Q moduli [l]
+    const uint64_t* d_tqnegMD = nullptr;
+    std::vector<uint64_t> pinv, pinvp;         // P^-1 mod q_i
+    std::vector<uint64_t> s1, s1p, s2, s2p;    // rescale constants dropping tower l-1
+};
+
+// Resident parameter set: Q moduli (L), P moduli (K), part width alpha.
+class Ctx {
+public:
+    Ctx(uint32_t n, const std::vector<uint64_t>& q, const std::vector<uint64_t>& p,
+        const std::vector<uint64_t>& roots, uint32_t alpha);
+    uint32_t n, logn, L, K, alpha, dnum;
+    NttGeom geom;
+    std::vector<uint64_t> mods, roots;        // L + K rows: Q then P
+    std::vector<RowConst> rc;
+    std::vector<RootTable> rt;
+    std::vector<LevelConst> lv;               // index l = 1..L (0 unused)
+    DevBuf consts;                            // all per-level device constants
+    uint32_t row_q(uint32_t i) const { return i; }
+    uint32_t row_p(uint32_t k) const { return L + k; }
+    uint32_t num_parts(uint32_t l) const { return (l + alpha - 1) / alpha; }
+    // workspace (uint64 words) needed by one operation at the top level
+    size_t ws_words() const;
+    size_t tower_bytes() const { return (size_t)n * 8; }
+};
+
+// Resident switching key: rows_key = fullQ + K rows per part, Montgomery form.
+struct DevKey {
+    uint32_t nparts = 0, rows = 0, fullQ = 0;
+    DevBuf a, b;  // [nparts][rows][n]
+};
+std::unique_ptr<DevKey> upload_key(const Ctx& C, const uint64_t* av, const uint64_t* bv,
+                                   uint32_t nparts, uint32_t rows, uint32_t fullQ,
+                                   hipStream_t s);
+
+// One launch step of a plan
+struct Step {
+    enum Kind { NTT_F, NTT_I, TENSOR, CONV, INNER, COMBINE, SWITCH, FUSE, ADD, PERM, COPY } kind;
+    size_t off;     // byte offset of the first job in the plan's device buffer
+    uint32_t count;
+};
+
+class Plan {
+public:
+    std::vector<Step> steps;
+    DevBuf dev;
+    uint32_t n = 0, logn = 0;
+    NttGeom geom{};
+    void run(hipStream_t s) const;
+    // hipGraph of the whole sequence (captured on first use)
+    hipGraphExec_t graph = nullptr;
+    void run_graph(hipStream_t s);
+    ~Plan();
+};
+
+// Collects jobs stage by stage for any number of operations, then uploads.
+class PlanBuilder {
+public:
+    explicit PlanBuilder(const Ctx& C) : C_(C) {}
+    // per-op recipes (all buffers are device pointers, tower-major)
+    void keyswitch(uint32_t l, const uint64_t* d, uint64_t* ba0, uint64_t* ba1, const DevKey& key,
+                   uint64_t* ws, const uint64_t* t0 = nullptr, const uint64_t* t1 = nullptr);
+    void evalmult(uint32_t l, const uint64_t* c0a, const uint64_t* c1a, const uint64_t* c0b,
+                  const uint64_t* c1b, uint64_t* r0, uint64_t* r1, const DevKey& key, uint64_t* ws);
+    void rescale(uint32_t l, const uint64_t* c0, const uint64_t* c1, uint64_t* o0, uint64_t* o1,
+                 uint64_t* ws);
+    void evalmult_rescale(uint32_t l, const uint64_t* c0a, const uint64_t* c1a, const uint64_t* c0b,
+                          const uint64_t* c1b, uint64_t* o0, uint64_t* o1, const DevKey& key,
+                          uint64_t* ws);
+    void rotate(uint32_t l, uint64_t k, const uint64_t* c0, const uint64_t* c1, uint64_t* o0,
+                uint64_t* o1, const DevKey& key, uint64_t* ws);
+    void add(uint32_t l, const uint64_t* a0, const uint64_t* a1, const uint64_t* b0,
+             const uint64_t* b1, uint64_t* o0, uint64_t* o1);
+    std::unique_ptr<Plan> finish();
+
+private:
+    const Ctx& C_;
+    // stage lists in execution order
+    std::vector<TensorJob> tensor_;
+    std::vector<PermJob> perm_;
+    std::vector<NttJob> intt_up_;
+    std::vector<ConvJob> conv_up_;
+    std::vector<NttJob> ntt_dig_;
+    std::vector<InnerJob> inner_;
+    std::vector<NttJob> intt_p_;
+    std::vector<ConvJob> conv_dn_;
+    std::vector<NttJob> ntt_conv_;
+    std::vector<CombineJob> combine_;
+    std::vector<AddJob> add_rot_;
+    std::vector<CopyJob> copy_rot_;
+    std::vector<NttJob> intt_drop_;
+    std::vector<SwitchJob> switch_;
+    std::vector<NttJob> ntt_sw_;
+    std::vector<FuseJob> fuse_;
+    std::vector<AddJob> add_;
+    void rescale_comp(uint32_t l, const uint64_t* c, uint64_t* o, uint64_t* drop, uint64_t* sw);
+};
+
+// workspace offsets inside one op's workspace (in towers)
+struct WsLayout {
+    size_t t0, t1, t2, x, dig, res0, res1, xp, drop, total;
+    static WsLayout make(const Ctx& C);
+};
+
+}  // namespace gf

@@ -1,0 +1,130 @@
+// Internal job descriptors and launch helpers shared by the kernels and the
+// host orchestration.  Every batched kernel walks a device array of jobs, one
+// job per (operation, tower); the grid's y dimension indexes the job, so one
+// launch covers any mix of ciphertexts, components and levels at one ring size.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <stdexcept>
+#include <string>
+
+#include "modarith.hpp"
+#include "ntt_engine.hpp"
+
+namespace gf {
+
+enum { MAXP = 16, PW_THREADS = 256, PW_EPT = 4, CONV_MAX_SRC = 16 };
+
+// per-modulus constants (row = index in the Q|P modulus list of a context)
+struct RowConst {
+    uint64_t q, qneg, r2, r1, ninv, ninvp;
+};
+
+// t0 = a0 b0, t1 = a0 b1 + a1 b0, t2 = a1 b1 (one tower)
+struct TensorJob {
+    const uint64_t *c0a, *c1a, *c0b, *c1b;
+    uint64_t *t0, *t1, *t2;
+    uint64_t q, qneg, r2;
+};
+
+// dst[j] = (sum_i src[i] * tab[i*ntgt + j]) * R^-1 mod p_j for targets j in
+// [j0, j1); tab holds the conversion constants in Montgomery form, so one REDC
+// of the 128-bit sum gives the exact remainder.  src/dst are tower-contiguous.
+struct ConvJob {
+    const uint64_t* src;
+    uint64_t* dst;
+    const uint64_t* tab;
+    const uint64_t* tq;     // target moduli [ntgt]
+    const uint64_t* tqneg;  // target -q^-1 mod 2^64 [ntgt]
+    uint32_t nsrc, ntgt, j0, j1;
+};
+
+// o0 = sum_p dig[p] * kb[p], o1 = sum_p dig[p] * ka[p] (keys in Montgomery form)
+struct InnerJob {
+    const uint64_t* dig[MAXP];
+    const uint64_t* kb[MAXP];
+    const uint64_t* ka[MAXP];
+    uint64_t* o0;
+    uint64_t* o1;
+    uint64_t q, qneg;
+    uint32_t np;
+};
+
+// out = t + (res - conv) * pinv   (ApproxModDown combine fused with r = t + ba)
+struct CombineJob {
+    const uint64_t* res;
+    const uint64_t* conv;
+    const uint64_t* t;      // may be null: out = (res - conv) * pinv
+    uint64_t* out;
+    uint64_t q, pinv, pinvp;
+};
+
+// centred modulus switch q_old -> q_new
+struct SwitchJob {
+    const uint64_t* in;
+    uint64_t* out;
+    uint64_t q_old, q_new, half, diff;
+    uint32_t new_larger;
+};
+
+// out = a s1 + b s2
+struct FuseJob {
+    const uint64_t* a;
+    const uint64_t* b;
+    uint64_t* out;
+    uint64_t q, s1, s1p, s2, s2p;
+};
+
+// out = a + b
+struct AddJob {
+    const uint64_t* a;
+    const uint64_t* b;
+    uint64_t* out;
+    uint64_t q;
+};
+
+// evaluation-domain automorphism: out[i] = in[perm_k(i)]
+struct PermJob {
+    const uint64_t* in;
+    uint64_t* out;
+    uint64_t k;
+};
+
+// copy one tower
+struct CopyJob {
+    const uint64_t* in;
+    uint64_t* out;
+};
+
+inline void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess)
+        throw std::runtime_error(std::string("gpufhe: ") + what + ": " + hipGetErrorString(e));
+}
+
+// batched launches (jobs in device memory)
+void launch_ntt(const NttJob* d_jobs, uint32_t njobs, const NttGeom& g, bool fwd, hipStream_t s);
+void launch_ntt_one(const NttJob& job, const NttGeom& g, bool fwd, hipStream_t s);
+void launch_tensor(const TensorJob* d, uint32_t nj, uint32_t n, hipStream_t s);
+void launch_conv(const ConvJob* d, uint32_t nj, uint32_t n, hipStream_t s);
+void launch_inner(const InnerJob* d, uint32_t nj, uint32_t n, hipStream_t s);
+void launch_combine(const CombineJob* d, uint32_t nj, uint32_t n, hipStream_t s);
+void launch_switch(const SwitchJob* d, uint32_t nj, uint32_t n, hipStream_t s);
+void launch_fuse(const FuseJob* d, uint32_t nj, uint32_t n, hipStream_t s);
+void launch_add(const AddJob* d, uint32_t nj, uint32_t n, hipStream_t s);
+void launch_perm(const PermJob* d, uint32_t nj, uint32_t n, uint32_t logn, hipStream_t s);
+void launch_copy(const CopyJob* d, uint32_t nj, uint32_t n, hipStream_t s);
+
+inline SwitchJob make_switch(const uint64_t* in, uint64_t* out, uint64_t q_old, uint64_t q_new) {
+    SwitchJob j;
+    j.in = in;
+    j.out = out;
+    j.q_old = q_old;
+    j.q_new = q_new;
+    j.half = q_old >> 1;
+    j.new_larger = q_new > q_old;
+    j.diff = q_new > q_old ? q_new - q_old : (q_old - q_new) % q_new;
+    return j;
+}
+
+}  // namespace gf

@@ -1,0 +1,233 @@
+"""Python view of libgpufhe_amd.so (ctypes over include/gpufhe_capi.h and
+include/gpufhe_kernels.h).  The library is the product; this module only
+loads it, declares the signatures and wraps device buffers.  There is no
+fallback: if the shared library is missing, import fails loudly."""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgpufhe_amd.so")
+
+
+class GpuFheError(RuntimeError):
+    pass
+
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError("libgpufhe_amd.so is not built (run __graft_entry__.build() or "
+                      "python gpu-resident-library_amd/build.py)")
+lib = C.CDLL(LIB_PATH)
+
+vp = C.c_void_p
+u32, u64, sz = C.c_uint32, C.c_uint64, C.c_size_t
+u64p = C.POINTER(C.c_uint64)
+u32p = C.POINTER(C.c_uint32)
+vpp = C.POINTER(C.c_void_p)
+
+_capi = {
+    "gf_last_error": (C.c_char_p, []),
+    "gf_malloc": (C.c_int, [vpp, sz]),
+    "gf_free": (C.c_int, [vp]),
+    "gf_h2d": (C.c_int, [vp, vp, sz]),
+    "gf_d2h": (C.c_int, [vp, vp, sz]),
+    "gf_d2d": (C.c_int, [vp, vp, sz]),
+    "gf_device_sync": (C.c_int, []),
+    "gf_stream_create": (C.c_int, [vpp]),
+    "gf_stream_destroy": (C.c_int, [vp]),
+    "gf_stream_sync": (C.c_int, [vp]),
+    "gf_native_primes": (C.c_int, [u64p, u32, u32, u32, u64p, u32]),
+    "gf_native_root": (u64, [u32, u64]),
+    "gf_native_rescale_consts": (None, [u64p, u64p, u64p, u32]),
+    "gf_ctx_create": (vp, [u32, u64p, u32, u64p, u32, u64p, u32]),
+    "gf_ctx_destroy": (None, [vp]),
+    "gf_ws_bytes": (sz, [vp]),
+    "gf_key_create": (vp, [vp, u64p, u64p, u32, vp]),
+    "gf_key_destroy": (None, [vp]),
+    "gf_evalmult_rescale": (C.c_int, [vp, vp, u32] + [vp] * 7 + [vp]),
+    "gf_evalmult": (C.c_int, [vp, vp, u32] + [vp] * 7 + [vp]),
+    "gf_keyswitch": (C.c_int, [vp, vp, u32, vp, vp, vp, vp, vp]),
+    "gf_rescale": (C.c_int, [vp, u32, vp, vp, vp, vp, vp, vp]),
+    "gf_rotate": (C.c_int, [vp, vp, u32, u64, vp, vp, vp, vp, vp, vp]),
+    "gf_add": (C.c_int, [vp, u32, vp, vp, vp, vp, vp, vp, vp]),
+    "gf_plan_evalmult_rescale": (vp, [vp, vp, u32, u32p] + [vpp] * 7),
+    "gf_plan_rotate": (vp, [vp, vpp, u32, u32p, u64p] + [vpp] * 5),
+    "gf_plan_add": (vp, [vp, u32, u32p] + [vpp] * 6),
+    "gf_plan_run": (C.c_int, [vp, vp]),
+    "gf_plan_run_graph": (C.c_int, [vp, vp]),
+    "gf_plan_launches": (u32, [vp]),
+    "gf_plan_destroy": (None, [vp]),
+}
+_kern = {
+    "LaunchNTT_CT": [vp, vp, vp, u32, u64, vp],
+    "LaunchINTT_GS": [vp, vp, vp, u32, u64, u64, u64, vp],
+    "LaunchRNSMultTower": [vp, vp, vp, u64, u32, vp],
+    "LaunchSwitchModulus": [vp, u64, u64, u32, vp],
+    "LaunchRescaleFuse": [vp, vp, u64, u64, u64, u32, vp],
+    "LaunchApproxSwitchCRTBasis": [vp] * 9 + [u32, u32, u32, vp],
+    "LaunchKSMultAcc": [vp, vp, vp, vp, vp, u64, u32, vp],
+    "LaunchKSMultAccBatched": [vp] * 7 + [u32, u32, vp],
+    "LaunchModDownCombine": [vp, vp, vp, u64, u64, u32, vp],
+    "LaunchAutomorphPermute": [vp, vp, u32, u64, u64, vp],
+    "LaunchAddInto": [vp, vp, u64, u32, vp],
+    "LaunchPtMulAcc": [vp, vp, vp, vp, vp, u64, u32, vp],
+    "LaunchTensor": [vp] * 8 + [u32, u32, vp],
+    "LaunchCombine": [vp] * 7 + [u32, u32, vp],
+}
+for _n, (_r, _a) in _capi.items():
+    _f = getattr(lib, _n)
+    _f.restype = _r
+    _f.argtypes = _a
+for _n, _a in _kern.items():
+    _f = getattr(lib, _n)
+    _f.restype = None
+    _f.argtypes = _a
+
+
+def check(rc):
+    if rc != 0:
+        raise GpuFheError(lib.gf_last_error().decode())
+
+
+def P64(a):
+    return np.ascontiguousarray(a, dtype=np.uint64).ctypes.data_as(u64p)
+
+
+def ptr_array(ptrs):
+    return (C.c_void_p * len(ptrs))(*[int(p) for p in ptrs])
+
+
+class Dev:
+    """A device buffer of uint64 words."""
+
+    def __init__(self, words):
+        self.words = int(words)
+        p = C.c_void_p()
+        check(lib.gf_malloc(C.byref(p), max(self.words, 1) * 8))
+        self.ptr = p.value
+
+    @classmethod
+    def of(cls, arr):
+        a = np.ascontiguousarray(arr, dtype=np.uint64)
+        d = cls(a.size)
+        d.upload(a)
+        return d
+
+    def upload(self, arr, word_off=0):
+        a = np.ascontiguousarray(arr, dtype=np.uint64)
+        check(lib.gf_h2d(self.ptr + word_off * 8, a.ctypes.data, a.nbytes))
+
+    def download(self, words=None, word_off=0):
+        w = self.words - word_off if words is None else int(words)
+        out = np.empty(w, np.uint64)
+        check(lib.gf_d2h(out.ctypes.data, self.ptr + word_off * 8, w * 8))
+        return out
+
+    def at(self, word_off):
+        return self.ptr + int(word_off) * 8
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            lib.gf_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        self.free()
+
+
+class Stream:
+    def __init__(self):
+        s = C.c_void_p()
+        check(lib.gf_stream_create(C.byref(s)))
+        self.s = s.value
+
+    def sync(self):
+        check(lib.gf_stream_sync(self.s))
+
+    def __del__(self):
+        if getattr(self, "s", None):
+            lib.gf_stream_destroy(self.s)
+            self.s = None
+
+
+def device_sync():
+    check(lib.gf_device_sync())
+
+
+class Context:
+    """Resident parameter set (Q moduli, P moduli, roots, dnum)."""
+
+    def __init__(self, n, q, p, roots, dnum):
+        self.n, self.L, self.K, self.dnum = n, len(q), len(p), dnum
+        self.q = np.asarray(q, np.uint64)
+        self.p = np.asarray(p, np.uint64)
+        self.h = lib.gf_ctx_create(n, P64(q), len(q), P64(p), len(p), P64(roots), dnum)
+        if not self.h:
+            raise GpuFheError(lib.gf_last_error().decode())
+        self.ws_bytes = lib.gf_ws_bytes(self.h)
+
+    def workspace(self):
+        return Dev(self.ws_bytes // 8)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.gf_ctx_destroy(self.h)
+            self.h = None
+
+
+class Key:
+    def __init__(self, ctx, av, bv, nparts, stream=None):
+        self.ctx = ctx
+        self.h = lib.gf_key_create(ctx.h, P64(av), P64(bv), nparts, stream.s if stream else None)
+        if not self.h:
+            raise GpuFheError(lib.gf_last_error().decode())
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.gf_key_destroy(self.h)
+            self.h = None
+
+
+class Plan:
+    def __init__(self, h, keep=()):
+        if not h:
+            raise GpuFheError(lib.gf_last_error().decode())
+        self.h = h
+        self.keep = keep
+        self.launches = lib.gf_plan_launches(h)
+
+    def run(self, stream):
+        check(lib.gf_plan_run(self.h, stream.s if stream else None))
+
+    def run_graph(self, stream):
+        check(lib.gf_plan_run_graph(self.h, stream.s))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.gf_plan_destroy(self.h)
+            self.h = None
+
+
+def plan_evalmult_rescale(ctx, key, levels, c0a, c1a, c0b, c1b, o0, o1, ws):
+    cnt = len(levels)
+    lv = np.asarray(levels, np.uint32)
+    arrs = [ptr_array(x) for x in (c0a, c1a, c0b, c1b, o0, o1, ws)]
+    return Plan(lib.gf_plan_evalmult_rescale(ctx.h, key.h, cnt, lv.ctypes.data_as(u32p), *arrs),
+                keep=(lv, arrs))
+
+
+def plan_rotate(ctx, keys, levels, ks, c0, c1, o0, o1, ws):
+    cnt = len(levels)
+    lv = np.asarray(levels, np.uint32)
+    kk = np.asarray(ks, np.uint64)
+    kh = ptr_array([k.h for k in keys])
+    arrs = [ptr_array(x) for x in (c0, c1, o0, o1, ws)]
+    return Plan(lib.gf_plan_rotate(ctx.h, kh, cnt, lv.ctypes.data_as(u32p), kk.ctypes.data_as(u64p), *arrs),
+                keep=(lv, kk, kh, arrs, keys))
+
+
+def plan_add(ctx, levels, a0, a1, b0, b1, o0, o1):
+    lv = np.asarray(levels, np.uint32)
+    arrs = [ptr_array(x) for x in (a0, a1, b0, b1, o0, o1)]
+    return Plan(lib.gf_plan_add(ctx.h, len(levels), lv.ctypes.data_as(u32p), *arrs), keep=(lv, arrs))

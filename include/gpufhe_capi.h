@@ -1,0 +1,118 @@
+/*
+ * Handle-based C ABI of the device-resident CKKS engine (gfx950).
+ *
+ * The reference has no library-level EvalMult / Rescale / Rotate / KeySwitch:
+ * its callers compose them from kernel launchers and the C++ resident API
+ * (tests/test_evalmult_rescale.cpp:69-73, bench/bench_evalmult_resident.cpp:96-100,
+ * src/rotate.cpp:118-132, src/keyswitch_resident.cpp:139-244).  These entry
+ * points expose those compositions as first-class calls, over the same data
+ * layout (tower-major [tower][n] uint64, evaluation form), plus batch plans
+ * that run one operation over many ciphertexts in one launch per stage.
+ *
+ * Conventions: pointers named d_* / c* / o* / ws are device pointers; host
+ * arrays are named h_*.  Functions returning int give 0 on success and -1 on
+ * error (message in gf_last_error()); constructors return NULL on error.
+ * Operations are asynchronous on the given stream; the caller synchronises.
+ */
+#ifndef GPUFHE_CAPI_H
+#define GPUFHE_CAPI_H
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gf_ctx gf_ctx;
+typedef struct gf_key gf_key;
+typedef struct gf_plan gf_plan;
+
+const char* gf_last_error(void);
+
+/* ---- device memory and streams (plumbing for callers without a runtime) ---- */
+int gf_malloc(void** d_ptr, size_t bytes);
+int gf_free(void* d_ptr);
+int gf_h2d(void* d_dst, const void* h_src, size_t bytes);
+int gf_d2h(void* h_dst, const void* d_src, size_t bytes);
+int gf_d2d(void* d_dst, const void* d_src, size_t bytes);
+int gf_device_sync(void);
+int gf_stream_create(hipStream_t* s);
+int gf_stream_destroy(hipStream_t s);
+int gf_stream_sync(hipStream_t s);
+
+/* ---- parameters ----
+ * Native parameter generation of src/params.cpp:29-75 (OpenFHE-free path). */
+int gf_native_primes(uint64_t* h_out, uint32_t count, uint32_t bits, uint32_t n,
+                     const uint64_t* h_avoid, uint32_t navoid);
+uint64_t gf_native_root(uint32_t n, uint64_t q);
+void gf_native_rescale_consts(uint64_t* h_s1, uint64_t* h_s2, const uint64_t* h_mod,
+                              uint32_t last);
+
+/* Resident context for Q = q[0..L), P = p[0..K), one 2n-th root per modulus
+ * (Q then P), dnum digits (part width alpha = ceil(L/dnum)).  Uploads root
+ * tables and every level's ModUp / ModDown / rescale constants once.
+ * Replaces ks_context_create's table and constant setup
+ * (src/keyswitch_resident.cpp:32-101) for all levels at once. */
+gf_ctx* gf_ctx_create(uint32_t n, const uint64_t* h_q, uint32_t L, const uint64_t* h_p,
+                      uint32_t K, const uint64_t* h_roots, uint32_t dnum);
+void gf_ctx_destroy(gf_ctx* c);
+/* bytes of device workspace one in-flight operation needs (the DeviceKSWork
+ * of src/keyswitch_resident.cpp:103-114, extended to EvalMult + Rescale) */
+size_t gf_ws_bytes(const gf_ctx* c);
+
+/* Switching key (relinearisation or rotation): host arrays av, bv laid out
+ * [nparts][L+K][n] in evaluation form, as produced by evalkeygen_host_sold
+ * (src/evalkeygen.cpp:42-96).  Uploaded once, kept resident. */
+gf_key* gf_key_create(gf_ctx* c, const uint64_t* h_av, const uint64_t* h_bv, uint32_t nparts,
+                      hipStream_t s);
+void gf_key_destroy(gf_key* k);
+
+/* ---- single operations (level = number of Q towers of the inputs) ---- */
+/* tensor -> keyswitch -> combine -> rescale; outputs have level-1 towers */
+int gf_evalmult_rescale(gf_ctx* c, gf_key* rlk, uint32_t level, const uint64_t* c0a,
+                        const uint64_t* c1a, const uint64_t* c0b, const uint64_t* c1b,
+                        uint64_t* o0, uint64_t* o1, uint64_t* ws, hipStream_t s);
+/* tensor -> keyswitch -> combine (no rescale) */
+int gf_evalmult(gf_ctx* c, gf_key* rlk, uint32_t level, const uint64_t* c0a, const uint64_t* c1a,
+                const uint64_t* c0b, const uint64_t* c1b, uint64_t* o0, uint64_t* o1, uint64_t* ws,
+                hipStream_t s);
+/* hybrid keyswitch of d: (ba0, ba1) over Q_level */
+int gf_keyswitch(gf_ctx* c, gf_key* k, uint32_t level, const uint64_t* d, uint64_t* ba0,
+                 uint64_t* ba1, uint64_t* ws, hipStream_t s);
+/* drop the last tower of both components: outputs have level-1 towers */
+int gf_rescale(gf_ctx* c, uint32_t level, const uint64_t* c0, const uint64_t* c1, uint64_t* o0,
+               uint64_t* o1, uint64_t* ws, hipStream_t s);
+/* (sigma_k(c0) + ba0, ba1), (ba0, ba1) = KS(sigma_k(c1)); k odd, < 2n */
+int gf_rotate(gf_ctx* c, gf_key* rk, uint32_t level, uint64_t k, const uint64_t* c0,
+              const uint64_t* c1, uint64_t* o0, uint64_t* o1, uint64_t* ws, hipStream_t s);
+/* componentwise sum (EvalAdd) */
+int gf_add(gf_ctx* c, uint32_t level, const uint64_t* a0, const uint64_t* a1, const uint64_t* b0,
+           const uint64_t* b1, uint64_t* o0, uint64_t* o1, hipStream_t s);
+
+/* ---- batch plans: one launch per stage for `count` operations ----
+ * Host arrays of device pointers, one entry per operation; levels may differ.
+ * Each operation needs its own workspace of gf_ws_bytes(). */
+gf_plan* gf_plan_evalmult_rescale(gf_ctx* c, gf_key* rlk, uint32_t count, const uint32_t* h_levels,
+                                  const uint64_t* const* c0a, const uint64_t* const* c1a,
+                                  const uint64_t* const* c0b, const uint64_t* const* c1b,
+                                  uint64_t* const* o0, uint64_t* const* o1, uint64_t* const* ws);
+/* per-operation key handles (e.g. one rotation key per distinct index) */
+gf_plan* gf_plan_rotate(gf_ctx* c, gf_key* const* rks, uint32_t count, const uint32_t* h_levels,
+                        const uint64_t* h_k, const uint64_t* const* c0, const uint64_t* const* c1,
+                        uint64_t* const* o0, uint64_t* const* o1, uint64_t* const* ws);
+gf_plan* gf_plan_add(gf_ctx* c, uint32_t count, const uint32_t* h_levels,
+                     const uint64_t* const* a0, const uint64_t* const* a1,
+                     const uint64_t* const* b0, const uint64_t* const* b1, uint64_t* const* o0,
+                     uint64_t* const* o1);
+int gf_plan_run(gf_plan* p, hipStream_t s);
+/* capture the plan into a hipGraph on first call (s must not be the null
+ * stream), then replay it */
+int gf_plan_run_graph(gf_plan* p, hipStream_t s);
+uint32_t gf_plan_launches(const gf_plan* p);
+void gf_plan_destroy(gf_plan* p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,253 @@
+"""Composed operations through the handle C ABI vs the oracle, bit-exact:
+hybrid keyswitch, EvalMult(+Rescale), Rescale, Rotate, EvalAdd; batch plans
+over uneven levels; hipGraph replays; decrypt-level correctness."""
+import numpy as np
+import pytest
+
+import oracle as O
+from common import KeySet, decode, encode, make_params, poly_mul_negacyclic_small
+
+pytestmark = pytest.mark.gpu
+
+# (n, L, dnum): small ring, two-phase NTT ring, the BASELINE config shapes
+SHAPES = [(1 << 10, 4, 2), (1 << 13, 6, 3), (1 << 15, 4, 2), (1 << 15, 12, 3), (1 << 16, 24, 4)]
+
+
+@pytest.fixture(scope="module")
+def G():
+    import gpufhe
+    return gpufhe
+
+
+_ctx_cache = {}
+
+
+def setup(G, n, L, dnum, seed=1):
+    """Context, random switching key (bit parity needs no valid key) and a
+    random ciphertext generator."""
+    key = (n, L, dnum, seed)
+    if key in _ctx_cache:
+        return _ctx_cache[key]
+    P = make_params(n, L, dnum)
+    rng = np.random.default_rng(seed)
+    rows = P.L + P.K
+    av = np.concatenate([rng.integers(0, int(m), n, dtype=np.uint64) for _ in range(P.dnum) for m in P.mods])
+    bv = np.concatenate([rng.integers(0, int(m), n, dtype=np.uint64) for _ in range(P.dnum) for m in P.mods])
+    ctx = G.Context(n, P.q, P.p, P.roots, dnum)
+    k = G.Key(ctx, av, bv, P.dnum)
+    assert av.size == P.dnum * rows * n
+    out = (P, ctx, k, av, bv)
+    _ctx_cache[key] = out
+    return out
+
+
+def rand_ct(P, level, rng):
+    return np.concatenate([rng.integers(0, int(q), P.n, dtype=np.uint64) for q in P.q[:level]])
+
+
+def levels_for(L, dnum):
+    a = -(-L // dnum)
+    return sorted({L, max(1, L - 1), min(L, a + 1), 1})
+
+
+@pytest.mark.parametrize("n,L,dnum", SHAPES)
+def test_keyswitch(G, n, L, dnum):
+    P, ctx, k, av, bv = setup(G, n, L, dnum)
+    rng = np.random.default_rng(88)
+    s = G.Stream()
+    ws = ctx.workspace()
+    for level in levels_for(L, dnum):
+        d = rand_ct(P, level, rng)
+        dd, b0, b1 = G.Dev.of(d), G.Dev(level * n), G.Dev(level * n)
+        G.check(G.lib.gf_keyswitch(ctx.h, k.h, level, dd.ptr, b0.ptr, b1.ptr, ws.ptr, s.s))
+        s.sync()
+        w0, w1 = P.keyswitch(level, d, av, bv)
+        assert np.array_equal(b0.download(), w0), ("ba0", level)
+        assert np.array_equal(b1.download(), w1), ("ba1", level)
+
+
+@pytest.mark.parametrize("n,L,dnum", SHAPES)
+def test_evalmult_rescale(G, n, L, dnum):
+    P, ctx, k, av, bv = setup(G, n, L, dnum)
+    rng = np.random.default_rng(101)
+    s = G.Stream()
+    ws = ctx.workspace()
+    for level in sorted({L, 2}):
+        cts = [rand_ct(P, level, rng) for _ in range(4)]
+        d = [G.Dev.of(c) for c in cts]
+        o0, o1 = G.Dev((level - 1) * n), G.Dev((level - 1) * n)
+        G.check(G.lib.gf_evalmult_rescale(ctx.h, k.h, level, *[x.ptr for x in d], o0.ptr, o1.ptr, ws.ptr, s.s))
+        s.sync()
+        w0, w1 = P.evalmult_rescale(level, *cts, av, bv)
+        assert np.array_equal(o0.download(), w0), level
+        assert np.array_equal(o1.download(), w1), level
+
+
+@pytest.mark.parametrize("n,L,dnum", SHAPES[:3])
+def test_evalmult_norescale_and_rescale(G, n, L, dnum):
+    P, ctx, k, av, bv = setup(G, n, L, dnum)
+    rng = np.random.default_rng(7)
+    s = G.Stream()
+    ws = ctx.workspace()
+    cts = [rand_ct(P, L, rng) for _ in range(4)]
+    d = [G.Dev.of(c) for c in cts]
+    r0, r1 = G.Dev(L * n), G.Dev(L * n)
+    G.check(G.lib.gf_evalmult(ctx.h, k.h, L, *[x.ptr for x in d], r0.ptr, r1.ptr, ws.ptr, s.s))
+    s.sync()
+    w0, w1 = P.evalmult(L, *cts, av, bv)
+    assert np.array_equal(r0.download(), w0) and np.array_equal(r1.download(), w1)
+    o0, o1 = G.Dev((L - 1) * n), G.Dev((L - 1) * n)
+    G.check(G.lib.gf_rescale(ctx.h, L, r0.ptr, r1.ptr, o0.ptr, o1.ptr, ws.ptr, s.s))
+    s.sync()
+    assert np.array_equal(o0.download(), P.rescale(w0, L))
+    assert np.array_equal(o1.download(), P.rescale(w1, L))
+
+
+@pytest.mark.parametrize("n,L,dnum", [SHAPES[0], SHAPES[1], (1 << 16, 20, 4)])
+def test_rotate(G, n, L, dnum):
+    P, ctx, k, av, bv = setup(G, n, L, dnum)
+    rng = np.random.default_rng(404)
+    s = G.Stream()
+    ws = ctx.workspace()
+    for level, kk in ((L, 5), (max(1, L // 2), pow(5, 3, 2 * n)), (L, 2 * n - 1)):
+        c0, c1 = rand_ct(P, level, rng), rand_ct(P, level, rng)
+        d0, d1 = G.Dev.of(c0), G.Dev.of(c1)
+        o0, o1 = G.Dev(level * n), G.Dev(level * n)
+        G.check(G.lib.gf_rotate(ctx.h, k.h, level, kk, d0.ptr, d1.ptr, o0.ptr, o1.ptr, ws.ptr, s.s))
+        s.sync()
+        w0, w1 = P.rotate(level, c0, c1, kk, av, bv)
+        assert np.array_equal(o0.download(), w0), (level, kk)
+        assert np.array_equal(o1.download(), w1), (level, kk)
+
+
+def test_add(G):
+    P, ctx, k, av, bv = setup(G, *SHAPES[1])
+    n, L = P.n, P.L
+    rng = np.random.default_rng(5)
+    cts = [rand_ct(P, L, rng) for _ in range(4)]
+    d = [G.Dev.of(c) for c in cts]
+    o0, o1 = G.Dev(L * n), G.Dev(L * n)
+    s = G.Stream()
+    G.check(G.lib.gf_add(ctx.h, L, *[x.ptr for x in d], o0.ptr, o1.ptr, s.s))
+    s.sync()
+    for o, a, b in ((o0, cts[0], cts[2]), (o1, cts[1], cts[3])):
+        w = a.copy()
+        for t in range(L):
+            O.lib.or_add_into(O.P(w[t * n:(t + 1) * n]), O.P(b[t * n:(t + 1) * n].copy()), int(P.q[t]), n)
+        assert np.array_equal(o.download(), w)
+
+
+def test_batch_plan_uneven_levels_and_graph(G):
+    """One plan over ops of uneven level == per-op results; graph replays ==
+    eager, with eager work between replays."""
+    n, L, dnum = 1 << 13, 6, 3
+    P, ctx, k, av, bv = setup(G, n, L, dnum)
+    rng = np.random.default_rng(2)
+    levels = [6, 2, 5, 3, 6]
+    ins, outs, wss = [], [], []
+    for lv in levels:
+        ins.append([G.Dev.of(rand_ct(P, lv, rng)) for _ in range(4)])
+        outs.append([G.Dev((lv - 1) * n), G.Dev((lv - 1) * n)])
+        wss.append(ctx.workspace())
+    col = lambda j: [x[j].ptr for x in ins]  # noqa: E731
+    plan = G.plan_evalmult_rescale(ctx, k, levels, col(0), col(1), col(2), col(3),
+                                   [o[0].ptr for o in outs], [o[1].ptr for o in outs], [w.ptr for w in wss])
+    s = G.Stream()
+    plan.run(s)
+    s.sync()
+    want = []
+    for i, lv in enumerate(levels):
+        cts = [x.download() for x in ins[i]]
+        w0, w1 = P.evalmult_rescale(lv, *cts, av, bv)
+        want.append((w0, w1))
+        assert np.array_equal(outs[i][0].download(), w0), i
+        assert np.array_equal(outs[i][1].download(), w1), i
+    for rep in range(3):
+        for o in outs:
+            o[0].upload(np.zeros(o[0].words, np.uint64))
+            o[1].upload(np.zeros(o[1].words, np.uint64))
+        plan.run_graph(s)
+        s.sync()
+        # unrelated eager work between replays
+        G.check(G.lib.gf_add(ctx.h, 2, ins[1][0].ptr, ins[1][1].ptr, ins[1][2].ptr, ins[1][3].ptr,
+                             wss[0].ptr, wss[0].at(2 * n), s.s))
+        s.sync()
+        for i in range(len(levels)):
+            assert np.array_equal(outs[i][0].download(), want[i][0]), (rep, i)
+            assert np.array_equal(outs[i][1].download(), want[i][1]), (rep, i)
+    assert plan.launches < 30
+
+
+def test_rotate_plan_multi_keys(G):
+    n, L, dnum = 1 << 12, 5, 5
+    P = make_params(n, L, dnum)
+    ctx = G.Context(n, P.q, P.p, P.roots, dnum)
+    rng = np.random.default_rng(9)
+    keys, hk = [], []
+    for _ in range(3):
+        av = np.concatenate([rng.integers(0, int(m), n, dtype=np.uint64) for _ in range(dnum) for m in P.mods])
+        bv = np.concatenate([rng.integers(0, int(m), n, dtype=np.uint64) for _ in range(dnum) for m in P.mods])
+        keys.append((av, bv))
+        hk.append(G.Key(ctx, av, bv, dnum))
+    ks = [5, 25, 125]
+    levels = [5, 3, 4]
+    c = [(rand_ct(P, lv, rng), rand_ct(P, lv, rng)) for lv in levels]
+    dc = [(G.Dev.of(a), G.Dev.of(b)) for a, b in c]
+    do = [(G.Dev(lv * n), G.Dev(lv * n)) for lv in levels]
+    ws = [ctx.workspace() for _ in levels]
+    plan = G.plan_rotate(ctx, hk, levels, ks, [x[0].ptr for x in dc], [x[1].ptr for x in dc],
+                         [x[0].ptr for x in do], [x[1].ptr for x in do], [w.ptr for w in ws])
+    s = G.Stream()
+    plan.run(s)
+    s.sync()
+    for i, lv in enumerate(levels):
+        w0, w1 = P.rotate(lv, c[i][0], c[i][1], ks[i], *keys[i])
+        assert np.array_equal(do[i][0].download(), w0) and np.array_equal(do[i][1].download(), w1)
+
+
+def test_decrypt_exact_product_cfg1_shape(G):
+    """tests/test_evalmult_rescale.cpp bar on the GPU path: exact small-integer
+    product after round(dec * q_l / D^2), n = 2^15, 60 + 3 x 50-bit Q, dnum 2."""
+    P = make_params(1 << 15, 4, 2)
+    ks = KeySet(P)
+    av, bv = ks.rlk()
+    n, L = P.n, P.L
+    ctx = G.Context(n, P.q, P.p, P.roots, P.dnum)
+    key = G.Key(ctx, av, bv, P.dnum)
+    D = 1 << 40
+    a1 = np.zeros(n, np.int64)
+    a2 = np.zeros(n, np.int64)
+    a1[0], a1[1], a1[n - 1] = 3, -2, 7
+    a2[0], a2[2] = 4, 5
+    c0a, c1a = O.encrypt(n, P.q, P.roots[:L], a1 * D, ks.pkA, ks.pkB, seed=303)
+    c0b, c1b = O.encrypt(n, P.q, P.roots[:L], a2 * D, ks.pkA, ks.pkB, seed=304)
+    d = [G.Dev.of(x) for x in (c0a, c1a, c0b, c1b)]
+    o0, o1 = G.Dev((L - 1) * n), G.Dev((L - 1) * n)
+    ws = ctx.workspace()
+    s = G.Stream()
+    G.check(G.lib.gf_evalmult_rescale(ctx.h, key.h, L, *[x.ptr for x in d], o0.ptr, o1.ptr, ws.ptr, s.s))
+    s.sync()
+    dec = O.decrypt(n, int(P.q[0]), int(P.roots[0]), o0.download(), o1.download(), ks.sQ)
+    ql = int(P.q[L - 1])
+    got = np.array([round(int(x) * ql / (D * D)) for x in dec], dtype=object)
+    assert np.max(np.abs((got - poly_mul_negacyclic_small(a1, a2, n)).astype(np.int64))) == 0
+
+
+def test_decrypt_slots_rotation(G):
+    P = make_params(1 << 12, 4, 2)
+    ks = KeySet(P)
+    n, L = P.n, P.L
+    ctx = G.Context(n, P.q, P.p, P.roots, P.dnum)
+    z = np.random.default_rng(4).uniform(-1, 1, n // 2)
+    delta = float(1 << 45)
+    c0, c1 = O.encrypt(n, P.q, P.roots[:L], encode(z, n, delta), ks.pkA, ks.pkB, seed=505)
+    kk = pow(5, 2, 2 * n)
+    av, bv = ks.rotkey(kk)
+    key = G.Key(ctx, av, bv, P.dnum)
+    d0, d1, o0, o1 = G.Dev.of(c0), G.Dev.of(c1), G.Dev(L * n), G.Dev(L * n)
+    ws = ctx.workspace()
+    s = G.Stream()
+    G.check(G.lib.gf_rotate(ctx.h, key.h, L, kk, d0.ptr, d1.ptr, o0.ptr, o1.ptr, ws.ptr, s.s))
+    s.sync()
+    dec = O.decrypt(n, int(P.q[0]), int(P.roots[0]), o0.download(), o1.download(), ks.sQ)
+    assert np.max(np.abs(decode(dec, n, delta).real - np.roll(z, -2))) < 1e-6

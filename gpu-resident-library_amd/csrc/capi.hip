@@ -241,4 +241,53 @@ uint32_t gf_plan_launches(const gf_plan* p) {
 }
 void gf_plan_destroy(gf_plan* p) { delete p; }
 
+uint32_t gf_plan_num_steps(const gf_plan* p) { return (uint32_t)p->P->steps.size(); }
+int gf_plan_step_info(const gf_plan* p, uint32_t i, uint32_t* kind, uint32_t* jobs) {
+    return guard([&] {
+        if (i >= p->P->steps.size()) throw std::runtime_error("gpufhe: step index out of range");
+        *kind = (uint32_t)p->P->steps[i].kind;
+        *jobs = p->P->steps[i].count;
+    });
+}
+int gf_plan_run_step(gf_plan* p, uint32_t i, hipStream_t s) {
+    return guard([&] {
+        if (i >= p->P->steps.size()) throw std::runtime_error("gpufhe: step index out of range");
+        p->P->run_step(i, s);
+    });
+}
+int gf_ctx_set_ntt_radix(gf_ctx* c, uint32_t r) {
+    return guard([&] {
+        if (r < 2 || r > 4) throw std::runtime_error("gpufhe: NTT radix log2 must be 2, 3 or 4");
+        std::lock_guard<std::mutex> lk(c->mu);
+        c->C->geom.rmax = r;
+        c->cache.clear();   // cached single-op plans carry the old geometry
+    });
+}
+uint32_t gf_ctx_get_ntt_radix(const gf_ctx* c) { return c->C->geom.rmax; }
+int gf_ctx_autotune(gf_ctx* c, hipStream_t s, uint32_t* r, float* ms) {
+    return guard([&] {
+        std::lock_guard<std::mutex> lk(c->mu);
+        float m = 0;
+        const uint32_t best = c->C->autotune(s, &m);
+        c->cache.clear();
+        if (r) *r = best;
+        if (ms) *ms = m;
+    });
+}
+int gf_fill_uniform(gf_ctx* c, uint64_t* d, uint32_t level, uint64_t seed, hipStream_t s) {
+    return guard([&] {
+        if (level < 1 || level > c->C->L) throw std::runtime_error("gpufhe: bad level");
+        launch_fill(d, c->C->lv[c->C->L].d_tqMD, level, c->C->n, seed, s);
+    });
+}
+int gf_event_create(hipEvent_t* e) { return guard([&] { hip_check(hipEventCreate(e), "event"); }); }
+int gf_event_destroy(hipEvent_t e) { return guard([&] { hip_check(hipEventDestroy(e), "event"); }); }
+int gf_event_record(hipEvent_t e, hipStream_t s) { return guard([&] { hip_check(hipEventRecord(e, s), "event"); }); }
+int gf_event_elapsed_ms(float* ms, hipEvent_t a, hipEvent_t b) {
+    return guard([&] {
+        hip_check(hipEventSynchronize(b), "event sync");
+        hip_check(hipEventElapsedTime(ms, a, b), "event elapsed");
+    });
+}
+
 }  // extern "C"

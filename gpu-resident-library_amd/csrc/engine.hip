@@ -23,7 +23,7 @@ RootTable root_table(uint32_t n, uint64_t q, uint64_t root) {
     if (it != cache.end()) return it->second;
     uint32_t logn = 0;
     while ((1u << logn) < n) ++logn;
-    std::vector<uint64_t> h((size_t)4 * n), pw(n);
+    std::vector<uint64_t> h((size_t)8 * n), pw(n);
     const uint64_t bases[2] = {root, host_invmod(root, q)};
     for (int s = 0; s < 2; ++s) {
         pw[0] = 1;
@@ -35,6 +35,9 @@ RootTable root_table(uint32_t n, uint64_t q, uint64_t root) {
             for (uint32_t k = 0; k < logn; ++k) { r = (r << 1) | (v & 1); v >>= 1; }
             tr[i] = pw[r];
             tp[i] = host_precon(pw[r], q);
+            uint64_t* pr = h.data() + (size_t)(4 + 2 * s) * n;   // paired copy
+            pr[2 * i] = tr[i];
+            pr[2 * i + 1] = tp[i];
         }
     }
     void* d = nullptr;
@@ -46,6 +49,8 @@ RootTable root_table(uint32_t n, uint64_t q, uint64_t root) {
     t.fp = b + n;
     t.ir = b + 2 * (size_t)n;
     t.ip = b + 3 * (size_t)n;
+    t.fw2 = b + 4 * (size_t)n;
+    t.iv2 = b + 6 * (size_t)n;
     t.ninv = host_invmod(n, q);
     t.ninvp = host_precon(t.ninv, q);
     cache.emplace(key, t);
@@ -197,6 +202,49 @@ WsLayout WsLayout::make(const Ctx& C) {
 }
 size_t Ctx::ws_words() const { return WsLayout::make(*this).total * (size_t)n; }
 
+uint32_t Ctx::autotune(hipStream_t s, float* best_ms) {
+    // 96 towers cycling through every modulus, forward then inverse, in place
+    const uint32_t towers = 96;
+    DevBuf buf((size_t)towers * n * 8);
+    std::vector<NttJob> fj, ij;
+    for (uint32_t t = 0; t < towers; ++t) {
+        const uint32_t r = t % (L + K);
+        uint64_t* x = buf.as<uint64_t>() + (size_t)t * n;
+        fj.push_back(NttJob{x, x, rt[r].fw2, nullptr, mods[r], 0, 0});
+        ij.push_back(NttJob{x, x, rt[r].iv2, nullptr, mods[r], rt[r].ninv, rt[r].ninvp});
+    }
+    DevBuf dj((fj.size() + ij.size()) * sizeof(NttJob));
+    hip_check(hipMemcpy(dj.p, fj.data(), fj.size() * sizeof(NttJob), hipMemcpyHostToDevice), "upload");
+    hip_check(hipMemcpy(dj.as<NttJob>() + fj.size(), ij.data(), ij.size() * sizeof(NttJob), hipMemcpyHostToDevice),
+              "upload");
+    hip_check(hipMemsetAsync(buf.p, 0, buf.bytes, s), "clear");
+    hipEvent_t e0, e1;
+    hip_check(hipEventCreate(&e0), "event");
+    hip_check(hipEventCreate(&e1), "event");
+    uint32_t best = geom.rmax;
+    float bms = 1e30f;
+    for (uint32_t r = 2; r <= 4; ++r) {
+        NttGeom g = make_geom(n, r);
+        launch_ntt(dj.as<NttJob>(), towers, g, true, s);   // warm
+        launch_ntt(dj.as<NttJob>() + towers, towers, g, false, s);
+        hip_check(hipEventRecord(e0, s), "event");
+        for (int rep = 0; rep < 5; ++rep) {
+            launch_ntt(dj.as<NttJob>(), towers, g, true, s);
+            launch_ntt(dj.as<NttJob>() + towers, towers, g, false, s);
+        }
+        hip_check(hipEventRecord(e1, s), "event");
+        hip_check(hipEventSynchronize(e1), "event sync");
+        float ms = 0;
+        hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+        if (ms < bms) { bms = ms; best = r; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    geom.rmax = best;
+    if (best_ms) *best_ms = bms / 5;
+    return best;
+}
+
 // ------------------------------------------------------------------ keys --
 std::unique_ptr<DevKey> upload_key(const Ctx& C, const uint64_t* av, const uint64_t* bv,
                                    uint32_t nparts, uint32_t rows, uint32_t fullQ, hipStream_t s) {
@@ -238,8 +286,13 @@ Plan::~Plan() {
 }
 
 void Plan::run(hipStream_t s) const {
+    for (size_t i = 0; i < steps.size(); ++i) run_step(i, s);
+}
+
+void Plan::run_step(size_t i, hipStream_t s) const {
     const char* base = (const char*)dev.p;
-    for (const Step& st : steps) {
+    {
+        const Step& st = steps[i];
         const void* p = base + st.off;
         switch (st.kind) {
             case Step::NTT_F: launch_ntt((const NttJob*)p, st.count, geom, true, s); break;
@@ -289,7 +342,7 @@ void PlanBuilder::keyswitch(uint32_t l, const uint64_t* d, uint64_t* ba0, uint64
     // 1. part towers to coefficient form, scaled by n^-1 QHatInv (ModUp's first factor)
     for (uint32_t i = 0; i < l; ++i) {
         const uint32_t pi = i / C.alpha, ii = i - pi * C.alpha;
-        intt_up_.push_back(NttJob{d + (size_t)i * n, x + (size_t)i * n, C.rt[i].ir, C.rt[i].ip,
+        intt_up_.push_back(NttJob{d + (size_t)i * n, x + (size_t)i * n, C.rt[i].iv2, nullptr,
                                   C.mods[i], lv.parts[pi].post[ii], lv.parts[pi].postp[ii]});
     }
     // 2-3. ModUp: convert every part to its complement, back to evaluation form
@@ -301,7 +354,7 @@ void PlanBuilder::keyswitch(uint32_t l, const uint64_t* d, uint64_t* ba0, uint64
                                        j0, j0 + CONV_CHUNK < P.sc ? j0 + CONV_CHUNK : P.sc});
         for (uint32_t j = 0; j < P.sc; ++j) {
             const uint32_t r = P.compl_rows[j];
-            ntt_dig_.push_back(NttJob{dp + (size_t)j * n, dp + (size_t)j * n, C.rt[r].fr, C.rt[r].fp,
+            ntt_dig_.push_back(NttJob{dp + (size_t)j * n, dp + (size_t)j * n, C.rt[r].fw2, nullptr,
                                       C.mods[r], 0, 0});
         }
     }
@@ -332,15 +385,15 @@ void PlanBuilder::keyswitch(uint32_t l, const uint64_t* d, uint64_t* ba0, uint64
     for (int c = 0; c < 2; ++c) {
         for (uint32_t k = 0; k < K; ++k) {
             const uint32_t r = C.row_p(k);
-            intt_p_.push_back(NttJob{res[c] + (size_t)(l + k) * n, xp + (size_t)(c * K + k) * n, C.rt[r].ir,
-                                     C.rt[r].ip, C.mods[r], lv.pd_post[k], lv.pd_postp[k]});
+            intt_p_.push_back(NttJob{res[c] + (size_t)(l + k) * n, xp + (size_t)(c * K + k) * n, C.rt[r].iv2,
+                                     nullptr, C.mods[r], lv.pd_post[k], lv.pd_postp[k]});
         }
         uint64_t* cv = conv + (size_t)c * l * n;
         for (uint32_t j0 = 0; j0 < l; j0 += CONV_CHUNK)
             conv_dn_.push_back(ConvJob{xp + (size_t)c * K * n, cv, lv.d_tabMD, lv.d_tqMD, lv.d_tqnegMD, K, l,
                                        j0, j0 + CONV_CHUNK < l ? j0 + CONV_CHUNK : l});
         for (uint32_t i = 0; i < l; ++i) {
-            ntt_conv_.push_back(NttJob{cv + (size_t)i * n, cv + (size_t)i * n, C.rt[i].fr, C.rt[i].fp,
+            ntt_conv_.push_back(NttJob{cv + (size_t)i * n, cv + (size_t)i * n, C.rt[i].fw2, nullptr,
                                        C.mods[i], 0, 0});
             combine_.push_back(CombineJob{res[c] + (size_t)i * n, cv + (size_t)i * n,
                                           ts[c] ? ts[c] + (size_t)i * n : nullptr, outs[c] + (size_t)i * n,
@@ -370,12 +423,12 @@ void PlanBuilder::rescale_comp(uint32_t l, const uint64_t* c, uint64_t* o, uint6
     const Ctx& C = C_;
     const uint32_t n = C.n, last = l - 1;
     const LevelConst& lv = C.lv[l];
-    intt_drop_.push_back(NttJob{c + (size_t)last * n, drop, C.rt[last].ir, C.rt[last].ip, C.mods[last],
+    intt_drop_.push_back(NttJob{c + (size_t)last * n, drop, C.rt[last].iv2, nullptr, C.mods[last],
                                 C.rt[last].ninv, C.rt[last].ninvp});
     for (uint32_t i = 0; i < last; ++i) {
         uint64_t* si = sw + (size_t)i * n;
         switch_.push_back(make_switch(drop, si, C.mods[last], C.mods[i]));
-        ntt_sw_.push_back(NttJob{si, si, C.rt[i].fr, C.rt[i].fp, C.mods[i], 0, 0});
+        ntt_sw_.push_back(NttJob{si, si, C.rt[i].fw2, nullptr, C.mods[i], 0, 0});
         fuse_.push_back(FuseJob{c + (size_t)i * n, si, o + (size_t)i * n, C.mods[i], lv.s1[i], lv.s1p[i],
                                 lv.s2[i], lv.s2p[i]});
     }

@@ -42,7 +42,8 @@ struct DevBuf {
 
 // device root tables of one modulus (shared process-wide, never freed)
 struct RootTable {
-    const uint64_t *fr, *fp, *ir, *ip;
+    const uint64_t *fr, *fp, *ir, *ip;   // reference layout (separate arrays)
+    const uint64_t *fw2, *iv2;           // interleaved (w, precon) pairs used by the plans
     uint64_t ninv, ninvp;
 };
 RootTable root_table(uint32_t n, uint64_t q, uint64_t root);
@@ -84,6 +85,9 @@ public:
     uint32_t num_parts(uint32_t l) const { return (l + alpha - 1) / alpha; }
     // workspace (uint64 words) needed by one operation at the top level
     size_t ws_words() const;
+    // times the NTT variants on a synthetic tower batch and keeps the fastest
+    // register radix in geom (plans built afterwards use it); returns it
+    uint32_t autotune(hipStream_t s, float* best_ms = nullptr);
     size_t tower_bytes() const { return (size_t)n * 8; }
 };
 
@@ -110,6 +114,7 @@ public:
     uint32_t n = 0, logn = 0;
     NttGeom geom{};
     void run(hipStream_t s) const;
+    void run_step(size_t i, hipStream_t s) const;
     // hipGraph of the whole sequence (captured on first use)
     hipGraphExec_t graph = nullptr;
     void run_graph(hipStream_t s);

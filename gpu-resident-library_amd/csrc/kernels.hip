@@ -25,43 +25,62 @@ struct JobsOne {
     __device__ NttJob get(uint32_t) const { return j; }
 };
 
-template <bool FWD, class J>
+template <bool FWD, bool PAIRED, uint32_t S1, uint32_t RM, class J>
 __global__ __launch_bounds__(NTT_THREADS) void k_ntt_A(J jobs, NttGeom g) {
     extern __shared__ uint64_t lds[];
     const NttJob j = jobs.get(blockIdx.y);
-    phaseA_body<FWD>(j, g, blockIdx.x, lds, threadIdx.x, blockDim.x, DevSync{});
+    phaseA_body<FWD, PAIRED, S1, RM>(j, g, blockIdx.x, lds, threadIdx.x, blockDim.x, DevSync{});
 }
-template <bool FWD, class J>
+template <bool FWD, bool PAIRED, uint32_t S2, uint32_t RM, class J>
 __global__ __launch_bounds__(NTT_THREADS) void k_ntt_B(J jobs, NttGeom g) {
     extern __shared__ uint64_t lds[];
     const NttJob j = jobs.get(blockIdx.y);
-    phaseB_body<FWD>(j, g, blockIdx.x, lds, threadIdx.x, blockDim.x, DevSync{});
+    phaseB_body<FWD, PAIRED, S2, RM>(j, g, blockIdx.x, lds, threadIdx.x, blockDim.x, DevSync{});
 }
-template <bool FWD, class J>
-__global__ __launch_bounds__(NTT_THREADS) void k_ntt_S(J jobs, NttGeom g) {
+template <bool FWD, bool PAIRED, uint32_t S, uint32_t RM, class J>
+__global__ __launch_bounds__(NTT_THREADS) void k_ntt_S(J jobs) {
     extern __shared__ uint64_t lds[];
     const NttJob j = jobs.get(blockIdx.y);
-    single_body<FWD>(j, g, lds, threadIdx.x, blockDim.x, DevSync{});
+    single_body<FWD, PAIRED, S, RM>(j, lds, threadIdx.x, blockDim.x, DevSync{});
 }
 
-template <class J>
-static void ntt_dispatch(J jobs, uint32_t njobs, const NttGeom& g, bool fwd, hipStream_t s) {
+// PAIRED: library plans use interleaved (w, Shoup precon) tables; the
+// reference ABI launchers take the two arrays separately.  g.rmax selects the
+// register radix (the tuning knob the autotuner sets).
+template <bool FWD, bool PAIRED, class J>
+static void ntt_dispatch_dir(J jobs, uint32_t njobs, const NttGeom& g, hipStream_t s) {
     const dim3 thr(NTT_THREADS);
-    if (g.single) {
-        const size_t sh = (size_t)g.n * 8;
-        if (fwd) k_ntt_S<true><<<dim3(1, njobs), thr, sh, s>>>(jobs, g);
-        else k_ntt_S<false><<<dim3(1, njobs), thr, sh, s>>>(jobs, g);
-        return;
-    }
-    const size_t sh = (size_t)NTT_TILE * 8;
-    const dim3 gA((1u << g.logN2) / g.C, njobs), gB((1u << g.logN1) / g.R, njobs);
-    if (fwd) {
-        k_ntt_A<true><<<gA, thr, sh, s>>>(jobs, g);
-        k_ntt_B<true><<<gB, thr, sh, s>>>(jobs, g);
-    } else {
-        k_ntt_B<false><<<gB, thr, sh, s>>>(jobs, g);
-        k_ntt_A<false><<<gA, thr, sh, s>>>(jobs, g);
-    }
+    bool ok = with_radix(g.rmax, [&](auto Rc) {
+        constexpr uint32_t RM = decltype(Rc)::value;
+        bool geom_ok;
+        if (g.single) {
+            geom_ok = with_single(g.logn, [&](auto Sc) {
+                constexpr uint32_t S = decltype(Sc)::value;
+                k_ntt_S<FWD, PAIRED, S, RM><<<dim3(1, njobs), thr, (size_t)padded(1u << S) * 8, s>>>(jobs);
+            });
+        } else {
+            geom_ok = with_split(g.logN1, g.logN2, [&](auto A, auto B) {
+                constexpr uint32_t S1 = decltype(A)::value, S2 = decltype(B)::value;
+                const dim3 gA((1u << S2) / g.C, njobs), gB((1u << S1) / g.R, njobs);
+                const size_t shA = (size_t)NTT_TILE * 8, shB = (size_t)NTT_TILE_PADDED * 8;
+                if (FWD) {
+                    k_ntt_A<FWD, PAIRED, S1, RM><<<gA, thr, shA, s>>>(jobs, g);
+                    k_ntt_B<FWD, PAIRED, S2, RM><<<gB, thr, shB, s>>>(jobs, g);
+                } else {
+                    k_ntt_B<FWD, PAIRED, S2, RM><<<gB, thr, shB, s>>>(jobs, g);
+                    k_ntt_A<FWD, PAIRED, S1, RM><<<gA, thr, shA, s>>>(jobs, g);
+                }
+            });
+        }
+        if (!geom_ok) throw std::runtime_error("gpufhe: unsupported ring dimension for the NTT");
+    });
+    if (!ok) throw std::runtime_error("gpufhe: unsupported NTT radix");
+}
+
+template <bool PAIRED, class J>
+static void ntt_dispatch(J jobs, uint32_t njobs, const NttGeom& g, bool fwd, hipStream_t s) {
+    if (fwd) ntt_dispatch_dir<true, PAIRED>(jobs, njobs, g, s);
+    else ntt_dispatch_dir<false, PAIRED>(jobs, njobs, g, s);
 }
 
 static const uint32_t MAXY = 65535;
@@ -69,11 +88,11 @@ static const uint32_t MAXY = 65535;
 void launch_ntt(const NttJob* d_jobs, uint32_t njobs, const NttGeom& g, bool fwd, hipStream_t s) {
     for (uint32_t off = 0; off < njobs; off += MAXY) {
         uint32_t cnt = njobs - off < MAXY ? njobs - off : MAXY;
-        ntt_dispatch(JobsDev{d_jobs, off}, cnt, g, fwd, s);
+        ntt_dispatch<true>(JobsDev{d_jobs, off}, cnt, g, fwd, s);
     }
 }
 void launch_ntt_one(const NttJob& job, const NttGeom& g, bool fwd, hipStream_t s) {
-    ntt_dispatch(JobsOne{job}, 1, g, fwd, s);
+    ntt_dispatch<false>(JobsOne{job}, 1, g, fwd, s);
 }
 
 // ------------------------------------------------------ pointwise kernels --
@@ -184,6 +203,20 @@ __global__ __launch_bounds__(PW_THREADS) void k_copy(const CopyJob* __restrict__
     PW_FOR(n) { j.out[i] = j.in[i]; }
 }
 
+// synthetic uniform residues: splitmix64 of (seed, tower, index) reduced mod q
+__global__ __launch_bounds__(PW_THREADS) void k_fill(uint64_t* d, const uint64_t* __restrict__ mods,
+                                                     uint32_t n, uint64_t seed) {
+    const uint32_t t = blockIdx.y;
+    const uint64_t q = mods[t];
+    PW_FOR(n) {
+        uint64_t z = seed + 0x9E3779B97F4A7C15ull * ((uint64_t)t * n + i + 1);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        d[(uint64_t)t * n + i] = z % q;
+    }
+}
+
 static dim3 pw_grid(uint32_t n, uint32_t nj) {
     return dim3((n + PW_THREADS * PW_EPT - 1) / (PW_THREADS * PW_EPT), nj);
 }
@@ -203,6 +236,11 @@ GF_PW_LAUNCH(launch_switch, k_switch, SwitchJob)
 GF_PW_LAUNCH(launch_fuse, k_fuse, FuseJob)
 GF_PW_LAUNCH(launch_add, k_add, AddJob)
 GF_PW_LAUNCH(launch_copy, k_copy, CopyJob)
+
+void launch_fill(uint64_t* d, const uint64_t* d_mods, uint32_t towers, uint32_t n, uint64_t seed,
+                 hipStream_t s) {
+    if (towers) k_fill<<<pw_grid(n, towers), PW_THREADS, 0, s>>>(d, d_mods, n, seed);
+}
 
 void launch_perm(const PermJob* d, uint32_t nj, uint32_t n, uint32_t logn, hipStream_t s) {
     for (uint32_t off = 0; off < nj; off += MAXY) {

@@ -65,6 +65,12 @@ GF_HD uint64_t shoup(uint64_t x, uint64_t w, uint64_t wp, uint64_t q) {
     uint64_t r = x * w - mulhi64(x, wp) * q;
     return r >= q ? r - q : r;
 }
+// Shoup without the final correction: x*w mod q + {0 or q}, in [0, 2q), any x < 2^64
+GF_HD uint64_t shoup_lazy(uint64_t x, uint64_t w, uint64_t wp, uint64_t q) {
+    return x * w - mulhi64(x, wp) * q;
+}
+// x in [0, 2m) -> [0, m)
+GF_HD uint64_t csub(uint64_t x, uint64_t m) { return x >= m ? x - m : x; }
 
 // Montgomery reduction, R = 2^64, qneg = -q^{-1} mod 2^64.
 // For T < q * 2^64 returns T * R^{-1} mod q, in [0, q).

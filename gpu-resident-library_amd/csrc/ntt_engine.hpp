@@ -4,21 +4,26 @@
 // forward = Cooley-Tukey, standard order in, bit-reversed order out, twiddle of
 // stage m, group i = tw[m + i] with tw[k] = psi^brv(k); inverse = Gentleman-
 // Sande with psi^-brv(k), followed by a multiply by n^-1 (here folded into the
-// last store together with any caller constant).
+// last store together with any caller constant).  Outputs are in [0, q).
 //
 // Decomposition: n = N1 * N2.  The first log2(N1) CT stages only couple
 // elements with the same index mod N2 ("columns"); the last log2(N2) stages
 // only couple elements inside one contiguous row of N2.  Phase A transforms a
 // tile of C columns, phase B a tile of R rows; each phase is one global read
 // and one global write of the tower.  Inside a phase the tile lives in LDS and
-// is processed in radix-2^r register passes (r <= 3), one LDS round trip per
+// is processed in radix-2^r register passes (r <= 4), one LDS round trip per
 // pass.  For n <= 4096 the whole tower is one tile (single phase).
+//
+// Butterflies are lazy (Harvey): forward values stay in [0, 4q), inverse
+// values in [0, 2q), reduced once at the final store (needs 4q < 2^64).
 //
 // Twiddle index for CT stage st (local to the phase), sub-transform s, group i:
 //     (G(s) << st) + i,   G(s) = g0 + s * gstep
 // phase A / single phase: G = 1 (the table prefix, stage m = 2^st);
 // phase B, row a:         G = N1 + a   (global stage m = N1 2^st).
 #pragma once
+#include <type_traits>
+
 #include "modarith.hpp"
 
 namespace gf {
@@ -26,8 +31,8 @@ namespace gf {
 struct NttJob {
     const uint64_t* in;   // tower read by the first phase
     uint64_t* out;        // tower written by every phase (may equal in)
-    const uint64_t* tw;   // psi^brv (forward) or psi^-brv (inverse), n entries
-    const uint64_t* twp;  // Shoup precons of tw
+    const uint64_t* tw;   // twiddles: paired table (w, Shoup precon) or w alone
+    const uint64_t* twp;  // Shoup precons when tw is not paired, else unused
     uint64_t q;
     uint64_t post, postp; // inverse only: final multiplier (n^-1 * c) and precon
 };
@@ -38,13 +43,19 @@ struct NttGeom {
     uint32_t C;             // columns per phase-A tile
     uint32_t R;             // rows per phase-B tile
     uint32_t single;        // 1: whole tower in one tile
+    uint32_t rmax;          // largest register radix log2 (tuning knob, 2..4)
 };
 
 enum { NTT_LOG_TILE = 12, NTT_TILE = 1 << NTT_LOG_TILE, NTT_SINGLE_MAX = 4096, NTT_THREADS = 256 };
 
-inline NttGeom make_geom(uint32_t n) {
+// row-tile padding: one spare word every 16, so strided radix units hit distinct LDS banks
+GF_HD constexpr uint32_t padded(uint32_t k) { return k + (k >> 4); }
+enum { NTT_TILE_PADDED = NTT_TILE + NTT_TILE / 16 };
+
+inline NttGeom make_geom(uint32_t n, uint32_t rmax = 3) {
     NttGeom g{};
     g.n = n;
+    g.rmax = rmax;
     uint32_t l = 0;
     while ((1u << l) < n) ++l;
     g.logn = l;
@@ -65,97 +76,145 @@ inline NttGeom make_geom(uint32_t n) {
     return g;
 }
 
-// radix-2^r chunking of S stages: ceil(S/3) chunks, the larger ones first
-GF_HD uint32_t num_chunks(uint32_t S) { return (S + 2) / 3; }
-GF_HD uint32_t chunk_size(uint32_t S, uint32_t c) {
-    uint32_t nc = num_chunks(S), base = S / nc, extra = S % nc;
-    return base + (c < extra ? 1u : 0u);
-}
-GF_HD uint32_t chunk_start(uint32_t S, uint32_t c) {
-    uint32_t st = 0;
-    for (uint32_t i = 0; i < c; ++i) st += chunk_size(S, i);
-    return st;
+// twiddle accessors: paired (w, wp) table -> one 16-byte load; or two arrays
+struct TwPaired {
+    const uint64_t* t;
+    GF_HD void get(uint64_t i, uint64_t& w, uint64_t& wp) const {
+        w = t[2 * i];
+        wp = t[2 * i + 1];
+    }
+};
+struct TwSplit {
+    const uint64_t* w;
+    const uint64_t* p;
+    GF_HD void get(uint64_t i, uint64_t& ww, uint64_t& wp) const {
+        ww = w[i];
+        wp = p[i];
+    }
+};
+template <bool PAIRED>
+GF_HD auto tw_of(const NttJob& j) {
+    if constexpr (PAIRED) return TwPaired{j.tw};
+    else return TwSplit{j.tw, j.twp};
 }
 
-// One radix-2^r pass over every unit of the tile owned by this thread.
-// Element k of sub-transform s lives at a[k*KS + s*SS].
-template <bool FWD>
-GF_HD void tile_pass(uint64_t* a, uint32_t S, uint32_t st0, uint32_t r, uint32_t logT,
-                     uint32_t KS, uint32_t SS, const uint64_t* tw, const uint64_t* twp,
-                     uint64_t g0, uint32_t gstep, uint64_t q, bool sub_fastest, uint32_t tid,
+// radix-2^r chunking of S stages: ceil(S/RM) chunks, the larger ones first
+GF_HD constexpr uint32_t num_chunks(uint32_t S, uint32_t RM) { return (S + RM - 1) / RM; }
+GF_HD constexpr uint32_t chunk_size(uint32_t S, uint32_t RM, uint32_t c) {
+    return S / num_chunks(S, RM) + (c < S % num_chunks(S, RM) ? 1u : 0u);
+}
+GF_HD constexpr uint32_t chunk_start(uint32_t S, uint32_t RM, uint32_t c) {
+    return c * (S / num_chunks(S, RM)) + (c < S % num_chunks(S, RM) ? c : S % num_chunks(S, RM));
+}
+
+// Runtime description of how a tile is laid out and which twiddles it uses.
+// Element k of sub-transform s lives at a[k*KS + s*SS]; sub-transform s uses
+// twiddle group base G(s) = g0 + s*gstep.
+struct TileDesc {
+    uint32_t logT, KS, SS;
+    uint64_t g0;
+    uint32_t gstep;
+    bool sub_fastest;   // unit order: consecutive threads on consecutive sub-transforms
+    bool pad;           // element k of a sub-transform at padded(k) (KS must be 1)
+};
+
+// One radix-2^R pass (stages ST0 .. ST0+R-1 of an S-stage transform) over all
+// units of the tile owned by this thread; fully unrolled, the R-stage unit
+// stays in registers.
+template <bool FWD, uint32_t S, uint32_t ST0, uint32_t R, class TW>
+GF_HD void tile_pass(uint64_t* a, const TileDesc& d, const TW& tw, uint64_t q, uint32_t tid,
                      uint32_t nthr) {
-    const uint32_t T = 1u << logT;
-    const uint32_t logtl = S - st0 - r;          // log2(t_last)
-    const uint32_t tl = 1u << logtl;
-    const uint32_t per_sub = 1u << (S - r);     // units per sub-transform
-    const uint32_t U = T * per_sub;
-    const uint32_t R8 = 1u << r;
+    constexpr uint32_t LOGTL = S - ST0 - R;   // log2(t_last)
+    constexpr uint32_t TL = 1u << LOGTL;
+    constexpr uint32_t LOGPS = S - R;         // log2(units per sub-transform)
+    constexpr uint32_t R8 = 1u << R;
+    const uint32_t T = 1u << d.logT;
+    const uint32_t U = T << LOGPS;
+    const uint64_t q2 = 2 * q;
     for (uint32_t uid = tid; uid < U; uid += nthr) {
         uint32_t s, rest;
-        if (sub_fastest) {
+        if (d.sub_fastest) {
             s = uid & (T - 1);
-            rest = uid >> logT;
+            rest = uid >> d.logT;
         } else {
-            rest = uid & (per_sub - 1);
-            s = uid >> (S - r);
+            rest = uid & ((1u << LOGPS) - 1);
+            s = uid >> LOGPS;
         }
-        const uint32_t l = rest & (tl - 1);
-        const uint32_t g = rest >> logtl;
-        const uint32_t k0 = (g << (S - st0)) + l;
-        uint64_t* base = a + (uint64_t)k0 * KS + (uint64_t)s * SS;
-        const uint64_t step = (uint64_t)tl * KS;
-        uint64_t x[8];
-        for (uint32_t u = 0; u < R8; ++u) x[u] = base[u * step];
-        const uint64_t G = g0 + (uint64_t)s * gstep;
+        const uint32_t l = rest & (TL - 1);
+        const uint32_t g = rest >> LOGTL;
+        const uint32_t k0 = (g << (S - ST0)) + l;
+        uint64_t* base = a + (uint64_t)s * d.SS;
+        uint32_t at[R8];
+#pragma unroll
+        for (uint32_t u = 0; u < R8; ++u)
+            at[u] = d.pad ? padded(k0 + u * TL) : (uint32_t)((uint64_t)(k0 + u * TL) * d.KS);
+        uint64_t x[R8];
+#pragma unroll
+        for (uint32_t u = 0; u < R8; ++u) x[u] = base[at[u]];
+        const uint64_t G = d.g0 + (uint64_t)s * d.gstep;
         if (FWD) {
-            for (uint32_t c = 0; c < r; ++c) {
-                const uint32_t h = 1u << (r - 1 - c);
+#pragma unroll
+            for (uint32_t c = 0; c < R; ++c) {
+                const uint32_t h = 1u << (R - 1 - c);
+                const uint64_t tb = (G << (ST0 + c)) + ((uint64_t)g << c);
+#pragma unroll
                 for (uint32_t u = 0; u < R8; ++u) {
                     if (u & h) continue;
-                    const uint64_t ti = (G << (st0 + c)) + ((uint64_t)g << c) + (u >> (r - c));
-                    const uint64_t y = shoup(x[u + h], tw[ti], twp[ti], q);
-                    const uint64_t xv = x[u];
-                    x[u] = addmod(xv, y, q);
-                    x[u + h] = submod(xv, y, q);
+                    uint64_t w, wp;
+                    tw.get(tb + (u >> (R - c)), w, wp);
+                    const uint64_t xv = csub(x[u], q2);               // [0, 2q)
+                    const uint64_t t = shoup_lazy(x[u + h], w, wp, q);  // [0, 2q)
+                    x[u] = xv + t;                                     // [0, 4q)
+                    x[u + h] = xv - t + q2;                            // (0, 4q)
                 }
             }
         } else {
-            for (int c = (int)r - 1; c >= 0; --c) {
-                const uint32_t h = 1u << (r - 1 - c);
+#pragma unroll
+            for (int c = (int)R - 1; c >= 0; --c) {
+                const uint32_t h = 1u << (R - 1 - c);
+                const uint64_t tb = (G << (ST0 + c)) + ((uint64_t)g << c);
+#pragma unroll
                 for (uint32_t u = 0; u < R8; ++u) {
                     if (u & h) continue;
-                    const uint64_t ti = (G << (st0 + c)) + ((uint64_t)g << c) + (u >> (r - c));
-                    const uint64_t xv = x[u], yv = x[u + h];
-                    x[u] = addmod(xv, yv, q);
-                    x[u + h] = shoup(submod(xv, yv, q), tw[ti], twp[ti], q);
+                    uint64_t w, wp;
+                    tw.get(tb + (u >> (R - c)), w, wp);
+                    const uint64_t xv = x[u], yv = x[u + h];           // [0, 2q)
+                    x[u] = csub(xv + yv, q2);                          // [0, 2q)
+                    x[u + h] = shoup_lazy(xv - yv + q2, w, wp, q);     // [0, 2q)
                 }
             }
         }
-        for (uint32_t u = 0; u < R8; ++u) base[u * step] = x[u];
+#pragma unroll
+        for (uint32_t u = 0; u < R8; ++u) base[at[u]] = x[u];
     }
 }
 
-// All passes of one phase.  Sync is a functor run between passes (the
-// workgroup barrier on the device, nothing in the sequential host emulation).
-template <bool FWD, class Sync>
-GF_HD void tile_transform(uint64_t* a, uint32_t S, uint32_t logT, uint32_t KS, uint32_t SS,
-                          const uint64_t* tw, const uint64_t* twp, uint64_t g0, uint32_t gstep,
-                          uint64_t q, bool sub_fastest, uint32_t tid, uint32_t nthr, Sync sync) {
-    const uint32_t nc = num_chunks(S);
-    for (uint32_t i = 0; i < nc; ++i) {
-        const uint32_t c = FWD ? i : nc - 1 - i;
-        tile_pass<FWD>(a, S, chunk_start(S, c), chunk_size(S, c), logT, KS, SS, tw, twp, g0, gstep, q,
-                       sub_fastest, tid, nthr);
+// All passes of one S-stage phase, chunk order fixed at compile time.
+template <bool FWD, uint32_t S, uint32_t RM, uint32_t I, class TW, class Sync>
+GF_HD void tile_chunks(uint64_t* a, const TileDesc& d, const TW& tw, uint64_t q, uint32_t tid,
+                       uint32_t nthr, Sync sync) {
+    if constexpr (I < num_chunks(S, RM)) {
+        constexpr uint32_t C = FWD ? I : num_chunks(S, RM) - 1 - I;
+        tile_pass<FWD, S, chunk_start(S, RM, C), chunk_size(S, RM, C)>(a, d, tw, q, tid, nthr);
         sync();
+        tile_chunks<FWD, S, RM, I + 1>(a, d, tw, q, tid, nthr, sync);
     }
+}
+
+// final store value: forward [0,4q) -> [0,q); inverse: times post (n^-1 c)
+template <bool FWD>
+GF_HD uint64_t ntt_final(uint64_t v, const NttJob& j) {
+    if (FWD) return csub(csub(v, 2 * j.q), j.q);
+    return shoup(v, j.post, j.postp, j.q);
 }
 
 // ---- phase bodies (shared by the kernels and the host emulation) ----
-// Phase A: tile = columns [b0, b0+C) x all N1 rows.
-template <bool FWD, class Sync>
+// Phase A: tile = columns [b0, b0+C) x all N1 = 2^S1 rows.
+template <bool FWD, bool PAIRED, uint32_t S1, uint32_t RM, class Sync>
 GF_HD void phaseA_body(const NttJob& j, const NttGeom& g, uint32_t tile, uint64_t* lds,
                        uint32_t tid, uint32_t nthr, Sync sync) {
-    const uint32_t C = g.C, N1 = 1u << g.logN1, logN2 = g.logN2, logC = NTT_LOG_TILE - g.logN1;
+    constexpr uint32_t N1 = 1u << S1, logC = NTT_LOG_TILE - S1, C = 1u << logC;
+    const uint32_t logN2 = g.logN2;
     const uint32_t b0 = tile * C;
     const uint64_t* src = FWD ? j.in : j.out;   // forward: first phase; inverse: second
     for (uint32_t idx = tid; idx < C * N1; idx += nthr) {
@@ -163,43 +222,68 @@ GF_HD void phaseA_body(const NttJob& j, const NttGeom& g, uint32_t tile, uint64_
         lds[idx] = src[((uint64_t)row << logN2) + b0 + c];
     }
     sync();
-    tile_transform<FWD>(lds, g.logN1, logC, C, 1, j.tw, j.twp, 1, 0, j.q, true, tid, nthr, sync);
+    const TileDesc d{logC, C, 1, 1, 0, true, false};
+    tile_chunks<FWD, S1, RM, 0>(lds, d, tw_of<PAIRED>(j), j.q, tid, nthr, sync);
     for (uint32_t idx = tid; idx < C * N1; idx += nthr) {
         const uint32_t row = idx >> logC, c = idx & (C - 1);
-        uint64_t v = lds[idx];
-        if (!FWD) v = shoup(v, j.post, j.postp, j.q);
-        j.out[((uint64_t)row << logN2) + b0 + c] = v;
+        const uint64_t v = lds[idx];
+        // forward phase A hands lazy values in [0,4q) to phase B
+        j.out[((uint64_t)row << logN2) + b0 + c] = FWD ? v : ntt_final<false>(v, j);
     }
 }
 
-// Phase B: tile = rows [a0, a0+R), each a contiguous run of N2.
-template <bool FWD, class Sync>
+// Phase B: tile = rows [a0, a0+R), each a contiguous run of N2 = 2^S2.
+template <bool FWD, bool PAIRED, uint32_t S2, uint32_t RM, class Sync>
 GF_HD void phaseB_body(const NttJob& j, const NttGeom& g, uint32_t tile, uint64_t* lds,
                        uint32_t tid, uint32_t nthr, Sync sync) {
-    const uint32_t R = g.R, N2 = 1u << g.logN2, N1 = 1u << g.logN1;
+    constexpr uint32_t N2 = 1u << S2, logR = NTT_LOG_TILE - S2, R = 1u << logR;
+    const uint32_t N1 = 1u << g.logN1;
     const uint32_t a0 = tile * R;
     const uint64_t* src = FWD ? j.out : j.in;   // forward: second phase; inverse: first
-    const uint64_t off = (uint64_t)a0 << g.logN2;
-    for (uint32_t idx = tid; idx < R * N2; idx += nthr) lds[idx] = src[off + idx];
+    const uint64_t off = (uint64_t)a0 << S2;
+    for (uint32_t idx = tid; idx < R * N2; idx += nthr) lds[padded(idx)] = src[off + idx];
     sync();
-    tile_transform<FWD>(lds, g.logN2, NTT_LOG_TILE - g.logN2, 1, N2, j.tw, j.twp, (uint64_t)N1 + a0,
-                        1, j.q, false, tid, nthr, sync);
-    for (uint32_t idx = tid; idx < R * N2; idx += nthr) j.out[off + idx] = lds[idx];
+    const TileDesc d{logR, 1, padded(N2), (uint64_t)N1 + a0, 1, false, true};
+    tile_chunks<FWD, S2, RM, 0>(lds, d, tw_of<PAIRED>(j), j.q, tid, nthr, sync);
+    // inverse phase B hands lazy values in [0,2q) to phase A
+    for (uint32_t idx = tid; idx < R * N2; idx += nthr)
+        j.out[off + idx] = FWD ? ntt_final<true>(lds[padded(idx)], j) : lds[padded(idx)];
 }
 
-// Single phase: the whole tower (n <= NTT_SINGLE_MAX) is one tile.
-template <bool FWD, class Sync>
-GF_HD void single_body(const NttJob& j, const NttGeom& g, uint64_t* lds, uint32_t tid,
-                       uint32_t nthr, Sync sync) {
-    const uint32_t n = g.n;
-    for (uint32_t idx = tid; idx < n; idx += nthr) lds[idx] = j.in[idx];
+// Single phase: the whole tower (n = 2^S <= NTT_SINGLE_MAX) is one tile.
+template <bool FWD, bool PAIRED, uint32_t S, uint32_t RM, class Sync>
+GF_HD void single_body(const NttJob& j, uint64_t* lds, uint32_t tid, uint32_t nthr, Sync sync) {
+    constexpr uint32_t n = 1u << S;
+    for (uint32_t idx = tid; idx < n; idx += nthr) lds[padded(idx)] = j.in[idx];
     sync();
-    tile_transform<FWD>(lds, g.logn, 0, 1, n, j.tw, j.twp, 1, 0, j.q, false, tid, nthr, sync);
-    for (uint32_t idx = tid; idx < n; idx += nthr) {
-        uint64_t v = lds[idx];
-        if (!FWD) v = shoup(v, j.post, j.postp, j.q);
-        j.out[idx] = v;
-    }
+    const TileDesc d{0, 1, padded(n), 1, 0, false, true};
+    tile_chunks<FWD, S, RM, 0>(lds, d, tw_of<PAIRED>(j), j.q, tid, nthr, sync);
+    for (uint32_t idx = tid; idx < n; idx += nthr) j.out[idx] = ntt_final<FWD>(lds[padded(idx)], j);
+}
+
+// Calls f(integral_constant S1, integral_constant S2) for the phase split of
+// logn (two-phase sizes) -- the set of instantiated geometries.
+template <class F>
+inline bool with_split(uint32_t logN1, uint32_t logN2, F&& f) {
+#define GF_SPLIT(a, b) if (logN1 == a && logN2 == b) { f(std::integral_constant<uint32_t, a>{}, std::integral_constant<uint32_t, b>{}); return true; }
+    GF_SPLIT(6, 7) GF_SPLIT(7, 7) GF_SPLIT(7, 8) GF_SPLIT(8, 8) GF_SPLIT(9, 8)
+#undef GF_SPLIT
+    return false;
+}
+template <class F>
+inline bool with_radix(uint32_t r, F&& f) {
+    if (r == 2) { f(std::integral_constant<uint32_t, 2>{}); return true; }
+    if (r == 3) { f(std::integral_constant<uint32_t, 3>{}); return true; }
+    if (r == 4) { f(std::integral_constant<uint32_t, 4>{}); return true; }
+    return false;
+}
+template <class F>
+inline bool with_single(uint32_t logn, F&& f) {
+#define GF_ONE(a) if (logn == a) { f(std::integral_constant<uint32_t, a>{}); return true; }
+    GF_ONE(1) GF_ONE(2) GF_ONE(3) GF_ONE(4) GF_ONE(5) GF_ONE(6) GF_ONE(7) GF_ONE(8) GF_ONE(9)
+    GF_ONE(10) GF_ONE(11) GF_ONE(12)
+#undef GF_ONE
+    return false;
 }
 
 }  // namespace gf

@@ -58,6 +58,17 @@ _capi = {
     "gf_plan_run_graph": (C.c_int, [vp, vp]),
     "gf_plan_launches": (u32, [vp]),
     "gf_plan_destroy": (None, [vp]),
+    "gf_plan_num_steps": (u32, [vp]),
+    "gf_plan_step_info": (C.c_int, [vp, u32, u32p, u32p]),
+    "gf_plan_run_step": (C.c_int, [vp, u32, vp]),
+    "gf_fill_uniform": (C.c_int, [vp, vp, u32, u64, vp]),
+    "gf_ctx_set_ntt_radix": (C.c_int, [vp, u32]),
+    "gf_ctx_get_ntt_radix": (u32, [vp]),
+    "gf_ctx_autotune": (C.c_int, [vp, vp, u32p, C.POINTER(C.c_float)]),
+    "gf_event_create": (C.c_int, [vpp]),
+    "gf_event_destroy": (C.c_int, [vp]),
+    "gf_event_record": (C.c_int, [vp, vp]),
+    "gf_event_elapsed_ms": (C.c_int, [C.POINTER(C.c_float), vp, vp]),
 }
 _kern = {
     "LaunchNTT_CT": [vp, vp, vp, u32, u64, vp],
@@ -170,6 +181,17 @@ class Context:
     def workspace(self):
         return Dev(self.ws_bytes // 8)
 
+    def set_ntt_radix(self, r):
+        check(lib.gf_ctx_set_ntt_radix(self.h, r))
+
+    def ntt_radix(self):
+        return lib.gf_ctx_get_ntt_radix(self.h)
+
+    def autotune(self, stream):
+        r, ms = C.c_uint32(), C.c_float()
+        check(lib.gf_ctx_autotune(self.h, stream.s, C.byref(r), C.byref(ms)))
+        return r.value, ms.value
+
     def __del__(self):
         if getattr(self, "h", None):
             lib.gf_ctx_destroy(self.h)
@@ -203,10 +225,45 @@ class Plan:
     def run_graph(self, stream):
         check(lib.gf_plan_run_graph(self.h, stream.s))
 
+    def steps(self):
+        out = []
+        k, j = C.c_uint32(), C.c_uint32()
+        for i in range(lib.gf_plan_num_steps(self.h)):
+            check(lib.gf_plan_step_info(self.h, i, C.byref(k), C.byref(j)))
+            out.append((STEP_KINDS[k.value], j.value))
+        return out
+
+    def run_step(self, i, stream):
+        check(lib.gf_plan_run_step(self.h, i, stream.s))
+
     def __del__(self):
         if getattr(self, "h", None):
             lib.gf_plan_destroy(self.h)
             self.h = None
+
+
+STEP_KINDS = ["ntt_fwd", "ntt_inv", "tensor", "conv", "inner", "combine", "switch", "fuse", "add",
+              "perm", "copy"]
+
+
+class Event:
+    def __init__(self):
+        e = C.c_void_p()
+        check(lib.gf_event_create(C.byref(e)))
+        self.e = e.value
+
+    def record(self, stream):
+        check(lib.gf_event_record(self.e, stream.s))
+
+    def elapsed_ms(self, later):
+        ms = C.c_float()
+        check(lib.gf_event_elapsed_ms(C.byref(ms), self.e, later.e))
+        return ms.value
+
+    def __del__(self):
+        if getattr(self, "e", None):
+            lib.gf_event_destroy(self.e)
+            self.e = None
 
 
 def plan_evalmult_rescale(ctx, key, levels, c0a, c1a, c0b, c1b, o0, o1, ws):

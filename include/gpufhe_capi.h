@@ -112,6 +112,31 @@ int gf_plan_run_graph(gf_plan* p, hipStream_t s);
 uint32_t gf_plan_launches(const gf_plan* p);
 void gf_plan_destroy(gf_plan* p);
 
+/* ---- plan introspection, for timing one stage on its own stream ----
+ * kind: 0 fwd NTT, 1 inverse NTT, 2 tensor, 3 basis conversion, 4 key inner
+ * product, 5 ModDown combine, 6 switch modulus, 7 rescale fuse, 8 add,
+ * 9 automorphism, 10 copy.  jobs = towers (or conversion chunks) per launch. */
+uint32_t gf_plan_num_steps(const gf_plan* p);
+int gf_plan_step_info(const gf_plan* p, uint32_t i, uint32_t* kind, uint32_t* jobs);
+int gf_plan_run_step(gf_plan* p, uint32_t i, hipStream_t s);
+
+/* ---- tuning ----
+ * NTT register radix log2 (2, 3 or 4) used by plans built afterwards; the
+ * autotuner times each on a synthetic tower batch and keeps the fastest. */
+int gf_ctx_set_ntt_radix(gf_ctx* c, uint32_t rmax);
+uint32_t gf_ctx_get_ntt_radix(const gf_ctx* c);
+int gf_ctx_autotune(gf_ctx* c, hipStream_t s, uint32_t* rmax_out, float* ms_out);
+
+/* ---- synthetic data: fill `level` towers at d with residues mod q_0..q_{level-1}
+ * derived from (seed, index) by a 64-bit mixing hash (benchmark inputs) ---- */
+int gf_fill_uniform(gf_ctx* c, uint64_t* d, uint32_t level, uint64_t seed, hipStream_t s);
+
+/* ---- events (plumbing for timing on a caller stream) ---- */
+int gf_event_create(hipEvent_t* e);
+int gf_event_destroy(hipEvent_t e);
+int gf_event_record(hipEvent_t e, hipStream_t s);
+int gf_event_elapsed_ms(float* ms, hipEvent_t start, hipEvent_t stop);
+
 #ifdef __cplusplus
 }
 #endif

@@ -2,33 +2,65 @@
 // Compiles the same headers the HIP kernels use with a plain C++ compiler and
 // runs each workgroup sequentially, so index and arithmetic mistakes show up
 // on the CPU against the oracle before any GPU run.
+#include <cstddef>
 #include <cstdint>
 #include <vector>
+
 #include "ntt_engine.hpp"
 
 using namespace gf;
 
 namespace {
-struct NoSync { void operator()() const {} };
-}
+struct NoSync {
+    void operator()() const {}
+};
 
-extern "C" void emu_ntt(uint64_t* x, uint32_t n, uint64_t q, const uint64_t* tw, const uint64_t* twp,
-                        int fwd, uint64_t post, uint64_t postp) {
-    NttGeom g = make_geom(n);
-    NttJob j{x, x, tw, twp, q, post, postp};
-    std::vector<uint64_t> lds(NTT_TILE > n ? NTT_TILE : n);
+template <bool PAIRED, uint32_t RM>
+void emu_ntt_t(const NttJob& j, uint32_t n, int fwd) {
+    NttGeom g = make_geom(n, RM);
+    std::vector<uint64_t> lds(NTT_TILE_PADDED > padded(n) ? NTT_TILE_PADDED : padded(n));
     if (g.single) {
-        if (fwd) single_body<true>(j, g, lds.data(), 0, 1, NoSync{});
-        else single_body<false>(j, g, lds.data(), 0, 1, NoSync{});
+        with_single(g.logn, [&](auto Sc) {
+            constexpr uint32_t S = decltype(Sc)::value;
+            if (fwd) single_body<true, PAIRED, S, RM>(j, lds.data(), 0, 1, NoSync{});
+            else single_body<false, PAIRED, S, RM>(j, lds.data(), 0, 1, NoSync{});
+        });
         return;
     }
-    const uint32_t tilesA = (1u << g.logN2) / g.C, tilesB = (1u << g.logN1) / g.R;
-    if (fwd) {
-        for (uint32_t t = 0; t < tilesA; ++t) phaseA_body<true>(j, g, t, lds.data(), 0, 1, NoSync{});
-        for (uint32_t t = 0; t < tilesB; ++t) phaseB_body<true>(j, g, t, lds.data(), 0, 1, NoSync{});
+    with_split(g.logN1, g.logN2, [&](auto A, auto B) {
+        constexpr uint32_t S1 = decltype(A)::value, S2 = decltype(B)::value;
+        const uint32_t tilesA = (1u << S2) / g.C, tilesB = (1u << S1) / g.R;
+        if (fwd) {
+            for (uint32_t t = 0; t < tilesA; ++t) phaseA_body<true, PAIRED, S1, RM>(j, g, t, lds.data(), 0, 1, NoSync{});
+            for (uint32_t t = 0; t < tilesB; ++t) phaseB_body<true, PAIRED, S2, RM>(j, g, t, lds.data(), 0, 1, NoSync{});
+        } else {
+            for (uint32_t t = 0; t < tilesB; ++t) phaseB_body<false, PAIRED, S2, RM>(j, g, t, lds.data(), 0, 1, NoSync{});
+            for (uint32_t t = 0; t < tilesA; ++t) phaseA_body<false, PAIRED, S1, RM>(j, g, t, lds.data(), 0, 1, NoSync{});
+        }
+    });
+}
+
+template <bool PAIRED>
+void emu_ntt_r(const NttJob& j, uint32_t n, int fwd, uint32_t rmax) {
+    with_radix(rmax, [&](auto Rc) { emu_ntt_t<PAIRED, decltype(Rc)::value>(j, n, fwd); });
+}
+}  // namespace
+
+// In-place transform of x (n residues mod q).  paired != 0 builds an
+// interleaved (w, precon) table from tw/twp; rmax = register radix log2.
+extern "C" void emu_ntt(uint64_t* x, uint32_t n, uint64_t q, const uint64_t* tw, const uint64_t* twp,
+                        int fwd, uint64_t post, uint64_t postp, int paired, uint32_t rmax) {
+    if (paired) {
+        std::vector<uint64_t> t2(2 * (std::size_t)n);
+        for (uint32_t i = 0; i < n; ++i) {
+            t2[2 * i] = tw[i];
+            t2[2 * i + 1] = twp[i];
+        }
+        NttJob j{x, x, t2.data(), nullptr, q, post, postp};
+        emu_ntt_r<true>(j, n, fwd, rmax);
     } else {
-        for (uint32_t t = 0; t < tilesB; ++t) phaseB_body<false>(j, g, t, lds.data(), 0, 1, NoSync{});
-        for (uint32_t t = 0; t < tilesA; ++t) phaseA_body<false>(j, g, t, lds.data(), 0, 1, NoSync{});
+        NttJob j{x, x, tw, twp, q, post, postp};
+        emu_ntt_r<false>(j, n, fwd, rmax);
     }
 }
 
@@ -39,3 +71,6 @@ extern "C" uint64_t emu_shoup(uint64_t x, uint64_t w, uint64_t q) {
     return shoup(x, w, host_precon(w, q), q);
 }
 extern "C" uint64_t emu_mulhi(uint64_t a, uint64_t b) { return mulhi64(a, b); }
+extern "C" uint64_t emu_redc(uint64_t lo, uint64_t hi, uint64_t q) {
+    return redc(U128{lo, hi}, q, host_qneg(q));
+}

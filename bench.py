@@ -56,6 +56,18 @@ def step_bytes(n, L, K, alpha, np_parts, kinds):
     T = 8 * n
     sc = [L - min(alpha, L - alpha * p) + K for p in range(np_parts)]
     sp = [min(alpha, L - alpha * p) for p in range(np_parts)]
+    if len(kinds) == 9:   # fused ModDown + rescale plan
+        return [
+            ("tensor", 7 * T * L),
+            ("intt_modup", 2 * T * (L + 2)),             # + INTT of t0/t1's last tower
+            ("conv_modup", T * (sum(sp) + sum(sc))),
+            ("ntt_digits", 2 * T * sum(sc)),
+            ("key_inner_product", T * (L + K) * (3 * np_parts + 2)),
+            ("intt_moddown", 2 * T * (2 * K + 2)),       # + INTT of res0/res1's last tower
+            ("drop_tower", T * 2 * (K + 3)),
+            ("moddown_rescale_prologue", T * 2 * (K + 1 + (L - 1))),
+            ("ntt_rescale_epilogue", 4 * T * 2 * (L - 1)),   # transform + t, res operands
+        ]
     table = {
         0: ("tensor", 7 * T * L),
         1: ("intt_modup", 2 * T * L),

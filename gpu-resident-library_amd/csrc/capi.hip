@@ -264,6 +264,13 @@ int gf_ctx_set_ntt_radix(gf_ctx* c, uint32_t r) {
     });
 }
 uint32_t gf_ctx_get_ntt_radix(const gf_ctx* c) { return c->C->geom.rmax; }
+int gf_ctx_set_fusion(gf_ctx* c, int on) {
+    return guard([&] {
+        std::lock_guard<std::mutex> lk(c->mu);
+        c->C->fused = on != 0;
+        c->cache.clear();
+    });
+}
 int gf_ctx_autotune(gf_ctx* c, hipStream_t s, uint32_t* r, float* ms) {
     return guard([&] {
         std::lock_guard<std::mutex> lk(c->mu);

@@ -168,14 +168,29 @@ Ctx::Ctx(uint32_t n_, const std::vector<uint64_t>& q, const std::vector<uint64_t
         // (s2 = floor(Q' [Q'^-1]_{q_l} / q_l) mod q_i is congruent to -s1, src/params.cpp:50-75)
         if (l >= 2) {
             const uint64_t ql = mods[l - 1];
+            std::vector<uint64_t> tabRS((size_t)K * (l - 1)), tg;
             for (uint32_t i = 0; i + 1 < l; ++i) {
-                const uint64_t s1 = host_invmod(ql % mods[i], mods[i]);
-                const uint64_t s2 = s1 ? mods[i] - s1 : 0;
+                const uint64_t qi = mods[i];
+                const uint64_t s1 = host_invmod(ql % qi, qi);
+                const uint64_t s2 = s1 ? qi - s1 : 0;
                 c.s1.push_back(s1);
-                c.s1p.push_back(host_precon(s1, mods[i]));
+                c.s1p.push_back(host_precon(s1, qi));
                 c.s2.push_back(s2);
-                c.s2p.push_back(host_precon(s2, mods[i]));
+                c.s2p.push_back(host_precon(s2, qi));
+                // fused ModDown + rescale: P^-1 s1 (epilogue) and -P^-1 s1 folded into PHatModq
+                const uint64_t ps1 = host_mulmod(c.pinv[i], s1, qi);
+                c.ps1.push_back(ps1);
+                c.ps1p.push_back(host_precon(ps1, qi));
+                const uint64_t neg = ps1 ? qi - ps1 : 0;
+                for (uint32_t k = 0; k < K; ++k)
+                    tabRS[(size_t)k * (l - 1) + i] = host_mulmod(tab[(size_t)k * l + i], neg, qi);
+                const SwitchJob sj = make_switch(nullptr, nullptr, ql, qi);
+                const RsTarget t{qi, host_qneg(qi), sj.diff, sj.new_larger, s2, host_precon(s2, qi)};
+                const uint64_t* tw = (const uint64_t*)&t;
+                tg.insert(tg.end(), tw, tw + sizeof(RsTarget) / 8);
             }
+            put(tabRS, &c.d_tabRS);
+            put(tg, (const uint64_t**)&c.d_rstg);
         }
     }
     consts.alloc(blob.size() * 8);
@@ -197,7 +212,9 @@ WsLayout WsLayout::make(const Ctx& C) {
     w.res1 = w.res0 + W;
     w.xp = w.res1 + W;
     w.drop = w.xp + 2 * K;
-    w.total = w.drop + 2;
+    w.dt = w.drop + 2;     // INTT of t0/t1's last tower (fused rescale)
+    w.dr = w.dt + 2;       // INTT of res0/res1's last tower
+    w.total = w.dr + 2;
     return w;
 }
 size_t Ctx::ws_words() const { return WsLayout::make(*this).total * (size_t)n; }
@@ -306,6 +323,8 @@ void Plan::run_step(size_t i, hipStream_t s) const {
             case Step::ADD: launch_add((const AddJob*)p, st.count, n, s); break;
             case Step::PERM: launch_perm((const PermJob*)p, st.count, n, logn, s); break;
             case Step::COPY: launch_copy((const CopyJob*)p, st.count, n, s); break;
+            case Step::DROP: launch_drop((const DropJob*)p, st.count, n, s); break;
+            case Step::RSV: launch_rs((const RsJob*)p, st.count, n, s); break;
         }
     }
 }
@@ -325,9 +344,13 @@ void Plan::run_graph(hipStream_t s) {
 // --------------------------------------------------------------- builder --
 static const uint32_t CONV_CHUNK = 8;
 
-void PlanBuilder::keyswitch(uint32_t l, const uint64_t* d, uint64_t* ba0, uint64_t* ba1,
-                            const DevKey& key, uint64_t* ws, const uint64_t* t0,
-                            const uint64_t* t1) {
+// Hybrid keyswitch up to the ModDown sources (src/keyswitch_resident.cpp:139-201):
+// ModUp of every digit, the key inner product over Q_l P, and the INTT of both
+// accumulators' P towers scaled by n^-1 PHatInv.  Optional extra inverse
+// transforms (plain n^-1) ride in the first INTT launch.
+void PlanBuilder::keyswitch_core(uint32_t l, const uint64_t* d, const DevKey& key, uint64_t* ws,
+                                 const uint64_t* extra0, const uint64_t* extra1, uint64_t* dx0,
+                                 uint64_t* dx1) {
     const Ctx& C = C_;
     if (l < 1 || l > C.L) throw std::runtime_error("gpufhe: bad level");
     const uint32_t n = C.n, K = C.K, W = l + K;
@@ -338,13 +361,18 @@ void PlanBuilder::keyswitch(uint32_t l, const uint64_t* d, uint64_t* ba0, uint64
     uint64_t* dig = ws + w.dig * n;
     uint64_t* res[2] = {ws + w.res0 * n, ws + w.res1 * n};
     uint64_t* xp = ws + w.xp * n;
-    uint64_t* conv = x;  // the part towers are dead once ModUp has converted them
     // 1. part towers to coefficient form, scaled by n^-1 QHatInv (ModUp's first factor)
     for (uint32_t i = 0; i < l; ++i) {
         const uint32_t pi = i / C.alpha, ii = i - pi * C.alpha;
         intt_up_.push_back(NttJob{d + (size_t)i * n, x + (size_t)i * n, C.rt[i].iv2, nullptr,
                                   C.mods[i], lv.parts[pi].post[ii], lv.parts[pi].postp[ii]});
     }
+    const uint64_t* ex[2] = {extra0, extra1};
+    uint64_t* dx[2] = {dx0, dx1};
+    for (int c = 0; c < 2; ++c)
+        if (ex[c])
+            intt_up_.push_back(NttJob{ex[c], dx[c], C.rt[l - 1].iv2, nullptr, C.mods[l - 1], C.rt[l - 1].ninv,
+                                      C.rt[l - 1].ninvp});
     // 2-3. ModUp: convert every part to its complement, back to evaluation form
     for (uint32_t pi = 0; pi < lv.np; ++pi) {
         const PartConst& P = lv.parts[pi];
@@ -379,15 +407,30 @@ void PlanBuilder::keyswitch(uint32_t l, const uint64_t* d, uint64_t* ba0, uint64
         J.qneg = C.rc[row].qneg;
         inner_.push_back(J);
     }
-    // 5-8. ApproxModDown of both accumulators, fused with r = t + ba
-    uint64_t* outs[2] = {ba0, ba1};
-    const uint64_t* ts[2] = {t0, t1};
-    for (int c = 0; c < 2; ++c) {
+    // 5. ModDown sources: P towers of both accumulators to coefficient form
+    for (int c = 0; c < 2; ++c)
         for (uint32_t k = 0; k < K; ++k) {
             const uint32_t r = C.row_p(k);
             intt_p_.push_back(NttJob{res[c] + (size_t)(l + k) * n, xp + (size_t)(c * K + k) * n, C.rt[r].iv2,
                                      nullptr, C.mods[r], lv.pd_post[k], lv.pd_postp[k]});
         }
+}
+
+void PlanBuilder::keyswitch(uint32_t l, const uint64_t* d, uint64_t* ba0, uint64_t* ba1,
+                            const DevKey& key, uint64_t* ws, const uint64_t* t0,
+                            const uint64_t* t1) {
+    const Ctx& C = C_;
+    keyswitch_core(l, d, key, ws, nullptr, nullptr, nullptr, nullptr);
+    const uint32_t n = C.n, K = C.K;
+    const LevelConst& lv = C.lv[l];
+    const WsLayout w = WsLayout::make(C);
+    uint64_t* res[2] = {ws + w.res0 * n, ws + w.res1 * n};
+    uint64_t* xp = ws + w.xp * n;
+    uint64_t* conv = ws + w.x * n;  // the part towers are dead once ModUp has converted them
+    // 6-8. ApproxModDown of both accumulators, fused with r = t + ba (:203-224)
+    uint64_t* outs[2] = {ba0, ba1};
+    const uint64_t* ts[2] = {t0, t1};
+    for (int c = 0; c < 2; ++c) {
         uint64_t* cv = conv + (size_t)c * l * n;
         for (uint32_t j0 = 0; j0 < l; j0 += CONV_CHUNK)
             conv_dn_.push_back(ConvJob{xp + (size_t)c * K * n, cv, lv.d_tabMD, lv.d_tqMD, lv.d_tqnegMD, K, l,
@@ -398,6 +441,57 @@ void PlanBuilder::keyswitch(uint32_t l, const uint64_t* d, uint64_t* ba0, uint64
             combine_.push_back(CombineJob{res[c] + (size_t)i * n, cv + (size_t)i * n,
                                           ts[c] ? ts[c] + (size_t)i * n : nullptr, outs[c] + (size_t)i * n,
                                           C.mods[i], lv.pinv[i], lv.pinvp[i]});
+        }
+    }
+}
+
+// EvalMult + Rescale with ModDown and the rescale merged through NTT linearity:
+//   drop_c = INTT(t_c,last) + (INTT(res_c,last) - conv_c,last) P^-1      (coefficient form)
+//   y_c,i  = t_c,i s1_i + res_c,i P^-1 s1_i + NTT_i(s2_i switch(drop_c) - P^-1 s1_i conv_c,i)
+// which equals rescale(t + ModDown(res)) of the reference
+// (tests/test_evalmult_rescale.cpp:69-73) residue for residue, with one
+// forward transform per surviving tower instead of two.
+void PlanBuilder::evalmult_rescale_fused(uint32_t l, const uint64_t* c0a, const uint64_t* c1a,
+                                         const uint64_t* c0b, const uint64_t* c1b, uint64_t* o0,
+                                         uint64_t* o1, const DevKey& key, uint64_t* ws) {
+    const Ctx& C = C_;
+    const uint32_t n = C.n, K = C.K, last = l - 1;
+    const LevelConst& lv = C.lv[l];
+    const WsLayout w = WsLayout::make(C);
+    uint64_t* t[3] = {ws + w.t0 * n, ws + w.t1 * n, ws + w.t2 * n};
+    for (uint32_t i = 0; i < l; ++i) {
+        const size_t o = (size_t)i * n;
+        tensor_.push_back(TensorJob{c0a + o, c1a + o, c0b + o, c1b + o, t[0] + o, t[1] + o, t[2] + o,
+                                    C.rc[i].q, C.rc[i].qneg, C.rc[i].r2});
+    }
+    uint64_t* dt = ws + w.dt * n;
+    uint64_t* dr = ws + w.dr * n;
+    keyswitch_core(l, t[2], key, ws, t[0] + (size_t)last * n, t[1] + (size_t)last * n, dt, dt + n);
+    uint64_t* res[2] = {ws + w.res0 * n, ws + w.res1 * n};
+    uint64_t* xp = ws + w.xp * n;
+    uint64_t* drop = ws + w.drop * n;
+    uint64_t* v = ws + w.dig * n;     // digits are dead after the inner product
+    uint64_t* outs[2] = {o0, o1};
+    for (int c = 0; c < 2; ++c) {
+        intt_p_.push_back(NttJob{res[c] + (size_t)last * n, dr + (size_t)c * n, C.rt[last].iv2, nullptr,
+                                 C.mods[last], C.rt[last].ninv, C.rt[last].ninvp});
+        drop_.push_back(DropJob{xp + (size_t)c * K * n, lv.d_tabMD + last, dt + (size_t)c * n, dr + (size_t)c * n,
+                                drop + (size_t)c * n, C.rc[last].q, C.rc[last].qneg, lv.pinv[last],
+                                lv.pinvp[last], K, l});
+        uint64_t* vc = v + (size_t)c * last * n;
+        for (uint32_t j0 = 0; j0 < last; j0 += CONV_CHUNK)
+            rs_.push_back(RsJob{xp + (size_t)c * K * n, drop + (size_t)c * n, vc, lv.d_tabRS, lv.d_rstg,
+                                C.mods[last], C.mods[last] >> 1, K, last, j0,
+                                j0 + CONV_CHUNK < last ? j0 + CONV_CHUNK : last});
+        for (uint32_t i = 0; i < last; ++i) {
+            NttJob J{vc + (size_t)i * n, outs[c] + (size_t)i * n, C.rt[i].fw2, nullptr, C.mods[i], 0, 0};
+            J.e0 = t[c] + (size_t)i * n;
+            J.c0 = lv.s1[i];
+            J.c0p = lv.s1p[i];
+            J.e1 = res[c] + (size_t)i * n;
+            J.c1 = lv.ps1[i];
+            J.c1p = lv.ps1p[i];
+            ntt_v_.push_back(J);
         }
     }
 }
@@ -451,6 +545,10 @@ void PlanBuilder::evalmult_rescale(uint32_t l, const uint64_t* c0a, const uint64
                                    const DevKey& key, uint64_t* ws) {
     const Ctx& C = C_;
     if (l < 2) throw std::runtime_error("gpufhe: EvalMult+Rescale needs at least two towers");
+    if (C.fused) {
+        evalmult_rescale_fused(l, c0a, c1a, c0b, c1b, o0, o1, key, ws);
+        return;
+    }
     const WsLayout w = WsLayout::make(C);
     uint64_t* r0 = ws + w.t0 * C.n;
     uint64_t* r1 = ws + w.t1 * C.n;
@@ -515,6 +613,9 @@ std::unique_ptr<Plan> PlanBuilder::finish() {
     GF_STAGE(SWITCH, switch_);
     GF_STAGE(NTT_F, ntt_sw_);
     GF_STAGE(FUSE, fuse_);
+    GF_STAGE(DROP, drop_);
+    GF_STAGE(RSV, rs_);
+    GF_STAGE(NTT_F, ntt_v_);
     GF_STAGE(ADD, add_);
 #undef GF_STAGE
     P->dev.alloc(blob.size());

@@ -66,6 +66,9 @@ struct LevelConst {
     const uint64_t* d_tqnegMD = nullptr;
     std::vector<uint64_t> pinv, pinvp;         // P^-1 mod q_i
     std::vector<uint64_t> s1, s1p, s2, s2p;    // rescale constants dropping tower l-1
+    std::vector<uint64_t> ps1, ps1p;           // P^-1 s1 mod q_i (fused ModDown + rescale)
+    const uint64_t* d_tabRS = nullptr;         // PHatModq * (-P^-1 s1) * R mod q_i, [K][l-1]
+    const RsTarget* d_rstg = nullptr;          // [l-1]
 };
 
 // Resident parameter set: Q moduli (L), P moduli (K), part width alpha.
@@ -80,6 +83,7 @@ public:
     std::vector<RootTable> rt;
     std::vector<LevelConst> lv;               // index l = 1..L (0 unused)
     DevBuf consts;                            // all per-level device constants
+    bool fused = true;                        // EvalMult+Rescale with merged ModDown/rescale
     uint32_t row_q(uint32_t i) const { return i; }
     uint32_t row_p(uint32_t k) const { return L + k; }
     uint32_t num_parts(uint32_t l) const { return (l + alpha - 1) / alpha; }
@@ -102,7 +106,7 @@ std::unique_ptr<DevKey> upload_key(const Ctx& C, const uint64_t* av, const uint6
 
 // One launch step of a plan
 struct Step {
-    enum Kind { NTT_F, NTT_I, TENSOR, CONV, INNER, COMBINE, SWITCH, FUSE, ADD, PERM, COPY } kind;
+    enum Kind { NTT_F, NTT_I, TENSOR, CONV, INNER, COMBINE, SWITCH, FUSE, ADD, PERM, COPY, DROP, RSV } kind;
     size_t off;     // byte offset of the first job in the plan's device buffer
     uint32_t count;
 };
@@ -160,13 +164,21 @@ private:
     std::vector<SwitchJob> switch_;
     std::vector<NttJob> ntt_sw_;
     std::vector<FuseJob> fuse_;
+    std::vector<DropJob> drop_;
+    std::vector<RsJob> rs_;
+    std::vector<NttJob> ntt_v_;
+    void evalmult_rescale_fused(uint32_t l, const uint64_t* c0a, const uint64_t* c1a,
+                                const uint64_t* c0b, const uint64_t* c1b, uint64_t* o0, uint64_t* o1,
+                                const DevKey& key, uint64_t* ws);
+    void keyswitch_core(uint32_t l, const uint64_t* d, const DevKey& key, uint64_t* ws,
+                        const uint64_t* extra0, const uint64_t* extra1, uint64_t* dx0, uint64_t* dx1);
     std::vector<AddJob> add_;
     void rescale_comp(uint32_t l, const uint64_t* c, uint64_t* o, uint64_t* drop, uint64_t* sw);
 };
 
 // workspace offsets inside one op's workspace (in towers)
 struct WsLayout {
-    size_t t0, t1, t2, x, dig, res0, res1, xp, drop, total;
+    size_t t0, t1, t2, x, dig, res0, res1, xp, drop, dt, dr, total;
     static WsLayout make(const Ctx& C);
 };
 

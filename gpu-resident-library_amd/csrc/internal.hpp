@@ -91,6 +91,36 @@ struct PermJob {
     uint64_t k;
 };
 
+// Dropped tower of r = t + ba, built in coefficient form (q = q_last):
+//   out = dt + (dr - REDC(sum_k xp[k] * tab[k*stride])) * pinv
+// dt = INTT(t_last), dr = INTT(res_last), xp = INTT'd P towers scaled by n^-1 PHatInv.
+struct DropJob {
+    const uint64_t* xp;
+    const uint64_t* tab;
+    const uint64_t* dt;
+    const uint64_t* dr;
+    uint64_t* out;
+    uint64_t q, qneg, pinv, pinvp;
+    uint32_t K, stride;
+};
+
+// per surviving tower constants of the fused ModDown + rescale prologue
+struct RsTarget {
+    uint64_t q, qneg, diff, new_larger, s2, s2p;
+};
+// v_i = s2_i * switch(drop -> q_i) + REDC(sum_k xp[k] * tab[k*ntgt + i]) for i in [j0, j1);
+// tab folds -P^-1 s1 into PHatModq, so NTT(v_i) + t_i s1 + res_i P^-1 s1 is the
+// rescaled output tower.
+struct RsJob {
+    const uint64_t* xp;
+    const uint64_t* drop;
+    uint64_t* out;
+    const uint64_t* tab;
+    const RsTarget* tg;
+    uint64_t q_old, half;
+    uint32_t K, ntgt, j0, j1;
+};
+
 // copy one tower
 struct CopyJob {
     const uint64_t* in;
@@ -116,6 +146,8 @@ void launch_perm(const PermJob* d, uint32_t nj, uint32_t n, uint32_t logn, hipSt
 void launch_copy(const CopyJob* d, uint32_t nj, uint32_t n, hipStream_t s);
 void launch_fill(uint64_t* d, const uint64_t* d_mods, uint32_t towers, uint32_t n, uint64_t seed,
                  hipStream_t s);
+void launch_drop(const DropJob* d, uint32_t nj, uint32_t n, hipStream_t s);
+void launch_rs(const RsJob* d, uint32_t nj, uint32_t n, hipStream_t s);
 
 inline SwitchJob make_switch(const uint64_t* in, uint64_t* out, uint64_t q_old, uint64_t q_new) {
     SwitchJob j;

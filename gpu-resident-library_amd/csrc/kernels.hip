@@ -177,6 +177,44 @@ __global__ __launch_bounds__(PW_THREADS) void k_fuse(const FuseJob* __restrict__
     }
 }
 
+__global__ __launch_bounds__(PW_THREADS) void k_drop(const DropJob* __restrict__ jobs,
+                                                     uint32_t off, uint32_t n) {
+    const DropJob j = jobs[off + blockIdx.y];
+    PW_FOR(n) {
+        U128 acc{0, 0};
+        for (uint32_t k = 0; k < j.K; ++k) acc = add128(acc, mul128(j.xp[(uint64_t)k * n + i], j.tab[k * j.stride]));
+        const uint64_t conv = redc(acc, j.q, j.qneg);
+        j.out[i] = addmod(j.dt[i], shoup(submod(j.dr[i], conv, j.q), j.pinv, j.pinvp, j.q), j.q);
+    }
+}
+
+__global__ __launch_bounds__(PW_THREADS) void k_rs(const RsJob* __restrict__ jobs, uint32_t off,
+                                                   uint32_t n) {
+    const RsJob j = jobs[off + blockIdx.y];
+    PW_FOR(n) {
+        uint64_t x[CONV_MAX_SRC];
+        for (uint32_t k = 0; k < j.K; ++k) x[k] = j.xp[(uint64_t)k * n + i];
+        const uint64_t dv = j.drop[i];
+        const bool hi = dv > j.half;
+        for (uint32_t t = j.j0; t < j.j1; ++t) {
+            const RsTarget tg = j.tg[t];
+            U128 acc{0, 0};
+            for (uint32_t k = 0; k < j.K; ++k) acc = add128(acc, mul128(x[k], j.tab[k * j.ntgt + t]));
+            const uint64_t conv = redc(acc, tg.q, tg.qneg);
+            // centred switch q_old -> q_t (kernels/switch_modulus.cu:11-27 semantics)
+            uint64_t sw;
+            if (tg.new_larger) {
+                sw = dv + (hi ? tg.diff : 0);
+            } else {
+                const uint64_t d = hi ? tg.diff : 0;
+                const uint64_t xm = dv >= tg.q ? dv % tg.q : dv;
+                sw = xm >= d ? xm - d : xm + tg.q - d;
+            }
+            j.out[(uint64_t)t * n + i] = addmod(shoup(sw, tg.s2, tg.s2p, tg.q), conv, tg.q);
+        }
+    }
+}
+
 __global__ __launch_bounds__(PW_THREADS) void k_add(const AddJob* __restrict__ jobs,
                                                     uint32_t off, uint32_t n) {
     const AddJob j = jobs[off + blockIdx.y];
@@ -236,6 +274,8 @@ GF_PW_LAUNCH(launch_switch, k_switch, SwitchJob)
 GF_PW_LAUNCH(launch_fuse, k_fuse, FuseJob)
 GF_PW_LAUNCH(launch_add, k_add, AddJob)
 GF_PW_LAUNCH(launch_copy, k_copy, CopyJob)
+GF_PW_LAUNCH(launch_drop, k_drop, DropJob)
+GF_PW_LAUNCH(launch_rs, k_rs, RsJob)
 
 void launch_fill(uint64_t* d, const uint64_t* d_mods, uint32_t towers, uint32_t n, uint64_t seed,
                  hipStream_t s) {

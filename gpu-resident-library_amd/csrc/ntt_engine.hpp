@@ -35,6 +35,10 @@ struct NttJob {
     const uint64_t* twp;  // Shoup precons when tw is not paired, else unused
     uint64_t q;
     uint64_t post, postp; // inverse only: final multiplier (n^-1 * c) and precon
+    // forward only, optional epilogue: out = NTT(in) + e0*c0 + e1*c1 (mod q)
+    const uint64_t* e0;
+    const uint64_t* e1;
+    uint64_t c0, c0p, c1, c1p;
 };
 
 struct NttGeom {
@@ -207,6 +211,15 @@ GF_HD uint64_t ntt_final(uint64_t v, const NttJob& j) {
     if (FWD) return csub(csub(v, 2 * j.q), j.q);
     return shoup(v, j.post, j.postp, j.q);
 }
+// forward final store with the optional linear epilogue at global index gi
+GF_HD uint64_t ntt_final_ep(uint64_t v, const NttJob& j, uint64_t gi) {
+    uint64_t r = csub(csub(v, 2 * j.q), j.q);
+    if (j.e0) {
+        r = addmod(r, shoup(j.e0[gi], j.c0, j.c0p, j.q), j.q);
+        r = addmod(r, shoup(j.e1[gi], j.c1, j.c1p, j.q), j.q);
+    }
+    return r;
+}
 
 // ---- phase bodies (shared by the kernels and the host emulation) ----
 // Phase A: tile = columns [b0, b0+C) x all N1 = 2^S1 rows.
@@ -247,7 +260,7 @@ GF_HD void phaseB_body(const NttJob& j, const NttGeom& g, uint32_t tile, uint64_
     tile_chunks<FWD, S2, RM, 0>(lds, d, tw_of<PAIRED>(j), j.q, tid, nthr, sync);
     // inverse phase B hands lazy values in [0,2q) to phase A
     for (uint32_t idx = tid; idx < R * N2; idx += nthr)
-        j.out[off + idx] = FWD ? ntt_final<true>(lds[padded(idx)], j) : lds[padded(idx)];
+        j.out[off + idx] = FWD ? ntt_final_ep(lds[padded(idx)], j, off + idx) : lds[padded(idx)];
 }
 
 // Single phase: the whole tower (n = 2^S <= NTT_SINGLE_MAX) is one tile.
@@ -258,7 +271,8 @@ GF_HD void single_body(const NttJob& j, uint64_t* lds, uint32_t tid, uint32_t nt
     sync();
     const TileDesc d{0, 1, padded(n), 1, 0, false, true};
     tile_chunks<FWD, S, RM, 0>(lds, d, tw_of<PAIRED>(j), j.q, tid, nthr, sync);
-    for (uint32_t idx = tid; idx < n; idx += nthr) j.out[idx] = ntt_final<FWD>(lds[padded(idx)], j);
+    for (uint32_t idx = tid; idx < n; idx += nthr)
+        j.out[idx] = FWD ? ntt_final_ep(lds[padded(idx)], j, idx) : ntt_final<false>(lds[padded(idx)], j);
 }
 
 // Calls f(integral_constant S1, integral_constant S2) for the phase split of

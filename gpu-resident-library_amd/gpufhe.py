@@ -65,6 +65,7 @@ _capi = {
     "gf_ctx_set_ntt_radix": (C.c_int, [vp, u32]),
     "gf_ctx_get_ntt_radix": (u32, [vp]),
     "gf_ctx_autotune": (C.c_int, [vp, vp, u32p, C.POINTER(C.c_float)]),
+    "gf_ctx_set_fusion": (C.c_int, [vp, C.c_int]),
     "gf_event_create": (C.c_int, [vpp]),
     "gf_event_destroy": (C.c_int, [vp]),
     "gf_event_record": (C.c_int, [vp, vp]),
@@ -184,6 +185,9 @@ class Context:
     def set_ntt_radix(self, r):
         check(lib.gf_ctx_set_ntt_radix(self.h, r))
 
+    def set_fusion(self, on):
+        check(lib.gf_ctx_set_fusion(self.h, 1 if on else 0))
+
     def ntt_radix(self):
         return lib.gf_ctx_get_ntt_radix(self.h)
 
@@ -243,7 +247,7 @@ class Plan:
 
 
 STEP_KINDS = ["ntt_fwd", "ntt_inv", "tensor", "conv", "inner", "combine", "switch", "fuse", "add",
-              "perm", "copy"]
+              "perm", "copy", "drop", "rsv"]
 
 
 class Event:

@@ -115,7 +115,8 @@ void gf_plan_destroy(gf_plan* p);
 /* ---- plan introspection, for timing one stage on its own stream ----
  * kind: 0 fwd NTT, 1 inverse NTT, 2 tensor, 3 basis conversion, 4 key inner
  * product, 5 ModDown combine, 6 switch modulus, 7 rescale fuse, 8 add,
- * 9 automorphism, 10 copy.  jobs = towers (or conversion chunks) per launch. */
+ * 9 automorphism, 10 copy, 11 dropped-tower build, 12 fused ModDown/rescale
+ * prologue.  jobs = towers (or conversion chunks) per launch. */
 uint32_t gf_plan_num_steps(const gf_plan* p);
 int gf_plan_step_info(const gf_plan* p, uint32_t i, uint32_t* kind, uint32_t* jobs);
 int gf_plan_run_step(gf_plan* p, uint32_t i, hipStream_t s);
@@ -126,6 +127,11 @@ int gf_plan_run_step(gf_plan* p, uint32_t i, hipStream_t s);
 int gf_ctx_set_ntt_radix(gf_ctx* c, uint32_t rmax);
 uint32_t gf_ctx_get_ntt_radix(const gf_ctx* c);
 int gf_ctx_autotune(gf_ctx* c, hipStream_t s, uint32_t* rmax_out, float* ms_out);
+/* 1 (default): EvalMult+Rescale merges ModDown and the rescale through NTT
+ * linearity (one forward transform per surviving tower); 0: the reference's
+ * stage sequence (ModDown NTTs, combine, INTT/switch/NTT/fuse).  Results are
+ * identical; only plans built afterwards are affected. */
+int gf_ctx_set_fusion(gf_ctx* c, int on);
 
 /* ---- synthetic data: fill `level` towers at d with residues mod q_0..q_{level-1}
  * derived from (seed, index) by a 64-bit mixing hash (benchmark inputs) ---- */

@@ -83,6 +83,28 @@ def test_evalmult_rescale(G, n, L, dnum):
         assert np.array_equal(o1.download(), w1), level
 
 
+@pytest.mark.parametrize("n,L,dnum", [SHAPES[1], SHAPES[4]])
+def test_evalmult_rescale_unfused_sequence(G, n, L, dnum):
+    """The reference's stage sequence (ModDown NTTs, combine, INTT/switch/NTT/
+    fuse) and the merged default give the same residues as the oracle."""
+    P = make_params(n, L, dnum)
+    rng = np.random.default_rng(3)
+    av = np.concatenate([rng.integers(0, int(m), n, dtype=np.uint64) for _ in range(P.dnum) for m in P.mods])
+    bv = np.concatenate([rng.integers(0, int(m), n, dtype=np.uint64) for _ in range(P.dnum) for m in P.mods])
+    ctx = G.Context(n, P.q, P.p, P.roots, dnum)
+    ctx.set_fusion(False)
+    k = G.Key(ctx, av, bv, P.dnum)
+    s = G.Stream()
+    ws = ctx.workspace()
+    cts = [rand_ct(P, L, rng) for _ in range(4)]
+    d = [G.Dev.of(c) for c in cts]
+    o0, o1 = G.Dev((L - 1) * n), G.Dev((L - 1) * n)
+    G.check(G.lib.gf_evalmult_rescale(ctx.h, k.h, L, *[x.ptr for x in d], o0.ptr, o1.ptr, ws.ptr, s.s))
+    s.sync()
+    w0, w1 = P.evalmult_rescale(L, *cts, av, bv)
+    assert np.array_equal(o0.download(), w0) and np.array_equal(o1.download(), w1)
+
+
 @pytest.mark.parametrize("n,L,dnum", SHAPES[:3])
 def test_evalmult_norescale_and_rescale(G, n, L, dnum):
     P, ctx, k, av, bv = setup(G, n, L, dnum)

@@ -3,6 +3,7 @@
 
 #include <cstring>
 #include <map>
+#include <algorithm>
 #include <tuple>
 
 #include "engine.hpp"
@@ -583,7 +584,21 @@ void PlanBuilder::add(uint32_t l, const uint64_t* a0, const uint64_t* a1, const 
     }
 }
 
+// Jobs of one stage are independent; ordering them so that consecutive
+// workgroups share a twiddle table (transforms), a key row (inner product) or a
+// constant table (conversions) keeps those operands in L2 across the batch
+// instead of re-reading them per ciphertext.
+template <class T, class K>
+static void group_by(std::vector<T>& v, K key) {
+    std::stable_sort(v.begin(), v.end(), [&](const T& a, const T& b) { return key(a) < key(b); });
+}
+
 std::unique_ptr<Plan> PlanBuilder::finish() {
+    for (auto* v : {&intt_up_, &ntt_dig_, &intt_p_, &ntt_conv_, &intt_drop_, &ntt_sw_, &ntt_v_})
+        group_by(*v, [](const NttJob& j) { return (uintptr_t)j.tw; });
+    group_by(inner_, [](const InnerJob& j) { return (uintptr_t)j.kb[0]; });
+    group_by(conv_up_, [](const ConvJob& j) { return (uintptr_t)j.tab; });
+    group_by(tensor_, [](const TensorJob& j) { return j.q; });
     auto P = std::make_unique<Plan>();
     P->n = C_.n;
     P->logn = C_.logn;

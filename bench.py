@@ -48,25 +48,44 @@ def native_params(G, n, L, dnum):
     return q, p, roots, K, alpha
 
 
+STEP_KERNELS = {"ntt_fwd": ("k_ntt_A", "k_ntt_B"), "ntt_inv": ("k_ntt_B", "k_ntt_A"),
+                "tensor": ("k_tensor",), "conv": ("k_conv",), "inner": ("k_inner",),
+                "combine": ("k_combine",), "switch": ("k_switch",), "fuse": ("k_fuse",),
+                "add": ("k_add",), "perm": ("k_perm",), "copy": ("k_copy",), "drop": ("k_drop",),
+                "rsv": ("k_rs",)}
+
+
+def kernel_bytes(name, kind, nkern, total, epilogue):
+    """Algorithmic bytes of each kernel of a step.  A transform's 16 n B per
+    tower is split evenly over its two passes; operands read by a fused
+    epilogue belong to the second (row) pass of a forward transform."""
+    if nkern == 1:
+        return [total]
+    tr = total - epilogue
+    return [tr // 2, tr - tr // 2 + epilogue]
+
+
 def step_bytes(n, L, K, alpha, np_parts, kinds):
     """Algorithmic HBM bytes per operation of each plan step, in plan order:
     each transform reads and writes its tower once (16 n B), pointwise steps
     move their operands once, conversions read sources once and write
-    targets once, the key inner product reads every digit and key row once."""
+    targets once, the key inner product reads every digit and key row once.
+    Entries are (stage, bytes, bytes read by a fused transform epilogue)."""
     T = 8 * n
     sc = [L - min(alpha, L - alpha * p) + K for p in range(np_parts)]
     sp = [min(alpha, L - alpha * p) for p in range(np_parts)]
     if len(kinds) == 9:   # fused ModDown + rescale plan
         return [
-            ("tensor", 7 * T * L),
-            ("intt_modup", 2 * T * (L + 2)),             # + INTT of t0/t1's last tower
-            ("conv_modup", T * (sum(sp) + sum(sc))),
-            ("ntt_digits", 2 * T * sum(sc)),
-            ("key_inner_product", T * (L + K) * (3 * np_parts + 2)),
-            ("intt_moddown", 2 * T * (2 * K + 2)),       # + INTT of res0/res1's last tower
-            ("drop_tower", T * 2 * (K + 3)),
-            ("moddown_rescale_prologue", T * 2 * (K + 1 + (L - 1))),
-            ("ntt_rescale_epilogue", 4 * T * 2 * (L - 1)),   # transform + t, res operands
+            ("tensor", 7 * T * L, 0),
+            ("intt_modup", 2 * T * (L + 2), 0),             # + INTT of t0/t1's last tower
+            ("conv_modup", T * (sum(sp) + sum(sc)), 0),
+            ("ntt_digits", 2 * T * sum(sc), 0),
+            ("key_inner_product", T * (L + K) * (3 * np_parts + 2), 0),
+            ("intt_moddown", 2 * T * (2 * K + 2), 0),       # + INTT of res0/res1's last tower
+            ("drop_tower", T * 2 * (K + 3), 0),
+            ("moddown_rescale_prologue", T * 2 * (K + 1 + (L - 1)), 0),
+            # transform + the t and res operands its epilogue reads
+            ("ntt_rescale_epilogue", 4 * T * 2 * (L - 1), 2 * T * 2 * (L - 1)),
         ]
     table = {
         0: ("tensor", 7 * T * L),
@@ -84,24 +103,43 @@ def step_bytes(n, L, K, alpha, np_parts, kinds):
         12: ("rescale_fuse", 3 * T * 2 * (L - 1)),
     }
     assert len(kinds) == len(table), kinds
-    return [table[i] for i in range(len(kinds))]
+    return [table[i] + (0,) for i in range(len(kinds))]
 
 
-def cpu_baseline(q, p, roots, av, bv, seconds=12.0):
-    """Oracle EvalMult+Rescale, single thread, on the same workload shape."""
+def cpu_baseline(q, p, roots, av, bv, seconds=12.0, threads=1):
+    """Oracle EvalMult+Rescale on the same workload shape, one independent
+    operation per thread at a time (the oracle call releases the GIL), for a
+    bounded wall-clock sample.  Returns (ops/s, ops, seconds)."""
+    import threading
     sys.path.insert(0, ROOT)
     import oracle as O
     P = O.Params(N, q, p, roots, DNUM)
-    rng = np.random.default_rng(7)
-    cts = [np.concatenate([rng.integers(0, int(m), N, dtype=np.uint64) for m in q]) for _ in range(4)]
-    done, t0 = 0, time.perf_counter()
-    while True:
-        P.evalmult_rescale(L, *cts, av, bv)
-        done += 1
-        el = time.perf_counter() - t0
-        if el >= seconds or done >= 64:
-            break
-    return done / el, done, el
+    av, bv = O.U(av), O.U(bv)
+    counts = [0] * threads
+    t0 = time.perf_counter()
+
+    def worker(t):
+        rng = np.random.default_rng(7 + t)
+        cts = [np.concatenate([rng.integers(0, int(m), N, dtype=np.uint64) for m in q]) for _ in range(4)]
+        while time.perf_counter() - t0 < seconds:
+            P.evalmult_rescale(L, *cts, av, bv)
+            counts[t] += 1
+
+    ths = [threading.Thread(target=worker, args=(t,)) for t in range(threads)]
+    for th in ths:
+        th.start()
+    for th in ths:
+        th.join()
+    el = time.perf_counter() - t0
+    return sum(counts) / el, sum(counts), el
+
+
+def host_threads():
+    """CPU threads for the multi-core baseline: the job's thread allowance
+    (OMP_NUM_THREADS) when set, else the usable cores, at most 16."""
+    env = os.environ.get("OMP_NUM_THREADS")
+    n = int(env) if env and env.isdigit() else len(os.sched_getaffinity(0))
+    return max(1, min(16, n))
 
 
 def run_batched(G, ctx, key, B, steps, warmup, stream):
@@ -132,18 +170,44 @@ def run_batched(G, ctx, key, B, steps, warmup, stream):
 
 
 def time_steps(G, plan, stream, reps=5):
-    """Each plan step on its own, HIP events on the plan's stream."""
+    """Each plan step on its own, and each kernel of it on its own, timed with
+    HIP events on the stream the kernels are launched on.  Returns
+    (kind, jobs, step ms, [kernel ms, ...]) per step."""
     e0, e1 = G.Event(), G.Event()
-    out = []
-    for i, (kind, jobs) in enumerate(plan.steps()):
-        plan.run_step(i, stream)          # warm
+
+    def timed(fn):
+        fn()                               # warm
         e0.record(stream)
         for _ in range(reps):
-            plan.run_step(i, stream)
+            fn()
         e1.record(stream)
         stream.sync()
-        out.append((kind, jobs, e0.elapsed_ms(e1) / reps))
+        return e0.elapsed_ms(e1) / reps
+
+    out = []
+    for i, (kind, jobs) in enumerate(plan.steps()):
+        ms = timed(lambda: plan.run_step(i, stream))
+        kms = [timed(lambda w=w: plan.run_step_kernel(i, w, stream))
+               for w in range(1, plan.step_kernels(i) + 1)]
+        out.append((kind, jobs, ms, kms))
     return out
+
+
+def read_traffic(kernel, batch):
+    """HBM bytes per launch of `kernel` in the batch-`batch` plan, from the
+    committed rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE, each
+    scaled by its reading for a plain copy of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_evalmult_rescale_n65536_L24.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.strip().strip("|").split("|")
+        if len(f) >= 5 and f[0].strip() == kernel and f[1].strip() == str(batch):
+            try:
+                return int(float(f[4].strip()))
+            except ValueError:
+                return None
+    return None
 
 
 def run_per_op(G, ctx, key, nops, reps, stream):
@@ -181,6 +245,9 @@ def main():
     ap.add_argument("--perop-ops", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="finish with two device-to-device copies of this many MiB (the known-size "
+                         "reference for scaling FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs no collectives or multi-GPU work "
@@ -211,14 +278,21 @@ def main():
     stages = time_steps(G, res["plan"], stream)
     np_parts = -(-L // alpha)
     sb = step_bytes(N, L, K, alpha, np_parts, [s[0] for s in stages])
-    stage_report = []
-    for (kind, jobs, ms), (name, bytes_per_op) in zip(stages, sb):
+    stage_report, kernels = [], []
+    for (kind, jobs, ms, kms), (name, bytes_per_op, epi) in zip(stages, sb):
         gbs = bytes_per_op * B / (ms / 1e3) / 1e9
-        stage_report.append(dict(stage=name, kind=kind, jobs=jobs, ms=round(ms, 4), GBs=round(gbs, 1)))
-        log("  %-18s %-8s jobs=%6d %8.3f ms %8.1f GB/s" % (name, kind, jobs, ms, gbs))
-    dom = max(stage_report, key=lambda r: r["ms"])
-    dom_bytes = dict(sb)[dom["stage"]] * B
-    achieved = dom_bytes / (dom["ms"] / 1e3) / 1e9
+        kb = kernel_bytes(name, kind, len(kms), bytes_per_op * B, epi * B)
+        kinfo = []
+        for kname, kt, kbytes in zip(STEP_KERNELS[kind], kms, kb):
+            kernels.append(dict(stage=name, kernel=kname, ms=kt, bytes=kbytes))
+            kinfo.append(dict(kernel=kname, ms=round(kt, 4), GBs=round(kbytes / (kt / 1e3) / 1e9, 1)))
+        stage_report.append(dict(stage=name, kind=kind, jobs=jobs, ms=round(ms, 4), GBs=round(gbs, 1),
+                                 kernels=kinfo))
+        log("  %-24s %-8s jobs=%6d %8.3f ms %8.1f GB/s  [%s]" % (
+            name, kind, jobs, ms, gbs, ", ".join("%s %.3f ms" % (k["kernel"], k["ms"]) for k in kinfo)))
+    dom = max(kernels, key=lambda r: r["ms"])
+    achieved = dom["bytes"] / (dom["ms"] / 1e3) / 1e9
+    traffic = read_traffic(dom["stage"] + ":" + dom["kernel"], B)
 
     perop_ops, perop_ms = run_per_op(G, ctx, key, args.perop_ops, 3, stream)
     log("per-op: %.1f ops/s (%.3f ms/op)" % (perop_ops, perop_ms))
@@ -235,10 +309,14 @@ def main():
     cpu = None
     if not args.no_cpu:
         try:
-            cps, cnt, el = cpu_baseline(q, p, roots, av, bv, args.cpu_seconds)
-            cpu = dict(value=round(cps, 4), unit="ops/s", cores=1, kind="port",
-                       sample="%d EvalMult+Rescale ops at n=2^16 L=24 dnum=4 (%.1f s, 1 thread)" % (cnt, el))
-            log("cpu oracle: %.4f ops/s" % cps)
+            nt = host_threads()
+            c1, n1, e1 = cpu_baseline(q, p, roots, av, bv, args.cpu_seconds / 2, 1)
+            cps, cnt, el = cpu_baseline(q, p, roots, av, bv, args.cpu_seconds, nt)
+            cpu = dict(value=round(cps, 4), unit="ops/s", cores=nt, kind="port",
+                       sample="%d EvalMult+Rescale ops at n=2^16 L=24 dnum=4 on %d threads (%.1f s); "
+                              "1 thread: %d ops in %.1f s" % (cnt, nt, el, n1, e1),
+                       single_thread_value=round(c1, 4))
+            log("cpu oracle: %.4f ops/s on %d threads, %.4f ops/s on 1" % (cps, nt, c1))
         except Exception as e:  # the baseline is reported, never required
             log("cpu baseline skipped: %s" % e)
 
@@ -263,12 +341,20 @@ def main():
         "batched_over_per_op": round(value / perop_ops, 3),
         "batch_sweep_ops_s": sweep,
         "stages": stage_report,
-        "roofline": {"bound": "hbm", "kernel": dom["stage"], "achieved": round(achieved, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None},
+        "roofline": {"bound": "hbm", "kernel": dom["stage"] + ":" + dom["kernel"],
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "algorithmic_bytes": dom["bytes"], "launch_ms": round(dom["ms"], 4)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+    if args.calib_copy_mib > 0:
+        words = args.calib_copy_mib << 17
+        a, b = G.Dev(words), G.Dev(words)
+        for _ in range(2):
+            G.check(G.lib.gf_d2d(b.ptr, a.ptr, words * 8))
+        G.device_sync()
+        log("calibration copies: 2 x %d MiB device to device" % args.calib_copy_mib)
 
 
 if __name__ == "__main__":

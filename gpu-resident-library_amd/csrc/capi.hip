@@ -255,6 +255,17 @@ int gf_plan_run_step(gf_plan* p, uint32_t i, hipStream_t s) {
         p->P->run_step(i, s);
     });
 }
+uint32_t gf_plan_step_kernels(const gf_plan* p, uint32_t i) {
+    return i < p->P->steps.size() ? p->P->step_kernels(i) : 0u;
+}
+int gf_plan_run_step_kernel(gf_plan* p, uint32_t i, uint32_t which, hipStream_t s) {
+    return guard([&] {
+        if (i >= p->P->steps.size()) throw std::runtime_error("gpufhe: step index out of range");
+        if (which < 1 || which > p->P->step_kernels(i))
+            throw std::runtime_error("gpufhe: kernel index out of range for this step");
+        p->P->run_step(i, s, which);
+    });
+}
 int gf_ctx_set_ntt_radix(gf_ctx* c, uint32_t r) {
     return guard([&] {
         if (r < 2 || r > 4) throw std::runtime_error("gpufhe: NTT radix log2 must be 2, 3 or 4");

@@ -307,14 +307,20 @@ void Plan::run(hipStream_t s) const {
     for (size_t i = 0; i < steps.size(); ++i) run_step(i, s);
 }
 
-void Plan::run_step(size_t i, hipStream_t s) const {
+uint32_t Plan::step_kernels(size_t i) const {
+    const Step::Kind k = steps[i].kind;
+    return (k == Step::NTT_F || k == Step::NTT_I) && !geom.single ? 2u : 1u;
+}
+
+void Plan::run_step(size_t i, hipStream_t s, uint32_t which) const {
     const char* base = (const char*)dev.p;
     {
         const Step& st = steps[i];
         const void* p = base + st.off;
+        if (st.kind != Step::NTT_F && st.kind != Step::NTT_I && !(which & 1)) return;
         switch (st.kind) {
-            case Step::NTT_F: launch_ntt((const NttJob*)p, st.count, geom, true, s); break;
-            case Step::NTT_I: launch_ntt((const NttJob*)p, st.count, geom, false, s); break;
+            case Step::NTT_F: launch_ntt((const NttJob*)p, st.count, geom, true, s, which); break;
+            case Step::NTT_I: launch_ntt((const NttJob*)p, st.count, geom, false, s, which); break;
             case Step::TENSOR: launch_tensor((const TensorJob*)p, st.count, n, s); break;
             case Step::CONV: launch_conv((const ConvJob*)p, st.count, n, s); break;
             case Step::INNER: launch_inner((const InnerJob*)p, st.count, n, s); break;

@@ -118,7 +118,9 @@ public:
     uint32_t n = 0, logn = 0;
     NttGeom geom{};
     void run(hipStream_t s) const;
-    void run_step(size_t i, hipStream_t s) const;
+    // which: 3 = every kernel of step i; 1 / 2 = only its first / second kernel
+    void run_step(size_t i, hipStream_t s, uint32_t which = 3) const;
+    uint32_t step_kernels(size_t i) const;   // 2 for a two-phase transform, else 1
     // hipGraph of the whole sequence (captured on first use)
     hipGraphExec_t graph = nullptr;
     void run_graph(hipStream_t s);

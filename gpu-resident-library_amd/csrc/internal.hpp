@@ -133,7 +133,10 @@ inline void hip_check(hipError_t e, const char* what) {
 }
 
 // batched launches (jobs in device memory)
-void launch_ntt(const NttJob* d_jobs, uint32_t njobs, const NttGeom& g, bool fwd, hipStream_t s);
+// which: 3 = the whole transform; 1 / 2 = only its first / second kernel
+// (two-phase rings; a single-phase ring has one kernel, run when bit 0 is set)
+void launch_ntt(const NttJob* d_jobs, uint32_t njobs, const NttGeom& g, bool fwd, hipStream_t s,
+                uint32_t which = 3);
 void launch_ntt_one(const NttJob& job, const NttGeom& g, bool fwd, hipStream_t s);
 void launch_tensor(const TensorJob* d, uint32_t nj, uint32_t n, hipStream_t s);
 void launch_conv(const ConvJob* d, uint32_t nj, uint32_t n, hipStream_t s);

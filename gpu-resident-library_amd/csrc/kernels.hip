@@ -48,7 +48,7 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_S(J jobs) {
 // reference ABI launchers take the two arrays separately.  g.rmax selects the
 // register radix (the tuning knob the autotuner sets).
 template <bool FWD, bool PAIRED, class J>
-static void ntt_dispatch_dir(J jobs, uint32_t njobs, const NttGeom& g, hipStream_t s) {
+static void ntt_dispatch_dir(J jobs, uint32_t njobs, const NttGeom& g, hipStream_t s, uint32_t which) {
     const dim3 thr(NTT_THREADS);
     bool ok = with_radix(g.rmax, [&](auto Rc) {
         constexpr uint32_t RM = decltype(Rc)::value;
@@ -56,19 +56,21 @@ static void ntt_dispatch_dir(J jobs, uint32_t njobs, const NttGeom& g, hipStream
         if (g.single) {
             geom_ok = with_single(g.logn, [&](auto Sc) {
                 constexpr uint32_t S = decltype(Sc)::value;
-                k_ntt_S<FWD, PAIRED, S, RM><<<dim3(1, njobs), thr, (size_t)padded(1u << S) * 8, s>>>(jobs);
+                if (which & 1)
+                    k_ntt_S<FWD, PAIRED, S, RM><<<dim3(1, njobs), thr, (size_t)padded(1u << S) * 8, s>>>(jobs);
             });
         } else {
             geom_ok = with_split(g.logN1, g.logN2, [&](auto A, auto B) {
                 constexpr uint32_t S1 = decltype(A)::value, S2 = decltype(B)::value;
                 const dim3 gA((1u << S2) / g.C, njobs), gB((1u << S1) / g.R, njobs);
                 const size_t shA = (size_t)NTT_TILE * 8, shB = (size_t)NTT_TILE_PADDED * 8;
+                // which: bit 0 = first kernel of the transform, bit 1 = second
                 if (FWD) {
-                    k_ntt_A<FWD, PAIRED, S1, RM><<<gA, thr, shA, s>>>(jobs, g);
-                    k_ntt_B<FWD, PAIRED, S2, RM><<<gB, thr, shB, s>>>(jobs, g);
+                    if (which & 1) k_ntt_A<FWD, PAIRED, S1, RM><<<gA, thr, shA, s>>>(jobs, g);
+                    if (which & 2) k_ntt_B<FWD, PAIRED, S2, RM><<<gB, thr, shB, s>>>(jobs, g);
                 } else {
-                    k_ntt_B<FWD, PAIRED, S2, RM><<<gB, thr, shB, s>>>(jobs, g);
-                    k_ntt_A<FWD, PAIRED, S1, RM><<<gA, thr, shA, s>>>(jobs, g);
+                    if (which & 1) k_ntt_B<FWD, PAIRED, S2, RM><<<gB, thr, shB, s>>>(jobs, g);
+                    if (which & 2) k_ntt_A<FWD, PAIRED, S1, RM><<<gA, thr, shA, s>>>(jobs, g);
                 }
             });
         }
@@ -78,17 +80,19 @@ static void ntt_dispatch_dir(J jobs, uint32_t njobs, const NttGeom& g, hipStream
 }
 
 template <bool PAIRED, class J>
-static void ntt_dispatch(J jobs, uint32_t njobs, const NttGeom& g, bool fwd, hipStream_t s) {
-    if (fwd) ntt_dispatch_dir<true, PAIRED>(jobs, njobs, g, s);
-    else ntt_dispatch_dir<false, PAIRED>(jobs, njobs, g, s);
+static void ntt_dispatch(J jobs, uint32_t njobs, const NttGeom& g, bool fwd, hipStream_t s,
+                         uint32_t which = 3) {
+    if (fwd) ntt_dispatch_dir<true, PAIRED>(jobs, njobs, g, s, which);
+    else ntt_dispatch_dir<false, PAIRED>(jobs, njobs, g, s, which);
 }
 
 static const uint32_t MAXY = 65535;
 
-void launch_ntt(const NttJob* d_jobs, uint32_t njobs, const NttGeom& g, bool fwd, hipStream_t s) {
+void launch_ntt(const NttJob* d_jobs, uint32_t njobs, const NttGeom& g, bool fwd, hipStream_t s,
+                uint32_t which) {
     for (uint32_t off = 0; off < njobs; off += MAXY) {
         uint32_t cnt = njobs - off < MAXY ? njobs - off : MAXY;
-        ntt_dispatch<true>(JobsDev{d_jobs, off}, cnt, g, fwd, s);
+        ntt_dispatch<true>(JobsDev{d_jobs, off}, cnt, g, fwd, s, which);
     }
 }
 void launch_ntt_one(const NttJob& job, const NttGeom& g, bool fwd, hipStream_t s) {

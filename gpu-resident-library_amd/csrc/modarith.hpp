@@ -72,6 +72,18 @@ GF_HD uint64_t shoup_lazy(uint64_t x, uint64_t w, uint64_t wp, uint64_t q) {
 // x in [0, 2m) -> [0, m)
 GF_HD uint64_t csub(uint64_t x, uint64_t m) { return x >= m ? x - m : x; }
 
+GF_HD uint32_t umulhi32(uint32_t a, uint32_t b) { return (uint32_t)(((uint64_t)a * b) >> 32); }
+// floor(a b / 2^64) minus 0, 1 or 2: the a0*b0 product and the low halves of
+// the cross products are dropped (their sum is < 3 * 2^64 / 2^64)
+GF_HD uint64_t mulhi64_approx(uint64_t a, uint64_t b) {
+    const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32), b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
+    return mad32(a1, b1, (uint64_t)umulhi32(a0, b1) + umulhi32(a1, b0));
+}
+// Shoup with the approximate quotient: x*w mod q + {0, q, 2q, 3q}, in [0, 4q), any x < 2^64
+GF_HD uint64_t shoup_lazy4(uint64_t x, uint64_t w, uint64_t wp, uint64_t q) {
+    return x * w - mulhi64_approx(x, wp) * q;
+}
+
 // Montgomery reduction, R = 2^64, qneg = -q^{-1} mod 2^64.
 // For T < q * 2^64 returns T * R^{-1} mod q, in [0, q).
 GF_HD uint64_t redc(U128 T, uint64_t q, uint64_t qneg) {

@@ -61,6 +61,8 @@ _capi = {
     "gf_plan_num_steps": (u32, [vp]),
     "gf_plan_step_info": (C.c_int, [vp, u32, u32p, u32p]),
     "gf_plan_run_step": (C.c_int, [vp, u32, vp]),
+    "gf_plan_step_kernels": (u32, [vp, u32]),
+    "gf_plan_run_step_kernel": (C.c_int, [vp, u32, u32, vp]),
     "gf_fill_uniform": (C.c_int, [vp, vp, u32, u64, vp]),
     "gf_ctx_set_ntt_radix": (C.c_int, [vp, u32]),
     "gf_ctx_get_ntt_radix": (u32, [vp]),
@@ -239,6 +241,12 @@ class Plan:
 
     def run_step(self, i, stream):
         check(lib.gf_plan_run_step(self.h, i, stream.s))
+
+    def step_kernels(self, i):
+        return int(lib.gf_plan_step_kernels(self.h, i))
+
+    def run_step_kernel(self, i, which, stream):
+        check(lib.gf_plan_run_step_kernel(self.h, i, which, stream.s))
 
     def __del__(self):
         if getattr(self, "h", None):

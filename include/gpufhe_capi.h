@@ -120,6 +120,10 @@ void gf_plan_destroy(gf_plan* p);
 uint32_t gf_plan_num_steps(const gf_plan* p);
 int gf_plan_step_info(const gf_plan* p, uint32_t i, uint32_t* kind, uint32_t* jobs);
 int gf_plan_run_step(gf_plan* p, uint32_t i, hipStream_t s);
+/* Kernels in step i (2 for a transform of a ring split into two passes, else
+ * 1) and one of them alone (which = 1 or 2): for per-kernel timing. */
+uint32_t gf_plan_step_kernels(const gf_plan* p, uint32_t i);
+int gf_plan_run_step_kernel(gf_plan* p, uint32_t i, uint32_t which, hipStream_t s);
 
 /* ---- tuning ----
  * NTT register radix log2 (2, 3 or 4) used by plans built afterwards; the

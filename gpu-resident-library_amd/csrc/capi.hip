@@ -141,6 +141,75 @@ gf_key* gf_key_create(gf_ctx* c, const uint64_t* av, const uint64_t* bv, uint32_
 }
 void gf_key_destroy(gf_key* k) { delete k; }
 
+gf_ctx* gf_ctx_create_alpha(uint32_t n, const uint64_t* q, uint32_t L, const uint64_t* p, uint32_t K,
+                            const uint64_t* roots, uint32_t alpha) {
+    return guard_ptr<gf_ctx>([&] {
+        if (alpha == 0 || alpha > L) throw std::runtime_error("gpufhe: part width alpha must be in [1, L]");
+        auto c = new gf_ctx;
+        c->C = std::make_unique<Ctx>(n, std::vector<uint64_t>(q, q + L), std::vector<uint64_t>(p, p + K),
+                                     std::vector<uint64_t>(roots, roots + L + K), alpha);
+        return c;
+    });
+}
+gf_key* gf_key_create_rows(gf_ctx* c, const uint64_t* av, const uint64_t* bv, uint32_t nparts, uint32_t rows,
+                           uint32_t fullQ, hipStream_t s) {
+    return guard_ptr<gf_key>([&] {
+        auto k = new gf_key;
+        k->ctx = c;
+        k->K = upload_key(*c->C, av, bv, nparts, rows, fullQ, s);
+        return k;
+    });
+}
+uint32_t gf_ctx_towers(const gf_ctx* c) { return c->C->L; }
+
+int gf_root_tables(uint32_t n, uint64_t q, uint64_t root, const uint64_t** fr, const uint64_t** fp,
+                   const uint64_t** ir, const uint64_t** ip, uint64_t* ninv, uint64_t* ninv_precon) {
+    return guard([&] {
+        if (n < 2 || (n & (n - 1)) || n > (1u << 17)) throw std::runtime_error("gpufhe: ring dimension must be a power of two in [2, 2^17]");
+        const RootTable t = root_table(n, q, root);
+        *fr = t.fr;
+        *fp = t.fp;
+        *ir = t.ir;
+        *ip = t.ip;
+        *ninv = t.ninv;
+        *ninv_precon = t.ninvp;
+    });
+}
+int gf_ntt_towers(uint32_t n, uint64_t* d_x, uint32_t towers, const uint64_t* mods, const uint64_t* roots,
+                  int inverse, hipStream_t s) {
+    return guard([&] {
+        if (n < 2 || (n & (n - 1)) || n > (1u << 17)) throw std::runtime_error("gpufhe: ring dimension must be a power of two in [2, 2^17]");
+        const NttGeom g = make_geom(n);
+        for (uint32_t t = 0; t < towers; ++t) {
+            const RootTable r = root_table(n, mods[t], roots[t]);
+            uint64_t* x = d_x + (size_t)t * n;
+            NttJob j = inverse ? NttJob{x, x, r.ir, r.ip, mods[t], r.ninv, r.ninvp}
+                               : NttJob{x, x, r.fr, r.fp, mods[t], 0, 0};
+            launch_ntt_one(j, g, !inverse, s);
+        }
+        hip_check(hipGetLastError(), "NTT launch");
+    });
+}
+int gf_automorph(uint32_t n, const uint64_t* d_in, uint64_t* d_out, uint32_t towers, uint64_t k, hipStream_t s) {
+    return guard([&] {
+        if (n < 2 || (n & (n - 1))) throw std::runtime_error("gpufhe: ring dimension must be a power of two");
+        if (!(k & 1) || k >= 2ull * n) throw std::runtime_error("gpufhe: automorphism index must be odd and < 2n");
+        if (d_in == d_out) throw std::runtime_error("gpufhe: the automorphism cannot run in place");
+        uint32_t logn = 0;
+        while ((1u << logn) < n) ++logn;
+        for (uint32_t t = 0; t < towers; ++t)
+            launch_perm_one(PermJob{d_in + (size_t)t * n, d_out + (size_t)t * n, k}, n, logn, s);
+        hip_check(hipGetLastError(), "automorphism launch");
+    });
+}
+int gf_rescale_comp(gf_ctx* c, uint32_t l, const uint64_t* d_c, uint64_t* d_o, uint64_t* ws, hipStream_t s) {
+    return guard([&] {
+        Plan* P = cached_plan(c, {7, l, U(d_c), U(d_o), U(ws)}, s,
+                              [&](PlanBuilder& b) { b.rescale_one(l, d_c, d_o, ws); });
+        P->run(s);
+    });
+}
+
 int gf_evalmult_rescale(gf_ctx* c, gf_key* k, uint32_t l, const uint64_t* c0a, const uint64_t* c1a,
                         const uint64_t* c0b, const uint64_t* c1b, uint64_t* o0, uint64_t* o1, uint64_t* ws,
                         hipStream_t s) {

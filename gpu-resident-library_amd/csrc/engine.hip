@@ -547,6 +547,13 @@ void PlanBuilder::rescale(uint32_t l, const uint64_t* c0, const uint64_t* c1, ui
     rescale_comp(l, c1, o1, drop + n, sw + (size_t)(l - 1) * n);
 }
 
+void PlanBuilder::rescale_one(uint32_t l, const uint64_t* c, uint64_t* o, uint64_t* ws) {
+    const Ctx& C = C_;
+    if (l < 2 || l > C.L) throw std::runtime_error("gpufhe: rescale needs at least two towers");
+    const WsLayout w = WsLayout::make(C);
+    rescale_comp(l, c, o, ws + w.drop * C.n, ws + w.dig * C.n);
+}
+
 void PlanBuilder::evalmult_rescale(uint32_t l, const uint64_t* c0a, const uint64_t* c1a,
                                    const uint64_t* c0b, const uint64_t* c1b, uint64_t* o0, uint64_t* o1,
                                    const DevKey& key, uint64_t* ws) {

@@ -138,6 +138,8 @@ public:
                   const uint64_t* c1b, uint64_t* r0, uint64_t* r1, const DevKey& key, uint64_t* ws);
     void rescale(uint32_t l, const uint64_t* c0, const uint64_t* c1, uint64_t* o0, uint64_t* o1,
                  uint64_t* ws);
+    // one component (o may equal c: every surviving tower is read before it is written)
+    void rescale_one(uint32_t l, const uint64_t* c, uint64_t* o, uint64_t* ws);
     void evalmult_rescale(uint32_t l, const uint64_t* c0a, const uint64_t* c1a, const uint64_t* c0b,
                           const uint64_t* c1b, uint64_t* o0, uint64_t* o1, const DevKey& key,
                           uint64_t* ws);

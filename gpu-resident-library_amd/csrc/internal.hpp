@@ -146,6 +146,7 @@ void launch_switch(const SwitchJob* d, uint32_t nj, uint32_t n, hipStream_t s);
 void launch_fuse(const FuseJob* d, uint32_t nj, uint32_t n, hipStream_t s);
 void launch_add(const AddJob* d, uint32_t nj, uint32_t n, hipStream_t s);
 void launch_perm(const PermJob* d, uint32_t nj, uint32_t n, uint32_t logn, hipStream_t s);
+void launch_perm_one(const PermJob& j, uint32_t n, uint32_t logn, hipStream_t s);
 void launch_copy(const CopyJob* d, uint32_t nj, uint32_t n, hipStream_t s);
 void launch_fill(uint64_t* d, const uint64_t* d_mods, uint32_t towers, uint32_t n, uint64_t seed,
                  hipStream_t s);

@@ -229,9 +229,18 @@ __device__ __forceinline__ uint32_t brv_bits(uint32_t v, uint32_t logn) {
     return logn ? (__brev(v) >> (32 - logn)) : 0;
 }
 
-__global__ __launch_bounds__(PW_THREADS) void k_perm(const PermJob* __restrict__ jobs,
-                                                     uint32_t off, uint32_t n, uint32_t logn) {
-    const PermJob j = jobs[off + blockIdx.y];
+struct PermList {
+    const PermJob* p;
+    uint32_t off;
+    __device__ PermJob get(uint32_t i) const { return p[off + i]; }
+};
+struct PermOne {
+    PermJob j;
+    __device__ PermJob get(uint32_t) const { return j; }
+};
+template <class J>
+__global__ __launch_bounds__(PW_THREADS) void k_perm(J jobs, uint32_t n, uint32_t logn) {
+    const PermJob j = jobs.get(blockIdx.y);
     const uint64_t M = 2ull * n;
     PW_FOR(n) {
         const uint64_t e = ((2ull * brv_bits(i, logn) + 1) * j.k) & (M - 1);
@@ -289,8 +298,11 @@ void launch_fill(uint64_t* d, const uint64_t* d_mods, uint32_t towers, uint32_t 
 void launch_perm(const PermJob* d, uint32_t nj, uint32_t n, uint32_t logn, hipStream_t s) {
     for (uint32_t off = 0; off < nj; off += MAXY) {
         uint32_t cnt = nj - off < MAXY ? nj - off : MAXY;
-        k_perm<<<pw_grid(n, cnt), PW_THREADS, 0, s>>>(d, off, n, logn);
+        k_perm<<<pw_grid(n, cnt), PW_THREADS, 0, s>>>(PermList{d, off}, n, logn);
     }
+}
+void launch_perm_one(const PermJob& j, uint32_t n, uint32_t logn, hipStream_t s) {
+    k_perm<<<pw_grid(n, 1), PW_THREADS, 0, s>>>(PermOne{j}, n, logn);
 }
 
 // ------------------------------------------- reference single-tower ABI ---

@@ -45,6 +45,13 @@ _capi = {
     "gf_ws_bytes": (sz, [vp]),
     "gf_key_create": (vp, [vp, u64p, u64p, u32, vp]),
     "gf_key_destroy": (None, [vp]),
+    "gf_ctx_create_alpha": (vp, [u32, u64p, u32, u64p, u32, u64p, u32]),
+    "gf_ctx_towers": (u32, [vp]),
+    "gf_key_create_rows": (vp, [vp, u64p, u64p, u32, u32, u32, vp]),
+    "gf_root_tables": (C.c_int, [u32, u64, u64, vpp, vpp, vpp, vpp, u64p, u64p]),
+    "gf_ntt_towers": (C.c_int, [u32, vp, u32, u64p, u64p, C.c_int, vp]),
+    "gf_automorph": (C.c_int, [u32, vp, vp, u32, u64, vp]),
+    "gf_rescale_comp": (C.c_int, [vp, u32, vp, vp, vp, vp]),
     "gf_evalmult_rescale": (C.c_int, [vp, vp, u32] + [vp] * 7 + [vp]),
     "gf_evalmult": (C.c_int, [vp, vp, u32] + [vp] * 7 + [vp]),
     "gf_keyswitch": (C.c_int, [vp, vp, u32, vp, vp, vp, vp, vp]),

@@ -16,7 +16,7 @@
  */
 #ifndef GPUFHE_CAPI_H
 #define GPUFHE_CAPI_H
-#include <hip/hip_runtime.h>
+#include <hip/hip_runtime_api.h>
 #include <stddef.h>
 #include <stdint.h>
 
@@ -68,6 +68,34 @@ gf_key* gf_key_create(gf_ctx* c, const uint64_t* h_av, const uint64_t* h_bv, uin
                       hipStream_t s);
 void gf_key_destroy(gf_key* k);
 
+/* Context with an explicit part width alpha (KeySwitchConstants::alpha of
+ * include/keyswitch.h:17-19; ks_context_create builds one per level). */
+gf_ctx* gf_ctx_create_alpha(uint32_t n, const uint64_t* h_q, uint32_t L, const uint64_t* h_p,
+                            uint32_t K, const uint64_t* h_roots, uint32_t alpha);
+uint32_t gf_ctx_towers(const gf_ctx* c);
+/* Key rows in the reference's eval-key layout: h_av/h_bv [nparts][rows][n]
+ * with rows = fullQ + K; Q tower i reads row i, P tower k reads row fullQ+k
+ * (src/keyswitch_resident.cpp:84-93, KeySwitchConstants::evalKeyTowers). */
+gf_key* gf_key_create_rows(gf_ctx* c, const uint64_t* h_av, const uint64_t* h_bv, uint32_t nparts,
+                           uint32_t rows, uint32_t fullQ, hipStream_t s);
+
+/* ---- tower utilities ----
+ * Device root tables of (n, q, root) in the reference layout
+ * (src/keyswitch.cpp:22-35): fr = psi^brv, ir = psi^-brv, Shoup precons.
+ * Shared process-wide and never freed (src/keyswitch_resident.cpp:47-68). */
+int gf_root_tables(uint32_t n, uint64_t q, uint64_t root, const uint64_t** fr, const uint64_t** fp,
+                   const uint64_t** ir, const uint64_t** ip, uint64_t* ninv, uint64_t* ninv_precon);
+/* In-place forward (inverse = 0) or inverse NTT of `towers` consecutive
+ * towers, tower t under h_mods[t] / h_roots[t] (one launch per tower; no
+ * allocation, so it may be captured). */
+int gf_ntt_towers(uint32_t n, uint64_t* d_x, uint32_t towers, const uint64_t* h_mods,
+                  const uint64_t* h_roots, int inverse, hipStream_t s);
+/* Galois automorphism sigma_k on evaluation-form towers, out of place
+ * (the INTT -> LaunchAutomorphPermute -> NTT of src/rotate.cpp:70-84 as
+ * one permutation of the bit-reversed evaluation points). */
+int gf_automorph(uint32_t n, const uint64_t* d_in, uint64_t* d_out, uint32_t towers, uint64_t k,
+                 hipStream_t s);
+
 /* ---- single operations (level = number of Q towers of the inputs) ---- */
 /* tensor -> keyswitch -> combine -> rescale; outputs have level-1 towers */
 int gf_evalmult_rescale(gf_ctx* c, gf_key* rlk, uint32_t level, const uint64_t* c0a,
@@ -83,6 +111,10 @@ int gf_keyswitch(gf_ctx* c, gf_key* k, uint32_t level, const uint64_t* d, uint64
 /* drop the last tower of both components: outputs have level-1 towers */
 int gf_rescale(gf_ctx* c, uint32_t level, const uint64_t* c0, const uint64_t* c1, uint64_t* o0,
                uint64_t* o1, uint64_t* ws, hipStream_t s);
+/* one component: d_o (level-1 towers) may equal d_c (in place, as
+ * rescale_resident_raw, src/keyswitch_resident.cpp:227-244) */
+int gf_rescale_comp(gf_ctx* c, uint32_t level, const uint64_t* d_c, uint64_t* d_o, uint64_t* ws,
+                    hipStream_t s);
 /* (sigma_k(c0) + ba0, ba1), (ba0, ba1) = KS(sigma_k(c1)); k odd, < 2n */
 int gf_rotate(gf_ctx* c, gf_key* rk, uint32_t level, uint64_t k, const uint64_t* c0,
               const uint64_t* c1, uint64_t* o0, uint64_t* o1, uint64_t* ws, hipStream_t s);

@@ -10,7 +10,7 @@
  */
 #ifndef GPUFHE_KERNELS_H
 #define GPUFHE_KERNELS_H
-#include <hip/hip_runtime.h>
+#include <hip/hip_runtime_api.h>
 #include <stdint.h>
 
 #ifdef __cplusplus

@@ -210,8 +210,10 @@ def read_traffic(kernel, batch):
     return None
 
 
-def run_per_op(G, ctx, key, nops, reps, stream):
-    """Same kernels, one operation per launch sequence, no graph."""
+def run_per_op(G, ctx, key, nops, reps, stream, per_job=False):
+    """Same kernels, one operation per launch sequence, no graph.  per_job:
+    additionally one launch per job (tower of a stage), the launch structure
+    of the reference's resident path."""
     T = ctx.n
     ins = [[G.Dev(L * T) for _ in range(4)] for _ in range(nops)]
     outs = [[G.Dev((L - 1) * T) for _ in range(2)] for _ in range(nops)]
@@ -221,18 +223,92 @@ def run_per_op(G, ctx, key, nops, reps, stream):
             G.check(G.lib.gf_fill_uniform(ctx.h, ins[b][j].ptr, L, 7777 + 10 * b + j, stream.s))
     plans = [G.plan_evalmult_rescale(ctx, key, [L], *[[ins[b][j].ptr] for j in range(4)], [outs[b][0].ptr],
                                      [outs[b][1].ptr], [ws[b].ptr]) for b in range(nops)]
+    go = (lambda p: p.run_per_job(stream)) if per_job else (lambda p: p.run(stream))
     for p in plans:
-        p.run(stream)
+        go(p)
     stream.sync()
     e0, e1 = G.Event(), G.Event()
     e0.record(stream)
     for _ in range(reps):
         for p in plans:
-            p.run(stream)
+            go(p)
     e1.record(stream)
     stream.sync()
     ms = e0.elapsed_ms(e1)
     return nops * reps / (ms / 1e3), ms / (nops * reps)
+
+
+SERVE_RINGS = ((1 << 14, 0.3), (1 << 15, 0.4), (1 << 16, 0.3))
+SERVE_L, SERVE_DNUM, SERVE_KEYS = 20, 4, 16
+SERVE_MIX = (0.6, 0.3, 0.1)      # EvalMult+Rescale, Rotate, EvalAdd
+
+
+def run_serving(G, stream, seconds, rate, seed=5, budget=32 << 30, lanes=4, max_batch=64):
+    """Poisson trace of mixed requests replayed in real time through the
+    continuous-batching server.  Ring sizes 2^14/2^15/2^16 (weights 0.3/0.4/0.3),
+    levels uniform in 3..20, 60% EvalMult+Rescale, 30% Rotate (one of 16
+    resident rotation keys), 10% EvalAdd.  Ciphertexts and keys are resident;
+    latency = device completion - scheduled arrival."""
+    rng = np.random.default_rng(seed)
+    srv = G.Server(ws_budget_bytes=budget, lanes=lanes, max_batch=max_batch)
+    rings = []
+    for ri, (n, _) in enumerate(SERVE_RINGS):
+        q, p, roots, K, alpha = native_params(G, n, SERVE_L, SERVE_DNUM)
+        ctx = G.Context(n, q, p, roots, SERVE_DNUM)
+        ctx.autotune(stream)
+        rlk = G.Key.uniform(ctx, SERVE_DNUM, 1000 * ri + 1, stream)
+        rot_k = [pow(5, 1 << j, 2 * n) for j in range(SERVE_KEYS)]
+        rks = [G.Key.uniform(ctx, SERVE_DNUM, 1000 * ri + 10 + j, stream) for j in range(SERVE_KEYS)]
+        rid = srv.add_ring(ctx, rlk, rks, rot_k)
+        ins = {}
+        for lv in range(3, SERVE_L + 1):
+            bufs = [G.Dev(lv * n) for _ in range(4)]
+            for j, d in enumerate(bufs):
+                G.check(G.lib.gf_fill_uniform(ctx.h, d.ptr, lv, 7919 * lv + 31 * j + ri, stream.s))
+            ins[lv] = bufs
+        rings.append(dict(id=rid, ctx=ctx, rlk=rlk, rks=rks, ins=ins))
+    stream.sync()
+
+    def trace(count, gap_ms):
+        ring = rng.choice(len(SERVE_RINGS), count, p=[w for _, w in SERVE_RINGS])
+        u = rng.random(count)
+        op = np.where(u < SERVE_MIX[0], G.OP_EVALMULT_RESCALE,
+                      np.where(u < SERVE_MIX[0] + SERVE_MIX[1], G.OP_ROTATE, G.OP_ADD))
+        lv = rng.integers(3, SERVE_L + 1, count)
+        rot = rng.integers(0, SERVE_KEYS, count)
+        t = np.cumsum(rng.exponential(gap_ms, count))
+        reqs = []
+        for i in range(count):
+            b = rings[ring[i]]["ins"][int(lv[i])]
+            reqs.append(G.Request(rings[ring[i]]["id"], int(op[i]), int(lv[i]), int(rot[i]),
+                                  b[0].ptr, b[1].ptr, b[2].ptr, b[3].ptr, None, None, float(t[i])))
+        return reqs
+
+    srv.run_trace(trace(max(200, int(rate * 0.5)), 1000.0 / rate))     # grows the slot pool, warms plans
+    st0 = srv.stats()
+    count = int(rate * seconds)
+    t0 = time.perf_counter()
+    comps = srv.run_trace(trace(count, 1000.0 / rate))
+    wall = time.perf_counter() - t0
+    st1 = srv.stats()
+    lat = np.array([c.done_ms - c.arrival_ms for c in comps])
+    arr = np.array([c.arrival_ms for c in comps])
+    done = np.array([c.done_ms for c in comps])
+    ops = np.array([c.op for c in comps])
+    names = {G.OP_EVALMULT_RESCALE: "evalmult_rescale", G.OP_ROTATE: "rotate", G.OP_ADD: "add"}
+    by_op = {names[o]: dict(count=int((ops == o).sum()),
+                            p50_ms=round(float(np.percentile(lat[ops == o], 50)), 3),
+                            p99_ms=round(float(np.percentile(lat[ops == o], 99)), 3))
+             for o in names if (ops == o).any()}
+    batches = st1["batches"] - st0["batches"]
+    return dict(requests=count, offered_rate_req_s=rate, seconds=seconds,
+                ops_s=round(count / ((done.max() - arr.min()) / 1e3), 1),
+                p50_ms=round(float(np.percentile(lat, 50)), 3), p99_ms=round(float(np.percentile(lat, 99)), 3),
+                mean_ms=round(float(lat.mean()), 3), max_ms=round(float(lat.max()), 3),
+                batches=int(batches), mean_batch=round(count / max(1, batches), 2),
+                slot_pool_gib=round(st1["slot_bytes"] / 2**30, 2), wall_s=round(wall, 2), by_op=by_op,
+                rings=[n for n, _ in SERVE_RINGS], levels="3..20", dnum=SERVE_DNUM, lanes=lanes,
+                max_batch=max_batch)
 
 
 def main():
@@ -245,6 +321,11 @@ def main():
     ap.add_argument("--perop-ops", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--ntt-radix", type=int, default=0, choices=(0, 2, 3, 4),
+                    help="pin the NTT register radix (log2) instead of autotuning")
+    ap.add_argument("--serve-seconds", type=float, default=10.0,
+                    help="length of the serving trace (0 skips it)")
+    ap.add_argument("--serve-rate", type=float, default=2000.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="finish with two device-to-device copies of this many MiB (the known-size "
                          "reference for scaling FETCH_SIZE / WRITE_SIZE counters)")
@@ -264,10 +345,16 @@ def main():
     av = np.concatenate([rng.integers(0, int(m), N, dtype=np.uint64) for _ in range(DNUM) for m in mods])
     bv = np.concatenate([rng.integers(0, int(m), N, dtype=np.uint64) for _ in range(DNUM) for m in mods])
     key = G.Key(ctx, av, bv, DNUM, stream)
-    radix, ntt_ms = ctx.autotune(stream)   # picks the NTT register radix for this context
-    log("context ready: K=%d alpha=%d ws=%.1f MiB/op, autotuned NTT radix 2^%d (%.3f ms per 96 fwd+inv)"
+    if args.ntt_radix:
+        # pinned register radix (profiling runs, where per-dispatch overhead skews the autotuner)
+        ctx.set_ntt_radix(args.ntt_radix)
+        radix, ntt_ms = args.ntt_radix, float("nan")
+    else:
+        radix, ntt_ms = ctx.autotune(stream)   # picks the NTT register radix for this context
+    log("context ready: K=%d alpha=%d ws=%.1f MiB/op, NTT radix 2^%d (%.3f ms per 96 fwd+inv)"
         % (K, alpha, ctx.ws_bytes / 2**20, radix, ntt_ms))
-    tuned = {"ntt_radix_log2": radix, "ntt_96_fwd_inv_ms": round(ntt_ms, 4)}
+    tuned = {"ntt_radix_log2": radix, "pinned": bool(args.ntt_radix),
+             "ntt_96_fwd_inv_ms": None if args.ntt_radix else round(ntt_ms, 4)}
 
     B = args.batch
     res = run_batched(G, ctx, key, B, args.steps, args.warmup, stream)
@@ -296,6 +383,8 @@ def main():
 
     perop_ops, perop_ms = run_per_op(G, ctx, key, args.perop_ops, 3, stream)
     log("per-op: %.1f ops/s (%.3f ms/op)" % (perop_ops, perop_ms))
+    perjob_ops, perjob_ms = run_per_op(G, ctx, key, 4, 2, stream, per_job=True)
+    log("per-op, one launch per tower and stage: %.1f ops/s (%.3f ms/op)" % (perjob_ops, perjob_ms))
 
     sweep = {}
     if args.sweep:
@@ -305,6 +394,14 @@ def main():
             log("sweep B=%d: %.1f ops/s" % (b, sweep[str(b)]))
             del r
     sweep[str(B)] = round(value, 1)
+
+    serving = None
+    if args.serve_seconds > 0:
+        del res
+        serving = run_serving(G, stream, args.serve_seconds, args.serve_rate)
+        log("serving: %d requests at %.0f req/s: p50 %.3f ms, p99 %.3f ms, %.1f ops/s, mean batch %.2f"
+            % (serving["requests"], args.serve_rate, serving["p50_ms"], serving["p99_ms"], serving["ops_s"],
+               serving["mean_batch"]))
 
     cpu = None
     if not args.no_cpu:
@@ -339,12 +436,15 @@ def main():
         "autotune": tuned,
         "per_op_ops_s": round(perop_ops, 2),
         "batched_over_per_op": round(value / perop_ops, 3),
+        "per_op_per_tower_launch_ops_s": round(perjob_ops, 2),
+        "batched_over_per_tower_launch": round(value / perjob_ops, 3),
         "batch_sweep_ops_s": sweep,
         "stages": stage_report,
         "roofline": {"bound": "hbm", "kernel": dom["stage"] + ":" + dom["kernel"],
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algorithmic_bytes": dom["bytes"], "launch_ms": round(dom["ms"], 4)},
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -9,7 +9,7 @@
 #include <vector>
 
 #include "../../include/gpufhe_capi.h"
-#include "engine.hpp"
+#include "handles.hpp"
 
 namespace gpufhe {
 void native_primes(std::vector<uint64_t>&, uint32_t, uint32_t, uint32_t, const std::vector<uint64_t>&);
@@ -20,39 +20,8 @@ void native_rescale_consts(std::vector<uint64_t>&, std::vector<uint64_t>&, const
 
 using namespace gf;
 
-struct gf_ctx {
-    std::unique_ptr<Ctx> C;
-    std::mutex mu;
-    std::map<std::vector<uint64_t>, std::unique_ptr<Plan>> cache;  // single-op plans
-};
-struct gf_key {
-    gf_ctx* ctx;
-    std::unique_ptr<DevKey> K;
-};
-struct gf_plan {
-    std::unique_ptr<Plan> P;
-};
-
 static thread_local std::string g_err;
-template <class F>
-static int guard(F&& f) {
-    try {
-        f();
-        return 0;
-    } catch (const std::exception& e) {
-        g_err = e.what();
-        return -1;
-    }
-}
-template <class T, class F>
-static T* guard_ptr(F&& f) {
-    try {
-        return f();
-    } catch (const std::exception& e) {
-        g_err = e.what();
-        return nullptr;
-    }
-}
+void gf::set_last_error(const char* msg) { g_err = msg; }
 
 // single-op plans are cached by (op, arguments); building one uploads its job
 // arrays synchronously, which a stream capture cannot contain
@@ -161,6 +130,32 @@ gf_key* gf_key_create_rows(gf_ctx* c, const uint64_t* av, const uint64_t* bv, ui
     });
 }
 uint32_t gf_ctx_towers(const gf_ctx* c) { return c->C->L; }
+
+gf_key* gf_key_create_uniform(gf_ctx* c, uint32_t nparts, uint64_t seed, hipStream_t s) {
+    return guard_ptr<gf_key>([&] {
+        const Ctx& C = *c->C;
+        if (nparts < C.dnum) throw std::runtime_error("gpufhe: key has fewer parts than the context needs");
+        auto K = std::make_unique<DevKey>();
+        K->nparts = nparts;
+        K->rows = C.L + C.K;
+        K->fullQ = C.L;
+        const size_t per = (size_t)K->rows * C.n, words = (size_t)nparts * per;
+        K->a.alloc(words * 8);
+        K->b.alloc(words * 8);
+        DevBuf dm(C.mods.size() * 8);
+        hip_check(hipMemcpy(dm.p, C.mods.data(), dm.bytes, hipMemcpyHostToDevice), "upload moduli");
+        // uniform residues are uniform in Montgomery form too: no conversion needed
+        for (uint32_t p = 0; p < nparts; ++p) {
+            launch_fill(K->a.as<uint64_t>() + p * per, dm.as<uint64_t>(), K->rows, C.n, seed * 1000003 + 2 * p, s);
+            launch_fill(K->b.as<uint64_t>() + p * per, dm.as<uint64_t>(), K->rows, C.n, seed * 1000003 + 2 * p + 1, s);
+        }
+        hip_check(hipStreamSynchronize(s), "key fill");
+        auto k = new gf_key;
+        k->ctx = c;
+        k->K = std::move(K);
+        return k;
+    });
+}
 
 int gf_root_tables(uint32_t n, uint64_t q, uint64_t root, const uint64_t** fr, const uint64_t** fp,
                    const uint64_t** ir, const uint64_t** ip, uint64_t* ninv, uint64_t* ninv_precon) {
@@ -296,6 +291,7 @@ gf_plan* gf_plan_add(gf_ctx* c, uint32_t count, const uint32_t* levels, const ui
     });
 }
 int gf_plan_run(gf_plan* p, hipStream_t s) { return guard([&] { p->P->run(s); }); }
+int gf_plan_run_per_job(gf_plan* p, hipStream_t s) { return guard([&] { p->P->run_per_job(s); }); }
 int gf_plan_run_graph(gf_plan* p, hipStream_t s) {
     return guard([&] {
         if (!s) throw std::runtime_error("gpufhe: graph capture needs a non-null stream");

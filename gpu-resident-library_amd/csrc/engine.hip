@@ -312,28 +312,35 @@ uint32_t Plan::step_kernels(size_t i) const {
     return (k == Step::NTT_F || k == Step::NTT_I) && !geom.single ? 2u : 1u;
 }
 
-void Plan::run_step(size_t i, hipStream_t s, uint32_t which) const {
-    const char* base = (const char*)dev.p;
-    {
-        const Step& st = steps[i];
-        const void* p = base + st.off;
-        if (st.kind != Step::NTT_F && st.kind != Step::NTT_I && !(which & 1)) return;
-        switch (st.kind) {
-            case Step::NTT_F: launch_ntt((const NttJob*)p, st.count, geom, true, s, which); break;
-            case Step::NTT_I: launch_ntt((const NttJob*)p, st.count, geom, false, s, which); break;
-            case Step::TENSOR: launch_tensor((const TensorJob*)p, st.count, n, s); break;
-            case Step::CONV: launch_conv((const ConvJob*)p, st.count, n, s); break;
-            case Step::INNER: launch_inner((const InnerJob*)p, st.count, n, s); break;
-            case Step::COMBINE: launch_combine((const CombineJob*)p, st.count, n, s); break;
-            case Step::SWITCH: launch_switch((const SwitchJob*)p, st.count, n, s); break;
-            case Step::FUSE: launch_fuse((const FuseJob*)p, st.count, n, s); break;
-            case Step::ADD: launch_add((const AddJob*)p, st.count, n, s); break;
-            case Step::PERM: launch_perm((const PermJob*)p, st.count, n, logn, s); break;
-            case Step::COPY: launch_copy((const CopyJob*)p, st.count, n, s); break;
-            case Step::DROP: launch_drop((const DropJob*)p, st.count, n, s); break;
-            case Step::RSV: launch_rs((const RsJob*)p, st.count, n, s); break;
-        }
+void Plan::run_step(size_t i, hipStream_t s, uint32_t which) const { run_jobs(i, s, which, 0, steps[i].count); }
+
+void Plan::run_jobs(size_t i, hipStream_t s, uint32_t which, uint32_t first, uint32_t cnt) const {
+    const char* base = (const char*)(this->base ? this->base : dev.p);
+    const Step& st = steps[i];
+    const char* p = base + st.off;
+    if (st.kind != Step::NTT_F && st.kind != Step::NTT_I && !(which & 1)) return;
+#define GF_J(T) ((const T*)p + first)
+    switch (st.kind) {
+        case Step::NTT_F: launch_ntt(GF_J(NttJob), cnt, geom, true, s, which); break;
+        case Step::NTT_I: launch_ntt(GF_J(NttJob), cnt, geom, false, s, which); break;
+        case Step::TENSOR: launch_tensor(GF_J(TensorJob), cnt, n, s); break;
+        case Step::CONV: launch_conv(GF_J(ConvJob), cnt, n, s); break;
+        case Step::INNER: launch_inner(GF_J(InnerJob), cnt, n, s); break;
+        case Step::COMBINE: launch_combine(GF_J(CombineJob), cnt, n, s); break;
+        case Step::SWITCH: launch_switch(GF_J(SwitchJob), cnt, n, s); break;
+        case Step::FUSE: launch_fuse(GF_J(FuseJob), cnt, n, s); break;
+        case Step::ADD: launch_add(GF_J(AddJob), cnt, n, s); break;
+        case Step::PERM: launch_perm(GF_J(PermJob), cnt, n, logn, s); break;
+        case Step::COPY: launch_copy(GF_J(CopyJob), cnt, n, s); break;
+        case Step::DROP: launch_drop(GF_J(DropJob), cnt, n, s); break;
+        case Step::RSV: launch_rs(GF_J(RsJob), cnt, n, s); break;
     }
+#undef GF_J
+}
+
+void Plan::run_per_job(hipStream_t s) const {
+    for (size_t i = 0; i < steps.size(); ++i)
+        for (uint32_t j = 0; j < steps[i].count; ++j) run_jobs(i, s, 3, j, 1);
 }
 
 void Plan::run_graph(hipStream_t s) {
@@ -607,6 +614,28 @@ static void group_by(std::vector<T>& v, K key) {
 }
 
 std::unique_ptr<Plan> PlanBuilder::finish() {
+    std::vector<char> blob;
+    auto P = layout(blob);
+    P->dev.alloc(blob.size());
+    if (!blob.empty())
+        hip_check(hipMemcpy(P->dev.p, blob.data(), blob.size(), hipMemcpyHostToDevice), "upload plan");
+    P->base = P->dev.p;
+    return P;
+}
+
+std::unique_ptr<Plan> PlanBuilder::finish_into(void* d_arena, void* h_stage, size_t cap, hipStream_t s) {
+    std::vector<char> blob;
+    auto P = layout(blob);
+    if (blob.size() > cap) throw std::runtime_error("gpufhe: batch plan larger than the job arena");
+    if (!blob.empty()) {
+        std::memcpy(h_stage, blob.data(), blob.size());
+        hip_check(hipMemcpyAsync(d_arena, h_stage, blob.size(), hipMemcpyHostToDevice, s), "stage plan");
+    }
+    P->base = d_arena;
+    return P;
+}
+
+std::unique_ptr<Plan> PlanBuilder::layout(std::vector<char>& blob) {
     for (auto* v : {&intt_up_, &ntt_dig_, &intt_p_, &ntt_conv_, &intt_drop_, &ntt_sw_, &ntt_v_})
         group_by(*v, [](const NttJob& j) { return (uintptr_t)j.tw; });
     group_by(inner_, [](const InnerJob& j) { return (uintptr_t)j.kb[0]; });
@@ -616,7 +645,7 @@ std::unique_ptr<Plan> PlanBuilder::finish() {
     P->n = C_.n;
     P->logn = C_.logn;
     P->geom = C_.geom;
-    std::vector<char> blob;
+    blob.clear();
     auto stage = [&](Step::Kind kind, const void* data, size_t cnt, size_t sz) {
         if (!cnt) return;
         size_t off = (blob.size() + 15) & ~(size_t)15;
@@ -646,9 +675,6 @@ std::unique_ptr<Plan> PlanBuilder::finish() {
     GF_STAGE(NTT_F, ntt_v_);
     GF_STAGE(ADD, add_);
 #undef GF_STAGE
-    P->dev.alloc(blob.size());
-    if (!blob.empty())
-        hip_check(hipMemcpy(P->dev.p, blob.data(), blob.size(), hipMemcpyHostToDevice), "upload plan");
     return P;
 }
 

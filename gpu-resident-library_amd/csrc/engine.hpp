@@ -114,13 +114,18 @@ struct Step {
 class Plan {
 public:
     std::vector<Step> steps;
-    DevBuf dev;
+    DevBuf dev;                  // job arrays, when the plan owns them
+    const void* base = nullptr;  // where the job arrays live (dev.p or a caller arena)
     uint32_t n = 0, logn = 0;
     NttGeom geom{};
     void run(hipStream_t s) const;
     // which: 3 = every kernel of step i; 1 / 2 = only its first / second kernel
     void run_step(size_t i, hipStream_t s, uint32_t which = 3) const;
     uint32_t step_kernels(size_t i) const;   // 2 for a two-phase transform, else 1
+    void run_jobs(size_t i, hipStream_t s, uint32_t which, uint32_t first, uint32_t cnt) const;
+    // every job of every step as its own launch: the one-launch-per-tower
+    // structure of the reference's resident path, for comparison
+    void run_per_job(hipStream_t s) const;
     // hipGraph of the whole sequence (captured on first use)
     hipGraphExec_t graph = nullptr;
     void run_graph(hipStream_t s);
@@ -148,6 +153,11 @@ public:
     void add(uint32_t l, const uint64_t* a0, const uint64_t* a1, const uint64_t* b0,
              const uint64_t* b1, uint64_t* o0, uint64_t* o1);
     std::unique_ptr<Plan> finish();
+    // the same, with the job arrays staged in pinned host memory h_stage and
+    // copied on stream s into the caller's device arena (both `cap` bytes): no
+    // allocation and no host synchronisation.  The arena and the staging area
+    // must stay untouched until the plan's work on s has completed.
+    std::unique_ptr<Plan> finish_into(void* d_arena, void* h_stage, size_t cap, hipStream_t s);
 
 private:
     const Ctx& C_;
@@ -177,6 +187,7 @@ private:
     void keyswitch_core(uint32_t l, const uint64_t* d, const DevKey& key, uint64_t* ws,
                         const uint64_t* extra0, const uint64_t* extra1, uint64_t* dx0, uint64_t* dx1);
     std::vector<AddJob> add_;
+    std::unique_ptr<Plan> layout(std::vector<char>& blob);
     void rescale_comp(uint32_t l, const uint64_t* c, uint64_t* o, uint64_t* drop, uint64_t* sw);
 };
 

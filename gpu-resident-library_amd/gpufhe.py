@@ -48,6 +48,7 @@ _capi = {
     "gf_ctx_create_alpha": (vp, [u32, u64p, u32, u64p, u32, u64p, u32]),
     "gf_ctx_towers": (u32, [vp]),
     "gf_key_create_rows": (vp, [vp, u64p, u64p, u32, u32, u32, vp]),
+    "gf_key_create_uniform": (vp, [vp, u32, u64, vp]),
     "gf_root_tables": (C.c_int, [u32, u64, u64, vpp, vpp, vpp, vpp, u64p, u64p]),
     "gf_ntt_towers": (C.c_int, [u32, vp, u32, u64p, u64p, C.c_int, vp]),
     "gf_automorph": (C.c_int, [u32, vp, vp, u32, u64, vp]),
@@ -62,6 +63,7 @@ _capi = {
     "gf_plan_rotate": (vp, [vp, vpp, u32, u32p, u64p] + [vpp] * 5),
     "gf_plan_add": (vp, [vp, u32, u32p] + [vpp] * 6),
     "gf_plan_run": (C.c_int, [vp, vp]),
+    "gf_plan_run_per_job": (C.c_int, [vp, vp]),
     "gf_plan_run_graph": (C.c_int, [vp, vp]),
     "gf_plan_launches": (u32, [vp]),
     "gf_plan_destroy": (None, [vp]),
@@ -218,6 +220,16 @@ class Key:
         if not self.h:
             raise GpuFheError(lib.gf_last_error().decode())
 
+    @classmethod
+    def uniform(cls, ctx, nparts, seed, stream=None):
+        """Synthetic key of uniform residues generated on the device."""
+        k = cls.__new__(cls)
+        k.ctx = ctx
+        k.h = lib.gf_key_create_uniform(ctx.h, nparts, seed, stream.s if stream else None)
+        if not k.h:
+            raise GpuFheError(lib.gf_last_error().decode())
+        return k
+
     def __del__(self):
         if getattr(self, "h", None):
             lib.gf_key_destroy(self.h)
@@ -234,6 +246,9 @@ class Plan:
 
     def run(self, stream):
         check(lib.gf_plan_run(self.h, stream.s if stream else None))
+
+    def run_per_job(self, stream):
+        check(lib.gf_plan_run_per_job(self.h, stream.s if stream else None))
 
     def run_graph(self, stream):
         check(lib.gf_plan_run_graph(self.h, stream.s))
@@ -307,3 +322,96 @@ def plan_add(ctx, levels, a0, a1, b0, b1, o0, o1):
     lv = np.asarray(levels, np.uint32)
     arrs = [ptr_array(x) for x in (a0, a1, b0, b1, o0, o1)]
     return Plan(lib.gf_plan_add(ctx.h, len(levels), lv.ctypes.data_as(u32p), *arrs), keep=(lv, arrs))
+
+
+# ---- serving ----
+OP_EVALMULT_RESCALE, OP_ROTATE, OP_ADD = 0, 1, 2
+
+
+class Request(C.Structure):
+    _fields_ = [("ring", C.c_uint32), ("op", C.c_uint32), ("level", C.c_uint32), ("rot", C.c_uint32),
+                ("a0", C.c_void_p), ("a1", C.c_void_p), ("b0", C.c_void_p), ("b1", C.c_void_p),
+                ("o0", C.c_void_p), ("o1", C.c_void_p), ("arrival_ms", C.c_double)]
+
+
+class Completion(C.Structure):
+    _fields_ = [("id", C.c_uint64), ("arrival_ms", C.c_double), ("launch_ms", C.c_double),
+                ("done_ms", C.c_double), ("batch_size", C.c_uint32), ("ring", C.c_uint32),
+                ("op", C.c_uint32), ("level", C.c_uint32)]
+
+
+_serve = {
+    "gf_server_create": (vp, [sz, u32, u32]),
+    "gf_server_destroy": (None, [vp]),
+    "gf_server_add_ring": (C.c_int, [vp, vp, vp, vpp, u64p, u32, u32p]),
+    "gf_server_slot_bytes": (sz, [vp, u32]),
+    "gf_server_now_ms": (C.c_double, [vp]),
+    "gf_server_submit": (C.c_int, [vp, C.POINTER(Request), C.POINTER(C.c_uint64)]),
+    "gf_server_step": (C.c_int, [vp, u32p]),
+    "gf_server_poll": (u32, [vp, C.POINTER(Completion), u32]),
+    "gf_server_pending": (u32, [vp]),
+    "gf_server_drain": (C.c_int, [vp]),
+    "gf_server_run_trace": (C.c_int, [vp, u32, C.POINTER(Request), C.POINTER(Completion)]),
+    "gf_server_stats": (C.c_int, [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_size_t)]),
+}
+for _n, (_r, _a) in _serve.items():
+    _f = getattr(lib, _n)
+    _f.restype = _r
+    _f.argtypes = _a
+
+
+class Server:
+    """Continuous-batching server over resident ciphertexts (gf_server_*)."""
+
+    def __init__(self, ws_budget_bytes, lanes=4, max_batch=64):
+        self.h = lib.gf_server_create(ws_budget_bytes, lanes, max_batch)
+        if not self.h:
+            raise GpuFheError(lib.gf_last_error().decode())
+        self._keep = []
+
+    def add_ring(self, ctx, rlk=None, rot_keys=(), rot_k=()):
+        arr = (C.c_void_p * max(1, len(rot_keys)))(*[k.h for k in rot_keys])
+        ks = np.asarray(list(rot_k) or [1], np.uint64)
+        out = C.c_uint32()
+        check(lib.gf_server_add_ring(self.h, ctx.h, rlk.h if rlk else None, arr, ks.ctypes.data_as(u64p),
+                                     len(rot_keys), C.byref(out)))
+        self._keep.append((ctx, rlk, tuple(rot_keys)))
+        return out.value
+
+    def slot_bytes(self, ring):
+        return int(lib.gf_server_slot_bytes(self.h, ring))
+
+    def run_trace(self, reqs):
+        n = len(reqs)
+        arr = (Request * n)(*reqs)
+        out = (Completion * n)()
+        check(lib.gf_server_run_trace(self.h, n, arr, out))
+        return list(out)
+
+    def submit(self, req):
+        rid = C.c_uint64()
+        check(lib.gf_server_submit(self.h, C.byref(req), C.byref(rid)))
+        return rid.value
+
+    def step(self):
+        n = C.c_uint32()
+        check(lib.gf_server_step(self.h, C.byref(n)))
+        return n.value
+
+    def poll(self, max_n=4096):
+        out = (Completion * max_n)()
+        k = lib.gf_server_poll(self.h, out, max_n)
+        return list(out[:k])
+
+    def drain(self):
+        check(lib.gf_server_drain(self.h))
+
+    def stats(self):
+        b, a, s = C.c_uint64(), C.c_uint64(), C.c_size_t()
+        lib.gf_server_stats(self.h, C.byref(b), C.byref(a), C.byref(s))
+        return dict(batches=b.value, admitted=a.value, slot_bytes=s.value)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.gf_server_destroy(self.h)
+            self.h = None

@@ -78,6 +78,9 @@ uint32_t gf_ctx_towers(const gf_ctx* c);
  * (src/keyswitch_resident.cpp:84-93, KeySwitchConstants::evalKeyTowers). */
 gf_key* gf_key_create_rows(gf_ctx* c, const uint64_t* h_av, const uint64_t* h_bv, uint32_t nparts,
                            uint32_t rows, uint32_t fullQ, hipStream_t s);
+/* Synthetic switching key of uniform residues, generated on the device from
+ * seed (benchmark key material: a switching key is computationally uniform). */
+gf_key* gf_key_create_uniform(gf_ctx* c, uint32_t nparts, uint64_t seed, hipStream_t s);
 
 /* ---- tower utilities ----
  * Device root tables of (n, q, root) in the reference layout
@@ -138,6 +141,9 @@ gf_plan* gf_plan_add(gf_ctx* c, uint32_t count, const uint32_t* h_levels,
                      const uint64_t* const* b0, const uint64_t* const* b1, uint64_t* const* o0,
                      uint64_t* const* o1);
 int gf_plan_run(gf_plan* p, hipStream_t s);
+/* the same work with every job (one tower of one stage) as its own launch:
+ * the launch structure of the reference's resident path, for comparison */
+int gf_plan_run_per_job(gf_plan* p, hipStream_t s);
 /* capture the plan into a hipGraph on first call (s must not be the null
  * stream), then replay it */
 int gf_plan_run_graph(gf_plan* p, hipStream_t s);
@@ -178,6 +184,54 @@ int gf_event_create(hipEvent_t* e);
 int gf_event_destroy(hipEvent_t e);
 int gf_event_record(hipEvent_t e, hipStream_t s);
 int gf_event_elapsed_ms(float* ms, hipEvent_t start, hipEvent_t stop);
+
+/* ---- serving: continuous batching over resident ciphertexts ----
+ * No counterpart in the reference (it serves nothing).  A server owns a few
+ * lanes (stream + job arena) and a pool of per-operation workspace slots
+ * capped by ws_budget_bytes (admission by HBM footprint).  Requests wait in
+ * arrival order; each step retires finished batches and, for every idle lane,
+ * admits the waiting requests of the oldest request's ring that fit, as one
+ * plan.  Batches on different lanes overlap.  Times are milliseconds on the
+ * server clock (completion = device event).  Single-threaded: call a server
+ * from one thread at a time. */
+enum { GF_OP_EVALMULT_RESCALE = 0, GF_OP_ROTATE = 1, GF_OP_ADD = 2 };
+typedef struct gf_server gf_server;
+typedef struct {
+    uint32_t ring;        /* from gf_server_add_ring */
+    uint32_t op;          /* GF_OP_* */
+    uint32_t level;       /* Q towers of the inputs */
+    uint32_t rot;         /* rotation-key index of the ring (GF_OP_ROTATE) */
+    const uint64_t *a0, *a1, *b0, *b1;   /* device inputs (b unused by rotate) */
+    uint64_t *o0, *o1;    /* device outputs, or NULL: result in the server's slot,
+                             valid until the slot serves another request */
+    double arrival_ms;    /* submit: < 0 means now; trace: offset from the start */
+} gf_request;
+typedef struct {
+    uint64_t id;
+    double arrival_ms, launch_ms, done_ms;
+    uint32_t batch_size, ring, op, level;
+} gf_completion;
+
+gf_server* gf_server_create(size_t ws_budget_bytes, uint32_t lanes, uint32_t max_batch);
+void gf_server_destroy(gf_server* srv);
+/* a ring: context, relinearisation key (may be NULL), nrot rotation keys with
+ * their automorphism indices */
+int gf_server_add_ring(gf_server* srv, gf_ctx* c, gf_key* rlk, gf_key* const* rot_keys,
+                       const uint64_t* h_rot_k, uint32_t nrot, uint32_t* ring_out);
+size_t gf_server_slot_bytes(const gf_server* srv, uint32_t ring);
+double gf_server_now_ms(const gf_server* srv);
+int gf_server_submit(gf_server* srv, const gf_request* req, uint64_t* id_out);
+/* retire finished batches and launch new ones; never blocks on the device */
+int gf_server_step(gf_server* srv, uint32_t* launched);
+uint32_t gf_server_poll(gf_server* srv, gf_completion* out, uint32_t max);
+uint32_t gf_server_pending(const gf_server* srv);
+int gf_server_drain(gf_server* srv);
+/* replay `count` requests in real time (arrival_ms ascending, offsets from
+ * the call); out[i] receives request i's completion */
+int gf_server_run_trace(gf_server* srv, uint32_t count, const gf_request* reqs,
+                        gf_completion* out);
+int gf_server_stats(const gf_server* srv, uint64_t* batches, uint64_t* admitted,
+                    size_t* slot_bytes_used);
 
 #ifdef __cplusplus
 }

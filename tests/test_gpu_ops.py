@@ -198,6 +198,15 @@ def test_batch_plan_uneven_levels_and_graph(G):
             assert np.array_equal(outs[i][0].download(), want[i][0]), (rep, i)
             assert np.array_equal(outs[i][1].download(), want[i][1]), (rep, i)
     assert plan.launches < 30
+    # the same jobs, one launch each (the per-tower launch structure)
+    for o in outs:
+        o[0].upload(np.zeros(o[0].words, np.uint64))
+        o[1].upload(np.zeros(o[1].words, np.uint64))
+    plan.run_per_job(s)
+    s.sync()
+    for i in range(len(levels)):
+        assert np.array_equal(outs[i][0].download(), want[i][0]), ("per job", i)
+        assert np.array_equal(outs[i][1].download(), want[i][1]), ("per job", i)
 
 
 def test_rotate_plan_multi_keys(G):

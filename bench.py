@@ -238,6 +238,115 @@ def run_per_op(G, ctx, key, nops, reps, stream, per_job=False):
     return nops * reps / (ms / 1e3), ms / (nops * reps)
 
 
+def run_chain(G, stream, n=1 << 15, L=12, dnum=3, batch=16, depth=10, reps=5):
+    """Batched resident multiply-rescale chain: `batch` independent ciphertexts,
+    each multiplied (EvalMult+Rescale) `depth` times by a resident operand of
+    the matching level, L -> L - depth.  One plan per chain step over the
+    whole batch, the steps captured as hipGraphs and replayed in order."""
+    q, p, roots, K, alpha = native_params(G, n, L, dnum)
+    ctx = G.Context(n, q, p, roots, dnum)
+    ctx.autotune(stream)
+    key = G.Key.uniform(ctx, dnum, 77, stream)
+    cts = [[G.Dev(lv * n) for _ in range(2)] for lv in range(L, L - depth - 1, -1) for _ in range(batch)]
+    ops = [[G.Dev(lv * n) for _ in range(2)] for lv in range(L, L - depth, -1) for _ in range(batch)]
+    for i, x in enumerate(cts[:batch] + ops):
+        lv = x[0].words // n
+        for j, d in enumerate(x):
+            G.check(G.lib.gf_fill_uniform(ctx.h, d.ptr, lv, 31 * i + j, stream.s))
+    ws = [ctx.workspace() for _ in range(batch)]
+    plans = []
+    for d in range(depth):
+        lv = L - d
+        src = cts[d * batch:(d + 1) * batch]
+        dst = cts[(d + 1) * batch:(d + 2) * batch]
+        opd = ops[d * batch:(d + 1) * batch]
+        plans.append(G.plan_evalmult_rescale(ctx, key, [lv] * batch, [x[0].ptr for x in src], [x[1].ptr for x in src],
+                                             [x[0].ptr for x in opd], [x[1].ptr for x in opd],
+                                             [x[0].ptr for x in dst], [x[1].ptr for x in dst], [w.ptr for w in ws]))
+    for pl in plans:
+        pl.run_graph(stream)
+    stream.sync()
+    e0, e1 = G.Event(), G.Event()
+    e0.record(stream)
+    for _ in range(reps):
+        for pl in plans:
+            pl.run_graph(stream)
+    e1.record(stream)
+    stream.sync()
+    ms = e0.elapsed_ms(e1) / reps
+    return dict(workload="EvalMult+Rescale chain, n=32768, L=12 -> 2 (60+11x50-bit), dnum=3, K=%d" % K,
+                batch=batch, depth=depth, ms_per_chain_batch=round(ms, 3),
+                ops_s=round(batch * depth / (ms / 1e3), 1), launches_per_step=plans[0].launches)
+
+
+def run_rotations(G, stream, n=1 << 16, L=20, dnum=4, count=64, nkeys=16, reps=10, seed=9):
+    """Batched rotations: `count` resident ciphertexts at levels uniform in
+    8..L, each rotated by one of `nkeys` power-of-two slot offsets (Galois
+    index 5^(2^j) mod 2n, one resident key per offset), one plan, hipGraph."""
+    rng = np.random.default_rng(seed)
+    q, p, roots, K, alpha = native_params(G, n, L, dnum)
+    ctx = G.Context(n, q, p, roots, dnum)
+    ctx.autotune(stream)
+    keys = [G.Key.uniform(ctx, dnum, 500 + j, stream) for j in range(nkeys)]
+    ks = [pow(5, 1 << j, 2 * n) for j in range(nkeys)]
+    levels = [int(x) for x in rng.integers(8, L + 1, count)]
+    which = [i % nkeys for i in range(count)]
+    cin = [[G.Dev(lv * n) for _ in range(2)] for lv in levels]
+    cout = [[G.Dev(lv * n) for _ in range(2)] for lv in levels]
+    for i, x in enumerate(cin):
+        for j, d in enumerate(x):
+            G.check(G.lib.gf_fill_uniform(ctx.h, d.ptr, levels[i], 91 * i + j, stream.s))
+    ws = [ctx.workspace() for _ in range(count)]
+    plan = G.plan_rotate(ctx, [keys[w] for w in which], levels, [ks[w] for w in which],
+                         [x[0].ptr for x in cin], [x[1].ptr for x in cin], [x[0].ptr for x in cout],
+                         [x[1].ptr for x in cout], [w.ptr for w in ws])
+    plan.run_graph(stream)
+    stream.sync()
+    e0, e1 = G.Event(), G.Event()
+    e0.record(stream)
+    for _ in range(reps):
+        plan.run_graph(stream)
+    e1.record(stream)
+    stream.sync()
+    ms = e0.elapsed_ms(e1) / reps
+    out = dict(workload="batched rotations, n=65536, L=20 (60+19x50-bit), dnum=4, K=%d, 16 keys" % K,
+               count=count, levels="8..20 (mean %.1f)" % (sum(levels) / count), ms_per_batch=round(ms, 3),
+               ops_s=round(count / (ms / 1e3), 1), launches=plan.launches)
+    del plan, cin, cout, ws
+
+    # hoisting: 4 ciphertexts at level L, each rotated by all 16 indices, one
+    # decomposition per ciphertext, against the same 64 rotations unhoisted
+    nct = count // nkeys
+    hin = [[G.Dev(L * n) for _ in range(2)] for _ in range(nct)]
+    for i, x in enumerate(hin):
+        for j, d in enumerate(x):
+            G.check(G.lib.gf_fill_uniform(ctx.h, d.ptr, L, 7 * i + j + 3, stream.s))
+    hout = [[G.Dev(L * n) for _ in range(2)] for _ in range(nct * nkeys)]
+    wb = [ctx.workspace() for _ in range(nct)]
+    wr = [ctx.workspace() for _ in range(nct * nkeys)]
+    hp = G.plan_rotate_hoisted(ctx, [L] * nct, [x[0].ptr for x in hin], [x[1].ptr for x in hin], keys, ks,
+                               [x[0].ptr for x in hout], [x[1].ptr for x in hout], [w.ptr for w in wb],
+                               [w.ptr for w in wr])
+    up = G.plan_rotate(ctx, [keys[j] for _ in range(nct) for j in range(nkeys)], [L] * (nct * nkeys),
+                       [ks[j] for _ in range(nct) for j in range(nkeys)],
+                       [hin[i][0].ptr for i in range(nct) for _ in range(nkeys)],
+                       [hin[i][1].ptr for i in range(nct) for _ in range(nkeys)],
+                       [x[0].ptr for x in hout], [x[1].ptr for x in hout], [w.ptr for w in wr])
+    res = {}
+    for name, pl in (("hoisted", hp), ("unhoisted", up)):
+        pl.run_graph(stream)
+        stream.sync()
+        e0.record(stream)
+        for _ in range(reps):
+            pl.run_graph(stream)
+        e1.record(stream)
+        stream.sync()
+        res[name] = round(nct * nkeys / (e0.elapsed_ms(e1) / reps / 1e3), 1)
+    out["hoisting"] = dict(ciphertexts=nct, rotations_each=nkeys, level=L, hoisted_ops_s=res["hoisted"],
+                           unhoisted_ops_s=res["unhoisted"])
+    return out
+
+
 SERVE_RINGS = ((1 << 14, 0.3), (1 << 15, 0.4), (1 << 16, 0.3))
 SERVE_L, SERVE_DNUM, SERVE_KEYS = 20, 4, 16
 SERVE_MIX = (0.6, 0.3, 0.1)      # EvalMult+Rescale, Rotate, EvalAdd
@@ -321,11 +430,14 @@ def main():
     ap.add_argument("--perop-ops", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-extra", action="store_true", help="skip the chain and rotation workloads")
     ap.add_argument("--ntt-radix", type=int, default=0, choices=(0, 2, 3, 4),
                     help="pin the NTT register radix (log2) instead of autotuning")
     ap.add_argument("--serve-seconds", type=float, default=10.0,
                     help="length of the serving trace (0 skips it)")
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--serve-saturate", type=float, default=20000.0,
+                    help="offered rate of a 2 s overload trace (0 skips it)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="finish with two device-to-device copies of this many MiB (the known-size "
                          "reference for scaling FETCH_SIZE / WRITE_SIZE counters)")
@@ -395,13 +507,29 @@ def main():
             del r
     sweep[str(B)] = round(value, 1)
 
+    del res
+    chain = rotations = None
+    if not args.no_extra:
+        chain = run_chain(G, stream)
+        log("chain (n=2^15, L=12, dnum=3, 16 x 10 deep): %.1f ops/s" % chain["ops_s"])
+        rotations = run_rotations(G, stream)
+        log("rotations (n=2^16, L=20, 64 cts, levels 8..20, 16 keys): %.1f ops/s; 4 cts x 16 at L=20: "
+            "hoisted %.1f, unhoisted %.1f ops/s" % (rotations["ops_s"], rotations["hoisting"]["hoisted_ops_s"],
+                                                   rotations["hoisting"]["unhoisted_ops_s"]))
+
     serving = None
     if args.serve_seconds > 0:
-        del res
         serving = run_serving(G, stream, args.serve_seconds, args.serve_rate)
         log("serving: %d requests at %.0f req/s: p50 %.3f ms, p99 %.3f ms, %.1f ops/s, mean batch %.2f"
             % (serving["requests"], args.serve_rate, serving["p50_ms"], serving["p99_ms"], serving["ops_s"],
                serving["mean_batch"]))
+        if args.serve_saturate > 0:
+            # the same mix offered faster than it can be served: capacity and batching under load
+            sat = run_serving(G, stream, 2.0, args.serve_saturate, seed=6)
+            serving["saturated"] = {k: sat[k] for k in ("requests", "offered_rate_req_s", "ops_s", "p50_ms",
+                                                        "p99_ms", "mean_batch", "batches", "slot_pool_gib")}
+            log("serving, offered %.0f req/s: %.1f ops/s, p50 %.3f ms, p99 %.3f ms, mean batch %.2f"
+                % (args.serve_saturate, sat["ops_s"], sat["p50_ms"], sat["p99_ms"], sat["mean_batch"]))
 
     cpu = None
     if not args.no_cpu:
@@ -444,6 +572,8 @@ def main():
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algorithmic_bytes": dom["bytes"], "launch_ms": round(dom["ms"], 4)},
+        "chain": chain,
+        "rotations": rotations,
         "serving": serving,
         "cpu_baseline": cpu,
     }

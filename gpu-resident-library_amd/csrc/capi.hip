@@ -279,6 +279,23 @@ gf_plan* gf_plan_rotate(gf_ctx* c, gf_key* const* rks, uint32_t count, const uin
         return p;
     });
 }
+gf_plan* gf_plan_rotate_hoisted(gf_ctx* c, uint32_t count, const uint32_t* levels, const uint64_t* const* c0,
+                                const uint64_t* const* c1, uint32_t m, gf_key* const* rks, const uint64_t* ks,
+                                uint64_t* const* o0, uint64_t* const* o1, uint64_t* const* ws_base,
+                                uint64_t* const* ws_rot) {
+    return guard_ptr<gf_plan>([&] {
+        if (m == 0) throw std::runtime_error("gpufhe: hoisted rotation needs at least one index");
+        std::vector<const DevKey*> keys(m);
+        for (uint32_t j = 0; j < m; ++j) keys[j] = rks[j]->K.get();
+        PlanBuilder b(*c->C);
+        for (uint32_t i = 0; i < count; ++i)
+            b.rotate_hoisted(levels[i], m, ks, c0[i], c1[i], o0 + (size_t)i * m, o1 + (size_t)i * m, keys.data(),
+                             ws_base[i], ws_rot + (size_t)i * m);
+        auto p = new gf_plan;
+        p->P = b.finish();
+        return p;
+    });
+}
 gf_plan* gf_plan_add(gf_ctx* c, uint32_t count, const uint32_t* levels, const uint64_t* const* a0,
                      const uint64_t* const* a1, const uint64_t* const* b0, const uint64_t* const* b1,
                      uint64_t* const* o0, uint64_t* const* o1) {

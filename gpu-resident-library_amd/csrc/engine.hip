@@ -367,14 +367,22 @@ void PlanBuilder::keyswitch_core(uint32_t l, const uint64_t* d, const DevKey& ke
                                  uint64_t* dx1) {
     const Ctx& C = C_;
     if (l < 1 || l > C.L) throw std::runtime_error("gpufhe: bad level");
+    if (key.nparts < C.lv[l].np) throw std::runtime_error("gpufhe: key has too few parts for this level");
+    modup(l, d, ws, extra0, extra1, dx0, dx1);
+    const uint64_t* dig = ws + WsLayout::make(C).dig * C.n;
+    inner_moddown_src(l, key, ws, [&](uint32_t pi, uint32_t i) { return digit_tower(l, d, dig, pi, i); });
+}
+
+// 1-3. ModUp of d into the workspace digits (eval form); extra towers (the
+// last tower of t0/t1 in the fused EvalMult+Rescale) ride along the INTT.
+void PlanBuilder::modup(uint32_t l, const uint64_t* d, uint64_t* ws, const uint64_t* extra0,
+                        const uint64_t* extra1, uint64_t* dx0, uint64_t* dx1) {
+    const Ctx& C = C_;
     const uint32_t n = C.n, K = C.K, W = l + K;
     const LevelConst& lv = C.lv[l];
-    if (key.nparts < lv.np) throw std::runtime_error("gpufhe: key has too few parts for this level");
     const WsLayout w = WsLayout::make(C);
     uint64_t* x = ws + w.x * n;
     uint64_t* dig = ws + w.dig * n;
-    uint64_t* res[2] = {ws + w.res0 * n, ws + w.res1 * n};
-    uint64_t* xp = ws + w.xp * n;
     // 1. part towers to coefficient form, scaled by n^-1 QHatInv (ModUp's first factor)
     for (uint32_t i = 0; i < l; ++i) {
         const uint32_t pi = i / C.alpha, ii = i - pi * C.alpha;
@@ -400,17 +408,37 @@ void PlanBuilder::keyswitch_core(uint32_t l, const uint64_t* d, const DevKey& ke
                                       C.mods[r], 0, 0});
         }
     }
-    // 4. inner product with the key over Q_l P (src/keyswitch_resident.cpp:171-200)
+}
+
+// Tower i (of Q_l P) of digit pi: the input's own tower inside the part,
+// else the part's converted complement (src/keyswitch_resident.cpp:191-194).
+const uint64_t* PlanBuilder::digit_tower(uint32_t l, const uint64_t* d, const uint64_t* dig, uint32_t pi,
+                                         uint32_t i) const {
+    const Ctx& C = C_;
+    const uint32_t n = C.n, W = l + C.K;
+    const PartConst& P = C.lv[l].parts[pi];
+    if (i >= P.st && i < P.st + P.sp) return d + (size_t)i * n;
+    return dig + ((size_t)pi * W + (i < P.st ? i : i - P.sp)) * n;
+}
+
+// 4. inner product with the key over Q_l P (src/keyswitch_resident.cpp:171-200)
+// into the workspace accumulators, 5. their P towers to coefficient form
+// (ModDown's sources).  digit(pi, i) names the digit towers.
+template <class F>
+void PlanBuilder::inner_moddown_src(uint32_t l, const DevKey& key, uint64_t* ws, F digit) {
+    const Ctx& C = C_;
+    const uint32_t n = C.n, K = C.K, W = l + K;
+    const LevelConst& lv = C.lv[l];
+    const WsLayout w = WsLayout::make(C);
+    uint64_t* res[2] = {ws + w.res0 * n, ws + w.res1 * n};
+    uint64_t* xp = ws + w.xp * n;
     for (uint32_t i = 0; i < W; ++i) {
         const uint32_t row = i < l ? C.row_q(i) : C.row_p(i - l);
         const uint32_t krow = i < l ? i : key.fullQ + (i - l);
         InnerJob J{};
         J.np = lv.np;
         for (uint32_t pi = 0; pi < lv.np; ++pi) {
-            const PartConst& P = lv.parts[pi];
-            const uint64_t* dp = dig + (size_t)pi * W * n;
-            if (i >= P.st && i < P.st + P.sp) J.dig[pi] = d + (size_t)i * n;
-            else J.dig[pi] = dp + (size_t)(i < P.st ? i : i - P.sp) * n;
+            J.dig[pi] = digit(pi, i);
             const size_t ko = ((size_t)pi * key.rows + krow) * n;
             J.kb[pi] = key.b.as<uint64_t>() + ko;
             J.ka[pi] = key.a.as<uint64_t>() + ko;
@@ -421,7 +449,6 @@ void PlanBuilder::keyswitch_core(uint32_t l, const uint64_t* d, const DevKey& ke
         J.qneg = C.rc[row].qneg;
         inner_.push_back(J);
     }
-    // 5. ModDown sources: P towers of both accumulators to coefficient form
     for (int c = 0; c < 2; ++c)
         for (uint32_t k = 0; k < K; ++k) {
             const uint32_t r = C.row_p(k);
@@ -433,8 +460,15 @@ void PlanBuilder::keyswitch_core(uint32_t l, const uint64_t* d, const DevKey& ke
 void PlanBuilder::keyswitch(uint32_t l, const uint64_t* d, uint64_t* ba0, uint64_t* ba1,
                             const DevKey& key, uint64_t* ws, const uint64_t* t0,
                             const uint64_t* t1) {
-    const Ctx& C = C_;
     keyswitch_core(l, d, key, ws, nullptr, nullptr, nullptr, nullptr);
+    moddown_tail(l, ws, ba0, ba1, t0, t1);
+}
+
+// 6-8. ApproxModDown of both workspace accumulators, fused with
+// out_c = t_c + ba_c (src/keyswitch_resident.cpp:203-224)
+void PlanBuilder::moddown_tail(uint32_t l, uint64_t* ws, uint64_t* ba0, uint64_t* ba1, const uint64_t* t0,
+                               const uint64_t* t1) {
+    const Ctx& C = C_;
     const uint32_t n = C.n, K = C.K;
     const LevelConst& lv = C.lv[l];
     const WsLayout w = WsLayout::make(C);
@@ -594,6 +628,37 @@ void PlanBuilder::rotate(uint32_t l, uint64_t k, const uint64_t* c0, const uint6
     keyswitch(l, a1, o0, o1, key, ws, a0, nullptr);
 }
 
+// m rotations of one ciphertext from one decomposition of c1: ModUp once into
+// ws_base; rotation j permutes every digit tower (and c0) into its own
+// workspace, then runs the inner product with key j and ModDown there.
+void PlanBuilder::rotate_hoisted(uint32_t l, uint32_t m, const uint64_t* ks, const uint64_t* c0,
+                                 const uint64_t* c1, uint64_t* const* o0, uint64_t* const* o1,
+                                 const DevKey* const* keys, uint64_t* ws_base, uint64_t* const* ws_rot) {
+    const Ctx& C = C_;
+    const uint32_t n = C.n, W = l + C.K;
+    if (l < 1 || l > C.L) throw std::runtime_error("gpufhe: bad level");
+    const LevelConst& lv = C.lv[l];
+    for (uint32_t j = 0; j < m; ++j) {
+        if (!(ks[j] & 1) || ks[j] >= 2ull * n)
+            throw std::runtime_error("gpufhe: automorphism index must be odd and < 2n");
+        if (keys[j]->nparts < lv.np) throw std::runtime_error("gpufhe: key has too few parts for this level");
+    }
+    const WsLayout w = WsLayout::make(C);
+    modup(l, c1, ws_base, nullptr, nullptr, nullptr, nullptr);
+    const uint64_t* dig = ws_base + w.dig * n;
+    for (uint32_t j = 0; j < m; ++j) {
+        uint64_t* ws = ws_rot[j];
+        uint64_t* a0 = ws + w.t0 * n;
+        uint64_t* pd = ws + w.dig * n;   // permuted digits, full layout [np][W]
+        for (uint32_t i = 0; i < l; ++i) perm_.push_back(PermJob{c0 + (size_t)i * n, a0 + (size_t)i * n, ks[j]});
+        for (uint32_t pi = 0; pi < lv.np; ++pi)
+            for (uint32_t i = 0; i < W; ++i)
+                perm_dig_.push_back(PermJob{digit_tower(l, c1, dig, pi, i), pd + ((size_t)pi * W + i) * n, ks[j]});
+        inner_moddown_src(l, *keys[j], ws, [&](uint32_t pi, uint32_t i) { return pd + ((size_t)pi * W + i) * n; });
+        moddown_tail(l, ws, o0[j], o1[j], a0, nullptr);
+    }
+}
+
 void PlanBuilder::add(uint32_t l, const uint64_t* a0, const uint64_t* a1, const uint64_t* b0,
                       const uint64_t* b1, uint64_t* o0, uint64_t* o1) {
     const uint32_t n = C_.n;
@@ -659,6 +724,7 @@ std::unique_ptr<Plan> PlanBuilder::layout(std::vector<char>& blob) {
     GF_STAGE(NTT_I, intt_up_);
     GF_STAGE(CONV, conv_up_);
     GF_STAGE(NTT_F, ntt_dig_);
+    GF_STAGE(PERM, perm_dig_);
     GF_STAGE(INNER, inner_);
     GF_STAGE(NTT_I, intt_p_);
     GF_STAGE(CONV, conv_dn_);

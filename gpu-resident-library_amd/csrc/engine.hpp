@@ -150,6 +150,11 @@ public:
                           uint64_t* ws);
     void rotate(uint32_t l, uint64_t k, const uint64_t* c0, const uint64_t* c1, uint64_t* o0,
                 uint64_t* o1, const DevKey& key, uint64_t* ws);
+    // m rotations of (c0, c1) sharing one digit decomposition (ws_base), one
+    // workspace per rotation
+    void rotate_hoisted(uint32_t l, uint32_t m, const uint64_t* ks, const uint64_t* c0, const uint64_t* c1,
+                        uint64_t* const* o0, uint64_t* const* o1, const DevKey* const* keys, uint64_t* ws_base,
+                        uint64_t* const* ws_rot);
     void add(uint32_t l, const uint64_t* a0, const uint64_t* a1, const uint64_t* b0,
              const uint64_t* b1, uint64_t* o0, uint64_t* o1);
     std::unique_ptr<Plan> finish();
@@ -186,6 +191,15 @@ private:
                                 const DevKey& key, uint64_t* ws);
     void keyswitch_core(uint32_t l, const uint64_t* d, const DevKey& key, uint64_t* ws,
                         const uint64_t* extra0, const uint64_t* extra1, uint64_t* dx0, uint64_t* dx1);
+    void modup(uint32_t l, const uint64_t* d, uint64_t* ws, const uint64_t* extra0, const uint64_t* extra1,
+               uint64_t* dx0, uint64_t* dx1);
+    const uint64_t* digit_tower(uint32_t l, const uint64_t* d, const uint64_t* dig, uint32_t pi,
+                                uint32_t i) const;
+    template <class F>
+    void inner_moddown_src(uint32_t l, const DevKey& key, uint64_t* ws, F digit);
+    void moddown_tail(uint32_t l, uint64_t* ws, uint64_t* ba0, uint64_t* ba1, const uint64_t* t0,
+                      const uint64_t* t1);
+    std::vector<PermJob> perm_dig_;
     std::vector<AddJob> add_;
     std::unique_ptr<Plan> layout(std::vector<char>& blob);
     void rescale_comp(uint32_t l, const uint64_t* c, uint64_t* o, uint64_t* drop, uint64_t* sw);

@@ -62,6 +62,7 @@ _capi = {
     "gf_plan_evalmult_rescale": (vp, [vp, vp, u32, u32p] + [vpp] * 7),
     "gf_plan_rotate": (vp, [vp, vpp, u32, u32p, u64p] + [vpp] * 5),
     "gf_plan_add": (vp, [vp, u32, u32p] + [vpp] * 6),
+    "gf_plan_rotate_hoisted": (vp, [vp, u32, u32p, vpp, vpp, u32, vpp, u64p, vpp, vpp, vpp, vpp]),
     "gf_plan_run": (C.c_int, [vp, vp]),
     "gf_plan_run_per_job": (C.c_int, [vp, vp]),
     "gf_plan_run_graph": (C.c_int, [vp, vp]),
@@ -316,6 +317,19 @@ def plan_rotate(ctx, keys, levels, ks, c0, c1, o0, o1, ws):
     arrs = [ptr_array(x) for x in (c0, c1, o0, o1, ws)]
     return Plan(lib.gf_plan_rotate(ctx.h, kh, cnt, lv.ctypes.data_as(u32p), kk.ctypes.data_as(u64p), *arrs),
                 keep=(lv, kk, kh, arrs, keys))
+
+
+def plan_rotate_hoisted(ctx, levels, c0, c1, keys, ks, o0, o1, ws_base, ws_rot):
+    """Each ciphertext i rotated by every ks[j]; o0/o1/ws_rot indexed [i*m + j]."""
+    m = len(ks)
+    lv = np.asarray(levels, np.uint32)
+    kk = np.asarray(ks, np.uint64)
+    kh = ptr_array([k.h for k in keys])
+    arrs = [ptr_array(x) for x in (c0, c1)]
+    outs = [ptr_array(x) for x in (o0, o1, ws_base, ws_rot)]
+    return Plan(lib.gf_plan_rotate_hoisted(ctx.h, len(levels), lv.ctypes.data_as(u32p), arrs[0], arrs[1], m, kh,
+                                           kk.ctypes.data_as(u64p), *outs),
+                keep=(lv, kk, kh, arrs, outs, keys))
 
 
 def plan_add(ctx, levels, a0, a1, b0, b1, o0, o1):

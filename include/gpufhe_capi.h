@@ -136,6 +136,16 @@ gf_plan* gf_plan_evalmult_rescale(gf_ctx* c, gf_key* rlk, uint32_t count, const 
 gf_plan* gf_plan_rotate(gf_ctx* c, gf_key* const* rks, uint32_t count, const uint32_t* h_levels,
                         const uint64_t* h_k, const uint64_t* const* c0, const uint64_t* const* c1,
                         uint64_t* const* o0, uint64_t* const* o1, uint64_t* const* ws);
+/* hoisted rotations: each of `count` ciphertexts rotated by all m indices
+ * h_k[0..m) (rotation keys rks[0..m)); the digit decomposition of c1 is done
+ * once per ciphertext in ws_base[i].  Outputs and per-rotation workspaces
+ * ws_rot are indexed [i*m + j].  Not in the reference (it decomposes
+ * sigma_k(c1) for every rotation, src/rotate.cpp:118-132): results decrypt to
+ * the same rotation with different (equally small) noise. */
+gf_plan* gf_plan_rotate_hoisted(gf_ctx* c, uint32_t count, const uint32_t* h_levels,
+                                const uint64_t* const* c0, const uint64_t* const* c1, uint32_t m,
+                                gf_key* const* rks, const uint64_t* h_k, uint64_t* const* o0,
+                                uint64_t* const* o1, uint64_t* const* ws_base, uint64_t* const* ws_rot);
 gf_plan* gf_plan_add(gf_ctx* c, uint32_t count, const uint32_t* h_levels,
                      const uint64_t* const* a0, const uint64_t* const* a1,
                      const uint64_t* const* b0, const uint64_t* const* b1, uint64_t* const* o0,

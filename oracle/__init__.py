@@ -111,6 +111,8 @@ _sig = {
     "or_automorph_eval": (C.c_int, [C.POINTER(OrParams), u64p, C.c_uint32, C.c_uint32, C.c_int]),
     "or_rotate": (C.c_int, [C.POINTER(OrParams), C.c_uint32, u64p, u64p, C.c_uint32, u64p, u64p,
                             u64p, u64p]),
+    "or_rotate_hoisted": (C.c_int, [C.POINTER(OrParams), C.c_uint32, u64p, u64p, C.c_uint32, u32p,
+                                    C.POINTER(u64p), C.POINTER(u64p), C.POINTER(u64p), C.POINTER(u64p)]),
     "or_keygen": (None, [C.c_uint32, u64p, u64p, C.c_uint32, C.c_uint64, C.c_double, C.c_uint64,
                          C.c_uint32, u64p, u64p, u64p]),
     "or_evalkeygen_sold": (None, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, u64p,
@@ -259,6 +261,21 @@ class Params:
         if rc:
             raise RuntimeError("or_rotate failed")
         return o0, o1
+
+    def rotate_hoisted(self, level, c0, c1, ks, keys):
+        """m rotations of one ciphertext sharing one decomposition of c1;
+        keys = [(av, bv), ...] one per automorphism index in ks."""
+        m, T = len(ks), level * self.n
+        outs = [(np.zeros(T, np.uint64), np.zeros(T, np.uint64)) for _ in range(m)]
+        avs = [U(a) for a, _ in keys]
+        bvs = [U(b) for _, b in keys]
+        arr = lambda xs: (u64p * m)(*[P(x) for x in xs])  # noqa: E731
+        kk = np.asarray(ks, np.uint32)
+        rc = lib.or_rotate_hoisted(self.h, level, P(U(c0)), P(U(c1)), m, P(kk), arr(avs), arr(bvs),
+                                   arr([o[0] for o in outs]), arr([o[1] for o in outs]))
+        if rc:
+            raise RuntimeError("or_rotate_hoisted failed")
+        return outs
 
 
 def keygen(n, mods, roots, ns=1, sigma=3.19, seed=101, hamming=0):

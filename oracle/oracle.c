@@ -575,14 +575,28 @@ int or_ks_digits(const or_params* P, uint32_t sizeQl, const uint64_t* d, uint64_
  * key rows (row = i for Q towers, i + (L - sizeQl) for P towers, :93-96,181),
  * ApproxModDown of both accumulators (:203-224).  av/bv: [dnum][L+K][n].
  */
+/* inner product of the digits with the key and ApproxModDown of both
+ * accumulators: the second half of or_keyswitch (:171-224 of the file above) */
+static int ks_from_digits(const or_params* P, uint32_t sizeQl, const uint64_t* digits,
+                          const uint64_t* av, const uint64_t* bv, uint64_t* ba0, uint64_t* ba1);
+
 int or_keyswitch(const or_params* P, uint32_t sizeQl, const uint64_t* d, const uint64_t* av,
                  const uint64_t* bv, uint64_t* ba0, uint64_t* ba1) {
     const uint32_t n = P->n, L = P->L, K = P->K, W = sizeQl + K;
     const uint32_t np = or_num_parts(P, sizeQl);
-    const uint32_t delta = L - sizeQl;
     if (sizeQl == 0 || sizeQl > L || np > P->dnum) return -1;
     uint64_t* digits = (uint64_t*)malloc((size_t)np * W * n * 8);
     if (or_ks_digits(P, sizeQl, d, digits)) { free(digits); return -1; }
+    int rc = ks_from_digits(P, sizeQl, digits, av, bv, ba0, ba1);
+    free(digits);
+    return rc;
+}
+
+static int ks_from_digits(const or_params* P, uint32_t sizeQl, const uint64_t* digits,
+                          const uint64_t* av, const uint64_t* bv, uint64_t* ba0, uint64_t* ba1) {
+    const uint32_t n = P->n, L = P->L, K = P->K, W = sizeQl + K;
+    const uint32_t np = or_num_parts(P, sizeQl);
+    const uint32_t delta = L - sizeQl;
     uint64_t* res0 = (uint64_t*)calloc((size_t)W * n, 8);
     uint64_t* res1 = (uint64_t*)calloc((size_t)W * n, 8);
     const size_t keyStride = (size_t)(L + K) * n;
@@ -593,7 +607,6 @@ int or_keyswitch(const or_params* P, uint32_t sizeQl, const uint64_t* d, const u
                        digits + ((size_t)p * W + i) * n, bv + p * keyStride + (size_t)row * n,
                        av + p * keyStride + (size_t)row * n, P->mod[row], n);
         }
-    free(digits);
     or_ksc* C = or_ks_constants(P->mod, sizeQl, P->mod + L, K, np, P->alpha);
     uint64_t* pw = (uint64_t*)malloc((size_t)K * n * 8);
     uint64_t* qsw = (uint64_t*)malloc((size_t)sizeQl * n * 8);
@@ -692,6 +705,43 @@ int or_automorph_eval(const or_params* P, uint64_t* v, uint32_t towers, uint32_t
     }
     free(tmp);
     return 0;
+}
+
+/*
+ * Hoisted rotations of one ciphertext by m automorphisms.  The digits of c1
+ * (or_ks_digits) are computed once; rotation j applies sigma_kj to every
+ * digit tower (over Q_l P) and to c0, then runs the inner product with
+ * rotation key j and ApproxModDown:  out_j = (sigma(c0) + ba0, ba1).
+ * The reference has no hoisting (it decomposes sigma_k(c1) for every rotation,
+ * src/rotate.cpp:118-132); because ModUp is approximate, hoisted and
+ * unhoisted results decrypt to the same message with different noise, and
+ * coincide exactly for k = 1.  av[j], bv[j]: [dnum][L+K][n].
+ */
+int or_rotate_hoisted(const or_params* P, uint32_t sizeQl, const uint64_t* c0, const uint64_t* c1,
+                      uint32_t m, const uint32_t* ks, const uint64_t* const* av,
+                      const uint64_t* const* bv, uint64_t* const* out0, uint64_t* const* out1) {
+    const uint32_t n = P->n, K = P->K, W = sizeQl + K;
+    const uint32_t np = or_num_parts(P, sizeQl);
+    if (sizeQl == 0 || sizeQl > P->L || np > P->dnum) return -1;
+    const size_t T = (size_t)sizeQl * n, D = (size_t)np * W * n;
+    uint64_t* digits = (uint64_t*)malloc(D * 8);
+    if (or_ks_digits(P, sizeQl, c1, digits)) { free(digits); return -1; }
+    uint64_t* pd = (uint64_t*)malloc(D * 8);
+    uint64_t* a0 = (uint64_t*)malloc(T * 8);
+    uint64_t* b0 = (uint64_t*)malloc(T * 8);
+    int rc = 0;
+    for (uint32_t j = 0; j < m && rc == 0; ++j) {
+        memcpy(pd, digits, D * 8);
+        for (uint32_t p = 0; p < np; ++p) or_automorph_eval(P, pd + (size_t)p * W * n, W, ks[j], (int)K);
+        memcpy(a0, c0, T * 8);
+        or_automorph_eval(P, a0, sizeQl, ks[j], 0);
+        rc = ks_from_digits(P, sizeQl, pd, av[j], bv[j], b0, out1[j]);
+        for (uint32_t t = 0; t < sizeQl && rc == 0; ++t)
+            or_add_into(a0 + (size_t)t * n, b0 + (size_t)t * n, P->mod[t], n);
+        if (rc == 0) memcpy(out0[j], a0, T * 8);
+    }
+    free(digits); free(pd); free(a0); free(b0);
+    return rc;
 }
 
 /* (sigma_k(c0) + ba0, ba1) with (ba0,ba1) = KS(sigma_k(c1)): src/rotate.cpp:118-132 */

@@ -29,6 +29,9 @@
  *                        digit-sum identity sum_p digit_p * gadget_p == d*P.
  *   or_evalmult_rescale  exact small-integer product recovery (maxerr 0).
  *   or_automorph_*       eval-domain permutation identity and slot rotation.
+ *   or_rotate_hoisted    equals or_rotate exactly for k = 1 (identity
+ *                        automorphism) at every level; rotated slots decode
+ *                        to the rotated message for k = 5^r.
  *   or_keygen, or_evalkeygen, or_encrypt, or_decrypt
  *                        RLWE decryption identities (noise bounds), enc/dec
  *                        round trip with maxerr 0.
@@ -142,6 +145,10 @@ int or_automorph_eval(const or_params* P, uint64_t* v, uint32_t towers, uint32_t
 int or_rotate(const or_params* P, uint32_t sizeQl, const uint64_t* c0, const uint64_t* c1,
               uint32_t k, const uint64_t* av, const uint64_t* bv, uint64_t* out0,
               uint64_t* out1);
+/* m rotations of one ciphertext sharing one digit decomposition of c1 */
+int or_rotate_hoisted(const or_params* P, uint32_t sizeQl, const uint64_t* c0, const uint64_t* c1,
+                      uint32_t m, const uint32_t* ks, const uint64_t* const* av,
+                      const uint64_t* const* bv, uint64_t* const* out0, uint64_t* const* out1);
 
 /* ---- keys, encryption (sampler described in oracle.c) ---- */
 void or_keygen(uint32_t n, const uint64_t* mods, const uint64_t* roots, uint32_t count,

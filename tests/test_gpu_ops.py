@@ -236,6 +236,77 @@ def test_rotate_plan_multi_keys(G):
         assert np.array_equal(do[i][0].download(), w0) and np.array_equal(do[i][1].download(), w1)
 
 
+@pytest.mark.parametrize("n,L,dnum", [(1 << 12, 6, 3), (1 << 16, 20, 4)])
+def test_rotate_hoisted_vs_oracle(G, n, L, dnum):
+    """Hoisted rotations (one decomposition per ciphertext, several indices)
+    bit-exact against the oracle's hoisted restatement, uneven levels."""
+    P = make_params(n, L, dnum)
+    ctx = G.Context(n, P.q, P.p, P.roots, dnum)
+    rng = np.random.default_rng(31)
+    m = 3
+    keys, hk = [], []
+    for _ in range(m):
+        av = np.concatenate([rng.integers(0, int(q), n, dtype=np.uint64) for _ in range(dnum) for q in P.mods])
+        bv = np.concatenate([rng.integers(0, int(q), n, dtype=np.uint64) for _ in range(dnum) for q in P.mods])
+        keys.append((av, bv))
+        hk.append(G.Key(ctx, av, bv, dnum))
+    ks = [pow(5, 1 << j, 2 * n) for j in range(m)]
+    levels = [L, L - 2] if n < (1 << 16) else [L - 7]
+    cts = [(rand_ct(P, lv, rng), rand_ct(P, lv, rng)) for lv in levels]
+    dc = [(G.Dev.of(a), G.Dev.of(b)) for a, b in cts]
+    do = [(G.Dev(lv * n), G.Dev(lv * n)) for lv in levels for _ in range(m)]
+    wb = [ctx.workspace() for _ in levels]
+    wr = [ctx.workspace() for _ in range(len(levels) * m)]
+    plan = G.plan_rotate_hoisted(ctx, levels, [x[0].ptr for x in dc], [x[1].ptr for x in dc], hk, ks,
+                                 [x[0].ptr for x in do], [x[1].ptr for x in do], [w.ptr for w in wb],
+                                 [w.ptr for w in wr])
+    s = G.Stream()
+    plan.run(s)
+    s.sync()
+    for i, lv in enumerate(levels):
+        want = P.rotate_hoisted(lv, cts[i][0], cts[i][1], ks, keys)
+        for j in range(m):
+            assert np.array_equal(do[i * m + j][0].download(), want[j][0]), (i, j)
+            assert np.array_equal(do[i * m + j][1].download(), want[j][1]), (i, j)
+
+
+def test_chain_plans_graphs(G):
+    """Multiply-rescale chain (tests/test_chain.cpp shape): a batch of
+    ciphertexts multiplied depth times, one plan per step over the batch,
+    replayed as graphs; every step equals the oracle's sequential result."""
+    n, L, dnum, batch, depth = 1 << 12, 6, 3, 3, 4
+    P, ctx, k, av, bv = setup(G, n, L, dnum)
+    rng = np.random.default_rng(21)
+    cur = [rand_ct(P, L, rng) for _ in range(2 * batch)]
+    cts = [[G.Dev.of(cur[2 * b]), G.Dev.of(cur[2 * b + 1])] for b in range(batch)]
+    chain = [cts]
+    opnds, host_ops = [], []
+    for d in range(depth):
+        lv = L - d
+        ho = [(rand_ct(P, lv, rng), rand_ct(P, lv, rng)) for _ in range(batch)]
+        host_ops.append(ho)
+        opnds.append([[G.Dev.of(a), G.Dev.of(b)] for a, b in ho])
+        chain.append([[G.Dev((lv - 1) * n), G.Dev((lv - 1) * n)] for _ in range(batch)])
+    ws = [ctx.workspace() for _ in range(batch)]
+    plans = []
+    for d in range(depth):
+        src, dst, op = chain[d], chain[d + 1], opnds[d]
+        plans.append(G.plan_evalmult_rescale(ctx, k, [L - d] * batch, [x[0].ptr for x in src], [x[1].ptr for x in src],
+                                             [x[0].ptr for x in op], [x[1].ptr for x in op],
+                                             [x[0].ptr for x in dst], [x[1].ptr for x in dst], [w.ptr for w in ws]))
+    s = G.Stream()
+    for rep in range(2):
+        for pl in plans:
+            pl.run_graph(s)
+        s.sync()
+        for b in range(batch):
+            c0, c1 = cur[2 * b], cur[2 * b + 1]
+            for d in range(depth):
+                c0, c1 = P.evalmult_rescale(L - d, c0, c1, host_ops[d][b][0], host_ops[d][b][1], av, bv)
+                assert np.array_equal(chain[d + 1][b][0].download(), c0), (rep, b, d)
+                assert np.array_equal(chain[d + 1][b][1].download(), c1), (rep, b, d)
+
+
 def test_decrypt_exact_product_cfg1_shape(G):
     """tests/test_evalmult_rescale.cpp bar on the GPU path: exact small-integer
     product after round(dec * q_l / D^2), n = 2^15, 60 + 3 x 50-bit Q, dnum 2."""

@@ -495,6 +495,41 @@ def test_rotation_slots():
         assert np.max(np.abs(zz - np.roll(z, -r))) < 1e-6
 
 
+def test_rotate_hoisted_identity_equals_keyswitch_path():
+    """k = 1 is the identity automorphism: hoisted and unhoisted rotation must
+    agree residue for residue at every level (pins the digit layout, the key
+    rows and the ModDown of the hoisted path)."""
+    P = make_params(1 << 10, 6, 3)
+    rng = np.random.default_rng(12)
+    n = P.n
+    av = np.concatenate([rng.integers(0, int(m), n, dtype=np.uint64) for _ in range(P.dnum) for m in P.mods])
+    bv = np.concatenate([rng.integers(0, int(m), n, dtype=np.uint64) for _ in range(P.dnum) for m in P.mods])
+    for level in (6, 5, 3, 1):
+        c0 = np.concatenate([rng.integers(0, int(q), n, dtype=np.uint64) for q in P.q[:level]])
+        c1 = np.concatenate([rng.integers(0, int(q), n, dtype=np.uint64) for q in P.q[:level]])
+        (h0, h1), = P.rotate_hoisted(level, c0, c1, [1], [(av, bv)])
+        w0, w1 = P.rotate(level, c0, c1, 1, av, bv)
+        assert np.array_equal(h0, w0) and np.array_equal(h1, w1), level
+
+
+def test_rotate_hoisted_slots():
+    """Several rotations of one ciphertext from one decomposition decrypt to
+    the rotated messages."""
+    P = make_params(1 << 11, 4, 2)
+    ks = KeySet(P)
+    n, level = P.n, 4
+    rng = np.random.default_rng(14)
+    z = rng.uniform(-1, 1, n // 2)
+    delta = float(1 << 45)
+    c0, c1 = O.encrypt(n, P.q, P.roots[:4], encode(z, n, delta), ks.pkA, ks.pkB, seed=606)
+    rs = (1, 2, 5)
+    kk = [pow(5, r, 2 * n) for r in rs]
+    outs = P.rotate_hoisted(level, c0, c1, kk, [ks.rotkey(k) for k in kk])
+    for r, (o0, o1) in zip(rs, outs):
+        dec = O.decrypt(n, int(P.q[0]), int(P.roots[0]), o0, o1, ks.sQ)
+        assert np.max(np.abs(decode(dec, n, delta).real - np.roll(z, -r))) < 1e-6, r
+
+
 # -------------------------------------------------------------- encode ----
 @needs_ref
 def test_fast_encode_matches_reference_encode():

@@ -186,7 +186,7 @@ Ctx::Ctx(uint32_t n_, const std::vector<uint64_t>& q, const std::vector<uint64_t
                 for (uint32_t k = 0; k < K; ++k)
                     tabRS[(size_t)k * (l - 1) + i] = host_mulmod(tab[(size_t)k * l + i], neg, qi);
                 const SwitchJob sj = make_switch(nullptr, nullptr, ql, qi);
-                const RsTarget t{qi, host_qneg(qi), sj.diff, sj.new_larger, s2, host_precon(s2, qi)};
+                const RsTarget t{qi, host_qneg(qi), sj.diff, sj.new_larger, s2, host_precon(s2, qi), sj.red};
                 const uint64_t* tw = (const uint64_t*)&t;
                 tg.insert(tg.end(), tw, tw + sizeof(RsTarget) / 8);
             }

@@ -65,6 +65,7 @@ struct SwitchJob {
     const uint64_t* in;
     uint64_t* out;
     uint64_t q_old, q_new, half, diff;
+    uint64_t red;   // floor(2^64 / q_new), for x mod q_new without a divide
     uint32_t new_larger;
 };
 
@@ -107,6 +108,7 @@ struct DropJob {
 // per surviving tower constants of the fused ModDown + rescale prologue
 struct RsTarget {
     uint64_t q, qneg, diff, new_larger, s2, s2p;
+    uint64_t red;   // floor(2^64 / q): Shoup reduction x mod q = shoup(x, 1, red, q)
 };
 // v_i = s2_i * switch(drop -> q_i) + REDC(sum_k xp[k] * tab[k*ntgt + i]) for i in [j0, j1);
 // tab folds -P^-1 s1 into PHatModq, so NTT(v_i) + t_i s1 + res_i P^-1 s1 is the
@@ -162,6 +164,7 @@ inline SwitchJob make_switch(const uint64_t* in, uint64_t* out, uint64_t q_old, 
     j.half = q_old >> 1;
     j.new_larger = q_new > q_old;
     j.diff = q_new > q_old ? q_new - q_old : (q_old - q_new) % q_new;
+    j.red = host_precon(1, q_new);
     return j;
 }
 

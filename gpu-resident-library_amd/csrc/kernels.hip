@@ -117,18 +117,62 @@ __global__ __launch_bounds__(PW_THREADS) void k_tensor(const TensorJob* __restri
     }
 }
 
+// Tables that every lane of a block reads at the same index (conversion
+// matrices, target moduli, job pointers) are read through the constant
+// address space, so they become scalar loads shared by the wave.
+typedef const __attribute__((address_space(4))) uint64_t* cu64p;
+__device__ __forceinline__ cu64p uni(const uint64_t* p) { return (cu64p)p; }
+
+// Calls f(integral_constant<NS>) for a source count 1..CONV_UNROLLED known
+// only at run time (uniform per block), so the source column lives in
+// registers and every multiply-accumulate is unrolled; larger counts take
+// f(integral_constant<0>), a loop that re-reads its sources.  (The register
+// budget of a kernel is that of its largest case, hence the cap.)
+enum { CONV_UNROLLED = 8 };
+template <class F>
+__device__ __forceinline__ void with_nsrc(uint32_t ns, F&& f) {
+    switch (ns) {
+#define GF_NS(k) case k: f(std::integral_constant<uint32_t, k>{}); break;
+        GF_NS(1) GF_NS(2) GF_NS(3) GF_NS(4) GF_NS(5) GF_NS(6) GF_NS(7) GF_NS(8)
+#undef GF_NS
+        default: f(std::integral_constant<uint32_t, 0>{}); break;
+    }
+}
+
+// sum_s x_s tab[s*stride + t] reduced mod q (Montgomery constants in tab);
+// NS = 0: runtime count, sources read from memory
+template <uint32_t NS>
+__device__ __forceinline__ U128 dot_src(const uint64_t* src, uint64_t n, uint32_t i, uint32_t ns, cu64p tab,
+                                        uint32_t stride, uint32_t t, const uint64_t* x) {
+    U128 acc{0, 0};
+    if constexpr (NS > 0) {
+#pragma unroll
+        for (uint32_t s = 0; s < NS; ++s) acc = add128(acc, mul128(x[s], tab[s * stride + t]));
+    } else {
+        for (uint32_t s = 0; s < ns; ++s) acc = add128(acc, mul128(src[(uint64_t)s * n + i], tab[s * stride + t]));
+    }
+    return acc;
+}
+
+template <uint32_t NS>
+__device__ __forceinline__ void conv_one(const ConvJob& j, uint32_t n, uint32_t i) {
+    uint64_t x[NS > 0 ? NS : 1];
+#pragma unroll
+    for (uint32_t s = 0; s < NS; ++s) x[s] = j.src[(uint64_t)s * n + i];
+    const cu64p tab = uni(j.tab), tq = uni(j.tq), tqn = uni(j.tqneg);
+    for (uint32_t t = j.j0; t < j.j1; ++t) {
+        const U128 acc = dot_src<NS>(j.src, n, i, j.nsrc, tab, j.ntgt, t, x);
+        j.dst[(uint64_t)t * n + i] = redc(acc, tq[t], tqn[t]);
+    }
+}
+
 __global__ __launch_bounds__(PW_THREADS) void k_conv(const ConvJob* __restrict__ jobs,
                                                      uint32_t off, uint32_t n) {
     const ConvJob j = jobs[off + blockIdx.y];
-    PW_FOR(n) {
-        uint64_t x[CONV_MAX_SRC];
-        for (uint32_t s = 0; s < j.nsrc; ++s) x[s] = j.src[(uint64_t)s * n + i];
-        for (uint32_t t = j.j0; t < j.j1; ++t) {
-            U128 acc{0, 0};
-            for (uint32_t s = 0; s < j.nsrc; ++s) acc = add128(acc, mul128(x[s], j.tab[s * j.ntgt + t]));
-            j.dst[(uint64_t)t * n + i] = redc(acc, j.tq[t], j.tqneg[t]);
-        }
-    }
+    with_nsrc(j.nsrc, [&](auto NSc) {
+        constexpr uint32_t NS = decltype(NSc)::value;
+        PW_FOR(n) { conv_one<NS>(j, n, i); }
+    });
 }
 
 __global__ __launch_bounds__(PW_THREADS) void k_inner(const InnerJob* __restrict__ jobs,
@@ -163,7 +207,7 @@ __global__ __launch_bounds__(PW_THREADS) void k_combine(const CombineJob* __rest
 __device__ __forceinline__ uint64_t switch_one(uint64_t x, const SwitchJob& j) {
     const uint64_t d = x > j.half ? j.diff : 0;
     if (j.new_larger) return x + d;
-    const uint64_t xm = x >= j.q_new ? x % j.q_new : x;
+    const uint64_t xm = shoup(x, 1, j.red, j.q_new);   // x mod q_new, exact
     return xm >= d ? xm - d : xm + j.q_new - d;
 }
 
@@ -184,39 +228,62 @@ __global__ __launch_bounds__(PW_THREADS) void k_fuse(const FuseJob* __restrict__
 __global__ __launch_bounds__(PW_THREADS) void k_drop(const DropJob* __restrict__ jobs,
                                                      uint32_t off, uint32_t n) {
     const DropJob j = jobs[off + blockIdx.y];
-    PW_FOR(n) {
-        U128 acc{0, 0};
-        for (uint32_t k = 0; k < j.K; ++k) acc = add128(acc, mul128(j.xp[(uint64_t)k * n + i], j.tab[k * j.stride]));
-        const uint64_t conv = redc(acc, j.q, j.qneg);
-        j.out[i] = addmod(j.dt[i], shoup(submod(j.dr[i], conv, j.q), j.pinv, j.pinvp, j.q), j.q);
+    const cu64p tab = uni(j.tab);
+    with_nsrc(j.K, [&](auto Kc) {
+        constexpr uint32_t K = decltype(Kc)::value;
+        PW_FOR(n) {
+            uint64_t x[K > 0 ? K : 1];
+#pragma unroll
+            for (uint32_t k = 0; k < K; ++k) x[k] = j.xp[(uint64_t)k * n + i];
+            const U128 acc = dot_src<K>(j.xp, n, i, j.K, tab, j.stride, 0, x);
+            const uint64_t conv = redc(acc, j.q, j.qneg);
+            j.out[i] = addmod(j.dt[i], shoup(submod(j.dr[i], conv, j.q), j.pinv, j.pinvp, j.q), j.q);
+        }
+    });
+}
+
+template <uint32_t K>
+__device__ __forceinline__ void rs_one(const RsJob& j, uint32_t n, uint32_t i) {
+    uint64_t x[K > 0 ? K : 1];
+#pragma unroll
+    for (uint32_t k = 0; k < K; ++k) x[k] = j.xp[(uint64_t)k * n + i];
+    const uint64_t dv = j.drop[i];
+    const bool hi = dv > j.half;
+    const cu64p tab = uni(j.tab);
+    static_assert(sizeof(RsTarget) == 7 * 8, "RsTarget layout");
+    for (uint32_t t = j.j0; t < j.j1; ++t) {
+        const cu64p w = uni((const uint64_t*)j.tg) + 7 * t;
+        RsTarget tg;
+        tg.q = w[0];
+        tg.qneg = w[1];
+        tg.diff = w[2];
+        tg.new_larger = w[3];
+        tg.s2 = w[4];
+        tg.s2p = w[5];
+        tg.red = w[6];
+        const U128 acc = dot_src<K>(j.xp, n, i, j.K, tab, j.ntgt, t, x);
+        const uint64_t conv = redc(acc, tg.q, tg.qneg);
+        // centred switch q_old -> q_t (kernels/switch_modulus.cu:11-27 semantics);
+        // dv mod q_t by a Shoup reduction with w = 1 (exact for any dv < 2^64)
+        uint64_t sw;
+        if (tg.new_larger) {
+            sw = dv + (hi ? tg.diff : 0);
+        } else {
+            const uint64_t d = hi ? tg.diff : 0;
+            const uint64_t xm = shoup(dv, 1, tg.red, tg.q);
+            sw = xm >= d ? xm - d : xm + tg.q - d;
+        }
+        j.out[(uint64_t)t * n + i] = addmod(shoup(sw, tg.s2, tg.s2p, tg.q), conv, tg.q);
     }
 }
 
 __global__ __launch_bounds__(PW_THREADS) void k_rs(const RsJob* __restrict__ jobs, uint32_t off,
                                                    uint32_t n) {
     const RsJob j = jobs[off + blockIdx.y];
-    PW_FOR(n) {
-        uint64_t x[CONV_MAX_SRC];
-        for (uint32_t k = 0; k < j.K; ++k) x[k] = j.xp[(uint64_t)k * n + i];
-        const uint64_t dv = j.drop[i];
-        const bool hi = dv > j.half;
-        for (uint32_t t = j.j0; t < j.j1; ++t) {
-            const RsTarget tg = j.tg[t];
-            U128 acc{0, 0};
-            for (uint32_t k = 0; k < j.K; ++k) acc = add128(acc, mul128(x[k], j.tab[k * j.ntgt + t]));
-            const uint64_t conv = redc(acc, tg.q, tg.qneg);
-            // centred switch q_old -> q_t (kernels/switch_modulus.cu:11-27 semantics)
-            uint64_t sw;
-            if (tg.new_larger) {
-                sw = dv + (hi ? tg.diff : 0);
-            } else {
-                const uint64_t d = hi ? tg.diff : 0;
-                const uint64_t xm = dv >= tg.q ? dv % tg.q : dv;
-                sw = xm >= d ? xm - d : xm + tg.q - d;
-            }
-            j.out[(uint64_t)t * n + i] = addmod(shoup(sw, tg.s2, tg.s2p, tg.q), conv, tg.q);
-        }
-    }
+    with_nsrc(j.K, [&](auto Kc) {
+        constexpr uint32_t K = decltype(Kc)::value;
+        PW_FOR(n) { rs_one<K>(j, n, i); }
+    });
 }
 
 __global__ __launch_bounds__(PW_THREADS) void k_add(const AddJob* __restrict__ jobs,

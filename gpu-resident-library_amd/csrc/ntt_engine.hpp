@@ -28,6 +28,21 @@
 
 namespace gf {
 
+// Device code reads towers and twiddle tables through global-address-space
+// pointers: global loads take a scalar base plus a 32-bit offset and are
+// counted apart from LDS traffic (generic pointers would compile to flat
+// accesses that do neither).  The host build of this header (the CPU
+// emulation) uses plain pointers.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define GF_GLOBAL __attribute__((address_space(1)))
+#else
+#define GF_GLOBAL
+#endif
+template <class T>
+GF_HD const GF_GLOBAL T* gcast(const T* p) { return (const GF_GLOBAL T*)p; }
+template <class T>
+GF_HD GF_GLOBAL T* gcast(T* p) { return (GF_GLOBAL T*)p; }
+
 struct NttJob {
     const uint64_t* in;   // tower read by the first phase
     uint64_t* out;        // tower written by every phase (may equal in)
@@ -82,24 +97,24 @@ inline NttGeom make_geom(uint32_t n, uint32_t rmax = 3) {
 
 // twiddle accessors: paired (w, wp) table -> one 16-byte load; or two arrays
 struct TwPaired {
-    const uint64_t* t;
-    GF_HD void get(uint64_t i, uint64_t& w, uint64_t& wp) const {
+    const GF_GLOBAL uint64_t* t;
+    GF_HD void get(uint32_t i, uint64_t& w, uint64_t& wp) const {
         w = t[2 * i];
         wp = t[2 * i + 1];
     }
 };
 struct TwSplit {
-    const uint64_t* w;
-    const uint64_t* p;
-    GF_HD void get(uint64_t i, uint64_t& ww, uint64_t& wp) const {
+    const GF_GLOBAL uint64_t* w;
+    const GF_GLOBAL uint64_t* p;
+    GF_HD void get(uint32_t i, uint64_t& ww, uint64_t& wp) const {
         ww = w[i];
         wp = p[i];
     }
 };
 template <bool PAIRED>
 GF_HD auto tw_of(const NttJob& j) {
-    if constexpr (PAIRED) return TwPaired{j.tw};
-    else return TwSplit{j.tw, j.twp};
+    if constexpr (PAIRED) return TwPaired{gcast(j.tw)};
+    else return TwSplit{gcast(j.tw), gcast(j.twp)};
 }
 
 // radix-2^r chunking of S stages: ceil(S/RM) chunks, the larger ones first
@@ -116,7 +131,7 @@ GF_HD constexpr uint32_t chunk_start(uint32_t S, uint32_t RM, uint32_t c) {
 // twiddle group base G(s) = g0 + s*gstep.
 struct TileDesc {
     uint32_t logT, KS, SS;
-    uint64_t g0;
+    uint32_t g0;   // twiddle indices stay below n <= 2^17
     uint32_t gstep;
     bool sub_fastest;   // unit order: consecutive threads on consecutive sub-transforms
     bool pad;           // element k of a sub-transform at padded(k) (KS must be 1)
@@ -155,12 +170,12 @@ GF_HD void tile_pass(uint64_t* a, const TileDesc& d, const TW& tw, uint64_t q, u
         uint64_t x[R8];
 #pragma unroll
         for (uint32_t u = 0; u < R8; ++u) x[u] = base[at[u]];
-        const uint64_t G = d.g0 + (uint64_t)s * d.gstep;
+        const uint32_t G = d.g0 + s * d.gstep;
         if (FWD) {
 #pragma unroll
             for (uint32_t c = 0; c < R; ++c) {
                 const uint32_t h = 1u << (R - 1 - c);
-                const uint64_t tb = (G << (ST0 + c)) + ((uint64_t)g << c);
+                const uint32_t tb = (G << (ST0 + c)) + (g << c);
 #pragma unroll
                 for (uint32_t u = 0; u < R8; ++u) {
                     if (u & h) continue;
@@ -176,7 +191,7 @@ GF_HD void tile_pass(uint64_t* a, const TileDesc& d, const TW& tw, uint64_t q, u
 #pragma unroll
             for (int c = (int)R - 1; c >= 0; --c) {
                 const uint32_t h = 1u << (R - 1 - c);
-                const uint64_t tb = (G << (ST0 + c)) + ((uint64_t)g << c);
+                const uint32_t tb = (G << (ST0 + c)) + (g << c);
 #pragma unroll
                 for (uint32_t u = 0; u < R8; ++u) {
                     if (u & h) continue;
@@ -215,8 +230,8 @@ GF_HD uint64_t ntt_final(uint64_t v, const NttJob& j) {
 GF_HD uint64_t ntt_final_ep(uint64_t v, const NttJob& j, uint64_t gi) {
     uint64_t r = csub(csub(v, 2 * j.q), j.q);
     if (j.e0) {
-        r = addmod(r, shoup(j.e0[gi], j.c0, j.c0p, j.q), j.q);
-        r = addmod(r, shoup(j.e1[gi], j.c1, j.c1p, j.q), j.q);
+        r = addmod(r, shoup(gcast(j.e0)[gi], j.c0, j.c0p, j.q), j.q);
+        r = addmod(r, shoup(gcast(j.e1)[gi], j.c1, j.c1p, j.q), j.q);
     }
     return r;
 }
@@ -229,7 +244,8 @@ GF_HD void phaseA_body(const NttJob& j, const NttGeom& g, uint32_t tile, uint64_
     constexpr uint32_t N1 = 1u << S1, logC = NTT_LOG_TILE - S1, C = 1u << logC;
     const uint32_t logN2 = g.logN2;
     const uint32_t b0 = tile * C;
-    const uint64_t* src = FWD ? j.in : j.out;   // forward: first phase; inverse: second
+    const GF_GLOBAL uint64_t* src = gcast(FWD ? j.in : j.out);   // forward: first phase; inverse: second
+    GF_GLOBAL uint64_t* dst = gcast(j.out);
     for (uint32_t idx = tid; idx < C * N1; idx += nthr) {
         const uint32_t row = idx >> logC, c = idx & (C - 1);
         lds[idx] = src[((uint64_t)row << logN2) + b0 + c];
@@ -241,7 +257,7 @@ GF_HD void phaseA_body(const NttJob& j, const NttGeom& g, uint32_t tile, uint64_
         const uint32_t row = idx >> logC, c = idx & (C - 1);
         const uint64_t v = lds[idx];
         // forward phase A hands lazy values in [0,4q) to phase B
-        j.out[((uint64_t)row << logN2) + b0 + c] = FWD ? v : ntt_final<false>(v, j);
+        dst[((uint64_t)row << logN2) + b0 + c] = FWD ? v : ntt_final<false>(v, j);
     }
 }
 
@@ -252,27 +268,30 @@ GF_HD void phaseB_body(const NttJob& j, const NttGeom& g, uint32_t tile, uint64_
     constexpr uint32_t N2 = 1u << S2, logR = NTT_LOG_TILE - S2, R = 1u << logR;
     const uint32_t N1 = 1u << g.logN1;
     const uint32_t a0 = tile * R;
-    const uint64_t* src = FWD ? j.out : j.in;   // forward: second phase; inverse: first
+    const GF_GLOBAL uint64_t* src = gcast(FWD ? j.out : j.in);   // forward: second phase; inverse: first
+    GF_GLOBAL uint64_t* dst = gcast(j.out);
     const uint64_t off = (uint64_t)a0 << S2;
     for (uint32_t idx = tid; idx < R * N2; idx += nthr) lds[padded(idx)] = src[off + idx];
     sync();
-    const TileDesc d{logR, 1, padded(N2), (uint64_t)N1 + a0, 1, false, true};
+    const TileDesc d{logR, 1, padded(N2), N1 + a0, 1, false, true};
     tile_chunks<FWD, S2, RM, 0>(lds, d, tw_of<PAIRED>(j), j.q, tid, nthr, sync);
     // inverse phase B hands lazy values in [0,2q) to phase A
     for (uint32_t idx = tid; idx < R * N2; idx += nthr)
-        j.out[off + idx] = FWD ? ntt_final_ep(lds[padded(idx)], j, off + idx) : lds[padded(idx)];
+        dst[off + idx] = FWD ? ntt_final_ep(lds[padded(idx)], j, off + idx) : lds[padded(idx)];
 }
 
 // Single phase: the whole tower (n = 2^S <= NTT_SINGLE_MAX) is one tile.
 template <bool FWD, bool PAIRED, uint32_t S, uint32_t RM, class Sync>
 GF_HD void single_body(const NttJob& j, uint64_t* lds, uint32_t tid, uint32_t nthr, Sync sync) {
     constexpr uint32_t n = 1u << S;
-    for (uint32_t idx = tid; idx < n; idx += nthr) lds[padded(idx)] = j.in[idx];
+    const GF_GLOBAL uint64_t* src = gcast(j.in);
+    GF_GLOBAL uint64_t* dst = gcast(j.out);
+    for (uint32_t idx = tid; idx < n; idx += nthr) lds[padded(idx)] = src[idx];
     sync();
     const TileDesc d{0, 1, padded(n), 1, 0, false, true};
     tile_chunks<FWD, S, RM, 0>(lds, d, tw_of<PAIRED>(j), j.q, tid, nthr, sync);
     for (uint32_t idx = tid; idx < n; idx += nthr)
-        j.out[idx] = FWD ? ntt_final_ep(lds[padded(idx)], j, idx) : ntt_final<false>(lds[padded(idx)], j);
+        dst[idx] = FWD ? ntt_final_ep(lds[padded(idx)], j, idx) : ntt_final<false>(lds[padded(idx)], j);
 }
 
 // Calls f(integral_constant S1, integral_constant S2) for the phase split of

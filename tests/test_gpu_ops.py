@@ -11,6 +11,9 @@ pytestmark = pytest.mark.gpu
 
 # (n, L, dnum): small ring, two-phase NTT ring, the BASELINE config shapes
 SHAPES = [(1 << 10, 4, 2), (1 << 13, 6, 3), (1 << 15, 4, 2), (1 << 15, 12, 3), (1 << 16, 24, 4)]
+# edge shapes: one digit (alpha = L), one tower per digit (alpha = 1), the
+# largest ring the transforms support, the smallest ring
+EDGE_SHAPES = [(1 << 12, 4, 1), (1 << 12, 6, 6), (1 << 17, 3, 1), (1 << 4, 3, 3)]
 
 
 @pytest.fixture(scope="module")
@@ -64,6 +67,36 @@ def test_keyswitch(G, n, L, dnum):
         w0, w1 = P.keyswitch(level, d, av, bv)
         assert np.array_equal(b0.download(), w0), ("ba0", level)
         assert np.array_equal(b1.download(), w1), ("ba1", level)
+
+
+@pytest.mark.parametrize("n,L,dnum", EDGE_SHAPES)
+def test_edge_shapes_keyswitch_evalmult_rotate(G, n, L, dnum):
+    P, ctx, k, av, bv = setup(G, n, L, dnum)
+    rng = np.random.default_rng(404)
+    s = G.Stream()
+    ws = ctx.workspace()
+    for level in sorted({L, 2, 1}):
+        d = rand_ct(P, level, rng)
+        dd, b0, b1 = G.Dev.of(d), G.Dev(level * n), G.Dev(level * n)
+        G.check(G.lib.gf_keyswitch(ctx.h, k.h, level, dd.ptr, b0.ptr, b1.ptr, ws.ptr, s.s))
+        s.sync()
+        w0, w1 = P.keyswitch(level, d, av, bv)
+        assert np.array_equal(b0.download(), w0) and np.array_equal(b1.download(), w1), ("ks", level)
+        c1 = rand_ct(P, level, rng)
+        d1 = G.Dev.of(c1)
+        kk = 5 % (2 * n) if n > 2 else 1
+        G.check(G.lib.gf_rotate(ctx.h, k.h, level, kk, dd.ptr, d1.ptr, b0.ptr, b1.ptr, ws.ptr, s.s))
+        s.sync()
+        w0, w1 = P.rotate(level, d, c1, kk, av, bv)
+        assert np.array_equal(b0.download(), w0) and np.array_equal(b1.download(), w1), ("rot", level)
+        if level >= 2:
+            cts = [rand_ct(P, level, rng) for _ in range(4)]
+            dc = [G.Dev.of(c) for c in cts]
+            o0, o1 = G.Dev((level - 1) * n), G.Dev((level - 1) * n)
+            G.check(G.lib.gf_evalmult_rescale(ctx.h, k.h, level, *[x.ptr for x in dc], o0.ptr, o1.ptr, ws.ptr, s.s))
+            s.sync()
+            w0, w1 = P.evalmult_rescale(level, *cts, av, bv)
+            assert np.array_equal(o0.download(), w0) and np.array_equal(o1.download(), w1), ("emr", level)
 
 
 @pytest.mark.parametrize("n,L,dnum", SHAPES)

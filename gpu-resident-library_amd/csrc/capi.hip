@@ -131,6 +131,24 @@ gf_key* gf_key_create_rows(gf_ctx* c, const uint64_t* av, const uint64_t* bv, ui
 }
 uint32_t gf_ctx_towers(const gf_ctx* c) { return c->C->L; }
 
+gf_key* gf_key_generate(gf_ctx* c, const uint64_t* d_sold, const uint64_t* d_pka, const uint64_t* d_pkb,
+                        uint64_t ns, double sigma, uint64_t seed, hipStream_t s) {
+    return guard_ptr<gf_key>([&] {
+        if (!(sigma > 0.0) || sigma > 20.0) throw std::runtime_error("gpufhe: sigma must be in (0, 20]");
+        auto k = new gf_key;
+        k->ctx = c;
+        k->K = generate_key(*c->C, d_sold, d_pka, d_pkb, ns, sigma, seed, s);
+        return k;
+    });
+}
+int gf_key_download(const gf_key* k, uint64_t* h_av, uint64_t* h_bv, hipStream_t s) {
+    return guard([&] { download_key(*k->ctx->C, *k->K, h_av, h_bv, s); });
+}
+void gf_key_shape(const gf_key* k, uint32_t* nparts, uint32_t* rows, uint32_t* fullQ) {
+    *nparts = k->K->nparts;
+    *rows = k->K->rows;
+    *fullQ = k->K->fullQ;
+}
 gf_key* gf_key_create_uniform(gf_ctx* c, uint32_t nparts, uint64_t seed, hipStream_t s) {
     return guard_ptr<gf_key>([&] {
         const Ctx& C = *c->C;

@@ -7,6 +7,7 @@
 #include <tuple>
 
 #include "engine.hpp"
+#include "sampler.hpp"
 
 namespace gf {
 
@@ -296,6 +297,111 @@ std::unique_ptr<DevKey> upload_key(const Ctx& C, const uint64_t* av, const uint6
     launch_fuse(dj.as<FuseJob>(), (uint32_t)jobs.size(), C.n, s);
     hip_check(hipStreamSynchronize(s), "key conversion");
     return k;
+}
+
+// Switching key generated on the device (src/evalkeygen.cpp:42-96): the
+// small polynomials u, e0, e1 of every part are drawn on the host in the
+// reference's order, embedded into every Q and P tower, transformed and
+// combined with the public key and sOld on the device, and stored in
+// Montgomery form.  sold, pka, pkb: device arrays [L+K][n], evaluation form.
+std::unique_ptr<DevKey> generate_key(const Ctx& C, const uint64_t* sold, const uint64_t* pka, const uint64_t* pkb,
+                                     uint64_t ns, double sigma, uint64_t seed, hipStream_t s) {
+    const uint32_t n = C.n, W = C.L + C.K, parts = C.dnum;
+    std::vector<int64_t> u, e0, e1;
+    sample_switch_key(n, parts, sigma, seed, u, e0, e1);
+    auto k = std::make_unique<DevKey>();
+    k->nparts = parts;
+    k->rows = W;
+    k->fullQ = C.L;
+    const size_t per = (size_t)W * n;
+    k->a.alloc((size_t)parts * per * 8);
+    k->b.alloc((size_t)parts * per * 8);
+    // small polynomials (3 per part), then their towers [part][3][W][n]
+    DevBuf coef((size_t)parts * 3 * n * 8), tow((size_t)parts * 3 * per * 8);
+    std::vector<int64_t> hc((size_t)parts * 3 * n);
+    for (uint32_t p = 0; p < parts; ++p) {
+        std::copy(u.begin() + (size_t)p * n, u.begin() + (size_t)(p + 1) * n, hc.begin() + (size_t)(3 * p) * n);
+        std::copy(e0.begin() + (size_t)p * n, e0.begin() + (size_t)(p + 1) * n, hc.begin() + (size_t)(3 * p + 1) * n);
+        std::copy(e1.begin() + (size_t)p * n, e1.begin() + (size_t)(p + 1) * n, hc.begin() + (size_t)(3 * p + 2) * n);
+    }
+    hip_check(hipMemcpyAsync(coef.p, hc.data(), coef.bytes, hipMemcpyHostToDevice, s), "upload samples");
+    std::vector<EmbedJob> ej;
+    std::vector<NttJob> nj;
+    std::vector<EkJob> kj;
+    const int64_t* cb = coef.as<int64_t>();
+    uint64_t* tb = tow.as<uint64_t>();
+    for (uint32_t p = 0; p < parts; ++p) {
+        const uint32_t st = C.alpha * p, en = std::min(C.L, st + C.alpha);
+        for (uint32_t v = 0; v < 3; ++v)
+            for (uint32_t t = 0; t < W; ++t) {
+                uint64_t* x = tb + ((size_t)(3 * p + v) * W + t) * n;
+                ej.push_back(EmbedJob{cb + (size_t)(3 * p + v) * n, x, C.mods[t]});
+                nj.push_back(NttJob{x, x, C.rt[t].fw2, nullptr, C.mods[t], 0, 0});
+            }
+        for (uint32_t t = 0; t < W; ++t) {
+            const uint64_t q = C.mods[t];
+            uint64_t pq = 1 % q;
+            for (uint32_t j = 0; j < C.K; ++j) pq = host_mulmod(pq, C.mods[C.L + j] % q, q);
+            const uint64_t nsq = ns % q;
+            const size_t o = (size_t)t * n;
+            EkJob J{};
+            J.pka = pka + o;
+            J.pkb = pkb + o;
+            J.u = tb + ((size_t)(3 * p) * W + t) * n;
+            J.e0 = tb + ((size_t)(3 * p + 1) * W + t) * n;
+            J.e1 = tb + ((size_t)(3 * p + 2) * W + t) * n;
+            J.sold = (t >= st && t < en) ? sold + o : nullptr;
+            J.a_out = k->a.as<uint64_t>() + (size_t)p * per + o;
+            J.b_out = k->b.as<uint64_t>() + (size_t)p * per + o;
+            J.q = q;
+            J.qneg = C.rc[t].qneg;
+            J.r2 = C.rc[t].r2;
+            J.ns = nsq;
+            J.nsp = host_precon(nsq, q);
+            J.pq = pq;
+            J.pqp = host_precon(pq, q);
+            kj.push_back(J);
+        }
+    }
+    DevBuf dej(ej.size() * sizeof(EmbedJob)), dnj(nj.size() * sizeof(NttJob)), dkj(kj.size() * sizeof(EkJob));
+    hip_check(hipMemcpyAsync(dej.p, ej.data(), dej.bytes, hipMemcpyHostToDevice, s), "upload jobs");
+    hip_check(hipMemcpyAsync(dnj.p, nj.data(), dnj.bytes, hipMemcpyHostToDevice, s), "upload jobs");
+    hip_check(hipMemcpyAsync(dkj.p, kj.data(), dkj.bytes, hipMemcpyHostToDevice, s), "upload jobs");
+    launch_embed(dej.as<EmbedJob>(), (uint32_t)ej.size(), n, s);
+    launch_ntt(dnj.as<NttJob>(), (uint32_t)nj.size(), C.geom, true, s);
+    launch_ekgen(dkj.as<EkJob>(), (uint32_t)kj.size(), n, s);
+    hip_check(hipGetLastError(), "key generation launch");
+    hip_check(hipStreamSynchronize(s), "key generation");
+    return k;
+}
+
+// the key's rows back in plain form (x R^-1 mod q), [nparts][rows][n] on the host
+void download_key(const Ctx& C, const DevKey& k, uint64_t* av, uint64_t* bv, hipStream_t s) {
+    const uint32_t n = C.n;
+    const size_t words = (size_t)k.nparts * k.rows * n;
+    DevBuf ta(words * 8), tbuf(words * 8);
+    std::vector<FuseJob> jobs;
+    for (uint32_t p = 0; p < k.nparts; ++p)
+        for (uint32_t r = 0; r < k.rows; ++r) {
+            uint32_t row;
+            if (r < C.L) row = r;
+            else if (r >= k.fullQ) row = C.L + (r - k.fullQ);
+            else continue;   // rows this context never reads stay as uploaded
+            const uint64_t q = C.mods[row], rinv = host_invmod(C.rc[row].r1, q);
+            const size_t o = ((size_t)p * k.rows + r) * n;
+            jobs.push_back(FuseJob{k.a.as<uint64_t>() + o, k.a.as<uint64_t>() + o, ta.as<uint64_t>() + o, q, rinv,
+                                   host_precon(rinv, q), 0, 0});
+            jobs.push_back(FuseJob{k.b.as<uint64_t>() + o, k.b.as<uint64_t>() + o, tbuf.as<uint64_t>() + o, q, rinv,
+                                   host_precon(rinv, q), 0, 0});
+        }
+    hip_check(hipMemcpyAsync(ta.p, k.a.p, ta.bytes, hipMemcpyDeviceToDevice, s), "copy key");
+    hip_check(hipMemcpyAsync(tbuf.p, k.b.p, tbuf.bytes, hipMemcpyDeviceToDevice, s), "copy key");
+    DevBuf dj(jobs.size() * sizeof(FuseJob));
+    hip_check(hipMemcpyAsync(dj.p, jobs.data(), dj.bytes, hipMemcpyHostToDevice, s), "upload jobs");
+    launch_fuse(dj.as<FuseJob>(), (uint32_t)jobs.size(), n, s);
+    hip_check(hipMemcpyAsync(av, ta.p, ta.bytes, hipMemcpyDeviceToHost, s), "download key");
+    hip_check(hipMemcpyAsync(bv, tbuf.p, tbuf.bytes, hipMemcpyDeviceToHost, s), "download key");
+    hip_check(hipStreamSynchronize(s), "download key");
 }
 
 // ------------------------------------------------------------------ Plan --

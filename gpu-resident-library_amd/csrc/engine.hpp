@@ -103,6 +103,9 @@ struct DevKey {
 std::unique_ptr<DevKey> upload_key(const Ctx& C, const uint64_t* av, const uint64_t* bv,
                                    uint32_t nparts, uint32_t rows, uint32_t fullQ,
                                    hipStream_t s);
+std::unique_ptr<DevKey> generate_key(const Ctx& C, const uint64_t* sold, const uint64_t* pka, const uint64_t* pkb,
+                                     uint64_t ns, double sigma, uint64_t seed, hipStream_t s);
+void download_key(const Ctx& C, const DevKey& k, uint64_t* av, uint64_t* bv, hipStream_t s);
 
 // One launch step of a plan
 struct Step {

@@ -123,6 +123,21 @@ struct RsJob {
     uint32_t K, ntgt, j0, j1;
 };
 
+// one tower of a freshly sampled small polynomial: dst = src mod q (signed src)
+struct EmbedJob {
+    const int64_t* src;
+    uint64_t* dst;
+    uint64_t q;
+};
+// switching-key rows of one part and one tower (src/evalkeygen.cpp:42-96):
+// a = pkA u + ns e1, b = pkB u + ns e0 (+ (P mod q) sOld on the part's own Q
+// towers, sold null elsewhere), written in Montgomery form (x R mod q)
+struct EkJob {
+    const uint64_t *pka, *pkb, *u, *e0, *e1, *sold;
+    uint64_t *a_out, *b_out;
+    uint64_t q, qneg, r2, ns, nsp, pq, pqp;
+};
+
 // copy one tower
 struct CopyJob {
     const uint64_t* in;
@@ -154,6 +169,8 @@ void launch_fill(uint64_t* d, const uint64_t* d_mods, uint32_t towers, uint32_t 
                  hipStream_t s);
 void launch_drop(const DropJob* d, uint32_t nj, uint32_t n, hipStream_t s);
 void launch_rs(const RsJob* d, uint32_t nj, uint32_t n, hipStream_t s);
+void launch_embed(const EmbedJob* d, uint32_t nj, uint32_t n, hipStream_t s);
+void launch_ekgen(const EkJob* d, uint32_t nj, uint32_t n, hipStream_t s);
 
 inline SwitchJob make_switch(const uint64_t* in, uint64_t* out, uint64_t q_old, uint64_t q_new) {
     SwitchJob j;

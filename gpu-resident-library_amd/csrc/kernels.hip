@@ -286,6 +286,31 @@ __global__ __launch_bounds__(PW_THREADS) void k_rs(const RsJob* __restrict__ job
     });
 }
 
+__global__ __launch_bounds__(PW_THREADS) void k_embed(const EmbedJob* __restrict__ jobs,
+                                                      uint32_t off, uint32_t n) {
+    const EmbedJob j = jobs[off + blockIdx.y];
+    PW_FOR(n) {
+        const int64_t v = j.src[i];
+        const uint64_t m = (uint64_t)(v < 0 ? -v : v);
+        const uint64_t r = m < j.q ? m : m % j.q;   // |v| is small: the remainder is not taken in practice
+        j.dst[i] = (v < 0 && r) ? j.q - r : r;
+    }
+}
+
+__global__ __launch_bounds__(PW_THREADS) void k_ekgen(const EkJob* __restrict__ jobs,
+                                                      uint32_t off, uint32_t n) {
+    const EkJob j = jobs[off + blockIdx.y];
+    PW_FOR(n) {
+        const uint64_t u = j.u[i];
+        uint64_t a = addmod(mulmod(j.pka[i], u, j.q, j.qneg, j.r2), shoup(j.e1[i], j.ns, j.nsp, j.q), j.q);
+        uint64_t b = addmod(mulmod(j.pkb[i], u, j.q, j.qneg, j.r2), shoup(j.e0[i], j.ns, j.nsp, j.q), j.q);
+        if (j.sold) b = addmod(b, shoup(j.sold[i], j.pq, j.pqp, j.q), j.q);
+        // Montgomery form: REDC(x R^2) = x R
+        j.a_out[i] = redc(mul128(a, j.r2), j.q, j.qneg);
+        j.b_out[i] = redc(mul128(b, j.r2), j.q, j.qneg);
+    }
+}
+
 __global__ __launch_bounds__(PW_THREADS) void k_add(const AddJob* __restrict__ jobs,
                                                     uint32_t off, uint32_t n) {
     const AddJob j = jobs[off + blockIdx.y];
@@ -356,6 +381,8 @@ GF_PW_LAUNCH(launch_add, k_add, AddJob)
 GF_PW_LAUNCH(launch_copy, k_copy, CopyJob)
 GF_PW_LAUNCH(launch_drop, k_drop, DropJob)
 GF_PW_LAUNCH(launch_rs, k_rs, RsJob)
+GF_PW_LAUNCH(launch_embed, k_embed, EmbedJob)
+GF_PW_LAUNCH(launch_ekgen, k_ekgen, EkJob)
 
 void launch_fill(uint64_t* d, const uint64_t* d_mods, uint32_t towers, uint32_t n, uint64_t seed,
                  hipStream_t s) {

@@ -4,7 +4,7 @@
 // reference (src/keygen.cpp:55-112, src/evalkeygen.cpp:42-96,
 // src/encdec.cpp:41-64).  The reference rounds std::normal_distribution
 // draws, whose values are implementation-defined; the Gaussian here is a
-// fixed cumulative-table sampler (gauss_sample below), so keys and
+// fixed cumulative-table sampler (GaussSampler, sampler.hpp), so keys and
 // ciphertexts are reproducible on every platform.
 //
 // Transforms run on the device (gf_ntt_towers); the pointwise key algebra is
@@ -18,6 +18,7 @@
 
 #include "../../include/gpufhe.hpp"
 #include "../../include/gpufhe_capi.h"
+#include "sampler.hpp"
 
 namespace gpufhe {
 
@@ -39,40 +40,7 @@ uint64_t embed(int64_t v, uint64_t q) {
     return r < 0 ? (uint64_t)(r + (int64_t)q) : (uint64_t)r;
 }
 
-// Discrete Gaussian of standard deviation sigma on [-12 sigma, 12 sigma]:
-// weights w_0 = 1, w_k = 2 exp(-k^2 / (2 sigma^2)); thresholds
-// T[k] = (uint64)(cumulative_k / total * 2^63); draw r = rng() >> 1, take the
-// first k with r < T[k] (the last index if none), then a sign bit rng() & 1
-// when k > 0.
-class Gauss {
-public:
-    explicit Gauss(double sigma) {
-        K_ = (int)std::ceil(sigma * 12.0);
-        if (K_ > 256) K_ = 256;
-        std::vector<double> w(K_ + 1);
-        double total = 0.0, cum = 0.0;
-        for (int k = 0; k <= K_; ++k) {
-            w[k] = k == 0 ? 1.0 : 2.0 * std::exp(-(double)k * (double)k / (2.0 * sigma * sigma));
-            total += w[k];
-        }
-        T_.resize(K_ + 1);
-        for (int k = 0; k <= K_; ++k) {
-            cum += w[k];
-            T_[k] = (uint64_t)std::ldexp(cum / total, 63);
-        }
-    }
-    int64_t operator()(std::mt19937_64& g) const {
-        const uint64_t r = g() >> 1;
-        int k = 0;
-        while (k < K_ && r >= T_[k]) ++k;
-        if (k == 0) return 0;
-        return (g() & 1) ? -(int64_t)k : (int64_t)k;
-    }
-
-private:
-    int K_ = 0;
-    std::vector<uint64_t> T_;
-};
+using Gauss = gf::GaussSampler;
 
 // in-place transforms of host towers on the device: v is towers x n
 void xform(std::vector<uint64_t>& v, uint32_t towers, uint32_t n, const uint64_t* mods, const uint64_t* roots,

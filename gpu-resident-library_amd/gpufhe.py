@@ -49,6 +49,16 @@ _capi = {
     "gf_ctx_towers": (u32, [vp]),
     "gf_key_create_rows": (vp, [vp, u64p, u64p, u32, u32, u32, vp]),
     "gf_key_create_uniform": (vp, [vp, u32, u64, vp]),
+    "gf_key_generate": (vp, [vp, vp, vp, vp, u64, C.c_double, u64, vp]),
+    "gf_key_download": (C.c_int, [vp, u64p, u64p, vp]),
+    "gf_key_shape": (None, [vp, u32p, u32p, u32p]),
+    "gf_ct_packed_bytes": (sz, [u32, u32]),
+    "gf_ct_pack": (C.c_int, [u64p, u64p, u32, u32, u64p, C.c_double, vp, sz]),
+    "gf_ct_unpack_header": (C.c_int, [vp, sz, u32p, u32p, C.POINTER(C.c_double)]),
+    "gf_ct_unpack": (C.c_int, [vp, sz, u64p, u64p, u64p]),
+    "gf_key_packed_bytes": (sz, [vp]),
+    "gf_key_pack": (C.c_int, [vp, vp, sz, vp]),
+    "gf_key_unpack": (vp, [vp, vp, sz, vp]),
     "gf_root_tables": (C.c_int, [u32, u64, u64, vpp, vpp, vpp, vpp, u64p, u64p]),
     "gf_ntt_towers": (C.c_int, [u32, vp, u32, u64p, u64p, C.c_int, vp]),
     "gf_automorph": (C.c_int, [u32, vp, vp, u32, u64, vp]),
@@ -220,6 +230,27 @@ class Key:
         self.h = lib.gf_key_create(ctx.h, P64(av), P64(bv), nparts, stream.s if stream else None)
         if not self.h:
             raise GpuFheError(lib.gf_last_error().decode())
+
+    @classmethod
+    def generate(cls, ctx, d_sold, d_pka, d_pkb, seed, ns=1, sigma=3.19, stream=None):
+        """Switching key generated on the device from device arrays [L+K][n]."""
+        k = cls.__new__(cls)
+        k.ctx = ctx
+        k.h = lib.gf_key_generate(ctx.h, d_sold.ptr, d_pka.ptr, d_pkb.ptr, ns, sigma, seed,
+                                  stream.s if stream else None)
+        if not k.h:
+            raise GpuFheError(lib.gf_last_error().decode())
+        return k
+
+    def download(self, stream=None):
+        """(av, bv) in plain form, [nparts][rows][n] flattened."""
+        p, r, f = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        lib.gf_key_shape(self.h, C.byref(p), C.byref(r), C.byref(f))
+        words = p.value * r.value * self.ctx.n
+        av, bv = np.empty(words, np.uint64), np.empty(words, np.uint64)
+        check(lib.gf_key_download(self.h, av.ctypes.data_as(u64p), bv.ctypes.data_as(u64p),
+                                  stream.s if stream else None))
+        return av, bv
 
     @classmethod
     def uniform(cls, ctx, nparts, seed, stream=None):
@@ -429,3 +460,43 @@ class Server:
         if getattr(self, "h", None):
             lib.gf_server_destroy(self.h)
             self.h = None
+
+
+# ---- wire format ----
+def pack_ciphertext(c0, c1, n, moduli, scale=0.0):
+    """Host arrays (towers x n each) -> bytes with header and checksum."""
+    c0 = np.ascontiguousarray(c0, np.uint64)
+    c1 = np.ascontiguousarray(c1, np.uint64)
+    mods = np.ascontiguousarray(moduli, np.uint64)
+    towers = len(mods)
+    out = C.create_string_buffer(lib.gf_ct_packed_bytes(n, towers))
+    check(lib.gf_ct_pack(c0.ctypes.data_as(u64p), c1.ctypes.data_as(u64p), n, towers, mods.ctypes.data_as(u64p),
+                         scale, out, len(out)))
+    return out.raw
+
+
+def unpack_ciphertext(buf):
+    """bytes -> (c0, c1, n, moduli, scale); raises on any mismatch."""
+    n, t, sc = C.c_uint32(), C.c_uint32(), C.c_double()
+    check(lib.gf_ct_unpack_header(buf, len(buf), C.byref(n), C.byref(t), C.byref(sc)))
+    mods = np.empty(t.value, np.uint64)
+    c0 = np.empty(t.value * n.value, np.uint64)
+    c1 = np.empty(t.value * n.value, np.uint64)
+    check(lib.gf_ct_unpack(buf, len(buf), mods.ctypes.data_as(u64p), c0.ctypes.data_as(u64p),
+                           c1.ctypes.data_as(u64p)))
+    return c0, c1, n.value, mods, sc.value
+
+
+def pack_key(key, stream=None):
+    out = C.create_string_buffer(lib.gf_key_packed_bytes(key.h))
+    check(lib.gf_key_pack(key.h, out, len(out), stream.s if stream else None))
+    return out.raw
+
+
+def unpack_key(ctx, buf, stream=None):
+    k = Key.__new__(Key)
+    k.ctx = ctx
+    k.h = lib.gf_key_unpack(ctx.h, buf, len(buf), stream.s if stream else None)
+    if not k.h:
+        raise GpuFheError(lib.gf_last_error().decode())
+    return k

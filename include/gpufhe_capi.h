@@ -81,6 +81,18 @@ gf_key* gf_key_create_rows(gf_ctx* c, const uint64_t* h_av, const uint64_t* h_bv
 /* Synthetic switching key of uniform residues, generated on the device from
  * seed (benchmark key material: a switching key is computationally uniform). */
 gf_key* gf_key_create_uniform(gf_ctx* c, uint32_t nparts, uint64_t seed, hipStream_t s);
+/* Switching key generated on the device (evalkeygen_host_sold of
+ * src/evalkeygen.cpp:42-96 without the host round trips): per part
+ * av = pkA u + ns e1, bv = pkB u + ns e0 + (P mod q) sOld on the part's Q
+ * towers, over all L+K towers of the context.  d_sold, d_pka, d_pkb: device
+ * arrays [L+K][n] in evaluation form.  u, e0, e1 are drawn on the host from
+ * mt19937_64(seed) in the reference's order with the library's Gaussian
+ * (sigma), so the key equals evalkeygen_host_sold's for the same inputs. */
+gf_key* gf_key_generate(gf_ctx* c, const uint64_t* d_sold, const uint64_t* d_pka, const uint64_t* d_pkb,
+                        uint64_t ns, double sigma, uint64_t seed, hipStream_t s);
+/* rows of a resident key in plain form, h_av/h_bv [nparts][rows][n] */
+int gf_key_download(const gf_key* k, uint64_t* h_av, uint64_t* h_bv, hipStream_t s);
+void gf_key_shape(const gf_key* k, uint32_t* nparts, uint32_t* rows, uint32_t* fullQ);
 
 /* ---- tower utilities ----
  * Device root tables of (n, q, root) in the reference layout
@@ -194,6 +206,21 @@ int gf_event_create(hipEvent_t* e);
 int gf_event_destroy(hipEvent_t e);
 int gf_event_record(hipEvent_t e, hipStream_t s);
 int gf_event_elapsed_ms(float* ms, hipEvent_t start, hipEvent_t stop);
+
+/* ---- wire format: ciphertexts and switching keys with a parameter header ----
+ * The reference keeps raw tower-major arrays (src/device_ciphertext.cpp:43-61);
+ * these frame the same arrays with the ring size, tower count, moduli, a
+ * scale and an FNV-1a checksum, and refuse data that does not match.
+ * Ciphertext functions work on host arrays (towers x n each). */
+size_t gf_ct_packed_bytes(uint32_t n, uint32_t towers);
+int gf_ct_pack(const uint64_t* h_c0, const uint64_t* h_c1, uint32_t n, uint32_t towers,
+               const uint64_t* h_moduli, double scale, void* out, size_t cap);
+int gf_ct_unpack_header(const void* buf, size_t len, uint32_t* n, uint32_t* towers, double* scale);
+int gf_ct_unpack(const void* buf, size_t len, uint64_t* h_moduli, uint64_t* h_c0, uint64_t* h_c1);
+/* a resident key to bytes (plain form) and back onto a context with the same moduli */
+size_t gf_key_packed_bytes(const gf_key* k);
+int gf_key_pack(const gf_key* k, void* out, size_t cap, hipStream_t s);
+gf_key* gf_key_unpack(gf_ctx* c, const void* buf, size_t len, hipStream_t s);
 
 /* ---- serving: continuous batching over resident ciphertexts ----
  * No counterpart in the reference (it serves nothing).  A server owns a few

@@ -303,6 +303,27 @@ def test_rotate_hoisted_vs_oracle(G, n, L, dnum):
             assert np.array_equal(do[i * m + j][1].download(), want[j][1]), (i, j)
 
 
+@pytest.mark.parametrize("n,L,dnum", [(1 << 12, 4, 2), (1 << 13, 6, 3)])
+def test_device_keygen_matches_oracle(G, n, L, dnum):
+    """gf_key_generate (relinearisation and rotation keys from device-resident
+    secret and public key) equals the oracle's evalkeygen, and a key uploaded
+    from the host reads back unchanged."""
+    P = make_params(n, L, dnum)
+    ks = KeySet(P)
+    ctx = G.Context(n, P.q, P.p, P.roots, dnum)
+    dpa, dpb = G.Dev.of(ks.pkAQP), G.Dev.of(ks.pkBQP)
+    for sold, seed in ((ks.s2(), 202), (P.automorph(ks.sQP, P.L + P.K, 25, P.K), 77)):
+        ds = G.Dev.of(sold)
+        key = G.Key.generate(ctx, ds, dpa, dpb, seed)
+        av, bv = key.download()
+        wa, wb = O.evalkeygen_sold(P.n, P.L, P.K, P.alpha, P.dnum, P.mods, P.roots, sold, ks.pkAQP,
+                                   ks.pkBQP, seed=seed)
+        assert np.array_equal(av, wa) and np.array_equal(bv, wb), seed
+    up = G.Key(ctx, wa, wb, dnum)
+    ra, rb = up.download()
+    assert np.array_equal(ra, wa) and np.array_equal(rb, wb)
+
+
 def test_chain_plans_graphs(G):
     """Multiply-rescale chain (tests/test_chain.cpp shape): a batch of
     ciphertexts multiplied depth times, one plan per step over the batch,

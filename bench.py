@@ -52,7 +52,7 @@ STEP_KERNELS = {"ntt_fwd": ("k_ntt_A", "k_ntt_B"), "ntt_inv": ("k_ntt_B", "k_ntt
                 "tensor": ("k_tensor",), "conv": ("k_conv",), "inner": ("k_inner",),
                 "combine": ("k_combine",), "switch": ("k_switch",), "fuse": ("k_fuse",),
                 "add": ("k_add",), "perm": ("k_perm",), "copy": ("k_copy",), "drop": ("k_drop",),
-                "rsv": ("k_rs",)}
+                "rsv": ("k_rs",), "ptacc": ("k_ptacc",)}
 
 
 def kernel_bytes(name, kind, nkern, total, epilogue):

@@ -24,12 +24,14 @@ static thread_local std::string g_err;
 void gf::set_last_error(const char* msg) { g_err = msg; }
 
 // single-op plans are cached by (op, arguments); building one uploads its job
-// arrays synchronously, which a stream capture cannot contain
-static Plan* cached_plan(gf_ctx* c, std::vector<uint64_t> key, hipStream_t s,
-                         const std::function<void(PlanBuilder&)>& build) {
+// arrays synchronously, which a stream capture cannot contain.  The caller
+// holds a reference while it launches, so eviction by another thread never
+// frees a plan in use.
+static std::shared_ptr<Plan> cached_plan(gf_ctx* c, std::vector<uint64_t> key, hipStream_t s,
+                                         const std::function<void(PlanBuilder&)>& build) {
     std::lock_guard<std::mutex> lk(c->mu);
     auto it = c->cache.find(key);
-    if (it != c->cache.end()) return it->second.get();
+    if (it != c->cache.end()) return it->second;
     hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
     if (s && hipStreamIsCapturing(s, &st) == hipSuccess && st != hipStreamCaptureStatusNone)
         throw std::runtime_error("gpufhe: first call of an operation with new arguments inside a stream capture; "
@@ -37,10 +39,9 @@ static Plan* cached_plan(gf_ctx* c, std::vector<uint64_t> key, hipStream_t s,
     if (c->cache.size() > 4096) c->cache.clear();
     PlanBuilder b(*c->C);
     build(b);
-    auto P = b.finish();
-    Plan* raw = P.get();
-    c->cache.emplace(std::move(key), std::move(P));
-    return raw;
+    std::shared_ptr<Plan> P(b.finish());
+    c->cache.emplace(std::move(key), P);
+    return P;
 }
 
 #define U(x) ((uint64_t)(uintptr_t)(x))
@@ -217,7 +218,7 @@ int gf_automorph(uint32_t n, const uint64_t* d_in, uint64_t* d_out, uint32_t tow
 }
 int gf_rescale_comp(gf_ctx* c, uint32_t l, const uint64_t* d_c, uint64_t* d_o, uint64_t* ws, hipStream_t s) {
     return guard([&] {
-        Plan* P = cached_plan(c, {7, l, U(d_c), U(d_o), U(ws)}, s,
+        auto P = cached_plan(c, {7, l, U(d_c), U(d_o), U(ws)}, s,
                               [&](PlanBuilder& b) { b.rescale_one(l, d_c, d_o, ws); });
         P->run(s);
     });
@@ -227,7 +228,7 @@ int gf_evalmult_rescale(gf_ctx* c, gf_key* k, uint32_t l, const uint64_t* c0a, c
                         const uint64_t* c0b, const uint64_t* c1b, uint64_t* o0, uint64_t* o1, uint64_t* ws,
                         hipStream_t s) {
     return guard([&] {
-        Plan* P = cached_plan(c, {1, l, U(k), U(c0a), U(c1a), U(c0b), U(c1b), U(o0), U(o1), U(ws)}, s,
+        auto P = cached_plan(c, {1, l, U(k), U(c0a), U(c1a), U(c0b), U(c1b), U(o0), U(o1), U(ws)}, s,
                               [&](PlanBuilder& b) { b.evalmult_rescale(l, c0a, c1a, c0b, c1b, o0, o1, *k->K, ws); });
         P->run(s);
     });
@@ -235,7 +236,7 @@ int gf_evalmult_rescale(gf_ctx* c, gf_key* k, uint32_t l, const uint64_t* c0a, c
 int gf_evalmult(gf_ctx* c, gf_key* k, uint32_t l, const uint64_t* c0a, const uint64_t* c1a, const uint64_t* c0b,
                 const uint64_t* c1b, uint64_t* o0, uint64_t* o1, uint64_t* ws, hipStream_t s) {
     return guard([&] {
-        Plan* P = cached_plan(c, {2, l, U(k), U(c0a), U(c1a), U(c0b), U(c1b), U(o0), U(o1), U(ws)}, s,
+        auto P = cached_plan(c, {2, l, U(k), U(c0a), U(c1a), U(c0b), U(c1b), U(o0), U(o1), U(ws)}, s,
                               [&](PlanBuilder& b) { b.evalmult(l, c0a, c1a, c0b, c1b, o0, o1, *k->K, ws); });
         P->run(s);
     });
@@ -243,7 +244,7 @@ int gf_evalmult(gf_ctx* c, gf_key* k, uint32_t l, const uint64_t* c0a, const uin
 int gf_keyswitch(gf_ctx* c, gf_key* k, uint32_t l, const uint64_t* d, uint64_t* ba0, uint64_t* ba1, uint64_t* ws,
                  hipStream_t s) {
     return guard([&] {
-        Plan* P = cached_plan(c, {3, l, U(k), U(d), U(ba0), U(ba1), U(ws)}, s,
+        auto P = cached_plan(c, {3, l, U(k), U(d), U(ba0), U(ba1), U(ws)}, s,
                               [&](PlanBuilder& b) { b.keyswitch(l, d, ba0, ba1, *k->K, ws); });
         P->run(s);
     });
@@ -251,7 +252,7 @@ int gf_keyswitch(gf_ctx* c, gf_key* k, uint32_t l, const uint64_t* d, uint64_t* 
 int gf_rescale(gf_ctx* c, uint32_t l, const uint64_t* c0, const uint64_t* c1, uint64_t* o0, uint64_t* o1,
                uint64_t* ws, hipStream_t s) {
     return guard([&] {
-        Plan* P = cached_plan(c, {4, l, U(c0), U(c1), U(o0), U(o1), U(ws)}, s,
+        auto P = cached_plan(c, {4, l, U(c0), U(c1), U(o0), U(o1), U(ws)}, s,
                               [&](PlanBuilder& b) { b.rescale(l, c0, c1, o0, o1, ws); });
         P->run(s);
     });
@@ -259,7 +260,7 @@ int gf_rescale(gf_ctx* c, uint32_t l, const uint64_t* c0, const uint64_t* c1, ui
 int gf_rotate(gf_ctx* c, gf_key* k, uint32_t l, uint64_t kk, const uint64_t* c0, const uint64_t* c1, uint64_t* o0,
               uint64_t* o1, uint64_t* ws, hipStream_t s) {
     return guard([&] {
-        Plan* P = cached_plan(c, {5, l, U(k), kk, U(c0), U(c1), U(o0), U(o1), U(ws)}, s,
+        auto P = cached_plan(c, {5, l, U(k), kk, U(c0), U(c1), U(o0), U(o1), U(ws)}, s,
                               [&](PlanBuilder& b) { b.rotate(l, kk, c0, c1, o0, o1, *k->K, ws); });
         P->run(s);
     });
@@ -267,7 +268,7 @@ int gf_rotate(gf_ctx* c, gf_key* k, uint32_t l, uint64_t kk, const uint64_t* c0,
 int gf_add(gf_ctx* c, uint32_t l, const uint64_t* a0, const uint64_t* a1, const uint64_t* b0, const uint64_t* b1,
            uint64_t* o0, uint64_t* o1, hipStream_t s) {
     return guard([&] {
-        Plan* P = cached_plan(c, {6, l, U(a0), U(a1), U(b0), U(b1), U(o0), U(o1)}, s,
+        auto P = cached_plan(c, {6, l, U(a0), U(a1), U(b0), U(b1), U(o0), U(o1)}, s,
                               [&](PlanBuilder& b) { b.add(l, a0, a1, b0, b1, o0, o1); });
         P->run(s);
     });
@@ -309,6 +310,22 @@ gf_plan* gf_plan_rotate_hoisted(gf_ctx* c, uint32_t count, const uint32_t* level
         for (uint32_t i = 0; i < count; ++i)
             b.rotate_hoisted(levels[i], m, ks, c0[i], c1[i], o0 + (size_t)i * m, o1 + (size_t)i * m, keys.data(),
                              ws_base[i], ws_rot + (size_t)i * m);
+        auto p = new gf_plan;
+        p->P = b.finish();
+        return p;
+    });
+}
+gf_plan* gf_plan_linear_transform(gf_ctx* c, uint32_t count, const uint32_t* levels, const uint64_t* const* c0,
+                                  const uint64_t* const* c1, uint32_t m, gf_key* const* rks, const uint64_t* ks,
+                                  const uint64_t* const* pts, uint64_t* const* o0, uint64_t* const* o1,
+                                  uint64_t* const* ws_base, uint64_t* const* ws_rot) {
+    return guard_ptr<gf_plan>([&] {
+        std::vector<const DevKey*> keys(m);
+        for (uint32_t j = 0; j < m; ++j) keys[j] = rks[j] ? rks[j]->K.get() : nullptr;
+        PlanBuilder b(*c->C);
+        for (uint32_t i = 0; i < count; ++i)
+            b.linear_transform(levels[i], m, ks, c0[i], c1[i], pts, o0[i], o1[i], keys.data(), ws_base[i],
+                               ws_rot + (size_t)i * m);
         auto p = new gf_plan;
         p->P = b.finish();
         return p;

@@ -440,6 +440,7 @@ void Plan::run_jobs(size_t i, hipStream_t s, uint32_t which, uint32_t first, uin
         case Step::COPY: launch_copy(GF_J(CopyJob), cnt, n, s); break;
         case Step::DROP: launch_drop(GF_J(DropJob), cnt, n, s); break;
         case Step::RSV: launch_rs(GF_J(RsJob), cnt, n, s); break;
+        case Step::PTACC: launch_ptacc(GF_J(PtAccJob), cnt, n, s); break;
     }
 #undef GF_J
 }
@@ -765,6 +766,64 @@ void PlanBuilder::rotate_hoisted(uint32_t l, uint32_t m, const uint64_t* ks, con
     }
 }
 
+// sum_j pt_j * rot_kj(c0, c1): the rotations share one decomposition
+// (rotate_hoisted, results left in each rotation's workspace), a term with
+// no key and k = 1 is the ciphertext itself, and one accumulate launch per
+// tower weights and sums all terms (kernels/stage_acc.cu:12-28 for every
+// term at once).  pts[j]: plaintext in evaluation form, at least l towers.
+void PlanBuilder::linear_transform(uint32_t l, uint32_t m, const uint64_t* ks, const uint64_t* c0,
+                                   const uint64_t* c1, const uint64_t* const* pts, uint64_t* o0, uint64_t* o1,
+                                   const DevKey* const* keys, uint64_t* ws_base, uint64_t* const* ws_rot) {
+    const Ctx& C = C_;
+    const uint32_t n = C.n;
+    if (m == 0 || m > MAXP) throw std::runtime_error("gpufhe: a linear transform takes 1 to 16 terms");
+    if (l < 1 || l > C.L) throw std::runtime_error("gpufhe: bad level");
+    // the accumulate keeps sum_j pt_j x_j in 128 bits and reduces it with one
+    // Montgomery step, which needs sum < q 2^64, i.e. m q < 2^64
+    for (uint32_t i = 0; i < l; ++i)
+        if (C.mods[i] > ~0ull / m) throw std::runtime_error("gpufhe: too many linear-transform terms for this modulus");
+    const WsLayout w = WsLayout::make(C);
+    std::vector<const uint64_t*> x0(m), x1(m);
+    std::vector<uint64_t> rk;
+    std::vector<uint64_t*> r0, r1, rws;
+    std::vector<const DevKey*> rkey;
+    for (uint32_t j = 0; j < m; ++j) {
+        if (!keys[j]) {
+            if (ks[j] != 1) throw std::runtime_error("gpufhe: a term without a rotation key must have k = 1");
+            x0[j] = c0;
+            x1[j] = c1;
+            continue;
+        }
+        uint64_t* ws = ws_rot[j];
+        r0.push_back(ws + w.t0 * n);   // sigma(c0) + ba0 overwrites sigma(c0) element by element
+        r1.push_back(ws + w.t1 * n);
+        rk.push_back(ks[j]);
+        rkey.push_back(keys[j]);
+        rws.push_back(ws);
+        x0[j] = r0.back();
+        x1[j] = r1.back();
+    }
+    if (!rk.empty())
+        rotate_hoisted(l, (uint32_t)rk.size(), rk.data(), c0, c1, r0.data(), r1.data(), rkey.data(), ws_base,
+                       rws.data());
+    for (uint32_t i = 0; i < l; ++i) {
+        const size_t o = (size_t)i * n;
+        PtAccJob J{};
+        J.m = m;
+        for (uint32_t j = 0; j < m; ++j) {
+            J.pt[j] = pts[j] + o;
+            J.x0[j] = x0[j] + o;
+            J.x1[j] = x1[j] + o;
+        }
+        J.o0 = o0 + o;
+        J.o1 = o1 + o;
+        J.q = C.rc[i].q;
+        J.qneg = C.rc[i].qneg;
+        J.r2 = C.rc[i].r2;
+        ptacc_.push_back(J);
+    }
+}
+
 void PlanBuilder::add(uint32_t l, const uint64_t* a0, const uint64_t* a1, const uint64_t* b0,
                       const uint64_t* b1, uint64_t* o0, uint64_t* o1) {
     const uint32_t n = C_.n;
@@ -846,6 +905,7 @@ std::unique_ptr<Plan> PlanBuilder::layout(std::vector<char>& blob) {
     GF_STAGE(RSV, rs_);
     GF_STAGE(NTT_F, ntt_v_);
     GF_STAGE(ADD, add_);
+    GF_STAGE(PTACC, ptacc_);
 #undef GF_STAGE
     return P;
 }

@@ -109,7 +109,7 @@ void download_key(const Ctx& C, const DevKey& k, uint64_t* av, uint64_t* bv, hip
 
 // One launch step of a plan
 struct Step {
-    enum Kind { NTT_F, NTT_I, TENSOR, CONV, INNER, COMBINE, SWITCH, FUSE, ADD, PERM, COPY, DROP, RSV } kind;
+    enum Kind { NTT_F, NTT_I, TENSOR, CONV, INNER, COMBINE, SWITCH, FUSE, ADD, PERM, COPY, DROP, RSV, PTACC } kind;
     size_t off;     // byte offset of the first job in the plan's device buffer
     uint32_t count;
 };
@@ -160,6 +160,11 @@ public:
                         uint64_t* const* ws_rot);
     void add(uint32_t l, const uint64_t* a0, const uint64_t* a1, const uint64_t* b0,
              const uint64_t* b1, uint64_t* o0, uint64_t* o1);
+    // sum_j pts[j] * rot_ks[j](c0, c1) with hoisted rotations; keys[j] null and
+    // ks[j] = 1 for the unrotated term
+    void linear_transform(uint32_t l, uint32_t m, const uint64_t* ks, const uint64_t* c0, const uint64_t* c1,
+                          const uint64_t* const* pts, uint64_t* o0, uint64_t* o1, const DevKey* const* keys,
+                          uint64_t* ws_base, uint64_t* const* ws_rot);
     std::unique_ptr<Plan> finish();
     // the same, with the job arrays staged in pinned host memory h_stage and
     // copied on stream s into the caller's device arena (both `cap` bytes): no
@@ -203,6 +208,7 @@ private:
     void moddown_tail(uint32_t l, uint64_t* ws, uint64_t* ba0, uint64_t* ba1, const uint64_t* t0,
                       const uint64_t* t1);
     std::vector<PermJob> perm_dig_;
+    std::vector<PtAccJob> ptacc_;
     std::vector<AddJob> add_;
     std::unique_ptr<Plan> layout(std::vector<char>& blob);
     void rescale_comp(uint32_t l, const uint64_t* c, uint64_t* o, uint64_t* drop, uint64_t* sw);

@@ -12,7 +12,7 @@
 struct gf_ctx {
     std::unique_ptr<gf::Ctx> C;
     std::mutex mu;
-    std::map<std::vector<uint64_t>, std::unique_ptr<gf::Plan>> cache;  // single-op plans
+    std::map<std::vector<uint64_t>, std::shared_ptr<gf::Plan>> cache;  // single-op plans
 };
 struct gf_key {
     gf_ctx* ctx;

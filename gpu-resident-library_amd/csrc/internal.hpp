@@ -138,6 +138,19 @@ struct EkJob {
     uint64_t q, qneg, r2, ns, nsp, pq, pqp;
 };
 
+// plaintext-weighted sum of m ciphertext towers (linear transforms):
+// out0 = sum_j pt_j x0_j, out1 = sum_j pt_j x1_j mod q (the accumulate of
+// kernels/stage_acc.cu:12-28 over all terms at once)
+struct PtAccJob {
+    const uint64_t* pt[MAXP];
+    const uint64_t* x0[MAXP];
+    const uint64_t* x1[MAXP];
+    uint64_t* o0;
+    uint64_t* o1;
+    uint64_t q, qneg, r2;
+    uint32_t m;
+};
+
 // copy one tower
 struct CopyJob {
     const uint64_t* in;
@@ -171,6 +184,7 @@ void launch_drop(const DropJob* d, uint32_t nj, uint32_t n, hipStream_t s);
 void launch_rs(const RsJob* d, uint32_t nj, uint32_t n, hipStream_t s);
 void launch_embed(const EmbedJob* d, uint32_t nj, uint32_t n, hipStream_t s);
 void launch_ekgen(const EkJob* d, uint32_t nj, uint32_t n, hipStream_t s);
+void launch_ptacc(const PtAccJob* d, uint32_t nj, uint32_t n, hipStream_t s);
 
 inline SwitchJob make_switch(const uint64_t* in, uint64_t* out, uint64_t q_old, uint64_t q_new) {
     SwitchJob j;

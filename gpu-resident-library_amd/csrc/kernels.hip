@@ -311,6 +311,24 @@ __global__ __launch_bounds__(PW_THREADS) void k_ekgen(const EkJob* __restrict__ 
     }
 }
 
+__global__ __launch_bounds__(PW_THREADS) void k_ptacc(const PtAccJob* __restrict__ jobs,
+                                                      uint32_t off, uint32_t n) {
+    const PtAccJob& j = jobs[off + blockIdx.y];
+    const uint32_t m = j.m;
+    const uint64_t q = j.q, qneg = j.qneg, r2 = j.r2;
+    PW_FOR(n) {
+        U128 a0{0, 0}, a1{0, 0};
+        for (uint32_t t = 0; t < m; ++t) {
+            const uint64_t p = j.pt[t][i];
+            a0 = add128(a0, mul128(p, j.x0[t][i]));
+            a1 = add128(a1, mul128(p, j.x1[t][i]));
+        }
+        // REDC(REDC(sum) R^2) = sum mod q (needs m q < 2^64)
+        j.o0[i] = redc(mul128(redc(a0, q, qneg), r2), q, qneg);
+        j.o1[i] = redc(mul128(redc(a1, q, qneg), r2), q, qneg);
+    }
+}
+
 __global__ __launch_bounds__(PW_THREADS) void k_add(const AddJob* __restrict__ jobs,
                                                     uint32_t off, uint32_t n) {
     const AddJob j = jobs[off + blockIdx.y];
@@ -383,6 +401,7 @@ GF_PW_LAUNCH(launch_drop, k_drop, DropJob)
 GF_PW_LAUNCH(launch_rs, k_rs, RsJob)
 GF_PW_LAUNCH(launch_embed, k_embed, EmbedJob)
 GF_PW_LAUNCH(launch_ekgen, k_ekgen, EkJob)
+GF_PW_LAUNCH(launch_ptacc, k_ptacc, PtAccJob)
 
 void launch_fill(uint64_t* d, const uint64_t* d_mods, uint32_t towers, uint32_t n, uint64_t seed,
                  hipStream_t s) {

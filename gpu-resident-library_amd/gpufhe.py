@@ -73,6 +73,7 @@ _capi = {
     "gf_plan_rotate": (vp, [vp, vpp, u32, u32p, u64p] + [vpp] * 5),
     "gf_plan_add": (vp, [vp, u32, u32p] + [vpp] * 6),
     "gf_plan_rotate_hoisted": (vp, [vp, u32, u32p, vpp, vpp, u32, vpp, u64p, vpp, vpp, vpp, vpp]),
+    "gf_plan_linear_transform": (vp, [vp, u32, u32p, vpp, vpp, u32, vpp, u64p, vpp, vpp, vpp, vpp, vpp]),
     "gf_plan_run": (C.c_int, [vp, vp]),
     "gf_plan_run_per_job": (C.c_int, [vp, vp]),
     "gf_plan_run_graph": (C.c_int, [vp, vp]),
@@ -309,7 +310,7 @@ class Plan:
 
 
 STEP_KINDS = ["ntt_fwd", "ntt_inv", "tensor", "conv", "inner", "combine", "switch", "fuse", "add",
-              "perm", "copy", "drop", "rsv"]
+              "perm", "copy", "drop", "rsv", "ptacc"]
 
 
 class Event:
@@ -361,6 +362,19 @@ def plan_rotate_hoisted(ctx, levels, c0, c1, keys, ks, o0, o1, ws_base, ws_rot):
     return Plan(lib.gf_plan_rotate_hoisted(ctx.h, len(levels), lv.ctypes.data_as(u32p), arrs[0], arrs[1], m, kh,
                                            kk.ctypes.data_as(u64p), *outs),
                 keep=(lv, kk, kh, arrs, outs, keys))
+
+
+def plan_linear_transform(ctx, levels, c0, c1, keys, ks, pts, o0, o1, ws_base, ws_rot):
+    """o_i = sum_j pts[j] * rot_ks[j](c_i); keys[j] None with ks[j] = 1 for the
+    unrotated term; ws_rot indexed [i*m + j]."""
+    m = len(ks)
+    lv = np.asarray(levels, np.uint32)
+    kk = np.asarray(ks, np.uint64)
+    kh = ptr_array([k.h if k is not None else 0 for k in keys])
+    arrs = [ptr_array(x) for x in (c0, c1, pts, o0, o1, ws_base, ws_rot)]
+    return Plan(lib.gf_plan_linear_transform(ctx.h, len(levels), lv.ctypes.data_as(u32p), arrs[0], arrs[1], m, kh,
+                                             kk.ctypes.data_as(u64p), *arrs[2:]),
+                keep=(lv, kk, kh, arrs, keys))
 
 
 def plan_add(ctx, levels, a0, a1, b0, b1, o0, o1):

@@ -158,6 +158,19 @@ gf_plan* gf_plan_rotate_hoisted(gf_ctx* c, uint32_t count, const uint32_t* h_lev
                                 const uint64_t* const* c0, const uint64_t* const* c1, uint32_t m,
                                 gf_key* const* rks, const uint64_t* h_k, uint64_t* const* o0,
                                 uint64_t* const* o1, uint64_t* const* ws_base, uint64_t* const* ws_rot);
+/* plaintext-weighted sums of rotations (the diagonal method of linear
+ * transforms): o_i = sum_j pts[j] * rot_{h_k[j]}(c_i) for each of `count`
+ * ciphertexts, the m rotations of one ciphertext hoisted over one
+ * decomposition; rks[j] may be NULL with h_k[j] = 1 for the unrotated term.
+ * pts[j]: device plaintexts in evaluation form (at least the ciphertext's
+ * towers), shared by all ciphertexts.  The accumulate generalises
+ * LaunchPtMulAcc (kernels/stage_acc.cu:12-28) to all terms in one pass.
+ * ws_rot indexed [i*m + j]; m <= 16. */
+gf_plan* gf_plan_linear_transform(gf_ctx* c, uint32_t count, const uint32_t* h_levels,
+                                  const uint64_t* const* c0, const uint64_t* const* c1, uint32_t m,
+                                  gf_key* const* rks, const uint64_t* h_k, const uint64_t* const* pts,
+                                  uint64_t* const* o0, uint64_t* const* o1, uint64_t* const* ws_base,
+                                  uint64_t* const* ws_rot);
 gf_plan* gf_plan_add(gf_ctx* c, uint32_t count, const uint32_t* h_levels,
                      const uint64_t* const* a0, const uint64_t* const* a1,
                      const uint64_t* const* b0, const uint64_t* const* b1, uint64_t* const* o0,

@@ -113,6 +113,8 @@ _sig = {
                             u64p, u64p]),
     "or_rotate_hoisted": (C.c_int, [C.POINTER(OrParams), C.c_uint32, u64p, u64p, C.c_uint32, u32p,
                                     C.POINTER(u64p), C.POINTER(u64p), C.POINTER(u64p), C.POINTER(u64p)]),
+    "or_linear_transform": (C.c_int, [C.POINTER(OrParams), C.c_uint32, u64p, u64p, C.c_uint32, u32p,
+                                      C.POINTER(u64p), C.POINTER(u64p), C.POINTER(u64p), u64p, u64p]),
     "or_keygen": (None, [C.c_uint32, u64p, u64p, C.c_uint32, C.c_uint64, C.c_double, C.c_uint64,
                          C.c_uint32, u64p, u64p, u64p]),
     "or_evalkeygen_sold": (None, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, u64p,
@@ -276,6 +278,23 @@ class Params:
         if rc:
             raise RuntimeError("or_rotate_hoisted failed")
         return outs
+
+    def linear_transform(self, level, c0, c1, ks, keys, pts):
+        """sum_j pts[j] * rot_ks[j](c0, c1); keys[j] = (av, bv), or None with
+        ks[j] = 1 for the unrotated term.  pts[j]: level*n, evaluation form."""
+        m, T = len(ks), level * self.n
+        o0 = np.zeros(T, np.uint64)
+        o1 = np.zeros(T, np.uint64)
+        avs = [None if k is None else U(k[0]) for k in keys]
+        bvs = [None if k is None else U(k[1]) for k in keys]
+        ptv = [U(p) for p in pts]
+        arr = lambda xs: (u64p * m)(*[None if x is None else P(x) for x in xs])  # noqa: E731
+        kk = np.asarray(ks, np.uint32)
+        rc = lib.or_linear_transform(self.h, level, P(U(c0)), P(U(c1)), m, P(kk), arr(avs), arr(bvs),
+                                     arr(ptv), P(o0), P(o1))
+        if rc:
+            raise RuntimeError("or_linear_transform failed")
+        return o0, o1
 
 
 def keygen(n, mods, roots, ns=1, sigma=3.19, seed=101, hamming=0):

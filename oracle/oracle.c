@@ -744,6 +744,64 @@ int or_rotate_hoisted(const or_params* P, uint32_t sizeQl, const uint64_t* c0, c
     return rc;
 }
 
+/*
+ * Plaintext-weighted sum of rotations (diagonal method of a linear transform):
+ * out = sum_j pt_j * rot_kj(c0, c1).  Terms with a key use the hoisted
+ * rotations above; a term with av[j] == NULL must have ks[j] == 1 and is the
+ * ciphertext itself.  The weighting is the pointwise multiply-accumulate of
+ * kernels/stage_acc.cu:12-28 applied to every term.  pts[j]: sizeQl x n, eval.
+ */
+int or_linear_transform(const or_params* P, uint32_t sizeQl, const uint64_t* c0, const uint64_t* c1,
+                        uint32_t m, const uint32_t* ks, const uint64_t* const* av,
+                        const uint64_t* const* bv, const uint64_t* const* pts, uint64_t* out0,
+                        uint64_t* out1) {
+    const uint32_t n = P->n;
+    const size_t T = (size_t)sizeQl * n;
+    uint32_t* rk = (uint32_t*)malloc((size_t)m * 4);
+    const uint64_t** ra = (const uint64_t**)malloc((size_t)m * sizeof(void*));
+    const uint64_t** rb = (const uint64_t**)malloc((size_t)m * sizeof(void*));
+    uint64_t** r0 = (uint64_t**)malloc((size_t)m * sizeof(void*));
+    uint64_t** r1 = (uint64_t**)malloc((size_t)m * sizeof(void*));
+    const uint64_t** x0 = (const uint64_t**)malloc((size_t)m * sizeof(void*));
+    const uint64_t** x1 = (const uint64_t**)malloc((size_t)m * sizeof(void*));
+    uint32_t nr = 0;
+    int rc = 0;
+    for (uint32_t j = 0; j < m; ++j) {
+        if (!av[j]) {
+            if (ks[j] != 1) rc = -1;
+            x0[j] = c0;
+            x1[j] = c1;
+            continue;
+        }
+        rk[nr] = ks[j];
+        ra[nr] = av[j];
+        rb[nr] = bv[j];
+        r0[nr] = (uint64_t*)malloc(T * 8);
+        r1[nr] = (uint64_t*)malloc(T * 8);
+        x0[j] = r0[nr];
+        x1[j] = r1[nr];
+        ++nr;
+    }
+    if (rc == 0 && nr) rc = or_rotate_hoisted(P, sizeQl, c0, c1, nr, rk, ra, rb, r0, r1);
+    if (rc == 0)
+        for (uint32_t t = 0; t < sizeQl; ++t) {
+            const uint64_t q = P->mod[t];
+            for (uint32_t k = 0; k < n; ++k) {
+                const size_t x = (size_t)t * n + k;
+                uint64_t s0 = 0, s1 = 0;
+                for (uint32_t j = 0; j < m; ++j) {
+                    s0 = addm(s0, or_mulmod(pts[j][x], x0[j][x], q), q);
+                    s1 = addm(s1, or_mulmod(pts[j][x], x1[j][x], q), q);
+                }
+                out0[x] = s0;
+                out1[x] = s1;
+            }
+        }
+    for (uint32_t j = 0; j < nr; ++j) { free(r0[j]); free(r1[j]); }
+    free(rk); free(ra); free(rb); free(r0); free(r1); free(x0); free(x1);
+    return rc;
+}
+
 /* (sigma_k(c0) + ba0, ba1) with (ba0,ba1) = KS(sigma_k(c1)): src/rotate.cpp:118-132 */
 int or_rotate(const or_params* P, uint32_t sizeQl, const uint64_t* c0, const uint64_t* c1,
               uint32_t k, const uint64_t* av, const uint64_t* bv, uint64_t* out0,

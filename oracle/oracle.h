@@ -32,6 +32,9 @@
  *   or_rotate_hoisted    equals or_rotate exactly for k = 1 (identity
  *                        automorphism) at every level; rotated slots decode
  *                        to the rotated message for k = 5^r.
+ *   or_linear_transform  one term with pt = 1 equals the (hoisted) rotation;
+ *                        the unrotated term equals pt (.) c; linear in the
+ *                        plaintexts; decodes to sum_j diag_j (.) rot_j(z).
  *   or_keygen, or_evalkeygen, or_encrypt, or_decrypt
  *                        RLWE decryption identities (noise bounds), enc/dec
  *                        round trip with maxerr 0.
@@ -145,6 +148,11 @@ int or_automorph_eval(const or_params* P, uint64_t* v, uint32_t towers, uint32_t
 int or_rotate(const or_params* P, uint32_t sizeQl, const uint64_t* c0, const uint64_t* c1,
               uint32_t k, const uint64_t* av, const uint64_t* bv, uint64_t* out0,
               uint64_t* out1);
+/* sum_j pts[j] * rot_kj(c0, c1); av[j] NULL (with ks[j] = 1) for the unrotated term */
+int or_linear_transform(const or_params* P, uint32_t sizeQl, const uint64_t* c0, const uint64_t* c1,
+                        uint32_t m, const uint32_t* ks, const uint64_t* const* av,
+                        const uint64_t* const* bv, const uint64_t* const* pts, uint64_t* out0,
+                        uint64_t* out1);
 /* m rotations of one ciphertext sharing one digit decomposition of c1 */
 int or_rotate_hoisted(const or_params* P, uint32_t sizeQl, const uint64_t* c0, const uint64_t* c1,
                       uint32_t m, const uint32_t* ks, const uint64_t* const* av,

@@ -303,6 +303,54 @@ def test_rotate_hoisted_vs_oracle(G, n, L, dnum):
             assert np.array_equal(do[i * m + j][1].download(), want[j][1]), (i, j)
 
 
+@pytest.mark.parametrize("n,L,dnum", [(1 << 12, 6, 3), (1 << 16, 20, 4)])
+def test_linear_transform_vs_oracle(G, n, L, dnum):
+    """Plaintext-weighted sum of hoisted rotations with an unrotated term
+    (no key, k = 1), bit-exact against the oracle, two ciphertexts at uneven
+    levels in one plan, replayed twice."""
+    P = make_params(n, L, dnum)
+    ctx = G.Context(n, P.q, P.p, P.roots, dnum)
+    rng = np.random.default_rng(33)
+    ks = [1, pow(5, 1, 2 * n), pow(5, 6, 2 * n)]
+    m = len(ks)
+    keys, hk = [None], [None]
+    for _ in range(m - 1):
+        av = np.concatenate([rng.integers(0, int(q), n, dtype=np.uint64) for _ in range(dnum) for q in P.mods])
+        bv = np.concatenate([rng.integers(0, int(q), n, dtype=np.uint64) for _ in range(dnum) for q in P.mods])
+        keys.append((av, bv))
+        hk.append(G.Key(ctx, av, bv, dnum))
+    pts = [rand_ct(P, L, rng) for _ in range(m)]
+    dp = [G.Dev.of(p) for p in pts]
+    levels = [L, L - 3] if n < (1 << 16) else [L - 5]
+    cts = [(rand_ct(P, lv, rng), rand_ct(P, lv, rng)) for lv in levels]
+    dc = [(G.Dev.of(a), G.Dev.of(b)) for a, b in cts]
+    do = [(G.Dev(lv * n), G.Dev(lv * n)) for lv in levels]
+    wb = [ctx.workspace() for _ in levels]
+    wr = [ctx.workspace() for _ in range(len(levels) * m)]
+    plan = G.plan_linear_transform(ctx, levels, [x[0].ptr for x in dc], [x[1].ptr for x in dc], hk, ks,
+                                   [p.ptr for p in dp], [x[0].ptr for x in do], [x[1].ptr for x in do],
+                                   [w.ptr for w in wb], [w.ptr for w in wr])
+    s = G.Stream()
+    want = [P.linear_transform(lv, cts[i][0], cts[i][1], ks, keys, [p[: lv * n] for p in pts])
+            for i, lv in enumerate(levels)]
+    for rep in range(2):
+        plan.run(s)
+        s.sync()
+        for i in range(len(levels)):
+            assert np.array_equal(do[i][0].download(), want[i][0]), (rep, i)
+            assert np.array_equal(do[i][1].download(), want[i][1]), (rep, i)
+
+
+def test_linear_transform_rejects_keyless_rotation(G):
+    P = make_params(1 << 10, 3, 3)
+    ctx = G.Context(P.n, P.q, P.p, P.roots, 3)
+    d = G.Dev(3 * P.n)
+    ws = ctx.workspace()
+    with pytest.raises(RuntimeError):
+        G.plan_linear_transform(ctx, [3], [d.ptr], [d.ptr], [None], [5], [d.ptr], [d.ptr], [d.ptr],
+                                [ws.ptr], [ws.ptr])
+
+
 @pytest.mark.parametrize("n,L,dnum", [(1 << 12, 4, 2), (1 << 13, 6, 3)])
 def test_device_keygen_matches_oracle(G, n, L, dnum):
     """gf_key_generate (relinearisation and rotation keys from device-resident

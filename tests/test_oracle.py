@@ -530,6 +530,60 @@ def test_rotate_hoisted_slots():
         assert np.max(np.abs(decode(dec, n, delta).real - np.roll(z, -r))) < 1e-6, r
 
 
+def _pt_eval(P, coeffs, level):
+    """Integer coefficient vector -> evaluation form over the first `level` towers."""
+    out = []
+    for t in range(level):
+        q = int(P.q[t])
+        fr, fp, *_ = O.root_tables(P.n, q, int(P.roots[t]))
+        out.append(O.ntt(np.mod(coeffs, q).astype(np.uint64), q, fr, fp))
+    return np.concatenate(out)
+
+
+def test_linear_transform_unit_weights():
+    """A single term with the all-ones plaintext is the hoisted rotation; the
+    unrotated term with weight w is w (.) c residue for residue."""
+    P = make_params(1 << 10, 5, 2)
+    rng = np.random.default_rng(15)
+    n, level = P.n, 4
+    av = np.concatenate([rng.integers(0, int(m), n, dtype=np.uint64) for _ in range(P.dnum) for m in P.mods])
+    bv = np.concatenate([rng.integers(0, int(m), n, dtype=np.uint64) for _ in range(P.dnum) for m in P.mods])
+    rnd = lambda: np.concatenate([rng.integers(0, int(q), n, dtype=np.uint64) for q in P.q[:level]])  # noqa: E731
+    c0, c1 = rnd(), rnd()
+    ones = np.ones(level * n, np.uint64)
+    k = pow(5, 3, 2 * n)
+    o0, o1 = P.linear_transform(level, c0, c1, [k], [(av, bv)], [ones])
+    (h0, h1), = P.rotate_hoisted(level, c0, c1, [k], [(av, bv)])
+    assert np.array_equal(o0, h0) and np.array_equal(o1, h1)
+    w = rnd()
+    o0, o1 = P.linear_transform(level, c0, c1, [1], [None], [w])
+    for t in range(level):
+        sl = slice(t * n, (t + 1) * n)
+        assert np.array_equal(o0[sl], mulmod_vec(w[sl], c0[sl], P.q[t]))
+        assert np.array_equal(o1[sl], mulmod_vec(w[sl], c1[sl], P.q[t]))
+
+
+def test_linear_transform_slots():
+    """Diagonal method: sum_j diag_j (.) rot_rj(z) decrypts from the weighted sum
+    of rotations (the unrotated term included)."""
+    P = make_params(1 << 11, 4, 2)
+    ks = KeySet(P)
+    n, level = P.n, 4
+    rng = np.random.default_rng(16)
+    z = rng.uniform(-1, 1, n // 2)
+    d_ct, d_pt = float(1 << 30), float(1 << 20)
+    c0, c1 = O.encrypt(n, P.q, P.roots[:4], encode(z, n, d_ct), ks.pkA, ks.pkB, seed=607)
+    rs = (0, 1, 7)
+    kk = [pow(5, r, 2 * n) for r in rs]
+    diags = [rng.uniform(-1, 1, n // 2) for _ in rs]
+    pts = [_pt_eval(P, encode(dg, n, d_pt), level) for dg in diags]
+    keys = [None if r == 0 else ks.rotkey(k) for r, k in zip(rs, kk)]
+    o0, o1 = P.linear_transform(level, c0, c1, kk, keys, pts)
+    dec = O.decrypt(n, int(P.q[0]), int(P.roots[0]), o0, o1, ks.sQ)
+    want = sum(dg * np.roll(z, -r) for dg, r in zip(diags, rs))
+    assert np.max(np.abs(decode(dec, n, d_ct * d_pt).real - want)) < 1e-4
+
+
 # -------------------------------------------------------------- encode ----
 @needs_ref
 def test_fast_encode_matches_reference_encode():

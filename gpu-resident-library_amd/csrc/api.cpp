@@ -477,5 +477,102 @@ void EvalAdd(const DeviceCiphertext& a, const DeviceCiphertext& b, DeviceCiphert
     ck(gf_add(C.engine(), l, a.c0(), a.c1(), b.c0(), b.c1(), out.c0(), out.c1(), s), "EvalAdd");
     while (out.num_towers() > l) out.drop_tower();
 }
+void EvalMultPlain(const DeviceCiphertext& a, const uint64_t* d_pt, DeviceCiphertext& out, const DeviceKSContext& C,
+                   hipStream_t s) {
+    same_level(a, a, C);
+    const uint32_t l = a.num_towers();
+    check_out(out, l, C.n, "EvalMultPlain");
+    ck(gf_mul_plain(C.engine(), l, a.c0(), a.c1(), d_pt, out.c0(), out.c1(), s), "EvalMultPlain");
+    while (out.num_towers() > l) out.drop_tower();
+}
+void EvalAddPlain(const DeviceCiphertext& a, const uint64_t* d_pt, DeviceCiphertext& out, const DeviceKSContext& C,
+                  hipStream_t s) {
+    same_level(a, a, C);
+    const uint32_t l = a.num_towers();
+    check_out(out, l, C.n, "EvalAddPlain");
+    ck(gf_add_plain(C.engine(), l, a.c0(), a.c1(), d_pt, out.c0(), out.c1(), s), "EvalAddPlain");
+    while (out.num_towers() > l) out.drop_tower();
+}
+
+// The rotation contexts of one call must share the parameter set: their keys
+// then have one layout and run on the first context's engine.
+static const DeviceKSContext& rotation_engine(const DeviceCiphertext& ct,
+                                              const std::vector<const DeviceKSContext*>& rotC,
+                                              std::vector<gf_key*>& keys, const char* op) {
+    const DeviceKSContext* E = nullptr;
+    for (const DeviceKSContext* r : rotC)
+        if (r) {
+            E = r;
+            break;
+        }
+    if (!E) throw std::runtime_error(std::string(op) + ": needs at least one rotation context");
+    for (const DeviceKSContext* r : rotC)
+        if (r && (r->modList != E->modList || r->numPart != E->numPart || r->sizeQl != E->sizeQl))
+            throw std::runtime_error(std::string(op) + ": rotation contexts of different parameter sets");
+    const uint32_t l = ct.num_towers();
+    if (ct.n() != E->n || l == 0 || l > E->sizeQl)
+        throw std::runtime_error(std::string(op) + ": level outside the context");
+    keys.clear();
+    for (const DeviceKSContext* r : rotC) keys.push_back(r ? r->key() : nullptr);
+    return *E;
+}
+// Wrot[j] serves term j; a term without a key (the unrotated one) uses none,
+// so its entry may be empty
+static std::vector<uint64_t*> work_ptrs(std::vector<DeviceKSWork>& Wrot, const std::vector<gf_key*>& keys,
+                                        const char* op) {
+    if (Wrot.size() < keys.size()) throw std::runtime_error(std::string(op) + ": one workspace per term needed");
+    std::vector<uint64_t*> p;
+    for (size_t j = 0; j < keys.size(); ++j) {
+        if (keys[j]) need_ws(Wrot[j]);
+        p.push_back(Wrot[j].d_ws);
+    }
+    return p;
+}
+
+void RotateHoisted(const DeviceCiphertext& ct, const std::vector<uint32_t>& ks,
+                   const std::vector<const DeviceKSContext*>& rotC, std::vector<DeviceCiphertext>& outs,
+                   DeviceKSWork& W, std::vector<DeviceKSWork>& Wrot, hipStream_t s) {
+    const size_t m = ks.size();
+    if (m == 0 || rotC.size() != m || outs.size() != m)
+        throw std::runtime_error("RotateHoisted: ks, rotC and outs differ in length");
+    for (const DeviceKSContext* r : rotC)
+        if (!r) throw std::runtime_error("RotateHoisted: every index needs a rotation context");
+    need_ws(W);
+    std::vector<gf_key*> keys;
+    const DeviceKSContext& E = rotation_engine(ct, rotC, keys, "RotateHoisted");
+    const uint32_t l = ct.num_towers();
+    std::vector<uint64_t*> o0, o1;
+    for (DeviceCiphertext& o : outs) {
+        check_out(o, l, E.n, "RotateHoisted");
+        o0.push_back(o.c0());
+        o1.push_back(o.c1());
+    }
+    std::vector<uint64_t*> wr = work_ptrs(Wrot, keys, "RotateHoisted");
+    std::vector<uint64_t> k64(ks.begin(), ks.end());
+    ck(gf_rotate_hoisted(E.engine(), l, ct.c0(), ct.c1(), (uint32_t)m, keys.data(), k64.data(), o0.data(), o1.data(),
+                         W.d_ws, wr.data(), s),
+       "RotateHoisted");
+    for (DeviceCiphertext& o : outs)
+        while (o.num_towers() > l) o.drop_tower();
+}
+
+void LinearTransform(const DeviceCiphertext& ct, const std::vector<uint32_t>& ks,
+                     const std::vector<const DeviceKSContext*>& rotC, const std::vector<const uint64_t*>& d_pts,
+                     DeviceCiphertext& out, DeviceKSWork& W, std::vector<DeviceKSWork>& Wrot, hipStream_t s) {
+    const size_t m = ks.size();
+    if (m == 0 || rotC.size() != m || d_pts.size() != m)
+        throw std::runtime_error("LinearTransform: ks, rotC and d_pts differ in length");
+    need_ws(W);
+    std::vector<gf_key*> keys;
+    const DeviceKSContext& E = rotation_engine(ct, rotC, keys, "LinearTransform");
+    const uint32_t l = ct.num_towers();
+    check_out(out, l, E.n, "LinearTransform");
+    std::vector<uint64_t*> wr = work_ptrs(Wrot, keys, "LinearTransform");
+    std::vector<uint64_t> k64(ks.begin(), ks.end());
+    ck(gf_linear_transform(E.engine(), l, ct.c0(), ct.c1(), (uint32_t)m, keys.data(), k64.data(), d_pts.data(),
+                           out.c0(), out.c1(), W.d_ws, wr.data(), s),
+       "LinearTransform");
+    while (out.num_towers() > l) out.drop_tower();
+}
 
 }  // namespace gpufhe

@@ -331,6 +331,78 @@ gf_plan* gf_plan_linear_transform(gf_ctx* c, uint32_t count, const uint32_t* lev
         return p;
     });
 }
+int gf_mul_plain(gf_ctx* c, uint32_t l, const uint64_t* c0, const uint64_t* c1, const uint64_t* pt, uint64_t* o0,
+                 uint64_t* o1, hipStream_t s) {
+    return guard([&] {
+        auto P = cached_plan(c, {10, l, U(c0), U(c1), U(pt), U(o0), U(o1)}, s,
+                             [&](PlanBuilder& b) { b.mul_plain(l, c0, c1, pt, o0, o1); });
+        P->run(s);
+    });
+}
+int gf_add_plain(gf_ctx* c, uint32_t l, const uint64_t* c0, const uint64_t* c1, const uint64_t* pt, uint64_t* o0,
+                 uint64_t* o1, hipStream_t s) {
+    return guard([&] {
+        auto P = cached_plan(c, {11, l, U(c0), U(c1), U(pt), U(o0), U(o1)}, s,
+                             [&](PlanBuilder& b) { b.add_plain(l, c0, c1, pt, o0, o1); });
+        P->run(s);
+    });
+}
+int gf_rotate_hoisted(gf_ctx* c, uint32_t l, const uint64_t* c0, const uint64_t* c1, uint32_t m, gf_key* const* rks,
+                      const uint64_t* ks, uint64_t* const* o0, uint64_t* const* o1, uint64_t* ws_base,
+                      uint64_t* const* ws_rot, hipStream_t s) {
+    return guard([&] {
+        if (m == 0) throw std::runtime_error("gpufhe: hoisted rotation needs at least one index");
+        std::vector<uint64_t> key{12, l, m, U(c0), U(c1), U(ws_base)};
+        std::vector<const DevKey*> keys(m);
+        for (uint32_t j = 0; j < m; ++j) {
+            if (!rks[j]) throw std::runtime_error("gpufhe: hoisted rotation needs a key per index");
+            keys[j] = rks[j]->K.get();
+            key.insert(key.end(), {U(rks[j]), ks[j], U(o0[j]), U(o1[j]), U(ws_rot[j])});
+        }
+        auto P = cached_plan(c, std::move(key), s, [&](PlanBuilder& b) {
+            b.rotate_hoisted(l, m, ks, c0, c1, o0, o1, keys.data(), ws_base, ws_rot);
+        });
+        P->run(s);
+    });
+}
+int gf_linear_transform(gf_ctx* c, uint32_t l, const uint64_t* c0, const uint64_t* c1, uint32_t m,
+                        gf_key* const* rks, const uint64_t* ks, const uint64_t* const* pts, uint64_t* o0,
+                        uint64_t* o1, uint64_t* ws_base, uint64_t* const* ws_rot, hipStream_t s) {
+    return guard([&] {
+        std::vector<uint64_t> key{13, l, m, U(c0), U(c1), U(o0), U(o1), U(ws_base)};
+        std::vector<const DevKey*> keys(m);
+        for (uint32_t j = 0; j < m; ++j) {
+            keys[j] = rks[j] ? rks[j]->K.get() : nullptr;
+            key.insert(key.end(), {U(rks[j]), ks[j], U(pts[j]), U(ws_rot[j])});
+        }
+        auto P = cached_plan(c, std::move(key), s, [&](PlanBuilder& b) {
+            b.linear_transform(l, m, ks, c0, c1, pts, o0, o1, keys.data(), ws_base, ws_rot);
+        });
+        P->run(s);
+    });
+}
+gf_plan* gf_plan_mul_plain(gf_ctx* c, uint32_t count, const uint32_t* levels, const uint64_t* const* c0,
+                           const uint64_t* const* c1, const uint64_t* const* pt, uint64_t* const* o0,
+                           uint64_t* const* o1) {
+    return guard_ptr<gf_plan>([&] {
+        PlanBuilder b(*c->C);
+        for (uint32_t i = 0; i < count; ++i) b.mul_plain(levels[i], c0[i], c1[i], pt[i], o0[i], o1[i]);
+        auto p = new gf_plan;
+        p->P = b.finish();
+        return p;
+    });
+}
+gf_plan* gf_plan_add_plain(gf_ctx* c, uint32_t count, const uint32_t* levels, const uint64_t* const* c0,
+                           const uint64_t* const* c1, const uint64_t* const* pt, uint64_t* const* o0,
+                           uint64_t* const* o1) {
+    return guard_ptr<gf_plan>([&] {
+        PlanBuilder b(*c->C);
+        for (uint32_t i = 0; i < count; ++i) b.add_plain(levels[i], c0[i], c1[i], pt[i], o0[i], o1[i]);
+        auto p = new gf_plan;
+        p->P = b.finish();
+        return p;
+    });
+}
 gf_plan* gf_plan_add(gf_ctx* c, uint32_t count, const uint32_t* levels, const uint64_t* const* a0,
                      const uint64_t* const* a1, const uint64_t* const* b0, const uint64_t* const* b1,
                      uint64_t* const* o0, uint64_t* const* o1) {

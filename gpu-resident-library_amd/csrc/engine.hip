@@ -834,6 +834,39 @@ void PlanBuilder::add(uint32_t l, const uint64_t* a0, const uint64_t* a1, const 
     }
 }
 
+// (c0 pt, c1 pt) per tower: src/ptops.cpp:20-26, one accumulate job per tower
+void PlanBuilder::mul_plain(uint32_t l, const uint64_t* c0, const uint64_t* c1, const uint64_t* pt, uint64_t* o0,
+                            uint64_t* o1) {
+    if (l < 1 || l > C_.L) throw std::runtime_error("gpufhe: bad level");
+    const uint32_t n = C_.n;
+    for (uint32_t i = 0; i < l; ++i) {
+        const size_t o = (size_t)i * n;
+        PtAccJob J{};
+        J.m = 1;
+        J.pt[0] = pt + o;
+        J.x0[0] = c0 + o;
+        J.x1[0] = c1 + o;
+        J.o0 = o0 + o;
+        J.o1 = o1 + o;
+        J.q = C_.rc[i].q;
+        J.qneg = C_.rc[i].qneg;
+        J.r2 = C_.rc[i].r2;
+        ptacc_.push_back(J);
+    }
+}
+
+// (c0 + pt, c1): the ct_add_pt of src/ptops.cpp:1-5 (c1 copied when out is not in place)
+void PlanBuilder::add_plain(uint32_t l, const uint64_t* c0, const uint64_t* c1, const uint64_t* pt, uint64_t* o0,
+                            uint64_t* o1) {
+    if (l < 1 || l > C_.L) throw std::runtime_error("gpufhe: bad level");
+    const uint32_t n = C_.n;
+    for (uint32_t i = 0; i < l; ++i) {
+        const size_t o = (size_t)i * n;
+        add_.push_back(AddJob{c0 + o, pt + o, o0 + o, C_.mods[i]});
+        if (o1 != c1) copy_.push_back(CopyJob{c1 + o, o1 + o});
+    }
+}
+
 // Jobs of one stage are independent; ordering them so that consecutive
 // workgroups share a twiddle table (transforms), a key row (inner product) or a
 // constant table (conversions) keeps those operands in L2 across the batch
@@ -905,6 +938,7 @@ std::unique_ptr<Plan> PlanBuilder::layout(std::vector<char>& blob) {
     GF_STAGE(RSV, rs_);
     GF_STAGE(NTT_F, ntt_v_);
     GF_STAGE(ADD, add_);
+    GF_STAGE(COPY, copy_);
     GF_STAGE(PTACC, ptacc_);
 #undef GF_STAGE
     return P;

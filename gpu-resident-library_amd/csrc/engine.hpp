@@ -165,6 +165,11 @@ public:
     void linear_transform(uint32_t l, uint32_t m, const uint64_t* ks, const uint64_t* c0, const uint64_t* c1,
                           const uint64_t* const* pts, uint64_t* o0, uint64_t* o1, const DevKey* const* keys,
                           uint64_t* ws_base, uint64_t* const* ws_rot);
+    // plaintext (evaluation form, at least l towers) times / plus a ciphertext
+    void mul_plain(uint32_t l, const uint64_t* c0, const uint64_t* c1, const uint64_t* pt, uint64_t* o0,
+                   uint64_t* o1);
+    void add_plain(uint32_t l, const uint64_t* c0, const uint64_t* c1, const uint64_t* pt, uint64_t* o0,
+                   uint64_t* o1);
     std::unique_ptr<Plan> finish();
     // the same, with the job arrays staged in pinned host memory h_stage and
     // copied on stream s into the caller's device arena (both `cap` bytes): no
@@ -210,6 +215,7 @@ private:
     std::vector<PermJob> perm_dig_;
     std::vector<PtAccJob> ptacc_;
     std::vector<AddJob> add_;
+    std::vector<CopyJob> copy_;
     std::unique_ptr<Plan> layout(std::vector<char>& blob);
     void rescale_comp(uint32_t l, const uint64_t* c, uint64_t* o, uint64_t* drop, uint64_t* sw);
 };

@@ -310,6 +310,19 @@ void ct_add_ct_host(std::vector<uint64_t>& a0, std::vector<uint64_t>& a1, const 
             a1[x] = addm(a1[x], b1[x], mod[t]);
         }
 }
+// src/ptops.cpp:20-26
+void ct_mul_pt_host(std::vector<uint64_t>& c0, std::vector<uint64_t>& c1, const std::vector<uint64_t>& ptEval,
+                    uint32_t towers, uint32_t n, const std::vector<uint64_t>& mod) {
+    const size_t T = (size_t)towers * n;
+    if (c0.size() < T || c1.size() < T || ptEval.size() < T || mod.size() < towers)
+        throw std::runtime_error("ct_mul_pt_host: inputs shorter than towers x n");
+    for (uint32_t t = 0; t < towers; ++t)
+        for (uint32_t k = 0; k < n; ++k) {
+            const size_t x = (size_t)t * n + k;
+            c0[x] = mulm(c0[x], ptEval[x], mod[t]);
+            c1[x] = mulm(c1[x], ptEval[x], mod[t]);
+        }
+}
 
 // ------------------------------------------------------------ encoding --
 // Slots z_k sit at the evaluation points zeta^(r_k), zeta = exp(i pi / N),

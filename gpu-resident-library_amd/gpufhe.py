@@ -69,6 +69,12 @@ _capi = {
     "gf_rescale": (C.c_int, [vp, u32, vp, vp, vp, vp, vp, vp]),
     "gf_rotate": (C.c_int, [vp, vp, u32, u64, vp, vp, vp, vp, vp, vp]),
     "gf_add": (C.c_int, [vp, u32, vp, vp, vp, vp, vp, vp, vp]),
+    "gf_mul_plain": (C.c_int, [vp, u32, vp, vp, vp, vp, vp, vp]),
+    "gf_add_plain": (C.c_int, [vp, u32, vp, vp, vp, vp, vp, vp]),
+    "gf_rotate_hoisted": (C.c_int, [vp, u32, vp, vp, u32, vpp, u64p, vpp, vpp, vp, vpp, vp]),
+    "gf_linear_transform": (C.c_int, [vp, u32, vp, vp, u32, vpp, u64p, vpp, vp, vp, vp, vpp, vp]),
+    "gf_plan_mul_plain": (vp, [vp, u32, u32p] + [vpp] * 5),
+    "gf_plan_add_plain": (vp, [vp, u32, u32p] + [vpp] * 5),
     "gf_plan_evalmult_rescale": (vp, [vp, vp, u32, u32p] + [vpp] * 7),
     "gf_plan_rotate": (vp, [vp, vpp, u32, u32p, u64p] + [vpp] * 5),
     "gf_plan_add": (vp, [vp, u32, u32p] + [vpp] * 6),
@@ -375,6 +381,40 @@ def plan_linear_transform(ctx, levels, c0, c1, keys, ks, pts, o0, o1, ws_base, w
     return Plan(lib.gf_plan_linear_transform(ctx.h, len(levels), lv.ctypes.data_as(u32p), arrs[0], arrs[1], m, kh,
                                              kk.ctypes.data_as(u64p), *arrs[2:]),
                 keep=(lv, kk, kh, arrs, keys))
+
+
+def plan_mul_plain(ctx, levels, c0, c1, pt, o0, o1):
+    lv = np.asarray(levels, np.uint32)
+    arrs = [ptr_array(x) for x in (c0, c1, pt, o0, o1)]
+    return Plan(lib.gf_plan_mul_plain(ctx.h, len(levels), lv.ctypes.data_as(u32p), *arrs), keep=(lv, arrs))
+
+
+def plan_add_plain(ctx, levels, c0, c1, pt, o0, o1):
+    lv = np.asarray(levels, np.uint32)
+    arrs = [ptr_array(x) for x in (c0, c1, pt, o0, o1)]
+    return Plan(lib.gf_plan_add_plain(ctx.h, len(levels), lv.ctypes.data_as(u32p), *arrs), keep=(lv, arrs))
+
+
+def mul_plain(ctx, level, c0, c1, pt, o0, o1, stream):
+    check(lib.gf_mul_plain(ctx.h, level, c0, c1, pt, o0, o1, stream.s))
+
+
+def add_plain(ctx, level, c0, c1, pt, o0, o1, stream):
+    check(lib.gf_add_plain(ctx.h, level, c0, c1, pt, o0, o1, stream.s))
+
+
+def rotate_hoisted(ctx, level, c0, c1, keys, ks, o0, o1, ws_base, ws_rot, stream):
+    kk = np.asarray(ks, np.uint64)
+    check(lib.gf_rotate_hoisted(ctx.h, level, c0, c1, len(ks), ptr_array([k.h for k in keys]),
+                                kk.ctypes.data_as(u64p), ptr_array(o0), ptr_array(o1), ws_base, ptr_array(ws_rot),
+                                stream.s))
+
+
+def linear_transform(ctx, level, c0, c1, keys, ks, pts, o0, o1, ws_base, ws_rot, stream):
+    kk = np.asarray(ks, np.uint64)
+    check(lib.gf_linear_transform(ctx.h, level, c0, c1, len(ks),
+                                  ptr_array([k.h if k is not None else 0 for k in keys]), kk.ctypes.data_as(u64p),
+                                  ptr_array(pts), o0, o1, ws_base, ptr_array(ws_rot), stream.s))
 
 
 def plan_add(ctx, levels, a0, a1, b0, b1, o0, o1):

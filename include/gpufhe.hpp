@@ -173,6 +173,28 @@ void KeySwitch(const uint64_t* d_a, uint32_t towers, uint64_t* d_ba0, uint64_t* 
                const DeviceKSContext& C, DeviceKSWork& W, hipStream_t s);
 void EvalAdd(const DeviceCiphertext& a, const DeviceCiphertext& b, DeviceCiphertext& out,
              const DeviceKSContext& C, hipStream_t s);
+// plaintext ops (src/ptops.cpp:1-26 on the device): d_pt holds a.num_towers()
+// towers in evaluation form (pt_to_eval_host, uploaded); out may be a
+void EvalMultPlain(const DeviceCiphertext& a, const uint64_t* d_pt, DeviceCiphertext& out,
+                   const DeviceKSContext& C, hipStream_t s);
+void EvalAddPlain(const DeviceCiphertext& a, const uint64_t* d_pt, DeviceCiphertext& out,
+                  const DeviceKSContext& C, hipStream_t s);
+// Hoisted rotations: ct.c1 is decomposed once (in W) and rotated by every
+// Galois index ks[j] with the key of rotC[j] (rotation contexts of one
+// parameter set); Wrot: one workspace per index.  outs[j] receives the
+// rotation by ks[j].  Decrypts to the same rotations as Rotate, with other
+// (equally small) noise, since the decomposition is of c1, not of sigma(c1).
+void RotateHoisted(const DeviceCiphertext& ct, const std::vector<uint32_t>& ks,
+                   const std::vector<const DeviceKSContext*>& rotC, std::vector<DeviceCiphertext>& outs,
+                   DeviceKSWork& W, std::vector<DeviceKSWork>& Wrot, hipStream_t s);
+// out = sum_j pt_j * rot_ks[j](ct) (diagonal method; kernels/stage_acc.cu:12-28
+// for all terms at once), rotations hoisted as above; rotC[j] may be null
+// with ks[j] = 1 for the unrotated term (at least one term must have a key).
+// d_pts[j]: ct.num_towers() towers in evaluation form; at most 16 terms.
+// Wrot[j] serves term j (it may be empty for the unrotated term); out may be ct.
+void LinearTransform(const DeviceCiphertext& ct, const std::vector<uint32_t>& ks,
+                     const std::vector<const DeviceKSContext*>& rotC, const std::vector<const uint64_t*>& d_pts,
+                     DeviceCiphertext& out, DeviceKSWork& W, std::vector<DeviceKSWork>& Wrot, hipStream_t s);
 
 // ---- keys, encryption, rotation (include/keygen.h:6-45) ----
 struct KeyPairHost {
@@ -221,6 +243,9 @@ void pt_to_eval_host(std::vector<uint64_t>& out, const std::vector<int64_t>& m, 
 void ct_add_ct_host(std::vector<uint64_t>& a0, std::vector<uint64_t>& a1, const std::vector<uint64_t>& b0,
                     const std::vector<uint64_t>& b1, uint32_t towers, uint32_t n,
                     const std::vector<uint64_t>& mod);
+// src/ptops.cpp:20-26 -- (c0 pt, c1 pt), pt in evaluation form
+void ct_mul_pt_host(std::vector<uint64_t>& c0, std::vector<uint64_t>& c1, const std::vector<uint64_t>& ptEval,
+                    uint32_t towers, uint32_t n, const std::vector<uint64_t>& mod);
 
 // ---- parameters (src/params.cpp:29-75) ----
 void native_primes(std::vector<uint64_t>& out, uint32_t count, uint32_t bits, uint32_t n,

@@ -136,6 +136,23 @@ int gf_rotate(gf_ctx* c, gf_key* rk, uint32_t level, uint64_t k, const uint64_t*
 /* componentwise sum (EvalAdd) */
 int gf_add(gf_ctx* c, uint32_t level, const uint64_t* a0, const uint64_t* a1, const uint64_t* b0,
            const uint64_t* b1, uint64_t* o0, uint64_t* o1, hipStream_t s);
+/* plaintext ops; pt: `level` towers in evaluation form (the reference's pt_eval,
+ * src/ptops.cpp:1-5).  mul: (c0 pt, c1 pt) as ct_mul_pt_host
+ * (src/ptops.cpp:20-26) on the device; add: (c0 + pt, c1).  Outputs may
+ * equal the inputs. */
+int gf_mul_plain(gf_ctx* c, uint32_t level, const uint64_t* c0, const uint64_t* c1, const uint64_t* pt,
+                 uint64_t* o0, uint64_t* o1, hipStream_t s);
+int gf_add_plain(gf_ctx* c, uint32_t level, const uint64_t* c0, const uint64_t* c1, const uint64_t* pt,
+                 uint64_t* o0, uint64_t* o1, hipStream_t s);
+/* single-ciphertext forms of gf_plan_rotate_hoisted / gf_plan_linear_transform
+ * below (same arguments with count = 1; o0/o1/ws_rot hold m entries for the
+ * hoisted rotation) */
+int gf_rotate_hoisted(gf_ctx* c, uint32_t level, const uint64_t* c0, const uint64_t* c1, uint32_t m,
+                      gf_key* const* rks, const uint64_t* h_k, uint64_t* const* o0, uint64_t* const* o1,
+                      uint64_t* ws_base, uint64_t* const* ws_rot, hipStream_t s);
+int gf_linear_transform(gf_ctx* c, uint32_t level, const uint64_t* c0, const uint64_t* c1, uint32_t m,
+                        gf_key* const* rks, const uint64_t* h_k, const uint64_t* const* pts, uint64_t* o0,
+                        uint64_t* o1, uint64_t* ws_base, uint64_t* const* ws_rot, hipStream_t s);
 
 /* ---- batch plans: one launch per stage for `count` operations ----
  * Host arrays of device pointers, one entry per operation; levels may differ.
@@ -175,6 +192,13 @@ gf_plan* gf_plan_add(gf_ctx* c, uint32_t count, const uint32_t* h_levels,
                      const uint64_t* const* a0, const uint64_t* const* a1,
                      const uint64_t* const* b0, const uint64_t* const* b1, uint64_t* const* o0,
                      uint64_t* const* o1);
+/* batch forms of gf_mul_plain / gf_add_plain: one plaintext per operation */
+gf_plan* gf_plan_mul_plain(gf_ctx* c, uint32_t count, const uint32_t* h_levels, const uint64_t* const* c0,
+                           const uint64_t* const* c1, const uint64_t* const* pt, uint64_t* const* o0,
+                           uint64_t* const* o1);
+gf_plan* gf_plan_add_plain(gf_ctx* c, uint32_t count, const uint32_t* h_levels, const uint64_t* const* c0,
+                           const uint64_t* const* c1, const uint64_t* const* pt, uint64_t* const* o0,
+                           uint64_t* const* o1);
 int gf_plan_run(gf_plan* p, hipStream_t s);
 /* the same work with every job (one tower of one stage) as its own launch:
  * the launch structure of the reference's resident path, for comparison */

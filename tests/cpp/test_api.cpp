@@ -251,6 +251,19 @@ static void cpu_cases() {
         same(a0, w0, "c0");
         same(a1, w1, "c1");
     });
+    run("ct_mul_pt_host", [] {
+        Params P(1 << 10, 3, 3, 1);
+        std::mt19937_64 g(6);
+        V a0 = P.rand_towers(3, g), a1 = P.rand_towers(3, g), pt = P.rand_towers(3, g);
+        V w0(a0.size()), w1(a1.size());
+        for (uint32_t t = 0; t < 3; ++t) {
+            or_rns_mult(a0.data() + t * P.n, pt.data() + t * P.n, w0.data() + t * P.n, P.q[t], P.n);
+            or_rns_mult(a1.data() + t * P.n, pt.data() + t * P.n, w1.data() + t * P.n, P.q[t], P.n);
+        }
+        ct_mul_pt_host(a0, a1, pt, 3, P.n, P.q);
+        same(a0, w0, "c0");
+        same(a1, w1, "c1");
+    });
     run("ks_context_create_rejects_bad_parts", [] {
         Params P(1 << 10, 4, 2, 2);
         KeySwitchConstants Kc = P.consts(4);
@@ -365,6 +378,100 @@ static void gpu_cases() {
             ks_work_destroy(W);
             ks_context_destroy(C);
         }
+    });
+
+    run("plaintext_ops_vs_oracle", [] {
+        // src/ptops.cpp gates on the device: (c0 pt, c1 pt) and (c0 + pt, c1)
+        Params P(1 << 12, 5, 5, 1);
+        const uint32_t n = P.n, l = 4;
+        std::mt19937_64 g(21);
+        KeySwitchConstants Kc = P.consts(P.L);
+        auto C = ks_context_create(Kc);
+        const V c0 = P.rand_towers(l, g), c1 = P.rand_towers(l, g), pt = P.rand_towers(l, g);
+        Dev dpt(pt);
+        DeviceCiphertext a(Scheme::CKKS, n, l), out(Scheme::CKKS, n, P.L);
+        a.upload(c0, c1);
+        EvalMultPlain(a, dpt.p, out, C, nullptr);
+        hip_ok(hipDeviceSynchronize());
+        expect(out.num_towers() == l, "EvalMultPlain level");
+        V h0, h1, w0(c0.size()), w1(c1.size());
+        out.to_host(h0, h1);
+        for (uint32_t t = 0; t < l; ++t) {
+            or_rns_mult(c0.data() + t * n, pt.data() + t * n, w0.data() + t * n, P.q[t], n);
+            or_rns_mult(c1.data() + t * n, pt.data() + t * n, w1.data() + t * n, P.q[t], n);
+        }
+        same(h0, w0, "EvalMultPlain c0");
+        same(h1, w1, "EvalMultPlain c1");
+        EvalAddPlain(a, dpt.p, a, C, nullptr);   // in place
+        hip_ok(hipDeviceSynchronize());
+        a.to_host(h0, h1);
+        V s0 = c0;
+        for (uint32_t t = 0; t < l; ++t) or_add_into(s0.data() + t * n, pt.data() + t * n, P.q[t], n);
+        same(h0, s0, "EvalAddPlain c0");
+        same(h1, c1, "EvalAddPlain c1");
+        ks_context_destroy(C);
+    });
+
+    run("rotate_hoisted_and_linear_transform_vs_oracle", [] {
+        // hoisted rotations and the diagonal-method sum through the C++ API,
+        // residue for residue against the oracle's restatements
+        Params P(1 << 12, 6, 3, 2);
+        const uint32_t n = P.n, l = 5;
+        std::mt19937_64 g(31);
+        const uint32_t ks[2] = {5, 125};
+        V av[2], bv[2];
+        std::vector<KeySwitchConstants> Kc(2);
+        std::vector<DeviceKSContext> C;
+        for (int j = 0; j < 2; ++j) {
+            av[j] = P.rand_key(g);
+            bv[j] = P.rand_key(g);
+            Kc[j] = P.consts(P.L);
+            set_key(Kc[j], P, av[j], bv[j]);
+            C.push_back(ks_context_create(Kc[j]));
+        }
+        const V c0 = P.rand_towers(l, g), c1 = P.rand_towers(l, g);
+        DeviceCiphertext ct(Scheme::CKKS, n, l);
+        ct.upload(c0, c1);
+        auto W = ks_work_create(C[0]);
+        std::vector<DeviceKSWork> Wr{ks_work_create(C[0]), ks_work_create(C[1])};
+        std::vector<DeviceCiphertext> outs;
+        outs.emplace_back(Scheme::CKKS, n, l);
+        outs.emplace_back(Scheme::CKKS, n, l);
+        RotateHoisted(ct, {ks[0], ks[1]}, {&C[0], &C[1]}, outs, W, Wr, nullptr);
+        hip_ok(hipDeviceSynchronize());
+        std::vector<V> w0(2, V((size_t)l * n)), w1(2, V((size_t)l * n));
+        const u64* oa[2] = {av[0].data(), av[1].data()};
+        const u64* ob[2] = {bv[0].data(), bv[1].data()};
+        u64* p0[2] = {w0[0].data(), w0[1].data()};
+        u64* p1[2] = {w1[0].data(), w1[1].data()};
+        expect(or_rotate_hoisted(P.P, l, c0.data(), c1.data(), 2, ks, oa, ob, p0, p1) == 0, "oracle hoisted");
+        for (int j = 0; j < 2; ++j) {
+            V h0, h1;
+            outs[j].to_host(h0, h1);
+            same(h0, w0[j], "RotateHoisted c0 " + std::to_string(j));
+            same(h1, w1[j], "RotateHoisted c1 " + std::to_string(j));
+        }
+        // three terms, the first unrotated (no context, no workspace)
+        const V pt0 = P.rand_towers(l, g), pt1 = P.rand_towers(l, g), pt2 = P.rand_towers(l, g);
+        Dev d0(pt0), d1(pt1), d2(pt2);
+        std::vector<DeviceKSWork> Wl{DeviceKSWork{}, Wr[0], Wr[1]};
+        DeviceCiphertext out(Scheme::CKKS, n, l);
+        LinearTransform(ct, {1, ks[0], ks[1]}, {nullptr, &C[0], &C[1]}, {d0.p, d1.p, d2.p}, out, W, Wl, nullptr);
+        hip_ok(hipDeviceSynchronize());
+        const uint32_t lk[3] = {1, ks[0], ks[1]};
+        const u64* la[3] = {nullptr, av[0].data(), av[1].data()};
+        const u64* lb[3] = {nullptr, bv[0].data(), bv[1].data()};
+        const u64* lp[3] = {pt0.data(), pt1.data(), pt2.data()};
+        V x0((size_t)l * n), x1((size_t)l * n);
+        expect(or_linear_transform(P.P, l, c0.data(), c1.data(), 3, lk, la, lb, lp, x0.data(), x1.data()) == 0,
+               "oracle linear transform");
+        V h0, h1;
+        out.to_host(h0, h1);
+        same(h0, x0, "LinearTransform c0");
+        same(h1, x1, "LinearTransform c1");
+        ks_work_destroy(W);
+        for (auto& w : Wr) ks_work_destroy(w);
+        for (auto& c : C) ks_context_destroy(c);
     });
 
     run("keyreuse_lower_level_context", [] {

@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from common import KeySet, decode, encode, make_params, poly_mul_negacyclic_small
+from common import KeySet, decode, encode, make_params, mulmod_vec, poly_mul_negacyclic_small
 
 pytestmark = pytest.mark.gpu
 
@@ -339,6 +339,90 @@ def test_linear_transform_vs_oracle(G, n, L, dnum):
         for i in range(len(levels)):
             assert np.array_equal(do[i][0].download(), want[i][0]), (rep, i)
             assert np.array_equal(do[i][1].download(), want[i][1]), (rep, i)
+
+
+def test_plaintext_ops_vs_oracle(G):
+    """Plaintext multiply and add (src/ptops.cpp:1-26 on the device), batch
+    form at uneven levels and single form in place, against per-tower
+    oracle products and sums."""
+    P = make_params(1 << 12, 6, 3)
+    n = P.n
+    ctx = G.Context(n, P.q, P.p, P.roots, 3)
+    rng = np.random.default_rng(35)
+    levels = [6, 4, 1]
+    cts = [(rand_ct(P, lv, rng), rand_ct(P, lv, rng)) for lv in levels]
+    pts = [rand_ct(P, lv, rng) for lv in levels]
+    dc = [(G.Dev.of(a), G.Dev.of(b)) for a, b in cts]
+    dp = [G.Dev.of(p) for p in pts]
+    om = [(G.Dev(lv * n), G.Dev(lv * n)) for lv in levels]
+    oa = [(G.Dev(lv * n), G.Dev(lv * n)) for lv in levels]
+    pm = G.plan_mul_plain(ctx, levels, [x[0].ptr for x in dc], [x[1].ptr for x in dc], [p.ptr for p in dp],
+                          [x[0].ptr for x in om], [x[1].ptr for x in om])
+    pa = G.plan_add_plain(ctx, levels, [x[0].ptr for x in dc], [x[1].ptr for x in dc], [p.ptr for p in dp],
+                          [x[0].ptr for x in oa], [x[1].ptr for x in oa])
+    s = G.Stream()
+    pm.run(s)
+    pa.run(s)
+    s.sync()
+
+    def per_tower(f, a, b, lv):
+        return np.concatenate([f(a[t * n:(t + 1) * n], b[t * n:(t + 1) * n], P.q[t]) for t in range(lv)])
+
+    addv = lambda a, b, q: ((a.astype(object) + b.astype(object)) % int(q)).astype(np.uint64)  # noqa: E731
+    for i, lv in enumerate(levels):
+        assert np.array_equal(om[i][0].download(), per_tower(mulmod_vec, cts[i][0], pts[i], lv)), i
+        assert np.array_equal(om[i][1].download(), per_tower(mulmod_vec, cts[i][1], pts[i], lv)), i
+        assert np.array_equal(oa[i][0].download(), per_tower(addv, cts[i][0], pts[i], lv)), i
+        assert np.array_equal(oa[i][1].download(), cts[i][1]), i
+    # single-op forms, in place
+    G.mul_plain(ctx, 6, dc[0][0].ptr, dc[0][1].ptr, dp[0].ptr, dc[0][0].ptr, dc[0][1].ptr, s)
+    s.sync()
+    assert np.array_equal(dc[0][0].download(), om[0][0].download())
+    assert np.array_equal(dc[0][1].download(), om[0][1].download())
+    G.add_plain(ctx, 4, dc[1][0].ptr, dc[1][1].ptr, dp[1].ptr, dc[1][0].ptr, dc[1][1].ptr, s)
+    s.sync()
+    assert np.array_equal(dc[1][0].download(), oa[1][0].download())
+    assert np.array_equal(dc[1][1].download(), cts[1][1])
+
+
+def test_single_op_hoisted_and_linear_transform(G):
+    """gf_rotate_hoisted and gf_linear_transform (one ciphertext, plan cached
+    by arguments) against the oracle, called twice."""
+    P = make_params(1 << 12, 6, 3)
+    n, lv = P.n, 5
+    ctx = G.Context(n, P.q, P.p, P.roots, 3)
+    rng = np.random.default_rng(37)
+    ks = [pow(5, 2, 2 * n), pow(5, 9, 2 * n)]
+    keys, hk = [], []
+    for _ in ks:
+        av = np.concatenate([rng.integers(0, int(q), n, dtype=np.uint64) for _ in range(3) for q in P.mods])
+        bv = np.concatenate([rng.integers(0, int(q), n, dtype=np.uint64) for _ in range(3) for q in P.mods])
+        keys.append((av, bv))
+        hk.append(G.Key(ctx, av, bv, 3))
+    c0, c1 = rand_ct(P, lv, rng), rand_ct(P, lv, rng)
+    d0, d1 = G.Dev.of(c0), G.Dev.of(c1)
+    outs = [(G.Dev(lv * n), G.Dev(lv * n)) for _ in ks]
+    wb = ctx.workspace()
+    wr = [ctx.workspace() for _ in ks]
+    s = G.Stream()
+    want = P.rotate_hoisted(lv, c0, c1, ks, keys)
+    for _ in range(2):
+        G.rotate_hoisted(ctx, lv, d0.ptr, d1.ptr, hk, ks, [o[0].ptr for o in outs], [o[1].ptr for o in outs],
+                         wb.ptr, [w.ptr for w in wr], s)
+        s.sync()
+        for j in range(len(ks)):
+            assert np.array_equal(outs[j][0].download(), want[j][0]), j
+            assert np.array_equal(outs[j][1].download(), want[j][1]), j
+    pts = [rand_ct(P, lv, rng) for _ in range(3)]
+    dp = [G.Dev.of(p) for p in pts]
+    o0, o1 = G.Dev(lv * n), G.Dev(lv * n)
+    lk = [1] + ks
+    w0, w1 = P.linear_transform(lv, c0, c1, lk, [None] + keys, pts)
+    for _ in range(2):
+        G.linear_transform(ctx, lv, d0.ptr, d1.ptr, [None] + hk, lk, [p.ptr for p in dp], o0.ptr, o1.ptr, wb.ptr,
+                           [0] + [w.ptr for w in wr], s)
+        s.sync()
+        assert np.array_equal(o0.download(), w0) and np.array_equal(o1.download(), w1)
 
 
 def test_linear_transform_rejects_keyless_rotation(G):

@@ -10,7 +10,9 @@ operation at a time (no graph) give the per-op rate.  The CPU baseline is the
 oracle's single-threaded EvalMult+Rescale on the same workload (bounded
 sample).  Prints one JSON line.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--sweep]
+Batches 16 and 256 are measured beside B = 64 (batch_sweep_ops_s).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--sweep 16,256]
 """
 import argparse
 import json
@@ -426,7 +428,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--batch", type=int, default=64)
-    ap.add_argument("--sweep", action="store_true", help="also measure batch 16 and 256")
+    ap.add_argument("--sweep", default="16,256",
+                    help="other batch sizes to measure beside --batch (comma list; empty skips)")
     ap.add_argument("--perop-ops", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -499,8 +502,8 @@ def main():
     log("per-op, one launch per tower and stage: %.1f ops/s (%.3f ms/op)" % (perjob_ops, perjob_ms))
 
     sweep = {}
-    if args.sweep:
-        for b in (16, 256):
+    if args.sweep and not args.no_extra:
+        for b in [int(x) for x in args.sweep.split(",") if x.strip()]:
             r = run_batched(G, ctx, key, b, max(3, args.steps // 2), 2, stream)
             sweep[str(b)] = round(b * max(3, args.steps // 2) / (r["ms_total"] / 1e3), 1)
             log("sweep B=%d: %.1f ops/s" % (b, sweep[str(b)]))

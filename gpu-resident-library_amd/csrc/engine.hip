@@ -222,8 +222,11 @@ WsLayout WsLayout::make(const Ctx& C) {
 size_t Ctx::ws_words() const { return WsLayout::make(*this).total * (size_t)n; }
 
 uint32_t Ctx::autotune(hipStream_t s, float* best_ms) {
-    // 96 towers cycling through every modulus, forward then inverse, in place
-    const uint32_t towers = 96;
+    // 256 MiB of towers (enough workgroups to fill the chip several times
+    // over, as the batched stages do) cycling through every modulus, forward
+    // then inverse, in place; each radix is timed twice, interleaved, and
+    // keeps its faster round
+    const uint32_t towers = std::max<uint32_t>(96, (1u << 25) / n);
     DevBuf buf((size_t)towers * n * 8);
     std::vector<NttJob> fj, ij;
     for (uint32_t t = 0; t < towers; ++t) {
@@ -242,25 +245,27 @@ uint32_t Ctx::autotune(hipStream_t s, float* best_ms) {
     hip_check(hipEventCreate(&e1), "event");
     uint32_t best = geom.rmax;
     float bms = 1e30f;
-    for (uint32_t r = 2; r <= 4; ++r) {
-        NttGeom g = make_geom(n, r);
-        launch_ntt(dj.as<NttJob>(), towers, g, true, s);   // warm
-        launch_ntt(dj.as<NttJob>() + towers, towers, g, false, s);
-        hip_check(hipEventRecord(e0, s), "event");
-        for (int rep = 0; rep < 5; ++rep) {
-            launch_ntt(dj.as<NttJob>(), towers, g, true, s);
+    for (int round = 0; round < 2; ++round)
+        for (uint32_t r = 2; r <= 4; ++r) {
+            NttGeom g = make_geom(n, r);
+            launch_ntt(dj.as<NttJob>(), towers, g, true, s);   // warm
             launch_ntt(dj.as<NttJob>() + towers, towers, g, false, s);
+            hip_check(hipEventRecord(e0, s), "event");
+            for (int rep = 0; rep < 3; ++rep) {
+                launch_ntt(dj.as<NttJob>(), towers, g, true, s);
+                launch_ntt(dj.as<NttJob>() + towers, towers, g, false, s);
+            }
+            hip_check(hipEventRecord(e1, s), "event");
+            hip_check(hipEventSynchronize(e1), "event sync");
+            float ms = 0;
+            hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+            if (ms < bms) { bms = ms; best = r; }
         }
-        hip_check(hipEventRecord(e1, s), "event");
-        hip_check(hipEventSynchronize(e1), "event sync");
-        float ms = 0;
-        hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
-        if (ms < bms) { bms = ms; best = r; }
-    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     geom.rmax = best;
-    if (best_ms) *best_ms = bms / 5;
+    // reported per 96 towers, forward and inverse
+    if (best_ms) *best_ms = bms / 3 * 96.0f / (float)towers;
     return best;
 }
 

@@ -14,6 +14,12 @@ namespace gf {
 // ------------------------------------------------------------------ NTT ---
 struct DevSync {
     __device__ void operator()() const { __syncthreads(); }
+    // LDS words written by this wave become visible to this wave's lanes
+    __device__ void wave() const {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
 };
 struct JobsDev {
     const NttJob* p;

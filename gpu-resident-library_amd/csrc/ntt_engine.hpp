@@ -137,12 +137,24 @@ struct TileDesc {
     bool pad;           // element k of a sub-transform at padded(k) (KS must be 1)
 };
 
+// Element access of a register pass.  A pass names each element by its
+// sub-transform s, its index k inside the sub-transform and its LDS slot `at`;
+// the access policy decides where the value comes from or goes to.
+struct LdsIn {
+    const uint64_t* a;
+    GF_HD uint64_t operator()(uint32_t, uint32_t, uint32_t at) const { return a[at]; }
+};
+struct LdsOut {
+    uint64_t* a;
+    GF_HD void operator()(uint32_t, uint32_t, uint32_t at, uint64_t v) const { a[at] = v; }
+};
+
 // One radix-2^R pass (stages ST0 .. ST0+R-1 of an S-stage transform) over all
 // units of the tile owned by this thread; fully unrolled, the R-stage unit
-// stays in registers.
-template <bool FWD, uint32_t S, uint32_t ST0, uint32_t R, class TW>
-GF_HD void tile_pass(uint64_t* a, const TileDesc& d, const TW& tw, uint64_t q, uint32_t tid,
-                     uint32_t nthr) {
+// stays in registers.  `in` supplies the unit's values, `out` takes them.
+template <bool FWD, uint32_t S, uint32_t ST0, uint32_t R, class TW, class IN, class OUT>
+GF_HD void tile_pass(const TileDesc& d, const TW& tw, uint64_t q, uint32_t tid, uint32_t nthr,
+                     const IN& in, const OUT& out) {
     constexpr uint32_t LOGTL = S - ST0 - R;   // log2(t_last)
     constexpr uint32_t TL = 1u << LOGTL;
     constexpr uint32_t LOGPS = S - R;         // log2(units per sub-transform)
@@ -162,14 +174,14 @@ GF_HD void tile_pass(uint64_t* a, const TileDesc& d, const TW& tw, uint64_t q, u
         const uint32_t l = rest & (TL - 1);
         const uint32_t g = rest >> LOGTL;
         const uint32_t k0 = (g << (S - ST0)) + l;
-        uint64_t* base = a + (uint64_t)s * d.SS;
+        const uint32_t sbase = s * d.SS;
         uint32_t at[R8];
 #pragma unroll
         for (uint32_t u = 0; u < R8; ++u)
-            at[u] = d.pad ? padded(k0 + u * TL) : (uint32_t)((uint64_t)(k0 + u * TL) * d.KS);
+            at[u] = sbase + (d.pad ? padded(k0 + u * TL) : (k0 + u * TL) * d.KS);
         uint64_t x[R8];
 #pragma unroll
-        for (uint32_t u = 0; u < R8; ++u) x[u] = base[at[u]];
+        for (uint32_t u = 0; u < R8; ++u) x[u] = in(s, k0 + u * TL, at[u]);
         const uint32_t G = d.g0 + s * d.gstep;
         if (FWD) {
 #pragma unroll
@@ -204,19 +216,44 @@ GF_HD void tile_pass(uint64_t* a, const TileDesc& d, const TW& tw, uint64_t q, u
             }
         }
 #pragma unroll
-        for (uint32_t u = 0; u < R8; ++u) base[at[u]] = x[u];
+        for (uint32_t u = 0; u < R8; ++u) out(s, k0 + u * TL, at[u], x[u]);
     }
 }
 
-// All passes of one S-stage phase, chunk order fixed at compile time.
-template <bool FWD, uint32_t S, uint32_t RM, uint32_t I, class TW, class Sync>
+// Pass I (in execution order) of an S-stage phase: its chunk, and whether its
+// units are runs of consecutive elements with stride TL >= 8, so that a wave
+// touching element k of 8 consecutive units reads 64 contiguous bytes.
+template <bool FWD, uint32_t S, uint32_t RM, uint32_t I>
+struct PassOf {
+    static constexpr uint32_t NC = num_chunks(S, RM);
+    static constexpr uint32_t C = FWD ? I : NC - 1 - I;
+    static constexpr uint32_t ST0 = chunk_start(S, RM, C), R = chunk_size(S, RM, C);
+    static constexpr bool wide_runs = S - ST0 - R >= 3;
+};
+
+// All passes of one S-stage phase, chunk order fixed at compile time.  The
+// first pass reads through `gin` and the last writes through `gout` (global
+// memory, or LDS when the caller stages that end); the passes in between go
+// through LDS with a barrier after each.  A one-pass phase never touches LDS.
+template <bool FWD, uint32_t S, uint32_t RM, uint32_t I, class TW, class Sync, class GIN, class GOUT>
 GF_HD void tile_chunks(uint64_t* a, const TileDesc& d, const TW& tw, uint64_t q, uint32_t tid,
-                       uint32_t nthr, Sync sync) {
-    if constexpr (I < num_chunks(S, RM)) {
-        constexpr uint32_t C = FWD ? I : num_chunks(S, RM) - 1 - I;
-        tile_pass<FWD, S, chunk_start(S, RM, C), chunk_size(S, RM, C)>(a, d, tw, q, tid, nthr);
-        sync();
-        tile_chunks<FWD, S, RM, I + 1>(a, d, tw, q, tid, nthr, sync);
+                       uint32_t nthr, Sync sync, const GIN& gin, const GOUT& gout) {
+    using P = PassOf<FWD, S, RM, I>;
+    if constexpr (I < P::NC) {
+        constexpr bool first = I == 0, last = I + 1 == P::NC;
+        if constexpr (first && last) {
+            tile_pass<FWD, S, P::ST0, P::R>(d, tw, q, tid, nthr, gin, gout);
+        } else if constexpr (first) {
+            tile_pass<FWD, S, P::ST0, P::R>(d, tw, q, tid, nthr, gin, LdsOut{a});
+        } else if constexpr (last) {
+            tile_pass<FWD, S, P::ST0, P::R>(d, tw, q, tid, nthr, LdsIn{a}, gout);
+        } else {
+            tile_pass<FWD, S, P::ST0, P::R>(d, tw, q, tid, nthr, LdsIn{a}, LdsOut{a});
+        }
+        if constexpr (!last) {
+            sync();
+            tile_chunks<FWD, S, RM, I + 1>(a, d, tw, q, tid, nthr, sync, gin, gout);
+        }
     }
 }
 
@@ -237,27 +274,112 @@ GF_HD uint64_t ntt_final_ep(uint64_t v, const NttJob& j, uint64_t gi) {
 }
 
 // ---- phase bodies (shared by the kernels and the host emulation) ----
+// Every phase reads its tile from HBM in its first register pass and writes it
+// in its last, straight from registers, wherever those accesses come in runs
+// of consecutive words; LDS then only carries the exchange between passes.
+// All reads of a tile precede the barrier before its first write, so in-place
+// transforms (in == out) are safe.
+
+// Phase A access: element k of column s of the tile at word (k << logN2) + b0 + s.
+// Lanes run over consecutive columns, so both ends are always direct.
+struct ColIn {
+    const GF_GLOBAL uint64_t* src;
+    uint32_t logN2, b0;
+    GF_HD uint64_t operator()(uint32_t s, uint32_t k, uint32_t) const { return src[(k << logN2) + b0 + s]; }
+};
+template <bool FWD>
+struct ColOut {
+    GF_GLOBAL uint64_t* dst;
+    uint32_t logN2, b0;
+    const NttJob& j;
+    // forward phase A hands lazy values in [0,4q) to phase B
+    GF_HD void operator()(uint32_t s, uint32_t k, uint32_t, uint64_t v) const {
+        dst[(k << logN2) + b0 + s] = FWD ? v : ntt_final<false>(v, j);
+    }
+};
+
 // Phase A: tile = columns [b0, b0+C) x all N1 = 2^S1 rows.
 template <bool FWD, bool PAIRED, uint32_t S1, uint32_t RM, class Sync>
 GF_HD void phaseA_body(const NttJob& j, const NttGeom& g, uint32_t tile, uint64_t* lds,
                        uint32_t tid, uint32_t nthr, Sync sync) {
-    constexpr uint32_t N1 = 1u << S1, logC = NTT_LOG_TILE - S1, C = 1u << logC;
-    const uint32_t logN2 = g.logN2;
+    constexpr uint32_t logC = NTT_LOG_TILE - S1, C = 1u << logC;
     const uint32_t b0 = tile * C;
-    const GF_GLOBAL uint64_t* src = gcast(FWD ? j.in : j.out);   // forward: first phase; inverse: second
-    GF_GLOBAL uint64_t* dst = gcast(j.out);
-    for (uint32_t idx = tid; idx < C * N1; idx += nthr) {
-        const uint32_t row = idx >> logC, c = idx & (C - 1);
-        lds[idx] = src[((uint64_t)row << logN2) + b0 + c];
-    }
-    sync();
     const TileDesc d{logC, C, 1, 1, 0, true, false};
-    tile_chunks<FWD, S1, RM, 0>(lds, d, tw_of<PAIRED>(j), j.q, tid, nthr, sync);
-    for (uint32_t idx = tid; idx < C * N1; idx += nthr) {
-        const uint32_t row = idx >> logC, c = idx & (C - 1);
-        const uint64_t v = lds[idx];
-        // forward phase A hands lazy values in [0,4q) to phase B
-        dst[((uint64_t)row << logN2) + b0 + c] = FWD ? v : ntt_final<false>(v, j);
+    // forward: phase A is the first phase (reads in); inverse: the second
+    const ColIn gin{gcast(FWD ? j.in : j.out), g.logN2, b0};
+    const ColOut<FWD> gout{gcast(j.out), g.logN2, b0, j};
+    tile_chunks<FWD, S1, RM, 0>(lds, d, tw_of<PAIRED>(j), j.q, tid, nthr, sync, gin, gout);
+}
+
+// Row access (phase B, single phase): element k of row s at word base + s*N + k.
+struct RowIn {
+    const GF_GLOBAL uint64_t* src;   // tile start
+    uint32_t N;
+    GF_HD uint64_t operator()(uint32_t s, uint32_t k, uint32_t) const { return src[s * N + k]; }
+};
+// final store of a row tile: forward values through the epilogue (FIN = 2),
+// inverse values through the n^-1 multiply (FIN = 1) or lazy (FIN = 0)
+template <int FIN>
+struct RowOut {
+    GF_GLOBAL uint64_t* dst;         // tower start
+    uint32_t off, N;
+    const NttJob& j;
+    GF_HD void operator()(uint32_t s, uint32_t k, uint32_t, uint64_t v) const {
+        const uint32_t gi = off + s * N + k;
+        dst[gi] = FIN == 2 ? ntt_final_ep(v, j, gi) : FIN == 1 ? ntt_final<false>(v, j) : v;
+    }
+};
+
+// Word indices of the units a wave owns in a row-tile pass of radix 2^R whose
+// units are short runs (TL < 8).  The loop of tile_pass gives a wave 64
+// consecutive units per round, and those hold 64 << R consecutive words of the
+// tile (units with the same group are interleaved by TL and adjacent in
+// number), so f(idx) visits exactly the wave's own words with lanes on
+// consecutive idx: a coalesced copy that needs only a wave-level fence.
+// A tile of fewer than 64 units leaves the wave's last round partial: the
+// words past the tile then belong to no unit and are skipped.
+template <uint32_t R, uint32_t WORDS, class F>
+GF_HD void wave_units(uint32_t tid, uint32_t nthr, F f) {
+    const uint32_t W = nthr < 64 ? nthr : 64;
+    const uint32_t lane = tid & (W - 1);
+    constexpr uint32_t U = WORDS >> R;
+    for (uint32_t u0 = tid - lane; u0 < U; u0 += nthr) {
+#pragma unroll
+        for (uint32_t m = 0; m < (1u << R); ++m) {
+            const uint32_t idx = (u0 << R) + m * W + lane;
+            if (U % 64 == 0 || idx < WORDS) f(idx);
+        }
+    }
+}
+
+// A row tile whose first or last pass holds short runs (TL < 8) stages that
+// end through LDS, wave by wave (no block barrier).
+template <bool FWD, bool PAIRED, uint32_t S, uint32_t RM, uint32_t ROWS, int FIN, class Sync>
+GF_HD void row_tile(const NttJob& j, const TileDesc& d, const GF_GLOBAL uint64_t* src, GF_GLOBAL uint64_t* dst,
+                    uint32_t off, uint64_t* lds, uint32_t tid, uint32_t nthr, Sync sync) {
+    constexpr uint32_t N = 1u << S, NC = num_chunks(S, RM), WORDS = ROWS * N;
+    using P0 = PassOf<FWD, S, RM, 0>;
+    using PL = PassOf<FWD, S, RM, NC - 1>;
+    constexpr bool in_direct = P0::wide_runs;
+    constexpr bool out_direct = PL::wide_runs;
+    if constexpr (!in_direct) {
+        wave_units<P0::R, WORDS>(tid, nthr, [&](uint32_t idx) { lds[padded(idx)] = src[off + idx]; });
+        sync.wave();
+    }
+    const RowIn gin{src + off, N};
+    const RowOut<FIN> gout{dst, off, N, j};
+    const auto tw = tw_of<PAIRED>(j);
+    if constexpr (in_direct && out_direct)
+        tile_chunks<FWD, S, RM, 0>(lds, d, tw, j.q, tid, nthr, sync, gin, gout);
+    else if constexpr (in_direct)
+        tile_chunks<FWD, S, RM, 0>(lds, d, tw, j.q, tid, nthr, sync, gin, LdsOut{lds});
+    else if constexpr (out_direct)
+        tile_chunks<FWD, S, RM, 0>(lds, d, tw, j.q, tid, nthr, sync, LdsIn{lds}, gout);
+    else
+        tile_chunks<FWD, S, RM, 0>(lds, d, tw, j.q, tid, nthr, sync, LdsIn{lds}, LdsOut{lds});
+    if constexpr (!out_direct) {
+        sync.wave();
+        wave_units<PL::R, WORDS>(tid, nthr, [&](uint32_t idx) { gout(0, idx, 0, lds[padded(idx)]); });
     }
 }
 
@@ -268,30 +390,18 @@ GF_HD void phaseB_body(const NttJob& j, const NttGeom& g, uint32_t tile, uint64_
     constexpr uint32_t N2 = 1u << S2, logR = NTT_LOG_TILE - S2, R = 1u << logR;
     const uint32_t N1 = 1u << g.logN1;
     const uint32_t a0 = tile * R;
-    const GF_GLOBAL uint64_t* src = gcast(FWD ? j.out : j.in);   // forward: second phase; inverse: first
-    GF_GLOBAL uint64_t* dst = gcast(j.out);
-    const uint64_t off = (uint64_t)a0 << S2;
-    for (uint32_t idx = tid; idx < R * N2; idx += nthr) lds[padded(idx)] = src[off + idx];
-    sync();
     const TileDesc d{logR, 1, padded(N2), N1 + a0, 1, false, true};
-    tile_chunks<FWD, S2, RM, 0>(lds, d, tw_of<PAIRED>(j), j.q, tid, nthr, sync);
-    // inverse phase B hands lazy values in [0,2q) to phase A
-    for (uint32_t idx = tid; idx < R * N2; idx += nthr)
-        dst[off + idx] = FWD ? ntt_final_ep(lds[padded(idx)], j, off + idx) : lds[padded(idx)];
+    // forward: phase B is the second phase (reads out); inverse: the first.
+    // Inverse phase B hands lazy values in [0,2q) to phase A.
+    row_tile<FWD, PAIRED, S2, RM, R, FWD ? 2 : 0>(j, d, gcast(FWD ? j.out : j.in), gcast(j.out), a0 << S2, lds,
+                                                  tid, nthr, sync);
 }
 
 // Single phase: the whole tower (n = 2^S <= NTT_SINGLE_MAX) is one tile.
 template <bool FWD, bool PAIRED, uint32_t S, uint32_t RM, class Sync>
 GF_HD void single_body(const NttJob& j, uint64_t* lds, uint32_t tid, uint32_t nthr, Sync sync) {
-    constexpr uint32_t n = 1u << S;
-    const GF_GLOBAL uint64_t* src = gcast(j.in);
-    GF_GLOBAL uint64_t* dst = gcast(j.out);
-    for (uint32_t idx = tid; idx < n; idx += nthr) lds[padded(idx)] = src[idx];
-    sync();
-    const TileDesc d{0, 1, padded(n), 1, 0, false, true};
-    tile_chunks<FWD, S, RM, 0>(lds, d, tw_of<PAIRED>(j), j.q, tid, nthr, sync);
-    for (uint32_t idx = tid; idx < n; idx += nthr)
-        dst[idx] = FWD ? ntt_final_ep(lds[padded(idx)], j, idx) : ntt_final<false>(lds[padded(idx)], j);
+    const TileDesc d{0, 1, padded(1u << S), 1, 0, false, true};
+    row_tile<FWD, PAIRED, S, RM, 1, FWD ? 2 : 1>(j, d, gcast(j.in), gcast(j.out), 0, lds, tid, nthr, sync);
 }
 
 // Calls f(integral_constant S1, integral_constant S2) for the phase split of

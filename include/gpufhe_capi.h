@@ -224,7 +224,8 @@ int gf_plan_run_step_kernel(gf_plan* p, uint32_t i, uint32_t which, hipStream_t 
 
 /* ---- tuning ----
  * NTT register radix log2 (2, 3 or 4) used by plans built afterwards; the
- * autotuner times each on a synthetic tower batch and keeps the fastest. */
+ * autotuner times each on a synthetic 256 MiB tower batch (forward and
+ * inverse, in place) and keeps the fastest; ms_out is per 96 towers. */
 int gf_ctx_set_ntt_radix(gf_ctx* c, uint32_t rmax);
 uint32_t gf_ctx_get_ntt_radix(const gf_ctx* c);
 int gf_ctx_autotune(gf_ctx* c, hipStream_t s, uint32_t* rmax_out, float* ms_out);

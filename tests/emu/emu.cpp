@@ -13,6 +13,7 @@ using namespace gf;
 namespace {
 struct NoSync {
     void operator()() const {}
+    void wave() const {}
 };
 
 template <bool PAIRED, uint32_t RM>

@@ -91,6 +91,7 @@ Ctx::Ctx(uint32_t n_, const std::vector<uint64_t>& q, const std::vector<uint64_t
         if ((m - 1) % (2ull * n)) throw std::runtime_error("gpufhe: modulus is not 1 mod 2n");
         qmax = m > qmax ? m : qmax;
     }
+    wide = (qmax >> 60) ? 1u : 0u;
     // one REDC of a sum of up to max(alpha, K, dnum) products must stay below q 2^64
     const uint32_t terms = alpha > K ? (alpha > dnum ? alpha : dnum) : (K > dnum ? K : dnum);
     if ((unsigned __int128)terms * qmax >= ((unsigned __int128)1 << 64))
@@ -513,7 +514,7 @@ void PlanBuilder::modup(uint32_t l, const uint64_t* d, uint64_t* ws, const uint6
         uint64_t* dp = dig + (size_t)pi * W * n;
         for (uint32_t j0 = 0; j0 < P.sc; j0 += CONV_CHUNK)
             conv_up_.push_back(ConvJob{x + (size_t)P.st * n, dp, P.d_tab, P.d_tq, P.d_tqneg, P.sp, P.sc,
-                                       j0, j0 + CONV_CHUNK < P.sc ? j0 + CONV_CHUNK : P.sc});
+                                       j0, j0 + CONV_CHUNK < P.sc ? j0 + CONV_CHUNK : P.sc, C.wide});
         for (uint32_t j = 0; j < P.sc; ++j) {
             const uint32_t r = P.compl_rows[j];
             ntt_dig_.push_back(NttJob{dp + (size_t)j * n, dp + (size_t)j * n, C.rt[r].fw2, nullptr,
@@ -594,7 +595,7 @@ void PlanBuilder::moddown_tail(uint32_t l, uint64_t* ws, uint64_t* ba0, uint64_t
         uint64_t* cv = conv + (size_t)c * l * n;
         for (uint32_t j0 = 0; j0 < l; j0 += CONV_CHUNK)
             conv_dn_.push_back(ConvJob{xp + (size_t)c * K * n, cv, lv.d_tabMD, lv.d_tqMD, lv.d_tqnegMD, K, l,
-                                       j0, j0 + CONV_CHUNK < l ? j0 + CONV_CHUNK : l});
+                                       j0, j0 + CONV_CHUNK < l ? j0 + CONV_CHUNK : l, C.wide});
         for (uint32_t i = 0; i < l; ++i) {
             ntt_conv_.push_back(NttJob{cv + (size_t)i * n, cv + (size_t)i * n, C.rt[i].fw2, nullptr,
                                        C.mods[i], 0, 0});
@@ -637,12 +638,12 @@ void PlanBuilder::evalmult_rescale_fused(uint32_t l, const uint64_t* c0a, const 
                                  C.mods[last], C.rt[last].ninv, C.rt[last].ninvp});
         drop_.push_back(DropJob{xp + (size_t)c * K * n, lv.d_tabMD + last, dt + (size_t)c * n, dr + (size_t)c * n,
                                 drop + (size_t)c * n, C.rc[last].q, C.rc[last].qneg, lv.pinv[last],
-                                lv.pinvp[last], K, l});
+                                lv.pinvp[last], K, l, C.wide});
         uint64_t* vc = v + (size_t)c * last * n;
         for (uint32_t j0 = 0; j0 < last; j0 += CONV_CHUNK)
             rs_.push_back(RsJob{xp + (size_t)c * K * n, drop + (size_t)c * n, vc, lv.d_tabRS, lv.d_rstg,
                                 C.mods[last], C.mods[last] >> 1, K, last, j0,
-                                j0 + CONV_CHUNK < last ? j0 + CONV_CHUNK : last});
+                                j0 + CONV_CHUNK < last ? j0 + CONV_CHUNK : last, C.wide});
         for (uint32_t i = 0; i < last; ++i) {
             NttJob J{vc + (size_t)i * n, outs[c] + (size_t)i * n, C.rt[i].fw2, nullptr, C.mods[i], 0, 0};
             J.e0 = t[c] + (size_t)i * n;

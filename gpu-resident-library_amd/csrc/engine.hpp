@@ -84,6 +84,7 @@ public:
     std::vector<LevelConst> lv;               // index l = 1..L (0 unused)
     DevBuf consts;                            // all per-level device constants
     bool fused = true;                        // EvalMult+Rescale with merged ModDown/rescale
+    uint32_t wide = 0;                        // 1: a modulus is 2^60 or more (generic conversion sums)
     uint32_t row_q(uint32_t i) const { return i; }
     uint32_t row_p(uint32_t k) const { return L + k; }
     uint32_t num_parts(uint32_t l) const { return (l + alpha - 1) / alpha; }

@@ -38,6 +38,7 @@ struct ConvJob {
     const uint64_t* tq;     // target moduli [ntgt]
     const uint64_t* tqneg;  // target -q^-1 mod 2^64 [ntgt]
     uint32_t nsrc, ntgt, j0, j1;
+    uint32_t wide;          // 1: some modulus is 2^60 or more (full 128-bit products)
 };
 
 // o0 = sum_p dig[p] * kb[p], o1 = sum_p dig[p] * ka[p] (keys in Montgomery form)
@@ -103,6 +104,7 @@ struct DropJob {
     uint64_t* out;
     uint64_t q, qneg, pinv, pinvp;
     uint32_t K, stride;
+    uint32_t wide;          // as ConvJob::wide
 };
 
 // per surviving tower constants of the fused ModDown + rescale prologue
@@ -121,6 +123,7 @@ struct RsJob {
     const RsTarget* tg;
     uint64_t q_old, half;
     uint32_t K, ntgt, j0, j1;
+    uint32_t wide;          // as ConvJob::wide
 };
 
 // one tower of a freshly sampled small polynomial: dst = src mod q (signed src)

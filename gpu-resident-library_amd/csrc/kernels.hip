@@ -145,26 +145,41 @@ __device__ __forceinline__ void with_nsrc(uint32_t ns, F&& f) {
     }
 }
 
-// sum_s x_s tab[s*stride + t] reduced mod q (Montgomery constants in tab);
-// NS = 0: runtime count, sources read from memory
+// The sources of one element for a templated count NS, split at bit 30 for
+// dot30 (contexts whose moduli are all below 2^60); NS = 0 holds nothing.
+template <uint32_t NS>
+struct SrcCols {
+    uint32_t lo[NS > 0 ? NS : 1], hi[NS > 0 ? NS : 1];
+    __device__ __forceinline__ void load(const uint64_t* src, uint64_t n, uint32_t i) {
+#pragma unroll
+        for (uint32_t s = 0; s < NS; ++s) {
+            const uint64_t v = src[(uint64_t)s * n + i];
+            lo[s] = split30_lo(v);
+            hi[s] = split30_hi(v);
+        }
+    }
+};
+
+// sum_s x_s tab[s*stride + t] as 128 bits (Montgomery constants in tab, one
+// REDC by the caller).  NS > 0: column sums over the pre-split sources;
+// NS = 0 (more than 8 sources, or a modulus of 2^60 or more): runtime count,
+// full 128-bit products, sources read from memory.
 template <uint32_t NS>
 __device__ __forceinline__ U128 dot_src(const uint64_t* src, uint64_t n, uint32_t i, uint32_t ns, cu64p tab,
-                                        uint32_t stride, uint32_t t, const uint64_t* x) {
-    U128 acc{0, 0};
+                                        uint32_t stride, uint32_t t, const SrcCols<NS>& x) {
     if constexpr (NS > 0) {
-#pragma unroll
-        for (uint32_t s = 0; s < NS; ++s) acc = add128(acc, mul128(x[s], tab[s * stride + t]));
+        return dot30<NS>(x.lo, x.hi, [&](uint32_t s) { return tab[s * stride + t]; });
     } else {
+        U128 acc{0, 0};
         for (uint32_t s = 0; s < ns; ++s) acc = add128(acc, mul128(src[(uint64_t)s * n + i], tab[s * stride + t]));
+        return acc;
     }
-    return acc;
 }
 
 template <uint32_t NS>
 __device__ __forceinline__ void conv_one(const ConvJob& j, uint32_t n, uint32_t i) {
-    uint64_t x[NS > 0 ? NS : 1];
-#pragma unroll
-    for (uint32_t s = 0; s < NS; ++s) x[s] = j.src[(uint64_t)s * n + i];
+    SrcCols<NS> x;
+    x.load(j.src, n, i);
     const cu64p tab = uni(j.tab), tq = uni(j.tq), tqn = uni(j.tqneg);
     for (uint32_t t = j.j0; t < j.j1; ++t) {
         const U128 acc = dot_src<NS>(j.src, n, i, j.nsrc, tab, j.ntgt, t, x);
@@ -175,7 +190,7 @@ __device__ __forceinline__ void conv_one(const ConvJob& j, uint32_t n, uint32_t 
 __global__ __launch_bounds__(PW_THREADS) void k_conv(const ConvJob* __restrict__ jobs,
                                                      uint32_t off, uint32_t n) {
     const ConvJob j = jobs[off + blockIdx.y];
-    with_nsrc(j.nsrc, [&](auto NSc) {
+    with_nsrc(j.wide ? 0 : j.nsrc, [&](auto NSc) {
         constexpr uint32_t NS = decltype(NSc)::value;
         PW_FOR(n) { conv_one<NS>(j, n, i); }
     });
@@ -235,12 +250,11 @@ __global__ __launch_bounds__(PW_THREADS) void k_drop(const DropJob* __restrict__
                                                      uint32_t off, uint32_t n) {
     const DropJob j = jobs[off + blockIdx.y];
     const cu64p tab = uni(j.tab);
-    with_nsrc(j.K, [&](auto Kc) {
+    with_nsrc(j.wide ? 0 : j.K, [&](auto Kc) {
         constexpr uint32_t K = decltype(Kc)::value;
         PW_FOR(n) {
-            uint64_t x[K > 0 ? K : 1];
-#pragma unroll
-            for (uint32_t k = 0; k < K; ++k) x[k] = j.xp[(uint64_t)k * n + i];
+            SrcCols<K> x;
+            x.load(j.xp, n, i);
             const U128 acc = dot_src<K>(j.xp, n, i, j.K, tab, j.stride, 0, x);
             const uint64_t conv = redc(acc, j.q, j.qneg);
             j.out[i] = addmod(j.dt[i], shoup(submod(j.dr[i], conv, j.q), j.pinv, j.pinvp, j.q), j.q);
@@ -250,9 +264,8 @@ __global__ __launch_bounds__(PW_THREADS) void k_drop(const DropJob* __restrict__
 
 template <uint32_t K>
 __device__ __forceinline__ void rs_one(const RsJob& j, uint32_t n, uint32_t i) {
-    uint64_t x[K > 0 ? K : 1];
-#pragma unroll
-    for (uint32_t k = 0; k < K; ++k) x[k] = j.xp[(uint64_t)k * n + i];
+    SrcCols<K> x;
+    x.load(j.xp, n, i);
     const uint64_t dv = j.drop[i];
     const bool hi = dv > j.half;
     const cu64p tab = uni(j.tab);
@@ -286,7 +299,7 @@ __device__ __forceinline__ void rs_one(const RsJob& j, uint32_t n, uint32_t i) {
 __global__ __launch_bounds__(PW_THREADS) void k_rs(const RsJob* __restrict__ jobs, uint32_t off,
                                                    uint32_t n) {
     const RsJob j = jobs[off + blockIdx.y];
-    with_nsrc(j.K, [&](auto Kc) {
+    with_nsrc(j.wide ? 0 : j.K, [&](auto Kc) {
         constexpr uint32_t K = decltype(Kc)::value;
         PW_FOR(n) { rs_one<K>(j, n, i); }
     });

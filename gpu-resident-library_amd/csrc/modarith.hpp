@@ -25,6 +25,26 @@ struct U128 {
     uint64_t lo, hi;
 };
 
+GF_HD uint32_t umulhi32(uint32_t a, uint32_t b) { return (uint32_t)(((uint64_t)a * b) >> 32); }
+
+// The value itself, made opaque to the optimiser on the device: a 64-bit
+// addend assembled from two 32-bit halves then feeds v_mad_u64_u32 as one
+// register pair instead of being split back into a separate 64-bit add.
+// Emits no instruction.
+GF_HD uint64_t opaque64(uint64_t v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm("" : "+v"(v));
+#endif
+    return v;
+}
+
+// a + b of two 32-bit words as a 33-bit value (carry from the add itself)
+GF_HD uint64_t add33(uint32_t a, uint32_t b) {
+    uint32_t s;
+    const uint32_t cy = __builtin_add_overflow(a, b, &s) ? 1u : 0u;
+    return opaque64(((uint64_t)cy << 32) | s);
+}
+
 // full 64x64 -> 128 product from four 32x32 pieces
 GF_HD U128 mul128(uint64_t a, uint64_t b) {
     uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32);
@@ -33,7 +53,7 @@ GF_HD U128 mul128(uint64_t a, uint64_t b) {
     uint64_t t = mad32(a0, b1, p00 >> 32);
     uint64_t u = mad32(a1, b0, (uint32_t)t);
     U128 r;
-    r.hi = mad32(a1, b1, (t >> 32) + (u >> 32));
+    r.hi = mad32(a1, b1, add33((uint32_t)(t >> 32), (uint32_t)(u >> 32)));
     r.lo = (u << 32) | (uint32_t)p00;
     return r;
 }
@@ -42,9 +62,9 @@ GF_HD U128 mul128(uint64_t a, uint64_t b) {
 GF_HD uint64_t mulhi64(uint64_t a, uint64_t b) {
     uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32);
     uint32_t b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
-    uint64_t t = mad32(a0, b1, ((uint64_t)a0 * b0) >> 32);
+    uint64_t t = mad32(a0, b1, umulhi32(a0, b0));
     uint64_t u = mad32(a1, b0, (uint32_t)t);
-    return mad32(a1, b1, (t >> 32) + (u >> 32));
+    return mad32(a1, b1, add33((uint32_t)(t >> 32), (uint32_t)(u >> 32)));
 }
 
 GF_HD U128 add128(U128 a, U128 b) {
@@ -69,19 +89,64 @@ GF_HD uint64_t shoup(uint64_t x, uint64_t w, uint64_t wp, uint64_t q) {
 GF_HD uint64_t shoup_lazy(uint64_t x, uint64_t w, uint64_t wp, uint64_t q) {
     return x * w - mulhi64(x, wp) * q;
 }
-// x in [0, 2m) -> [0, m)
-GF_HD uint64_t csub(uint64_t x, uint64_t m) { return x >= m ? x - m : x; }
-
-GF_HD uint32_t umulhi32(uint32_t a, uint32_t b) { return (uint32_t)(((uint64_t)a * b) >> 32); }
-// floor(a b / 2^64) minus 0, 1 or 2: the a0*b0 product and the low halves of
-// the cross products are dropped (their sum is < 3 * 2^64 / 2^64)
-GF_HD uint64_t mulhi64_approx(uint64_t a, uint64_t b) {
-    const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32), b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
-    return mad32(a1, b1, (uint64_t)umulhi32(a0, b1) + umulhi32(a1, b0));
+// x in [0, 2m) -> [0, m); the select keys on the subtraction's own borrow
+GF_HD uint64_t csub(uint64_t x, uint64_t m) {
+    uint64_t d;
+    return __builtin_sub_overflow(x, m, &d) ? x : d;
 }
-// Shoup with the approximate quotient: x*w mod q + {0, q, 2q, 3q}, in [0, 4q), any x < 2^64
-GF_HD uint64_t shoup_lazy4(uint64_t x, uint64_t w, uint64_t wp, uint64_t q) {
-    return x * w - mulhi64_approx(x, wp) * q;
+
+// Shoup product for the transform butterflies:
+//   quotient  qh = x1 wp1 + floor(x0 wp1 / 2^32) + floor(x1 wp0 / 2^32)
+//             (x = x1 2^32 + x0, wp likewise), which is floor(x wp / 2^64)
+//             minus 0, 1 or 2: the terms left out (x0 wp0 / 2^64 and the two
+//             fractional parts) sum to less than 3;
+//   remainder x w - qh q  computed mod 2^64 as lo64(x w) + lo64(qh nq),
+//             nq = 2^64 - q, in one multiply-add chain.
+// With wp = floor(w 2^64 / q) the exact quotient leaves [0, 2q) for any
+// x < 2^64, so the result is x w mod q + {0, q, 2q, 3q}, in [0, 4q).
+GF_HD uint64_t shoup_lazy4(uint64_t x, uint64_t w, uint64_t wp, uint64_t nq) {
+    const uint32_t x0 = (uint32_t)x, x1 = (uint32_t)(x >> 32);
+    const uint32_t w0 = (uint32_t)w, w1 = (uint32_t)(w >> 32);
+    const uint32_t p0 = (uint32_t)wp, p1 = (uint32_t)(wp >> 32);
+    const uint64_t qh = mad32(x1, p1, add33(umulhi32(x0, p1), umulhi32(x1, p0)));
+    const uint32_t h0 = (uint32_t)qh, h1 = (uint32_t)(qh >> 32);
+    const uint32_t n0 = (uint32_t)nq, n1 = (uint32_t)(nq >> 32);
+    const uint64_t p = (uint64_t)x0 * w0;
+    const uint32_t hi = (uint32_t)(p >> 32) + x0 * w1 + x1 * w0 + h0 * n1 + h1 * n0;
+    return mad32(h0, n0, opaque64(((uint64_t)hi << 32) | (uint32_t)p));
+}
+
+// Exact sum of NS <= 8 products a_s b_s of values below 2^60, as 128 bits.
+// Each operand is split at bit 30 (a = a1 2^30 + a0, a0 < 2^30, a1 < 2^30),
+// and the partial products go to three 64-bit columns with one
+// v_mad_u64_u32 each:
+//   lo = sum a0 b0 < 8 2^60,  mid = sum a0 b1 + a1 b0 < 8 2^61,  hi = sum a1 b1 < 8 2^60,
+// none of which can overflow; the total is lo + mid 2^30 + hi 2^60.  The
+// sources come pre-split (alo, ahi); b(s) returns the s-th constant.
+enum : uint32_t { SPLIT30_MASK = (1u << 30) - 1 };
+GF_HD uint32_t split30_lo(uint64_t v) { return (uint32_t)v & SPLIT30_MASK; }
+GF_HD uint32_t split30_hi(uint64_t v) { return (uint32_t)(v >> 30); }
+template <uint32_t NS, class B>
+GF_HD U128 dot30(const uint32_t* alo, const uint32_t* ahi, B b) {
+    static_assert(NS >= 1 && NS <= 8, "column sums are exact for at most 8 terms");
+    uint64_t lo = 0, mid = 0, hi = 0;
+#pragma unroll
+    for (uint32_t s = 0; s < NS; ++s) {
+        const uint64_t v = b(s);
+        const uint32_t b0 = split30_lo(v), b1 = split30_hi(v);
+        lo = mad32(alo[s], b0, lo);
+        mid = mad32(alo[s], b1, mid);
+        mid = mad32(ahi[s], b0, mid);
+        hi = mad32(ahi[s], b1, hi);
+    }
+    U128 T;
+    T.lo = lo + (mid << 30);
+    uint64_t c = T.lo < lo ? 1u : 0u;
+    const uint64_t h = hi << 60;
+    T.lo += h;
+    c += T.lo < h ? 1u : 0u;
+    T.hi = (mid >> 34) + (hi >> 4) + c;
+    return T;
 }
 
 // Montgomery reduction, R = 2^64, qneg = -q^{-1} mod 2^64.

@@ -10,12 +10,14 @@
 // elements with the same index mod N2 ("columns"); the last log2(N2) stages
 // only couple elements inside one contiguous row of N2.  Phase A transforms a
 // tile of C columns, phase B a tile of R rows; each phase is one global read
-// and one global write of the tower.  Inside a phase the tile lives in LDS and
-// is processed in radix-2^r register passes (r <= 4), one LDS round trip per
-// pass.  For n <= 4096 the whole tower is one tile (single phase).
+// and one global write of the tower.  Inside a phase the tile is processed in
+// radix-2^r register passes (r <= 4): the first pass loads from HBM, the last
+// stores to HBM, and LDS carries the exchange between passes.  For n <= 4096
+// the whole tower is one tile (single phase).
 //
-// Butterflies are lazy (Harvey): forward values stay in [0, 4q), inverse
-// values in [0, 2q), reduced once at the final store (needs 4q < 2^64).
+// Butterflies are lazy (Harvey), with the 4q-bounded Shoup product of
+// modarith.hpp: forward values stay in [0, 8q), inverse values in [0, 4q),
+// reduced once at the final store (needs 8q < 2^64, i.e. q < 2^61).
 //
 // Twiddle index for CT stage st (local to the phase), sub-transform s, group i:
 //     (G(s) << st) + i,   G(s) = g0 + s * gstep
@@ -161,7 +163,7 @@ GF_HD void tile_pass(const TileDesc& d, const TW& tw, uint64_t q, uint32_t tid, 
     constexpr uint32_t R8 = 1u << R;
     const uint32_t T = 1u << d.logT;
     const uint32_t U = T << LOGPS;
-    const uint64_t q2 = 2 * q;
+    const uint64_t q4 = 4 * q, nq = 0 - q;
     for (uint32_t uid = tid; uid < U; uid += nthr) {
         uint32_t s, rest;
         if (d.sub_fastest) {
@@ -193,10 +195,10 @@ GF_HD void tile_pass(const TileDesc& d, const TW& tw, uint64_t q, uint32_t tid, 
                     if (u & h) continue;
                     uint64_t w, wp;
                     tw.get(tb + (u >> (R - c)), w, wp);
-                    const uint64_t xv = csub(x[u], q2);               // [0, 2q)
-                    const uint64_t t = shoup_lazy(x[u + h], w, wp, q);  // [0, 2q)
-                    x[u] = xv + t;                                     // [0, 4q)
-                    x[u + h] = xv - t + q2;                            // (0, 4q)
+                    const uint64_t xv = csub(x[u], q4);                // [0, 4q)
+                    const uint64_t t = shoup_lazy4(x[u + h], w, wp, nq);  // [0, 4q)
+                    x[u] = xv + t;                                     // [0, 8q)
+                    x[u + h] = xv - t + q4;                            // (0, 8q)
                 }
             }
         } else {
@@ -209,9 +211,9 @@ GF_HD void tile_pass(const TileDesc& d, const TW& tw, uint64_t q, uint32_t tid, 
                     if (u & h) continue;
                     uint64_t w, wp;
                     tw.get(tb + (u >> (R - c)), w, wp);
-                    const uint64_t xv = x[u], yv = x[u + h];           // [0, 2q)
-                    x[u] = csub(xv + yv, q2);                          // [0, 2q)
-                    x[u + h] = shoup_lazy(xv - yv + q2, w, wp, q);     // [0, 2q)
+                    const uint64_t xv = x[u], yv = x[u + h];           // [0, 4q)
+                    x[u] = csub(xv + yv, q4);                          // [0, 4q)
+                    x[u + h] = shoup_lazy4(xv - yv + q4, w, wp, nq);   // [0, 4q)
                 }
             }
         }
@@ -257,15 +259,15 @@ GF_HD void tile_chunks(uint64_t* a, const TileDesc& d, const TW& tw, uint64_t q,
     }
 }
 
-// final store value: forward [0,4q) -> [0,q); inverse: times post (n^-1 c)
+// final store value: forward [0,8q) -> [0,q); inverse: times post (n^-1 c)
 template <bool FWD>
 GF_HD uint64_t ntt_final(uint64_t v, const NttJob& j) {
-    if (FWD) return csub(csub(v, 2 * j.q), j.q);
+    if (FWD) return csub(csub(csub(v, 4 * j.q), 2 * j.q), j.q);
     return shoup(v, j.post, j.postp, j.q);
 }
 // forward final store with the optional linear epilogue at global index gi
 GF_HD uint64_t ntt_final_ep(uint64_t v, const NttJob& j, uint64_t gi) {
-    uint64_t r = csub(csub(v, 2 * j.q), j.q);
+    uint64_t r = csub(csub(csub(v, 4 * j.q), 2 * j.q), j.q);
     if (j.e0) {
         r = addmod(r, shoup(gcast(j.e0)[gi], j.c0, j.c0p, j.q), j.q);
         r = addmod(r, shoup(gcast(j.e1)[gi], j.c1, j.c1p, j.q), j.q);
@@ -292,7 +294,7 @@ struct ColOut {
     GF_GLOBAL uint64_t* dst;
     uint32_t logN2, b0;
     const NttJob& j;
-    // forward phase A hands lazy values in [0,4q) to phase B
+    // forward phase A hands lazy values in [0,8q) to phase B
     GF_HD void operator()(uint32_t s, uint32_t k, uint32_t, uint64_t v) const {
         dst[(k << logN2) + b0 + s] = FWD ? v : ntt_final<false>(v, j);
     }
@@ -392,7 +394,7 @@ GF_HD void phaseB_body(const NttJob& j, const NttGeom& g, uint32_t tile, uint64_
     const uint32_t a0 = tile * R;
     const TileDesc d{logR, 1, padded(N2), N1 + a0, 1, false, true};
     // forward: phase B is the second phase (reads out); inverse: the first.
-    // Inverse phase B hands lazy values in [0,2q) to phase A.
+    // Inverse phase B hands lazy values in [0,4q) to phase A.
     row_tile<FWD, PAIRED, S2, RM, R, FWD ? 2 : 0>(j, d, gcast(FWD ? j.out : j.in), gcast(j.out), a0 << S2, lds,
                                                   tid, nthr, sync);
 }

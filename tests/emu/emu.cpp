@@ -68,6 +68,33 @@ extern "C" void emu_ntt(uint64_t* x, uint32_t n, uint64_t q, const uint64_t* tw,
 extern "C" uint64_t emu_mulmod(uint64_t a, uint64_t b, uint64_t q) {
     return mulmod(a, b, q, host_qneg(q), host_r2(q));
 }
+extern "C" uint64_t emu_shoup_lazy4(uint64_t x, uint64_t w, uint64_t q) {
+    return shoup_lazy4(x, w, host_precon(w, q), 0 - q);
+}
+extern "C" uint64_t emu_csub(uint64_t x, uint64_t m) { return csub(x, m); }
+// sum_{s<ns} a[s] b[s] through the column accumulator (ns in 1..8) -> out[0] lo, out[1] hi
+extern "C" void emu_dot30(const uint64_t* a, const uint64_t* b, uint32_t ns, uint64_t* out) {
+    uint32_t lo[8], hi[8];
+    for (uint32_t s = 0; s < ns; ++s) {
+        lo[s] = split30_lo(a[s]);
+        hi[s] = split30_hi(a[s]);
+    }
+    U128 T{0, 0};
+    auto f = [&](uint32_t s) { return b[s]; };
+    switch (ns) {
+        case 1: T = dot30<1>(lo, hi, f); break;
+        case 2: T = dot30<2>(lo, hi, f); break;
+        case 3: T = dot30<3>(lo, hi, f); break;
+        case 4: T = dot30<4>(lo, hi, f); break;
+        case 5: T = dot30<5>(lo, hi, f); break;
+        case 6: T = dot30<6>(lo, hi, f); break;
+        case 7: T = dot30<7>(lo, hi, f); break;
+        case 8: T = dot30<8>(lo, hi, f); break;
+        default: break;
+    }
+    out[0] = T.lo;
+    out[1] = T.hi;
+}
 extern "C" uint64_t emu_shoup(uint64_t x, uint64_t w, uint64_t q) {
     return shoup(x, w, host_precon(w, q), q);
 }

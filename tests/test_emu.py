@@ -30,11 +30,13 @@ def emu():
     lib = C.CDLL(so)
     lib.emu_ntt.argtypes = [u64p, C.c_uint32, C.c_uint64, u64p, u64p, C.c_int, C.c_uint64, C.c_uint64,
                             C.c_int, C.c_uint32]
-    for f in ("emu_mulmod", "emu_shoup", "emu_mulhi"):
+    for f in ("emu_mulmod", "emu_shoup", "emu_mulhi", "emu_shoup_lazy4", "emu_csub"):
         getattr(lib, f).restype = C.c_uint64
-        getattr(lib, f).argtypes = [C.c_uint64] * 3 if f != "emu_mulhi" else [C.c_uint64] * 2
+        getattr(lib, f).argtypes = [C.c_uint64] * (2 if f in ("emu_mulhi", "emu_csub") else 3)
     lib.emu_redc.restype = C.c_uint64
     lib.emu_redc.argtypes = [C.c_uint64] * 3
+    lib.emu_dot30.restype = None
+    lib.emu_dot30.argtypes = [u64p, u64p, C.c_uint32, u64p]
     yield lib
     shutil.rmtree(d, ignore_errors=True)
 
@@ -77,3 +79,44 @@ def test_modarith_edges(emu):
             T = int(rng.integers(0, q, dtype=np.uint64)) * int(rng.integers(0, 1 << 63, dtype=np.uint64))
             T %= q * R
             assert emu.emu_redc(T % R, T >> 64, q) == (T * pow(R, -1, q)) % q
+
+
+@pytest.mark.parametrize("bits", [30, 50, 59, 60])
+def test_shoup_lazy4_range(emu, bits):
+    """The transform butterflies' Shoup product: congruent to x w mod q and
+    inside [0, 4q) for every 64-bit x, including the extremes that drive the
+    quotient estimate furthest from the exact one (x0 = 2^32 - 1, x = 2^64 - 1);
+    and csub's borrow select."""
+    q = int(O.native_primes(1, bits, 1 << 10)[0])
+    rng = np.random.default_rng(bits)
+    ws = [0, 1, q - 1, q // 2] + [int(v) for v in rng.integers(0, q, 60, dtype=np.uint64)]
+    xs = [0, 1, q - 1, q, 3 * q - 1, 8 * q - 1, (1 << 32) - 1, (1 << 64) - 1, ((1 << 64) - 1) ^ (1 << 31),
+          (q << 2) | 0xFFFFFFFF] + [int(v) for v in rng.integers(0, 1 << 63, 200, dtype=np.uint64) * 2 + 1]
+    hit = set()
+    for w in ws:
+        for x in xs:
+            r = emu.emu_shoup_lazy4(x, w, q)
+            assert r % q == (x * w) % q and r < 4 * q, (x, w)
+            hit.add(r // q)
+    assert {0, 1, 2} <= hit <= {0, 1, 2, 3}   # the slack is used, never exceeded
+    for m in (q, 2 * q, 4 * q):
+        for x in (0, 1, m - 1, m, m + 1, 2 * m - 1):
+            assert emu.emu_csub(x, m) == (x - m if x >= m else x)
+
+
+def test_dot30_column_sums(emu):
+    """Conversion-kernel dot products: the 30-bit-split column sums equal the
+    big-integer sum of products for 1..8 terms, at the largest operands
+    (2^60 - 1, where every column is closest to overflow) and at random ones."""
+    rng = np.random.default_rng(7)
+    top = (1 << 60) - 1
+    cases = [[top] * 8, [top, 0] * 4, [(1 << 30) - 1] * 8, [1 << 30] * 8]
+    cases += [[int(v) for v in rng.integers(0, 1 << 60, 8, dtype=np.uint64)] for _ in range(300)]
+    for a in cases:
+        for b in (a[::-1], [top] * 8, [int(v) for v in rng.integers(0, 1 << 60, 8, dtype=np.uint64)]):
+            for ns in range(1, 9):
+                av, bv = np.asarray(a[:ns], np.uint64), np.asarray(b[:ns], np.uint64)
+                out = np.zeros(2, np.uint64)
+                emu.emu_dot30(O.P(av), O.P(bv), ns, O.P(out))
+                want = sum(x * y for x, y in zip(a[:ns], b[:ns]))
+                assert int(out[0]) | (int(out[1]) << 64) == want, (ns, a[:ns], b[:ns])

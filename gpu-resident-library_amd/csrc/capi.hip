@@ -459,7 +459,7 @@ int gf_ctx_set_ntt_radix(gf_ctx* c, uint32_t r) {
     return guard([&] {
         if (r < 2 || r > 4) throw std::runtime_error("gpufhe: NTT radix log2 must be 2, 3 or 4");
         std::lock_guard<std::mutex> lk(c->mu);
-        c->C->geom.rmax = r;
+        c->C->set_radix(r);
         c->cache.clear();   // cached single-op plans carry the old geometry
     });
 }

@@ -59,6 +59,33 @@ RootTable root_table(uint32_t n, uint64_t q, uint64_t root) {
     return t;
 }
 
+// Phase-B row tables (build_row_table) of one modulus for the geometry of
+// radix rmax, forward and inverse, built from the paired tables on first use
+// and kept for the process like the root tables.  Null for single-phase n.
+RowTables row_tables(uint32_t n, uint64_t q, uint64_t root, uint32_t rmax) {
+    static std::mutex mu;
+    static std::map<std::tuple<uint32_t, uint64_t, uint64_t, uint32_t>, RowTables> cache;
+    std::lock_guard<std::mutex> lk(mu);
+    const NttGeom g = make_geom(n, rmax);
+    if (g.single) return RowTables{};
+    auto key = std::make_tuple(n, q, root, rmax);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    const RootTable t = root_table(n, q, root);
+    const size_t words = row_table_words(g);
+    std::vector<uint64_t> pair(2 * (size_t)n), out(2 * words);
+    void* d = nullptr;
+    hip_check(hipMalloc(&d, out.size() * 8), "hipMalloc row table");
+    for (int dir = 0; dir < 2; ++dir) {
+        hip_check(hipMemcpy(pair.data(), dir ? t.iv2 : t.fw2, pair.size() * 8, hipMemcpyDeviceToHost), "read table");
+        build_row_table(pair.data(), g, out.data() + dir * words);
+    }
+    hip_check(hipMemcpy(d, out.data(), out.size() * 8, hipMemcpyHostToDevice), "upload row table");
+    RowTables r{(const uint64_t*)d, (const uint64_t*)d + words};
+    cache.emplace(key, r);
+    return r;
+}
+
 static uint64_t prod_mod(const std::vector<uint64_t>& mods, size_t b, size_t e, uint64_t q,
                          size_t skip) {
     uint64_t r = 1 % q;
@@ -103,6 +130,7 @@ Ctx::Ctx(uint32_t n_, const std::vector<uint64_t>& q, const std::vector<uint64_t
         rt[r] = root_table(n, m, roots[r]);
         rc[r] = RowConst{m, host_qneg(m), host_r2(m), host_r1(m), rt[r].ninv, rt[r].ninvp};
     }
+    set_radix(geom.rmax);
 
     // per-level constants; device tables go into one buffer
     std::vector<uint64_t> blob;
@@ -222,6 +250,22 @@ WsLayout WsLayout::make(const Ctx& C) {
 }
 size_t Ctx::ws_words() const { return WsLayout::make(*this).total * (size_t)n; }
 
+void Ctx::set_radix(uint32_t r) {
+    if (r < 2 || r > 4) throw std::runtime_error("gpufhe: NTT radix log2 must be 2, 3 or 4");
+    std::vector<RowTables> t(L + K);
+    for (uint32_t i = 0; i < L + K; ++i) t[i] = row_tables(n, mods[i], roots[i], r);
+    geom = make_geom(n, r);
+    rtr = std::move(t);
+}
+
+const uint64_t* Ctx::row_table_of(const uint64_t* paired) const {
+    for (uint32_t i = 0; i < L + K; ++i) {
+        if (paired == rt[i].fw2) return rtr[i].fwd;
+        if (paired == rt[i].iv2) return rtr[i].inv;
+    }
+    return nullptr;
+}
+
 uint32_t Ctx::autotune(hipStream_t s, float* best_ms) {
     // 256 MiB of towers (enough workgroups to fill the chip several times
     // over, as the batched stages do) cycling through every modulus, forward
@@ -229,17 +273,21 @@ uint32_t Ctx::autotune(hipStream_t s, float* best_ms) {
     // keeps its faster round
     const uint32_t towers = std::max<uint32_t>(96, (1u << 25) / n);
     DevBuf buf((size_t)towers * n * 8);
-    std::vector<NttJob> fj, ij;
-    for (uint32_t t = 0; t < towers; ++t) {
-        const uint32_t r = t % (L + K);
-        uint64_t* x = buf.as<uint64_t>() + (size_t)t * n;
-        fj.push_back(NttJob{x, x, rt[r].fw2, nullptr, mods[r], 0, 0});
-        ij.push_back(NttJob{x, x, rt[r].iv2, nullptr, mods[r], rt[r].ninv, rt[r].ninvp});
-    }
-    DevBuf dj((fj.size() + ij.size()) * sizeof(NttJob));
-    hip_check(hipMemcpy(dj.p, fj.data(), fj.size() * sizeof(NttJob), hipMemcpyHostToDevice), "upload");
-    hip_check(hipMemcpy(dj.as<NttJob>() + fj.size(), ij.data(), ij.size() * sizeof(NttJob), hipMemcpyHostToDevice),
-              "upload");
+    // job arrays per radix (the row tables differ): [r - 2][fwd towers, inv towers]
+    std::vector<NttJob> jobs;
+    for (uint32_t r = 2; r <= 4; ++r)
+        for (int inv = 0; inv < 2; ++inv)
+            for (uint32_t t = 0; t < towers; ++t) {
+                const uint32_t m = t % (L + K);
+                const RowTables rw = row_tables(n, mods[m], roots[m], r);
+                uint64_t* x = buf.as<uint64_t>() + (size_t)t * n;
+                NttJob j = inv ? NttJob{x, x, rt[m].iv2, nullptr, mods[m], rt[m].ninv, rt[m].ninvp}
+                               : NttJob{x, x, rt[m].fw2, nullptr, mods[m], 0, 0};
+                j.twr = inv ? rw.inv : rw.fwd;
+                jobs.push_back(j);
+            }
+    DevBuf dj(jobs.size() * sizeof(NttJob));
+    hip_check(hipMemcpy(dj.p, jobs.data(), dj.bytes, hipMemcpyHostToDevice), "upload");
     hip_check(hipMemsetAsync(buf.p, 0, buf.bytes, s), "clear");
     hipEvent_t e0, e1;
     hip_check(hipEventCreate(&e0), "event");
@@ -249,12 +297,14 @@ uint32_t Ctx::autotune(hipStream_t s, float* best_ms) {
     for (int round = 0; round < 2; ++round)
         for (uint32_t r = 2; r <= 4; ++r) {
             NttGeom g = make_geom(n, r);
-            launch_ntt(dj.as<NttJob>(), towers, g, true, s);   // warm
-            launch_ntt(dj.as<NttJob>() + towers, towers, g, false, s);
+            const NttJob* fw = dj.as<NttJob>() + (size_t)(r - 2) * 2 * towers;
+            const NttJob* iv = fw + towers;
+            launch_ntt(fw, towers, g, true, s);   // warm
+            launch_ntt(iv, towers, g, false, s);
             hip_check(hipEventRecord(e0, s), "event");
             for (int rep = 0; rep < 3; ++rep) {
-                launch_ntt(dj.as<NttJob>(), towers, g, true, s);
-                launch_ntt(dj.as<NttJob>() + towers, towers, g, false, s);
+                launch_ntt(fw, towers, g, true, s);
+                launch_ntt(iv, towers, g, false, s);
             }
             hip_check(hipEventRecord(e1, s), "event");
             hip_check(hipEventSynchronize(e1), "event sync");
@@ -264,7 +314,7 @@ uint32_t Ctx::autotune(hipStream_t s, float* best_ms) {
         }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    geom.rmax = best;
+    set_radix(best);
     // reported per 96 towers, forward and inverse
     if (best_ms) *best_ms = bms / 3 * 96.0f / (float)towers;
     return best;
@@ -342,7 +392,9 @@ std::unique_ptr<DevKey> generate_key(const Ctx& C, const uint64_t* sold, const u
             for (uint32_t t = 0; t < W; ++t) {
                 uint64_t* x = tb + ((size_t)(3 * p + v) * W + t) * n;
                 ej.push_back(EmbedJob{cb + (size_t)(3 * p + v) * n, x, C.mods[t]});
-                nj.push_back(NttJob{x, x, C.rt[t].fw2, nullptr, C.mods[t], 0, 0});
+                NttJob J{x, x, C.rt[t].fw2, nullptr, C.mods[t], 0, 0};
+                J.twr = C.rtr[t].fwd;
+                nj.push_back(J);
             }
         for (uint32_t t = 0; t < W; ++t) {
             const uint64_t q = C.mods[t];
@@ -905,8 +957,15 @@ std::unique_ptr<Plan> PlanBuilder::finish_into(void* d_arena, void* h_stage, siz
 }
 
 std::unique_ptr<Plan> PlanBuilder::layout(std::vector<char>& blob) {
-    for (auto* v : {&intt_up_, &ntt_dig_, &intt_p_, &ntt_conv_, &intt_drop_, &ntt_sw_, &ntt_v_})
+    for (auto* v : {&intt_up_, &ntt_dig_, &intt_p_, &ntt_conv_, &intt_drop_, &ntt_sw_, &ntt_v_}) {
         group_by(*v, [](const NttJob& j) { return (uintptr_t)j.tw; });
+        // every transform of a plan uses the context's paired tables; its phase B
+        // reads the matching row table of the current radix
+        for (NttJob& j : *v) {
+            j.twr = C_.row_table_of(j.tw);
+            if (!j.twr && !C_.geom.single) throw std::runtime_error("gpufhe: transform table of another context");
+        }
+    }
     group_by(inner_, [](const InnerJob& j) { return (uintptr_t)j.kb[0]; });
     group_by(conv_up_, [](const ConvJob& j) { return (uintptr_t)j.tab; });
     group_by(tensor_, [](const TensorJob& j) { return j.q; });

@@ -47,6 +47,12 @@ struct RootTable {
     uint64_t ninv, ninvp;
 };
 RootTable root_table(uint32_t n, uint64_t q, uint64_t root);
+// phase-B row tables (twiddles of the run-unit pass in lane order) for radix rmax
+struct RowTables {
+    const uint64_t* fwd = nullptr;
+    const uint64_t* inv = nullptr;
+};
+RowTables row_tables(uint32_t n, uint64_t q, uint64_t root, uint32_t rmax);
 
 // constants of one level (sizeQl = l towers of Q)
 struct PartConst {
@@ -85,6 +91,11 @@ public:
     DevBuf consts;                            // all per-level device constants
     bool fused = true;                        // EvalMult+Rescale with merged ModDown/rescale
     uint32_t wide = 0;                        // 1: a modulus is 2^60 or more (generic conversion sums)
+    std::vector<RowTables> rtr;               // per row: phase-B row tables for geom.rmax
+    // register radix of transforms in plans built from now on (2..4)
+    void set_radix(uint32_t r);
+    // the row table matching a paired table of this context (fw2 or iv2), else null
+    const uint64_t* row_table_of(const uint64_t* paired) const;
     uint32_t row_q(uint32_t i) const { return i; }
     uint32_t row_p(uint32_t k) const { return L + k; }
     uint32_t num_parts(uint32_t l) const { return (l + alpha - 1) / alpha; }

@@ -56,6 +56,9 @@ struct NttJob {
     const uint64_t* e0;
     const uint64_t* e1;
     uint64_t c0, c0p, c1, c1p;
+    // paired tables only: the twiddles of phase B's run-unit pass in lane
+    // order (build_row_table); required by two-phase transforms of paired jobs
+    const uint64_t* twr;
 };
 
 struct NttGeom {
@@ -99,6 +102,7 @@ inline NttGeom make_geom(uint32_t n, uint32_t rmax = 3) {
 
 // twiddle accessors: paired (w, wp) table -> one 16-byte load; or two arrays
 struct TwPaired {
+    static constexpr bool kRows = false;
     const GF_GLOBAL uint64_t* t;
     GF_HD void get(uint32_t i, uint64_t& w, uint64_t& wp) const {
         w = t[2 * i];
@@ -106,6 +110,7 @@ struct TwPaired {
     }
 };
 struct TwSplit {
+    static constexpr bool kRows = false;
     const GF_GLOBAL uint64_t* w;
     const GF_GLOBAL uint64_t* p;
     GF_HD void get(uint32_t i, uint64_t& ww, uint64_t& wp) const {
@@ -127,6 +132,50 @@ GF_HD constexpr uint32_t chunk_size(uint32_t S, uint32_t RM, uint32_t c) {
 GF_HD constexpr uint32_t chunk_start(uint32_t S, uint32_t RM, uint32_t c) {
     return c * (S / num_chunks(S, RM)) + (c < S % num_chunks(S, RM) ? c : S % num_chunks(S, RM));
 }
+
+// Phase B's last forward pass (its first inverse pass) is the last chunk,
+// stages ST0 .. S2-1 (R = S2 - ST0 of them); its units are runs of 2^R
+// consecutive elements, 2^(S2-R) units per row, and at local stage c the
+// unit g of row a uses twiddle index ((N1 + a) << (ST0 + c)) + (g << c) + v,
+// v < 2^c.  Every lane reads a different twiddle there, so the row table
+// stores them in lane order,
+//     entry(a, c, v, g) = a PR + ((2^c - 1) << LOGU) + (v << LOGU) + g,
+//     LOGU = S2 - R,  PR = (2^R - 1) << LOGU,
+// and the 2^LOGU lanes of one row read 2^LOGU consecutive 16-byte entries
+// per load.
+GF_HD constexpr uint32_t row_pass_r(uint32_t S2, uint32_t RM) { return chunk_size(S2, RM, num_chunks(S2, RM) - 1); }
+inline size_t row_table_words(const NttGeom& g) {
+    if (g.single) return 0;
+    const uint32_t R = row_pass_r(g.logN2, g.rmax), LOGU = g.logN2 - R;
+    return (size_t)2 * ((size_t)1 << g.logN1) * (((size_t)1 << R) - 1) * ((size_t)1 << LOGU);
+}
+// std_paired: (w, wp) pairs of the standard table tw[k]; out: row_table_words(g)
+inline void build_row_table(const uint64_t* std_paired, const NttGeom& g, uint64_t* out) {
+    if (g.single) return;
+    const uint32_t S2 = g.logN2, R = row_pass_r(S2, g.rmax), ST0 = S2 - R, LOGU = S2 - R;
+    const uint32_t N1 = 1u << g.logN1, PR = ((1u << R) - 1) << LOGU;
+    for (uint32_t a = 0; a < N1; ++a)
+        for (uint32_t c = 0; c < R; ++c)
+            for (uint32_t v = 0; v < (1u << c); ++v)
+                for (uint32_t u = 0; u < (1u << LOGU); ++u) {
+                    const size_t e = (size_t)a * PR + (((1u << c) - 1) << LOGU) + ((size_t)v << LOGU) + u;
+                    const size_t k = ((size_t)(N1 + a) << (ST0 + c)) + ((size_t)u << c) + v;
+                    out[2 * e] = std_paired[2 * k];
+                    out[2 * e + 1] = std_paired[2 * k + 1];
+                }
+}
+// the row table of one phase-B tile (rows a0 ..): entry(s, c, v, g) relative to row a0
+template <uint32_t R, uint32_t LOGU>
+struct TwRows {
+    static constexpr bool kRows = true;
+    static constexpr uint32_t PR = ((1u << R) - 1) << LOGU;
+    const GF_GLOBAL uint64_t* t;
+    GF_HD void get_rp(uint32_t c, uint32_t s, uint32_t g, uint32_t v, uint64_t& w, uint64_t& wp) const {
+        const uint32_t e = s * PR + (((1u << c) - 1) << LOGU) + (v << LOGU) + g;
+        w = t[2 * e];
+        wp = t[2 * e + 1];
+    }
+};
 
 // Runtime description of how a tile is laid out and which twiddles it uses.
 // Element k of sub-transform s lives at a[k*KS + s*SS]; sub-transform s uses
@@ -194,7 +243,8 @@ GF_HD void tile_pass(const TileDesc& d, const TW& tw, uint64_t q, uint32_t tid, 
                 for (uint32_t u = 0; u < R8; ++u) {
                     if (u & h) continue;
                     uint64_t w, wp;
-                    tw.get(tb + (u >> (R - c)), w, wp);
+                    if constexpr (TW::kRows) tw.get_rp(c, s, g, u >> (R - c), w, wp);
+                    else tw.get(tb + (u >> (R - c)), w, wp);
                     const uint64_t xv = csub(x[u], q4);                // [0, 4q)
                     const uint64_t t = shoup_lazy4(x[u + h], w, wp, nq);  // [0, 4q)
                     x[u] = xv + t;                                     // [0, 8q)
@@ -210,7 +260,8 @@ GF_HD void tile_pass(const TileDesc& d, const TW& tw, uint64_t q, uint32_t tid, 
                 for (uint32_t u = 0; u < R8; ++u) {
                     if (u & h) continue;
                     uint64_t w, wp;
-                    tw.get(tb + (u >> (R - c)), w, wp);
+                    if constexpr (TW::kRows) tw.get_rp((uint32_t)c, s, g, u >> (R - c), w, wp);
+                    else tw.get(tb + (u >> (R - c)), w, wp);
                     const uint64_t xv = x[u], yv = x[u + h];           // [0, 4q)
                     x[u] = csub(xv + yv, q4);                          // [0, 4q)
                     x[u + h] = shoup_lazy4(xv - yv + q4, w, wp, nq);   // [0, 4q)
@@ -237,24 +288,29 @@ struct PassOf {
 // first pass reads through `gin` and the last writes through `gout` (global
 // memory, or LDS when the caller stages that end); the passes in between go
 // through LDS with a barrier after each.  A one-pass phase never touches LDS.
-template <bool FWD, uint32_t S, uint32_t RM, uint32_t I, class TW, class Sync, class GIN, class GOUT>
-GF_HD void tile_chunks(uint64_t* a, const TileDesc& d, const TW& tw, uint64_t q, uint32_t tid,
+template <bool FWD, uint32_t S, uint32_t RM, uint32_t I, class TW, class TWL, class Sync, class GIN, class GOUT>
+GF_HD void tile_chunks(uint64_t* a, const TileDesc& d, const TW& tw, const TWL& twl, uint64_t q, uint32_t tid,
                        uint32_t nthr, Sync sync, const GIN& gin, const GOUT& gout) {
     using P = PassOf<FWD, S, RM, I>;
     if constexpr (I < P::NC) {
         constexpr bool first = I == 0, last = I + 1 == P::NC;
+        // the last chunk (stages up to S - 1) reads its twiddles through twl
+        auto run = [&](const auto& in, const auto& out) {
+            if constexpr (P::C + 1 == P::NC) tile_pass<FWD, S, P::ST0, P::R>(d, twl, q, tid, nthr, in, out);
+            else tile_pass<FWD, S, P::ST0, P::R>(d, tw, q, tid, nthr, in, out);
+        };
         if constexpr (first && last) {
-            tile_pass<FWD, S, P::ST0, P::R>(d, tw, q, tid, nthr, gin, gout);
+            run(gin, gout);
         } else if constexpr (first) {
-            tile_pass<FWD, S, P::ST0, P::R>(d, tw, q, tid, nthr, gin, LdsOut{a});
+            run(gin, LdsOut{a});
         } else if constexpr (last) {
-            tile_pass<FWD, S, P::ST0, P::R>(d, tw, q, tid, nthr, LdsIn{a}, gout);
+            run(LdsIn{a}, gout);
         } else {
-            tile_pass<FWD, S, P::ST0, P::R>(d, tw, q, tid, nthr, LdsIn{a}, LdsOut{a});
+            run(LdsIn{a}, LdsOut{a});
         }
         if constexpr (!last) {
             sync();
-            tile_chunks<FWD, S, RM, I + 1>(a, d, tw, q, tid, nthr, sync, gin, gout);
+            tile_chunks<FWD, S, RM, I + 1>(a, d, tw, twl, q, tid, nthr, sync, gin, gout);
         }
     }
 }
@@ -310,7 +366,8 @@ GF_HD void phaseA_body(const NttJob& j, const NttGeom& g, uint32_t tile, uint64_
     // forward: phase A is the first phase (reads in); inverse: the second
     const ColIn gin{gcast(FWD ? j.in : j.out), g.logN2, b0};
     const ColOut<FWD> gout{gcast(j.out), g.logN2, b0, j};
-    tile_chunks<FWD, S1, RM, 0>(lds, d, tw_of<PAIRED>(j), j.q, tid, nthr, sync, gin, gout);
+    const auto tw = tw_of<PAIRED>(j);
+    tile_chunks<FWD, S1, RM, 0>(lds, d, tw, tw, j.q, tid, nthr, sync, gin, gout);
 }
 
 // Row access (phase B, single phase): element k of row s at word base + s*N + k.
@@ -356,7 +413,9 @@ GF_HD void wave_units(uint32_t tid, uint32_t nthr, F f) {
 
 // A row tile whose first or last pass holds short runs (TL < 8) stages that
 // end through LDS, wave by wave (no block barrier).
-template <bool FWD, bool PAIRED, uint32_t S, uint32_t RM, uint32_t ROWS, int FIN, class Sync>
+// ROWTAB: the job's row table (j.twr) serves the last chunk; off = a0 << S for
+// the tile's first row a0.
+template <bool FWD, bool PAIRED, uint32_t S, uint32_t RM, uint32_t ROWS, int FIN, bool ROWTAB, class Sync>
 GF_HD void row_tile(const NttJob& j, const TileDesc& d, const GF_GLOBAL uint64_t* src, GF_GLOBAL uint64_t* dst,
                     uint32_t off, uint64_t* lds, uint32_t tid, uint32_t nthr, Sync sync) {
     constexpr uint32_t N = 1u << S, NC = num_chunks(S, RM), WORDS = ROWS * N;
@@ -371,14 +430,23 @@ GF_HD void row_tile(const NttJob& j, const TileDesc& d, const GF_GLOBAL uint64_t
     const RowIn gin{src + off, N};
     const RowOut<FIN> gout{dst, off, N, j};
     const auto tw = tw_of<PAIRED>(j);
-    if constexpr (in_direct && out_direct)
-        tile_chunks<FWD, S, RM, 0>(lds, d, tw, j.q, tid, nthr, sync, gin, gout);
-    else if constexpr (in_direct)
-        tile_chunks<FWD, S, RM, 0>(lds, d, tw, j.q, tid, nthr, sync, gin, LdsOut{lds});
-    else if constexpr (out_direct)
-        tile_chunks<FWD, S, RM, 0>(lds, d, tw, j.q, tid, nthr, sync, LdsIn{lds}, gout);
-    else
-        tile_chunks<FWD, S, RM, 0>(lds, d, tw, j.q, tid, nthr, sync, LdsIn{lds}, LdsOut{lds});
+    auto chunks = [&](const auto& twl) {
+        if constexpr (in_direct && out_direct)
+            tile_chunks<FWD, S, RM, 0>(lds, d, tw, twl, j.q, tid, nthr, sync, gin, gout);
+        else if constexpr (in_direct)
+            tile_chunks<FWD, S, RM, 0>(lds, d, tw, twl, j.q, tid, nthr, sync, gin, LdsOut{lds});
+        else if constexpr (out_direct)
+            tile_chunks<FWD, S, RM, 0>(lds, d, tw, twl, j.q, tid, nthr, sync, LdsIn{lds}, gout);
+        else
+            tile_chunks<FWD, S, RM, 0>(lds, d, tw, twl, j.q, tid, nthr, sync, LdsIn{lds}, LdsOut{lds});
+    };
+    if constexpr (ROWTAB) {
+        constexpr uint32_t RL = row_pass_r(S, RM), LOGU = S - RL;
+        using TR = TwRows<RL, LOGU>;
+        chunks(TR{gcast(j.twr) + (size_t)2 * (off >> S) * TR::PR});
+    } else {
+        chunks(tw);
+    }
     if constexpr (!out_direct) {
         sync.wave();
         wave_units<PL::R, WORDS>(tid, nthr, [&](uint32_t idx) { gout(0, idx, 0, lds[padded(idx)]); });
@@ -395,15 +463,15 @@ GF_HD void phaseB_body(const NttJob& j, const NttGeom& g, uint32_t tile, uint64_
     const TileDesc d{logR, 1, padded(N2), N1 + a0, 1, false, true};
     // forward: phase B is the second phase (reads out); inverse: the first.
     // Inverse phase B hands lazy values in [0,4q) to phase A.
-    row_tile<FWD, PAIRED, S2, RM, R, FWD ? 2 : 0>(j, d, gcast(FWD ? j.out : j.in), gcast(j.out), a0 << S2, lds,
-                                                  tid, nthr, sync);
+    row_tile<FWD, PAIRED, S2, RM, R, FWD ? 2 : 0, PAIRED>(j, d, gcast(FWD ? j.out : j.in), gcast(j.out), a0 << S2,
+                                                          lds, tid, nthr, sync);
 }
 
 // Single phase: the whole tower (n = 2^S <= NTT_SINGLE_MAX) is one tile.
 template <bool FWD, bool PAIRED, uint32_t S, uint32_t RM, class Sync>
 GF_HD void single_body(const NttJob& j, uint64_t* lds, uint32_t tid, uint32_t nthr, Sync sync) {
     const TileDesc d{0, 1, padded(1u << S), 1, 0, false, true};
-    row_tile<FWD, PAIRED, S, RM, 1, FWD ? 2 : 1>(j, d, gcast(j.in), gcast(j.out), 0, lds, tid, nthr, sync);
+    row_tile<FWD, PAIRED, S, RM, 1, FWD ? 2 : 1, false>(j, d, gcast(j.in), gcast(j.out), 0, lds, tid, nthr, sync);
 }
 
 // Calls f(integral_constant S1, integral_constant S2) for the phase split of

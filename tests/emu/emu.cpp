@@ -17,8 +17,15 @@ struct NoSync {
 };
 
 template <bool PAIRED, uint32_t RM>
-void emu_ntt_t(const NttJob& j, uint32_t n, int fwd) {
+void emu_ntt_t(const NttJob& j0, uint32_t n, int fwd) {
     NttGeom g = make_geom(n, RM);
+    NttJob j = j0;
+    std::vector<uint64_t> rows;   // paired jobs carry phase B's row table, as the plans do
+    if (PAIRED && !g.single) {
+        rows.resize(row_table_words(g));
+        build_row_table(j.tw, g, rows.data());
+        j.twr = rows.data();
+    }
     std::vector<uint64_t> lds(NTT_TILE_PADDED > padded(n) ? NTT_TILE_PADDED : padded(n));
     if (g.single) {
         with_single(g.logn, [&](auto Sc) {

@@ -612,6 +612,8 @@ void PlanBuilder::inner_moddown_src(uint32_t l, const DevKey& key, uint64_t* ws,
         J.o1 = res[1] + (size_t)i * n;
         J.q = C.rc[row].q;
         J.qneg = C.rc[row].qneg;
+        J.pairs = n % 2 == 0 && aligned16({J.o0, J.o1});
+        for (uint32_t pi = 0; pi < lv.np; ++pi) J.pairs &= aligned16({J.dig[pi], J.kb[pi], J.ka[pi]});
         inner_.push_back(J);
     }
     for (int c = 0; c < 2; ++c)
@@ -674,8 +676,8 @@ void PlanBuilder::evalmult_rescale_fused(uint32_t l, const uint64_t* c0a, const 
     uint64_t* t[3] = {ws + w.t0 * n, ws + w.t1 * n, ws + w.t2 * n};
     for (uint32_t i = 0; i < l; ++i) {
         const size_t o = (size_t)i * n;
-        tensor_.push_back(TensorJob{c0a + o, c1a + o, c0b + o, c1b + o, t[0] + o, t[1] + o, t[2] + o,
-                                    C.rc[i].q, C.rc[i].qneg, C.rc[i].r2});
+        tensor_.push_back(tensor_job(c0a + o, c1a + o, c0b + o, c1b + o, t[0] + o, t[1] + o, t[2] + o,
+                                     C.rc[i], n));
     }
     uint64_t* dt = ws + w.dt * n;
     uint64_t* dr = ws + w.dr * n;
@@ -720,8 +722,7 @@ void PlanBuilder::evalmult(uint32_t l, const uint64_t* c0a, const uint64_t* c1a,
     uint64_t* t2 = ws + w.t2 * n;
     for (uint32_t i = 0; i < l; ++i) {
         const size_t o = (size_t)i * n;
-        tensor_.push_back(TensorJob{c0a + o, c1a + o, c0b + o, c1b + o, t0 + o, t1 + o, t2 + o, C.rc[i].q,
-                                    C.rc[i].qneg, C.rc[i].r2});
+        tensor_.push_back(tensor_job(c0a + o, c1a + o, c0b + o, c1b + o, t0 + o, t1 + o, t2 + o, C.rc[i], n));
     }
     keyswitch(l, t2, r0, r1, key, ws, t0, t1);
 }

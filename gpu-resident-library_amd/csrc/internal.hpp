@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <initializer_list>
 #include <stdexcept>
 #include <string>
 
@@ -26,6 +27,7 @@ struct TensorJob {
     const uint64_t *c0a, *c1a, *c0b, *c1b;
     uint64_t *t0, *t1, *t2;
     uint64_t q, qneg, r2;
+    uint32_t pairs;   // 1: every pointer 16-byte aligned (two residues per lane and access)
 };
 
 // dst[j] = (sum_i src[i] * tab[i*ntgt + j]) * R^-1 mod p_j for targets j in
@@ -50,6 +52,7 @@ struct InnerJob {
     uint64_t* o1;
     uint64_t q, qneg;
     uint32_t np;
+    uint32_t pairs;   // as TensorJob::pairs
 };
 
 // out = t + (res - conv) * pinv   (ApproxModDown combine fused with r = t + ba)
@@ -159,6 +162,19 @@ struct CopyJob {
     const uint64_t* in;
     uint64_t* out;
 };
+
+// every pointer 16-byte aligned (pairs of residues load as one dwordx4)
+inline uint32_t aligned16(std::initializer_list<const void*> ps) {
+    for (const void* p : ps)
+        if ((uintptr_t)p & 15) return 0;
+    return 1;
+}
+
+inline TensorJob tensor_job(const uint64_t* c0a, const uint64_t* c1a, const uint64_t* c0b, const uint64_t* c1b,
+                            uint64_t* t0, uint64_t* t1, uint64_t* t2, const RowConst& rc, uint32_t n) {
+    const uint32_t pairs = n % 2 == 0 && aligned16({c0a, c1a, c0b, c1b, t0, t1, t2});
+    return TensorJob{c0a, c1a, c0b, c1b, t0, t1, t2, rc.q, rc.qneg, rc.r2, pairs};
+}
 
 inline void hip_check(hipError_t e, const char* what) {
     if (e != hipSuccess)

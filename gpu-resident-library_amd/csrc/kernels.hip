@@ -111,15 +111,45 @@ void launch_ntt_one(const NttJob& job, const NttGeom& g, bool fwd, hipStream_t s
         for (uint32_t i = (blockIdx.x * PW_EPT + e_) * PW_THREADS + threadIdx.x; i < (n); \
              i = 0xFFFFFFFFu)
 
+// The same four elements per lane as PW_FOR, taken as two adjacent pairs
+// (i, i+1) with i even: one 16-byte access per pair halves the memory
+// instructions of the address and data path (TA/TD), which bounds the
+// streaming kernels with several operand arrays.  Needs an even n and
+// 16-byte-aligned arrays (the jobs' `pairs` flag).
+#define PW2_FOR(n)                                                                        \
+    for (uint32_t e_ = 0; e_ < PW_EPT / 2; ++e_)                                          \
+        for (uint32_t i = ((blockIdx.x * (PW_EPT / 2) + e_) * PW_THREADS + threadIdx.x) * 2; \
+             i < (n); i = 0xFFFFFFFFu)
+
+struct alignas(16) Pair {
+    uint64_t v[2];
+};
+__device__ __forceinline__ Pair ld2(const uint64_t* p, uint32_t i) { return *(const Pair*)(p + i); }
+__device__ __forceinline__ void st2(uint64_t* p, uint32_t i, const Pair& x) { *(Pair*)(p + i) = x; }
+
+__device__ __forceinline__ void tensor_one(const TensorJob& j, uint64_t a0, uint64_t a1, uint64_t b0,
+                                           uint64_t b1, uint64_t& t0, uint64_t& t1, uint64_t& t2) {
+    t0 = mulmod(a0, b0, j.q, j.qneg, j.r2);
+    const U128 m = add128(mul128(a0, b1), mul128(a1, b0));
+    t1 = redc(mul128(redc(m, j.q, j.qneg), j.r2), j.q, j.qneg);
+    t2 = mulmod(a1, b1, j.q, j.qneg, j.r2);
+}
+
 __global__ __launch_bounds__(PW_THREADS) void k_tensor(const TensorJob* __restrict__ jobs,
                                                        uint32_t off, uint32_t n) {
     const TensorJob j = jobs[off + blockIdx.y];
-    PW_FOR(n) {
-        const uint64_t a0 = j.c0a[i], a1 = j.c1a[i], b0 = j.c0b[i], b1 = j.c1b[i];
-        j.t0[i] = mulmod(a0, b0, j.q, j.qneg, j.r2);
-        const U128 m = add128(mul128(a0, b1), mul128(a1, b0));
-        j.t1[i] = redc(mul128(redc(m, j.q, j.qneg), j.r2), j.q, j.qneg);
-        j.t2[i] = mulmod(a1, b1, j.q, j.qneg, j.r2);
+    if (j.pairs) {
+        PW2_FOR(n) {
+            const Pair a0 = ld2(j.c0a, i), a1 = ld2(j.c1a, i), b0 = ld2(j.c0b, i), b1 = ld2(j.c1b, i);
+            Pair t0, t1, t2;
+#pragma unroll
+            for (int k = 0; k < 2; ++k) tensor_one(j, a0.v[k], a1.v[k], b0.v[k], b1.v[k], t0.v[k], t1.v[k], t2.v[k]);
+            st2(j.t0, i, t0);
+            st2(j.t1, i, t1);
+            st2(j.t2, i, t2);
+        }
+    } else {
+        PW_FOR(n) { tensor_one(j, j.c0a[i], j.c1a[i], j.c0b[i], j.c1b[i], j.t0[i], j.t1[i], j.t2[i]); }
     }
 }
 
@@ -203,15 +233,31 @@ __global__ __launch_bounds__(PW_THREADS) void k_inner(const InnerJob* __restrict
     const uint64_t q = j.q, qneg = j.qneg;
     uint64_t* o0 = j.o0;
     uint64_t* o1 = j.o1;
-    PW_FOR(n) {
-        U128 a0{0, 0}, a1{0, 0};
-        for (uint32_t p = 0; p < np; ++p) {
-            const uint64_t d = j.dig[p][i];
-            a0 = add128(a0, mul128(d, j.kb[p][i]));
-            a1 = add128(a1, mul128(d, j.ka[p][i]));
+    if (j.pairs) {
+        PW2_FOR(n) {
+            U128 a0[2] = {{0, 0}, {0, 0}}, a1[2] = {{0, 0}, {0, 0}};
+            for (uint32_t p = 0; p < np; ++p) {
+                const Pair d = ld2(j.dig[p], i), kb = ld2(j.kb[p], i), ka = ld2(j.ka[p], i);
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    a0[k] = add128(a0[k], mul128(d.v[k], kb.v[k]));
+                    a1[k] = add128(a1[k], mul128(d.v[k], ka.v[k]));
+                }
+            }
+            st2(o0, i, Pair{{redc(a0[0], q, qneg), redc(a0[1], q, qneg)}});
+            st2(o1, i, Pair{{redc(a1[0], q, qneg), redc(a1[1], q, qneg)}});
         }
-        o0[i] = redc(a0, q, qneg);
-        o1[i] = redc(a1, q, qneg);
+    } else {
+        PW_FOR(n) {
+            U128 a0{0, 0}, a1{0, 0};
+            for (uint32_t p = 0; p < np; ++p) {
+                const uint64_t d = j.dig[p][i];
+                a0 = add128(a0, mul128(d, j.kb[p][i]));
+                a1 = add128(a1, mul128(d, j.ka[p][i]));
+            }
+            o0[i] = redc(a0, q, qneg);
+            o1[i] = redc(a1, q, qneg);
+        }
     }
 }
 

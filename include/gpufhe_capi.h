@@ -10,7 +10,9 @@
  * that run one operation over many ciphertexts in one launch per stage.
  *
  * Conventions: pointers named d_* / c* / o* / ws are device pointers; host
- * arrays are named h_*.  Functions returning int give 0 on success and -1 on
+ * arrays are named h_*.  Device arrays need 8-byte alignment; when every array
+ * of a stage is 16-byte aligned (as gf_malloc returns) the streaming kernels
+ * move two residues per access, which is faster, with identical results.  Functions returning int give 0 on success and -1 on
  * error (message in gf_last_error()); constructors return NULL on error.
  * Operations are asynchronous on the given stream; the caller synchronises.
  */

@@ -116,6 +116,37 @@ def test_evalmult_rescale(G, n, L, dnum):
         assert np.array_equal(o1.download(), w1), level
 
 
+@pytest.mark.parametrize("n,L,dnum", [SHAPES[1], SHAPES[3]])
+def test_evalmult_rescale_word_aligned_buffers(G, n, L, dnum):
+    """Operands, outputs and workspace only 8-byte aligned (one word past a
+    16-byte boundary): the streaming kernels take their one-residue-per-access
+    path and give the same residues as the oracle."""
+    P, ctx, k, av, bv = setup(G, n, L, dnum)
+    rng = np.random.default_rng(17)
+    s = G.Stream()
+    ws = G.Dev(ctx.ws_bytes // 8 + 1)
+    for level in sorted({L, 2}):
+        cts = [rand_ct(P, level, rng) for _ in range(4)]
+        d = []
+        for c in cts:
+            b = G.Dev(c.size + 1)
+            b.upload(c, word_off=1)
+            d.append(b)
+        o0, o1 = G.Dev((level - 1) * n + 1), G.Dev((level - 1) * n + 1)
+        G.check(G.lib.gf_evalmult_rescale(ctx.h, k.h, level, *[x.at(1) for x in d], o0.at(1), o1.at(1),
+                                          ws.at(1), s.s))
+        s.sync()
+        w0, w1 = P.evalmult_rescale(level, *cts, av, bv)
+        assert np.array_equal(o0.download(word_off=1), w0), level
+        assert np.array_equal(o1.download(word_off=1), w1), level
+        r0, r1 = G.Dev(level * n + 1), G.Dev(level * n + 1)
+        G.check(G.lib.gf_evalmult(ctx.h, k.h, level, *[x.at(1) for x in d], r0.at(1), r1.at(1), ws.at(1), s.s))
+        s.sync()
+        w0, w1 = P.evalmult(level, *cts, av, bv)
+        assert np.array_equal(r0.download(word_off=1), w0), level
+        assert np.array_equal(r1.download(word_off=1), w1), level
+
+
 @pytest.mark.parametrize("n,L,dnum", [SHAPES[1], SHAPES[4]])
 def test_evalmult_rescale_unfused_sequence(G, n, L, dnum):
     """The reference's stage sequence (ModDown NTTs, combine, INTT/switch/NTT/

@@ -284,6 +284,7 @@ uint32_t Ctx::autotune(hipStream_t s, float* best_ms) {
                 NttJob j = inv ? NttJob{x, x, rt[m].iv2, nullptr, mods[m], rt[m].ninv, rt[m].ninvp}
                                : NttJob{x, x, rt[m].fw2, nullptr, mods[m], 0, 0};
                 j.twr = inv ? rw.inv : rw.fwd;
+                if (!inv) j.red = lazy_red_of(mods[m]);   // as the plans' forward transforms
                 jobs.push_back(j);
             }
     DevBuf dj(jobs.size() * sizeof(NttJob));
@@ -612,6 +613,7 @@ void PlanBuilder::inner_moddown_src(uint32_t l, const DevKey& key, uint64_t* ws,
         J.o1 = res[1] + (size_t)i * n;
         J.q = C.rc[row].q;
         J.qneg = C.rc[row].qneg;
+        J.red = lazy_red_of(J.q);
         J.pairs = n % 2 == 0 && aligned16({J.o0, J.o1});
         for (uint32_t pi = 0; pi < lv.np; ++pi) J.pairs &= aligned16({J.dig[pi], J.kb[pi], J.ka[pi]});
         inner_.push_back(J);
@@ -967,6 +969,11 @@ std::unique_ptr<Plan> PlanBuilder::layout(std::vector<char>& blob) {
             if (!j.twr && !C_.geom.single) throw std::runtime_error("gpufhe: transform table of another context");
         }
     }
+    // forward transforms over moduli below 2^57 skip the butterflies'
+    // conditional subtractions; the digits stay unreduced for the inner product
+    for (auto* v : {&ntt_dig_, &ntt_conv_, &ntt_sw_, &ntt_v_})
+        for (NttJob& j : *v) j.red = lazy_red_of(j.q);
+    for (NttJob& j : ntt_dig_) j.lazy_out = j.red != 0;
     group_by(inner_, [](const InnerJob& j) { return (uintptr_t)j.kb[0]; });
     group_by(conv_up_, [](const ConvJob& j) { return (uintptr_t)j.tab; });
     group_by(tensor_, [](const TensorJob& j) { return j.q; });

@@ -51,6 +51,7 @@ struct InnerJob {
     uint64_t* o0;
     uint64_t* o1;
     uint64_t q, qneg;
+    uint64_t red;     // lazy_red_of(q): digits of this row may be unreduced (NttJob::lazy_out)
     uint32_t np;
     uint32_t pairs;   // as TensorJob::pairs
 };

@@ -31,23 +31,43 @@ struct JobsOne {
     __device__ NttJob get(uint32_t) const { return j; }
 };
 
+// Forward jobs with j.red take the butterflies without conditional
+// subtractions (LZ); the choice is uniform per workgroup (one job each).
+template <bool FWD, class F>
+__device__ __forceinline__ void with_lz(const NttJob& j, F&& f) {
+    if constexpr (FWD) {
+        if (j.red) f(std::true_type{});
+        else f(std::false_type{});
+    } else {
+        f(std::false_type{});
+    }
+}
+
 template <bool FWD, bool PAIRED, uint32_t S1, uint32_t RM, class J>
 __global__ __launch_bounds__(NTT_THREADS) void k_ntt_A(J jobs, NttGeom g) {
     extern __shared__ uint64_t lds[];
     const NttJob j = jobs.get(blockIdx.y);
-    phaseA_body<FWD, PAIRED, S1, RM>(j, g, blockIdx.x, lds, threadIdx.x, blockDim.x, DevSync{});
+    with_lz<FWD>(j, [&](auto L) {
+        phaseA_body<FWD, decltype(L)::value, PAIRED, S1, RM>(j, g, blockIdx.x, lds, threadIdx.x, blockDim.x,
+                                                             DevSync{});
+    });
 }
 template <bool FWD, bool PAIRED, uint32_t S2, uint32_t RM, class J>
 __global__ __launch_bounds__(NTT_THREADS) void k_ntt_B(J jobs, NttGeom g) {
     extern __shared__ uint64_t lds[];
     const NttJob j = jobs.get(blockIdx.y);
-    phaseB_body<FWD, PAIRED, S2, RM>(j, g, blockIdx.x, lds, threadIdx.x, blockDim.x, DevSync{});
+    with_lz<FWD>(j, [&](auto L) {
+        phaseB_body<FWD, decltype(L)::value, PAIRED, S2, RM>(j, g, blockIdx.x, lds, threadIdx.x, blockDim.x,
+                                                             DevSync{});
+    });
 }
 template <bool FWD, bool PAIRED, uint32_t S, uint32_t RM, class J>
 __global__ __launch_bounds__(NTT_THREADS) void k_ntt_S(J jobs) {
     extern __shared__ uint64_t lds[];
     const NttJob j = jobs.get(blockIdx.y);
-    single_body<FWD, PAIRED, S, RM>(j, lds, threadIdx.x, blockDim.x, DevSync{});
+    with_lz<FWD>(j, [&](auto L) {
+        single_body<FWD, decltype(L)::value, PAIRED, S, RM>(j, lds, threadIdx.x, blockDim.x, DevSync{});
+    });
 }
 
 // PAIRED: library plans use interleaved (w, Shoup precon) tables; the
@@ -226,11 +246,19 @@ __global__ __launch_bounds__(PW_THREADS) void k_conv(const ConvJob* __restrict__
     });
 }
 
+// An accumulated inner-product sum to [0, q).  With j.red the digits may be
+// the forward transforms' unreduced values (below 69q, NttJob::lazy_out):
+// the sum stays below 16 * 69 q^2 < 2^125 for q < 2^57, REDC without its
+// correction leaves less than 2^62, and reduce_red finishes.
+__device__ __forceinline__ uint64_t inner_out(const U128& T, const InnerJob& j) {
+    if (j.red) return redc_red(T, j.q, j.qneg, (uint32_t)j.red);
+    return redc(T, j.q, j.qneg);
+}
+
 __global__ __launch_bounds__(PW_THREADS) void k_inner(const InnerJob* __restrict__ jobs,
                                                       uint32_t off, uint32_t n) {
     const InnerJob& j = jobs[off + blockIdx.y];
     const uint32_t np = j.np;
-    const uint64_t q = j.q, qneg = j.qneg;
     uint64_t* o0 = j.o0;
     uint64_t* o1 = j.o1;
     if (j.pairs) {
@@ -244,8 +272,8 @@ __global__ __launch_bounds__(PW_THREADS) void k_inner(const InnerJob* __restrict
                     a1[k] = add128(a1[k], mul128(d.v[k], ka.v[k]));
                 }
             }
-            st2(o0, i, Pair{{redc(a0[0], q, qneg), redc(a0[1], q, qneg)}});
-            st2(o1, i, Pair{{redc(a1[0], q, qneg), redc(a1[1], q, qneg)}});
+            st2(o0, i, Pair{{inner_out(a0[0], j), inner_out(a0[1], j)}});
+            st2(o1, i, Pair{{inner_out(a1[0], j), inner_out(a1[1], j)}});
         }
     } else {
         PW_FOR(n) {
@@ -255,8 +283,8 @@ __global__ __launch_bounds__(PW_THREADS) void k_inner(const InnerJob* __restrict
                 a0 = add128(a0, mul128(d, j.kb[p][i]));
                 a1 = add128(a1, mul128(d, j.ka[p][i]));
             }
-            o0[i] = redc(a0, q, qneg);
-            o1[i] = redc(a1, q, qneg);
+            o0[i] = inner_out(a0, j);
+            o1[i] = inner_out(a1, j);
         }
     }
 }

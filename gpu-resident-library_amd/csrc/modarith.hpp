@@ -116,6 +116,17 @@ GF_HD uint64_t shoup_lazy4(uint64_t x, uint64_t w, uint64_t wp, uint64_t nq) {
     return mad32(h0, n0, opaque64(((uint64_t)hi << 32) | (uint32_t)p));
 }
 
+// x mod q for any 64-bit x, given 2^32 < q < 2^62 and m = floor(2^64 / q)
+// (so m < 2^32).  With x = xh 2^32 + xl, the estimate e = floor(xh m / 2^32)
+// satisfies x/q - 3 < e <= x/q: xh m / 2^32 lies within xh / 2^32 < 1 below
+// xh 2^32 / q, which lies within xl / q < 1 below x / q.  So x - e q is in
+// [0, 3q) and two conditional subtractions finish.
+GF_HD uint64_t reduce_red(uint64_t x, uint64_t q, uint32_t m) {
+    const uint32_t e = umulhi32((uint32_t)(x >> 32), m);
+    const uint64_t r = x - (uint64_t)e * q;
+    return csub(csub(r, 2 * q), q);
+}
+
 // Exact sum of NS <= 8 products a_s b_s of values below 2^60, as 128 bits.
 // Each operand is split at bit 30 (a = a1 2^30 + a0, a0 < 2^30, a1 < 2^30),
 // and the partial products go to three 64-bit columns with one
@@ -149,12 +160,22 @@ GF_HD U128 dot30(const uint32_t* alo, const uint32_t* ahi, B b) {
     return T;
 }
 
-// Montgomery reduction, R = 2^64, qneg = -q^{-1} mod 2^64.
+// Montgomery reduction, R = 2^64, qneg = -q^{-1} mod 2^64, without the final
+// correction: T R^{-1} mod q plus a multiple of q, below T / 2^64 + q + 1
+// (any T with T / 2^64 + q + 1 <= 2^64).
+GF_HD uint64_t redc_lazy(U128 T, uint64_t q, uint64_t qneg) {
+    uint64_t m = T.lo * qneg;
+    return T.hi + mulhi64(m, q) + (T.lo != 0 ? 1u : 0u);
+}
 // For T < q * 2^64 returns T * R^{-1} mod q, in [0, q).
 GF_HD uint64_t redc(U128 T, uint64_t q, uint64_t qneg) {
-    uint64_t m = T.lo * qneg;
-    uint64_t t = T.hi + mulhi64(m, q) + (T.lo != 0 ? 1u : 0u);
+    const uint64_t t = redc_lazy(T, q, qneg);
     return t >= q ? t - q : t;
+}
+// T * R^{-1} mod q in [0, q) for any T with T / 2^64 + q + 1 <= 2^64, given
+// 2^32 < q < 2^62 and m = floor(2^64 / q) (sums of products of unreduced values)
+GF_HD uint64_t redc_red(U128 T, uint64_t q, uint64_t qneg, uint32_t m) {
+    return reduce_red(redc_lazy(T, q, qneg), q, m);
 }
 
 // a*b mod q for arbitrary residues: REDC(REDC(a b) * R^2).  r2 = 2^128 mod q.

@@ -17,7 +17,11 @@
 //
 // Butterflies are lazy (Harvey), with the 4q-bounded Shoup product of
 // modarith.hpp: forward values stay in [0, 8q), inverse values in [0, 4q),
-// reduced once at the final store (needs 8q < 2^64, i.e. q < 2^61).
+// reduced once at the final store (needs 8q < 2^64, i.e. q < 2^61).  Forward
+// transforms over a modulus 2^32 < q < 2^57 (LZ) drop the forward butterfly's
+// conditional subtraction altogether: each stage adds at most 4q to the bound,
+// so values stay below (1 + 4 logn) q < 2^64 and one Barrett-style reduction
+// (reduce_red) at the final store replaces the per-stage ones.
 //
 // Twiddle index for CT stage st (local to the phase), sub-transform s, group i:
 //     (G(s) << st) + i,   G(s) = g0 + s * gstep
@@ -59,7 +63,22 @@ struct NttJob {
     // paired tables only: the twiddles of phase B's run-unit pass in lane
     // order (build_row_table); required by two-phase transforms of paired jobs
     const uint64_t* twr;
+    // forward only: floor(2^64 / q) for a modulus 2^32 < q < 2^57 (lazy_red_of),
+    // else 0.  Such a modulus leaves room in a 64-bit word for the whole
+    // transform's growth, so the butterflies skip their conditional subtraction
+    // (values stay below (1 + 4 logn) q <= 69 q) and the final store reduces once.
+    uint64_t red;
+    // with red: store those unreduced values, for a consumer that reduces them
+    // itself (the key inner product); ignored with an epilogue
+    uint32_t lazy_out;
 };
+
+// floor(2^64 / q) for the moduli whose forward transforms run without
+// per-butterfly reductions, else 0 (NttJob::red)
+inline uint64_t lazy_red_of(uint64_t q) {
+    if (q <= (1ull << 32) || q >= (1ull << 57)) return 0;
+    return (uint64_t)(((unsigned __int128)1 << 64) / q);
+}
 
 struct NttGeom {
     uint32_t n, logn;
@@ -203,9 +222,10 @@ struct LdsOut {
 // One radix-2^R pass (stages ST0 .. ST0+R-1 of an S-stage transform) over all
 // units of the tile owned by this thread; fully unrolled, the R-stage unit
 // stays in registers.  `in` supplies the unit's values, `out` takes them.
-template <bool FWD, uint32_t S, uint32_t ST0, uint32_t R, class TW, class IN, class OUT>
+template <bool FWD, bool LZ, uint32_t S, uint32_t ST0, uint32_t R, class TW, class IN, class OUT>
 GF_HD void tile_pass(const TileDesc& d, const TW& tw, uint64_t q, uint32_t tid, uint32_t nthr,
                      const IN& in, const OUT& out) {
+    static_assert(FWD || !LZ, "the lazy-growth butterflies are forward only");
     constexpr uint32_t LOGTL = S - ST0 - R;   // log2(t_last)
     constexpr uint32_t TL = 1u << LOGTL;
     constexpr uint32_t LOGPS = S - R;         // log2(units per sub-transform)
@@ -245,10 +265,12 @@ GF_HD void tile_pass(const TileDesc& d, const TW& tw, uint64_t q, uint32_t tid, 
                     uint64_t w, wp;
                     if constexpr (TW::kRows) tw.get_rp(c, s, g, u >> (R - c), w, wp);
                     else tw.get(tb + (u >> (R - c)), w, wp);
-                    const uint64_t xv = csub(x[u], q4);                // [0, 4q)
+                    // xv in [0, 4q) after the csub: outputs in [0, 8q).
+                    // LZ: xv below the stage's bound B, outputs below B + 4q.
+                    const uint64_t xv = LZ ? x[u] : csub(x[u], q4);
                     const uint64_t t = shoup_lazy4(x[u + h], w, wp, nq);  // [0, 4q)
-                    x[u] = xv + t;                                     // [0, 8q)
-                    x[u + h] = xv - t + q4;                            // (0, 8q)
+                    x[u] = xv + t;
+                    x[u + h] = xv - t + q4;
                 }
             }
         } else {
@@ -288,7 +310,7 @@ struct PassOf {
 // first pass reads through `gin` and the last writes through `gout` (global
 // memory, or LDS when the caller stages that end); the passes in between go
 // through LDS with a barrier after each.  A one-pass phase never touches LDS.
-template <bool FWD, uint32_t S, uint32_t RM, uint32_t I, class TW, class TWL, class Sync, class GIN, class GOUT>
+template <bool FWD, bool LZ, uint32_t S, uint32_t RM, uint32_t I, class TW, class TWL, class Sync, class GIN, class GOUT>
 GF_HD void tile_chunks(uint64_t* a, const TileDesc& d, const TW& tw, const TWL& twl, uint64_t q, uint32_t tid,
                        uint32_t nthr, Sync sync, const GIN& gin, const GOUT& gout) {
     using P = PassOf<FWD, S, RM, I>;
@@ -296,8 +318,8 @@ GF_HD void tile_chunks(uint64_t* a, const TileDesc& d, const TW& tw, const TWL& 
         constexpr bool first = I == 0, last = I + 1 == P::NC;
         // the last chunk (stages up to S - 1) reads its twiddles through twl
         auto run = [&](const auto& in, const auto& out) {
-            if constexpr (P::C + 1 == P::NC) tile_pass<FWD, S, P::ST0, P::R>(d, twl, q, tid, nthr, in, out);
-            else tile_pass<FWD, S, P::ST0, P::R>(d, tw, q, tid, nthr, in, out);
+            if constexpr (P::C + 1 == P::NC) tile_pass<FWD, LZ, S, P::ST0, P::R>(d, twl, q, tid, nthr, in, out);
+            else tile_pass<FWD, LZ, S, P::ST0, P::R>(d, tw, q, tid, nthr, in, out);
         };
         if constexpr (first && last) {
             run(gin, gout);
@@ -310,7 +332,7 @@ GF_HD void tile_chunks(uint64_t* a, const TileDesc& d, const TW& tw, const TWL& 
         }
         if constexpr (!last) {
             sync();
-            tile_chunks<FWD, S, RM, I + 1>(a, d, tw, twl, q, tid, nthr, sync, gin, gout);
+            tile_chunks<FWD, LZ, S, RM, I + 1>(a, d, tw, twl, q, tid, nthr, sync, gin, gout);
         }
     }
 }
@@ -321,14 +343,28 @@ GF_HD uint64_t ntt_final(uint64_t v, const NttJob& j) {
     if (FWD) return csub(csub(csub(v, 4 * j.q), 2 * j.q), j.q);
     return shoup(v, j.post, j.postp, j.q);
 }
-// forward final store with the optional linear epilogue at global index gi
+// forward final store with the optional linear epilogue at global index gi.
+// LZ: v is below 69q; with the epilogue's two lazy products below 77q < 2^64,
+// reduced once (or stored as is for a lazy_out job without epilogue).
+template <bool LZ>
 GF_HD uint64_t ntt_final_ep(uint64_t v, const NttJob& j, uint64_t gi) {
-    uint64_t r = csub(csub(csub(v, 4 * j.q), 2 * j.q), j.q);
-    if (j.e0) {
-        r = addmod(r, shoup(gcast(j.e0)[gi], j.c0, j.c0p, j.q), j.q);
-        r = addmod(r, shoup(gcast(j.e1)[gi], j.c1, j.c1p, j.q), j.q);
+    if constexpr (LZ) {
+        if (j.e0) {
+            const uint64_t nq = 0 - j.q;
+            v += shoup_lazy4(gcast(j.e0)[gi], j.c0, j.c0p, nq);
+            v += shoup_lazy4(gcast(j.e1)[gi], j.c1, j.c1p, nq);
+        } else if (j.lazy_out) {
+            return v;
+        }
+        return reduce_red(v, j.q, (uint32_t)j.red);
+    } else {
+        uint64_t r = csub(csub(csub(v, 4 * j.q), 2 * j.q), j.q);
+        if (j.e0) {
+            r = addmod(r, shoup(gcast(j.e0)[gi], j.c0, j.c0p, j.q), j.q);
+            r = addmod(r, shoup(gcast(j.e1)[gi], j.c1, j.c1p, j.q), j.q);
+        }
+        return r;
     }
-    return r;
 }
 
 // ---- phase bodies (shared by the kernels and the host emulation) ----
@@ -357,7 +393,7 @@ struct ColOut {
 };
 
 // Phase A: tile = columns [b0, b0+C) x all N1 = 2^S1 rows.
-template <bool FWD, bool PAIRED, uint32_t S1, uint32_t RM, class Sync>
+template <bool FWD, bool LZ, bool PAIRED, uint32_t S1, uint32_t RM, class Sync>
 GF_HD void phaseA_body(const NttJob& j, const NttGeom& g, uint32_t tile, uint64_t* lds,
                        uint32_t tid, uint32_t nthr, Sync sync) {
     constexpr uint32_t logC = NTT_LOG_TILE - S1, C = 1u << logC;
@@ -367,7 +403,7 @@ GF_HD void phaseA_body(const NttJob& j, const NttGeom& g, uint32_t tile, uint64_
     const ColIn gin{gcast(FWD ? j.in : j.out), g.logN2, b0};
     const ColOut<FWD> gout{gcast(j.out), g.logN2, b0, j};
     const auto tw = tw_of<PAIRED>(j);
-    tile_chunks<FWD, S1, RM, 0>(lds, d, tw, tw, j.q, tid, nthr, sync, gin, gout);
+    tile_chunks<FWD, LZ, S1, RM, 0>(lds, d, tw, tw, j.q, tid, nthr, sync, gin, gout);
 }
 
 // Row access (phase B, single phase): element k of row s at word base + s*N + k.
@@ -378,14 +414,14 @@ struct RowIn {
 };
 // final store of a row tile: forward values through the epilogue (FIN = 2),
 // inverse values through the n^-1 multiply (FIN = 1) or lazy (FIN = 0)
-template <int FIN>
+template <int FIN, bool LZ>
 struct RowOut {
     GF_GLOBAL uint64_t* dst;         // tower start
     uint32_t off, N;
     const NttJob& j;
     GF_HD void operator()(uint32_t s, uint32_t k, uint32_t, uint64_t v) const {
         const uint32_t gi = off + s * N + k;
-        dst[gi] = FIN == 2 ? ntt_final_ep(v, j, gi) : FIN == 1 ? ntt_final<false>(v, j) : v;
+        dst[gi] = FIN == 2 ? ntt_final_ep<LZ>(v, j, gi) : FIN == 1 ? ntt_final<false>(v, j) : v;
     }
 };
 
@@ -415,7 +451,7 @@ GF_HD void wave_units(uint32_t tid, uint32_t nthr, F f) {
 // end through LDS, wave by wave (no block barrier).
 // ROWTAB: the job's row table (j.twr) serves the last chunk; off = a0 << S for
 // the tile's first row a0.
-template <bool FWD, bool PAIRED, uint32_t S, uint32_t RM, uint32_t ROWS, int FIN, bool ROWTAB, class Sync>
+template <bool FWD, bool LZ, bool PAIRED, uint32_t S, uint32_t RM, uint32_t ROWS, int FIN, bool ROWTAB, class Sync>
 GF_HD void row_tile(const NttJob& j, const TileDesc& d, const GF_GLOBAL uint64_t* src, GF_GLOBAL uint64_t* dst,
                     uint32_t off, uint64_t* lds, uint32_t tid, uint32_t nthr, Sync sync) {
     constexpr uint32_t N = 1u << S, NC = num_chunks(S, RM), WORDS = ROWS * N;
@@ -428,17 +464,17 @@ GF_HD void row_tile(const NttJob& j, const TileDesc& d, const GF_GLOBAL uint64_t
         sync.wave();
     }
     const RowIn gin{src + off, N};
-    const RowOut<FIN> gout{dst, off, N, j};
+    const RowOut<FIN, LZ> gout{dst, off, N, j};
     const auto tw = tw_of<PAIRED>(j);
     auto chunks = [&](const auto& twl) {
         if constexpr (in_direct && out_direct)
-            tile_chunks<FWD, S, RM, 0>(lds, d, tw, twl, j.q, tid, nthr, sync, gin, gout);
+            tile_chunks<FWD, LZ, S, RM, 0>(lds, d, tw, twl, j.q, tid, nthr, sync, gin, gout);
         else if constexpr (in_direct)
-            tile_chunks<FWD, S, RM, 0>(lds, d, tw, twl, j.q, tid, nthr, sync, gin, LdsOut{lds});
+            tile_chunks<FWD, LZ, S, RM, 0>(lds, d, tw, twl, j.q, tid, nthr, sync, gin, LdsOut{lds});
         else if constexpr (out_direct)
-            tile_chunks<FWD, S, RM, 0>(lds, d, tw, twl, j.q, tid, nthr, sync, LdsIn{lds}, gout);
+            tile_chunks<FWD, LZ, S, RM, 0>(lds, d, tw, twl, j.q, tid, nthr, sync, LdsIn{lds}, gout);
         else
-            tile_chunks<FWD, S, RM, 0>(lds, d, tw, twl, j.q, tid, nthr, sync, LdsIn{lds}, LdsOut{lds});
+            tile_chunks<FWD, LZ, S, RM, 0>(lds, d, tw, twl, j.q, tid, nthr, sync, LdsIn{lds}, LdsOut{lds});
     };
     if constexpr (ROWTAB) {
         constexpr uint32_t RL = row_pass_r(S, RM), LOGU = S - RL;
@@ -454,7 +490,7 @@ GF_HD void row_tile(const NttJob& j, const TileDesc& d, const GF_GLOBAL uint64_t
 }
 
 // Phase B: tile = rows [a0, a0+R), each a contiguous run of N2 = 2^S2.
-template <bool FWD, bool PAIRED, uint32_t S2, uint32_t RM, class Sync>
+template <bool FWD, bool LZ, bool PAIRED, uint32_t S2, uint32_t RM, class Sync>
 GF_HD void phaseB_body(const NttJob& j, const NttGeom& g, uint32_t tile, uint64_t* lds,
                        uint32_t tid, uint32_t nthr, Sync sync) {
     constexpr uint32_t N2 = 1u << S2, logR = NTT_LOG_TILE - S2, R = 1u << logR;
@@ -463,15 +499,15 @@ GF_HD void phaseB_body(const NttJob& j, const NttGeom& g, uint32_t tile, uint64_
     const TileDesc d{logR, 1, padded(N2), N1 + a0, 1, false, true};
     // forward: phase B is the second phase (reads out); inverse: the first.
     // Inverse phase B hands lazy values in [0,4q) to phase A.
-    row_tile<FWD, PAIRED, S2, RM, R, FWD ? 2 : 0, PAIRED>(j, d, gcast(FWD ? j.out : j.in), gcast(j.out), a0 << S2,
+    row_tile<FWD, LZ, PAIRED, S2, RM, R, FWD ? 2 : 0, PAIRED>(j, d, gcast(FWD ? j.out : j.in), gcast(j.out), a0 << S2,
                                                           lds, tid, nthr, sync);
 }
 
 // Single phase: the whole tower (n = 2^S <= NTT_SINGLE_MAX) is one tile.
-template <bool FWD, bool PAIRED, uint32_t S, uint32_t RM, class Sync>
+template <bool FWD, bool LZ, bool PAIRED, uint32_t S, uint32_t RM, class Sync>
 GF_HD void single_body(const NttJob& j, uint64_t* lds, uint32_t tid, uint32_t nthr, Sync sync) {
     const TileDesc d{0, 1, padded(1u << S), 1, 0, false, true};
-    row_tile<FWD, PAIRED, S, RM, 1, FWD ? 2 : 1, false>(j, d, gcast(j.in), gcast(j.out), 0, lds, tid, nthr, sync);
+    row_tile<FWD, LZ, PAIRED, S, RM, 1, FWD ? 2 : 1, false>(j, d, gcast(j.in), gcast(j.out), 0, lds, tid, nthr, sync);
 }
 
 // Calls f(integral_constant S1, integral_constant S2) for the phase split of

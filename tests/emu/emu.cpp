@@ -16,7 +16,7 @@ struct NoSync {
     void wave() const {}
 };
 
-template <bool PAIRED, uint32_t RM>
+template <bool PAIRED, uint32_t RM, bool LZ>
 void emu_ntt_t(const NttJob& j0, uint32_t n, int fwd) {
     NttGeom g = make_geom(n, RM);
     NttJob j = j0;
@@ -30,8 +30,8 @@ void emu_ntt_t(const NttJob& j0, uint32_t n, int fwd) {
     if (g.single) {
         with_single(g.logn, [&](auto Sc) {
             constexpr uint32_t S = decltype(Sc)::value;
-            if (fwd) single_body<true, PAIRED, S, RM>(j, lds.data(), 0, 1, NoSync{});
-            else single_body<false, PAIRED, S, RM>(j, lds.data(), 0, 1, NoSync{});
+            if (fwd) single_body<true, LZ, PAIRED, S, RM>(j, lds.data(), 0, 1, NoSync{});
+            else single_body<false, false, PAIRED, S, RM>(j, lds.data(), 0, 1, NoSync{});
         });
         return;
     }
@@ -39,38 +39,72 @@ void emu_ntt_t(const NttJob& j0, uint32_t n, int fwd) {
         constexpr uint32_t S1 = decltype(A)::value, S2 = decltype(B)::value;
         const uint32_t tilesA = (1u << S2) / g.C, tilesB = (1u << S1) / g.R;
         if (fwd) {
-            for (uint32_t t = 0; t < tilesA; ++t) phaseA_body<true, PAIRED, S1, RM>(j, g, t, lds.data(), 0, 1, NoSync{});
-            for (uint32_t t = 0; t < tilesB; ++t) phaseB_body<true, PAIRED, S2, RM>(j, g, t, lds.data(), 0, 1, NoSync{});
+            for (uint32_t t = 0; t < tilesA; ++t) phaseA_body<true, LZ, PAIRED, S1, RM>(j, g, t, lds.data(), 0, 1, NoSync{});
+            for (uint32_t t = 0; t < tilesB; ++t) phaseB_body<true, LZ, PAIRED, S2, RM>(j, g, t, lds.data(), 0, 1, NoSync{});
         } else {
-            for (uint32_t t = 0; t < tilesB; ++t) phaseB_body<false, PAIRED, S2, RM>(j, g, t, lds.data(), 0, 1, NoSync{});
-            for (uint32_t t = 0; t < tilesA; ++t) phaseA_body<false, PAIRED, S1, RM>(j, g, t, lds.data(), 0, 1, NoSync{});
+            for (uint32_t t = 0; t < tilesB; ++t) phaseB_body<false, false, PAIRED, S2, RM>(j, g, t, lds.data(), 0, 1, NoSync{});
+            for (uint32_t t = 0; t < tilesA; ++t) phaseA_body<false, false, PAIRED, S1, RM>(j, g, t, lds.data(), 0, 1, NoSync{});
         }
     });
 }
 
 template <bool PAIRED>
 void emu_ntt_r(const NttJob& j, uint32_t n, int fwd, uint32_t rmax) {
-    with_radix(rmax, [&](auto Rc) { emu_ntt_t<PAIRED, decltype(Rc)::value>(j, n, fwd); });
+    with_radix(rmax, [&](auto Rc) {
+        // the kernels' per-job choice: forward jobs with red skip the butterfly reductions
+        if (fwd && j.red) emu_ntt_t<PAIRED, decltype(Rc)::value, true>(j, n, fwd);
+        else emu_ntt_t<PAIRED, decltype(Rc)::value, false>(j, n, fwd);
+    });
 }
 }  // namespace
 
 // In-place transform of x (n residues mod q).  paired != 0 builds an
 // interleaved (w, precon) table from tw/twp; rmax = register radix log2.
-extern "C" void emu_ntt(uint64_t* x, uint32_t n, uint64_t q, const uint64_t* tw, const uint64_t* twp,
-                        int fwd, uint64_t post, uint64_t postp, int paired, uint32_t rmax) {
+// lazy (forward only): 1 = red = lazy_red_of(q) (0 for moduli outside its
+// range), 2 = also lazy_out.  e0/e1 (may be null): the forward epilogue
+// out = NTT(x) + e0 c0 + e1 c1 with Shoup precons c0p/c1p.
+extern "C" void emu_ntt_ex(uint64_t* x, uint32_t n, uint64_t q, const uint64_t* tw, const uint64_t* twp,
+                           int fwd, uint64_t post, uint64_t postp, int paired, uint32_t rmax, int lazy,
+                           const uint64_t* e0, const uint64_t* e1, uint64_t c0, uint64_t c1) {
+    std::vector<uint64_t> t2;
+    NttJob j{x, x, tw, twp, q, post, postp};
     if (paired) {
-        std::vector<uint64_t> t2(2 * (std::size_t)n);
+        t2.resize(2 * (std::size_t)n);
         for (uint32_t i = 0; i < n; ++i) {
             t2[2 * i] = tw[i];
             t2[2 * i + 1] = twp[i];
         }
-        NttJob j{x, x, t2.data(), nullptr, q, post, postp};
-        emu_ntt_r<true>(j, n, fwd, rmax);
-    } else {
-        NttJob j{x, x, tw, twp, q, post, postp};
-        emu_ntt_r<false>(j, n, fwd, rmax);
+        j.tw = t2.data();
+        j.twp = nullptr;
     }
+    if (fwd && lazy) {
+        j.red = lazy_red_of(q);
+        j.lazy_out = lazy == 2;
+    }
+    if (fwd && e0) {
+        j.e0 = e0;
+        j.e1 = e1;
+        j.c0 = c0;
+        j.c0p = host_precon(c0, q);
+        j.c1 = c1;
+        j.c1p = host_precon(c1, q);
+    }
+    if (paired) emu_ntt_r<true>(j, n, fwd, rmax);
+    else emu_ntt_r<false>(j, n, fwd, rmax);
 }
+extern "C" void emu_ntt(uint64_t* x, uint32_t n, uint64_t q, const uint64_t* tw, const uint64_t* twp,
+                        int fwd, uint64_t post, uint64_t postp, int paired, uint32_t rmax) {
+    emu_ntt_ex(x, n, q, tw, twp, fwd, post, postp, paired, rmax, 0, nullptr, nullptr, 0, 0);
+}
+extern "C" uint64_t emu_lazy_red_of(uint64_t q) { return lazy_red_of(q); }
+// the key inner product's output with unreduced digits: sum_s a[s] b[s]
+// accumulated as 128 bits (mul128/add128), then redc_red
+extern "C" uint64_t emu_inner_red(const uint64_t* a, const uint64_t* b, uint32_t ns, uint64_t q) {
+    U128 T{0, 0};
+    for (uint32_t s = 0; s < ns; ++s) T = add128(T, mul128(a[s], b[s]));
+    return redc_red(T, q, host_qneg(q), (uint32_t)lazy_red_of(q));
+}
+extern "C" uint64_t emu_reduce_red(uint64_t x, uint64_t q) { return reduce_red(x, q, (uint32_t)lazy_red_of(q)); }
 
 extern "C" uint64_t emu_mulmod(uint64_t a, uint64_t b, uint64_t q) {
     return mulmod(a, b, q, host_qneg(q), host_r2(q));

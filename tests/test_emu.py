@@ -37,6 +37,14 @@ def emu():
     lib.emu_redc.argtypes = [C.c_uint64] * 3
     lib.emu_dot30.restype = None
     lib.emu_dot30.argtypes = [u64p, u64p, C.c_uint32, u64p]
+    lib.emu_ntt_ex.argtypes = [u64p, C.c_uint32, C.c_uint64, u64p, u64p, C.c_int, C.c_uint64, C.c_uint64,
+                               C.c_int, C.c_uint32, C.c_int, u64p, u64p, C.c_uint64, C.c_uint64]
+    lib.emu_lazy_red_of.restype = C.c_uint64
+    lib.emu_lazy_red_of.argtypes = [C.c_uint64]
+    lib.emu_inner_red.restype = C.c_uint64
+    lib.emu_inner_red.argtypes = [u64p, u64p, C.c_uint32, C.c_uint64]
+    lib.emu_reduce_red.restype = C.c_uint64
+    lib.emu_reduce_red.argtypes = [C.c_uint64, C.c_uint64]
     yield lib
     shutil.rmtree(d, ignore_errors=True)
 
@@ -120,3 +128,89 @@ def test_dot30_column_sums(emu):
                 emu.emu_dot30(O.P(av), O.P(bv), ns, O.P(out))
                 want = sum(x * y for x, y in zip(a[:ns], b[:ns]))
                 assert int(out[0]) | (int(out[1]) << 64) == want, (ns, a[:ns], b[:ns])
+
+
+@pytest.mark.parametrize("bits", [33, 40, 50, 56])
+def test_reduce_red(emu, bits):
+    """The one-step reduction behind the forward transforms' lazy path: x mod q
+    for every 64-bit x, at the edges of its quotient estimate (low word all
+    ones, x just below a multiple of q, the largest x) and at random x."""
+    q = int(O.native_primes(1, bits, 1 << 10)[0])
+    assert emu.emu_lazy_red_of(q) == (1 << 64) // q
+    rng = np.random.default_rng(bits)
+    xs = [0, 1, q - 1, q, 2 * q - 1, 3 * q - 1, 69 * q - 1, 77 * q - 1, (1 << 64) - 1, (1 << 64) - q,
+          ((1 << 64) - 1) // q * q - 1, (1 << 32) - 1, ((q >> 32) << 32) | 0xFFFFFFFF]
+    xs += [int(v) for v in rng.integers(0, 1 << 63, 3000, dtype=np.uint64) * 2 + 1]
+    for x in xs:
+        assert emu.emu_reduce_red(x, q) == x % q, x
+
+
+def test_lazy_red_range(emu):
+    """Only moduli in (2^32, 2^57) take the lazy path: a 69q bound must fit 64 bits."""
+    assert emu.emu_lazy_red_of((1 << 32) - 5) == 0
+    assert emu.emu_lazy_red_of((1 << 32) + 15) != 0
+    assert emu.emu_lazy_red_of((1 << 57) - 111) != 0
+    assert emu.emu_lazy_red_of((1 << 57) + 1) == 0
+    assert 69 * ((1 << 57) - 1) + 8 * ((1 << 57) - 1) < 1 << 64
+
+
+@pytest.mark.parametrize("logn", [4, 7, 12, 13, 15, 16, 17])
+def test_lazy_forward_transform(emu, logn):
+    """Forward transforms without per-butterfly reductions (moduli below 2^57):
+    reduced output equal to the oracle's, the unreduced output (lazy_out)
+    congruent to it and below (1 + 4 logn) q, and the linear epilogue
+    NTT(x) + e0 c0 + e1 c1 equal to its definition, for every radix and both
+    twiddle layouts; inputs at q - 1 push the growth bound hardest."""
+    n = 1 << logn
+    for bits in (40, 56):
+        q = int(O.native_primes(1, bits, n)[0])
+        psi = int(O.lib.or_native_root(n, q))
+        fr, fp, ir, ip, ni, nip = O.root_tables(n, q, psi)
+        rng = np.random.default_rng(logn * 100 + bits)
+        for x in (rng.integers(0, q, n, dtype=np.uint64), np.full(n, q - 1, np.uint64)):
+            want = O.ntt(x, q, fr, fp)
+            e0 = rng.integers(0, q, n, dtype=np.uint64)
+            e1 = rng.integers(0, q, n, dtype=np.uint64)
+            c0, c1 = int(rng.integers(0, q)), int(rng.integers(0, q))
+            want_ep = np.array([(int(a) + int(b) * c0 + int(c) * c1) % q for a, b, c in zip(want, e0, e1)],
+                               np.uint64)
+            top = 0
+            for rmax in (2, 3, 4):
+                for paired in (0, 1):
+                    y = x.copy()
+                    emu.emu_ntt_ex(O.P(y), n, q, O.P(fr), O.P(fp), 1, 0, 0, paired, rmax, 1, None, None, 0, 0)
+                    assert np.array_equal(y, want), (bits, rmax, paired)
+                    z = x.copy()
+                    emu.emu_ntt_ex(O.P(z), n, q, O.P(fr), O.P(fp), 1, 0, 0, paired, rmax, 2, None, None, 0, 0)
+                    assert np.array_equal(z % np.uint64(q), want), (bits, rmax, paired)
+                    assert int(z.max()) < (1 + 4 * logn) * q
+                    top = max(top, int(z.max()))
+                    e = x.copy()
+                    emu.emu_ntt_ex(O.P(e), n, q, O.P(fr), O.P(fp), 1, 0, 0, paired, rmax, 2, O.P(e0), O.P(e1),
+                                   c0, c1)
+                    assert np.array_equal(e, want_ep), (bits, rmax, paired)
+                    e = x.copy()
+                    emu.emu_ntt_ex(O.P(e), n, q, O.P(fr), O.P(fp), 1, 0, 0, paired, rmax, 0, O.P(e0), O.P(e1),
+                                   c0, c1)
+                    assert np.array_equal(e, want_ep), (bits, rmax, paired)
+            assert top >= 2 * q   # the unreduced values use their slack
+
+
+@pytest.mark.parametrize("bits", [33, 50, 56])
+def test_inner_product_unreduced_digits(emu, bits):
+    """Key inner product over digits left unreduced by the lazy forward
+    transforms (below 69q, up to 16 parts, keys in [0, q)): the 128-bit sum,
+    REDC without its correction and the one-step reduction give
+    sum * 2^-64 mod q, including at the largest digits and keys."""
+    q = int(O.native_primes(1, bits, 1 << 10)[0])
+    Rinv = pow(1 << 64, -1, q)
+    rng = np.random.default_rng(bits + 5)
+    cases = [([69 * q - 1] * 16, [q - 1] * 16), ([69 * q - 1] * 16, [0] * 16), ([0] * 16, [q - 1] * 16)]
+    for _ in range(400):
+        cases.append(([int(v) for v in rng.integers(0, 69 * q, 16, dtype=np.uint64)],
+                      [int(v) for v in rng.integers(0, q, 16, dtype=np.uint64)]))
+    for a, b in cases:
+        for ns in (1, 4, 16):
+            av, bv = np.asarray(a[:ns], np.uint64), np.asarray(b[:ns], np.uint64)
+            want = sum(x * y for x, y in zip(a[:ns], b[:ns])) * Rinv % q
+            assert emu.emu_inner_red(O.P(av), O.P(bv), ns, q) == want, (ns, a[:ns], b[:ns])

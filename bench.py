@@ -54,7 +54,7 @@ STEP_KERNELS = {"ntt_fwd": ("k_ntt_A", "k_ntt_B"), "ntt_inv": ("k_ntt_B", "k_ntt
                 "tensor": ("k_tensor",), "conv": ("k_conv",), "inner": ("k_inner",),
                 "combine": ("k_combine",), "switch": ("k_switch",), "fuse": ("k_fuse",),
                 "add": ("k_add",), "perm": ("k_perm",), "copy": ("k_copy",), "drop": ("k_drop",),
-                "rsv": ("k_rs",), "ptacc": ("k_ptacc",)}
+                "rsv": ("k_rs",), "ptacc": ("k_ptacc",), "ntt_fwd_a": ("k_ntt_A",), "ksi": ("k_ksi",)}
 
 
 def kernel_bytes(name, kind, nkern, total, epilogue):
@@ -77,12 +77,22 @@ def step_bytes(n, L, K, alpha, np_parts, kinds):
     sc = [L - min(alpha, L - alpha * p) + K for p in range(np_parts)]
     sp = [min(alpha, L - alpha * p) for p in range(np_parts)]
     if len(kinds) == 9:   # fused ModDown + rescale plan
+        if "ksi" in kinds:
+            # the digits' column pass, then their row pass fused with the key
+            # inner product: half of each digit transform's 16 n B, plus the
+            # inner product's operands other than the transformed digits
+            # (the input's own digit towers, both key rows, both outputs)
+            digits = [("ntt_digits_col", T * sum(sc), 0),
+                      ("rowpass_inner_product",
+                       T * sum(sc) + T * (sum(sp) + 2 * np_parts * (L + K) + 2 * (L + K)), 0)]
+        else:
+            digits = [("ntt_digits", 2 * T * sum(sc), 0),
+                      ("key_inner_product", T * (L + K) * (3 * np_parts + 2), 0)]
         return [
             ("tensor", 7 * T * L, 0),
             ("intt_modup", 2 * T * (L + 2), 0),             # + INTT of t0/t1's last tower
             ("conv_modup", T * (sum(sp) + sum(sc)), 0),
-            ("ntt_digits", 2 * T * sum(sc), 0),
-            ("key_inner_product", T * (L + K) * (3 * np_parts + 2), 0),
+        ] + digits + [
             ("intt_moddown", 2 * T * (2 * K + 2), 0),       # + INTT of res0/res1's last tower
             ("drop_tower", T * 2 * (K + 3), 0),
             ("moddown_rescale_prologue", T * 2 * (K + 1 + (L - 1)), 0),

@@ -500,6 +500,13 @@ void Plan::run_jobs(size_t i, hipStream_t s, uint32_t which, uint32_t first, uin
         case Step::DROP: launch_drop(GF_J(DropJob), cnt, n, s); break;
         case Step::RSV: launch_rs(GF_J(RsJob), cnt, n, s); break;
         case Step::PTACC: launch_ptacc(GF_J(PtAccJob), cnt, n, s); break;
+        case Step::NTT_FA: launch_ntt(GF_J(NttJob), cnt, geom, true, s, 1); break;
+        case Step::KSI:
+            for (const KsiSeg& g : ksi_segs) {
+                const uint32_t a = std::max(first, g.first), b = std::min(first + cnt, g.first + g.count);
+                if (a < b) launch_ksi((const KsiJob*)p + a, b - a, geom, g.regions, g.waves, s);
+            }
+            break;
     }
 #undef GF_J
 }
@@ -534,15 +541,70 @@ void PlanBuilder::keyswitch_core(uint32_t l, const uint64_t* d, const DevKey& ke
     const Ctx& C = C_;
     if (l < 1 || l > C.L) throw std::runtime_error("gpufhe: bad level");
     if (key.nparts < C.lv[l].np) throw std::runtime_error("gpufhe: key has too few parts for this level");
+    if (use_ksi(l)) {
+        modup(l, d, ws, extra0, extra1, dx0, dx1, true);
+        ksi_moddown_src(l, d, key, ws);
+        return;
+    }
     modup(l, d, ws, extra0, extra1, dx0, dx1);
     const uint64_t* dig = ws + WsLayout::make(C).dig * C.n;
     inner_moddown_src(l, key, ws, [&](uint32_t pi, uint32_t i) { return digit_tower(l, d, dig, pi, i); });
 }
 
+bool PlanBuilder::use_ksi(uint32_t l) const {
+    return C_.fused && !C_.geom.single && C_.n % KSI_TILE == 0 && C_.lv[l].np <= KSI_MAXP;
+}
+
+// 3b-5. The digits' row pass fused with the inner product over Q_l P
+// (src/keyswitch_resident.cpp:171-200), into the workspace accumulators;
+// then their P towers to coefficient form (ModDown's sources), as
+// inner_moddown_src.  Digit towers of part pi other than the input's own hold
+// the column-pass output of their forward transform (modup with
+// rowpass_in_ksi).
+void PlanBuilder::ksi_moddown_src(uint32_t l, const uint64_t* d, const DevKey& key, uint64_t* ws) {
+    const Ctx& C = C_;
+    const uint32_t n = C.n, K = C.K, W = l + K;
+    const LevelConst& lv = C.lv[l];
+    const WsLayout w = WsLayout::make(C);
+    const uint64_t* dig = ws + w.dig * n;
+    uint64_t* res[2] = {ws + w.res0 * n, ws + w.res1 * n};
+    uint64_t* xp = ws + w.xp * n;
+    for (uint32_t i = 0; i < W; ++i) {
+        const uint32_t row = i < l ? C.row_q(i) : C.row_p(i - l);
+        const uint32_t krow = i < l ? i : key.fullQ + (i - l);
+        KsiJob J{};
+        J.np = lv.np;
+        for (uint32_t pi = 0; pi < lv.np; ++pi) {
+            const PartConst& P = lv.parts[pi];
+            const bool own = i >= P.st && i < P.st + P.sp;
+            J.src[pi] = digit_tower(l, d, dig, pi, i);
+            if (!own) J.xf |= 1u << pi;
+            const size_t ko = ((size_t)pi * key.rows + krow) * n;
+            J.kb[pi] = key.b.as<uint64_t>() + ko;
+            J.ka[pi] = key.a.as<uint64_t>() + ko;
+        }
+        J.o0 = res[0] + (size_t)i * n;
+        J.o1 = res[1] + (size_t)i * n;
+        J.tw = C.rt[row].fw2;
+        J.q = C.rc[row].q;
+        J.qneg = C.rc[row].qneg;
+        J.red = lazy_red_of(J.q);
+        J.pairs = aligned16({J.o0, J.o1});
+        for (uint32_t pi = 0; pi < lv.np; ++pi) J.pairs &= aligned16({J.src[pi], J.kb[pi], J.ka[pi]});
+        ksi_.push_back(J);
+    }
+    for (int c = 0; c < 2; ++c)
+        for (uint32_t k = 0; k < K; ++k) {
+            const uint32_t r = C.row_p(k);
+            intt_p_.push_back(NttJob{res[c] + (size_t)(l + k) * n, xp + (size_t)(c * K + k) * n, C.rt[r].iv2,
+                                     nullptr, C.mods[r], lv.pd_post[k], lv.pd_postp[k]});
+        }
+}
+
 // 1-3. ModUp of d into the workspace digits (eval form); extra towers (the
 // last tower of t0/t1 in the fused EvalMult+Rescale) ride along the INTT.
 void PlanBuilder::modup(uint32_t l, const uint64_t* d, uint64_t* ws, const uint64_t* extra0,
-                        const uint64_t* extra1, uint64_t* dx0, uint64_t* dx1) {
+                        const uint64_t* extra1, uint64_t* dx0, uint64_t* dx1, bool rowpass_in_ksi) {
     const Ctx& C = C_;
     const uint32_t n = C.n, K = C.K, W = l + K;
     const LevelConst& lv = C.lv[l];
@@ -570,8 +632,8 @@ void PlanBuilder::modup(uint32_t l, const uint64_t* d, uint64_t* ws, const uint6
                                        j0, j0 + CONV_CHUNK < P.sc ? j0 + CONV_CHUNK : P.sc, C.wide});
         for (uint32_t j = 0; j < P.sc; ++j) {
             const uint32_t r = P.compl_rows[j];
-            ntt_dig_.push_back(NttJob{dp + (size_t)j * n, dp + (size_t)j * n, C.rt[r].fw2, nullptr,
-                                      C.mods[r], 0, 0});
+            (rowpass_in_ksi ? ntt_diga_ : ntt_dig_)
+                .push_back(NttJob{dp + (size_t)j * n, dp + (size_t)j * n, C.rt[r].fw2, nullptr, C.mods[r], 0, 0});
         }
     }
 }
@@ -960,7 +1022,7 @@ std::unique_ptr<Plan> PlanBuilder::finish_into(void* d_arena, void* h_stage, siz
 }
 
 std::unique_ptr<Plan> PlanBuilder::layout(std::vector<char>& blob) {
-    for (auto* v : {&intt_up_, &ntt_dig_, &intt_p_, &ntt_conv_, &intt_drop_, &ntt_sw_, &ntt_v_}) {
+    for (auto* v : {&intt_up_, &ntt_dig_, &ntt_diga_, &intt_p_, &ntt_conv_, &intt_drop_, &ntt_sw_, &ntt_v_}) {
         group_by(*v, [](const NttJob& j) { return (uintptr_t)j.tw; });
         // every transform of a plan uses the context's paired tables; its phase B
         // reads the matching row table of the current radix
@@ -971,9 +1033,24 @@ std::unique_ptr<Plan> PlanBuilder::layout(std::vector<char>& blob) {
     }
     // forward transforms over moduli below 2^57 skip the butterflies'
     // conditional subtractions; the digits stay unreduced for the inner product
-    for (auto* v : {&ntt_dig_, &ntt_conv_, &ntt_sw_, &ntt_v_})
+    for (auto* v : {&ntt_dig_, &ntt_diga_, &ntt_conv_, &ntt_sw_, &ntt_v_})
         for (NttJob& j : *v) j.red = lazy_red_of(j.q);
     for (NttJob& j : ntt_dig_) j.lazy_out = j.red != 0;
+    for (KsiJob& j : ksi_) {
+        j.twr = C_.row_table_of(j.tw);
+        if (!j.twr) throw std::runtime_error("gpufhe: transform table of another context");
+    }
+    auto nxf = [](const KsiJob& j) { return (uint32_t)__builtin_popcount(j.xf); };
+    group_by(ksi_, [](const KsiJob& j) { return (uintptr_t)j.kb[0]; });
+    group_by(ksi_, nxf);
+    std::vector<Plan::KsiSeg> ksi_segs;
+    for (uint32_t i = 0; i < ksi_.size();) {
+        uint32_t e = i;
+        while (e < ksi_.size() && nxf(ksi_[e]) == nxf(ksi_[i])) ++e;
+        const uint32_t x = nxf(ksi_[i]);
+        ksi_segs.push_back(Plan::KsiSeg{i, e - i, x, std::max(1u, std::min<uint32_t>(x, KSI_WAVES))});
+        i = e;
+    }
     group_by(inner_, [](const InnerJob& j) { return (uintptr_t)j.kb[0]; });
     group_by(conv_up_, [](const ConvJob& j) { return (uintptr_t)j.tab; });
     group_by(tensor_, [](const TensorJob& j) { return j.q; });
@@ -981,6 +1058,7 @@ std::unique_ptr<Plan> PlanBuilder::layout(std::vector<char>& blob) {
     P->n = C_.n;
     P->logn = C_.logn;
     P->geom = C_.geom;
+    P->ksi_segs = std::move(ksi_segs);
     blob.clear();
     auto stage = [&](Step::Kind kind, const void* data, size_t cnt, size_t sz) {
         if (!cnt) return;
@@ -995,8 +1073,10 @@ std::unique_ptr<Plan> PlanBuilder::layout(std::vector<char>& blob) {
     GF_STAGE(NTT_I, intt_up_);
     GF_STAGE(CONV, conv_up_);
     GF_STAGE(NTT_F, ntt_dig_);
+    GF_STAGE(NTT_FA, ntt_diga_);
     GF_STAGE(PERM, perm_dig_);
     GF_STAGE(INNER, inner_);
+    GF_STAGE(KSI, ksi_);
     GF_STAGE(NTT_I, intt_p_);
     GF_STAGE(CONV, conv_dn_);
     GF_STAGE(NTT_F, ntt_conv_);

@@ -89,7 +89,9 @@ public:
     std::vector<RootTable> rt;
     std::vector<LevelConst> lv;               // index l = 1..L (0 unused)
     DevBuf consts;                            // all per-level device constants
-    bool fused = true;                        // EvalMult+Rescale with merged ModDown/rescale
+    // EvalMult+Rescale with merged ModDown/rescale, and keyswitches whose
+    // digit row pass feeds the key inner product directly (KSI)
+    bool fused = true;
     uint32_t wide = 0;                        // 1: a modulus is 2^60 or more (generic conversion sums)
     std::vector<RowTables> rtr;               // per row: phase-B row tables for geom.rmax
     // register radix of transforms in plans built from now on (2..4)
@@ -121,7 +123,10 @@ void download_key(const Ctx& C, const DevKey& k, uint64_t* av, uint64_t* bv, hip
 
 // One launch step of a plan
 struct Step {
-    enum Kind { NTT_F, NTT_I, TENSOR, CONV, INNER, COMBINE, SWITCH, FUSE, ADD, PERM, COPY, DROP, RSV, PTACC } kind;
+    // NTT_FA: a forward transform's first (column) pass only; KSI: the fused
+    // row pass of the ModUp digits and the key inner product
+    enum Kind { NTT_F, NTT_I, TENSOR, CONV, INNER, COMBINE, SWITCH, FUSE, ADD, PERM, COPY, DROP, RSV, PTACC,
+                NTT_FA, KSI } kind;
     size_t off;     // byte offset of the first job in the plan's device buffer
     uint32_t count;
 };
@@ -133,6 +138,12 @@ public:
     const void* base = nullptr;  // where the job arrays live (dev.p or a caller arena)
     uint32_t n = 0, logn = 0;
     NttGeom geom{};
+    // the KSI step's jobs come sorted by transformed-part count; each run of
+    // equal count is one launch with that many waves and LDS regions
+    struct KsiSeg {
+        uint32_t first, count, regions, waves;
+    };
+    std::vector<KsiSeg> ksi_segs;
     void run(hipStream_t s) const;
     // which: 3 = every kernel of step i; 1 / 2 = only its first / second kernel
     void run_step(size_t i, hipStream_t s, uint32_t which = 3) const;
@@ -197,7 +208,9 @@ private:
     std::vector<NttJob> intt_up_;
     std::vector<ConvJob> conv_up_;
     std::vector<NttJob> ntt_dig_;
+    std::vector<NttJob> ntt_diga_;   // digits whose row pass runs inside the KSI step
     std::vector<InnerJob> inner_;
+    std::vector<KsiJob> ksi_;
     std::vector<NttJob> intt_p_;
     std::vector<ConvJob> conv_dn_;
     std::vector<NttJob> ntt_conv_;
@@ -217,7 +230,11 @@ private:
     void keyswitch_core(uint32_t l, const uint64_t* d, const DevKey& key, uint64_t* ws,
                         const uint64_t* extra0, const uint64_t* extra1, uint64_t* dx0, uint64_t* dx1);
     void modup(uint32_t l, const uint64_t* d, uint64_t* ws, const uint64_t* extra0, const uint64_t* extra1,
-               uint64_t* dx0, uint64_t* dx1);
+               uint64_t* dx0, uint64_t* dx1, bool rowpass_in_ksi = false);
+    // the digits' row pass fused with the key inner product (two-phase rings,
+    // up to KSI_MAXP parts, fused schedule on)
+    bool use_ksi(uint32_t l) const;
+    void ksi_moddown_src(uint32_t l, const uint64_t* d, const DevKey& key, uint64_t* ws);
     const uint64_t* digit_tower(uint32_t l, const uint64_t* d, const uint64_t* dig, uint32_t pi,
                                 uint32_t i) const;
     template <class F>

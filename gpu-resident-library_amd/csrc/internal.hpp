@@ -10,6 +10,7 @@
 #include <stdexcept>
 #include <string>
 
+#include "ksi.hpp"
 #include "modarith.hpp"
 #include "ntt_engine.hpp"
 
@@ -205,6 +206,9 @@ void launch_rs(const RsJob* d, uint32_t nj, uint32_t n, hipStream_t s);
 void launch_embed(const EmbedJob* d, uint32_t nj, uint32_t n, hipStream_t s);
 void launch_ekgen(const EkJob* d, uint32_t nj, uint32_t n, hipStream_t s);
 void launch_ptacc(const PtAccJob* d, uint32_t nj, uint32_t n, hipStream_t s);
+// fused digit row pass + key inner product: regions = LDS regions (at least every
+// job's transformed-part count), waves = workgroup size in waves (1..KSI_WAVES)
+void launch_ksi(const KsiJob* d, uint32_t nj, const NttGeom& g, uint32_t regions, uint32_t waves, hipStream_t s);
 
 inline SwitchJob make_switch(const uint64_t* in, uint64_t* out, uint64_t q_old, uint64_t q_new) {
     SwitchJob j;

@@ -450,8 +450,11 @@ GF_HD void wave_units(uint32_t tid, uint32_t nthr, F f) {
 // A row tile whose first or last pass holds short runs (TL < 8) stages that
 // end through LDS, wave by wave (no block barrier).
 // ROWTAB: the job's row table (j.twr) serves the last chunk; off = a0 << S for
-// the tile's first row a0.
-template <bool FWD, bool LZ, bool PAIRED, uint32_t S, uint32_t RM, uint32_t ROWS, int FIN, bool ROWTAB, class Sync>
+// the tile's first row a0.  KEEP: the tile's values stay in LDS after the last
+// pass (unreduced, element idx of the tile at lds[padded(idx)]) for a consumer
+// in the same workgroup, and dst is unused.
+template <bool FWD, bool LZ, bool PAIRED, uint32_t S, uint32_t RM, uint32_t ROWS, int FIN, bool ROWTAB,
+          class Sync, bool KEEP = false>
 GF_HD void row_tile(const NttJob& j, const TileDesc& d, const GF_GLOBAL uint64_t* src, GF_GLOBAL uint64_t* dst,
                     uint32_t off, uint64_t* lds, uint32_t tid, uint32_t nthr, Sync sync) {
     constexpr uint32_t N = 1u << S, NC = num_chunks(S, RM), WORDS = ROWS * N;
@@ -459,6 +462,7 @@ GF_HD void row_tile(const NttJob& j, const TileDesc& d, const GF_GLOBAL uint64_t
     using PL = PassOf<FWD, S, RM, NC - 1>;
     constexpr bool in_direct = P0::wide_runs;
     constexpr bool out_direct = PL::wide_runs;
+    static_assert(!KEEP || !out_direct, "a kept tile ends in LDS");
     if constexpr (!in_direct) {
         wave_units<P0::R, WORDS>(tid, nthr, [&](uint32_t idx) { lds[padded(idx)] = src[off + idx]; });
         sync.wave();
@@ -483,7 +487,7 @@ GF_HD void row_tile(const NttJob& j, const TileDesc& d, const GF_GLOBAL uint64_t
     } else {
         chunks(tw);
     }
-    if constexpr (!out_direct) {
+    if constexpr (!out_direct && !KEEP) {
         sync.wave();
         wave_units<PL::R, WORDS>(tid, nthr, [&](uint32_t idx) { gout(0, idx, 0, lds[padded(idx)]); });
     }

@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <vector>
 
+#include "ksi.hpp"
 #include "ntt_engine.hpp"
 
 using namespace gf;
@@ -97,6 +98,62 @@ extern "C" void emu_ntt(uint64_t* x, uint32_t n, uint64_t q, const uint64_t* tw,
     emu_ntt_ex(x, n, q, tw, twp, fwd, post, postp, paired, rmax, 0, nullptr, nullptr, 0, 0);
 }
 extern "C" uint64_t emu_lazy_red_of(uint64_t q) { return lazy_red_of(q); }
+
+namespace {
+template <uint32_t RM, bool LZ>
+void emu_ksi_t(uint32_t n, uint64_t q, const uint64_t* t2, uint32_t np, uint32_t xf, uint64_t* src,
+               const uint64_t* kb, const uint64_t* ka, uint64_t* o0, uint64_t* o1, uint32_t pairs) {
+    const NttGeom g = make_geom(n, RM);
+    std::vector<uint64_t> rows(row_table_words(g));
+    build_row_table(t2, g, rows.data());
+    KsiJob J{};
+    J.np = np;
+    J.xf = xf;
+    J.tw = t2;
+    J.twr = rows.data();
+    J.q = q;
+    J.qneg = host_qneg(q);
+    J.red = lazy_red_of(q);
+    J.o0 = o0;
+    J.o1 = o1;
+    J.pairs = pairs;
+    std::vector<uint64_t> lds(NTT_TILE_PADDED > (size_t)np * KSI_PART_WORDS ? NTT_TILE_PADDED : (size_t)np * KSI_PART_WORDS);
+    with_split(g.logN1, g.logN2, [&](auto A, auto B) {
+        constexpr uint32_t S1 = decltype(A)::value, S2 = decltype(B)::value;
+        for (uint32_t p = 0; p < np; ++p) {
+            J.src[p] = src + (size_t)p * n;
+            J.kb[p] = kb + (size_t)p * n;
+            J.ka[p] = ka + (size_t)p * n;
+            if ((xf >> p) & 1) {   // the column pass, in place, as the plans' NTT_FA step
+                NttJob t{src + (size_t)p * n, src + (size_t)p * n, t2, nullptr, q, 0, 0};
+                t.red = J.red;
+                for (uint32_t tile = 0; tile < (1u << S2) / g.C; ++tile)
+                    phaseA_body<true, LZ, true, S1, RM>(t, g, tile, lds.data(), 0, 1, NoSync{});
+            }
+        }
+        for (uint32_t tile = 0; tile < n / KSI_TILE; ++tile)
+            ksi_body<S2, RM, LZ>(J, 1u << S1, tile, lds.data(), 0, 1, 0, 1, 0, 1, NoSync{}, NoSync{});
+    });
+}
+}  // namespace
+
+// Fused digit row pass + key inner product over np parts: part p of src holds
+// a coefficient-form tower (bit p of xf: transformed here, column pass then the
+// fused row pass) or an evaluation-form digit; o = sum_p NTT?(src_p) k_p R^-1.
+extern "C" void emu_ksi(uint32_t n, uint64_t q, const uint64_t* tw, const uint64_t* twp, uint32_t rmax,
+                        uint32_t np, uint32_t xf, uint64_t* src, const uint64_t* kb, const uint64_t* ka,
+                        uint64_t* o0, uint64_t* o1, uint32_t pairs) {
+    std::vector<uint64_t> t2(2 * (std::size_t)n);
+    for (uint32_t i = 0; i < n; ++i) {
+        t2[2 * i] = tw[i];
+        t2[2 * i + 1] = twp[i];
+    }
+    with_radix(rmax, [&](auto Rc) {
+        constexpr uint32_t RM = decltype(Rc)::value;
+        if (lazy_red_of(q)) emu_ksi_t<RM, true>(n, q, t2.data(), np, xf, src, kb, ka, o0, o1, pairs);
+        else emu_ksi_t<RM, false>(n, q, t2.data(), np, xf, src, kb, ka, o0, o1, pairs);
+    });
+}
 // the key inner product's output with unreduced digits: sum_s a[s] b[s]
 // accumulated as 128 bits (mul128/add128), then redc_red
 extern "C" uint64_t emu_inner_red(const uint64_t* a, const uint64_t* b, uint32_t ns, uint64_t q) {

@@ -41,6 +41,9 @@ def emu():
                                C.c_int, C.c_uint32, C.c_int, u64p, u64p, C.c_uint64, C.c_uint64]
     lib.emu_lazy_red_of.restype = C.c_uint64
     lib.emu_lazy_red_of.argtypes = [C.c_uint64]
+    lib.emu_ksi.restype = None
+    lib.emu_ksi.argtypes = [C.c_uint32, C.c_uint64, u64p, u64p, C.c_uint32, C.c_uint32, C.c_uint32, u64p, u64p,
+                            u64p, u64p, u64p, C.c_uint32]
     lib.emu_inner_red.restype = C.c_uint64
     lib.emu_inner_red.argtypes = [u64p, u64p, C.c_uint32, C.c_uint64]
     lib.emu_reduce_red.restype = C.c_uint64
@@ -214,3 +217,43 @@ def test_inner_product_unreduced_digits(emu, bits):
             av, bv = np.asarray(a[:ns], np.uint64), np.asarray(b[:ns], np.uint64)
             want = sum(x * y for x, y in zip(a[:ns], b[:ns])) * Rinv % q
             assert emu.emu_inner_red(O.P(av), O.P(bv), ns, q) == want, (ns, a[:ns], b[:ns])
+
+
+@pytest.mark.parametrize("logn,np_,xf", [(13, 4, 0b1110), (14, 3, 0b101), (16, 4, 0b0111), (16, 1, 0b1),
+                                         (15, 7, 0b1011111)])
+def test_fused_rowpass_inner_product(emu, logn, np_, xf):
+    """The fused step of the keyswitch: each transformed part goes through the
+    column pass and then the row pass inside the fused kernel body, the other
+    parts are evaluation-form digits as given; the result equals
+    sum_p digit_p * key_p * 2^-64 mod q with digit_p the oracle's forward
+    transform, for a modulus below 2^57 (unreduced digits) and one above
+    (reduced digits), every register radix."""
+    n = 1 << logn
+    for bits in (50, 59):
+        q = int(O.native_primes(1, bits, n)[0])
+        psi = int(O.lib.or_native_root(n, q))
+        fr, fp, _, _, _, _ = O.root_tables(n, q, psi)
+        rng = np.random.default_rng(logn * 7 + bits + np_)
+        src = rng.integers(0, q, (np_, n), dtype=np.uint64)
+        src[0, :3] = q - 1
+        kb = rng.integers(0, q, (np_, n), dtype=np.uint64)
+        ka = rng.integers(0, q, (np_, n), dtype=np.uint64)
+        dig = [O.ntt(src[p], q, fr, fp) if (xf >> p) & 1 else src[p] for p in range(np_)]
+        Rinv = pow(1 << 64, -1, q)
+        w0 = np.zeros(n, dtype=object)
+        w1 = np.zeros(n, dtype=object)
+        for p in range(np_):
+            d = dig[p].astype(object)
+            w0 += d * kb[p].astype(object)
+            w1 += d * ka[p].astype(object)
+        w0 = np.array([int(v) * Rinv % q for v in w0], np.uint64)
+        w1 = np.array([int(v) * Rinv % q for v in w1], np.uint64)
+        for rmax in (2, 3, 4):
+            pairs = rmax % 2   # both access widths of the inner product
+            s = src.copy()
+            o0 = np.zeros(n, np.uint64)
+            o1 = np.zeros(n, np.uint64)
+            emu.emu_ksi(n, q, O.P(fr), O.P(fp), rmax, np_, xf, O.P(s), O.P(np.ascontiguousarray(kb)),
+                        O.P(np.ascontiguousarray(ka)), O.P(o0), O.P(o1), pairs)
+            assert np.array_equal(o0, w0), (bits, rmax, pairs)
+            assert np.array_equal(o1, w1), (bits, rmax, pairs)

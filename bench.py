@@ -67,27 +67,34 @@ def kernel_bytes(name, kind, nkern, total, epilogue):
     return [tr // 2, tr - tr // 2 + epilogue]
 
 
-def step_bytes(n, L, K, alpha, np_parts, kinds):
+def step_bytes(n, L, K, alpha, np_parts, kinds, batch):
     """Algorithmic HBM bytes per operation of each plan step, in plan order:
     each transform reads and writes its tower once (16 n B), pointwise steps
     move their operands once, conversions read sources once and write
-    targets once, the key inner product reads every digit and key row once.
-    Entries are (stage, bytes, bytes read by a fused transform epilogue)."""
+    targets once, the key inner product reads every digit once and every key
+    row once per launch: the batch's ops share one key, so a key row counts
+    1 / batch per op (B_key = batch).  Entries are (stage, bytes, bytes read
+    by a fused transform epilogue)."""
     T = 8 * n
+    key = 2 * np_parts / batch   # key rows read per tower row, per op
     sc = [L - min(alpha, L - alpha * p) + K for p in range(np_parts)]
     sp = [min(alpha, L - alpha * p) for p in range(np_parts)]
-    if len(kinds) == 9:   # fused ModDown + rescale plan
+    if "ksi" in kinds:
+        assert kinds.count("ksi") == 2 and len(kinds) == 10, kinds
+    if "ksi" in kinds or len(kinds) == 9:   # fused ModDown + rescale plan
         if "ksi" in kinds:
             # the digits' column pass, then their row pass fused with the key
-            # inner product: half of each digit transform's 16 n B, plus the
+            # inner product, one step for the Q rows (np - 1 transformed parts,
+            # the input's own digit read as is) and one for the P rows (np
+            # transformed): half of each digit transform's 16 n B, plus the
             # inner product's operands other than the transformed digits
             # (the input's own digit towers, both key rows, both outputs)
             digits = [("ntt_digits_col", T * sum(sc), 0),
-                      ("rowpass_inner_product",
-                       T * sum(sc) + T * (sum(sp) + 2 * np_parts * (L + K) + 2 * (L + K)), 0)]
+                      ("rowpass_inner_q", T * L * ((np_parts - 1) + 1 + key + 2), 0),
+                      ("rowpass_inner_p", T * K * (np_parts + key + 2), 0)]
         else:
             digits = [("ntt_digits", 2 * T * sum(sc), 0),
-                      ("key_inner_product", T * (L + K) * (3 * np_parts + 2), 0)]
+                      ("key_inner_product", T * (L + K) * (np_parts + key + 2), 0)]
         return [
             ("tensor", 7 * T * L, 0),
             ("intt_modup", 2 * T * (L + 2), 0),             # + INTT of t0/t1's last tower
@@ -104,7 +111,7 @@ def step_bytes(n, L, K, alpha, np_parts, kinds):
         1: ("intt_modup", 2 * T * L),
         2: ("conv_modup", T * (sum(sp) + sum(sc))),
         3: ("ntt_digits", 2 * T * sum(sc)),
-        4: ("key_inner_product", T * (L + K) * (3 * np_parts + 2)),
+        4: ("key_inner_product", T * (L + K) * (np_parts + key + 2)),
         5: ("intt_moddown", 2 * T * 2 * K),
         6: ("conv_moddown", T * 2 * (K + L)),
         7: ("ntt_moddown", 2 * T * 2 * L),
@@ -489,11 +496,11 @@ def main():
 
     stages = time_steps(G, res["plan"], stream)
     np_parts = -(-L // alpha)
-    sb = step_bytes(N, L, K, alpha, np_parts, [s[0] for s in stages])
+    sb = step_bytes(N, L, K, alpha, np_parts, [s[0] for s in stages], B)
     stage_report, kernels = [], []
     for (kind, jobs, ms, kms), (name, bytes_per_op, epi) in zip(stages, sb):
         gbs = bytes_per_op * B / (ms / 1e3) / 1e9
-        kb = kernel_bytes(name, kind, len(kms), bytes_per_op * B, epi * B)
+        kb = kernel_bytes(name, kind, len(kms), int(round(bytes_per_op * B)), int(round(epi * B)))
         kinfo = []
         for kname, kt, kbytes in zip(STEP_KERNELS[kind], kms, kb):
             kernels.append(dict(stage=name, kernel=kname, ms=kt, bytes=kbytes))

@@ -501,12 +501,7 @@ void Plan::run_jobs(size_t i, hipStream_t s, uint32_t which, uint32_t first, uin
         case Step::RSV: launch_rs(GF_J(RsJob), cnt, n, s); break;
         case Step::PTACC: launch_ptacc(GF_J(PtAccJob), cnt, n, s); break;
         case Step::NTT_FA: launch_ntt(GF_J(NttJob), cnt, geom, true, s, 1); break;
-        case Step::KSI:
-            for (const KsiSeg& g : ksi_segs) {
-                const uint32_t a = std::max(first, g.first), b = std::min(first + cnt, g.first + g.count);
-                if (a < b) launch_ksi((const KsiJob*)p + a, b - a, geom, g.regions, g.waves, s);
-            }
-            break;
+        case Step::KSI: launch_ksi(GF_J(KsiJob), cnt, geom, st.arg & 0xFF, st.arg >> 8, s); break;
     }
 #undef GF_J
 }
@@ -1040,17 +1035,11 @@ std::unique_ptr<Plan> PlanBuilder::layout(std::vector<char>& blob) {
         j.twr = C_.row_table_of(j.tw);
         if (!j.twr) throw std::runtime_error("gpufhe: transform table of another context");
     }
+    // KSI jobs by transformed-part count (one step each: that many LDS
+    // regions and waves), by key within a step
     auto nxf = [](const KsiJob& j) { return (uint32_t)__builtin_popcount(j.xf); };
     group_by(ksi_, [](const KsiJob& j) { return (uintptr_t)j.kb[0]; });
     group_by(ksi_, nxf);
-    std::vector<Plan::KsiSeg> ksi_segs;
-    for (uint32_t i = 0; i < ksi_.size();) {
-        uint32_t e = i;
-        while (e < ksi_.size() && nxf(ksi_[e]) == nxf(ksi_[i])) ++e;
-        const uint32_t x = nxf(ksi_[i]);
-        ksi_segs.push_back(Plan::KsiSeg{i, e - i, x, std::max(1u, std::min<uint32_t>(x, KSI_WAVES))});
-        i = e;
-    }
     group_by(inner_, [](const InnerJob& j) { return (uintptr_t)j.kb[0]; });
     group_by(conv_up_, [](const ConvJob& j) { return (uintptr_t)j.tab; });
     group_by(tensor_, [](const TensorJob& j) { return j.q; });
@@ -1058,7 +1047,6 @@ std::unique_ptr<Plan> PlanBuilder::layout(std::vector<char>& blob) {
     P->n = C_.n;
     P->logn = C_.logn;
     P->geom = C_.geom;
-    P->ksi_segs = std::move(ksi_segs);
     blob.clear();
     auto stage = [&](Step::Kind kind, const void* data, size_t cnt, size_t sz) {
         if (!cnt) return;
@@ -1076,7 +1064,14 @@ std::unique_ptr<Plan> PlanBuilder::layout(std::vector<char>& blob) {
     GF_STAGE(NTT_FA, ntt_diga_);
     GF_STAGE(PERM, perm_dig_);
     GF_STAGE(INNER, inner_);
-    GF_STAGE(KSI, ksi_);
+    for (size_t i = 0; i < ksi_.size();) {
+        size_t e = i;
+        while (e < ksi_.size() && nxf(ksi_[e]) == nxf(ksi_[i])) ++e;
+        const uint32_t x = nxf(ksi_[i]), waves = std::max(1u, std::min<uint32_t>(x, KSI_WAVES));
+        stage(Step::KSI, ksi_.data() + i, e - i, sizeof(KsiJob));
+        P->steps.back().arg = x | waves << 8;
+        i = e;
+    }
     GF_STAGE(NTT_I, intt_p_);
     GF_STAGE(CONV, conv_dn_);
     GF_STAGE(NTT_F, ntt_conv_);

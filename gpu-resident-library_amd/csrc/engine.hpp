@@ -129,6 +129,9 @@ struct Step {
                 NTT_FA, KSI } kind;
     size_t off;     // byte offset of the first job in the plan's device buffer
     uint32_t count;
+    // KSI: LDS regions (bits 0-7) and waves per workgroup (bits 8-15); the
+    // jobs of one KSI step share their transformed-part count
+    uint32_t arg = 0;
 };
 
 class Plan {
@@ -138,12 +141,6 @@ public:
     const void* base = nullptr;  // where the job arrays live (dev.p or a caller arena)
     uint32_t n = 0, logn = 0;
     NttGeom geom{};
-    // the KSI step's jobs come sorted by transformed-part count; each run of
-    // equal count is one launch with that many waves and LDS regions
-    struct KsiSeg {
-        uint32_t first, count, regions, waves;
-    };
-    std::vector<KsiSeg> ksi_segs;
     void run(hipStream_t s) const;
     // which: 3 = every kernel of step i; 1 / 2 = only its first / second kernel
     void run_step(size_t i, hipStream_t s, uint32_t which = 3) const;

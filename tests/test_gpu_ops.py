@@ -12,8 +12,11 @@ pytestmark = pytest.mark.gpu
 # (n, L, dnum): small ring, two-phase NTT ring, the BASELINE config shapes
 SHAPES = [(1 << 10, 4, 2), (1 << 13, 6, 3), (1 << 15, 4, 2), (1 << 15, 12, 3), (1 << 16, 24, 4)]
 # edge shapes: one digit (alpha = L), one tower per digit (alpha = 1), the
-# largest ring the transforms support, the smallest ring
-EDGE_SHAPES = [(1 << 12, 4, 1), (1 << 12, 6, 6), (1 << 17, 3, 1), (1 << 4, 3, 3)]
+# largest ring the transforms support, the smallest ring; at a two-phase ring
+# the most parts the fused row pass + inner product takes (7, more parts than
+# waves) and one part more (the separate kernels)
+EDGE_SHAPES = [(1 << 12, 4, 1), (1 << 12, 6, 6), (1 << 17, 3, 1), (1 << 4, 3, 3), (1 << 13, 7, 7),
+               (1 << 13, 8, 8)]
 
 
 @pytest.fixture(scope="module")

@@ -121,8 +121,10 @@ struct WaveSync {
     __device__ void wave() const { DevSync{}.wave(); }
 };
 
+// At most 96 VGPRs (5 waves per SIMD; 82 without spills): a 3-wave workgroup
+// of the Q rows takes 26 KiB of LDS, so 6 of them fit a CU.
 template <uint32_t S2, uint32_t RM>
-__global__ __launch_bounds__(KSI_THREADS) void k_ksi(const KsiJob* __restrict__ jobs, uint32_t off, uint32_t N1) {
+__global__ __launch_bounds__(KSI_THREADS) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_ksi(const KsiJob* __restrict__ jobs, uint32_t off, uint32_t N1) {
     extern __shared__ uint64_t lds[];
     const KsiJob& j = jobs[off + blockIdx.y];
     const uint32_t tid = threadIdx.x, nthr = blockDim.x;

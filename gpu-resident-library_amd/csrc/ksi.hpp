@@ -43,26 +43,27 @@ struct KsiJob {
     uint32_t pairs;         // every tower pointer 16-byte aligned: two positions per access
 };
 
-// V consecutive words (V = 2: one 16-byte access; the address is 16-byte aligned)
+// V consecutive words from word i of a tower (V = 2: one 16-byte access; the
+// address is 16-byte aligned), addressed as base + 32-bit byte offset (gat)
 struct alignas(16) Words2 {
     uint64_t w[2];
 };
 template <uint32_t V>
-GF_HD void load_words(const GF_GLOBAL uint64_t* p, uint64_t* out) {
+GF_HD void load_words(const GF_GLOBAL uint64_t* base, uint32_t i, uint64_t* out) {
     if constexpr (V == 2) {
-        const Words2 x = *(const GF_GLOBAL Words2*)p;
+        const Words2 x = gat((const GF_GLOBAL Words2*)base, i / 2);
         out[0] = x.w[0];
         out[1] = x.w[1];
     } else {
-        out[0] = p[0];
+        out[0] = gat(base, i);
     }
 }
 template <uint32_t V>
-GF_HD void store_words(GF_GLOBAL uint64_t* p, const uint64_t* in) {
+GF_HD void store_words(GF_GLOBAL uint64_t* base, uint32_t i, const uint64_t* in) {
     if constexpr (V == 2) {
-        *(GF_GLOBAL Words2*)p = Words2{{in[0], in[1]}};
+        gat((GF_GLOBAL Words2*)base, i / 2) = Words2{{in[0], in[1]}};
     } else {
-        p[0] = in[0];
+        gat(base, i) = in[0];
     }
 }
 
@@ -107,10 +108,10 @@ GF_HD void ksi_body(const KsiJob& j, uint32_t N1, uint32_t tile, uint64_t* lds, 
                         if constexpr (!LZ) dv[v] = ntt_final<true>(dv[v], t);
                     }
                 } else {
-                    load_words<V>(gcast(j.src[p]) + gi, dv);
+                    load_words<V>(gcast(j.src[p]), gi, dv);
                 }
-                load_words<V>(gcast(j.kb[p]) + gi, kb);
-                load_words<V>(gcast(j.ka[p]) + gi, ka);
+                load_words<V>(gcast(j.kb[p]), gi, kb);
+                load_words<V>(gcast(j.ka[p]), gi, ka);
                 for (uint32_t v = 0; v < V; ++v) {
                     s0[v] = add128(s0[v], mul128(dv[v], kb[v]));
                     s1[v] = add128(s1[v], mul128(dv[v], ka[v]));
@@ -126,8 +127,8 @@ GF_HD void ksi_body(const KsiJob& j, uint32_t N1, uint32_t tile, uint64_t* lds, 
                     r1[v] = redc(s1[v], q, qneg);
                 }
             }
-            store_words<V>(gcast(j.o0) + gi, r0);
-            store_words<V>(gcast(j.o1) + gi, r1);
+            store_words<V>(gcast(j.o0), gi, r0);
+            store_words<V>(gcast(j.o1), gi, r1);
         }
     };
     if (j.pairs) inner(std::integral_constant<uint32_t, 2>{});

@@ -48,6 +48,18 @@ template <class T>
 GF_HD const GF_GLOBAL T* gcast(const T* p) { return (const GF_GLOBAL T*)p; }
 template <class T>
 GF_HD GF_GLOBAL T* gcast(T* p) { return (GF_GLOBAL T*)p; }
+// Element i of a tower-sized array as a base plus a 32-bit byte offset (a
+// tower is at most 1 MiB): with a workgroup-uniform base, loads and stores take
+// the scalar-base form with one 32-bit VGPR offset, not a 64-bit VGPR address
+// built per element.
+template <class T>
+GF_HD T& gat(GF_GLOBAL T* base, uint32_t i) {
+    return *(GF_GLOBAL T*)((GF_GLOBAL char*)base + i * (uint32_t)sizeof(T));
+}
+template <class T>
+GF_HD const T& gat(const GF_GLOBAL T* base, uint32_t i) {
+    return *(const GF_GLOBAL T*)((const GF_GLOBAL char*)base + i * (uint32_t)sizeof(T));
+}
 
 struct NttJob {
     const uint64_t* in;   // tower read by the first phase
@@ -351,8 +363,8 @@ GF_HD uint64_t ntt_final_ep(uint64_t v, const NttJob& j, uint64_t gi) {
     if constexpr (LZ) {
         if (j.e0) {
             const uint64_t nq = 0 - j.q;
-            v += shoup_lazy4(gcast(j.e0)[gi], j.c0, j.c0p, nq);
-            v += shoup_lazy4(gcast(j.e1)[gi], j.c1, j.c1p, nq);
+            v += shoup_lazy4(gat(gcast(j.e0), gi), j.c0, j.c0p, nq);
+            v += shoup_lazy4(gat(gcast(j.e1), gi), j.c1, j.c1p, nq);
         } else if (j.lazy_out) {
             return v;
         }
@@ -360,8 +372,8 @@ GF_HD uint64_t ntt_final_ep(uint64_t v, const NttJob& j, uint64_t gi) {
     } else {
         uint64_t r = csub(csub(csub(v, 4 * j.q), 2 * j.q), j.q);
         if (j.e0) {
-            r = addmod(r, shoup(gcast(j.e0)[gi], j.c0, j.c0p, j.q), j.q);
-            r = addmod(r, shoup(gcast(j.e1)[gi], j.c1, j.c1p, j.q), j.q);
+            r = addmod(r, shoup(gat(gcast(j.e0), gi), j.c0, j.c0p, j.q), j.q);
+            r = addmod(r, shoup(gat(gcast(j.e1), gi), j.c1, j.c1p, j.q), j.q);
         }
         return r;
     }
@@ -379,7 +391,7 @@ GF_HD uint64_t ntt_final_ep(uint64_t v, const NttJob& j, uint64_t gi) {
 struct ColIn {
     const GF_GLOBAL uint64_t* src;
     uint32_t logN2, b0;
-    GF_HD uint64_t operator()(uint32_t s, uint32_t k, uint32_t) const { return src[(k << logN2) + b0 + s]; }
+    GF_HD uint64_t operator()(uint32_t s, uint32_t k, uint32_t) const { return gat(src, (k << logN2) + b0 + s); }
 };
 template <bool FWD>
 struct ColOut {
@@ -388,7 +400,7 @@ struct ColOut {
     const NttJob& j;
     // forward phase A hands lazy values in [0,8q) to phase B
     GF_HD void operator()(uint32_t s, uint32_t k, uint32_t, uint64_t v) const {
-        dst[(k << logN2) + b0 + s] = FWD ? v : ntt_final<false>(v, j);
+        gat(dst, (k << logN2) + b0 + s) = FWD ? v : ntt_final<false>(v, j);
     }
 };
 
@@ -410,7 +422,7 @@ GF_HD void phaseA_body(const NttJob& j, const NttGeom& g, uint32_t tile, uint64_
 struct RowIn {
     const GF_GLOBAL uint64_t* src;   // tile start
     uint32_t N;
-    GF_HD uint64_t operator()(uint32_t s, uint32_t k, uint32_t) const { return src[s * N + k]; }
+    GF_HD uint64_t operator()(uint32_t s, uint32_t k, uint32_t) const { return gat(src, s * N + k); }
 };
 // final store of a row tile: forward values through the epilogue (FIN = 2),
 // inverse values through the n^-1 multiply (FIN = 1) or lazy (FIN = 0)
@@ -421,7 +433,7 @@ struct RowOut {
     const NttJob& j;
     GF_HD void operator()(uint32_t s, uint32_t k, uint32_t, uint64_t v) const {
         const uint32_t gi = off + s * N + k;
-        dst[gi] = FIN == 2 ? ntt_final_ep<LZ>(v, j, gi) : FIN == 1 ? ntt_final<false>(v, j) : v;
+        gat(dst, gi) = FIN == 2 ? ntt_final_ep<LZ>(v, j, gi) : FIN == 1 ? ntt_final<false>(v, j) : v;
     }
 };
 
@@ -464,7 +476,7 @@ GF_HD void row_tile(const NttJob& j, const TileDesc& d, const GF_GLOBAL uint64_t
     constexpr bool out_direct = PL::wide_runs;
     static_assert(!KEEP || !out_direct, "a kept tile ends in LDS");
     if constexpr (!in_direct) {
-        wave_units<P0::R, WORDS>(tid, nthr, [&](uint32_t idx) { lds[padded(idx)] = src[off + idx]; });
+        wave_units<P0::R, WORDS>(tid, nthr, [&](uint32_t idx) { lds[padded(idx)] = gat(src, off + idx); });
         sync.wave();
     }
     const RowIn gin{src + off, N};

@@ -182,11 +182,19 @@ void launch_ntt_one(const NttJob& job, const NttGeom& g, bool fwd, hipStream_t s
         for (uint32_t i = ((blockIdx.x * (PW_EPT / 2) + e_) * PW_THREADS + threadIdx.x) * 2; \
              i < (n); i = 0xFFFFFFFFu)
 
+// Word i of a job's array as a global access with the job's (workgroup-
+// uniform) base and a 32-bit byte offset (gat): no flat access, no 64-bit
+// address per element.  Every array a job names spans fewer than 2^29 words.
+template <class T>
+__device__ __forceinline__ const T& ga(const T* p, uint32_t i) { return gat(gcast(p), i); }
+template <class T>
+__device__ __forceinline__ T& ga(T* p, uint32_t i) { return gat(gcast(p), i); }
+
 struct alignas(16) Pair {
     uint64_t v[2];
 };
-__device__ __forceinline__ Pair ld2(const uint64_t* p, uint32_t i) { return *(const Pair*)(p + i); }
-__device__ __forceinline__ void st2(uint64_t* p, uint32_t i, const Pair& x) { *(Pair*)(p + i) = x; }
+__device__ __forceinline__ Pair ld2(const uint64_t* p, uint32_t i) { return ga((const Pair*)p, i / 2); }
+__device__ __forceinline__ void st2(uint64_t* p, uint32_t i, const Pair& x) { ga((Pair*)p, i / 2) = x; }
 
 __device__ __forceinline__ void tensor_one(const TensorJob& j, uint64_t a0, uint64_t a1, uint64_t b0,
                                            uint64_t b1, uint64_t& t0, uint64_t& t1, uint64_t& t2) {
@@ -210,7 +218,13 @@ __global__ __launch_bounds__(PW_THREADS) void k_tensor(const TensorJob* __restri
             st2(j.t2, i, t2);
         }
     } else {
-        PW_FOR(n) { tensor_one(j, j.c0a[i], j.c1a[i], j.c0b[i], j.c1b[i], j.t0[i], j.t1[i], j.t2[i]); }
+        PW_FOR(n) {
+            uint64_t t0, t1, t2;
+            tensor_one(j, ga(j.c0a, i), ga(j.c1a, i), ga(j.c0b, i), ga(j.c1b, i), t0, t1, t2);
+            ga(j.t0, i) = t0;
+            ga(j.t1, i) = t1;
+            ga(j.t2, i) = t2;
+        }
     }
 }
 
@@ -241,10 +255,10 @@ __device__ __forceinline__ void with_nsrc(uint32_t ns, F&& f) {
 template <uint32_t NS>
 struct SrcCols {
     uint32_t lo[NS > 0 ? NS : 1], hi[NS > 0 ? NS : 1];
-    __device__ __forceinline__ void load(const uint64_t* src, uint64_t n, uint32_t i) {
+    __device__ __forceinline__ void load(const uint64_t* src, uint32_t n, uint32_t i) {
 #pragma unroll
         for (uint32_t s = 0; s < NS; ++s) {
-            const uint64_t v = src[(uint64_t)s * n + i];
+            const uint64_t v = ga(src, s * n + i);
             lo[s] = split30_lo(v);
             hi[s] = split30_hi(v);
         }
@@ -256,13 +270,13 @@ struct SrcCols {
 // NS = 0 (more than 8 sources, or a modulus of 2^60 or more): runtime count,
 // full 128-bit products, sources read from memory.
 template <uint32_t NS>
-__device__ __forceinline__ U128 dot_src(const uint64_t* src, uint64_t n, uint32_t i, uint32_t ns, cu64p tab,
+__device__ __forceinline__ U128 dot_src(const uint64_t* src, uint32_t n, uint32_t i, uint32_t ns, cu64p tab,
                                         uint32_t stride, uint32_t t, const SrcCols<NS>& x) {
     if constexpr (NS > 0) {
         return dot30<NS>(x.lo, x.hi, [&](uint32_t s) { return tab[s * stride + t]; });
     } else {
         U128 acc{0, 0};
-        for (uint32_t s = 0; s < ns; ++s) acc = add128(acc, mul128(src[(uint64_t)s * n + i], tab[s * stride + t]));
+        for (uint32_t s = 0; s < ns; ++s) acc = add128(acc, mul128(ga(src, s * n + i), tab[s * stride + t]));
         return acc;
     }
 }
@@ -274,7 +288,7 @@ __device__ __forceinline__ void conv_one(const ConvJob& j, uint32_t n, uint32_t 
     const cu64p tab = uni(j.tab), tq = uni(j.tq), tqn = uni(j.tqneg);
     for (uint32_t t = j.j0; t < j.j1; ++t) {
         const U128 acc = dot_src<NS>(j.src, n, i, j.nsrc, tab, j.ntgt, t, x);
-        j.dst[(uint64_t)t * n + i] = redc(acc, tq[t], tqn[t]);
+        ga(j.dst, t * n + i) = redc(acc, tq[t], tqn[t]);
     }
 }
 
@@ -320,12 +334,12 @@ __global__ __launch_bounds__(PW_THREADS) void k_inner(const InnerJob* __restrict
         PW_FOR(n) {
             U128 a0{0, 0}, a1{0, 0};
             for (uint32_t p = 0; p < np; ++p) {
-                const uint64_t d = j.dig[p][i];
-                a0 = add128(a0, mul128(d, j.kb[p][i]));
-                a1 = add128(a1, mul128(d, j.ka[p][i]));
+                const uint64_t d = ga(j.dig[p], i);
+                a0 = add128(a0, mul128(d, ga(j.kb[p], i)));
+                a1 = add128(a1, mul128(d, ga(j.ka[p], i)));
             }
-            o0[i] = inner_out(a0, j);
-            o1[i] = inner_out(a1, j);
+            ga(o0, i) = inner_out(a0, j);
+            ga(o1, i) = inner_out(a1, j);
         }
     }
 }
@@ -334,9 +348,9 @@ __global__ __launch_bounds__(PW_THREADS) void k_combine(const CombineJob* __rest
                                                         uint32_t off, uint32_t n) {
     const CombineJob j = jobs[off + blockIdx.y];
     PW_FOR(n) {
-        uint64_t v = shoup(submod(j.res[i], j.conv[i], j.q), j.pinv, j.pinvp, j.q);
-        if (j.t) v = addmod(v, j.t[i], j.q);
-        j.out[i] = v;
+        uint64_t v = shoup(submod(ga(j.res, i), ga(j.conv, i), j.q), j.pinv, j.pinvp, j.q);
+        if (j.t) v = addmod(v, ga(j.t, i), j.q);
+        ga(j.out, i) = v;
     }
 }
 
@@ -350,14 +364,14 @@ __device__ __forceinline__ uint64_t switch_one(uint64_t x, const SwitchJob& j) {
 __global__ __launch_bounds__(PW_THREADS) void k_switch(const SwitchJob* __restrict__ jobs,
                                                        uint32_t off, uint32_t n) {
     const SwitchJob j = jobs[off + blockIdx.y];
-    PW_FOR(n) { j.out[i] = switch_one(j.in[i], j); }
+    PW_FOR(n) { ga(j.out, i) = switch_one(ga(j.in, i), j); }
 }
 
 __global__ __launch_bounds__(PW_THREADS) void k_fuse(const FuseJob* __restrict__ jobs,
                                                      uint32_t off, uint32_t n) {
     const FuseJob j = jobs[off + blockIdx.y];
     PW_FOR(n) {
-        j.out[i] = addmod(shoup(j.a[i], j.s1, j.s1p, j.q), shoup(j.b[i], j.s2, j.s2p, j.q), j.q);
+        ga(j.out, i) = addmod(shoup(ga(j.a, i), j.s1, j.s1p, j.q), shoup(ga(j.b, i), j.s2, j.s2p, j.q), j.q);
     }
 }
 
@@ -372,7 +386,7 @@ __global__ __launch_bounds__(PW_THREADS) void k_drop(const DropJob* __restrict__
             x.load(j.xp, n, i);
             const U128 acc = dot_src<K>(j.xp, n, i, j.K, tab, j.stride, 0, x);
             const uint64_t conv = redc(acc, j.q, j.qneg);
-            j.out[i] = addmod(j.dt[i], shoup(submod(j.dr[i], conv, j.q), j.pinv, j.pinvp, j.q), j.q);
+            ga(j.out, i) = addmod(ga(j.dt, i), shoup(submod(ga(j.dr, i), conv, j.q), j.pinv, j.pinvp, j.q), j.q);
         }
     });
 }
@@ -381,7 +395,7 @@ template <uint32_t K>
 __device__ __forceinline__ void rs_one(const RsJob& j, uint32_t n, uint32_t i) {
     SrcCols<K> x;
     x.load(j.xp, n, i);
-    const uint64_t dv = j.drop[i];
+    const uint64_t dv = ga(j.drop, i);
     const bool hi = dv > j.half;
     const cu64p tab = uni(j.tab);
     static_assert(sizeof(RsTarget) == 7 * 8, "RsTarget layout");
@@ -407,7 +421,7 @@ __device__ __forceinline__ void rs_one(const RsJob& j, uint32_t n, uint32_t i) {
             const uint64_t xm = shoup(dv, 1, tg.red, tg.q);
             sw = xm >= d ? xm - d : xm + tg.q - d;
         }
-        j.out[(uint64_t)t * n + i] = addmod(shoup(sw, tg.s2, tg.s2p, tg.q), conv, tg.q);
+        ga(j.out, t * n + i) = addmod(shoup(sw, tg.s2, tg.s2p, tg.q), conv, tg.q);
     }
 }
 
@@ -453,20 +467,20 @@ __global__ __launch_bounds__(PW_THREADS) void k_ptacc(const PtAccJob* __restrict
     PW_FOR(n) {
         U128 a0{0, 0}, a1{0, 0};
         for (uint32_t t = 0; t < m; ++t) {
-            const uint64_t p = j.pt[t][i];
-            a0 = add128(a0, mul128(p, j.x0[t][i]));
-            a1 = add128(a1, mul128(p, j.x1[t][i]));
+            const uint64_t p = ga(j.pt[t], i);
+            a0 = add128(a0, mul128(p, ga(j.x0[t], i)));
+            a1 = add128(a1, mul128(p, ga(j.x1[t], i)));
         }
         // REDC(REDC(sum) R^2) = sum mod q (needs m q < 2^64)
-        j.o0[i] = redc(mul128(redc(a0, q, qneg), r2), q, qneg);
-        j.o1[i] = redc(mul128(redc(a1, q, qneg), r2), q, qneg);
+        ga(j.o0, i) = redc(mul128(redc(a0, q, qneg), r2), q, qneg);
+        ga(j.o1, i) = redc(mul128(redc(a1, q, qneg), r2), q, qneg);
     }
 }
 
 __global__ __launch_bounds__(PW_THREADS) void k_add(const AddJob* __restrict__ jobs,
                                                     uint32_t off, uint32_t n) {
     const AddJob j = jobs[off + blockIdx.y];
-    PW_FOR(n) { j.out[i] = addmod(j.a[i], j.b[i], j.q); }
+    PW_FOR(n) { ga(j.out, i) = addmod(ga(j.a, i), ga(j.b, i), j.q); }
 }
 
 __device__ __forceinline__ uint32_t brv_bits(uint32_t v, uint32_t logn) {
@@ -488,14 +502,14 @@ __global__ __launch_bounds__(PW_THREADS) void k_perm(J jobs, uint32_t n, uint32_
     const uint64_t M = 2ull * n;
     PW_FOR(n) {
         const uint64_t e = ((2ull * brv_bits(i, logn) + 1) * j.k) & (M - 1);
-        j.out[i] = j.in[brv_bits((uint32_t)((e - 1) >> 1), logn)];
+        ga(j.out, i) = ga(j.in, brv_bits((uint32_t)((e - 1) >> 1), logn));
     }
 }
 
 __global__ __launch_bounds__(PW_THREADS) void k_copy(const CopyJob* __restrict__ jobs,
                                                      uint32_t off, uint32_t n) {
     const CopyJob j = jobs[off + blockIdx.y];
-    PW_FOR(n) { j.out[i] = j.in[i]; }
+    PW_FOR(n) { ga(j.out, i) = ga(j.in, i); }
 }
 
 // synthetic uniform residues: splitmix64 of (seed, tower, index) reduced mod q

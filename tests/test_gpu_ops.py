@@ -119,6 +119,35 @@ def test_evalmult_rescale(G, n, L, dnum):
         assert np.array_equal(o1.download(), w1), level
 
 
+@pytest.mark.parametrize("n,L,dnum", [(1 << 13, 6, 3), (1 << 16, 5, 2)])
+def test_evalmult_rescale_ascending_primes(G, n, L, dnum):
+    """Q primes after q0 in ascending order, so the dropped modulus is larger
+    than the towers that survive the rescale: the centred switch subtracts
+    (q_last - q_i) mod q_i instead of adding q_i - q_last."""
+    D = make_params(n, L, dnum)
+    q = [int(D.q[0])] + sorted(int(v) for v in D.q[1:])
+    P = O.Params(n, q, [int(v) for v in D.p], [int(O.lib.or_native_root(n, m)) for m in q + [int(v) for v in D.p]],
+                 dnum)
+    rng = np.random.default_rng(23)
+    av = np.concatenate([rng.integers(0, int(m), n, dtype=np.uint64) for _ in range(P.dnum) for m in P.mods])
+    bv = np.concatenate([rng.integers(0, int(m), n, dtype=np.uint64) for _ in range(P.dnum) for m in P.mods])
+    ctx = G.Context(n, P.q, P.p, P.roots, dnum)
+    k = G.Key(ctx, av, bv, P.dnum)
+    s = G.Stream()
+    ws = ctx.workspace()
+    for fused in (True, False):
+        ctx.set_fusion(fused)
+        for level in sorted({L, 2}):
+            cts = [rand_ct(P, level, rng) for _ in range(4)]
+            d = [G.Dev.of(c) for c in cts]
+            o0, o1 = G.Dev((level - 1) * n), G.Dev((level - 1) * n)
+            G.check(G.lib.gf_evalmult_rescale(ctx.h, k.h, level, *[x.ptr for x in d], o0.ptr, o1.ptr, ws.ptr, s.s))
+            s.sync()
+            w0, w1 = P.evalmult_rescale(level, *cts, av, bv)
+            assert np.array_equal(o0.download(), w0), (fused, level)
+            assert np.array_equal(o1.download(), w1), (fused, level)
+
+
 @pytest.mark.parametrize("n,L,dnum", [SHAPES[1], SHAPES[3]])
 def test_evalmult_rescale_word_aligned_buffers(G, n, L, dnum):
     """Operands, outputs and workspace only 8-byte aligned (one word past a

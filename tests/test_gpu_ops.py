@@ -149,6 +149,37 @@ def test_evalmult_rescale_ascending_primes(G, n, L, dnum):
             assert np.array_equal(o1.download(), w1), (fused, level)
 
 
+@pytest.mark.parametrize("n,L,dnum", [(1 << 13, 5, 2), (1 << 16, 4, 2)])
+def test_wide_moduli_keyswitch_evalmult_rescale(G, n, L, dnum):
+    """A 61-bit q0 and 61-bit special primes beside 50-bit towers: conversions
+    take the full 128-bit products (moduli of 2^60 or more), the 61-bit rows
+    keep the reduced transform butterflies and digits, the 50-bit rows take
+    the lazy ones; every result equals the oracle."""
+    P = make_params(n, L, dnum, qbits=50, q0bits=61, pbits=61)
+    assert max(int(m) for m in P.mods) >> 60
+    rng = np.random.default_rng(61)
+    av = np.concatenate([rng.integers(0, int(m), n, dtype=np.uint64) for _ in range(P.dnum) for m in P.mods])
+    bv = np.concatenate([rng.integers(0, int(m), n, dtype=np.uint64) for _ in range(P.dnum) for m in P.mods])
+    ctx = G.Context(n, P.q, P.p, P.roots, dnum)
+    k = G.Key(ctx, av, bv, P.dnum)
+    s = G.Stream()
+    ws = ctx.workspace()
+    for level in sorted({L, 2}):
+        d = rand_ct(P, level, rng)
+        dd, b0, b1 = G.Dev.of(d), G.Dev(level * n), G.Dev(level * n)
+        G.check(G.lib.gf_keyswitch(ctx.h, k.h, level, dd.ptr, b0.ptr, b1.ptr, ws.ptr, s.s))
+        s.sync()
+        w0, w1 = P.keyswitch(level, d, av, bv)
+        assert np.array_equal(b0.download(), w0) and np.array_equal(b1.download(), w1), ("ks", level)
+        cts = [rand_ct(P, level, rng) for _ in range(4)]
+        dc = [G.Dev.of(c) for c in cts]
+        o0, o1 = G.Dev((level - 1) * n), G.Dev((level - 1) * n)
+        G.check(G.lib.gf_evalmult_rescale(ctx.h, k.h, level, *[x.ptr for x in dc], o0.ptr, o1.ptr, ws.ptr, s.s))
+        s.sync()
+        w0, w1 = P.evalmult_rescale(level, *cts, av, bv)
+        assert np.array_equal(o0.download(), w0) and np.array_equal(o1.download(), w1), ("emr", level)
+
+
 @pytest.mark.parametrize("n,L,dnum", [SHAPES[1], SHAPES[3]])
 def test_evalmult_rescale_word_aligned_buffers(G, n, L, dnum):
     """Operands, outputs and workspace only 8-byte aligned (one word past a

@@ -483,8 +483,9 @@ def main():
         radix, ntt_ms = args.ntt_radix, float("nan")
     else:
         radix, ntt_ms = ctx.autotune(stream)   # picks the NTT register radix for this context
-    log("context ready: K=%d alpha=%d ws=%.1f MiB/op, NTT radix 2^%d (%.3f ms per 96 fwd+inv)"
-        % (K, alpha, ctx.ws_bytes / 2**20, radix, ntt_ms))
+    log("context ready: K=%d alpha=%d ws=%.1f MiB/op, NTT radix 2^%d (%s)"
+        % (K, alpha, ctx.ws_bytes / 2**20, radix,
+           "pinned" if args.ntt_radix else "autotuned, %.3f ms per 96 fwd+inv" % ntt_ms))
     tuned = {"ntt_radix_log2": radix, "pinned": bool(args.ntt_radix),
              "ntt_96_fwd_inv_ms": None if args.ntt_radix else round(ntt_ms, 4)}
 

@@ -648,7 +648,7 @@ const uint64_t* PlanBuilder::digit_tower(uint32_t l, const uint64_t* d, const ui
 // into the workspace accumulators, 5. their P towers to coefficient form
 // (ModDown's sources).  digit(pi, i) names the digit towers.
 template <class F>
-void PlanBuilder::inner_moddown_src(uint32_t l, const DevKey& key, uint64_t* ws, F digit) {
+void PlanBuilder::inner_moddown_src(uint32_t l, const DevKey& key, uint64_t* ws, F digit, uint64_t gk) {
     const Ctx& C = C_;
     const uint32_t n = C.n, K = C.K, W = l + K;
     const LevelConst& lv = C.lv[l];
@@ -671,6 +671,7 @@ void PlanBuilder::inner_moddown_src(uint32_t l, const DevKey& key, uint64_t* ws,
         J.q = C.rc[row].q;
         J.qneg = C.rc[row].qneg;
         J.red = lazy_red_of(J.q);
+        J.gk = gk;
         J.pairs = n % 2 == 0 && aligned16({J.o0, J.o1});
         for (uint32_t pi = 0; pi < lv.np; ++pi) J.pairs &= aligned16({J.dig[pi], J.kb[pi], J.ka[pi]});
         inner_.push_back(J);
@@ -860,7 +861,7 @@ void PlanBuilder::rotate_hoisted(uint32_t l, uint32_t m, const uint64_t* ks, con
                                  const uint64_t* c1, uint64_t* const* o0, uint64_t* const* o1,
                                  const DevKey* const* keys, uint64_t* ws_base, uint64_t* const* ws_rot) {
     const Ctx& C = C_;
-    const uint32_t n = C.n, W = l + C.K;
+    const uint32_t n = C.n;
     if (l < 1 || l > C.L) throw std::runtime_error("gpufhe: bad level");
     const LevelConst& lv = C.lv[l];
     for (uint32_t j = 0; j < m; ++j) {
@@ -874,12 +875,10 @@ void PlanBuilder::rotate_hoisted(uint32_t l, uint32_t m, const uint64_t* ks, con
     for (uint32_t j = 0; j < m; ++j) {
         uint64_t* ws = ws_rot[j];
         uint64_t* a0 = ws + w.t0 * n;
-        uint64_t* pd = ws + w.dig * n;   // permuted digits, full layout [np][W]
         for (uint32_t i = 0; i < l; ++i) perm_.push_back(PermJob{c0 + (size_t)i * n, a0 + (size_t)i * n, ks[j]});
-        for (uint32_t pi = 0; pi < lv.np; ++pi)
-            for (uint32_t i = 0; i < W; ++i)
-                perm_dig_.push_back(PermJob{digit_tower(l, c1, dig, pi, i), pd + ((size_t)pi * W + i) * n, ks[j]});
-        inner_moddown_src(l, *keys[j], ws, [&](uint32_t pi, uint32_t i) { return pd + ((size_t)pi * W + i) * n; });
+        // the inner product reads the shared digits through sigma_k: no permuted copies
+        inner_moddown_src(
+            l, *keys[j], ws, [&](uint32_t pi, uint32_t i) { return digit_tower(l, c1, dig, pi, i); }, ks[j]);
         moddown_tail(l, ws, o0[j], o1[j], a0, nullptr);
     }
 }
@@ -1062,7 +1061,6 @@ std::unique_ptr<Plan> PlanBuilder::layout(std::vector<char>& blob) {
     GF_STAGE(CONV, conv_up_);
     GF_STAGE(NTT_F, ntt_dig_);
     GF_STAGE(NTT_FA, ntt_diga_);
-    GF_STAGE(PERM, perm_dig_);
     GF_STAGE(INNER, inner_);
     for (size_t i = 0; i < ksi_.size();) {
         size_t e = i;

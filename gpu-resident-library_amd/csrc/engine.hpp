@@ -234,11 +234,11 @@ private:
     void ksi_moddown_src(uint32_t l, const uint64_t* d, const DevKey& key, uint64_t* ws);
     const uint64_t* digit_tower(uint32_t l, const uint64_t* d, const uint64_t* dig, uint32_t pi,
                                 uint32_t i) const;
+    // gk != 0: the digits are read through the automorphism of index gk
     template <class F>
-    void inner_moddown_src(uint32_t l, const DevKey& key, uint64_t* ws, F digit);
+    void inner_moddown_src(uint32_t l, const DevKey& key, uint64_t* ws, F digit, uint64_t gk = 0);
     void moddown_tail(uint32_t l, uint64_t* ws, uint64_t* ba0, uint64_t* ba1, const uint64_t* t0,
                       const uint64_t* t1);
-    std::vector<PermJob> perm_dig_;
     std::vector<PtAccJob> ptacc_;
     std::vector<AddJob> add_;
     std::vector<CopyJob> copy_;

@@ -44,7 +44,10 @@ struct ConvJob {
     uint32_t wide;          // 1: some modulus is 2^60 or more (full 128-bit products)
 };
 
-// o0 = sum_p dig[p] * kb[p], o1 = sum_p dig[p] * ka[p] (keys in Montgomery form)
+// o0 = sum_p dig[p] * kb[p], o1 = sum_p dig[p] * ka[p] (keys in Montgomery form);
+// with gk != 0 the digits are read through the evaluation-domain automorphism
+// of index gk (digit p at position i is dig[p][perm_gk(i)]), which is the
+// inner product of a hoisted rotation without a permuted copy of the digits
 struct InnerJob {
     const uint64_t* dig[MAXP];
     const uint64_t* kb[MAXP];
@@ -53,8 +56,9 @@ struct InnerJob {
     uint64_t* o1;
     uint64_t q, qneg;
     uint64_t red;     // lazy_red_of(q): digits of this row may be unreduced (NttJob::lazy_out)
+    uint64_t gk;      // 0, or the automorphism index the digits are read through
     uint32_t np;
-    uint32_t pairs;   // as TensorJob::pairs
+    uint32_t pairs;   // as TensorJob::pairs (unused with gk)
 };
 
 // out = t + (res - conv) * pinv   (ApproxModDown combine fused with r = t + ba)

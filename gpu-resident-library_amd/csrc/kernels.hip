@@ -310,13 +310,36 @@ __device__ __forceinline__ uint64_t inner_out(const U128& T, const InnerJob& j) 
     return redc(T, j.q, j.qneg);
 }
 
+__device__ __forceinline__ uint32_t brv_bits(uint32_t v, uint32_t logn) {
+    return logn ? (__brev(v) >> (32 - logn)) : 0;
+}
+// evaluation-domain automorphism of index k: out[i] = in[perm_index(i)]
+// (slots in bit-reversed order; k odd, below 2n)
+__device__ __forceinline__ uint32_t perm_index(uint32_t i, uint64_t k, uint32_t n, uint32_t logn) {
+    const uint64_t e = ((2ull * brv_bits(i, logn) + 1) * k) & (2ull * n - 1);
+    return brv_bits((uint32_t)((e - 1) >> 1), logn);
+}
+
 __global__ __launch_bounds__(PW_THREADS) void k_inner(const InnerJob* __restrict__ jobs,
                                                       uint32_t off, uint32_t n) {
     const InnerJob& j = jobs[off + blockIdx.y];
     const uint32_t np = j.np;
     uint64_t* o0 = j.o0;
     uint64_t* o1 = j.o1;
-    if (j.pairs) {
+    if (j.gk) {
+        const uint32_t logn = 31 - __builtin_clz(n);
+        PW_FOR(n) {
+            const uint32_t src = perm_index(i, j.gk, n, logn);
+            U128 a0{0, 0}, a1{0, 0};
+            for (uint32_t p = 0; p < np; ++p) {
+                const uint64_t d = ga(j.dig[p], src);
+                a0 = add128(a0, mul128(d, ga(j.kb[p], i)));
+                a1 = add128(a1, mul128(d, ga(j.ka[p], i)));
+            }
+            ga(o0, i) = inner_out(a0, j);
+            ga(o1, i) = inner_out(a1, j);
+        }
+    } else if (j.pairs) {
         PW2_FOR(n) {
             U128 a0[2] = {{0, 0}, {0, 0}}, a1[2] = {{0, 0}, {0, 0}};
             for (uint32_t p = 0; p < np; ++p) {
@@ -483,10 +506,6 @@ __global__ __launch_bounds__(PW_THREADS) void k_add(const AddJob* __restrict__ j
     PW_FOR(n) { ga(j.out, i) = addmod(ga(j.a, i), ga(j.b, i), j.q); }
 }
 
-__device__ __forceinline__ uint32_t brv_bits(uint32_t v, uint32_t logn) {
-    return logn ? (__brev(v) >> (32 - logn)) : 0;
-}
-
 struct PermList {
     const PermJob* p;
     uint32_t off;
@@ -499,11 +518,7 @@ struct PermOne {
 template <class J>
 __global__ __launch_bounds__(PW_THREADS) void k_perm(J jobs, uint32_t n, uint32_t logn) {
     const PermJob j = jobs.get(blockIdx.y);
-    const uint64_t M = 2ull * n;
-    PW_FOR(n) {
-        const uint64_t e = ((2ull * brv_bits(i, logn) + 1) * j.k) & (M - 1);
-        ga(j.out, i) = ga(j.in, brv_bits((uint32_t)((e - 1) >> 1), logn));
-    }
+    PW_FOR(n) { ga(j.out, i) = ga(j.in, perm_index(i, j.k, n, logn)); }
 }
 
 __global__ __launch_bounds__(PW_THREADS) void k_copy(const CopyJob* __restrict__ jobs,

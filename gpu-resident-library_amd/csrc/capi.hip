@@ -145,6 +145,17 @@ gf_key* gf_key_generate(gf_ctx* c, const uint64_t* d_sold, const uint64_t* d_pka
 int gf_key_download(const gf_key* k, uint64_t* h_av, uint64_t* h_bv, hipStream_t s) {
     return guard([&] { download_key(*k->ctx->C, *k->K, h_av, h_bv, s); });
 }
+int gf_encrypt(gf_ctx* c, uint32_t level, const uint64_t* d_pka, const uint64_t* d_pkb, const int64_t* h_m,
+               uint64_t ns, double sigma, uint64_t seed, uint64_t* d_c0, uint64_t* d_c1, hipStream_t s) {
+    return guard([&] {
+        if (!(sigma > 0.0) || sigma > 20.0) throw std::runtime_error("gpufhe: sigma must be in (0, 20]");
+        encrypt_device(*c->C, level, d_pka, d_pkb, h_m, ns, sigma, seed, d_c0, d_c1, s);
+    });
+}
+int gf_decrypt(gf_ctx* c, const uint64_t* d_c0, const uint64_t* d_c1, const uint64_t* d_s, int64_t* h_m,
+               hipStream_t s) {
+    return guard([&] { decrypt_device(*c->C, d_c0, d_c1, d_s, h_m, s); });
+}
 void gf_key_shape(const gf_key* k, uint32_t* nparts, uint32_t* rows, uint32_t* fullQ) {
     *nparts = k->K->nparts;
     *rows = k->K->rows;

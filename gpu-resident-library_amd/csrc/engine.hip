@@ -434,6 +434,92 @@ std::unique_ptr<DevKey> generate_key(const Ctx& C, const uint64_t* sold, const u
     return k;
 }
 
+// Encryption on the device (src/encdec.cpp:41-64): v (ternary), e0, e1
+// (Gaussian) drawn on the host in the reference's order; the message and the
+// three small polynomials embedded into the `level` Q towers and transformed on
+// the device; then c0 = pkB v + ns e0 + m, c1 = pkA v + ns e1 by the
+// switching-key kernel in plain form (sold = m, P mod q replaced by 1).
+void encrypt_device(const Ctx& C, uint32_t level, const uint64_t* pka, const uint64_t* pkb, const int64_t* h_m,
+                    uint64_t ns, double sigma, uint64_t seed, uint64_t* c0, uint64_t* c1, hipStream_t s) {
+    if (level < 1 || level > C.L) throw std::runtime_error("gpufhe: bad level");
+    const uint32_t n = C.n;
+    std::vector<int64_t> v, e0, e1;
+    sample_encrypt(n, sigma, seed, v, e0, e1);
+    // coefficients [m, v, e0, e1][n]; their towers [4][level][n]
+    std::vector<int64_t> hc((size_t)4 * n);
+    std::copy(h_m, h_m + n, hc.begin());
+    std::copy(v.begin(), v.end(), hc.begin() + n);
+    std::copy(e0.begin(), e0.end(), hc.begin() + 2 * (size_t)n);
+    std::copy(e1.begin(), e1.end(), hc.begin() + 3 * (size_t)n);
+    DevBuf coef(hc.size() * 8), tow((size_t)4 * level * n * 8);
+    hip_check(hipMemcpyAsync(coef.p, hc.data(), coef.bytes, hipMemcpyHostToDevice, s), "upload samples");
+    const int64_t* cb = coef.as<int64_t>();
+    uint64_t* tb = tow.as<uint64_t>();
+    auto tower = [&](uint32_t k, uint32_t t) { return tb + ((size_t)k * level + t) * n; };
+    std::vector<EmbedJob> ej;
+    std::vector<NttJob> nj;
+    std::vector<EkJob> kj;
+    for (uint32_t k = 0; k < 4; ++k)
+        for (uint32_t t = 0; t < level; ++t) {
+            ej.push_back(EmbedJob{cb + (size_t)k * n, tower(k, t), C.mods[t]});
+            NttJob J{tower(k, t), tower(k, t), C.rt[t].fw2, nullptr, C.mods[t], 0, 0};
+            J.twr = C.rtr[t].fwd;
+            J.red = lazy_red_of(C.mods[t]);
+            nj.push_back(J);
+        }
+    for (uint32_t t = 0; t < level; ++t) {
+        const uint64_t q = C.mods[t], nsq = ns % q;
+        const size_t o = (size_t)t * n;
+        EkJob J{};
+        J.pka = pka + o;
+        J.pkb = pkb + o;
+        J.u = tower(1, t);
+        J.e0 = tower(2, t);
+        J.e1 = tower(3, t);
+        J.sold = tower(0, t);
+        J.a_out = c1 + o;
+        J.b_out = c0 + o;
+        J.q = q;
+        J.qneg = C.rc[t].qneg;
+        J.r2 = C.rc[t].r2;
+        J.ns = nsq;
+        J.nsp = host_precon(nsq, q);
+        J.pq = 1 % q;
+        J.pqp = host_precon(J.pq, q);
+        J.plain = 1;
+        kj.push_back(J);
+    }
+    DevBuf dej(ej.size() * sizeof(EmbedJob)), dnj(nj.size() * sizeof(NttJob)), dkj(kj.size() * sizeof(EkJob));
+    hip_check(hipMemcpyAsync(dej.p, ej.data(), dej.bytes, hipMemcpyHostToDevice, s), "upload jobs");
+    hip_check(hipMemcpyAsync(dnj.p, nj.data(), dnj.bytes, hipMemcpyHostToDevice, s), "upload jobs");
+    hip_check(hipMemcpyAsync(dkj.p, kj.data(), dkj.bytes, hipMemcpyHostToDevice, s), "upload jobs");
+    launch_embed(dej.as<EmbedJob>(), (uint32_t)ej.size(), n, s);
+    launch_ntt(dnj.as<NttJob>(), (uint32_t)nj.size(), C.geom, true, s);
+    launch_ekgen(dkj.as<EkJob>(), (uint32_t)kj.size(), n, s);
+    hip_check(hipGetLastError(), "encryption launch");
+    hip_check(hipStreamSynchronize(s), "encryption");   // before the staging buffers are freed
+}
+
+// Decryption (src/encdec.cpp:68-81): tower 0 only, INTT(c0 + c1 s) with n^-1,
+// centred on the host
+void decrypt_device(const Ctx& C, const uint64_t* c0, const uint64_t* c1, const uint64_t* sk, int64_t* h_m,
+                    hipStream_t s) {
+    const uint32_t n = C.n;
+    DevBuf ph((size_t)n * 8), dj(sizeof(DecJob)), dn(sizeof(NttJob));
+    const DecJob J{c0, c1, sk, ph.as<uint64_t>(), C.mods[0], C.rc[0].qneg, C.rc[0].r2};
+    NttJob T{ph.as<uint64_t>(), ph.as<uint64_t>(), C.rt[0].iv2, nullptr, C.mods[0], C.rt[0].ninv, C.rt[0].ninvp};
+    T.twr = C.rtr[0].inv;
+    hip_check(hipMemcpyAsync(dj.p, &J, sizeof J, hipMemcpyHostToDevice, s), "upload jobs");
+    hip_check(hipMemcpyAsync(dn.p, &T, sizeof T, hipMemcpyHostToDevice, s), "upload jobs");
+    launch_dec(dj.as<DecJob>(), 1, n, s);
+    launch_ntt(dn.as<NttJob>(), 1, C.geom, false, s);
+    std::vector<uint64_t> h(n);
+    hip_check(hipMemcpyAsync(h.data(), ph.p, (size_t)n * 8, hipMemcpyDeviceToHost, s), "download");
+    hip_check(hipStreamSynchronize(s), "decryption");
+    const uint64_t q = C.mods[0];
+    for (uint32_t k = 0; k < n; ++k) h_m[k] = h[k] > q / 2 ? (int64_t)h[k] - (int64_t)q : (int64_t)h[k];
+}
+
 // the key's rows back in plain form (x R^-1 mod q), [nparts][rows][n] on the host
 void download_key(const Ctx& C, const DevKey& k, uint64_t* av, uint64_t* bv, hipStream_t s) {
     const uint32_t n = C.n;

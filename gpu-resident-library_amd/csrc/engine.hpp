@@ -120,6 +120,14 @@ std::unique_ptr<DevKey> upload_key(const Ctx& C, const uint64_t* av, const uint6
 std::unique_ptr<DevKey> generate_key(const Ctx& C, const uint64_t* sold, const uint64_t* pka, const uint64_t* pkb,
                                      uint64_t ns, double sigma, uint64_t seed, hipStream_t s);
 void download_key(const Ctx& C, const DevKey& k, uint64_t* av, uint64_t* bv, hipStream_t s);
+// encryption of the coefficient vector h_m (n signed values) under the public
+// key (pka, pkb: device [level][n], evaluation form) into device c0, c1
+void encrypt_device(const Ctx& C, uint32_t level, const uint64_t* pka, const uint64_t* pkb, const int64_t* h_m,
+                    uint64_t ns, double sigma, uint64_t seed, uint64_t* c0, uint64_t* c1, hipStream_t s);
+// decryption of (c0, c1) under the secret s (device towers, evaluation form;
+// tower 0 is used) into n centred coefficients on the host
+void decrypt_device(const Ctx& C, const uint64_t* c0, const uint64_t* c1, const uint64_t* sk, int64_t* h_m,
+                    hipStream_t s);
 
 // One launch step of a plan
 struct Step {

@@ -148,6 +148,17 @@ struct EkJob {
     const uint64_t *pka, *pkb, *u, *e0, *e1, *sold;
     uint64_t *a_out, *b_out;
     uint64_t q, qneg, r2, ns, nsp, pq, pqp;
+    // 1: outputs in plain form -- an encryption (src/encdec.cpp:41-64): u = v,
+    // sold = the message, pq = 1, so b_out = c0 and a_out = c1
+    uint32_t plain;
+};
+
+// out = c0 + c1 s (mod q): the phase of a decryption before its inverse
+// transform (src/encdec.cpp:68-81)
+struct DecJob {
+    const uint64_t *c0, *c1, *s;
+    uint64_t* out;
+    uint64_t q, qneg, r2;
 };
 
 // plaintext-weighted sum of m ciphertext towers (linear transforms):
@@ -209,6 +220,7 @@ void launch_drop(const DropJob* d, uint32_t nj, uint32_t n, hipStream_t s);
 void launch_rs(const RsJob* d, uint32_t nj, uint32_t n, hipStream_t s);
 void launch_embed(const EmbedJob* d, uint32_t nj, uint32_t n, hipStream_t s);
 void launch_ekgen(const EkJob* d, uint32_t nj, uint32_t n, hipStream_t s);
+void launch_dec(const DecJob* d, uint32_t nj, uint32_t n, hipStream_t s);
 void launch_ptacc(const PtAccJob* d, uint32_t nj, uint32_t n, hipStream_t s);
 // fused digit row pass + key inner product: regions = LDS regions (at least every
 // job's transformed-part count), waves = workgroup size in waves (1..KSI_WAVES)

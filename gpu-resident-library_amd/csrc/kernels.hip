@@ -476,10 +476,20 @@ __global__ __launch_bounds__(PW_THREADS) void k_ekgen(const EkJob* __restrict__ 
         uint64_t a = addmod(mulmod(j.pka[i], u, j.q, j.qneg, j.r2), shoup(j.e1[i], j.ns, j.nsp, j.q), j.q);
         uint64_t b = addmod(mulmod(j.pkb[i], u, j.q, j.qneg, j.r2), shoup(j.e0[i], j.ns, j.nsp, j.q), j.q);
         if (j.sold) b = addmod(b, shoup(j.sold[i], j.pq, j.pqp, j.q), j.q);
-        // Montgomery form: REDC(x R^2) = x R
-        j.a_out[i] = redc(mul128(a, j.r2), j.q, j.qneg);
-        j.b_out[i] = redc(mul128(b, j.r2), j.q, j.qneg);
+        if (j.plain) {
+            j.a_out[i] = a;
+            j.b_out[i] = b;
+        } else {
+            // Montgomery form: REDC(x R^2) = x R
+            j.a_out[i] = redc(mul128(a, j.r2), j.q, j.qneg);
+            j.b_out[i] = redc(mul128(b, j.r2), j.q, j.qneg);
+        }
     }
+}
+
+__global__ __launch_bounds__(PW_THREADS) void k_dec(const DecJob* __restrict__ jobs, uint32_t off, uint32_t n) {
+    const DecJob j = jobs[off + blockIdx.y];
+    PW_FOR(n) { ga(j.out, i) = addmod(ga(j.c0, i), mulmod(ga(j.c1, i), ga(j.s, i), j.q, j.qneg, j.r2), j.q); }
 }
 
 __global__ __launch_bounds__(PW_THREADS) void k_ptacc(const PtAccJob* __restrict__ jobs,
@@ -564,6 +574,7 @@ GF_PW_LAUNCH(launch_drop, k_drop, DropJob)
 GF_PW_LAUNCH(launch_rs, k_rs, RsJob)
 GF_PW_LAUNCH(launch_embed, k_embed, EmbedJob)
 GF_PW_LAUNCH(launch_ekgen, k_ekgen, EkJob)
+GF_PW_LAUNCH(launch_dec, k_dec, DecJob)
 GF_PW_LAUNCH(launch_ptacc, k_ptacc, PtAccJob)
 
 void launch_fill(uint64_t* d, const uint64_t* d_mods, uint32_t towers, uint32_t n, uint64_t seed,

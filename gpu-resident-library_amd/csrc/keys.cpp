@@ -178,14 +178,8 @@ void encrypt_host(std::vector<uint64_t>& c0, std::vector<uint64_t>& c1, const st
     const size_t T = (size_t)L * n;
     if (m.size() < n || pkA.size() < T || pkB.size() < T || root.size() < L)
         throw std::runtime_error("encrypt_host: inputs shorter than towers x n");
-    std::mt19937_64 g(seed);
-    const Gauss gauss(sigma);
-    std::vector<int64_t> vc(n), e0c(n), e1c(n);
-    for (uint32_t k = 0; k < n; ++k) {
-        vc[k] = (int64_t)(g() % 3) - 1;
-        e0c[k] = gauss(g);
-        e1c[k] = gauss(g);
-    }
+    std::vector<int64_t> vc, e0c, e1c;
+    gf::sample_encrypt(n, sigma, seed, vc, e0c, e1c);
     const std::vector<uint64_t> v = embed_eval(vc, L, n, mod.data(), root.data());
     const std::vector<uint64_t> e0 = embed_eval(e0c, L, n, mod.data(), root.data());
     const std::vector<uint64_t> e1 = embed_eval(e1c, L, n, mod.data(), root.data());

@@ -63,4 +63,11 @@ inline void sample_switch_key(uint32_t n, uint32_t parts, double sigma, uint64_t
         }
 }
 
+// the draws of one encryption (src/encdec.cpp:47-53): for each coefficient
+// v (ternary), e0, e1 (Gaussian)
+inline void sample_encrypt(uint32_t n, double sigma, uint64_t seed, std::vector<int64_t>& v,
+                           std::vector<int64_t>& e0, std::vector<int64_t>& e1) {
+    sample_switch_key(n, 1, sigma, seed, v, e0, e1);
+}
+
 }  // namespace gf

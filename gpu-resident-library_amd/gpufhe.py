@@ -51,6 +51,8 @@ _capi = {
     "gf_key_create_uniform": (vp, [vp, u32, u64, vp]),
     "gf_key_generate": (vp, [vp, vp, vp, vp, u64, C.c_double, u64, vp]),
     "gf_key_download": (C.c_int, [vp, u64p, u64p, vp]),
+    "gf_encrypt": (C.c_int, [vp, u32, vp, vp, vp, u64, C.c_double, u64, vp, vp, vp]),
+    "gf_decrypt": (C.c_int, [vp, vp, vp, vp, vp, vp]),
     "gf_key_shape": (None, [vp, u32p, u32p, u32p]),
     "gf_ct_packed_bytes": (sz, [u32, u32]),
     "gf_ct_pack": (C.c_int, [u64p, u64p, u32, u32, u64p, C.c_double, vp, sz]),

@@ -12,8 +12,9 @@
  * Conventions: pointers named d_* / c* / o* / ws are device pointers; host
  * arrays are named h_*.  Device arrays need 8-byte alignment; when every array
  * of a stage is 16-byte aligned (as gf_malloc returns) the streaming kernels
- * move two residues per access, which is faster, with identical results.  Functions returning int give 0 on success and -1 on
- * error (message in gf_last_error()); constructors return NULL on error.
+ * move two residues per access, which is faster, with identical results.
+ * Functions returning int give 0 on success and -1 on error (message in
+ * gf_last_error()); constructors return NULL on error.
  * Operations are asynchronous on the given stream; the caller synchronises.
  */
 #ifndef GPUFHE_CAPI_H
@@ -94,6 +95,20 @@ gf_key* gf_key_generate(gf_ctx* c, const uint64_t* d_sold, const uint64_t* d_pka
                         uint64_t ns, double sigma, uint64_t seed, hipStream_t s);
 /* rows of a resident key in plain form, h_av/h_bv [nparts][rows][n] */
 int gf_key_download(const gf_key* k, uint64_t* h_av, uint64_t* h_bv, hipStream_t s);
+/* Encryption into resident buffers (encrypt_host of src/encdec.cpp:41-64 with
+ * the algebra on the device): c0 = pkB v + ns e0 + m, c1 = pkA v + ns e1 over
+ * the first `level` Q towers.  h_m: n signed message coefficients on the host;
+ * d_pka, d_pkb: device [level][n], evaluation form; d_c0, d_c1: device
+ * [level][n].  v, e0, e1 are drawn from mt19937_64(seed) in the reference's
+ * order with the library's Gaussian (sigma), so the result equals
+ * encrypt_host's for the same inputs.  Returns after the device work. */
+int gf_encrypt(gf_ctx* c, uint32_t level, const uint64_t* d_pka, const uint64_t* d_pkb, const int64_t* h_m,
+               uint64_t ns, double sigma, uint64_t seed, uint64_t* d_c0, uint64_t* d_c1, hipStream_t s);
+/* Decryption from resident buffers (decrypt_host of src/encdec.cpp:68-81):
+ * tower 0 of c0 + c1 s, inverse-transformed and centred into h_m (n values on
+ * the host).  d_s: the secret in evaluation form (at least tower 0). */
+int gf_decrypt(gf_ctx* c, const uint64_t* d_c0, const uint64_t* d_c1, const uint64_t* d_s, int64_t* h_m,
+               hipStream_t s);
 void gf_key_shape(const gf_key* k, uint32_t* nparts, uint32_t* rows, uint32_t* fullQ);
 
 /* ---- tower utilities ----

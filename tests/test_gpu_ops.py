@@ -551,6 +551,34 @@ def test_device_keygen_matches_oracle(G, n, L, dnum):
     assert np.array_equal(ra, wa) and np.array_equal(rb, wb)
 
 
+@pytest.mark.parametrize("n,L,dnum", [(1 << 12, 3, 3), (1 << 13, 6, 3), (1 << 16, 24, 4)])
+def test_device_encrypt_decrypt_match_oracle(G, n, L, dnum):
+    """gf_encrypt equals the oracle's encryption (same sampler and seed) bit
+    for bit at the top level and at level 1, gf_decrypt equals the oracle's
+    decryption, and a fresh ciphertext decrypts to its message up to the
+    encryption noise."""
+    P = make_params(n, L, dnum)
+    ks = KeySet(P)
+    ctx = G.Context(n, P.q, P.p, P.roots, dnum)
+    rng = np.random.default_rng(n + L)
+    m = np.ascontiguousarray(rng.integers(-(1 << 20), 1 << 20, n).astype(np.int64))
+    dpa, dpb, dsk = G.Dev.of(ks.pkA), G.Dev.of(ks.pkB), G.Dev.of(ks.sQ)
+    s = G.Stream()
+    for level in sorted({L, 1}):
+        seed = 303 + level
+        c0, c1 = G.Dev(level * n), G.Dev(level * n)
+        G.check(G.lib.gf_encrypt(ctx.h, level, dpa.ptr, dpb.ptr, m.ctypes.data, 1, 3.19, seed, c0.ptr, c1.ptr,
+                                 s.s))
+        w0, w1 = O.encrypt(n, P.q[:level], P.roots[:level], m, ks.pkA[:level * n], ks.pkB[:level * n], seed=seed)
+        assert np.array_equal(c0.download(), w0) and np.array_equal(c1.download(), w1), level
+        out = np.zeros(n, np.int64)
+        G.check(G.lib.gf_decrypt(ctx.h, c0.ptr, c1.ptr, dsk.ptr, out.ctypes.data, s.s))
+        want = O.decrypt(n, int(P.q[0]), int(P.roots[0]), w0[:n], w1[:n], ks.sQ[:n])
+        assert np.array_equal(out, want), level
+        # encryption noise v e + e1 s + e0: standard deviation about sigma sqrt(4n/3)
+        assert int(np.abs(out - m).max()) < 8 * 3.19 * (4 * n / 3) ** 0.5 + 50, level
+
+
 def test_chain_plans_graphs(G):
     """Multiply-rescale chain (tests/test_chain.cpp shape): a batch of
     ciphertexts multiplied depth times, one plan per step over the batch,

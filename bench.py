@@ -464,7 +464,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs no collectives or multi-GPU work "
-              "(ciphertext batches shard across GPUs with no exchange; see DESIGN.md)", file=sys.stderr)
+              "(ciphertext batches shard across GPUs with no exchange between them)", file=sys.stderr)
         sys.exit(2)
 
     import gpufhe as G

@@ -9,7 +9,7 @@ from common import KeySet, decode, encode, make_params, mulmod_vec, poly_mul_neg
 
 pytestmark = pytest.mark.gpu
 
-# (n, L, dnum): small ring, two-phase NTT ring, the BASELINE config shapes
+# (n, L, dnum): small ring, two-phase NTT ring, the benchmark configurations' shapes
 SHAPES = [(1 << 10, 4, 2), (1 << 13, 6, 3), (1 << 15, 4, 2), (1 << 15, 12, 3), (1 << 16, 24, 4)]
 # edge shapes: one digit (alpha = L), one tower per digit (alpha = 1), the
 # largest ring the transforms support, the smallest ring; at a two-phase ring

@@ -7,8 +7,9 @@ on one MI355X.
 A step is one batched EvalMult+Rescale over B independent ciphertext pairs,
 all resident in HBM, replayed as one hipGraph.  The same kernels launched one
 operation at a time (no graph) give the per-op rate.  The CPU baseline is the
-oracle's single-threaded EvalMult+Rescale on the same workload (bounded
-sample).  Prints one JSON line.
+oracle's EvalMult+Rescale on the same workload, one operation per thread on
+the host's allowed threads and on one thread (bounded samples).  Prints one
+JSON line.
 
 Batches 16 and 256 are measured beside B = 64 (batch_sweep_ops_s).
 

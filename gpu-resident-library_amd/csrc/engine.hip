@@ -610,7 +610,28 @@ void Plan::run_graph(hipStream_t s) {
 }
 
 // --------------------------------------------------------------- builder --
-static const uint32_t CONV_CHUNK = 8;
+// Targets per conversion job: a job loads and splits its source column once
+// for all of its targets, so large chunks save work; a stage with few jobs
+// is split back into small chunks at layout (split_chunks) to fill the chip.
+static const uint32_t CONV_CHUNK = 32, CONV_CHUNK_SMALL = 8;
+static const uint32_t FILL_WORKGROUPS = 8192;   // 32 per CU
+
+// Re-chunk conversion jobs ([j0, j1) target ranges) into CONV_CHUNK_SMALL
+// pieces when the stage would launch fewer than FILL_WORKGROUPS workgroups.
+template <class J>
+static void split_chunks(std::vector<J>& v, uint32_t n) {
+    const size_t blocks = (n + PW_THREADS * PW_EPT - 1) / (PW_THREADS * PW_EPT);
+    if (v.empty() || v.size() * blocks >= FILL_WORKGROUPS) return;
+    std::vector<J> out;
+    for (const J& j : v)
+        for (uint32_t a = j.j0; a < j.j1; a += CONV_CHUNK_SMALL) {
+            J k = j;
+            k.j0 = a;
+            k.j1 = std::min(j.j1, a + CONV_CHUNK_SMALL);
+            out.push_back(k);
+        }
+    v.swap(out);
+}
 
 // Hybrid keyswitch up to the ModDown sources (src/keyswitch_resident.cpp:139-201):
 // ModUp of every digit, the key inner product over Q_l P, and the INTT of both
@@ -1126,6 +1147,9 @@ std::unique_ptr<Plan> PlanBuilder::layout(std::vector<char>& blob) {
     group_by(ksi_, [](const KsiJob& j) { return (uintptr_t)j.kb[0]; });
     group_by(ksi_, nxf);
     group_by(inner_, [](const InnerJob& j) { return (uintptr_t)j.kb[0]; });
+    split_chunks(conv_up_, C_.n);
+    split_chunks(conv_dn_, C_.n);
+    split_chunks(rs_, C_.n);
     group_by(conv_up_, [](const ConvJob& j) { return (uintptr_t)j.tab; });
     group_by(tensor_, [](const TensorJob& j) { return j.q; });
     auto P = std::make_unique<Plan>();

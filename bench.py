@@ -54,7 +54,7 @@ def native_params(G, n, L, dnum):
 STEP_KERNELS = {"ntt_fwd": ("k_ntt_A", "k_ntt_B"), "ntt_inv": ("k_ntt_B", "k_ntt_A"),
                 "tensor": ("k_tensor",), "conv": ("k_conv",), "inner": ("k_inner",),
                 "combine": ("k_combine",), "switch": ("k_switch",), "fuse": ("k_fuse",),
-                "add": ("k_add",), "perm": ("k_perm",), "copy": ("k_copy",), "drop": ("k_drop",),
+                "add": ("k_add",), "perm": ("k_perm",), "copy": ("k_copy",),
                 "rsv": ("k_rs",), "ptacc": ("k_ptacc",), "ntt_fwd_a": ("k_ntt_A",), "ksi": ("k_ksi",)}
 
 
@@ -81,8 +81,8 @@ def step_bytes(n, L, K, alpha, np_parts, kinds, batch):
     sc = [L - min(alpha, L - alpha * p) + K for p in range(np_parts)]
     sp = [min(alpha, L - alpha * p) for p in range(np_parts)]
     if "ksi" in kinds:
-        assert kinds.count("ksi") == 2 and len(kinds) == 10, kinds
-    if "ksi" in kinds or len(kinds) == 9:   # fused ModDown + rescale plan
+        assert kinds.count("ksi") == 2 and len(kinds) == 9, kinds
+    if "ksi" in kinds or len(kinds) == 8:   # fused ModDown + rescale plan
         if "ksi" in kinds:
             # the digits' column pass, then their row pass fused with the key
             # inner product, one step for the Q rows (np - 1 transformed parts,
@@ -102,8 +102,9 @@ def step_bytes(n, L, K, alpha, np_parts, kinds, batch):
             ("conv_modup", T * (sum(sp) + sum(sc)), 0),
         ] + digits + [
             ("intt_moddown", 2 * T * (2 * K + 2), 0),       # + INTT of res0/res1's last tower
-            ("drop_tower", T * 2 * (K + 3), 0),
-            ("moddown_rescale_prologue", T * 2 * (K + 1 + (L - 1)), 0),
+            # per component: the K P-part sources and the dropped tower's two
+            # coefficient-form operands in, the L - 1 surviving towers out
+            ("moddown_rescale_prologue", T * 2 * (K + 2 + (L - 1)), 0),
             # transform + the t and res operands its epilogue reads
             ("ntt_rescale_epilogue", 4 * T * 2 * (L - 1), 2 * T * 2 * (L - 1)),
         ]

@@ -583,7 +583,6 @@ void Plan::run_jobs(size_t i, hipStream_t s, uint32_t which, uint32_t first, uin
         case Step::ADD: launch_add(GF_J(AddJob), cnt, n, s); break;
         case Step::PERM: launch_perm(GF_J(PermJob), cnt, n, logn, s); break;
         case Step::COPY: launch_copy(GF_J(CopyJob), cnt, n, s); break;
-        case Step::DROP: launch_drop(GF_J(DropJob), cnt, n, s); break;
         case Step::RSV: launch_rs(GF_J(RsJob), cnt, n, s); break;
         case Step::PTACC: launch_ptacc(GF_J(PtAccJob), cnt, n, s); break;
         case Step::NTT_FA: launch_ntt(GF_J(NttJob), cnt, geom, true, s, 1); break;
@@ -851,20 +850,34 @@ void PlanBuilder::evalmult_rescale_fused(uint32_t l, const uint64_t* c0a, const 
     keyswitch_core(l, t[2], key, ws, t[0] + (size_t)last * n, t[1] + (size_t)last * n, dt, dt + n);
     uint64_t* res[2] = {ws + w.res0 * n, ws + w.res1 * n};
     uint64_t* xp = ws + w.xp * n;
-    uint64_t* drop = ws + w.drop * n;
     uint64_t* v = ws + w.dig * n;     // digits are dead after the inner product
     uint64_t* outs[2] = {o0, o1};
     for (int c = 0; c < 2; ++c) {
         intt_p_.push_back(NttJob{res[c] + (size_t)last * n, dr + (size_t)c * n, C.rt[last].iv2, nullptr,
                                  C.mods[last], C.rt[last].ninv, C.rt[last].ninvp});
-        drop_.push_back(DropJob{xp + (size_t)c * K * n, lv.d_tabMD + last, dt + (size_t)c * n, dr + (size_t)c * n,
-                                drop + (size_t)c * n, C.rc[last].q, C.rc[last].qneg, lv.pinv[last],
-                                lv.pinvp[last], K, l, C.wide});
         uint64_t* vc = v + (size_t)c * last * n;
-        for (uint32_t j0 = 0; j0 < last; j0 += CONV_CHUNK)
-            rs_.push_back(RsJob{xp + (size_t)c * K * n, drop + (size_t)c * n, vc, lv.d_tabRS, lv.d_rstg,
-                                C.mods[last], C.mods[last] >> 1, K, last, j0,
-                                j0 + CONV_CHUNK < last ? j0 + CONV_CHUNK : last, C.wide});
+        for (uint32_t j0 = 0; j0 < last; j0 += CONV_CHUNK) {
+            RsJob R{};
+            R.xp = xp + (size_t)c * K * n;
+            R.dt = dt + (size_t)c * n;
+            R.dr = dr + (size_t)c * n;
+            R.out = vc;
+            R.tab = lv.d_tabRS;
+            R.tg = lv.d_rstg;
+            R.tabd = lv.d_tabMD + last;   // PHatModq column of q_last, row stride l
+            R.q_old = C.mods[last];
+            R.half = C.mods[last] >> 1;
+            R.qd_neg = C.rc[last].qneg;
+            R.pinv = lv.pinv[last];
+            R.pinvp = lv.pinvp[last];
+            R.K = K;
+            R.ntgt = last;
+            R.j0 = j0;
+            R.j1 = j0 + CONV_CHUNK < last ? j0 + CONV_CHUNK : last;
+            R.strided = l;
+            R.wide = C.wide;
+            rs_.push_back(R);
+        }
         for (uint32_t i = 0; i < last; ++i) {
             NttJob J{vc + (size_t)i * n, outs[c] + (size_t)i * n, C.rt[i].fw2, nullptr, C.mods[i], 0, 0};
             J.e0 = t[c] + (size_t)i * n;
@@ -1190,7 +1203,6 @@ std::unique_ptr<Plan> PlanBuilder::layout(std::vector<char>& blob) {
     GF_STAGE(SWITCH, switch_);
     GF_STAGE(NTT_F, ntt_sw_);
     GF_STAGE(FUSE, fuse_);
-    GF_STAGE(DROP, drop_);
     GF_STAGE(RSV, rs_);
     GF_STAGE(NTT_F, ntt_v_);
     GF_STAGE(ADD, add_);

@@ -133,7 +133,7 @@ void decrypt_device(const Ctx& C, const uint64_t* c0, const uint64_t* c1, const 
 struct Step {
     // NTT_FA: a forward transform's first (column) pass only; KSI: the fused
     // row pass of the ModUp digits and the key inner product
-    enum Kind { NTT_F, NTT_I, TENSOR, CONV, INNER, COMBINE, SWITCH, FUSE, ADD, PERM, COPY, DROP, RSV, PTACC,
+    enum Kind { NTT_F, NTT_I, TENSOR, CONV, INNER, COMBINE, SWITCH, FUSE, ADD, PERM, COPY, RSV, PTACC,
                 NTT_FA, KSI } kind;
     size_t off;     // byte offset of the first job in the plan's device buffer
     uint32_t count;
@@ -226,7 +226,6 @@ private:
     std::vector<SwitchJob> switch_;
     std::vector<NttJob> ntt_sw_;
     std::vector<FuseJob> fuse_;
-    std::vector<DropJob> drop_;
     std::vector<RsJob> rs_;
     std::vector<NttJob> ntt_v_;
     void evalmult_rescale_fused(uint32_t l, const uint64_t* c0a, const uint64_t* c1a,

@@ -102,36 +102,29 @@ struct PermJob {
     uint64_t k;
 };
 
-// Dropped tower of r = t + ba, built in coefficient form (q = q_last):
-//   out = dt + (dr - REDC(sum_k xp[k] * tab[k*stride])) * pinv
-// dt = INTT(t_last), dr = INTT(res_last), xp = INTT'd P towers scaled by n^-1 PHatInv.
-struct DropJob {
-    const uint64_t* xp;
-    const uint64_t* tab;
-    const uint64_t* dt;
-    const uint64_t* dr;
-    uint64_t* out;
-    uint64_t q, qneg, pinv, pinvp;
-    uint32_t K, stride;
-    uint32_t wide;          // as ConvJob::wide
-};
-
 // per surviving tower constants of the fused ModDown + rescale prologue
 struct RsTarget {
     uint64_t q, qneg, diff, new_larger, s2, s2p;
     uint64_t red;   // floor(2^64 / q): Shoup reduction x mod q = shoup(x, 1, red, q)
 };
-// v_i = s2_i * switch(drop -> q_i) + REDC(sum_k xp[k] * tab[k*ntgt + i]) for i in [j0, j1);
-// tab folds -P^-1 s1 into PHatModq, so NTT(v_i) + t_i s1 + res_i P^-1 s1 is the
-// rescaled output tower.
+// Fused ModDown + rescale prologue of one component.  First the dropped tower
+// of r = t + ba in coefficient form (q_old = q_last):
+//   drop = dt + (dr - REDC(sum_k xp[k] * tabd[k*strided])) * pinv
+// with dt = INTT(t_last), dr = INTT(res_last), xp = INTT'd P towers scaled by
+// n^-1 PHatInv; then for the surviving towers i in [j0, j1)
+//   v_i = s2_i * switch(drop -> q_i) + REDC(sum_k xp[k] * tab[k*ntgt + i]);
+// tab folds -P^-1 s1 into PHatModq, so NTT(v_i) + t_i s1 + res_i P^-1 s1 is
+// the rescaled output tower.
 struct RsJob {
     const uint64_t* xp;
-    const uint64_t* drop;
+    const uint64_t* dt;
+    const uint64_t* dr;
     uint64_t* out;
     const uint64_t* tab;
     const RsTarget* tg;
-    uint64_t q_old, half;
-    uint32_t K, ntgt, j0, j1;
+    const uint64_t* tabd;   // ModDown conversion column of the dropped tower
+    uint64_t q_old, half, qd_neg, pinv, pinvp;
+    uint32_t K, ntgt, j0, j1, strided;
     uint32_t wide;          // as ConvJob::wide
 };
 
@@ -216,7 +209,6 @@ void launch_perm_one(const PermJob& j, uint32_t n, uint32_t logn, hipStream_t s)
 void launch_copy(const CopyJob* d, uint32_t nj, uint32_t n, hipStream_t s);
 void launch_fill(uint64_t* d, const uint64_t* d_mods, uint32_t towers, uint32_t n, uint64_t seed,
                  hipStream_t s);
-void launch_drop(const DropJob* d, uint32_t nj, uint32_t n, hipStream_t s);
 void launch_rs(const RsJob* d, uint32_t nj, uint32_t n, hipStream_t s);
 void launch_embed(const EmbedJob* d, uint32_t nj, uint32_t n, hipStream_t s);
 void launch_ekgen(const EkJob* d, uint32_t nj, uint32_t n, hipStream_t s);

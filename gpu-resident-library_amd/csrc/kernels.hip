@@ -398,27 +398,14 @@ __global__ __launch_bounds__(PW_THREADS) void k_fuse(const FuseJob* __restrict__
     }
 }
 
-__global__ __launch_bounds__(PW_THREADS) void k_drop(const DropJob* __restrict__ jobs,
-                                                     uint32_t off, uint32_t n) {
-    const DropJob j = jobs[off + blockIdx.y];
-    const cu64p tab = uni(j.tab);
-    with_nsrc(j.wide ? 0 : j.K, [&](auto Kc) {
-        constexpr uint32_t K = decltype(Kc)::value;
-        PW_FOR(n) {
-            SrcCols<K> x;
-            x.load(j.xp, n, i);
-            const U128 acc = dot_src<K>(j.xp, n, i, j.K, tab, j.stride, 0, x);
-            const uint64_t conv = redc(acc, j.q, j.qneg);
-            ga(j.out, i) = addmod(ga(j.dt, i), shoup(submod(ga(j.dr, i), conv, j.q), j.pinv, j.pinvp, j.q), j.q);
-        }
-    });
-}
-
 template <uint32_t K>
 __device__ __forceinline__ void rs_one(const RsJob& j, uint32_t n, uint32_t i) {
     SrcCols<K> x;
     x.load(j.xp, n, i);
-    const uint64_t dv = ga(j.drop, i);
+    // the dropped tower, from the sources already in registers
+    const uint64_t cd = redc(dot_src<K>(j.xp, n, i, j.K, uni(j.tabd), j.strided, 0, x), j.q_old, j.qd_neg);
+    const uint64_t dv =
+        addmod(ga(j.dt, i), shoup(submod(ga(j.dr, i), cd, j.q_old), j.pinv, j.pinvp, j.q_old), j.q_old);
     const bool hi = dv > j.half;
     const cu64p tab = uni(j.tab);
     static_assert(sizeof(RsTarget) == 7 * 8, "RsTarget layout");
@@ -570,7 +557,6 @@ GF_PW_LAUNCH(launch_switch, k_switch, SwitchJob)
 GF_PW_LAUNCH(launch_fuse, k_fuse, FuseJob)
 GF_PW_LAUNCH(launch_add, k_add, AddJob)
 GF_PW_LAUNCH(launch_copy, k_copy, CopyJob)
-GF_PW_LAUNCH(launch_drop, k_drop, DropJob)
 GF_PW_LAUNCH(launch_rs, k_rs, RsJob)
 GF_PW_LAUNCH(launch_embed, k_embed, EmbedJob)
 GF_PW_LAUNCH(launch_ekgen, k_ekgen, EkJob)

@@ -318,7 +318,7 @@ class Plan:
 
 
 STEP_KINDS = ["ntt_fwd", "ntt_inv", "tensor", "conv", "inner", "combine", "switch", "fuse", "add",
-              "perm", "copy", "drop", "rsv", "ptacc", "ntt_fwd_a", "ksi"]
+              "perm", "copy", "rsv", "ptacc", "ntt_fwd_a", "ksi"]
 
 
 class Event:

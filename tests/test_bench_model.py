@@ -11,15 +11,14 @@ import bench  # noqa: E402
 
 N, L, K, ALPHA, NP, B = 1 << 16, 24, 6, 6, 4, 64
 T = 8 * N                                   # one tower
-FUSED_KSI = ["tensor", "ntt_inv", "conv", "ntt_fwd_a", "ksi", "ksi", "ntt_inv", "drop", "rsv", "ntt_fwd"]
+FUSED_KSI = ["tensor", "ntt_inv", "conv", "ntt_fwd_a", "ksi", "ksi", "ntt_inv", "rsv", "ntt_fwd"]
 
 
 def test_fused_plan_steps():
     steps = bench.step_bytes(N, L, K, ALPHA, NP, FUSED_KSI, B)
     names = [s[0] for s in steps]
     assert names == ["tensor", "intt_modup", "conv_modup", "ntt_digits_col", "rowpass_inner_q",
-                     "rowpass_inner_p", "intt_moddown", "drop_tower", "moddown_rescale_prologue",
-                     "ntt_rescale_epilogue"]
+                     "rowpass_inner_p", "intt_moddown", "moddown_rescale_prologue", "ntt_rescale_epilogue"]
     by = {s[0]: s[1] for s in steps}
     # each part converts its 6 own towers to the other 18 Q towers and the 6 P towers
     assert by["conv_modup"] == T * (4 * 6 + 4 * 24)
@@ -32,6 +31,8 @@ def test_fused_plan_steps():
     assert by["rowpass_inner_p"] == pytest.approx(T * 6 * (4 + 8 / B + 2))
     # per launch at B = 64, Q rows: 24 x (6 x 64 + 8) towers
     assert round(by["rowpass_inner_q"] * B) == 24 * (6 * 64 + 8) * T
+    # prologue with the dropped tower: per component 6 sources + dt + dr in, 23 towers out
+    assert by["moddown_rescale_prologue"] == T * 2 * (6 + 2 + 23)
 
 
 def test_key_rows_count_once_per_batch():

@@ -33,7 +33,8 @@ struct TensorJob {
 
 // dst[j] = (sum_i src[i] * tab[i*ntgt + j]) * R^-1 mod p_j for targets j in
 // [j0, j1); tab holds the conversion constants in Montgomery form, so one REDC
-// of the 128-bit sum gives the exact remainder.  src/dst are tower-contiguous.
+// of the 128-bit sum gives the remainder (left in [0, 2 p_j) when wide is 0:
+// the outputs only feed forward transforms).  src/dst are tower-contiguous.
 struct ConvJob {
     const uint64_t* src;
     uint64_t* dst;

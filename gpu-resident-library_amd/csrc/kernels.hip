@@ -281,6 +281,11 @@ __device__ __forceinline__ U128 dot_src(const uint64_t* src, uint32_t n, uint32_
     }
 }
 
+// Targets are left in [0, 2q) (REDC without its correction) where the sums
+// come from dot30 (at most 8 sources below 2^60: the sum is below 2^63 q, so
+// the uncorrected REDC is below 1.5 q + 1); every consumer of a conversion is
+// a forward transform, whose butterflies take inputs below 2q.  The generic
+// path (up to 16 sources, moduli up to 2^61) reduces fully.
 template <uint32_t NS>
 __device__ __forceinline__ void conv_one(const ConvJob& j, uint32_t n, uint32_t i) {
     SrcCols<NS> x;
@@ -288,7 +293,7 @@ __device__ __forceinline__ void conv_one(const ConvJob& j, uint32_t n, uint32_t 
     const cu64p tab = uni(j.tab), tq = uni(j.tq), tqn = uni(j.tqneg);
     for (uint32_t t = j.j0; t < j.j1; ++t) {
         const U128 acc = dot_src<NS>(j.src, n, i, j.nsrc, tab, j.ntgt, t, x);
-        ga(j.dst, t * n + i) = redc(acc, tq[t], tqn[t]);
+        ga(j.dst, t * n + i) = NS > 0 ? redc_lazy(acc, tq[t], tqn[t]) : redc(acc, tq[t], tqn[t]);
     }
 }
 
@@ -302,8 +307,8 @@ __global__ __launch_bounds__(PW_THREADS) void k_conv(const ConvJob* __restrict__
 }
 
 // An accumulated inner-product sum to [0, q).  With j.red the digits may be
-// the forward transforms' unreduced values (below 69q, NttJob::lazy_out):
-// the sum stays below 16 * 69 q^2 < 2^125 for q < 2^57, REDC without its
+// the forward transforms' unreduced values (below 70q, NttJob::lazy_out):
+// the sum stays below 16 * 70 q^2 < 2^125 for q < 2^57, REDC without its
 // correction leaves less than 2^62, and reduce_red finishes.
 __device__ __forceinline__ uint64_t inner_out(const U128& T, const InnerJob& j) {
     if (j.red) return redc_red(T, j.q, j.qneg, (uint32_t)j.red);

@@ -113,7 +113,8 @@ struct RsTarget {
 //   drop = dt + (dr - REDC(sum_k xp[k] * tabd[k*strided])) * pinv
 // with dt = INTT(t_last), dr = INTT(res_last), xp = INTT'd P towers scaled by
 // n^-1 PHatInv; then for the surviving towers i in [j0, j1)
-//   v_i = s2_i * switch(drop -> q_i) + REDC(sum_k xp[k] * tab[k*ntgt + i]);
+//   v_i = s2_i * switch(drop -> q_i) + REDC(sum_k xp[k] * tab[k*ntgt + i]),
+// left in [0, 4 q_i) when wide is 0 (the forward transform takes it so);
 // tab folds -P^-1 s1 into PHatModq, so NTT(v_i) + t_i s1 + res_i P^-1 s1 is
 // the rescaled output tower.
 struct RsJob {

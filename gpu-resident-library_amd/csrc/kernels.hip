@@ -284,7 +284,7 @@ __device__ __forceinline__ U128 dot_src(const uint64_t* src, uint32_t n, uint32_
 // Targets are left in [0, 2q) (REDC without its correction) where the sums
 // come from dot30 (at most 8 sources below 2^60: the sum is below 2^63 q, so
 // the uncorrected REDC is below 1.5 q + 1); every consumer of a conversion is
-// a forward transform, whose butterflies take inputs below 2q.  The generic
+// a forward transform, whose butterflies take inputs below 4q.  The generic
 // path (up to 16 sources, moduli up to 2^61) reduces fully.
 template <uint32_t NS>
 __device__ __forceinline__ void conv_one(const ConvJob& j, uint32_t n, uint32_t i) {
@@ -307,8 +307,8 @@ __global__ __launch_bounds__(PW_THREADS) void k_conv(const ConvJob* __restrict__
 }
 
 // An accumulated inner-product sum to [0, q).  With j.red the digits may be
-// the forward transforms' unreduced values (below 70q, NttJob::lazy_out):
-// the sum stays below 16 * 70 q^2 < 2^125 for q < 2^57, REDC without its
+// the forward transforms' unreduced values (below 72q, NttJob::lazy_out):
+// the sum stays below 16 * 72 q^2 < 2^125 for q < 2^57, REDC without its
 // correction leaves less than 2^62, and reduce_red finishes.
 __device__ __forceinline__ uint64_t inner_out(const U128& T, const InnerJob& j) {
     if (j.red) return redc_red(T, j.q, j.qneg, (uint32_t)j.red);
@@ -425,6 +425,22 @@ __device__ __forceinline__ void rs_one(const RsJob& j, uint32_t n, uint32_t i) {
         tg.s2p = w[5];
         tg.red = w[6];
         const U128 acc = dot_src<K>(j.xp, n, i, j.K, tab, j.ntgt, t, x);
+        if constexpr (K > 0) {
+            // the output only feeds the epilogue's forward transform, which
+            // takes inputs below 4q: the lazy Shoup product is below 2q and
+            // REDC without its correction below 1.5q + 1 (at most 8 sources
+            // below 2^60)
+            uint64_t sw;
+            if (tg.new_larger) {
+                sw = dv + (hi ? tg.diff : 0);
+            } else {
+                const uint64_t d = hi ? tg.diff : 0;
+                const uint64_t xm = shoup(dv, 1, tg.red, tg.q);
+                sw = xm >= d ? xm - d : xm + tg.q - d;
+            }
+            ga(j.out, t * n + i) = shoup_lazy(sw, tg.s2, tg.s2p, tg.q) + redc_lazy(acc, tg.q, tg.qneg);
+            continue;
+        }
         const uint64_t conv = redc(acc, tg.q, tg.qneg);
         // centred switch q_old -> q_t (kernels/switch_modulus.cu:11-27 semantics);
         // dv mod q_t by a Shoup reduction with w = 1 (exact for any dv < 2^64)

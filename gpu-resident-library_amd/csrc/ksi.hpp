@@ -104,7 +104,7 @@ GF_HD void ksi_body(const KsiJob& j, uint32_t N1, uint32_t tile, uint64_t* lds, 
                     const uint32_t k = __builtin_popcount(j.xf & ((1u << p) - 1));
                     for (uint32_t v = 0; v < V; ++v) {
                         dv[v] = lds[k * KSI_PART_WORDS + padded(idx + v)];
-                        // LZ: below 70q (the sums stay below 2^125); else [0, 8q) -> [0, q)
+                        // LZ: below 72q (the sums stay below 2^125); else [0, 8q) -> [0, q)
                         if constexpr (!LZ) dv[v] = ntt_final<true>(dv[v], t);
                     }
                 } else {

@@ -20,7 +20,7 @@
 // reduced once at the final store (needs 8q < 2^64, i.e. q < 2^61).  Forward
 // transforms over a modulus 2^32 < q < 2^57 (LZ) drop the forward butterfly's
 // conditional subtraction altogether: each stage adds at most 4q to the bound,
-// so inputs below 2q stay below (2 + 4 logn) q <= 70q < 2^64 and one Barrett-style reduction
+// so inputs below 4q stay below (4 + 4 logn) q <= 72q < 2^64 and one Barrett-style reduction
 // (reduce_red) at the final store replaces the per-stage ones.
 //
 // Twiddle index for CT stage st (local to the phase), sub-transform s, group i:
@@ -78,7 +78,7 @@ struct NttJob {
     // forward only: floor(2^64 / q) for a modulus 2^32 < q < 2^57 (lazy_red_of),
     // else 0.  Such a modulus leaves room in a 64-bit word for the whole
     // transform's growth, so the butterflies skip their conditional subtraction
-    // (inputs below 2q stay below (2 + 4 logn) q <= 70 q) and the final store reduces once.
+    // (inputs below 4q stay below (4 + 4 logn) q <= 72 q) and the final store reduces once.
     uint64_t red;
     // with red: store those unreduced values, for a consumer that reduces them
     // itself (the key inner product); ignored with an epilogue
@@ -356,7 +356,7 @@ GF_HD uint64_t ntt_final(uint64_t v, const NttJob& j) {
     return shoup(v, j.post, j.postp, j.q);
 }
 // forward final store with the optional linear epilogue at global index gi.
-// LZ: v is below 70q; with the epilogue's two lazy products below 78q < 2^64,
+// LZ: v is below 72q; with the epilogue's two lazy products below 80q < 2^64,
 // reduced once (or stored as is for a lazy_out job without epilogue).
 template <bool LZ>
 GF_HD uint64_t ntt_final_ep(uint64_t v, const NttJob& j, uint64_t gi) {

@@ -638,13 +638,13 @@ static void split_chunks(std::vector<J>& v, uint32_t n) {
 // transforms (plain n^-1) ride in the first INTT launch.
 void PlanBuilder::keyswitch_core(uint32_t l, const uint64_t* d, const DevKey& key, uint64_t* ws,
                                  const uint64_t* extra0, const uint64_t* extra1, uint64_t* dx0,
-                                 uint64_t* dx1) {
+                                 uint64_t* dx1, uint32_t lazy_rows) {
     const Ctx& C = C_;
     if (l < 1 || l > C.L) throw std::runtime_error("gpufhe: bad level");
     if (key.nparts < C.lv[l].np) throw std::runtime_error("gpufhe: key has too few parts for this level");
     if (use_ksi(l)) {
         modup(l, d, ws, extra0, extra1, dx0, dx1, true);
-        ksi_moddown_src(l, d, key, ws);
+        ksi_moddown_src(l, d, key, ws, lazy_rows);
         return;
     }
     modup(l, d, ws, extra0, extra1, dx0, dx1);
@@ -662,7 +662,8 @@ bool PlanBuilder::use_ksi(uint32_t l) const {
 // inner_moddown_src.  Digit towers of part pi other than the input's own hold
 // the column-pass output of their forward transform (modup with
 // rowpass_in_ksi).
-void PlanBuilder::ksi_moddown_src(uint32_t l, const uint64_t* d, const DevKey& key, uint64_t* ws) {
+void PlanBuilder::ksi_moddown_src(uint32_t l, const uint64_t* d, const DevKey& key, uint64_t* ws,
+                                  uint32_t lazy_rows) {
     const Ctx& C = C_;
     const uint32_t n = C.n, K = C.K, W = l + K;
     const LevelConst& lv = C.lv[l];
@@ -690,6 +691,7 @@ void PlanBuilder::ksi_moddown_src(uint32_t l, const uint64_t* d, const DevKey& k
         J.q = C.rc[row].q;
         J.qneg = C.rc[row].qneg;
         J.red = lazy_red_of(J.q);
+        J.lazy_out = i < lazy_rows;
         J.pairs = aligned16({J.o0, J.o1});
         for (uint32_t pi = 0; pi < lv.np; ++pi) J.pairs &= aligned16({J.src[pi], J.kb[pi], J.ka[pi]});
         ksi_.push_back(J);
@@ -847,7 +849,8 @@ void PlanBuilder::evalmult_rescale_fused(uint32_t l, const uint64_t* c0a, const 
     }
     uint64_t* dt = ws + w.dt * n;
     uint64_t* dr = ws + w.dr * n;
-    keyswitch_core(l, t[2], key, ws, t[0] + (size_t)last * n, t[1] + (size_t)last * n, dt, dt + n);
+    // accumulator rows below `last` only feed the epilogue's lazy Shoup products
+    keyswitch_core(l, t[2], key, ws, t[0] + (size_t)last * n, t[1] + (size_t)last * n, dt, dt + n, last);
     uint64_t* res[2] = {ws + w.res0 * n, ws + w.res1 * n};
     uint64_t* xp = ws + w.xp * n;
     uint64_t* v = ws + w.dig * n;     // digits are dead after the inner product

@@ -231,14 +231,17 @@ private:
     void evalmult_rescale_fused(uint32_t l, const uint64_t* c0a, const uint64_t* c1a,
                                 const uint64_t* c0b, const uint64_t* c1b, uint64_t* o0, uint64_t* o1,
                                 const DevKey& key, uint64_t* ws);
+    // lazy_rows: accumulator rows below it only feed the fused rescale
+    // epilogue, so the fused inner product may leave them unreduced
     void keyswitch_core(uint32_t l, const uint64_t* d, const DevKey& key, uint64_t* ws,
-                        const uint64_t* extra0, const uint64_t* extra1, uint64_t* dx0, uint64_t* dx1);
+                        const uint64_t* extra0, const uint64_t* extra1, uint64_t* dx0, uint64_t* dx1,
+                        uint32_t lazy_rows = 0);
     void modup(uint32_t l, const uint64_t* d, uint64_t* ws, const uint64_t* extra0, const uint64_t* extra1,
                uint64_t* dx0, uint64_t* dx1, bool rowpass_in_ksi = false);
     // the digits' row pass fused with the key inner product (two-phase rings,
     // up to KSI_MAXP parts, fused schedule on)
     bool use_ksi(uint32_t l) const;
-    void ksi_moddown_src(uint32_t l, const uint64_t* d, const DevKey& key, uint64_t* ws);
+    void ksi_moddown_src(uint32_t l, const uint64_t* d, const DevKey& key, uint64_t* ws, uint32_t lazy_rows);
     const uint64_t* digit_tower(uint32_t l, const uint64_t* d, const uint64_t* dig, uint32_t pi,
                                 uint32_t i) const;
     // gk != 0: the digits are read through the automorphism of index gk

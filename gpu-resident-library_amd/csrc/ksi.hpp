@@ -41,6 +41,9 @@ struct KsiJob {
     uint64_t red;           // lazy_red_of(q): the transforms skip their reductions
     uint32_t np, xf;
     uint32_t pairs;         // every tower pointer 16-byte aligned: two positions per access
+    // 1: the outputs only feed the rescale epilogue's lazy Shoup products
+    // (any 64-bit input): REDC without its correction, no final reduction
+    uint32_t lazy_out;
 };
 
 // V consecutive words from word i of a tower (V = 2: one 16-byte access; the
@@ -119,7 +122,10 @@ GF_HD void ksi_body(const KsiJob& j, uint32_t N1, uint32_t tile, uint64_t* lds, 
             }
             uint64_t r0[V], r1[V];
             for (uint32_t v = 0; v < V; ++v) {
-                if constexpr (LZ) {
+                if (j.lazy_out) {
+                    r0[v] = redc_lazy(s0[v], q, qneg);
+                    r1[v] = redc_lazy(s1[v], q, qneg);
+                } else if constexpr (LZ) {
                     r0[v] = redc_red(s0[v], q, qneg, (uint32_t)j.red);
                     r1[v] = redc_red(s1[v], q, qneg, (uint32_t)j.red);
                 } else {

@@ -102,7 +102,8 @@ extern "C" uint64_t emu_lazy_red_of(uint64_t q) { return lazy_red_of(q); }
 namespace {
 template <uint32_t RM, bool LZ>
 void emu_ksi_t(uint32_t n, uint64_t q, const uint64_t* t2, uint32_t np, uint32_t xf, uint64_t* src,
-               const uint64_t* kb, const uint64_t* ka, uint64_t* o0, uint64_t* o1, uint32_t pairs) {
+               const uint64_t* kb, const uint64_t* ka, uint64_t* o0, uint64_t* o1, uint32_t pairs,
+               uint32_t lazy_out) {
     const NttGeom g = make_geom(n, RM);
     std::vector<uint64_t> rows(row_table_words(g));
     build_row_table(t2, g, rows.data());
@@ -117,6 +118,7 @@ void emu_ksi_t(uint32_t n, uint64_t q, const uint64_t* t2, uint32_t np, uint32_t
     J.o0 = o0;
     J.o1 = o1;
     J.pairs = pairs;
+    J.lazy_out = lazy_out;
     std::vector<uint64_t> lds(NTT_TILE_PADDED > (size_t)np * KSI_PART_WORDS ? NTT_TILE_PADDED : (size_t)np * KSI_PART_WORDS);
     with_split(g.logN1, g.logN2, [&](auto A, auto B) {
         constexpr uint32_t S1 = decltype(A)::value, S2 = decltype(B)::value;
@@ -142,7 +144,7 @@ void emu_ksi_t(uint32_t n, uint64_t q, const uint64_t* t2, uint32_t np, uint32_t
 // fused row pass) or an evaluation-form digit; o = sum_p NTT?(src_p) k_p R^-1.
 extern "C" void emu_ksi(uint32_t n, uint64_t q, const uint64_t* tw, const uint64_t* twp, uint32_t rmax,
                         uint32_t np, uint32_t xf, uint64_t* src, const uint64_t* kb, const uint64_t* ka,
-                        uint64_t* o0, uint64_t* o1, uint32_t pairs) {
+                        uint64_t* o0, uint64_t* o1, uint32_t pairs, uint32_t lazy_out) {
     std::vector<uint64_t> t2(2 * (std::size_t)n);
     for (uint32_t i = 0; i < n; ++i) {
         t2[2 * i] = tw[i];
@@ -150,8 +152,8 @@ extern "C" void emu_ksi(uint32_t n, uint64_t q, const uint64_t* tw, const uint64
     }
     with_radix(rmax, [&](auto Rc) {
         constexpr uint32_t RM = decltype(Rc)::value;
-        if (lazy_red_of(q)) emu_ksi_t<RM, true>(n, q, t2.data(), np, xf, src, kb, ka, o0, o1, pairs);
-        else emu_ksi_t<RM, false>(n, q, t2.data(), np, xf, src, kb, ka, o0, o1, pairs);
+        if (lazy_red_of(q)) emu_ksi_t<RM, true>(n, q, t2.data(), np, xf, src, kb, ka, o0, o1, pairs, lazy_out);
+        else emu_ksi_t<RM, false>(n, q, t2.data(), np, xf, src, kb, ka, o0, o1, pairs, lazy_out);
     });
 }
 // the key inner product's output with unreduced digits: sum_s a[s] b[s]

@@ -43,7 +43,7 @@ def emu():
     lib.emu_lazy_red_of.argtypes = [C.c_uint64]
     lib.emu_ksi.restype = None
     lib.emu_ksi.argtypes = [C.c_uint32, C.c_uint64, u64p, u64p, C.c_uint32, C.c_uint32, C.c_uint32, u64p, u64p,
-                            u64p, u64p, u64p, C.c_uint32]
+                            u64p, u64p, u64p, C.c_uint32, C.c_uint32]
     lib.emu_inner_red.restype = C.c_uint64
     lib.emu_inner_red.argtypes = [u64p, u64p, C.c_uint32, C.c_uint64]
     lib.emu_reduce_red.restype = C.c_uint64
@@ -253,10 +253,14 @@ def test_fused_rowpass_inner_product(emu, logn, np_, xf):
         w1 = np.array([int(v) * Rinv % q for v in w1], np.uint64)
         for rmax in (2, 3, 4):
             pairs = rmax % 2   # both access widths of the inner product
-            s = src.copy()
-            o0 = np.zeros(n, np.uint64)
-            o1 = np.zeros(n, np.uint64)
-            emu.emu_ksi(n, q, O.P(fr), O.P(fp), rmax, np_, xf, O.P(s), O.P(np.ascontiguousarray(kb)),
-                        O.P(np.ascontiguousarray(ka)), O.P(o0), O.P(o1), pairs)
-            assert np.array_equal(o0, w0), (bits, rmax, pairs)
-            assert np.array_equal(o1, w1), (bits, rmax, pairs)
+            for lazy_out in (0, 1):
+                s = src.copy()
+                o0 = np.zeros(n, np.uint64)
+                o1 = np.zeros(n, np.uint64)
+                emu.emu_ksi(n, q, O.P(fr), O.P(fp), rmax, np_, xf, O.P(s), O.P(np.ascontiguousarray(kb)),
+                            O.P(np.ascontiguousarray(ka)), O.P(o0), O.P(o1), pairs, lazy_out)
+                if lazy_out:   # unreduced, for the epilogue's lazy products: congruent, below 2^62
+                    assert int(max(o0.max(), o1.max())) < 1 << 62
+                    o0, o1 = o0 % np.uint64(q), o1 % np.uint64(q)
+                assert np.array_equal(o0, w0), (bits, rmax, pairs, lazy_out)
+                assert np.array_equal(o1, w1), (bits, rmax, pairs, lazy_out)

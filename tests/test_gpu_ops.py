@@ -15,9 +15,10 @@ SHAPES = [(1 << 10, 4, 2), (1 << 13, 6, 3), (1 << 15, 4, 2), (1 << 15, 12, 3), (
 # largest ring the transforms support, the smallest ring; at a two-phase ring
 # the most parts the fused row pass + inner product takes (7, more parts than
 # waves) and one part more (the separate kernels); the 2^14 ring (the 128 x 128
-# transform split)
+# transform split); the benchmark ring with two digits (11 special primes), with 6 parts
+# (fused, more parts than waves) and with 8 parts (the separate kernels)
 EDGE_SHAPES = [(1 << 12, 4, 1), (1 << 12, 6, 6), (1 << 17, 3, 1), (1 << 4, 3, 3), (1 << 13, 7, 7),
-               (1 << 13, 8, 8), (1 << 14, 5, 2)]
+               (1 << 13, 8, 8), (1 << 14, 5, 2), (1 << 16, 24, 2), (1 << 16, 24, 6), (1 << 16, 24, 8)]
 
 
 @pytest.fixture(scope="module")

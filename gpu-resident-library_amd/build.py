@@ -1,5 +1,5 @@
 """Builds libgpufhe_amd.so for gfx950 from csrc/ with hipcc (in-tree, so the
-library travels with the source tree).  Usage: python build.py [-j N]"""
+library travels with the source tree).  Usage: python build.py [-j N] [--incremental]"""
 import concurrent.futures as cf
 import os
 import subprocess
@@ -16,8 +16,20 @@ FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-
 SOURCES = ["kernels.hip", "engine.hip", "capi.hip", "serve.hip", "params.cpp", "api.cpp", "keys.cpp", "wire.cpp"]
 
 
-def _compile(src):
+def _stale(src, obj):
+    """object missing or older than its source or any header of csrc/ and include/"""
+    if not os.path.exists(obj):
+        return True
+    t = os.path.getmtime(obj)
+    deps = [os.path.join(SRC, src)] + [os.path.join(d, f) for d in (SRC, os.path.join(HERE, "..", "include"))
+                                       for f in os.listdir(d) if f.endswith((".hpp", ".h"))]
+    return any(os.path.getmtime(p) > t for p in deps)
+
+
+def _compile(src, incremental=False):
     obj = os.path.join(OUT, src + ".o")
+    if incremental and not _stale(src, obj):
+        return obj, ""
     cmd = ["hipcc"] + FLAGS + (["-x", "hip"] if src.endswith(".hip") else []) + \
           ["-c", os.path.join(SRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -28,11 +40,13 @@ def _compile(src):
     return obj, msg
 
 
-def build(jobs=8, verbose=False):
+def build(jobs=8, verbose=False, incremental=False):
+    """Compiles every source (incremental: only those whose object is stale)
+    and links the library."""
     os.makedirs(OUT, exist_ok=True)
     srcs = [s for s in SOURCES if os.path.exists(os.path.join(SRC, s))]
     with cf.ThreadPoolExecutor(jobs) as ex:
-        res = list(ex.map(_compile, srcs))
+        res = list(ex.map(lambda s: _compile(s, incremental), srcs))
     for _, msg in res:
         if msg.strip() and verbose:
             print(msg)
@@ -45,4 +59,4 @@ def build(jobs=8, verbose=False):
 
 if __name__ == "__main__":
     j = int(sys.argv[sys.argv.index("-j") + 1]) if "-j" in sys.argv else 8
-    print(build(j, verbose=True))
+    print(build(j, verbose=True, incremental="--incremental" in sys.argv))

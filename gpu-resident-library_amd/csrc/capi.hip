@@ -1,6 +1,8 @@
 // extern "C" handle API (include/gpufhe_capi.h) over the engine.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <functional>
 #include <map>
@@ -23,15 +25,16 @@ using namespace gf;
 static thread_local std::string g_err;
 void gf::set_last_error(const char* msg) { g_err = msg; }
 
-// single-op plans are cached by (op, arguments); building one uploads its job
-// arrays synchronously, which a stream capture cannot contain.  The caller
-// holds a reference while it launches, so eviction by another thread never
-// frees a plan in use.
-static std::shared_ptr<Plan> cached_plan(gf_ctx* c, std::vector<uint64_t> key, hipStream_t s,
-                                         const std::function<void(PlanBuilder&)>& build) {
+// single-op plans are cached by (op, arguments), keys by serial number;
+// building one uploads its job arrays synchronously, which a stream capture
+// cannot contain.  The caller holds a reference while it launches, so
+// eviction by another thread never frees a plan in use.  `keys`: the serials
+// of the keys the plan reads (gf_key_destroy evicts those plans).
+static std::shared_ptr<Plan> cached_plan(gf_ctx* c, std::vector<uint64_t> key, std::vector<uint64_t> keys,
+                                         hipStream_t s, const std::function<void(PlanBuilder&)>& build) {
     std::lock_guard<std::mutex> lk(c->mu);
     auto it = c->cache.find(key);
-    if (it != c->cache.end()) return it->second;
+    if (it != c->cache.end()) return it->second.plan;
     hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
     if (s && hipStreamIsCapturing(s, &st) == hipSuccess && st != hipStreamCaptureStatusNone)
         throw std::runtime_error("gpufhe: first call of an operation with new arguments inside a stream capture; "
@@ -40,8 +43,13 @@ static std::shared_ptr<Plan> cached_plan(gf_ctx* c, std::vector<uint64_t> key, h
     PlanBuilder b(*c->C);
     build(b);
     std::shared_ptr<Plan> P(b.finish());
-    c->cache.emplace(std::move(key), P);
+    c->cache.emplace(std::move(key), gf_cached_plan{P, std::move(keys)});
     return P;
+}
+
+uint64_t gf_key::next_serial() {
+    static std::atomic<uint64_t> next{1};
+    return next.fetch_add(1);
 }
 
 #define U(x) ((uint64_t)(uintptr_t)(x))
@@ -109,7 +117,24 @@ gf_key* gf_key_create(gf_ctx* c, const uint64_t* av, const uint64_t* bv, uint32_
         return k;
     });
 }
-void gf_key_destroy(gf_key* k) { delete k; }
+void gf_key_destroy(gf_key* k) {
+    if (!k) return;
+    {
+        // plans that read this key hold its device pointers: drop them with it
+        std::lock_guard<std::mutex> lk(k->ctx->mu);
+        auto& cache = k->ctx->cache;
+        for (auto it = cache.begin(); it != cache.end();) {
+            const auto& ks = it->second.keys;
+            if (std::find(ks.begin(), ks.end(), k->serial) != ks.end()) it = cache.erase(it);
+            else ++it;
+        }
+    }
+    delete k;
+}
+uint32_t gf_ctx_cached_plans(gf_ctx* c) {
+    std::lock_guard<std::mutex> lk(c->mu);
+    return (uint32_t)c->cache.size();
+}
 
 gf_ctx* gf_ctx_create_alpha(uint32_t n, const uint64_t* q, uint32_t L, const uint64_t* p, uint32_t K,
                             const uint64_t* roots, uint32_t alpha) {
@@ -229,7 +254,7 @@ int gf_automorph(uint32_t n, const uint64_t* d_in, uint64_t* d_out, uint32_t tow
 }
 int gf_rescale_comp(gf_ctx* c, uint32_t l, const uint64_t* d_c, uint64_t* d_o, uint64_t* ws, hipStream_t s) {
     return guard([&] {
-        auto P = cached_plan(c, {7, l, U(d_c), U(d_o), U(ws)}, s,
+        auto P = cached_plan(c, {7, l, U(d_c), U(d_o), U(ws)}, {}, s,
                               [&](PlanBuilder& b) { b.rescale_one(l, d_c, d_o, ws); });
         P->run(s);
     });
@@ -239,7 +264,7 @@ int gf_evalmult_rescale(gf_ctx* c, gf_key* k, uint32_t l, const uint64_t* c0a, c
                         const uint64_t* c0b, const uint64_t* c1b, uint64_t* o0, uint64_t* o1, uint64_t* ws,
                         hipStream_t s) {
     return guard([&] {
-        auto P = cached_plan(c, {1, l, U(k), U(c0a), U(c1a), U(c0b), U(c1b), U(o0), U(o1), U(ws)}, s,
+        auto P = cached_plan(c, {1, l, k->serial, U(c0a), U(c1a), U(c0b), U(c1b), U(o0), U(o1), U(ws)}, {k->serial}, s,
                               [&](PlanBuilder& b) { b.evalmult_rescale(l, c0a, c1a, c0b, c1b, o0, o1, *k->K, ws); });
         P->run(s);
     });
@@ -247,7 +272,7 @@ int gf_evalmult_rescale(gf_ctx* c, gf_key* k, uint32_t l, const uint64_t* c0a, c
 int gf_evalmult(gf_ctx* c, gf_key* k, uint32_t l, const uint64_t* c0a, const uint64_t* c1a, const uint64_t* c0b,
                 const uint64_t* c1b, uint64_t* o0, uint64_t* o1, uint64_t* ws, hipStream_t s) {
     return guard([&] {
-        auto P = cached_plan(c, {2, l, U(k), U(c0a), U(c1a), U(c0b), U(c1b), U(o0), U(o1), U(ws)}, s,
+        auto P = cached_plan(c, {2, l, k->serial, U(c0a), U(c1a), U(c0b), U(c1b), U(o0), U(o1), U(ws)}, {k->serial}, s,
                               [&](PlanBuilder& b) { b.evalmult(l, c0a, c1a, c0b, c1b, o0, o1, *k->K, ws); });
         P->run(s);
     });
@@ -255,7 +280,7 @@ int gf_evalmult(gf_ctx* c, gf_key* k, uint32_t l, const uint64_t* c0a, const uin
 int gf_keyswitch(gf_ctx* c, gf_key* k, uint32_t l, const uint64_t* d, uint64_t* ba0, uint64_t* ba1, uint64_t* ws,
                  hipStream_t s) {
     return guard([&] {
-        auto P = cached_plan(c, {3, l, U(k), U(d), U(ba0), U(ba1), U(ws)}, s,
+        auto P = cached_plan(c, {3, l, k->serial, U(d), U(ba0), U(ba1), U(ws)}, {k->serial}, s,
                               [&](PlanBuilder& b) { b.keyswitch(l, d, ba0, ba1, *k->K, ws); });
         P->run(s);
     });
@@ -263,7 +288,7 @@ int gf_keyswitch(gf_ctx* c, gf_key* k, uint32_t l, const uint64_t* d, uint64_t* 
 int gf_rescale(gf_ctx* c, uint32_t l, const uint64_t* c0, const uint64_t* c1, uint64_t* o0, uint64_t* o1,
                uint64_t* ws, hipStream_t s) {
     return guard([&] {
-        auto P = cached_plan(c, {4, l, U(c0), U(c1), U(o0), U(o1), U(ws)}, s,
+        auto P = cached_plan(c, {4, l, U(c0), U(c1), U(o0), U(o1), U(ws)}, {}, s,
                               [&](PlanBuilder& b) { b.rescale(l, c0, c1, o0, o1, ws); });
         P->run(s);
     });
@@ -271,7 +296,7 @@ int gf_rescale(gf_ctx* c, uint32_t l, const uint64_t* c0, const uint64_t* c1, ui
 int gf_rotate(gf_ctx* c, gf_key* k, uint32_t l, uint64_t kk, const uint64_t* c0, const uint64_t* c1, uint64_t* o0,
               uint64_t* o1, uint64_t* ws, hipStream_t s) {
     return guard([&] {
-        auto P = cached_plan(c, {5, l, U(k), kk, U(c0), U(c1), U(o0), U(o1), U(ws)}, s,
+        auto P = cached_plan(c, {5, l, k->serial, kk, U(c0), U(c1), U(o0), U(o1), U(ws)}, {k->serial}, s,
                               [&](PlanBuilder& b) { b.rotate(l, kk, c0, c1, o0, o1, *k->K, ws); });
         P->run(s);
     });
@@ -279,7 +304,7 @@ int gf_rotate(gf_ctx* c, gf_key* k, uint32_t l, uint64_t kk, const uint64_t* c0,
 int gf_add(gf_ctx* c, uint32_t l, const uint64_t* a0, const uint64_t* a1, const uint64_t* b0, const uint64_t* b1,
            uint64_t* o0, uint64_t* o1, hipStream_t s) {
     return guard([&] {
-        auto P = cached_plan(c, {6, l, U(a0), U(a1), U(b0), U(b1), U(o0), U(o1)}, s,
+        auto P = cached_plan(c, {6, l, U(a0), U(a1), U(b0), U(b1), U(o0), U(o1)}, {}, s,
                               [&](PlanBuilder& b) { b.add(l, a0, a1, b0, b1, o0, o1); });
         P->run(s);
     });
@@ -345,7 +370,7 @@ gf_plan* gf_plan_linear_transform(gf_ctx* c, uint32_t count, const uint32_t* lev
 int gf_mul_plain(gf_ctx* c, uint32_t l, const uint64_t* c0, const uint64_t* c1, const uint64_t* pt, uint64_t* o0,
                  uint64_t* o1, hipStream_t s) {
     return guard([&] {
-        auto P = cached_plan(c, {10, l, U(c0), U(c1), U(pt), U(o0), U(o1)}, s,
+        auto P = cached_plan(c, {10, l, U(c0), U(c1), U(pt), U(o0), U(o1)}, {}, s,
                              [&](PlanBuilder& b) { b.mul_plain(l, c0, c1, pt, o0, o1); });
         P->run(s);
     });
@@ -353,7 +378,7 @@ int gf_mul_plain(gf_ctx* c, uint32_t l, const uint64_t* c0, const uint64_t* c1, 
 int gf_add_plain(gf_ctx* c, uint32_t l, const uint64_t* c0, const uint64_t* c1, const uint64_t* pt, uint64_t* o0,
                  uint64_t* o1, hipStream_t s) {
     return guard([&] {
-        auto P = cached_plan(c, {11, l, U(c0), U(c1), U(pt), U(o0), U(o1)}, s,
+        auto P = cached_plan(c, {11, l, U(c0), U(c1), U(pt), U(o0), U(o1)}, {}, s,
                              [&](PlanBuilder& b) { b.add_plain(l, c0, c1, pt, o0, o1); });
         P->run(s);
     });
@@ -363,14 +388,15 @@ int gf_rotate_hoisted(gf_ctx* c, uint32_t l, const uint64_t* c0, const uint64_t*
                       uint64_t* const* ws_rot, hipStream_t s) {
     return guard([&] {
         if (m == 0) throw std::runtime_error("gpufhe: hoisted rotation needs at least one index");
-        std::vector<uint64_t> key{12, l, m, U(c0), U(c1), U(ws_base)};
+        std::vector<uint64_t> key{12, l, m, U(c0), U(c1), U(ws_base)}, serials;
         std::vector<const DevKey*> keys(m);
         for (uint32_t j = 0; j < m; ++j) {
             if (!rks[j]) throw std::runtime_error("gpufhe: hoisted rotation needs a key per index");
             keys[j] = rks[j]->K.get();
-            key.insert(key.end(), {U(rks[j]), ks[j], U(o0[j]), U(o1[j]), U(ws_rot[j])});
+            key.insert(key.end(), {rks[j]->serial, ks[j], U(o0[j]), U(o1[j]), U(ws_rot[j])});
+            serials.push_back(rks[j]->serial);
         }
-        auto P = cached_plan(c, std::move(key), s, [&](PlanBuilder& b) {
+        auto P = cached_plan(c, std::move(key), std::move(serials), s, [&](PlanBuilder& b) {
             b.rotate_hoisted(l, m, ks, c0, c1, o0, o1, keys.data(), ws_base, ws_rot);
         });
         P->run(s);
@@ -380,13 +406,15 @@ int gf_linear_transform(gf_ctx* c, uint32_t l, const uint64_t* c0, const uint64_
                         gf_key* const* rks, const uint64_t* ks, const uint64_t* const* pts, uint64_t* o0,
                         uint64_t* o1, uint64_t* ws_base, uint64_t* const* ws_rot, hipStream_t s) {
     return guard([&] {
-        std::vector<uint64_t> key{13, l, m, U(c0), U(c1), U(o0), U(o1), U(ws_base)};
+        std::vector<uint64_t> key{13, l, m, U(c0), U(c1), U(o0), U(o1), U(ws_base)}, serials;
         std::vector<const DevKey*> keys(m);
         for (uint32_t j = 0; j < m; ++j) {
             keys[j] = rks[j] ? rks[j]->K.get() : nullptr;
-            key.insert(key.end(), {U(rks[j]), ks[j], U(pts[j]), U(ws_rot[j])});
+            const uint64_t ser = rks[j] ? rks[j]->serial : 0;
+            key.insert(key.end(), {ser, ks[j], U(pts[j]), U(ws_rot[j])});
+            if (rks[j]) serials.push_back(ser);
         }
-        auto P = cached_plan(c, std::move(key), s, [&](PlanBuilder& b) {
+        auto P = cached_plan(c, std::move(key), std::move(serials), s, [&](PlanBuilder& b) {
             b.linear_transform(l, m, ks, c0, c1, pts, o0, o1, keys.data(), ws_base, ws_rot);
         });
         P->run(s);

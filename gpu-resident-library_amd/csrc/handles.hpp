@@ -9,14 +9,25 @@
 
 #include "engine.hpp"
 
+// A cached single-op plan embeds the device pointers of the keys it reads;
+// `keys` lists those keys' serial numbers so that destroying a key evicts it.
+struct gf_cached_plan {
+    std::shared_ptr<gf::Plan> plan;
+    std::vector<uint64_t> keys;
+};
 struct gf_ctx {
     std::unique_ptr<gf::Ctx> C;
     std::mutex mu;
-    std::map<std::vector<uint64_t>, std::shared_ptr<gf::Plan>> cache;  // single-op plans
+    // single-op plans by (op, arguments); keys appear by serial, never by address
+    std::map<std::vector<uint64_t>, gf_cached_plan> cache;
 };
 struct gf_key {
     gf_ctx* ctx;
     std::unique_ptr<gf::DevKey> K;
+    // process-wide unique and never reused (a new key may reuse a freed
+    // handle's address, never its serial)
+    uint64_t serial = next_serial();
+    static uint64_t next_serial();
 };
 struct gf_plan {
     std::unique_ptr<gf::Plan> P;

@@ -47,6 +47,7 @@ _capi = {
     "gf_key_destroy": (None, [vp]),
     "gf_ctx_create_alpha": (vp, [u32, u64p, u32, u64p, u32, u64p, u32]),
     "gf_ctx_towers": (u32, [vp]),
+    "gf_ctx_cached_plans": (u32, [vp]),
     "gf_key_create_rows": (vp, [vp, u64p, u64p, u32, u32, u32, vp]),
     "gf_key_create_uniform": (vp, [vp, u32, u64, vp]),
     "gf_key_generate": (vp, [vp, vp, vp, vp, u64, C.c_double, u64, vp]),
@@ -221,6 +222,10 @@ class Context:
 
     def ntt_radix(self):
         return lib.gf_ctx_get_ntt_radix(self.h)
+
+    def cached_plans(self):
+        """single-op plans held in the context's cache"""
+        return lib.gf_ctx_cached_plans(self.h)
 
     def autotune(self, stream):
         r, ms = C.c_uint32(), C.c_float()

@@ -69,7 +69,12 @@ size_t gf_ws_bytes(const gf_ctx* c);
  * (src/evalkeygen.cpp:42-96).  Uploaded once, kept resident. */
 gf_key* gf_key_create(gf_ctx* c, const uint64_t* h_av, const uint64_t* h_bv, uint32_t nparts,
                       hipStream_t s);
+/* Frees the key and evicts the context's cached single-op plans that read it
+ * (they hold its device pointers). */
 void gf_key_destroy(gf_key* k);
+/* number of single-op plans the context holds in its cache (one per
+ * operation and argument set called so far) */
+uint32_t gf_ctx_cached_plans(gf_ctx* c);
 
 /* Context with an explicit part width alpha (KeySwitchConstants::alpha of
  * include/keyswitch.h:17-19; ks_context_create builds one per level). */

@@ -137,3 +137,16 @@ def _mulmod_vec(a, b, q):
 
 def mulmod_vec(a, b, q):
     return _mulmod_vec(np.asarray(a, np.uint64), np.asarray(b, np.uint64), int(q))
+
+
+def parallel_map(fn, items, workers=None):
+    """fn over items on a thread pool (the oracle's ctypes calls release the
+    GIL, so independent oracle operations run in parallel); results in order.
+    At most 16 threads, the host share of one GPU's box."""
+    import concurrent.futures as cf
+    import os
+    items = list(items)
+    if workers is None:
+        workers = max(1, min(16, len(os.sched_getaffinity(0)), len(items)))
+    with cf.ThreadPoolExecutor(workers) as ex:
+        return list(ex.map(fn, items))

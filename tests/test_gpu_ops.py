@@ -663,3 +663,51 @@ def test_decrypt_slots_rotation(G):
     s.sync()
     dec = O.decrypt(n, int(P.q[0]), int(P.roots[0]), o0.download(), o1.download(), ks.sQ)
     assert np.max(np.abs(decode(dec, n, delta).real - np.roll(z, -2))) < 1e-6
+
+
+def test_key_destroy_evicts_cached_plans(G):
+    """A single-op plan holds its key's device pointers: destroying the key
+    evicts it, and a new key on the same buffers (whose handle may reuse the
+    freed address) gets a plan of its own.  Key B has a different row layout
+    (fullQ = L + 2), so a stale plan would read the wrong rows."""
+    n, L, dnum = 1 << 12, 6, 3
+    P = make_params(n, L, dnum)
+    ctx = G.Context(n, P.q, P.p, P.roots, dnum)
+    rng = np.random.default_rng(77)
+    s = G.Stream()
+    ws = ctx.workspace()
+    d = rand_ct(P, L, rng)
+    dd, b0, b1 = G.Dev.of(d), G.Dev(L * n), G.Dev(L * n)
+
+    def rand_rows(rows_mods):
+        return np.concatenate([rng.integers(0, int(m), n, dtype=np.uint64) for _ in range(dnum) for m in rows_mods])
+
+    av, bv = rand_rows(P.mods), rand_rows(P.mods)
+    ka = G.Key(ctx, av, bv, dnum)
+    G.check(G.lib.gf_keyswitch(ctx.h, ka.h, L, dd.ptr, b0.ptr, b1.ptr, ws.ptr, s.s))
+    s.sync()
+    w0, w1 = P.keyswitch(L, d, av, bv)
+    assert np.array_equal(b0.download(), w0) and np.array_equal(b1.download(), w1)
+    before = ctx.cached_plans()
+    assert before >= 1
+    del ka   # gf_key_destroy
+    assert ctx.cached_plans() == before - 1
+    # key B: rows = fullQ + K with fullQ = L + 2 (two unused Q rows before the P rows)
+    fullQ = L + 2
+    row_mods = list(P.q) + [P.q[0], P.q[1]] + list(P.p)
+    av2, bv2 = rand_rows(row_mods), rand_rows(row_mods)
+    h = G.lib.gf_key_create_rows(ctx.h, G.P64(av2), G.P64(bv2), dnum, fullQ + P.K, fullQ, None)
+    assert h, G.lib.gf_last_error()
+    G.check(G.lib.gf_keyswitch(ctx.h, h, L, dd.ptr, b0.ptr, b1.ptr, ws.ptr, s.s))
+    s.sync()
+    # the same key in the context's own row layout for the oracle
+    rows = fullQ + P.K
+    pick = list(range(L)) + [fullQ + k for k in range(P.K)]
+    a3 = av2.reshape(dnum, rows, n)[:, pick, :].reshape(-1)
+    b3 = bv2.reshape(dnum, rows, n)[:, pick, :].reshape(-1)
+    w0, w1 = P.keyswitch(L, d, a3, b3)
+    try:
+        assert np.array_equal(b0.download(), w0) and np.array_equal(b1.download(), w1)
+    finally:
+        G.lib.gf_key_destroy(h)
+    assert ctx.cached_plans() == before - 1

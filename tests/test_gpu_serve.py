@@ -152,3 +152,29 @@ def test_serving_rejects_bad_requests(G):
         srv.submit(G.Request(rid + 1, G.OP_ADD, 4, 0, d.ptr, d.ptr, d.ptr, d.ptr, None, None, -1.0))
     with pytest.raises(G.GpuFheError):
         srv.submit(G.Request(rid, G.OP_EVALMULT_RESCALE, 9, 0, d.ptr, d.ptr, d.ptr, d.ptr, None, None, -1.0))
+
+
+def test_serving_clock_alignment(G):
+    """Latency subtracts a host-clock arrival from a device-event completion.
+    The two clocks are aligned once, when the server is created; this bounds
+    their skew over a few seconds: each request's device completion time
+    falls between the host time just before it was submitted and the host
+    time just after it was drained (plus 1 ms of slack for the event
+    timestamp's resolution)."""
+    import time
+    R = ring_setup(G, 1 << 12, 4, 2, 1, 12)
+    srv = G.Server(ws_budget_bytes=1 << 30, lanes=2, max_batch=4)
+    rid = srv.add_ring(R["ctx"], R["rlk"], R["rk"], R["rot_k"])
+    P = R["P"]
+    a = [G.Dev.of(x) for x in rand_ct(P, P.L, R["rng"])]
+    b = [G.Dev.of(x) for x in rand_ct(P, P.L, R["rng"])]
+    for i in range(12):
+        before = G.lib.gf_server_now_ms(srv.h)
+        srv.submit(G.Request(rid, G.OP_EVALMULT_RESCALE, P.L, 0, a[0].ptr, a[1].ptr, b[0].ptr, b[1].ptr,
+                             None, None, -1.0))
+        srv.drain()
+        after = G.lib.gf_server_now_ms(srv.h)
+        (c,) = srv.poll()
+        assert before <= c.arrival_ms <= c.launch_ms <= after, (i, before, c.arrival_ms, c.launch_ms, after)
+        assert before - 1.0 <= c.done_ms <= after + 1.0, (i, before, c.done_ms, after)
+        time.sleep(0.25)

@@ -156,11 +156,184 @@ def cpu_baseline(q, p, roots, av, bv, seconds=12.0, threads=1):
 
 
 def host_threads():
-    """CPU threads for the multi-core baseline: the job's thread allowance
-    (OMP_NUM_THREADS) when set, else the usable cores, at most 16."""
+    """CPU threads for the multi-core baseline: the cores this process may run
+    on (its affinity mask), limited by the job's thread allowance
+    OMP_NUM_THREADS when that is set (the GPU box sets it to its CPU share)."""
+    n = len(os.sched_getaffinity(0))
     env = os.environ.get("OMP_NUM_THREADS")
-    n = int(env) if env and env.isdigit() else len(os.sched_getaffinity(0))
-    return max(1, min(16, n))
+    if env and env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n)
+
+
+def host_core_info():
+    env = os.environ.get("OMP_NUM_THREADS")
+    return dict(host_cores=os.cpu_count(), allowed_cores=len(os.sched_getaffinity(0)),
+                omp_num_threads=int(env) if env and env.isdigit() else None, threads_used=host_threads())
+
+
+def boot_ci(d, it=4000, seed=7):
+    """95% bootstrap interval of the median of d: the reference benchmark's
+    statistic (bench/bench_evalmult_resident.cpp:22-32), with numpy resampling."""
+    d = np.asarray(d, float)
+    if d.size < 3:
+        return None
+    rng = np.random.default_rng(seed)
+    m = np.median(d[rng.integers(0, d.size, (it, d.size))], axis=1)
+    return [round(float(np.quantile(m, 0.025)), 4), round(float(np.quantile(m, 0.975)), 4)]
+
+
+def reference_sequence_legs(G, ctx, key, L, chains, stream, check_oracle=None):
+    """The reference benchmark's GPU legs (bench/bench_evalmult_resident.cpp:
+    96-128) at this context: `chains` independent EvalMult+Rescale chains, each
+    on its own stream and buffers, running the reference's launch sequence over
+    the drop-in boundary (LaunchTensor -> keyswitch_resident -> LaunchCombine ->
+    rescale_resident_raw x 2), launched directly and as one captured graph per
+    chain; beside them the same operations as one batched plan replayed as a
+    hipGraph, and one at a time through gf_evalmult_rescale.  Each leg is timed
+    by the host clock around its launches and a device synchronisation, as
+    the reference times its legs.  Returns ({leg: callable}, parity, chains):
+    parity is True when all legs give identical residues (and check_oracle,
+    if given, accepts the first chain)."""
+    n = ctx.n
+    d_mods = G.Dev.of(np.asarray(ctx.q, np.uint64))
+    ch = [G.RefSequence(ctx, key, L, d_mods, 10007 * (c + 1)) for c in range(chains)]
+    outs = [[G.Dev((L - 1) * n) for _ in range(2)] for _ in range(chains)]
+    col = lambda j: [c.ins[j].ptr for c in ch]  # noqa: E731
+    plan = G.plan_evalmult_rescale(ctx, key, [L] * chains, col(0), col(1), col(2), col(3),
+                                   [o[0].ptr for o in outs], [o[1].ptr for o in outs], [c.ws.ptr for c in ch])
+    per = [[G.Dev((L - 1) * n) for _ in range(2)] for _ in range(chains)]
+
+    def direct():
+        for c in ch:
+            c.launch()
+        G.device_sync()
+
+    def graphs():
+        for c in ch:
+            c.graph.launch(c.stream)
+        G.device_sync()
+
+    def batched():
+        plan.run_graph(stream)
+        stream.sync()
+
+    def per_op():
+        for c, o in zip(ch, per):
+            G.check(G.lib.gf_evalmult_rescale(ctx.h, key.h, L, *[x.ptr for x in c.ins], o[0].ptr, o[1].ptr,
+                                              c.ws.ptr, stream.s))
+        stream.sync()
+
+    direct()
+    for c in ch:
+        c.capture()
+    graphs()
+    batched()
+    per_op()
+    parity = True
+    for c, o, po in zip(ch, outs, per):
+        r0, r1 = c.outputs()
+        parity &= bool(np.array_equal(r0, o[0].download()) and np.array_equal(r1, o[1].download()))
+        parity &= bool(np.array_equal(r0, po[0].download()) and np.array_equal(r1, po[1].download()))
+    if check_oracle is not None:
+        parity &= bool(check_oracle(ch[0]))
+    legs = dict(reference_direct=direct, reference_graph=graphs, batched_plan_graph=batched, per_op=per_op)
+    legs["_keep"] = (plan, outs, per, d_mods)
+    return legs, parity, ch
+
+
+def time_legs(legs, reps, cpu=None):
+    """Per rep, every leg once in turn (then the CPU leg, when given): the
+    interleaved reps of the reference benchmark.  ms per leg per rep."""
+    names = [k for k in legs if not k.startswith("_")] + (["cpu"] if cpu else [])
+    times = {k: [] for k in names}
+    for _ in range(reps):
+        for k in names:
+            fn = cpu if k == "cpu" else legs[k]
+            t0 = time.perf_counter()
+            fn()
+            times[k].append((time.perf_counter() - t0) * 1e3)
+    return times
+
+
+def summarize_legs(times, ops):
+    out = {}
+    for k, v in times.items():
+        med = float(np.median(v))
+        out[k] = dict(median_ms=round(med, 4), ms_per_op=round(med / ops, 4), ci95_ms=boot_ci(v),
+                      ops_s=round(ops / (med / 1e3), 1))
+    return out
+
+
+def oracle_params(n, q, p, roots, dnum):
+    sys.path.insert(0, ROOT)
+    import oracle as O
+    return O.Params(n, q, p, roots, dnum)
+
+
+def run_cfg1(G, reps=9, chains=16, with_cpu=True):
+    """The reference's published configuration (README.md:92-96;
+    bench/bench_evalmult_resident.cpp:38-42): EvalMult+Rescale at n = 2^15
+    with 4 towers (60 + 3 x 50-bit), dnum = 2, K = 2 60-bit special primes, a
+    batch of 16.  Per rep, interleaved: the reference sequence direct and as
+    graphs, the batched plan as a graph, the per-op entry point, and the CPU
+    oracle on the same 16 operations (spread over the host threads); medians,
+    and bootstrap 95% intervals of the median and of the median paired
+    GPU - CPU difference."""
+    import concurrent.futures as cf
+    n, L, dnum = 1 << 15, 4, 2
+    q, p, roots, K, alpha = native_params(G, n, L, dnum)
+    stream = G.Stream()
+    ctx = G.Context(n, q, p, roots, dnum)
+    radix, _ = ctx.autotune(stream)
+    key = G.Key.uniform(ctx, dnum, 4242, stream)
+    av, bv = key.download(stream)
+    P = oracle_params(n, q, p, roots, dnum)
+
+    def check_oracle(c):
+        w0, w1 = P.evalmult_rescale(L, *[x.download() for x in c.ins], av, bv)
+        r0, r1 = c.outputs()
+        return np.array_equal(r0, w0) and np.array_equal(r1, w1)
+
+    legs, parity, ch = reference_sequence_legs(G, ctx, key, L, chains, stream, check_oracle)
+    cpu = pool = None
+    nt = host_threads()
+    if with_cpu:
+        host_in = [[x.download() for x in c.ins] for c in ch]
+        pool = cf.ThreadPoolExecutor(nt)
+
+        def cpu():
+            list(pool.map(lambda x: P.evalmult_rescale(L, *x, av, bv), host_in))
+
+        cpu()
+    times = time_legs(legs, reps, cpu)
+    if pool:
+        pool.shutdown()
+    out = dict(workload="EvalMult+Rescale, n=32768, L=4 (60+3x50-bit), dnum=2, K=%d (60-bit), batch %d" % (K, chains),
+               ntt_radix_log2=radix, reps=reps, parity_bit_exact=parity, legs=summarize_legs(times, chains))
+    if with_cpu:
+        out["cpu_threads"] = nt
+        for k in ("reference_graph", "batched_plan_graph"):
+            d = np.asarray(times[k]) - np.asarray(times["cpu"])
+            out["%s_minus_cpu" % k] = dict(median_ms=round(float(np.median(d)), 4), ci95_ms=boot_ci(d))
+            out["cpu_over_%s" % k] = round(float(np.median(times["cpu"]) / np.median(times[k])), 2)
+    out["reference_published"] = dict(gpu_ms_per_op=3.87, cpu_ms_per_op=53.1, hardware="RTX 2060 Max-Q; OpenFHE CPU",
+                                      source="README.md:92-96", note="context only: other hardware and CPU library")
+    return out
+
+
+def run_reference_sequence_cfg3(G, ctx, key, oracle_check=None, chains=16, reps=5):
+    """The reference's launch sequence (as in run_cfg1) at n = 2^16, L = 24,
+    dnum = 4: `chains` chains direct and as graphs, beside the batched plan
+    and the per-op entry point on the same operations; oracle_check(chain)
+    (when given) checks the first chain against the oracle."""
+    legs, parity, _ = reference_sequence_legs(G, ctx, key, L, chains, G.Stream(), oracle_check)
+    times = time_legs(legs, reps)
+    out = dict(workload="EvalMult+Rescale, n=65536, L=24, dnum=4, K=6, %d chains" % chains, reps=reps,
+               parity_bit_exact=parity, legs=summarize_legs(times, chains))
+    out["reference_graph_over_per_op"] = round(out["legs"]["reference_graph"]["ms_per_op"] /
+                                               out["legs"]["per_op"]["ms_per_op"], 3)
+    return out
 
 
 def run_batched(G, ctx, key, B, steps, warmup, stream):
@@ -178,16 +351,19 @@ def run_batched(G, ctx, key, B, steps, warmup, stream):
     for _ in range(warmup):
         plan.run_graph(stream)
     stream.sync()
-    e0, e1 = G.Event(), G.Event()
+    # one event between consecutive steps: the timed region is ev[0] -> ev[K],
+    # and each step's own duration is ev[i] -> ev[i + 1]
+    ev = [G.Event() for _ in range(steps + 1)]
     t0 = time.perf_counter()
-    e0.record(stream)
-    for _ in range(steps):
+    ev[0].record(stream)
+    for i in range(steps):
         plan.run_graph(stream)
-    e1.record(stream)
+        ev[i + 1].record(stream)
     stream.sync()
     host_s = time.perf_counter() - t0
-    ms = e0.elapsed_ms(e1)
-    return dict(plan=plan, ms_total=ms, host_s=host_s, bufs=(ins, outs, ws))
+    ms = ev[0].elapsed_ms(ev[-1])
+    per_step = [ev[i].elapsed_ms(ev[i + 1]) for i in range(steps)]
+    return dict(plan=plan, ms_total=ms, host_s=host_s, per_step_ms=per_step, bufs=(ins, outs, ws))
 
 
 def time_steps(G, plan, stream, reps=5):
@@ -496,6 +672,12 @@ def main():
     ms_per_step = res["ms_total"] / args.steps
     value = B * args.steps / (res["ms_total"] / 1e3)
     log("batched B=%d: %.3f ms/step, %.1f ops/s (host clock %.3f s)" % (B, ms_per_step, value, res["host_s"]))
+    steps_ms = res["per_step_ms"]
+    med = float(np.median(steps_ms))
+    ci = boot_ci(steps_ms)
+    ci95 = dict(ms_per_step_median=round(med, 4), ms_per_step_ci95=ci, ops_s_median=round(B / (med / 1e3), 1),
+                ops_s_ci95=[round(B / (ci[1] / 1e3), 1), round(B / (ci[0] / 1e3), 1)] if ci else None,
+                steps=len(steps_ms), method="median of per-step HIP event times, 4000-resample bootstrap")
 
     stages = time_steps(G, res["plan"], stream)
     np_parts = -(-L // alpha)
@@ -515,6 +697,14 @@ def main():
     dom = max(kernels, key=lambda r: r["ms"])
     achieved = dom["bytes"] / (dom["ms"] / 1e3) / 1e9
     traffic = read_traffic(dom["stage"] + ":" + dom["kernel"], B)
+    # the whole operation against HBM: compulsory bytes per EvalMult+Rescale
+    # (inputs 4L and outputs 2(L - 1) towers once, the key's 2 dnum (L + K)
+    # towers once per batch) at the measured rate, and the counter bytes the
+    # plan's kernels actually move per operation
+    b_op = 8 * N * (4 * L + 2 * (L - 1) + 2 * DNUM * (L + K) / B)
+    counter = [read_traffic(k["stage"] + ":" + k["kernel"], B) for k in kernels]
+    op_level = dict(compulsory_bytes_per_op=int(b_op), frac=round(value * b_op / (HBM_PEAK_GBS * 1e9), 4),
+                    counter_bytes_per_op=int(sum(counter) / B) if all(c is not None for c in counter) else None)
 
     perop_ops, perop_ms = run_per_op(G, ctx, key, args.perop_ops, 3, stream)
     log("per-op: %.1f ops/s (%.3f ms/op)" % (perop_ops, perop_ms))
@@ -531,8 +721,28 @@ def main():
     sweep[str(B)] = round(value, 1)
 
     del res
-    chain = rotations = None
+    chain = rotations = cfg1 = refseq = None
     if not args.no_extra:
+        cfg1 = run_cfg1(G, with_cpu=not args.no_cpu)
+        lg = cfg1["legs"]
+        log("cfg1 (n=2^15, L=4, dnum=2, 16 ops): reference sequence %.3f ms/op direct, %.3f graph; batched plan "
+            "%.3f; per-op %.3f%s; parity %s" % (lg["reference_direct"]["ms_per_op"], lg["reference_graph"]["ms_per_op"],
+                                                lg["batched_plan_graph"]["ms_per_op"], lg["per_op"]["ms_per_op"],
+                                                "; cpu %.3f ms/op" % lg["cpu"]["ms_per_op"] if "cpu" in lg else "",
+                                                cfg1["parity_bit_exact"]))
+        P3 = None if args.no_cpu else oracle_params(N, q, p, roots, DNUM)
+
+        def check3(c):
+            w0, w1 = P3.evalmult_rescale(L, *[x.download() for x in c.ins], av, bv)
+            r0, r1 = c.outputs()
+            return np.array_equal(r0, w0) and np.array_equal(r1, w1)
+
+        refseq = run_reference_sequence_cfg3(G, ctx, key, None if args.no_cpu else check3)
+        lg = refseq["legs"]
+        log("reference sequence at n=2^16 L=24 (16 chains): %.3f ms/op direct, %.3f graph; batched plan %.3f; "
+            "per-op %.3f; parity %s" % (lg["reference_direct"]["ms_per_op"], lg["reference_graph"]["ms_per_op"],
+                                        lg["batched_plan_graph"]["ms_per_op"], lg["per_op"]["ms_per_op"],
+                                        refseq["parity_bit_exact"]))
         chain = run_chain(G, stream)
         log("chain (n=2^15, L=12, dnum=3, 16 x 10 deep): %.1f ops/s" % chain["ops_s"])
         rotations = run_rotations(G, stream)
@@ -563,7 +773,7 @@ def main():
             cpu = dict(value=round(cps, 4), unit="ops/s", cores=nt, kind="port",
                        sample="%d EvalMult+Rescale ops at n=2^16 L=24 dnum=4 on %d threads (%.1f s); "
                               "1 thread: %d ops in %.1f s" % (cnt, nt, el, n1, e1),
-                       single_thread_value=round(c1, 4))
+                       single_thread_value=round(c1, 4), host=host_core_info())
             log("cpu oracle: %.4f ops/s on %d threads, %.4f ops/s on 1" % (cps, nt, c1))
         except Exception as e:  # the baseline is reported, never required
             log("cpu baseline skipped: %s" % e)
@@ -584,6 +794,8 @@ def main():
         "config": {"workload": "batched EvalMult+Rescale, n=65536, L=24 (60+23x50-bit), dnum=4, K=6 (60-bit)",
                    "batch": B, "ring_n": N, "towers": L, "dnum": DNUM, "special_primes": K,
                    "form": "hipGraph replay of one plan per step"},
+        "ci95": ci95,
+        "op_level": op_level,
         "autotune": tuned,
         "per_op_ops_s": round(perop_ops, 2),
         "batched_over_per_op": round(value / perop_ops, 3),
@@ -595,6 +807,8 @@ def main():
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algorithmic_bytes": dom["bytes"], "launch_ms": round(dom["ms"], 4)},
+        "cfg1": cfg1,
+        "reference_sequence": refseq,
         "chain": chain,
         "rotations": rotations,
         "serving": serving,

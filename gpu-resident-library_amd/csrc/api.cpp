@@ -244,20 +244,32 @@ struct KSImpl {
     gf_ctx* ctx = nullptr;
     gf_key* key = nullptr;
     std::vector<uint64_t> s1, s2;       // rescale constants of the context's top level
+    // rescale_resident_raw's scratch (gf_rescale_ws_bytes, L towers each): one
+    // allocated with the context and one more with every ks_work_create, kept
+    // in a pool; a stream takes one from the pool on its first call and keeps
+    // it, so calls on different streams never share scratch and callers that
+    // create one work set per stream never allocate in a call
     std::mutex mu;
-    std::map<hipStream_t, void*> ws;    // per-stream workspace for rescale_resident_raw
+    std::vector<void*> rs_all, rs_pool;
+    std::map<hipStream_t, void*> rs_of;
     ~KSImpl() {
-        for (auto& kv : ws) (void)gf_free(kv.second);
+        for (void* p : rs_all) (void)gf_free(p);
         if (key) gf_key_destroy(key);
         if (ctx) gf_ctx_destroy(ctx);
     }
-    uint64_t* stream_ws(hipStream_t s) {
-        std::lock_guard<std::mutex> lk(mu);
-        auto it = ws.find(s);
-        if (it != ws.end()) return (uint64_t*)it->second;
+    void add_rescale_scratch() {
         void* p = nullptr;
-        ck(gf_malloc(&p, gf_ws_bytes(ctx)), "rescale workspace");
-        ws.emplace(s, p);
+        ck(gf_malloc(&p, gf_rescale_ws_bytes(ctx)), "rescale scratch");
+        rs_all.push_back(p);
+        rs_pool.push_back(p);
+    }
+    uint64_t* rescale_scratch(hipStream_t s) {
+        auto it = rs_of.find(s);
+        if (it != rs_of.end()) return (uint64_t*)it->second;
+        if (rs_pool.empty()) add_rescale_scratch();   // more streams than work sets
+        void* p = rs_pool.back();
+        rs_pool.pop_back();
+        rs_of.emplace(s, p);
         return (uint64_t*)p;
     }
 };
@@ -317,6 +329,7 @@ DeviceKSContext ks_context_create(const KeySwitchConstants& K) {
     impl->ctx = gf_ctx_create_alpha(K.n, C.qModHost.data(), K.sizeQl, C.pModHost.data(), K.sizeP,
                                     C.rootList.data(), alpha);
     if (!impl->ctx) ck(-1, "engine context");
+    impl->add_rescale_scratch();
     if (!K.av.empty()) {
         const uint32_t rows = K.evalKeyTowers ? K.evalKeyTowers : C.fullQ + K.sizeP;
         if (K.av.size() < K.numPart || K.bv.size() < K.numPart)
@@ -337,6 +350,11 @@ DeviceKSContext ks_context_create(const KeySwitchConstants& K) {
 }
 
 DeviceKSWork ks_work_create(const DeviceKSContext& C) {
+    if (!C.impl) throw std::runtime_error("gpufhe: keyswitch context is empty or destroyed");
+    {
+        std::lock_guard<std::mutex> lk(C.impl->mu);
+        C.impl->add_rescale_scratch();   // for the stream this work set will serve
+    }
     DeviceKSWork W;
     W.bytes = gf_ws_bytes(C.engine());
     void* p = nullptr;
@@ -381,7 +399,12 @@ void rescale_resident_raw(uint64_t* d_c, uint32_t towers, const DeviceKSContext&
         !std::equal(e1.begin(), e1.begin() + (towers - 1), s1.begin()) ||
         !std::equal(e2.begin(), e2.begin() + (towers - 1), s2.begin()))
         throw std::runtime_error("rescale_resident_raw: s1/s2 are not the rescale constants of the context's moduli");
-    ck(gf_rescale_comp(C.engine(), towers, d_c, d_c, C.impl->stream_ws(s), s), "rescale_resident_raw");
+    uint64_t* scratch;
+    {
+        std::lock_guard<std::mutex> lk(C.impl->mu);
+        scratch = C.impl->rescale_scratch(s);
+    }
+    ck(gf_rescale_comp(C.engine(), towers, d_c, d_c, scratch, s), "rescale_resident_raw");
 }
 
 KeySwitchResult keyswitch_core_resident(const std::vector<uint64_t>& aTowers, const KeySwitchConstants& K) {

@@ -108,6 +108,7 @@ gf_ctx* gf_ctx_create(uint32_t n, const uint64_t* q, uint32_t L, const uint64_t*
 }
 void gf_ctx_destroy(gf_ctx* c) { delete c; }
 size_t gf_ws_bytes(const gf_ctx* c) { return c->C->ws_words() * 8; }
+size_t gf_rescale_ws_bytes(const gf_ctx* c) { return c->C->rescale_ws_words() * 8; }
 
 gf_key* gf_key_create(gf_ctx* c, const uint64_t* av, const uint64_t* bv, uint32_t nparts, hipStream_t s) {
     return guard_ptr<gf_key>([&] {
@@ -534,6 +535,25 @@ int gf_event_elapsed_ms(float* ms, hipEvent_t a, hipEvent_t b) {
         hip_check(hipEventSynchronize(b), "event sync");
         hip_check(hipEventElapsedTime(ms, a, b), "event elapsed");
     });
+}
+
+int gf_capture_begin(hipStream_t s) {
+    return guard([&] { hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture"); });
+}
+int gf_capture_end(hipStream_t s, hipGraphExec_t* out) {
+    return guard([&] {
+        hipGraph_t g = nullptr;
+        hip_check(hipStreamEndCapture(s, &g), "end capture");
+        const hipError_t e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        hip_check(e, "instantiate graph");
+    });
+}
+int gf_graph_launch(hipGraphExec_t g, hipStream_t s) {
+    return guard([&] { hip_check(hipGraphLaunch(g, s), "graph launch"); });
+}
+int gf_graph_destroy(hipGraphExec_t g) {
+    return guard([&] { hip_check(hipGraphExecDestroy(g), "graph destroy"); });
 }
 
 }  // extern "C"

@@ -249,6 +249,7 @@ WsLayout WsLayout::make(const Ctx& C) {
     return w;
 }
 size_t Ctx::ws_words() const { return WsLayout::make(*this).total * (size_t)n; }
+size_t Ctx::rescale_ws_words() const { return (size_t)L * n; }
 
 void Ctx::set_radix(uint32_t r) {
     if (r < 2 || r > 4) throw std::runtime_error("gpufhe: NTT radix log2 must be 2, 3 or 4");
@@ -937,11 +938,12 @@ void PlanBuilder::rescale(uint32_t l, const uint64_t* c0, const uint64_t* c1, ui
     rescale_comp(l, c1, o1, drop + n, sw + (size_t)(l - 1) * n);
 }
 
+// ws: L towers at least (Ctx::rescale_ws_words): the dropped tower's
+// coefficient form, then the l - 1 switched towers
 void PlanBuilder::rescale_one(uint32_t l, const uint64_t* c, uint64_t* o, uint64_t* ws) {
     const Ctx& C = C_;
     if (l < 2 || l > C.L) throw std::runtime_error("gpufhe: rescale needs at least two towers");
-    const WsLayout w = WsLayout::make(C);
-    rescale_comp(l, c, o, ws + w.drop * C.n, ws + w.dig * C.n);
+    rescale_comp(l, c, o, ws, ws + C.n);
 }
 
 void PlanBuilder::evalmult_rescale(uint32_t l, const uint64_t* c0a, const uint64_t* c1a,

@@ -103,6 +103,7 @@ public:
     uint32_t num_parts(uint32_t l) const { return (l + alpha - 1) / alpha; }
     // workspace (uint64 words) needed by one operation at the top level
     size_t ws_words() const;
+    size_t rescale_ws_words() const;   // scratch of a one-component rescale (L towers)
     // times the NTT variants on a synthetic tower batch and keeps the fastest
     // register radix in geom (plans built afterwards use it); returns it
     uint32_t autotune(hipStream_t s, float* best_ms = nullptr);
@@ -204,6 +205,9 @@ public:
     // allocation and no host synchronisation.  The arena and the staging area
     // must stay untouched until the plan's work on s has completed.
     std::unique_ptr<Plan> finish_into(void* d_arena, void* h_stage, size_t cap, hipStream_t s);
+    // the plan with its job arrays laid out in `blob` (host memory) only: the
+    // caller copies them to device memory and sets Plan::base
+    std::unique_ptr<Plan> finish_host(std::vector<char>& blob) { return layout(blob); }
 
 private:
     const Ctx& C_;

@@ -1,170 +1,16 @@
-// HIP kernels for gfx950 (CDNA4): NTT phases over job lists, the pointwise
-// RNS kernels of the multiply / keyswitch / rescale / rotate path, and the
-// reference's single-tower extern "C" launchers built on the same code.
+// HIP kernels for gfx950 (CDNA4): the pointwise RNS kernels of the multiply /
+// keyswitch / rescale / rotate path, and the reference's single-tower
+// extern "C" launchers built on the same code (the transforms are in
+// kernels_ntt_*.hip, the fused row pass + inner product in kernels_ksi.hip).
 //
 // All arithmetic is exact integer work on the VALU (64-bit products assembled
 // from 32-bit multiplies, see modarith.hpp).  Wave64, 256-thread workgroups.
 #include <hip/hip_runtime.h>
 
 #include "../../include/gpufhe_kernels.h"
-#include "internal.hpp"
+#include "ntt_kernels.hpp"
 
 namespace gf {
-
-// ------------------------------------------------------------------ NTT ---
-struct DevSync {
-    __device__ void operator()() const { __syncthreads(); }
-    // LDS words written by this wave become visible to this wave's lanes
-    __device__ void wave() const {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    }
-};
-struct JobsDev {
-    const NttJob* p;
-    uint32_t off;
-    __device__ NttJob get(uint32_t i) const { return p[off + i]; }
-};
-struct JobsOne {
-    NttJob j;
-    __device__ NttJob get(uint32_t) const { return j; }
-};
-
-// Forward jobs with j.red take the butterflies without conditional
-// subtractions (LZ); the choice is uniform per workgroup (one job each).
-template <bool FWD, class F>
-__device__ __forceinline__ void with_lz(const NttJob& j, F&& f) {
-    if constexpr (FWD) {
-        if (j.red) f(std::true_type{});
-        else f(std::false_type{});
-    } else {
-        f(std::false_type{});
-    }
-}
-
-template <bool FWD, bool PAIRED, uint32_t S1, uint32_t RM, class J>
-__global__ __launch_bounds__(NTT_THREADS) void k_ntt_A(J jobs, NttGeom g) {
-    extern __shared__ uint64_t lds[];
-    const NttJob j = jobs.get(blockIdx.y);
-    with_lz<FWD>(j, [&](auto L) {
-        phaseA_body<FWD, decltype(L)::value, PAIRED, S1, RM>(j, g, blockIdx.x, lds, threadIdx.x, blockDim.x,
-                                                             DevSync{});
-    });
-}
-template <bool FWD, bool PAIRED, uint32_t S2, uint32_t RM, class J>
-__global__ __launch_bounds__(NTT_THREADS) void k_ntt_B(J jobs, NttGeom g) {
-    extern __shared__ uint64_t lds[];
-    const NttJob j = jobs.get(blockIdx.y);
-    with_lz<FWD>(j, [&](auto L) {
-        phaseB_body<FWD, decltype(L)::value, PAIRED, S2, RM>(j, g, blockIdx.x, lds, threadIdx.x, blockDim.x,
-                                                             DevSync{});
-    });
-}
-template <bool FWD, bool PAIRED, uint32_t S, uint32_t RM, class J>
-__global__ __launch_bounds__(NTT_THREADS) void k_ntt_S(J jobs) {
-    extern __shared__ uint64_t lds[];
-    const NttJob j = jobs.get(blockIdx.y);
-    with_lz<FWD>(j, [&](auto L) {
-        single_body<FWD, decltype(L)::value, PAIRED, S, RM>(j, lds, threadIdx.x, blockDim.x, DevSync{});
-    });
-}
-
-// PAIRED: library plans use interleaved (w, Shoup precon) tables; the
-// reference ABI launchers take the two arrays separately.  g.rmax selects the
-// register radix (the tuning knob the autotuner sets).
-template <bool FWD, bool PAIRED, class J>
-static void ntt_dispatch_dir(J jobs, uint32_t njobs, const NttGeom& g, hipStream_t s, uint32_t which) {
-    const dim3 thr(NTT_THREADS);
-    bool ok = with_radix(g.rmax, [&](auto Rc) {
-        constexpr uint32_t RM = decltype(Rc)::value;
-        bool geom_ok;
-        if (g.single) {
-            geom_ok = with_single(g.logn, [&](auto Sc) {
-                constexpr uint32_t S = decltype(Sc)::value;
-                if (which & 1)
-                    k_ntt_S<FWD, PAIRED, S, RM><<<dim3(1, njobs), thr, (size_t)padded(1u << S) * 8, s>>>(jobs);
-            });
-        } else {
-            geom_ok = with_split(g.logN1, g.logN2, [&](auto A, auto B) {
-                constexpr uint32_t S1 = decltype(A)::value, S2 = decltype(B)::value;
-                const dim3 gA((1u << S2) / g.C, njobs), gB((1u << S1) / g.R, njobs);
-                const size_t shA = (size_t)NTT_TILE * 8, shB = (size_t)NTT_TILE_PADDED * 8;
-                // which: bit 0 = first kernel of the transform, bit 1 = second
-                if (FWD) {
-                    if (which & 1) k_ntt_A<FWD, PAIRED, S1, RM><<<gA, thr, shA, s>>>(jobs, g);
-                    if (which & 2) k_ntt_B<FWD, PAIRED, S2, RM><<<gB, thr, shB, s>>>(jobs, g);
-                } else {
-                    if (which & 1) k_ntt_B<FWD, PAIRED, S2, RM><<<gB, thr, shB, s>>>(jobs, g);
-                    if (which & 2) k_ntt_A<FWD, PAIRED, S1, RM><<<gA, thr, shA, s>>>(jobs, g);
-                }
-            });
-        }
-        if (!geom_ok) throw std::runtime_error("gpufhe: unsupported ring dimension for the NTT");
-    });
-    if (!ok) throw std::runtime_error("gpufhe: unsupported NTT radix");
-}
-
-template <bool PAIRED, class J>
-static void ntt_dispatch(J jobs, uint32_t njobs, const NttGeom& g, bool fwd, hipStream_t s,
-                         uint32_t which = 3) {
-    if (fwd) ntt_dispatch_dir<true, PAIRED>(jobs, njobs, g, s, which);
-    else ntt_dispatch_dir<false, PAIRED>(jobs, njobs, g, s, which);
-}
-
-static const uint32_t MAXY = 65535;
-
-// ------------------------------------------- fused digit row pass + inner --
-// One wave's LDS region is private to it during the row passes.
-struct WaveSync {
-    __device__ void operator()() const { DevSync{}.wave(); }
-    __device__ void wave() const { DevSync{}.wave(); }
-};
-
-// At most 96 VGPRs (5 waves per SIMD; 82 without spills): a 3-wave workgroup
-// of the Q rows takes 26 KiB of LDS, so 6 of them fit a CU.
-template <uint32_t S2, uint32_t RM>
-__global__ __launch_bounds__(KSI_THREADS) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_ksi(const KsiJob* __restrict__ jobs, uint32_t off, uint32_t N1) {
-    extern __shared__ uint64_t lds[];
-    const KsiJob& j = jobs[off + blockIdx.y];
-    const uint32_t tid = threadIdx.x, nthr = blockDim.x;
-    auto go = [&](auto L) {
-        ksi_body<S2, RM, decltype(L)::value>(j, N1, blockIdx.x, lds, tid >> 6, nthr >> 6, tid & 63, 64, tid, nthr,
-                                             WaveSync{}, DevSync{});
-    };
-    if (j.red) go(std::true_type{});
-    else go(std::false_type{});
-}
-
-void launch_ksi(const KsiJob* d, uint32_t nj, const NttGeom& g, uint32_t regions, uint32_t waves, hipStream_t s) {
-    if (g.single || g.n % KSI_TILE) throw std::runtime_error("gpufhe: fused inner product needs a two-phase ring");
-    if (regions > KSI_MAXP || waves < 1 || waves > KSI_WAVES)
-        throw std::runtime_error("gpufhe: fused inner product launch shape out of range");
-    const size_t shm = (size_t)regions * KSI_PART_WORDS * 8;
-    bool ok = with_radix(g.rmax, [&](auto Rc) {
-        constexpr uint32_t RM = decltype(Rc)::value;
-        const bool geom_ok = with_split(g.logN1, g.logN2, [&](auto, auto B) {
-            constexpr uint32_t S2 = decltype(B)::value;
-            for (uint32_t off = 0; off < nj; off += MAXY) {
-                const uint32_t cnt = nj - off < MAXY ? nj - off : MAXY;
-                k_ksi<S2, RM><<<dim3(g.n / KSI_TILE, cnt), 64 * waves, shm, s>>>(d, off, 1u << g.logN1);
-            }
-        });
-        if (!geom_ok) throw std::runtime_error("gpufhe: unsupported ring dimension for the NTT");
-    });
-    if (!ok) throw std::runtime_error("gpufhe: unsupported NTT radix");
-}
-
-void launch_ntt(const NttJob* d_jobs, uint32_t njobs, const NttGeom& g, bool fwd, hipStream_t s,
-                uint32_t which) {
-    for (uint32_t off = 0; off < njobs; off += MAXY) {
-        uint32_t cnt = njobs - off < MAXY ? njobs - off : MAXY;
-        ntt_dispatch<true>(JobsDev{d_jobs, off}, cnt, g, fwd, s, which);
-    }
-}
-void launch_ntt_one(const NttJob& job, const NttGeom& g, bool fwd, hipStream_t s) {
-    ntt_dispatch<false>(JobsOne{job}, 1, g, fwd, s);
-}
 
 // ------------------------------------------------------ pointwise kernels --
 #define PW_FOR(n)                                                                      \
@@ -204,9 +50,7 @@ __device__ __forceinline__ void tensor_one(const TensorJob& j, uint64_t a0, uint
     t2 = mulmod(a1, b1, j.q, j.qneg, j.r2);
 }
 
-__global__ __launch_bounds__(PW_THREADS) void k_tensor(const TensorJob* __restrict__ jobs,
-                                                       uint32_t off, uint32_t n) {
-    const TensorJob j = jobs[off + blockIdx.y];
+__device__ __forceinline__ void tensor_body(const TensorJob& j, uint32_t n) {
     if (j.pairs) {
         PW2_FOR(n) {
             const Pair a0 = ld2(j.c0a, i), a1 = ld2(j.c1a, i), b0 = ld2(j.c0b, i), b1 = ld2(j.c1b, i);
@@ -226,6 +70,11 @@ __global__ __launch_bounds__(PW_THREADS) void k_tensor(const TensorJob* __restri
             ga(j.t2, i) = t2;
         }
     }
+}
+
+__global__ __launch_bounds__(PW_THREADS) void k_tensor(const TensorJob* __restrict__ jobs,
+                                                       uint32_t off, uint32_t n) {
+    tensor_body(jobs[off + blockIdx.y], n);
 }
 
 // Tables that every lane of a block reads at the same index (conversion
@@ -600,16 +449,41 @@ void launch_perm_one(const PermJob& j, uint32_t n, uint32_t logn, hipStream_t s)
 }
 
 // ------------------------------------------- reference single-tower ABI ---
-// Montgomery constants of q derived on the device (no host precompute
-// available through these signatures): qneg by Newton, 2^128 mod q by doubling.
-__device__ __forceinline__ void dev_mont(uint64_t q, uint64_t& qneg, uint64_t& r2) {
+// Montgomery constants of an odd modulus q < 2^63, derived on the device (the
+// reference's launcher signatures pass the moduli in device memory only):
+// qneg = -q^-1 mod 2^64 by Newton's iteration (x = q is q^-1 to 3 bits, as
+// q^2 = 1 mod 8, and each step doubles the correct bits: 6 steps give 64+);
+// r2 = 2^128 mod q as REDC(r * r) with r = 2^96 mod q, itself 2^b mod q
+// (2^(b-1) <= q < 2^b, so 2^b - q) doubled 96 - b times.
+__device__ __forceinline__ void mont_consts(uint64_t q, uint64_t& qneg, uint64_t& r2) {
     uint64_t x = q;
     for (int k = 0; k < 6; ++k) x *= 2 - q * x;
     qneg = (uint64_t)0 - x;
-    uint64_t r = 1;
-    for (int k = 0; k < 128; ++k) r = addmod(r, r, q);
-    r2 = r;
+    const uint32_t b = 64 - (uint32_t)__builtin_clzll(q);
+    uint64_t r = (1ull << b) - q;
+    for (uint32_t k = b; k < 96; ++k) r = csub(r + r, q);
+    r2 = redc(mul128(r, r), q, qneg);
 }
+
+// The modulus of this workgroup's tower and its Montgomery constants, derived
+// once per workgroup by one lane and shared through LDS.
+struct WgMont {
+    uint64_t q, qneg, r2;
+};
+__device__ __forceinline__ WgMont wg_mont(const uint64_t* mods, uint32_t t) {
+    __shared__ uint64_t sh[3];
+    if (threadIdx.x == 0) {
+        const uint64_t q = mods[t];
+        uint64_t qneg, r2;
+        mont_consts(q, qneg, r2);
+        sh[0] = q;
+        sh[1] = qneg;
+        sh[2] = r2;
+    }
+    __syncthreads();
+    return WgMont{sh[0], sh[1], sh[2]};
+}
+__device__ __forceinline__ bool al16(const void* p) { return ((uintptr_t)p & 15) == 0; }
 
 // exact 128-bit remainder with mu = floor((2^128-1)/p): Barrett quotient from
 // the top limbs of a*mu, error at most a few p, removed by the final loop
@@ -626,6 +500,18 @@ __device__ __forceinline__ uint64_t barrett128(U128 a, uint64_t p, uint64_t mu_l
     return r;
 }
 
+__device__ uint64_t abi_conv_barrett(const uint64_t* src, const uint64_t* qhi, const uint64_t* qhip,
+                                     const uint64_t* q, const uint64_t* qhmp, uint64_t p, uint64_t mlo,
+                                     uint64_t mhi, uint32_t sizeQ, uint32_t sizeP, uint32_t t, uint32_t n,
+                                     uint32_t r) {
+    U128 acc{0, 0};
+    for (uint32_t i = 0; i < sizeQ; ++i) {
+        const uint64_t xi = shoup(src[(uint64_t)i * n + r], qhi[i], qhip[i], q[i]);
+        acc = add128(acc, mul128(xi, qhmp[(uint64_t)i * sizeP + t]));
+    }
+    return barrett128(acc, p, mlo, mhi);
+}
+
 __global__ void k_ref_mult(const uint64_t* a, const uint64_t* b, uint64_t* r, uint64_t q,
                            uint64_t qneg, uint64_t r2, uint32_t n) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -640,31 +526,101 @@ __global__ void k_ref_fuse(uint64_t* a, const uint64_t* b, uint64_t s1, uint64_t
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) a[i] = addmod(shoup(a[i], s1, s1p, q), shoup(b[i], s2, s2p, q), q);
 }
-__global__ void k_ref_approx_switch(const uint64_t* __restrict__ src, uint64_t* __restrict__ dst,
-                                    const uint64_t* __restrict__ qhi, const uint64_t* __restrict__ qhip,
-                                    const uint64_t* __restrict__ q, const uint64_t* __restrict__ qhmp,
-                                    const uint64_t* __restrict__ p, const uint64_t* __restrict__ mlo,
-                                    const uint64_t* __restrict__ mhi, uint32_t sizeQ, uint32_t sizeP,
-                                    uint32_t n) {
-    uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
-    if (sizeQ <= CONV_MAX_SRC) {
-        uint64_t x[CONV_MAX_SRC];
-        for (uint32_t i = 0; i < sizeQ; ++i) x[i] = shoup(src[(uint64_t)i * n + r], qhi[i], qhip[i], q[i]);
+// LaunchApproxSwitchCRTBasis (kernels/basis_convert.cu): each workgroup first
+// converts the QHatModp table to Montgomery form in LDS (one REDC of x * r2 per
+// entry, with each target's constants derived on the device), then every lane
+// converts PW_EPT coefficients: sources x_i = src_i * QHatInv_i mod q_i (Shoup
+// with the caller's precons), then per target one exact sum and one REDC.  The
+// sum of sizeQ products x_i * tab < sizeQ * q_max * p_t stays below p_t 2^64
+// when sizeQ * q_max < 2^64, so the REDC is exact after one subtraction; other
+// shapes (and more than ABI_CONV_MAXP targets) keep the Barrett form with the
+// caller's mu, the reference's arithmetic.
+enum { ABI_CONV_MAXP = 64 };
+__device__ uint64_t abi_conv_barrett(const uint64_t* src, const uint64_t* qhi, const uint64_t* qhip,
+                                     const uint64_t* q, const uint64_t* qhmp, uint64_t p, uint64_t mlo,
+                                     uint64_t mhi, uint32_t sizeQ, uint32_t sizeP, uint32_t t, uint32_t n,
+                                     uint32_t r);
+template <uint32_t NS>
+__device__ __forceinline__ void abi_conv_elem(const uint64_t* __restrict__ src, uint64_t* __restrict__ dst,
+                                              const uint64_t* qhi, const uint64_t* qhip, const uint64_t* q,
+                                              const uint64_t* tab, const uint64_t* pq, const uint64_t* pn,
+                                              uint32_t sizeQ, uint32_t sizeP, uint32_t n, uint32_t r) {
+    if constexpr (NS > 0) {
+        uint32_t lo[NS], hi[NS];
+#pragma unroll
+        for (uint32_t i = 0; i < NS; ++i) {
+            const uint64_t x = shoup(ga(src, i * n + r), qhi[i], qhip[i], q[i]);
+            lo[i] = split30_lo(x);
+            hi[i] = split30_hi(x);
+        }
+        for (uint32_t t = 0; t < sizeP; ++t) {
+            const U128 acc = dot30<NS>(lo, hi, [&](uint32_t i) { return tab[i * sizeP + t]; });
+            ga(dst, t * n + r) = redc(acc, pq[t], pn[t]);
+        }
+    } else {
         for (uint32_t t = 0; t < sizeP; ++t) {
             U128 acc{0, 0};
-            for (uint32_t i = 0; i < sizeQ; ++i) acc = add128(acc, mul128(x[i], qhmp[(uint64_t)i * sizeP + t]));
-            dst[(uint64_t)t * n + r] = barrett128(acc, p[t], mlo[t], mhi[t]);
+            for (uint32_t i = 0; i < sizeQ; ++i)
+                acc = add128(acc, mul128(shoup(ga(src, i * n + r), qhi[i], qhip[i], q[i]), tab[i * sizeP + t]));
+            ga(dst, t * n + r) = redc(acc, pq[t], pn[t]);
+        }
+    }
+}
+__global__ __launch_bounds__(PW_THREADS) void k_ref_approx_switch(
+    const uint64_t* __restrict__ src, uint64_t* __restrict__ dst, const uint64_t* __restrict__ qhi,
+    const uint64_t* __restrict__ qhip, const uint64_t* __restrict__ q, const uint64_t* __restrict__ qhmp,
+    const uint64_t* __restrict__ p, const uint64_t* __restrict__ mlo, const uint64_t* __restrict__ mhi,
+    uint32_t sizeQ, uint32_t sizeP, uint32_t n) {
+    __shared__ uint64_t s_tab[CONV_MAX_SRC * ABI_CONV_MAXP], s_pq[ABI_CONV_MAXP], s_pn[ABI_CONV_MAXP],
+        s_r2[ABI_CONV_MAXP], s_src[3 * CONV_MAX_SRC];
+    __shared__ uint32_t s_mode;   // 0: Barrett fallback, 1: generic REDC sums, 2: dot30 column sums
+    const uint32_t tid = threadIdx.x;
+    if (tid == 0) {
+        uint64_t qmax = 0, all = 0;
+        for (uint32_t i = 0; i < sizeQ; ++i) {
+            qmax = q[i] > qmax ? q[i] : qmax;
+            all |= q[i];
+        }
+        for (uint32_t t = 0; t < sizeP && t < ABI_CONV_MAXP; ++t) all |= p[t];
+        const bool fits = sizeQ >= 1 && sizeQ <= CONV_MAX_SRC && sizeP <= ABI_CONV_MAXP &&
+                          qmax < ~(uint64_t)0 / sizeQ;
+        s_mode = !fits ? 0u : (sizeQ <= CONV_UNROLLED && (all >> 60) == 0) ? 2u : 1u;
+    }
+    __syncthreads();
+    const uint32_t mode = s_mode;
+    if (mode == 0) {
+        PW_FOR(n) {
+            for (uint32_t t = 0; t < sizeP; ++t)
+                ga(dst, t * n + i) = abi_conv_barrett(src, qhi, qhip, q, qhmp, p[t], mlo[t], mhi[t], sizeQ, sizeP, t, n, i);
         }
         return;
     }
-    for (uint32_t t = 0; t < sizeP; ++t) {   // many sources: recompute instead of caching
-        U128 acc{0, 0};
-        for (uint32_t i = 0; i < sizeQ; ++i) {
-            const uint64_t xi = shoup(src[(uint64_t)i * n + r], qhi[i], qhip[i], q[i]);
-            acc = add128(acc, mul128(xi, qhmp[(uint64_t)i * sizeP + t]));
-        }
-        dst[(uint64_t)t * n + r] = barrett128(acc, p[t], mlo[t], mhi[t]);
+    for (uint32_t t = tid; t < sizeP; t += PW_THREADS) {
+        uint64_t qn, r2;
+        mont_consts(p[t], qn, r2);
+        s_pq[t] = p[t];
+        s_pn[t] = qn;
+        s_r2[t] = r2;
+    }
+    for (uint32_t i = tid; i < sizeQ; i += PW_THREADS) {
+        s_src[i] = q[i];
+        s_src[CONV_MAX_SRC + i] = qhi[i];
+        s_src[2 * CONV_MAX_SRC + i] = qhip[i];
+    }
+    __syncthreads();
+    for (uint32_t e = tid; e < sizeQ * sizeP; e += PW_THREADS) {
+        const uint32_t t = e % sizeP;
+        s_tab[e] = redc(mul128(qhmp[e], s_r2[t]), s_pq[t], s_pn[t]);   // x R mod p_t
+    }
+    __syncthreads();
+    const uint64_t *sq = s_src, *sh = s_src + CONV_MAX_SRC, *shp = s_src + 2 * CONV_MAX_SRC;
+    if (mode == 2) {
+        with_nsrc(sizeQ, [&](auto NSc) {
+            constexpr uint32_t NS = decltype(NSc)::value;
+            PW_FOR(n) { abi_conv_elem<NS>(src, dst, sh, shp, sq, s_tab, s_pq, s_pn, sizeQ, sizeP, n, i); }
+        });
+    } else {
+        PW_FOR(n) { abi_conv_elem<0>(src, dst, sh, shp, sq, s_tab, s_pq, s_pn, sizeQ, sizeP, n, i); }
     }
 }
 __global__ void k_ref_ksmacc(uint64_t* o0, uint64_t* o1, const uint64_t* d, const uint64_t* b,
@@ -674,19 +630,36 @@ __global__ void k_ref_ksmacc(uint64_t* o0, uint64_t* o1, const uint64_t* d, cons
     o0[k] = addmod(o0[k], mulmod(d[k], b[k], q, qneg, r2), q);
     o1[k] = addmod(o1[k], mulmod(d[k], a[k], q, qneg, r2), q);
 }
-__global__ void k_ref_ksmacc_batched(uint64_t* o0, uint64_t* o1, const uint64_t* const* digits,
-                                     const uint64_t* bv, const uint64_t* av, const uint32_t* keyRow,
-                                     const uint64_t* mods, uint32_t n) {
-    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
+// LaunchKSMultAccBatched (kernels/ks_fastcore.cu:71-81): tower blockIdx.y,
+// PW_EPT positions per lane, the tower's Montgomery constants derived once per
+// workgroup; two positions per 16-byte access when every array is aligned.
+__global__ __launch_bounds__(PW_THREADS) void k_ref_ksmacc_batched(uint64_t* o0, uint64_t* o1,
+                                                                   const uint64_t* const* digits,
+                                                                   const uint64_t* bv, const uint64_t* av,
+                                                                   const uint32_t* keyRow,
+                                                                   const uint64_t* mods, uint32_t n) {
     const uint32_t t = blockIdx.y;
-    const uint64_t q = mods[t];
-    uint64_t qneg, r2;
-    dev_mont(q, qneg, r2);
-    const uint64_t o = (uint64_t)t * n + k, kb = (uint64_t)keyRow[t] * n + k;
-    const uint64_t d = digits[t][k];
-    o0[o] = addmod(o0[o], mulmod(d, bv[kb], q, qneg, r2), q);
-    o1[o] = addmod(o1[o], mulmod(d, av[kb], q, qneg, r2), q);
+    const WgMont m = wg_mont(mods, t);
+    const uint64_t* d = digits[t];
+    const size_t ko = (size_t)keyRow[t] * n, to = (size_t)t * n;
+    uint64_t *r0 = o0 + to, *r1 = o1 + to;
+    const uint64_t *b = bv + ko, *a = av + ko;
+    auto one = [&](uint64_t x0, uint64_t x1, uint64_t dv, uint64_t kb, uint64_t ka, uint64_t& y0, uint64_t& y1) {
+        y0 = addmod(x0, mulmod(dv, kb, m.q, m.qneg, m.r2), m.q);
+        y1 = addmod(x1, mulmod(dv, ka, m.q, m.qneg, m.r2), m.q);
+    };
+    if ((n & 1) == 0 && al16(d) && al16(b) && al16(a) && al16(r0) && al16(r1)) {
+        PW2_FOR(n) {
+            const Pair x0 = ld2(r0, i), x1 = ld2(r1, i), dv = ld2(d, i), kb = ld2(b, i), ka = ld2(a, i);
+            Pair y0, y1;
+#pragma unroll
+            for (int k = 0; k < 2; ++k) one(x0.v[k], x1.v[k], dv.v[k], kb.v[k], ka.v[k], y0.v[k], y1.v[k]);
+            st2(r0, i, y0);
+            st2(r1, i, y1);
+        }
+    } else {
+        PW_FOR(n) { one(ga(r0, i), ga(r1, i), ga(d, i), ga(b, i), ga(a, i), ga(r0, i), ga(r1, i)); }
+    }
 }
 __global__ void k_ref_moddown(uint64_t* ans, const uint64_t* a, const uint64_t* b, uint64_t s,
                               uint64_t sp, uint64_t q, uint32_t n) {
@@ -713,29 +686,41 @@ __global__ void k_ref_ptmulacc(uint64_t* acc0, uint64_t* acc1, const uint64_t* b
     acc0[k] = addmod(acc0[k], mulmod(b0[k], d[k], q, qneg, r2), q);
     acc1[k] = addmod(acc1[k], mulmod(b1[k], d[k], q, qneg, r2), q);
 }
-__global__ void k_ref_tensor(const uint64_t* c0a, const uint64_t* c1a, const uint64_t* c0b,
-                             const uint64_t* c1b, uint64_t* t0, uint64_t* t1, uint64_t* t2,
-                             const uint64_t* mods, uint32_t n) {
-    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const uint64_t q = mods[blockIdx.y];
-    uint64_t qneg, r2;
-    dev_mont(q, qneg, r2);
-    const uint64_t x = (uint64_t)blockIdx.y * n + k;
-    const uint64_t a0 = c0a[x], a1 = c1a[x], b0 = c0b[x], b1 = c1b[x];
-    t0[x] = mulmod(a0, b0, q, qneg, r2);
-    t1[x] = addmod(mulmod(a0, b1, q, qneg, r2), mulmod(a1, b0, q, qneg, r2), q);
-    t2[x] = mulmod(a1, b1, q, qneg, r2);
+// LaunchTensor (kernels/tensor.cu:28-51): the plan's tensor kernel body on
+// tower blockIdx.y, with that tower's Montgomery constants derived once per
+// workgroup.
+__global__ __launch_bounds__(PW_THREADS) void k_ref_tensor(const uint64_t* c0a, const uint64_t* c1a,
+                                                           const uint64_t* c0b, const uint64_t* c1b, uint64_t* t0,
+                                                           uint64_t* t1, uint64_t* t2, const uint64_t* mods,
+                                                           uint32_t n) {
+    const WgMont m = wg_mont(mods, blockIdx.y);
+    const size_t o = (size_t)blockIdx.y * n;
+    TensorJob j{c0a + o, c1a + o, c0b + o, c1b + o, t0 + o, t1 + o, t2 + o, m.q, m.qneg, m.r2, 0};
+    j.pairs = (n & 1) == 0 && al16(j.c0a) && al16(j.c1a) && al16(j.c0b) && al16(j.c1b) && al16(j.t0) &&
+              al16(j.t1) && al16(j.t2);
+    tensor_body(j, n);
 }
-__global__ void k_ref_combine(uint64_t* r0, uint64_t* r1, const uint64_t* t0, const uint64_t* t1,
-                              const uint64_t* ba0, const uint64_t* ba1, const uint64_t* mods,
-                              uint32_t n) {
-    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
+// LaunchCombine (kernels/tensor.cu:53-61)
+__global__ __launch_bounds__(PW_THREADS) void k_ref_combine(uint64_t* r0, uint64_t* r1, const uint64_t* t0,
+                                                            const uint64_t* t1, const uint64_t* ba0,
+                                                            const uint64_t* ba1, const uint64_t* mods,
+                                                            uint32_t n) {
     const uint64_t q = mods[blockIdx.y];
-    const uint64_t x = (uint64_t)blockIdx.y * n + k;
-    r0[x] = addmod(t0[x], ba0[x], q);
-    r1[x] = addmod(t1[x], ba1[x], q);
+    const size_t o = (size_t)blockIdx.y * n;
+    uint64_t *x0 = r0 + o, *x1 = r1 + o;
+    const uint64_t *u0 = t0 + o, *u1 = t1 + o, *b0 = ba0 + o, *b1 = ba1 + o;
+    if ((n & 1) == 0 && al16(x0) && al16(x1) && al16(u0) && al16(u1) && al16(b0) && al16(b1)) {
+        PW2_FOR(n) {
+            const Pair a = ld2(u0, i), b = ld2(b0, i), c = ld2(u1, i), d = ld2(b1, i);
+            st2(x0, i, Pair{{addmod(a.v[0], b.v[0], q), addmod(a.v[1], b.v[1], q)}});
+            st2(x1, i, Pair{{addmod(c.v[0], d.v[0], q), addmod(c.v[1], d.v[1], q)}});
+        }
+    } else {
+        PW_FOR(n) {
+            ga(x0, i) = addmod(ga(u0, i), ga(b0, i), q);
+            ga(x1, i) = addmod(ga(u1, i), ga(b1, i), q);
+        }
+    }
 }
 
 }  // namespace gf
@@ -778,7 +763,7 @@ void LaunchApproxSwitchCRTBasis(const uint64_t* d_src, uint64_t* d_dst, const ui
                                 const uint64_t* d_QHatModp, const uint64_t* d_p, const uint64_t* d_mu_lo,
                                 const uint64_t* d_mu_hi, uint32_t sizeQ, uint32_t sizeP, uint32_t n,
                                 hipStream_t s) {
-    k_ref_approx_switch<<<blocks256(n), 256, 0, s>>>(d_src, d_dst, d_QHatInvModq, d_QHatInvModqPrecon,
+    k_ref_approx_switch<<<pw_grid(n, 1), PW_THREADS, 0, s>>>(d_src, d_dst, d_QHatInvModq, d_QHatInvModqPrecon,
                                                      d_q, d_QHatModp, d_p, d_mu_lo, d_mu_hi, sizeQ,
                                                      sizeP, n);
 }
@@ -792,7 +777,7 @@ void LaunchKSMultAcc(uint64_t* d_out0, uint64_t* d_out1, const uint64_t* d_digit
 void LaunchKSMultAccBatched(uint64_t* d_out0, uint64_t* d_out1, const uint64_t* const* d_digits,
                             const uint64_t* d_bv, const uint64_t* d_av, const uint32_t* d_keyRow,
                             const uint64_t* d_mods, uint32_t towers, uint32_t n, hipStream_t s) {
-    k_ref_ksmacc_batched<<<dim3(blocks256(n), towers), 256, 0, s>>>(d_out0, d_out1, d_digits, d_bv,
+    if (towers) k_ref_ksmacc_batched<<<pw_grid(n, towers), PW_THREADS, 0, s>>>(d_out0, d_out1, d_digits, d_bv,
                                                                     d_av, d_keyRow, d_mods, n);
 }
 
@@ -819,12 +804,12 @@ void LaunchPtMulAcc(uint64_t* d_acc0, uint64_t* d_acc1, const uint64_t* d_b0, co
 void LaunchTensor(const uint64_t* c0a, const uint64_t* c1a, const uint64_t* c0b, const uint64_t* c1b,
                   uint64_t* t0, uint64_t* t1, uint64_t* t2, const uint64_t* d_mods, uint32_t sizeQ,
                   uint32_t n, hipStream_t s) {
-    k_ref_tensor<<<dim3(blocks256(n), sizeQ), 256, 0, s>>>(c0a, c1a, c0b, c1b, t0, t1, t2, d_mods, n);
+    if (sizeQ) k_ref_tensor<<<pw_grid(n, sizeQ), PW_THREADS, 0, s>>>(c0a, c1a, c0b, c1b, t0, t1, t2, d_mods, n);
 }
 
 void LaunchCombine(uint64_t* r0, uint64_t* r1, const uint64_t* t0, const uint64_t* t1, const uint64_t* ba0,
                    const uint64_t* ba1, const uint64_t* d_mods, uint32_t sizeQ, uint32_t n, hipStream_t s) {
-    k_ref_combine<<<dim3(blocks256(n), sizeQ), 256, 0, s>>>(r0, r1, t0, t1, ba0, ba1, d_mods, n);
+    if (sizeQ) k_ref_combine<<<pw_grid(n, sizeQ), PW_THREADS, 0, s>>>(r0, r1, t0, t1, ba0, ba1, d_mods, n);
 }
 
 }  // extern "C"

@@ -22,6 +22,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstring>
 #include <deque>
 #include <thread>
 
@@ -133,6 +134,30 @@ struct gf_server {
         return (uint64_t*)p;
     }
 
+    // the lane's job arena and staging area hold at least `bytes` (the lane is
+    // idle: its last batch, and the copy that staged it, have completed)
+    void fit_lane(SLane& l, size_t bytes) {
+        if (bytes <= l.cap) return;
+        size_t cap = l.cap;
+        while (cap < bytes) cap *= 2;
+        void *arena = nullptr, *stage = nullptr;
+        hip_check(hipMalloc(&arena, cap), "lane job arena");
+        if (hipHostMalloc(&stage, cap, hipHostMallocDefault) != hipSuccess) {
+            (void)hipFree(arena);
+            throw std::runtime_error("gpufhe: lane staging: out of pinned host memory");
+        }
+        (void)hipFree(l.arena);
+        (void)hipHostFree(l.stage);
+        l.arena = arena;
+        l.stage = stage;
+        l.cap = cap;
+    }
+
+    // Admits waiting requests into every idle lane.  A lane's batch is
+    // committed (requests leave the queue, the lane turns busy) only once its
+    // plan has been staged and launched; on any failure before that the
+    // requests stay queued in order, their slots return to the pool, and the
+    // error propagates.
     uint32_t admit() {
         uint32_t launched = 0;
         for (auto& l : lanes) {
@@ -141,36 +166,54 @@ struct gf_server {
             SRing& R = rings[ring];
             PlanBuilder b(*R.ctx->C);
             const Ctx& C = *R.ctx->C;
-            for (auto it = queue.begin(); it != queue.end() && l.reqs.size() < max_batch;) {
-                if (it->r.ring != ring) {
-                    ++it;
-                    continue;
+            std::vector<size_t> pick;
+            std::vector<uint64_t*> slots;
+            std::unique_ptr<Plan> plan;
+            try {
+                size_t idx = 0;
+                for (auto it = queue.begin(); it != queue.end() && pick.size() < max_batch; ++it, ++idx) {
+                    if (it->r.ring != ring) continue;
+                    uint64_t* slot = take_slot(R);
+                    if (!slot) break;   // footprint: wait for slots to come back
+                    slots.push_back(slot);
+                    const gf_request& r = it->r;
+                    uint64_t* o0 = r.o0 ? r.o0 : slot + R.ws_words;
+                    uint64_t* o1 = r.o1 ? r.o1 : slot + R.ws_words + (size_t)C.L * C.n;
+                    switch (r.op) {
+                        case GF_OP_EVALMULT_RESCALE:
+                            b.evalmult_rescale(r.level, r.a0, r.a1, r.b0, r.b1, o0, o1, *R.rlk->K, slot);
+                            break;
+                        case GF_OP_ROTATE:
+                            b.rotate(r.level, R.rot_k[r.rot], r.a0, r.a1, o0, o1, *R.rot[r.rot]->K, slot);
+                            break;
+                        default:
+                            b.add(r.level, r.a0, r.a1, r.b0, r.b1, o0, o1);
+                            break;
+                    }
+                    pick.push_back(idx);
                 }
-                uint64_t* slot = take_slot(R);
-                if (!slot) break;   // footprint: wait for slots to come back
-                gf_request& r = it->r;
-                uint64_t* o0 = r.o0 ? r.o0 : slot + R.ws_words;
-                uint64_t* o1 = r.o1 ? r.o1 : slot + R.ws_words + (size_t)C.L * C.n;
-                switch (r.op) {
-                    case GF_OP_EVALMULT_RESCALE:
-                        b.evalmult_rescale(r.level, r.a0, r.a1, r.b0, r.b1, o0, o1, *R.rlk->K, slot);
-                        break;
-                    case GF_OP_ROTATE:
-                        b.rotate(r.level, R.rot_k[r.rot], r.a0, r.a1, o0, o1, *R.rot[r.rot]->K, slot);
-                        break;
-                    default:
-                        b.add(r.level, r.a0, r.a1, r.b0, r.b1, o0, o1);
-                        break;
+                if (pick.empty()) continue;
+                std::vector<char> blob;
+                plan = b.finish_host(blob);
+                fit_lane(l, blob.size());
+                if (!blob.empty()) {
+                    std::memcpy(l.stage, blob.data(), blob.size());
+                    hip_check(hipMemcpyAsync(l.arena, l.stage, blob.size(), hipMemcpyHostToDevice, l.s), "stage plan");
                 }
-                l.reqs.push_back(*it);
-                l.slots.push_back(slot);
-                it = queue.erase(it);
+                plan->base = l.arena;
+                plan->run(l.s);
+                hip_check(hipEventRecord(l.done, l.s), "completion event");
+                hip_check(hipGetLastError(), "batch launch");
+            } catch (...) {
+                for (uint64_t* s : slots) R.free_slots.push_back(s);
+                throw;
             }
-            if (l.reqs.empty()) continue;
-            l.plan = b.finish_into(l.arena, l.stage, l.cap, l.s);
-            l.plan->run(l.s);
-            hip_check(hipEventRecord(l.done, l.s), "completion event");
-            hip_check(hipGetLastError(), "batch launch");
+            // committed: move the picked requests (in queue order) to the lane
+            l.reqs.clear();
+            for (size_t i : pick) l.reqs.push_back(queue[i]);
+            for (size_t k = pick.size(); k-- > 0;) queue.erase(queue.begin() + (std::ptrdiff_t)pick[k]);
+            l.slots = std::move(slots);
+            l.plan = std::move(plan);
             l.ring = ring;
             l.launch_ms = now_ms();
             l.busy = true;
@@ -216,7 +259,7 @@ extern "C" {
 gf_server* gf_server_create(size_t ws_budget_bytes, uint32_t lanes, uint32_t max_batch) {
     return guard_ptr<gf_server>([&] {
         if (lanes == 0 || lanes > 8) throw std::runtime_error("gpufhe: lanes must be in [1, 8]");
-        if (max_batch == 0) throw std::runtime_error("gpufhe: max_batch must be positive");
+        if (max_batch == 0 || max_batch > 4096) throw std::runtime_error("gpufhe: max_batch must be in [1, 4096]");
         auto S = std::make_unique<gf_server>();
         S->budget = ws_budget_bytes;
         S->max_batch = max_batch;
@@ -224,7 +267,7 @@ gf_server* gf_server_create(size_t ws_budget_bytes, uint32_t lanes, uint32_t max
         for (auto& l : S->lanes) {
             hip_check(hipStreamCreateWithFlags(&l.s, hipStreamNonBlocking), "lane stream");
             hip_check(hipEventCreateWithFlags(&l.done, hipEventDefault), "lane event");
-            l.cap = (size_t)32 << 20;
+            l.cap = (size_t)32 << 20;   // grows (fit_lane) if a batch's job arrays need more
             hip_check(hipMalloc(&l.arena, l.cap), "lane job arena");
             hip_check(hipHostMalloc(&l.stage, l.cap, hipHostMallocDefault), "lane staging");
         }

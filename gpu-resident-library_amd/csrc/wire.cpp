@@ -47,6 +47,11 @@ uint64_t fnv1a(const unsigned char* p, size_t len) {
     }
     return h;
 }
+// bounds a header must meet before any length is computed from it
+constexpr uint32_t WIRE_MAX_N = 1u << 17, WIRE_MAX_TOWERS = 4096;
+bool ct_shape_ok(uint32_t n, uint32_t towers) {
+    return n >= 2 && n <= WIRE_MAX_N && (n & (n - 1)) == 0 && towers >= 1 && towers <= WIRE_MAX_TOWERS;
+}
 size_t ct_words(uint32_t n, uint32_t towers) { return (size_t)towers + 2 * (size_t)towers * n; }
 // every residue of tower t below its modulus
 void check_residues(const uint64_t* v, const uint64_t* mods, uint32_t towers, uint32_t n, size_t stride_towers) {
@@ -61,12 +66,15 @@ void check_residues(const uint64_t* v, const uint64_t* mods, uint32_t towers, ui
 
 extern "C" {
 
-size_t gf_ct_packed_bytes(uint32_t n, uint32_t towers) { return sizeof(CtHeader) + 8 * ct_words(n, towers); }
+size_t gf_ct_packed_bytes(uint32_t n, uint32_t towers) {
+    return ct_shape_ok(n, towers) ? sizeof(CtHeader) + 8 * ct_words(n, towers) : 0;
+}
 
 int gf_ct_pack(const uint64_t* h_c0, const uint64_t* h_c1, uint32_t n, uint32_t towers, const uint64_t* h_moduli,
                double scale, void* out, size_t cap) {
     return guard([&] {
         const size_t need = gf_ct_packed_bytes(n, towers);
+        if (!need) throw std::runtime_error("gpufhe: ciphertext shape outside the wire format (n a power of two <= 2^17, 1..4096 towers)");
         if (cap < need) throw std::runtime_error("gpufhe: output buffer smaller than gf_ct_packed_bytes");
         check_residues(h_c0, h_moduli, towers, n, towers);
         check_residues(h_c1, h_moduli, towers, n, towers);
@@ -92,6 +100,7 @@ int gf_ct_unpack_header(const void* buf, size_t len, uint32_t* n, uint32_t* towe
         CtHeader h;
         std::memcpy(&h, buf, sizeof h);
         if (std::memcmp(h.magic, CT_MAGIC, 8)) throw std::runtime_error("gpufhe: not a packed ciphertext");
+        if (!ct_shape_ok(h.n, h.towers)) throw std::runtime_error("gpufhe: packed ciphertext header out of range");
         if (len != gf_ct_packed_bytes(h.n, h.towers)) throw std::runtime_error("gpufhe: packed ciphertext length mismatch");
         *n = h.n;
         *towers = h.towers;
@@ -160,6 +169,8 @@ gf_key* gf_key_unpack(gf_ctx* c, const void* buf, size_t len, hipStream_t s) {
         if (std::memcmp(h.magic, KY_MAGIC, 8)) throw std::runtime_error("gpufhe: not a packed key");
         if (h.n != C.n || h.L != C.L || h.K != C.K)
             throw std::runtime_error("gpufhe: packed key belongs to another parameter set");
+        if (h.nparts == 0 || h.nparts > C.L || h.fullQ < C.L || h.fullQ > WIRE_MAX_TOWERS || h.rows != h.fullQ + C.K)
+            throw std::runtime_error("gpufhe: packed key shape out of range");
         const size_t words = (size_t)h.nparts * h.rows * h.n;
         const size_t need = sizeof(KeyHeader) + 8 * ((size_t)(h.L + h.K) + 2 * words);
         if (len != need) throw std::runtime_error("gpufhe: packed key length mismatch");
@@ -170,6 +181,16 @@ gf_key* gf_key_unpack(gf_ctx* c, const void* buf, size_t len, hipStream_t s) {
         std::vector<uint64_t> av(words), bv(words);
         std::memcpy(av.data(), body + (size_t)(h.L + h.K) * 8, words * 8);
         std::memcpy(bv.data(), body + (size_t)(h.L + h.K) * 8 + words * 8, words * 8);
+        // row r of a part is mod q_r (r < L) or p_(r - fullQ) (r >= fullQ); rows
+        // between L and fullQ belong to towers outside this context and are kept as they are
+        for (uint32_t part = 0; part < h.nparts; ++part)
+            for (uint32_t r = 0; r < h.rows; ++r) {
+                if (r >= C.L && r < h.fullQ) continue;
+                const uint64_t* m = &C.mods[r < C.L ? r : C.L + (r - h.fullQ)];
+                const size_t o = ((size_t)part * h.rows + r) * h.n;
+                check_residues(av.data() + o, m, 1, h.n, 1);
+                check_residues(bv.data() + o, m, 1, h.n, 1);
+            }
         auto k = new gf_key;
         k->ctx = c;
         try {

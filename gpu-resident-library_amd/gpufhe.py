@@ -43,6 +43,7 @@ _capi = {
     "gf_ctx_create": (vp, [u32, u64p, u32, u64p, u32, u64p, u32]),
     "gf_ctx_destroy": (None, [vp]),
     "gf_ws_bytes": (sz, [vp]),
+    "gf_rescale_ws_bytes": (sz, [vp]),
     "gf_key_create": (vp, [vp, u64p, u64p, u32, vp]),
     "gf_key_destroy": (None, [vp]),
     "gf_ctx_create_alpha": (vp, [u32, u64p, u32, u64p, u32, u64p, u32]),
@@ -102,6 +103,10 @@ _capi = {
     "gf_event_destroy": (C.c_int, [vp]),
     "gf_event_record": (C.c_int, [vp, vp]),
     "gf_event_elapsed_ms": (C.c_int, [C.POINTER(C.c_float), vp, vp]),
+    "gf_capture_begin": (C.c_int, [vp]),
+    "gf_capture_end": (C.c_int, [vp, vpp]),
+    "gf_graph_launch": (C.c_int, [vp, vp]),
+    "gf_graph_destroy": (C.c_int, [vp]),
 }
 _kern = {
     "LaunchNTT_CT": [vp, vp, vp, u32, u64, vp],
@@ -344,6 +349,71 @@ class Event:
         if getattr(self, "e", None):
             lib.gf_event_destroy(self.e)
             self.e = None
+
+
+class Graph:
+    """What fn() launches on `stream`, captured once and replayed."""
+
+    def __init__(self, stream, fn):
+        check(lib.gf_capture_begin(stream.s))
+        try:
+            fn()
+        finally:
+            g = C.c_void_p()
+            rc = lib.gf_capture_end(stream.s, C.byref(g))
+        check(rc)
+        self.g = g.value
+
+    def launch(self, stream):
+        check(lib.gf_graph_launch(self.g, stream.s))
+
+    def __del__(self):
+        if getattr(self, "g", None):
+            lib.gf_graph_destroy(self.g)
+            self.g = None
+
+
+class RefSequence:
+    """One chain of the reference's EvalMult+Rescale launch sequence
+    (bench/bench_evalmult_resident.cpp:96-100) over the drop-in boundary:
+    LaunchTensor -> keyswitch_resident (gf_keyswitch) -> LaunchCombine ->
+    rescale_resident_raw (gf_rescale_comp) on each component, on the chain's
+    own stream and buffers.  Outputs: r0, r1 (level - 1 towers, in place)."""
+
+    def __init__(self, ctx, key, level, d_mods, seed):
+        n = ctx.n
+        self.ctx, self.key, self.level, self.d_mods = ctx, key, level, d_mods
+        self.stream = Stream()
+        self.ins = [Dev(level * n) for _ in range(4)]
+        for j, d in enumerate(self.ins):
+            check(lib.gf_fill_uniform(ctx.h, d.ptr, level, seed + j, self.stream.s))
+        self.t = [Dev(level * n) for _ in range(3)]
+        self.ba = [Dev(level * n) for _ in range(2)]
+        self.r = [Dev(level * n) for _ in range(2)]
+        self.ws = ctx.workspace()
+        self.rs = Dev(lib.gf_rescale_ws_bytes(ctx.h) // 8)
+        self.graph = None
+        self.stream.sync()
+
+    def launch(self):
+        n, L, s = self.ctx.n, self.level, self.stream.s
+        a0, a1, b0, b1 = (d.ptr for d in self.ins)
+        t0, t1, t2 = (d.ptr for d in self.t)
+        lib.LaunchTensor(a0, a1, b0, b1, t0, t1, t2, self.d_mods.ptr, L, n, s)
+        check(lib.gf_keyswitch(self.ctx.h, self.key.h, L, t2, self.ba[0].ptr, self.ba[1].ptr, self.ws.ptr, s))
+        lib.LaunchCombine(self.r[0].ptr, self.r[1].ptr, t0, t1, self.ba[0].ptr, self.ba[1].ptr, self.d_mods.ptr,
+                          L, n, s)
+        for r in self.r:
+            check(lib.gf_rescale_comp(self.ctx.h, L, r.ptr, r.ptr, self.rs.ptr, s))
+
+    def capture(self):
+        """Capture the sequence as a graph (call launch() once first, so the
+        single-op plans it uses are built outside the capture)."""
+        self.graph = Graph(self.stream, self.launch)
+
+    def outputs(self):
+        w = (self.level - 1) * self.ctx.n
+        return self.r[0].download(w), self.r[1].download(w)
 
 
 def plan_evalmult_rescale(ctx, key, levels, c0a, c1a, c0b, c1b, o0, o1, ws):

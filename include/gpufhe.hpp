@@ -151,8 +151,11 @@ void keyswitch_resident(const uint64_t* d_a, uint64_t* d_ba0, uint64_t* d_ba1, c
                         DeviceKSWork& W, hipStream_t s);
 // src/keyswitch_resident.cpp:227-244 -- one component in place, towers -> towers-1.
 // s1/s2 must be the rescale constants of these moduli (native_rescale_consts);
-// scratch/drop are accepted for signature compatibility (the context keeps a
-// per-stream workspace, created on the first call on each stream).
+// scratch/drop are accepted for signature compatibility: the batched rescale
+// needs towers - 1 towers of scratch, so ks_context_create and every
+// ks_work_create each allocate one L-tower scratch into a pool of the context,
+// and a stream takes one from the pool on its first call and keeps it (a stream
+// beyond the pool's count allocates one on its first call).
 void rescale_resident_raw(uint64_t* d_c, uint32_t towers, const DeviceKSContext& C,
                           const std::vector<uint64_t>& s1, const std::vector<uint64_t>& s2,
                           uint64_t* d_scratch, uint64_t* d_drop, hipStream_t s);

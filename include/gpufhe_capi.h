@@ -149,9 +149,12 @@ int gf_keyswitch(gf_ctx* c, gf_key* k, uint32_t level, const uint64_t* d, uint64
 int gf_rescale(gf_ctx* c, uint32_t level, const uint64_t* c0, const uint64_t* c1, uint64_t* o0,
                uint64_t* o1, uint64_t* ws, hipStream_t s);
 /* one component: d_o (level-1 towers) may equal d_c (in place, as
- * rescale_resident_raw, src/keyswitch_resident.cpp:227-244) */
+ * rescale_resident_raw, src/keyswitch_resident.cpp:227-244).  ws: at least
+ * gf_rescale_ws_bytes (a gf_ws_bytes workspace also serves) */
 int gf_rescale_comp(gf_ctx* c, uint32_t level, const uint64_t* d_c, uint64_t* d_o, uint64_t* ws,
                     hipStream_t s);
+/* scratch of gf_rescale_comp: L towers of n words */
+size_t gf_rescale_ws_bytes(const gf_ctx* c);
 /* (sigma_k(c0) + ba0, ba1), (ba0, ba1) = KS(sigma_k(c1)); k odd, < 2n */
 int gf_rotate(gf_ctx* c, gf_key* rk, uint32_t level, uint64_t k, const uint64_t* c0,
               const uint64_t* c1, uint64_t* o0, uint64_t* o1, uint64_t* ws, hipStream_t s);
@@ -266,6 +269,15 @@ int gf_event_create(hipEvent_t* e);
 int gf_event_destroy(hipEvent_t e);
 int gf_event_record(hipEvent_t e, hipStream_t s);
 int gf_event_elapsed_ms(float* ms, hipEvent_t start, hipEvent_t stop);
+
+/* ---- stream capture: record what a caller launches on s (plans, the
+ * launchers of gpufhe_kernels.h, single-op calls whose plans are already
+ * cached) into a replayable graph, as the reference's benchmark does with
+ * cudaStreamBeginCapture (bench/bench_evalmult_resident.cpp:93-102) ---- */
+int gf_capture_begin(hipStream_t s);
+int gf_capture_end(hipStream_t s, hipGraphExec_t* out);
+int gf_graph_launch(hipGraphExec_t g, hipStream_t s);
+int gf_graph_destroy(hipGraphExec_t g);
 
 /* ---- wire format: ciphertexts and switching keys with a parameter header ----
  * The reference keeps raw tower-major arrays (src/device_ciphertext.cpp:43-61);

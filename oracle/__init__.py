@@ -70,7 +70,7 @@ class OrParams(C.Structure):
 
 
 lib = _load("liboracle.so")
-ref = _load(os.path.join("_ref", "libref.so"))
+_ref = []   # oracle/_ref/libref.so, loaded on first use of `ref` (tests that compare with it)
 
 _sig = {
     "or_mulmod": (C.c_uint64, [C.c_uint64] * 3),
@@ -129,20 +129,35 @@ for _name, (_res, _args) in _sig.items():
     _f.restype = _res
     _f.argtypes = _args
 
-if ref is not None:
-    ref.ref_native_primes.restype = C.c_int
-    ref.ref_native_primes.argtypes = [u64p, C.c_uint32, C.c_uint32, C.c_uint32, u64p, C.c_uint32]
-    ref.ref_native_root.restype = C.c_uint64
-    ref.ref_native_root.argtypes = [C.c_uint32, C.c_uint64]
-    ref.ref_rescale_consts.restype = None
-    ref.ref_rescale_consts.argtypes = [u64p, u64p, u64p, C.c_uint32, C.c_uint32]
-    ref.ref_ks_constants.restype = C.c_int
-    ref.ref_ks_constants.argtypes = ([u64p, C.c_uint32, u64p, C.c_uint32, C.c_uint32, u32p, u32p,
-                                      u32p, u32p] + [u64p] * 13)
-    ref.ref_encode.restype = None
-    ref.ref_encode.argtypes = [i64p, f64p, f64p, C.c_uint32, C.c_double]
-    ref.ref_decode.restype = None
-    ref.ref_decode.argtypes = [f64p, f64p, i64p, C.c_uint32, C.c_double]
+def _load_ref():
+    """The reference's own host code, built into oracle/_ref/libref.so from its
+    sources, or None where they were absent.  Loaded only on first access of
+    `oracle.ref`, so importing the oracle maps no reference-built code."""
+    if _ref:
+        return _ref[0]
+    r = _load(os.path.join("_ref", "libref.so"))
+    if r is not None:
+        r.ref_native_primes.restype = C.c_int
+        r.ref_native_primes.argtypes = [u64p, C.c_uint32, C.c_uint32, C.c_uint32, u64p, C.c_uint32]
+        r.ref_native_root.restype = C.c_uint64
+        r.ref_native_root.argtypes = [C.c_uint32, C.c_uint64]
+        r.ref_rescale_consts.restype = None
+        r.ref_rescale_consts.argtypes = [u64p, u64p, u64p, C.c_uint32, C.c_uint32]
+        r.ref_ks_constants.restype = C.c_int
+        r.ref_ks_constants.argtypes = ([u64p, C.c_uint32, u64p, C.c_uint32, C.c_uint32, u32p, u32p,
+                                        u32p, u32p] + [u64p] * 13)
+        r.ref_encode.restype = None
+        r.ref_encode.argtypes = [i64p, f64p, f64p, C.c_uint32, C.c_double]
+        r.ref_decode.restype = None
+        r.ref_decode.argtypes = [f64p, f64p, i64p, C.c_uint32, C.c_double]
+    _ref.append(r)
+    return r
+
+
+def __getattr__(name):   # module attribute hook: `oracle.ref` loads the library lazily
+    if name == "ref":
+        return _load_ref()
+    raise AttributeError(name)
 
 
 class MT64(C.Structure):
@@ -375,7 +390,7 @@ def ref_ks_constants(qmods, pmods, numPart):
     phi, phip = np.zeros(len(p), np.uint64), np.zeros(len(p), np.uint64)
     phmq = np.zeros(len(p) * len(q), np.uint64)
     dlo, dhi, pinv = (np.zeros(len(q), np.uint64) for _ in range(3))
-    ref.ref_ks_constants(P(q), len(q), P(p), len(p), numPart, P(alpha), P(sp), P(sc), P(st), P(qhi),
+    _load_ref().ref_ks_constants(P(q), len(q), P(p), len(p), numPart, P(alpha), P(sp), P(sc), P(st), P(qhi),
                          P(qhip), P(src), P(qhmp), P(cm), P(mlo), P(mhi), P(phi), P(phip), P(phmq),
                          P(dlo), P(dhi), P(pinv))
     return dict(alpha=int(alpha[0]), numPart=numPart, sizePart=sp, sizeCompl=sc, startIdx=st,

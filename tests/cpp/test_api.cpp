@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <memory>
 #include <random>
 #include <stdexcept>
 #include <string>
@@ -661,6 +662,40 @@ static void gpu_cases() {
         w1.resize((size_t)(L - 1) * n);
         same(h0, w0, "c0");
         same(h1, w1, "c1");
+    });
+
+    run("rescale_resident_raw_two_streams_vs_oracle", [] {
+        // the context's one rescale scratch shared by calls alternating over two
+        // streams (no allocation per call; each call on a new stream waits for
+        // the previous one on the device): every result equals the oracle
+        Params P(1 << 12, 6, 3, 2);
+        std::mt19937_64 g(17);
+        KeySwitchConstants Kc = P.consts(P.L);
+        auto C = ks_context_create(Kc);
+        hipStream_t st[2];
+        hip_ok(hipStreamCreate(&st[0]));
+        hip_ok(hipStreamCreate(&st[1]));
+        std::vector<V> in;
+        std::vector<std::unique_ptr<Dev>> d;
+        const uint32_t lv[6] = {6, 5, 6, 3, 2, 6};
+        for (uint32_t i = 0; i < 6; ++i) {
+            in.push_back(P.rand_towers(lv[i], g));
+            d.emplace_back(new Dev(in.back()));
+        }
+        for (uint32_t i = 0; i < 6; ++i) {
+            V s1, s2;
+            native_rescale_consts(s1, s2, P.q, lv[i] - 1);
+            rescale_resident_raw(d[i]->p, lv[i], C, s1, s2, nullptr, nullptr, st[i % 2]);
+        }
+        hip_ok(hipDeviceSynchronize());
+        for (uint32_t i = 0; i < 6; ++i) {
+            V w = in[i];
+            expect(or_rescale_raw(P.P, w.data(), lv[i]) == 0, "oracle");
+            w.resize((size_t)(lv[i] - 1) * P.n);
+            same(d[i]->get((size_t)(lv[i] - 1) * P.n), w, "rescale vs oracle");
+        }
+        hip_ok(hipStreamDestroy(st[0]));
+        hip_ok(hipStreamDestroy(st[1]));
     });
 
     run("rescale_resident_raw_rejects_foreign_constants", [] {

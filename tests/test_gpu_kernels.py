@@ -107,7 +107,8 @@ def test_rescale_fuse_moddown_add_ptmulacc(G):
     assert np.array_equal(acc1.download(), w1.astype(np.uint64))
 
 
-def test_ks_macc_single_and_batched(G):
+@pytest.mark.parametrize("off", [0, 1])
+def test_ks_macc_single_and_batched(G, off):
     P = make_params(1 << 12, 4, 2)
     n, L, K = P.n, P.L, P.K
     W = L + K
@@ -130,8 +131,8 @@ def test_ks_macc_single_and_batched(G):
         O.lib.or_ks_macc(O.P(t0), O.P(t1), O.P(dig[sl].copy()), O.P(bv[ks].copy()), O.P(av[ks].copy()),
                          int(mods[i]), n)
         w0[sl], w1[sl] = t0, t1
-    d0, d1, ddig, dbv, dav = (G.Dev.of(x) for x in (o0, o1, dig, bv, av))
-    ptrs = G.Dev.of(np.array([ddig.at(i * n) for i in range(W)], np.uint64))
+    d0, d1, ddig, dbv, dav = (Shifted(G, x, off=off) for x in (o0, o1, dig, bv, av))
+    ptrs = G.Dev.of(np.array([ddig.ptr + 8 * i * n for i in range(W)], np.uint64))
     krow = np.array(rows, np.uint32)
     dkrow = G.Dev((W + 1) // 2)
     G.check(G.lib.gf_h2d(dkrow.ptr, krow.ctypes.data, krow.nbytes))
@@ -146,17 +147,25 @@ def test_ks_macc_single_and_batched(G):
     assert np.array_equal(e0.download(), w0[:n]) and np.array_equal(e1.download(), w1[:n])
 
 
-def test_approx_switch_crt_basis(G):
-    P = make_params(1 << 15, 12, 3)
-    n = P.n
-    k = O.ks_constants(P.q, P.p, 3)
+# (n, L, dnum, bits): 4 sources of <= 60 bits (column sums), 10 sources
+# (generic 128-bit sums), 61-bit moduli (generic sums), and 9 sources of 61
+# bits, whose sums can reach p 2^64 (the Barrett form with the caller's mu)
+APPROX_CASES = [(1 << 15, 12, 3, (50, 60, 60)), (1 << 12, 10, 1, (50, 60, 60)), (1 << 12, 4, 1, (61, 61, 61)),
+                (1 << 12, 9, 1, (61, 61, 61))]
+
+
+@pytest.mark.parametrize("n,L,dnum,bits", APPROX_CASES)
+def test_approx_switch_crt_basis(G, n, L, dnum, bits):
+    P = make_params(n, L, dnum, qbits=bits[0], q0bits=bits[1], pbits=bits[2])
+    k = O.ks_constants(P.q, P.p, dnum)
     W = P.L + P.K
-    for part in range(3):
+    for part in range(dnum):
         a = k["alpha"]
         sp, sc = int(k["sizePart"][part]), int(k["sizeCompl"][part])
         qs = k["partSrcMod"][part * a: part * a + sp].copy()
         rng = np.random.default_rng(20260729 + part)
         src = np.concatenate([rng.integers(0, int(q), n, dtype=np.uint64) for q in qs])
+        src[::97] = np.repeat(qs - 1, n)[::97]       # largest residues
         qhi = k["partQHatInv"][part * a: part * a + sp].copy()
         qhip = k["partQHatInvPrec"][part * a: part * a + sp].copy()
         qhmp = k["partQHatModp"][part * a * W: part * a * W + sp * sc].copy()
@@ -173,21 +182,41 @@ def test_approx_switch_crt_basis(G):
         assert np.array_equal(dst.download(), want)
 
 
-def test_tensor_combine(G):
-    P = make_params(1 << 12, 5, 2)
+class Shifted:
+    """A device copy of `arr` starting `off` words into its allocation (off = 1:
+    only 8-byte aligned, so the launchers take their one-word access path)."""
+
+    def __init__(self, G, arr=None, words=None, off=0):
+        w = arr.size if arr is not None else words
+        self.buf = G.Dev(w + 1)
+        self.ptr = self.buf.at(off)
+        self.off, self.words = off, w
+        if arr is not None:
+            self.buf.upload(arr, off)
+
+    def download(self):
+        return self.buf.download(self.words, self.off)
+
+
+# moduli: 60 + 50-bit, a 40-bit modulus, 61-bit moduli (the Montgomery
+# constants each launcher derives on the device), both access widths
+@pytest.mark.parametrize("bits,off", [((50, 60), 0), ((50, 60), 1), ((40, 40), 0), ((61, 61), 1)])
+def test_tensor_combine(G, bits, off):
+    P = make_params(1 << 12, 5, 2, qbits=bits[0], q0bits=bits[1], pbits=61)
     n, L = P.n, 5
     rng = np.random.default_rng(303)
     cts = [np.concatenate([rng.integers(0, int(q), n, dtype=np.uint64) for q in P.q]) for _ in range(6)]
+    cts[0][::13] = np.repeat(P.q - 1, n)[::13]
     want = [np.zeros(L * n, np.uint64) for _ in range(3)]
     O.lib.or_tensor(*[O.P(c) for c in cts[:4]], *[O.P(w) for w in want], O.P(P.q), L, n)
-    d = [G.Dev.of(c) for c in cts]
-    t = [G.Dev(L * n) for _ in range(3)]
+    d = [Shifted(G, c, off=off) for c in cts]
+    t = [Shifted(G, words=L * n, off=off) for _ in range(3)]
     dm = G.Dev.of(P.q)
     G.lib.LaunchTensor(*[x.ptr for x in d[:4]], *[x.ptr for x in t], dm.ptr, L, n, None)
     G.device_sync()
     for i in range(3):
         assert np.array_equal(t[i].download(), want[i])
-    r = [G.Dev(L * n), G.Dev(L * n)]
+    r = [Shifted(G, words=L * n, off=off), Shifted(G, words=L * n, off=off)]
     G.lib.LaunchCombine(r[0].ptr, r[1].ptr, t[0].ptr, t[1].ptr, d[4].ptr, d[5].ptr, dm.ptr, L, n, None)
     G.device_sync()
     w0, w1 = np.zeros(L * n, np.uint64), np.zeros(L * n, np.uint64)

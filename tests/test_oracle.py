@@ -16,7 +16,9 @@ from common import (KeySet, centred, crt, decode, encode, make_params, mulmod_ve
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 GOLD = json.load(open(os.path.join(HERE, "golden", "golden_small.json")))
-needs_ref = pytest.mark.skipif(O.ref is None, reason="oracle/_ref not built (reference sources absent)")
+# file check only: collecting this module (also under -m gpu) must not load the reference-built library
+REF_SO = os.path.join(os.path.dirname(O.__file__), "_ref", "libref.so")
+needs_ref = pytest.mark.skipif(not os.path.exists(REF_SO), reason="oracle/_ref not built (reference sources absent)")
 
 
 def brv(v, bits):
@@ -25,6 +27,15 @@ def brv(v, bits):
         r = (r << 1) | (v & 1)
         v >>= 1
     return r
+
+
+def test_oracle_import_maps_no_reference_code():
+    """Importing the oracle loads liboracle.so only; oracle/_ref/libref.so maps
+    on first use of oracle.ref (so GPU test processes never load it)."""
+    code = ("import sys; sys.path.insert(0, %r); import oracle; "
+            "m = open('/proc/self/maps').read(); assert 'liboracle.so' in m; assert 'libref.so' not in m"
+            % os.path.dirname(os.path.dirname(O.__file__)))
+    subprocess.check_call([__import__("sys").executable, "-c", code])
 
 
 # ----------------------------------------------------------------- RNG ----

@@ -49,6 +49,51 @@ def test_ciphertext_rejects_damage(G):
         G.pack_ciphertext(over, c1, P.n, P.q)
 
 
+def fnv1a(b):
+    h = 0xcbf29ce484222325
+    for x in b:
+        h = ((h ^ x) * 0x100000001b3) & (2**64 - 1)
+    return h
+
+
+def test_ciphertext_header_bounds(G):
+    """Header fields are bounded before any length is computed from them: a
+    crafted header with huge n or towers (whose length would wrap) is refused,
+    and gf_ct_packed_bytes is 0 outside the format's shapes."""
+    import struct
+    assert G.lib.gf_ct_packed_bytes(3 * 1024, 2) == 0          # n not a power of two
+    assert G.lib.gf_ct_packed_bytes(1 << 18, 2) == 0           # n above 2^17
+    assert G.lib.gf_ct_packed_bytes(1024, 0) == 0
+    assert G.lib.gf_ct_packed_bytes(1024, 5000) == 0
+    assert G.lib.gf_ct_packed_bytes(1 << 17, 4096) > 0
+    for n, towers in ((1 << 31, 1 << 31), (1024, 0xFFFFFFFF), (1000, 2)):
+        hdr = b"GFHECT01" + struct.pack("<IIdQ", n, towers, 1.0, 0)
+        with pytest.raises(G.GpuFheError, match="out of range"):
+            G.unpack_ciphertext(hdr + bytes(64))
+
+
+@pytest.mark.gpu
+def test_key_unpack_checks_residues(G):
+    """A packed key whose checksum is consistent but which holds a residue not
+    below its row's modulus (a Q row and a P row) is refused."""
+    import struct
+    P = make_params(1 << 12, 4, 2)
+    ctx = G.Context(P.n, P.q, P.p, P.roots, P.dnum)
+    rng = np.random.default_rng(4)
+    av = np.concatenate([rng.integers(0, int(m), P.n, dtype=np.uint64) for _ in range(P.dnum) for m in P.mods])
+    bv = np.concatenate([rng.integers(0, int(m), P.n, dtype=np.uint64) for _ in range(P.dnum) for m in P.mods])
+    buf = bytearray(G.pack_key(G.Key(ctx, av, bv, P.dnum)))
+    W = len(P.mods)
+    body0 = 40 + 8 * W
+    for row, which in ((1, 0), (P.L + 1, 1)):      # av row 1 (mod q_1); bv of the second P row
+        bad = bytearray(buf)
+        word = (which * P.dnum * W + W + row) * P.n + 7    # part 1
+        bad[body0 + 8 * word: body0 + 8 * word + 8] = struct.pack("<Q", int(P.mods[row]))
+        bad[32:40] = struct.pack("<Q", fnv1a(bytes(bad[40:])))
+        with pytest.raises(G.GpuFheError, match="residue"):
+            G.unpack_key(ctx, bytes(bad))
+
+
 @pytest.mark.gpu
 def test_key_round_trip_and_parameter_check(G):
     P = make_params(1 << 12, 4, 2)

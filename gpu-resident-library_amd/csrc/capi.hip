@@ -47,6 +47,49 @@ static std::shared_ptr<Plan> cached_plan(gf_ctx* c, std::vector<uint64_t> key, s
     return P;
 }
 
+// ---- allocation-free EvalMult+Rescale (gf_evalmult_rescale_noalloc) ----
+// The workspace and the plan's job arrays live in one file-scope device array.
+// Sized for one operation at n = 2^16, L = 24, K = 6, dnum = 4 (a workspace of
+// 318 towers), plus 1 MiB of job arrays; each GPU holds its copy until the
+// process ends.  Host code never passes `scratch` itself to a kernel: each
+// device's address of it comes from hipGetSymbolAddress, resolved once per
+// device (with that device current) when a context is created there.
+namespace {
+constexpr size_t NOALLOC_WS_WORDS = (size_t)318 << 16;
+constexpr size_t NOALLOC_PLAN_BYTES = (size_t)1 << 20;
+constexpr size_t NOALLOC_WORDS = NOALLOC_WS_WORDS + NOALLOC_PLAN_BYTES / 8;
+}  // namespace
+static __device__ __align__(256) uint64_t scratch[NOALLOC_WORDS];
+
+static uint64_t* scratch_of_current_device(int* dev_out) {
+    static std::mutex mu;
+    static std::map<int, uint64_t*> addr;
+    int dev = -1;
+    hip_check(hipGetDevice(&dev), "current device");
+    *dev_out = dev;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = addr.find(dev);
+    if (it != addr.end()) return it->second;
+    uint64_t* p = nullptr;
+    hip_check(hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch)), "address of the static scratch array");
+    addr.emplace(dev, p);
+    return p;
+}
+
+static void ctx_setup(gf_ctx* c) {
+    c->scratch = scratch_of_current_device(&c->device);
+    hip_check(hipHostMalloc(&c->stage, NOALLOC_PLAN_BYTES, hipHostMallocDefault), "plan staging");
+    hip_check(hipEventCreateWithFlags(&c->stage_done, hipEventDisableTiming), "plan staging event");
+}
+
+gf_ctx::~gf_ctx() {
+    if (stage_done) {
+        (void)hipEventSynchronize(stage_done);
+        (void)hipEventDestroy(stage_done);
+    }
+    if (stage) (void)hipHostFree(stage);
+}
+
 uint64_t gf_key::next_serial() {
     static std::atomic<uint64_t> next{1};
     return next.fetch_add(1);
@@ -100,10 +143,11 @@ gf_ctx* gf_ctx_create(uint32_t n, const uint64_t* q, uint32_t L, const uint64_t*
                       const uint64_t* roots, uint32_t dnum) {
     return guard_ptr<gf_ctx>([&] {
         if (dnum == 0 || dnum > L) throw std::runtime_error("gpufhe: dnum must be in [1, L]");
-        auto c = new gf_ctx;
+        auto c = std::make_unique<gf_ctx>();
         c->C = std::make_unique<Ctx>(n, std::vector<uint64_t>(q, q + L), std::vector<uint64_t>(p, p + K),
                                      std::vector<uint64_t>(roots, roots + L + K), (L + dnum - 1) / dnum);
-        return c;
+        ctx_setup(c.get());
+        return c.release();
     });
 }
 void gf_ctx_destroy(gf_ctx* c) { delete c; }
@@ -141,10 +185,11 @@ gf_ctx* gf_ctx_create_alpha(uint32_t n, const uint64_t* q, uint32_t L, const uin
                             const uint64_t* roots, uint32_t alpha) {
     return guard_ptr<gf_ctx>([&] {
         if (alpha == 0 || alpha > L) throw std::runtime_error("gpufhe: part width alpha must be in [1, L]");
-        auto c = new gf_ctx;
+        auto c = std::make_unique<gf_ctx>();
         c->C = std::make_unique<Ctx>(n, std::vector<uint64_t>(q, q + L), std::vector<uint64_t>(p, p + K),
                                      std::vector<uint64_t>(roots, roots + L + K), alpha);
-        return c;
+        ctx_setup(c.get());
+        return c.release();
     });
 }
 gf_key* gf_key_create_rows(gf_ctx* c, const uint64_t* av, const uint64_t* bv, uint32_t nparts, uint32_t rows,
@@ -270,6 +315,41 @@ int gf_evalmult_rescale(gf_ctx* c, gf_key* k, uint32_t l, const uint64_t* c0a, c
         P->run(s);
     });
 }
+// gf_evalmult_rescale with its workspace and job arrays in the static scratch
+// array: no device or pinned allocation in the call.  The plan is rebuilt on
+// the host and its job arrays staged through the context's pinned buffer (after
+// that buffer's previous copy has completed) on every call.
+int gf_evalmult_rescale_noalloc(gf_ctx* c, gf_key* k, uint32_t l, const uint64_t* c0a, const uint64_t* c1a,
+                                const uint64_t* c0b, const uint64_t* c1b, uint64_t* o0, uint64_t* o1,
+                                hipStream_t s) {
+    return guard([&] {
+        const Ctx& C = *c->C;
+        if (C.ws_words() > NOALLOC_WS_WORDS)
+            throw std::runtime_error("gpufhe: the context's workspace exceeds the static scratch maximum "
+                                     "(318 towers of 2^16 words: n = 2^16, L = 24, K = 6, dnum = 4)");
+        int dev = -1;
+        hip_check(hipGetDevice(&dev), "current device");
+        if (dev != c->device) throw std::runtime_error("gpufhe: context created on another device than the current one");
+        PlanBuilder b(C);
+        b.evalmult_rescale(l, c0a, c1a, c0b, c1b, o0, o1, *k->K, c->scratch);
+        std::vector<char> blob;
+        std::unique_ptr<Plan> P = b.finish_host(blob);
+        if (blob.size() > NOALLOC_PLAN_BYTES) throw std::runtime_error("gpufhe: plan larger than the static job area");
+        uint64_t* jobs = c->scratch + NOALLOC_WS_WORDS;
+        std::lock_guard<std::mutex> lk(c->mu);
+        hip_check(hipEventSynchronize(c->stage_done), "plan staging");
+        if (!blob.empty()) {
+            std::memcpy(c->stage, blob.data(), blob.size());
+            hip_check(hipMemcpyAsync(jobs, c->stage, blob.size(), hipMemcpyHostToDevice, s), "stage plan");
+        }
+        hip_check(hipEventRecord(c->stage_done, s), "plan staging");
+        P->base = jobs;
+        P->run(s);
+        hip_check(hipGetLastError(), "launch");
+    });
+}
+size_t gf_noalloc_scratch_bytes(void) { return NOALLOC_WORDS * 8; }
+
 int gf_evalmult(gf_ctx* c, gf_key* k, uint32_t l, const uint64_t* c0a, const uint64_t* c1a, const uint64_t* c0b,
                 const uint64_t* c1b, uint64_t* o0, uint64_t* o1, uint64_t* ws, hipStream_t s) {
     return guard([&] {

@@ -1,6 +1,8 @@
 // Definitions behind the opaque handles of include/gpufhe_capi.h, shared by
 // the C ABI translation units, and the exception-to-return-code guards.
 #pragma once
+#include <hip/hip_runtime_api.h>
+
 #include <map>
 #include <memory>
 #include <mutex>
@@ -20,6 +22,15 @@ struct gf_ctx {
     std::mutex mu;
     // single-op plans by (op, arguments); keys appear by serial, never by address
     std::map<std::vector<uint64_t>, gf_cached_plan> cache;
+    // set up with the context for gf_evalmult_rescale_noalloc: the device the
+    // context was created on, that device's address of the library's static
+    // scratch array, and a pinned staging area (with the event of its last
+    // copy) for one plan's job arrays
+    int device = -1;
+    uint64_t* scratch = nullptr;
+    void* stage = nullptr;
+    hipEvent_t stage_done = nullptr;
+    ~gf_ctx();
 };
 struct gf_key {
     gf_ctx* ctx;

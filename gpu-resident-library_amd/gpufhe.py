@@ -69,6 +69,8 @@ _capi = {
     "gf_rescale_comp": (C.c_int, [vp, u32, vp, vp, vp, vp]),
     "gf_evalmult_rescale": (C.c_int, [vp, vp, u32] + [vp] * 7 + [vp]),
     "gf_evalmult": (C.c_int, [vp, vp, u32] + [vp] * 7 + [vp]),
+    "gf_evalmult_rescale_noalloc": (C.c_int, [vp, vp, u32] + [vp] * 6 + [vp]),
+    "gf_noalloc_scratch_bytes": (sz, []),
     "gf_keyswitch": (C.c_int, [vp, vp, u32, vp, vp, vp, vp, vp]),
     "gf_rescale": (C.c_int, [vp, u32, vp, vp, vp, vp, vp, vp]),
     "gf_rotate": (C.c_int, [vp, vp, u32, u64, vp, vp, vp, vp, vp, vp]),

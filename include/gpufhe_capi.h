@@ -138,6 +138,20 @@ int gf_automorph(uint32_t n, const uint64_t* d_in, uint64_t* d_out, uint32_t tow
 int gf_evalmult_rescale(gf_ctx* c, gf_key* rlk, uint32_t level, const uint64_t* c0a,
                         const uint64_t* c1a, const uint64_t* c0b, const uint64_t* c1b,
                         uint64_t* o0, uint64_t* o1, uint64_t* ws, hipStream_t s);
+/* gf_evalmult_rescale that allocates nothing: no counterpart in the reference;
+ * for callers that cannot allocate device memory (tight serving loops,
+ * harnesses that forbid hipMalloc).  The workspace and the plan's job arrays
+ * live in the library's static device array (gf_noalloc_scratch_bytes: 160 MiB
+ * per GPU, held until the process ends), sized for one operation up to
+ * n = 2^16, L = 24, K = 6, dnum = 4; larger contexts are rejected.  Every call
+ * on a GPU shares that array, so calls that overlap on one GPU (other streams,
+ * other threads) race; calls on one stream are ordered.  The context must have
+ * been created on the current device.  Nothing in the array needs to start at
+ * zero: every word an operation reads it wrote first. */
+int gf_evalmult_rescale_noalloc(gf_ctx* c, gf_key* rlk, uint32_t level, const uint64_t* c0a,
+                                const uint64_t* c1a, const uint64_t* c0b, const uint64_t* c1b,
+                                uint64_t* o0, uint64_t* o1, hipStream_t s);
+size_t gf_noalloc_scratch_bytes(void);
 /* tensor -> keyswitch -> combine (no rescale) */
 int gf_evalmult(gf_ctx* c, gf_key* rlk, uint32_t level, const uint64_t* c0a, const uint64_t* c1a,
                 const uint64_t* c0b, const uint64_t* c1b, uint64_t* o0, uint64_t* o1, uint64_t* ws,

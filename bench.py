@@ -285,7 +285,7 @@ def run_cfg1(G, reps=9, chains=16, with_cpu=True):
     q, p, roots, K, alpha = native_params(G, n, L, dnum)
     stream = G.Stream()
     ctx = G.Context(n, q, p, roots, dnum)
-    radix, _ = ctx.autotune(stream)
+    radix, _ = ctx.autotune(stream, chains)
     key = G.Key.uniform(ctx, dnum, 4242, stream)
     av, bv = key.download(stream)
     P = oracle_params(n, q, p, roots, dnum)
@@ -310,7 +310,8 @@ def run_cfg1(G, reps=9, chains=16, with_cpu=True):
     if pool:
         pool.shutdown()
     out = dict(workload="EvalMult+Rescale, n=32768, L=4 (60+3x50-bit), dnum=2, K=%d (60-bit), batch %d" % (K, chains),
-               ntt_radix_log2=radix, reps=reps, parity_bit_exact=parity, legs=summarize_legs(times, chains))
+               autotune=ctx.tuning_report(), reps=reps, parity_bit_exact=parity,
+               legs=summarize_legs(times, chains))
     if with_cpu:
         out["cpu_threads"] = nt
         for k in ("reference_graph", "batched_plan_graph"):
@@ -442,7 +443,7 @@ def run_chain(G, stream, n=1 << 15, L=12, dnum=3, batch=16, depth=10, reps=5):
     whole batch, the steps captured as hipGraphs and replayed in order."""
     q, p, roots, K, alpha = native_params(G, n, L, dnum)
     ctx = G.Context(n, q, p, roots, dnum)
-    ctx.autotune(stream)
+    ctx.autotune(stream, batch)
     key = G.Key.uniform(ctx, dnum, 77, stream)
     cts = [[G.Dev(lv * n) for _ in range(2)] for lv in range(L, L - depth - 1, -1) for _ in range(batch)]
     ops = [[G.Dev(lv * n) for _ in range(2)] for lv in range(L, L - depth, -1) for _ in range(batch)]
@@ -483,7 +484,7 @@ def run_rotations(G, stream, n=1 << 16, L=20, dnum=4, count=64, nkeys=16, reps=1
     rng = np.random.default_rng(seed)
     q, p, roots, K, alpha = native_params(G, n, L, dnum)
     ctx = G.Context(n, q, p, roots, dnum)
-    ctx.autotune(stream)
+    ctx.autotune(stream, count)
     keys = [G.Key.uniform(ctx, dnum, 500 + j, stream) for j in range(nkeys)]
     ks = [pow(5, 1 << j, 2 * n) for j in range(nkeys)]
     levels = [int(x) for x in rng.integers(8, L + 1, count)]
@@ -631,6 +632,8 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="skip the chain and rotation workloads")
     ap.add_argument("--ntt-radix", type=int, default=0, choices=(0, 2, 3, 4),
                     help="pin the NTT register radix (log2) instead of autotuning")
+    ap.add_argument("--knobs", default="",
+                    help="pin plan knobs instead of autotuning, e.g. ksi_extra=0,conv_chunk=32,rs_chunk=32")
     ap.add_argument("--serve-seconds", type=float, default=10.0,
                     help="length of the serving trace (0 skips it)")
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="Poisson arrival rate, requests/s")
@@ -655,17 +658,22 @@ def main():
     av = np.concatenate([rng.integers(0, int(m), N, dtype=np.uint64) for _ in range(DNUM) for m in mods])
     bv = np.concatenate([rng.integers(0, int(m), N, dtype=np.uint64) for _ in range(DNUM) for m in mods])
     key = G.Key(ctx, av, bv, DNUM, stream)
-    if args.ntt_radix:
-        # pinned register radix (profiling runs, where per-dispatch overhead skews the autotuner)
-        ctx.set_ntt_radix(args.ntt_radix)
-        radix, ntt_ms = args.ntt_radix, float("nan")
+    pinned = bool(args.ntt_radix or args.knobs)
+    if pinned:
+        # pinned knobs (profiling runs, where per-dispatch overhead skews the autotuner)
+        if args.ntt_radix:
+            ctx.set_ntt_radix(args.ntt_radix)
+        for kv in filter(None, args.knobs.split(",")):
+            k_, v_ = kv.split("=")
+            ctx.set_knob(k_.strip(), int(v_))
+        radix, ntt_ms = ctx.ntt_radix(), float("nan")
     else:
-        radix, ntt_ms = ctx.autotune(stream)   # picks the NTT register radix for this context
-    log("context ready: K=%d alpha=%d ws=%.1f MiB/op, NTT radix 2^%d (%s)"
-        % (K, alpha, ctx.ws_bytes / 2**20, radix,
-           "pinned" if args.ntt_radix else "autotuned, %.3f ms per 96 fwd+inv" % ntt_ms))
-    tuned = {"ntt_radix_log2": radix, "pinned": bool(args.ntt_radix),
-             "ntt_96_fwd_inv_ms": None if args.ntt_radix else round(ntt_ms, 4)}
+        radix, ntt_ms = ctx.autotune(stream, args.batch)   # NTT register radix, then the plan knobs at B
+    knobs = {k_: ctx.knob(k_) for k_ in ("ntt_radix", "ksi_extra", "conv_chunk", "rs_chunk")}
+    log("context ready: K=%d alpha=%d ws=%.1f MiB/op, knobs %s (%s)"
+        % (K, alpha, ctx.ws_bytes / 2**20, knobs, "pinned" if pinned else "autotuned"))
+    tuned = {"shape": {"n": N, "L": L, "dnum": DNUM}, "chosen": knobs, "pinned": pinned,
+             "search": None if pinned else ctx.tuning_report()}
 
     B = args.batch
     res = run_batched(G, ctx, key, B, args.steps, args.warmup, stream)

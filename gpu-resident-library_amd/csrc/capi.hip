@@ -601,6 +601,24 @@ int gf_ctx_autotune(gf_ctx* c, hipStream_t s, uint32_t* r, float* ms) {
         if (ms) *ms = m;
     });
 }
+int gf_ctx_autotune_plan(gf_ctx* c, uint32_t batch, hipStream_t s) {
+    return guard([&] {
+        std::lock_guard<std::mutex> lk(c->mu);
+        c->C->autotune_plan(s, batch);
+        c->cache.clear();
+    });
+}
+const char* gf_ctx_tuning_report(const gf_ctx* c) { return c->C->tune_report.c_str(); }
+int gf_ctx_set_knob(gf_ctx* c, const char* name, uint32_t value) {
+    return guard([&] {
+        std::lock_guard<std::mutex> lk(c->mu);
+        c->C->set_knob(name, value);
+        c->cache.clear();
+    });
+}
+int gf_ctx_get_knob(const gf_ctx* c, const char* name, uint32_t* value) {
+    return guard([&] { *value = c->C->knob(name); });
+}
 int gf_fill_uniform(gf_ctx* c, uint64_t* d, uint32_t level, uint64_t seed, hipStream_t s) {
     return guard([&] {
         if (level < 1 || level > c->C->L) throw std::runtime_error("gpufhe: bad level");

@@ -4,6 +4,7 @@
 #include <cstring>
 #include <map>
 #include <algorithm>
+#include <sstream>
 #include <tuple>
 
 #include "engine.hpp"
@@ -296,6 +297,7 @@ uint32_t Ctx::autotune(hipStream_t s, float* best_ms) {
     hip_check(hipEventCreate(&e1), "event");
     uint32_t best = geom.rmax;
     float bms = 1e30f;
+    float per[5] = {1e30f, 1e30f, 1e30f, 1e30f, 1e30f};
     for (int round = 0; round < 2; ++round)
         for (uint32_t r = 2; r <= 4; ++r) {
             NttGeom g = make_geom(n, r);
@@ -312,14 +314,134 @@ uint32_t Ctx::autotune(hipStream_t s, float* best_ms) {
             hip_check(hipEventSynchronize(e1), "event sync");
             float ms = 0;
             hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+            if (ms < per[r]) per[r] = ms;
             if (ms < bms) { bms = ms; best = r; }
         }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     set_radix(best);
+    {
+        std::ostringstream o;
+        o << "{\"workload_towers\": " << towers << ", \"ms_per_fwd_inv_pass\": {";
+        for (uint32_t r = 2; r <= 4; ++r) o << (r > 2 ? ", " : "") << "\"" << r << "\": " << per[r] / 3;
+        o << "}, \"chosen\": " << best << "}";
+        radix_report = o.str();
+    }
     // reported per 96 towers, forward and inverse
     if (best_ms) *best_ms = bms / 3 * 96.0f / (float)towers;
     return best;
+}
+
+void Ctx::set_knob(const std::string& name, uint32_t v) {
+    if (name == "ntt_radix") {
+        set_radix(v);
+    } else if (name == "ksi_extra") {
+        if (v > 1) throw std::runtime_error("gpufhe: ksi_extra must be 0 or 1");
+        ksi_extra = v;
+    } else if (name == "conv_chunk" || name == "rs_chunk") {
+        if (v < 1 || v > 64) throw std::runtime_error("gpufhe: chunk knobs must be in [1, 64]");
+        (name == "conv_chunk" ? conv_chunk : rs_chunk) = v;
+    } else if (name == "rechunk_wgs") {
+        rechunk_wgs = v;
+    } else {
+        throw std::runtime_error("gpufhe: unknown tuning knob " + name);
+    }
+}
+uint32_t Ctx::knob(const std::string& name) const {
+    if (name == "ntt_radix") return geom.rmax;
+    if (name == "ksi_extra") return ksi_extra;
+    if (name == "conv_chunk") return conv_chunk;
+    if (name == "rs_chunk") return rs_chunk;
+    if (name == "rechunk_wgs") return rechunk_wgs;
+    throw std::runtime_error("gpufhe: unknown tuning knob " + name);
+}
+
+// Coordinate search over the plan knobs on one batch of B EvalMult+Rescale at
+// the top level (uniform key and inputs): for each knob in turn, every
+// candidate builds the whole plan, runs it once, and times the steps the knob
+// shapes (the fused row pass + inner product steps, the conversion steps, the
+// rescale prologue) over 5 runs, twice, interleaved; the knob keeps its
+// fastest candidate before the next knob is searched.  Stage timing keeps
+// the other stages' noise out of the comparison; B should be the batch the
+// plans will run (a chunk width that wins at one batch can lose at another).
+void Ctx::autotune_plan(hipStream_t s, uint32_t B) {
+    const uint32_t l = L;
+    if (B < 1 || B > 256) throw std::runtime_error("gpufhe: tuning batch must be in [1, 256]");
+    std::ostringstream rep;
+    rep << "{\"ntt_radix\": " << (radix_report.empty() ? "null" : radix_report);
+    if (l < 2) {
+        rep << "}";
+        tune_report = rep.str();
+        return;
+    }
+    DevBuf dm(mods.size() * 8);
+    hip_check(hipMemcpy(dm.p, mods.data(), dm.bytes, hipMemcpyHostToDevice), "upload moduli");
+    DevKey key;
+    key.nparts = dnum;
+    key.rows = L + K;
+    key.fullQ = L;
+    const size_t per = (size_t)key.rows * n;
+    key.a.alloc((size_t)dnum * per * 8);
+    key.b.alloc((size_t)dnum * per * 8);
+    for (uint32_t p = 0; p < dnum; ++p) {
+        launch_fill(key.a.as<uint64_t>() + p * per, dm.as<uint64_t>(), key.rows, n, 9001 + 2 * p, s);
+        launch_fill(key.b.as<uint64_t>() + p * per, dm.as<uint64_t>(), key.rows, n, 9002 + 2 * p, s);
+    }
+    const size_t T = (size_t)l * n, To = (size_t)(l - 1) * n, W = ws_words();
+    DevBuf in((size_t)B * 4 * T * 8), out((size_t)B * 2 * To * 8), ws((size_t)B * W * 8);
+    for (uint32_t i = 0; i < 4 * B; ++i) launch_fill(in.as<uint64_t>() + i * T, dm.as<uint64_t>(), l, n, 77 + i, s);
+    hip_check(hipStreamSynchronize(s), "tuning inputs");
+    auto build = [&]() {
+        PlanBuilder b(*this);
+        for (uint32_t i = 0; i < B; ++i) {
+            const uint64_t* x = in.as<uint64_t>() + (size_t)4 * i * T;
+            uint64_t* o = out.as<uint64_t>() + (size_t)2 * i * To;
+            b.evalmult_rescale(l, x, x + T, x + 2 * T, x + 3 * T, o, o + To, key, ws.as<uint64_t>() + i * W);
+        }
+        return b.finish();
+    };
+    hipEvent_t e0, e1;
+    hip_check(hipEventCreate(&e0), "event");
+    hip_check(hipEventCreate(&e1), "event");
+    // the steps of the given kinds, timed together
+    auto time_steps = [&](const Plan& P, std::initializer_list<Step::Kind> kinds) {
+        P.run(s);
+        hip_check(hipEventRecord(e0, s), "event");
+        for (int r = 0; r < 5; ++r)
+            for (size_t i = 0; i < P.steps.size(); ++i)
+                for (Step::Kind k : kinds)
+                    if (P.steps[i].kind == k) P.run_step(i, s);
+        hip_check(hipEventRecord(e1, s), "event");
+        hip_check(hipEventSynchronize(e1), "event sync");
+        float ms = 0;
+        hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+        return ms / 5;
+    };
+    auto tune = [&](const char* name, uint32_t& knob, const std::vector<uint32_t>& cands,
+                    std::initializer_list<Step::Kind> kinds) {
+        std::vector<float> best(cands.size(), 1e30f);
+        for (int round = 0; round < 2; ++round)
+            for (size_t c = 0; c < cands.size(); ++c) {
+                knob = cands[c];
+                auto P = build();
+                best[c] = std::min(best[c], time_steps(*P, kinds));
+            }
+        size_t bi = 0;
+        for (size_t c = 1; c < cands.size(); ++c)
+            if (best[c] < best[bi]) bi = c;
+        knob = cands[bi];
+        rep << ", \"" << name << "\": {\"stage_ms\": {";
+        for (size_t c = 0; c < cands.size(); ++c) rep << (c ? ", " : "") << "\"" << cands[c] << "\": " << best[c];
+        rep << "}, \"chosen\": " << knob << "}";
+    };
+    rep << ", \"batch\": " << B << ", \"level\": " << l;
+    if (fused && !geom.single && n % KSI_TILE == 0) tune("ksi_extra", ksi_extra, {0, 1}, {Step::KSI});
+    tune("conv_chunk", conv_chunk, {8, 16, 32}, {Step::CONV});
+    if (fused) tune("rs_chunk", rs_chunk, {8, 16, 32}, {Step::RSV});
+    rep << "}";
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    tune_report = rep.str();
 }
 
 // ------------------------------------------------------------------ keys --
@@ -613,15 +735,15 @@ void Plan::run_graph(hipStream_t s) {
 // Targets per conversion job: a job loads and splits its source column once
 // for all of its targets, so large chunks save work; a stage with few jobs
 // is split back into small chunks at layout (split_chunks) to fill the chip.
-static const uint32_t CONV_CHUNK = 32, CONV_CHUNK_SMALL = 8;
-static const uint32_t FILL_WORKGROUPS = 8192;   // 32 per CU
+static const uint32_t CONV_CHUNK_SMALL = 8;
 
 // Re-chunk conversion jobs ([j0, j1) target ranges) into CONV_CHUNK_SMALL
-// pieces when the stage would launch fewer than FILL_WORKGROUPS workgroups.
+// pieces when the stage would launch fewer than min_wgs workgroups
+// (Ctx::rechunk_wgs, by default 8192: 32 per CU).
 template <class J>
-static void split_chunks(std::vector<J>& v, uint32_t n) {
+static void split_chunks(std::vector<J>& v, uint32_t n, uint32_t min_wgs) {
     const size_t blocks = (n + PW_THREADS * PW_EPT - 1) / (PW_THREADS * PW_EPT);
-    if (v.empty() || v.size() * blocks >= FILL_WORKGROUPS) return;
+    if (v.empty() || v.size() * blocks >= min_wgs) return;
     std::vector<J> out;
     for (const J& j : v)
         for (uint32_t a = j.j0; a < j.j1; a += CONV_CHUNK_SMALL) {
@@ -692,6 +814,7 @@ void PlanBuilder::ksi_moddown_src(uint32_t l, const uint64_t* d, const DevKey& k
         J.q = C.rc[row].q;
         J.qneg = C.rc[row].qneg;
         J.red = lazy_red_of(J.q);
+        J.d30 = ksi_d30(J.q, J.red);
         J.lazy_out = i < lazy_rows;
         J.pairs = aligned16({J.o0, J.o1});
         for (uint32_t pi = 0; pi < lv.np; ++pi) J.pairs &= aligned16({J.src[pi], J.kb[pi], J.ka[pi]});
@@ -731,9 +854,9 @@ void PlanBuilder::modup(uint32_t l, const uint64_t* d, uint64_t* ws, const uint6
     for (uint32_t pi = 0; pi < lv.np; ++pi) {
         const PartConst& P = lv.parts[pi];
         uint64_t* dp = dig + (size_t)pi * W * n;
-        for (uint32_t j0 = 0; j0 < P.sc; j0 += CONV_CHUNK)
+        for (uint32_t j0 = 0; j0 < P.sc; j0 += C.conv_chunk)
             conv_up_.push_back(ConvJob{x + (size_t)P.st * n, dp, P.d_tab, P.d_tq, P.d_tqneg, P.sp, P.sc,
-                                       j0, j0 + CONV_CHUNK < P.sc ? j0 + CONV_CHUNK : P.sc, C.wide});
+                                       j0, j0 + C.conv_chunk < P.sc ? j0 + C.conv_chunk : P.sc, C.wide});
         for (uint32_t j = 0; j < P.sc; ++j) {
             const uint32_t r = P.compl_rows[j];
             (rowpass_in_ksi ? ntt_diga_ : ntt_dig_)
@@ -816,9 +939,9 @@ void PlanBuilder::moddown_tail(uint32_t l, uint64_t* ws, uint64_t* ba0, uint64_t
     const uint64_t* ts[2] = {t0, t1};
     for (int c = 0; c < 2; ++c) {
         uint64_t* cv = conv + (size_t)c * l * n;
-        for (uint32_t j0 = 0; j0 < l; j0 += CONV_CHUNK)
+        for (uint32_t j0 = 0; j0 < l; j0 += C.conv_chunk)
             conv_dn_.push_back(ConvJob{xp + (size_t)c * K * n, cv, lv.d_tabMD, lv.d_tqMD, lv.d_tqnegMD, K, l,
-                                       j0, j0 + CONV_CHUNK < l ? j0 + CONV_CHUNK : l, C.wide});
+                                       j0, j0 + C.conv_chunk < l ? j0 + C.conv_chunk : l, C.wide});
         for (uint32_t i = 0; i < l; ++i) {
             ntt_conv_.push_back(NttJob{cv + (size_t)i * n, cv + (size_t)i * n, C.rt[i].fw2, nullptr,
                                        C.mods[i], 0, 0});
@@ -860,7 +983,7 @@ void PlanBuilder::evalmult_rescale_fused(uint32_t l, const uint64_t* c0a, const 
         intt_p_.push_back(NttJob{res[c] + (size_t)last * n, dr + (size_t)c * n, C.rt[last].iv2, nullptr,
                                  C.mods[last], C.rt[last].ninv, C.rt[last].ninvp});
         uint64_t* vc = v + (size_t)c * last * n;
-        for (uint32_t j0 = 0; j0 < last; j0 += CONV_CHUNK) {
+        for (uint32_t j0 = 0; j0 < last; j0 += C.rs_chunk) {
             RsJob R{};
             R.xp = xp + (size_t)c * K * n;
             R.dt = dt + (size_t)c * n;
@@ -877,7 +1000,7 @@ void PlanBuilder::evalmult_rescale_fused(uint32_t l, const uint64_t* c0a, const 
             R.K = K;
             R.ntgt = last;
             R.j0 = j0;
-            R.j1 = j0 + CONV_CHUNK < last ? j0 + CONV_CHUNK : last;
+            R.j1 = j0 + C.rs_chunk < last ? j0 + C.rs_chunk : last;
             R.strided = l;
             R.wide = C.wide;
             rs_.push_back(R);
@@ -1165,9 +1288,9 @@ std::unique_ptr<Plan> PlanBuilder::layout(std::vector<char>& blob) {
     group_by(ksi_, [](const KsiJob& j) { return (uintptr_t)j.kb[0]; });
     group_by(ksi_, nxf);
     group_by(inner_, [](const InnerJob& j) { return (uintptr_t)j.kb[0]; });
-    split_chunks(conv_up_, C_.n);
-    split_chunks(conv_dn_, C_.n);
-    split_chunks(rs_, C_.n);
+    split_chunks(conv_up_, C_.n, C_.rechunk_wgs);
+    split_chunks(conv_dn_, C_.n, C_.rechunk_wgs);
+    split_chunks(rs_, C_.n, C_.rechunk_wgs);
     group_by(conv_up_, [](const ConvJob& j) { return (uintptr_t)j.tab; });
     group_by(tensor_, [](const TensorJob& j) { return j.q; });
     auto P = std::make_unique<Plan>();
@@ -1193,7 +1316,7 @@ std::unique_ptr<Plan> PlanBuilder::layout(std::vector<char>& blob) {
     for (size_t i = 0; i < ksi_.size();) {
         size_t e = i;
         while (e < ksi_.size() && nxf(ksi_[e]) == nxf(ksi_[i])) ++e;
-        const uint32_t x = nxf(ksi_[i]), waves = std::max(1u, std::min<uint32_t>(x, KSI_WAVES));
+        const uint32_t x = nxf(ksi_[i]), waves = std::max(1u, std::min<uint32_t>(x + C_.ksi_extra, KSI_WAVES));
         stage(Step::KSI, ksi_.data() + i, e - i, sizeof(KsiJob));
         P->steps.back().arg = x | waves << 8;
         i = e;

@@ -7,6 +7,7 @@
 
 #include <memory>
 #include <mutex>
+#include <string>
 #include <vector>
 
 #include "internal.hpp"
@@ -107,6 +108,24 @@ public:
     // times the NTT variants on a synthetic tower batch and keeps the fastest
     // register radix in geom (plans built afterwards use it); returns it
     uint32_t autotune(hipStream_t s, float* best_ms = nullptr);
+    // Plan knobs, read when a plan is built: one extra wave per fused row pass
+    // + inner product workgroup (it only takes part in the inner product), and
+    // the targets per job of the ModUp conversion and of the rescale prologue.
+    uint32_t ksi_extra = 0, conv_chunk = 32, rs_chunk = 32;
+    // conversion and prologue stages that would launch fewer workgroups than
+    // this are re-split into 8-target jobs, whatever the chunk knobs say
+    uint32_t rechunk_wgs = 8192;
+    // times each plan knob's candidates on a batch of `batch` EvalMult+Rescale
+    // at the top level (the stage shapes plans of that batch run) and keeps
+    // the fastest; the search (every candidate's time) is left in tune_report
+    // as JSON
+    void autotune_plan(hipStream_t s, uint32_t batch);
+    std::string tune_report;
+    // sets or reads a knob by name ("ntt_radix", "ksi_extra", "conv_chunk",
+    // "rs_chunk", "rechunk_wgs")
+    void set_knob(const std::string& name, uint32_t v);
+    uint32_t knob(const std::string& name) const;
+    std::string radix_report;   // the radix search of autotune(), as JSON
     size_t tower_bytes() const { return (size_t)n * 8; }
 };
 

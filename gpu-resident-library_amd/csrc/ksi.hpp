@@ -7,8 +7,10 @@
 // the forward row pass (phase B) of the transformed parts w, w + nwaves, ...
 // (in part order) on LDS region w, w + nwaves, ..., reading the phase-A output
 // straight from HBM, and keeps the evaluation-form digits there; after one
-// block barrier every thread takes KSI_TILE / nthr positions and accumulates
-// sum_p digit_p * key_p for both key components as 128-bit sums.  A launch
+// block barrier every thread (including an optional extra wave that runs no
+// row pass) takes KSI_TILE / nthr positions and accumulates
+// sum_p digit_p * key_p for both key components (30-bit column sums where
+// every value is below 2^60 and there are at most 4 parts, else 128-bit sums).  A launch
 // gives its jobs one wave per transformed part (up to KSI_WAVES), so no wave
 // idles through the row passes.  The transformed digits therefore never
 // travel through HBM.  Parts whose tower i is the input's own (no transform)
@@ -23,7 +25,7 @@ enum {
     KSI_LOG_TILE = 10,
     KSI_TILE = 1 << KSI_LOG_TILE,                    // positions per workgroup
     KSI_PART_WORDS = KSI_TILE + KSI_TILE / 16,       // one part's padded LDS region
-    KSI_WAVES = 4,                                   // largest workgroup, in waves
+    KSI_WAVES = 5,                                   // largest workgroup, in waves
     KSI_THREADS = 64 * KSI_WAVES
 };
 
@@ -44,6 +46,9 @@ struct KsiJob {
     // 1: the outputs only feed the rescale epilogue's lazy Shoup products
     // (any 64-bit input): REDC without its correction, no final reduction
     uint32_t lazy_out;
+    // 1: every digit and key value of the row is below 2^60 (ksi_d30), so the
+    // inner product runs as 30-bit column sums
+    uint32_t d30;
 };
 
 // V consecutive words from word i of a tower (V = 2: one 16-byte access; the
@@ -70,6 +75,112 @@ GF_HD void store_words(GF_GLOBAL uint64_t* base, uint32_t i, const uint64_t* in)
     }
 }
 
+// Digits and keys of a row fit the 30-bit column sums when both are below
+// 2^60: keys are residues (< q); digits are residues too, or with red (the
+// forward transform's unreduced values) below 72 q.
+inline uint32_t ksi_d30(uint64_t q, uint64_t red) {
+    const unsigned __int128 dmax = red ? (unsigned __int128)72 * q : (unsigned __int128)q;
+    return dmax <= ((unsigned __int128)1 << 60) ? 1u : 0u;
+}
+
+// hi:lo of lo + mid 2^30 + hi 2^60 (the column sums of dot30's layout)
+GF_HD U128 cols_to_u128(uint64_t lo, uint64_t mid, uint64_t hi) {
+    U128 T;
+    T.lo = lo + (mid << 30);
+    uint64_t c = T.lo < lo ? 1u : 0u;
+    const uint64_t h = hi << 60;
+    T.lo += h;
+    c += T.lo < h ? 1u : 0u;
+    T.hi = (mid >> 34) + (hi >> 4) + c;
+    return T;
+}
+
+// The inner product of NP parts (NP known at compile time): positions
+// idx .. idx + V - 1 per step; all operands of a step (each part's digit, from
+// LDS or HBM, and both key rows) are loaded before any arithmetic, and the
+// sums are 30-bit column sums (j.d30): per part and component four 32 x 32
+// multiply-adds with no carries.  NP <= 7 keeps every column below 2^64
+// (lo, hi < 7 2^60; mid < 14 2^60).
+template <uint32_t NP, uint32_t V, bool LZ>
+GF_HD void ksi_inner_cols(const KsiJob& j, const uint64_t* lds, uint32_t off, uint32_t tid, uint32_t nthr,
+                          const NttJob& t) {
+    const GF_GLOBAL uint64_t *src[NP], *kb[NP], *ka[NP];
+    uint32_t reg[NP];
+    bool xf[NP];
+#pragma unroll
+    for (uint32_t p = 0; p < NP; ++p) {
+        src[p] = gcast(j.src[p]);
+        kb[p] = gcast(j.kb[p]);
+        ka[p] = gcast(j.ka[p]);
+        xf[p] = (j.xf >> p) & 1;
+        reg[p] = (uint32_t)__builtin_popcount(j.xf & ((1u << p) - 1)) * KSI_PART_WORDS;
+    }
+    const uint64_t q = j.q, qneg = j.qneg;
+    for (uint32_t idx = tid * V; idx < KSI_TILE; idx += nthr * V) {
+        const uint32_t gi = off + idx;
+        uint64_t dv[NP][V], bv[NP][V], av[NP][V];
+#pragma unroll
+        for (uint32_t p = 0; p < NP; ++p) {
+            if (xf[p]) {
+#pragma unroll
+                for (uint32_t v = 0; v < V; ++v) dv[p][v] = lds[reg[p] + padded(idx + v)];
+            } else {
+                load_words<V>(src[p], gi, dv[p]);
+            }
+            load_words<V>(kb[p], gi, bv[p]);
+            load_words<V>(ka[p], gi, av[p]);
+        }
+        uint64_t r0[V], r1[V];
+#pragma unroll
+        for (uint32_t v = 0; v < V; ++v) {
+            uint64_t l0 = 0, m0 = 0, h0 = 0, l1 = 0, m1 = 0, h1 = 0;
+#pragma unroll
+            for (uint32_t p = 0; p < NP; ++p) {
+                uint64_t d = dv[p][v];
+                // LZ: below 72q < 2^60 as is (j.d30); else [0, 8q) -> [0, q)
+                if (!LZ && xf[p]) d = ntt_final<true>(d, t);
+                const uint32_t d0 = split30_lo(d), d1 = split30_hi(d);
+                const uint64_t b = bv[p][v], a = av[p][v];
+                const uint32_t b0 = split30_lo(b), b1 = split30_hi(b), a0 = split30_lo(a), a1 = split30_hi(a);
+                l0 = mad32(d0, b0, l0);
+                m0 = mad32(d0, b1, m0);
+                m0 = mad32(d1, b0, m0);
+                h0 = mad32(d1, b1, h0);
+                l1 = mad32(d0, a0, l1);
+                m1 = mad32(d0, a1, m1);
+                m1 = mad32(d1, a0, m1);
+                h1 = mad32(d1, a1, h1);
+            }
+            const U128 s0 = cols_to_u128(l0, m0, h0), s1 = cols_to_u128(l1, m1, h1);
+            if (j.lazy_out) {
+                r0[v] = redc_lazy(s0, q, qneg);
+                r1[v] = redc_lazy(s1, q, qneg);
+            } else if constexpr (LZ) {
+                r0[v] = redc_red(s0, q, qneg, (uint32_t)j.red);
+                r1[v] = redc_red(s1, q, qneg, (uint32_t)j.red);
+            } else {
+                r0[v] = redc(s0, q, qneg);
+                r1[v] = redc(s1, q, qneg);
+            }
+        }
+        store_words<V>(gcast(j.o0), gi, r0);
+        store_words<V>(gcast(j.o1), gi, r1);
+    }
+}
+
+// ksi_inner_cols for the job's part count, when it has an instantiation
+template <uint32_t V, bool LZ>
+GF_HD bool ksi_inner_cols_np(const KsiJob& j, const uint64_t* lds, uint32_t off, uint32_t tid, uint32_t nthr,
+                             const NttJob& t) {
+    switch (j.np) {
+        case 1: ksi_inner_cols<1, V, LZ>(j, lds, off, tid, nthr, t); return true;
+        case 2: ksi_inner_cols<2, V, LZ>(j, lds, off, tid, nthr, t); return true;
+        case 3: ksi_inner_cols<3, V, LZ>(j, lds, off, tid, nthr, t); return true;
+        case 4: ksi_inner_cols<4, V, LZ>(j, lds, off, tid, nthr, t); return true;
+        default: return false;
+    }
+}
+
 // LZ = (j.red != 0).  The wave-level sync wsync separates the row passes of
 // one wave's region; bsync is the workgroup barrier before the inner product.
 template <uint32_t S2, uint32_t RM, bool LZ, class WSync, class BSync>
@@ -93,6 +204,11 @@ GF_HD void ksi_body(const KsiJob& j, uint32_t N1, uint32_t tile, uint64_t* lds, 
         ++k;
     }
     bsync();
+    // up to 4 parts with every value below 2^60: column sums, loads first
+    if (j.d30) {
+        if (j.pairs ? ksi_inner_cols_np<2, LZ>(j, lds, off, tid, nthr, t) : ksi_inner_cols_np<1, LZ>(j, lds, off, tid, nthr, t))
+            return;
+    }
     // positions idx .. idx + V - 1 per step (V = 2: one 16-byte access per operand)
     auto inner = [&](auto Vc) {
         constexpr uint32_t V = decltype(Vc)::value;

@@ -101,6 +101,10 @@ _capi = {
     "gf_ctx_get_ntt_radix": (u32, [vp]),
     "gf_ctx_autotune": (C.c_int, [vp, vp, u32p, C.POINTER(C.c_float)]),
     "gf_ctx_set_fusion": (C.c_int, [vp, C.c_int]),
+    "gf_ctx_autotune_plan": (C.c_int, [vp, u32, vp]),
+    "gf_ctx_tuning_report": (C.c_char_p, [vp]),
+    "gf_ctx_set_knob": (C.c_int, [vp, C.c_char_p, u32]),
+    "gf_ctx_get_knob": (C.c_int, [vp, C.c_char_p, u32p]),
     "gf_event_create": (C.c_int, [vpp]),
     "gf_event_destroy": (C.c_int, [vp]),
     "gf_event_record": (C.c_int, [vp, vp]),
@@ -234,10 +238,28 @@ class Context:
         """single-op plans held in the context's cache"""
         return lib.gf_ctx_cached_plans(self.h)
 
-    def autotune(self, stream):
+    def autotune(self, stream, batch=32):
+        """NTT register radix (synthetic tower batch), then the plan knobs on
+        an EvalMult+Rescale batch of `batch` ops at the top level (batch=0
+        skips them).  Returns (radix, ms per 96 forward + inverse transforms)."""
         r, ms = C.c_uint32(), C.c_float()
         check(lib.gf_ctx_autotune(self.h, stream.s, C.byref(r), C.byref(ms)))
+        if batch:
+            check(lib.gf_ctx_autotune_plan(self.h, batch, stream.s))
         return r.value, ms.value
+
+    def tuning_report(self):
+        import json
+        rep = lib.gf_ctx_tuning_report(self.h).decode()
+        return json.loads(rep) if rep else None
+
+    def set_knob(self, name, value):
+        check(lib.gf_ctx_set_knob(self.h, name.encode(), int(value)))
+
+    def knob(self, name):
+        v = C.c_uint32()
+        check(lib.gf_ctx_get_knob(self.h, name.encode(), C.byref(v)))
+        return v.value
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -268,6 +268,19 @@ int gf_plan_run_step_kernel(gf_plan* p, uint32_t i, uint32_t which, hipStream_t 
 int gf_ctx_set_ntt_radix(gf_ctx* c, uint32_t rmax);
 uint32_t gf_ctx_get_ntt_radix(const gf_ctx* c);
 int gf_ctx_autotune(gf_ctx* c, hipStream_t s, uint32_t* rmax_out, float* ms_out);
+/* Plan knobs, searched on a batch of `batch` (1..256) EvalMult+Rescale at the
+ * context's top level, the batch plans will run (each candidate builds the
+ * plan and times the stages the knob shapes): "ksi_extra" (one
+ * extra wave in the fused row pass + inner product workgroups, 0/1),
+ * "conv_chunk" and "rs_chunk" (targets per ModUp-conversion / rescale-prologue
+ * job).  The report is JSON with every candidate's time, the radix search of
+ * gf_ctx_autotune included; valid until the next tuning call.  Knobs (and
+ * "ntt_radix", "rechunk_wgs") can be set by name; plans built afterwards use
+ * them, and results are identical for every setting. */
+int gf_ctx_autotune_plan(gf_ctx* c, uint32_t batch, hipStream_t s);
+const char* gf_ctx_tuning_report(const gf_ctx* c);
+int gf_ctx_set_knob(gf_ctx* c, const char* name, uint32_t value);
+int gf_ctx_get_knob(const gf_ctx* c, const char* name, uint32_t* value);
 /* 1 (default): EvalMult+Rescale merges ModDown and the rescale through NTT
  * linearity (one forward transform per surviving tower); 0: the reference's
  * stage sequence (ModDown NTTs, combine, INTT/switch/NTT/fuse).  Results are

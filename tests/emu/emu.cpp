@@ -115,6 +115,7 @@ void emu_ksi_t(uint32_t n, uint64_t q, const uint64_t* t2, uint32_t np, uint32_t
     J.q = q;
     J.qneg = host_qneg(q);
     J.red = lazy_red_of(q);
+    J.d30 = ksi_d30(q, J.red);
     J.o0 = o0;
     J.o1 = o1;
     J.pairs = pairs;

@@ -229,10 +229,13 @@ def test_fused_rowpass_inner_product(emu, logn, np_, xf):
     column pass and then the row pass inside the fused kernel body, the other
     parts are evaluation-form digits as given; the result equals
     sum_p digit_p * key_p * 2^-64 mod q with digit_p the oracle's forward
-    transform, for a modulus below 2^57 (unreduced digits) and one above
-    (reduced digits), every register radix."""
+    transform, for moduli below 2^57 (unreduced digits) and above (reduced
+    digits), with digits and keys below 2^60 (column sums) and above (128-bit
+    sums), every register radix."""
     n = 1 << logn
-    for bits in (50, 59):
+    # 50 bits: unreduced digits, column sums; 55: unreduced digits above 2^60
+    # (128-bit sums); 59: reduced digits, column sums; 61: 128-bit sums
+    for bits in (50, 55, 59, 61):
         q = int(O.native_primes(1, bits, n)[0])
         psi = int(O.lib.or_native_root(n, q))
         fr, fp, _, _, _, _ = O.root_tables(n, q, psi)

@@ -338,6 +338,44 @@ def test_batch_plan_uneven_levels_and_graph(G):
         assert np.array_equal(outs[i][1].download(), want[i][1]), ("per job", i)
 
 
+@pytest.mark.parametrize("ksi_extra,conv_chunk,rs_chunk", [(0, 8, 8), (1, 16, 32), (1, 32, 16), (0, 5, 3)])
+def test_plan_knobs_keep_results(G, ksi_extra, conv_chunk, rs_chunk):
+    """Every setting of the autotuner's plan knobs (extra inner-product wave,
+    conversion and prologue chunk widths, re-chunking off so the widths take
+    effect) gives the oracle's results, on a batch of uneven levels (parts of
+    3 and 2 towers; ragged chunks with widths 5 and 3); the autotuner's own
+    search leaves a report naming its choices."""
+    n, L, dnum = 1 << 13, 6, 3
+    P, ctx, k, av, bv = setup(G, n, L, dnum, seed=77)   # a context of its own: the knobs stay here
+    for name, v in (("ksi_extra", ksi_extra), ("conv_chunk", conv_chunk), ("rs_chunk", rs_chunk),
+                    ("rechunk_wgs", 0)):
+        ctx.set_knob(name, v)
+        assert ctx.knob(name) == v
+    rng = np.random.default_rng(11 + conv_chunk)
+    levels = [6, 4, 5, 2]
+    ins = [[G.Dev.of(rand_ct(P, lv, rng)) for _ in range(4)] for lv in levels]
+    outs = [[G.Dev((lv - 1) * n), G.Dev((lv - 1) * n)] for lv in levels]
+    wss = [ctx.workspace() for _ in levels]
+    col = lambda j: [x[j].ptr for x in ins]  # noqa: E731
+    plan = G.plan_evalmult_rescale(ctx, k, levels, col(0), col(1), col(2), col(3),
+                                   [o[0].ptr for o in outs], [o[1].ptr for o in outs], [w.ptr for w in wss])
+    s = G.Stream()
+    plan.run_graph(s)
+    s.sync()
+    for i, lv in enumerate(levels):
+        w0, w1 = P.evalmult_rescale(lv, *[x.download() for x in ins[i]], av, bv)
+        assert np.array_equal(outs[i][0].download(), w0), (i, lv)
+        assert np.array_equal(outs[i][1].download(), w1), (i, lv)
+    with pytest.raises(G.GpuFheError, match="unknown tuning knob"):
+        ctx.set_knob("no_such_knob", 1)
+    if ksi_extra == 0 and conv_chunk == 8:
+        ctx.set_knob("rechunk_wgs", 8192)
+        ctx.autotune(s)
+        rep = ctx.tuning_report()
+        for name in ("ntt_radix", "ksi_extra", "conv_chunk", "rs_chunk"):
+            assert rep[name]["chosen"] == ctx.knob(name), name
+
+
 def test_rotate_plan_multi_keys(G):
     n, L, dnum = 1 << 12, 5, 5
     P = make_params(n, L, dnum)

@@ -213,6 +213,23 @@ gf_key* gf_key_generate(gf_ctx* c, const uint64_t* d_sold, const uint64_t* d_pka
         return k;
     });
 }
+gf_key* gf_key_generate_ctr(gf_ctx* c, const uint64_t* d_sold, const uint64_t* d_pka, const uint64_t* d_pkb,
+                            uint64_t ns, double sigma, uint64_t seed, hipStream_t s) {
+    return guard_ptr<gf_key>([&] {
+        if (!(sigma > 0.0) || sigma > 20.0) throw std::runtime_error("gpufhe: sigma must be in (0, 20]");
+        auto k = std::make_unique<gf_key>();
+        k->ctx = c;
+        k->K = generate_key(*c->C, d_sold, d_pka, d_pkb, ns, sigma, seed, s, true);
+        return k.release();
+    });
+}
+int gf_keygen(gf_ctx* c, uint64_t seed, double sigma, uint64_t ns, uint64_t* d_s, uint64_t* d_pka, uint64_t* d_pkb,
+              hipStream_t s) {
+    return guard([&] {
+        if (!(sigma > 0.0) || sigma > 20.0) throw std::runtime_error("gpufhe: sigma must be in (0, 20]");
+        keygen_device(*c->C, seed, sigma, ns, d_s, d_pka, d_pkb, s);
+    });
+}
 int gf_key_download(const gf_key* k, uint64_t* h_av, uint64_t* h_bv, hipStream_t s) {
     return guard([&] { download_key(*k->ctx->C, *k->K, h_av, h_bv, s); });
 }

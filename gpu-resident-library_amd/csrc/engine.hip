@@ -7,6 +7,7 @@
 #include <sstream>
 #include <tuple>
 
+#include "ctr_sampler.hpp"
 #include "engine.hpp"
 #include "sampler.hpp"
 
@@ -484,11 +485,68 @@ std::unique_ptr<DevKey> upload_key(const Ctx& C, const uint64_t* av, const uint6
 // reference's order, embedded into every Q and P tower, transformed and
 // combined with the public key and sOld on the device, and stored in
 // Montgomery form.  sold, pka, pkb: device arrays [L+K][n], evaluation form.
+// The counter sampler's Gaussian table for sigma in device memory
+static DevBuf gauss_table_dev(double sigma, uint32_t& K, hipStream_t s) {
+    const GaussSampler gs(sigma);
+    K = (uint32_t)gs.K();
+    DevBuf T(gs.table().size() * 8);
+    hip_check(hipMemcpyAsync(T.p, gs.table().data(), T.bytes, hipMemcpyHostToDevice, s), "upload Gaussian table");
+    hip_check(hipStreamSynchronize(s), "upload Gaussian table");
+    return T;
+}
+static uint64_t floor_2_64_over(uint64_t q) {
+    if (q <= (1ull << 32)) throw std::runtime_error("gpufhe: the counter sampler needs moduli above 2^32");
+    return (uint64_t)(((unsigned __int128)1 << 64) / q);
+}
+
+// Public key generation on the device (src/keygen.cpp:55-112) with the
+// counter sampler: s ternary (domain CTR_SECRET), e Gaussian (CTR_ERROR), a of
+// tower t uniform (CTR_UNIFORM + t, taken as evaluation form), over all L + K
+// towers; s and e embedded and transformed, b = ns e - a s.
+void keygen_device(const Ctx& C, uint64_t seed, double sigma, uint64_t ns, uint64_t* s_out, uint64_t* a_out,
+                   uint64_t* b_out, hipStream_t st) {
+    const uint32_t n = C.n, W = C.L + C.K;
+    uint32_t K = 0;
+    DevBuf T = gauss_table_dev(sigma, K, st);
+    DevBuf coef((size_t)2 * n * 8), et((size_t)W * n * 8);
+    int64_t* sc = coef.as<int64_t>();
+    int64_t* ec = sc + n;
+    std::vector<SampleJob> sj{SampleJob{sc, nullptr, 0, 0, CTR_SECRET, 0}, SampleJob{ec, nullptr, 0, 0, CTR_ERROR, 1}};
+    std::vector<EmbedJob> ej;
+    std::vector<NttJob> nj;
+    std::vector<PkJob> pj;
+    for (uint32_t t = 0; t < W; ++t) {
+        const size_t o = (size_t)t * n;
+        const uint64_t q = C.mods[t];
+        sj.push_back(SampleJob{nullptr, a_out + o, q, floor_2_64_over(q), CTR_UNIFORM + t, 2});
+        ej.push_back(EmbedJob{sc, s_out + o, q});
+        ej.push_back(EmbedJob{ec, et.as<uint64_t>() + o, q});
+        for (uint64_t* x : {s_out + o, et.as<uint64_t>() + o}) {
+            NttJob J{x, x, C.rt[t].fw2, nullptr, q, 0, 0};
+            J.twr = C.rtr[t].fwd;
+            nj.push_back(J);
+        }
+        const uint64_t nsq = ns % q;
+        pj.push_back(PkJob{a_out + o, s_out + o, et.as<uint64_t>() + o, b_out + o, q, C.rc[t].qneg, C.rc[t].r2, nsq,
+                           host_precon(nsq, q)});
+    }
+    DevBuf dsj(sj.size() * sizeof(SampleJob)), dej(ej.size() * sizeof(EmbedJob)), dnj(nj.size() * sizeof(NttJob)),
+        dpj(pj.size() * sizeof(PkJob));
+    hip_check(hipMemcpyAsync(dsj.p, sj.data(), dsj.bytes, hipMemcpyHostToDevice, st), "upload jobs");
+    hip_check(hipMemcpyAsync(dej.p, ej.data(), dej.bytes, hipMemcpyHostToDevice, st), "upload jobs");
+    hip_check(hipMemcpyAsync(dnj.p, nj.data(), dnj.bytes, hipMemcpyHostToDevice, st), "upload jobs");
+    hip_check(hipMemcpyAsync(dpj.p, pj.data(), dpj.bytes, hipMemcpyHostToDevice, st), "upload jobs");
+    launch_sample(dsj.as<SampleJob>(), (uint32_t)sj.size(), n, seed, T.as<uint64_t>(), K, st);
+    launch_embed(dej.as<EmbedJob>(), (uint32_t)ej.size(), n, st);
+    launch_ntt(dnj.as<NttJob>(), (uint32_t)nj.size(), C.geom, true, st);
+    launch_pkgen(dpj.as<PkJob>(), (uint32_t)pj.size(), n, st);
+    hip_check(hipGetLastError(), "key generation launch");
+    hip_check(hipStreamSynchronize(st), "key generation");
+}
+
 std::unique_ptr<DevKey> generate_key(const Ctx& C, const uint64_t* sold, const uint64_t* pka, const uint64_t* pkb,
-                                     uint64_t ns, double sigma, uint64_t seed, hipStream_t s) {
+                                     uint64_t ns, double sigma, uint64_t seed, hipStream_t s, bool ctr) {
     const uint32_t n = C.n, W = C.L + C.K, parts = C.dnum;
-    std::vector<int64_t> u, e0, e1;
-    sample_switch_key(n, parts, sigma, seed, u, e0, e1);
     auto k = std::make_unique<DevKey>();
     k->nparts = parts;
     k->rows = W;
@@ -498,13 +556,33 @@ std::unique_ptr<DevKey> generate_key(const Ctx& C, const uint64_t* sold, const u
     k->b.alloc((size_t)parts * per * 8);
     // small polynomials (3 per part), then their towers [part][3][W][n]
     DevBuf coef((size_t)parts * 3 * n * 8), tow((size_t)parts * 3 * per * 8);
-    std::vector<int64_t> hc((size_t)parts * 3 * n);
-    for (uint32_t p = 0; p < parts; ++p) {
-        std::copy(u.begin() + (size_t)p * n, u.begin() + (size_t)(p + 1) * n, hc.begin() + (size_t)(3 * p) * n);
-        std::copy(e0.begin() + (size_t)p * n, e0.begin() + (size_t)(p + 1) * n, hc.begin() + (size_t)(3 * p + 1) * n);
-        std::copy(e1.begin() + (size_t)p * n, e1.begin() + (size_t)(p + 1) * n, hc.begin() + (size_t)(3 * p + 2) * n);
+    DevBuf T, dsj;
+    if (ctr) {
+        // drawn on the device: part p's u (ternary), e0, e1 (Gaussian) in
+        // domains CTR_KEY + 3p + {0, 1, 2}
+        uint32_t K = 0;
+        T = gauss_table_dev(sigma, K, s);
+        std::vector<SampleJob> sj;
+        for (uint32_t p = 0; p < parts; ++p)
+            for (uint32_t v = 0; v < 3; ++v)
+                sj.push_back(SampleJob{coef.as<int64_t>() + (size_t)(3 * p + v) * n, nullptr, 0, 0, CTR_KEY + 3 * p + v,
+                                       v == 0 ? 0u : 1u});
+        dsj.alloc(sj.size() * sizeof(SampleJob));
+        hip_check(hipMemcpyAsync(dsj.p, sj.data(), dsj.bytes, hipMemcpyHostToDevice, s), "upload jobs");
+        launch_sample(dsj.as<SampleJob>(), (uint32_t)sj.size(), n, seed, T.as<uint64_t>(), K, s);
+    } else {
+        // drawn on the host in the reference's order (sampler.hpp)
+        std::vector<int64_t> u, e0, e1;
+        sample_switch_key(n, parts, sigma, seed, u, e0, e1);
+        std::vector<int64_t> hc((size_t)parts * 3 * n);
+        for (uint32_t p = 0; p < parts; ++p) {
+            std::copy(u.begin() + (size_t)p * n, u.begin() + (size_t)(p + 1) * n, hc.begin() + (size_t)(3 * p) * n);
+            std::copy(e0.begin() + (size_t)p * n, e0.begin() + (size_t)(p + 1) * n, hc.begin() + (size_t)(3 * p + 1) * n);
+            std::copy(e1.begin() + (size_t)p * n, e1.begin() + (size_t)(p + 1) * n, hc.begin() + (size_t)(3 * p + 2) * n);
+        }
+        hip_check(hipMemcpyAsync(coef.p, hc.data(), coef.bytes, hipMemcpyHostToDevice, s), "upload samples");
+        hip_check(hipStreamSynchronize(s), "upload samples");   // hc is pageable and goes out of scope
     }
-    hip_check(hipMemcpyAsync(coef.p, hc.data(), coef.bytes, hipMemcpyHostToDevice, s), "upload samples");
     std::vector<EmbedJob> ej;
     std::vector<NttJob> nj;
     std::vector<EkJob> kj;

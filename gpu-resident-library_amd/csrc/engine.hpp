@@ -137,8 +137,14 @@ struct DevKey {
 std::unique_ptr<DevKey> upload_key(const Ctx& C, const uint64_t* av, const uint64_t* bv,
                                    uint32_t nparts, uint32_t rows, uint32_t fullQ,
                                    hipStream_t s);
+// ctr: u, e0, e1 drawn on the device by the counter sampler (ctr_sampler.hpp),
+// else on the host in the reference's order (sampler.hpp)
 std::unique_ptr<DevKey> generate_key(const Ctx& C, const uint64_t* sold, const uint64_t* pka, const uint64_t* pkb,
-                                     uint64_t ns, double sigma, uint64_t seed, hipStream_t s);
+                                     uint64_t ns, double sigma, uint64_t seed, hipStream_t s, bool ctr = false);
+// public key on the device with the counter sampler: s (evaluation form), a,
+// b = ns e - a s over all L + K towers of the context ([L + K][n] each)
+void keygen_device(const Ctx& C, uint64_t seed, double sigma, uint64_t ns, uint64_t* s_out, uint64_t* a_out,
+                   uint64_t* b_out, hipStream_t st);
 void download_key(const Ctx& C, const DevKey& k, uint64_t* av, uint64_t* bv, hipStream_t s);
 // encryption of the coefficient vector h_m (n signed values) under the public
 // key (pka, pkb: device [level][n], evaluation form) into device c0, c1

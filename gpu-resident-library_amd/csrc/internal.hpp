@@ -148,6 +148,22 @@ struct EkJob {
     uint32_t plain;
 };
 
+// one polynomial drawn by the counter sampler (ctr_sampler.hpp): kind 0
+// ternary and 1 Gaussian into signed coefficients `coef`; kind 2 uniform mod q
+// into the residues `res` (red = floor(2^64 / q), q > 2^32)
+struct SampleJob {
+    int64_t* coef;
+    uint64_t* res;
+    uint64_t q, red;
+    uint32_t domain, kind;
+};
+// a public key's b = ns e - a s (mod q), every operand in evaluation form
+struct PkJob {
+    const uint64_t *a, *s, *e;
+    uint64_t* b;
+    uint64_t q, qneg, r2, ns, nsp;
+};
+
 // out = c0 + c1 s (mod q): the phase of a decryption before its inverse
 // transform (src/encdec.cpp:68-81)
 struct DecJob {
@@ -209,6 +225,10 @@ void launch_add(const AddJob* d, uint32_t nj, uint32_t n, hipStream_t s);
 void launch_perm(const PermJob* d, uint32_t nj, uint32_t n, uint32_t logn, hipStream_t s);
 void launch_perm_one(const PermJob& j, uint32_t n, uint32_t logn, hipStream_t s);
 void launch_copy(const CopyJob* d, uint32_t nj, uint32_t n, hipStream_t s);
+// T: the Gaussian table's K thresholds (device memory)
+void launch_sample(const SampleJob* d, uint32_t nj, uint32_t n, uint64_t seed, const uint64_t* T, uint32_t K,
+                   hipStream_t s);
+void launch_pkgen(const PkJob* d, uint32_t nj, uint32_t n, hipStream_t s);
 void launch_fill(uint64_t* d, const uint64_t* d_mods, uint32_t towers, uint32_t n, uint64_t seed,
                  hipStream_t s);
 void launch_rs(const RsJob* d, uint32_t nj, uint32_t n, hipStream_t s);

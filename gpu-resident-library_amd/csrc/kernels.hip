@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include "../../include/gpufhe_kernels.h"
+#include "ctr_sampler.hpp"
 #include "ntt_kernels.hpp"
 
 namespace gf {
@@ -325,6 +326,28 @@ __global__ __launch_bounds__(PW_THREADS) void k_embed(const EmbedJob* __restrict
     }
 }
 
+// The counter sampler (ctr_sampler.hpp) over one polynomial per job: every
+// coefficient drawn on its own; the Gaussian table is read through the
+// constant address space (all lanes read the same threshold).
+__global__ __launch_bounds__(PW_THREADS) void k_sample(const SampleJob* __restrict__ jobs, uint32_t off, uint32_t n,
+                                                       uint64_t seed, const uint64_t* __restrict__ T, uint32_t K) {
+    const SampleJob j = jobs[off + blockIdx.y];
+    const cu64p tab = (cu64p)T;
+    PW_FOR(n) {
+        const uint64_t r = ctr_draw(seed, j.domain, i);
+        if (j.kind == 0) j.coef[i] = ctr_ternary(r);
+        else if (j.kind == 1) j.coef[i] = ctr_gauss(r, K, [&](uint32_t x) { return tab[x]; });
+        else j.res[i] = reduce_red(r, j.q, (uint32_t)j.red);
+    }
+}
+
+__global__ __launch_bounds__(PW_THREADS) void k_pkgen(const PkJob* __restrict__ jobs, uint32_t off, uint32_t n) {
+    const PkJob j = jobs[off + blockIdx.y];
+    PW_FOR(n) {
+        j.b[i] = submod(shoup(j.e[i], j.ns, j.nsp, j.q), mulmod(j.a[i], j.s[i], j.q, j.qneg, j.r2), j.q);
+    }
+}
+
 __global__ __launch_bounds__(PW_THREADS) void k_ekgen(const EkJob* __restrict__ jobs,
                                                       uint32_t off, uint32_t n) {
     const EkJob j = jobs[off + blockIdx.y];
@@ -430,6 +453,15 @@ GF_PW_LAUNCH(launch_copy, k_copy, CopyJob)
 GF_PW_LAUNCH(launch_rs, k_rs, RsJob)
 GF_PW_LAUNCH(launch_embed, k_embed, EmbedJob)
 GF_PW_LAUNCH(launch_ekgen, k_ekgen, EkJob)
+GF_PW_LAUNCH(launch_pkgen, k_pkgen, PkJob)
+
+void launch_sample(const SampleJob* d, uint32_t nj, uint32_t n, uint64_t seed, const uint64_t* T, uint32_t K,
+                   hipStream_t s) {
+    for (uint32_t off = 0; off < nj; off += MAXY) {
+        const uint32_t cnt = nj - off < MAXY ? nj - off : MAXY;
+        k_sample<<<pw_grid(n, cnt), PW_THREADS, 0, s>>>(d, off, n, seed, T, K);
+    }
+}
 GF_PW_LAUNCH(launch_dec, k_dec, DecJob)
 GF_PW_LAUNCH(launch_ptacc, k_ptacc, PtAccJob)
 

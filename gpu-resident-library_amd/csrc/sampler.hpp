@@ -40,6 +40,10 @@ public:
         return (g() & 1) ? -(int64_t)k : (int64_t)k;
     }
 
+    // the thresholds T[0..K] (the counter sampler uses T[0..K-1])
+    const std::vector<uint64_t>& table() const { return T_; }
+    int K() const { return K_; }
+
 private:
     int K_ = 0;
     std::vector<uint64_t> T_;

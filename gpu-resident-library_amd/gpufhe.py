@@ -53,6 +53,8 @@ _capi = {
     "gf_key_create_uniform": (vp, [vp, u32, u64, vp]),
     "gf_key_generate": (vp, [vp, vp, vp, vp, u64, C.c_double, u64, vp]),
     "gf_key_download": (C.c_int, [vp, u64p, u64p, vp]),
+    "gf_key_generate_ctr": (vp, [vp, vp, vp, vp, u64, C.c_double, u64, vp]),
+    "gf_keygen": (C.c_int, [vp, u64, C.c_double, u64, vp, vp, vp, vp]),
     "gf_encrypt": (C.c_int, [vp, u32, vp, vp, vp, u64, C.c_double, u64, vp, vp, vp]),
     "gf_decrypt": (C.c_int, [vp, vp, vp, vp, vp, vp]),
     "gf_key_shape": (None, [vp, u32p, u32p, u32p]),
@@ -285,6 +287,17 @@ class Key:
             raise GpuFheError(lib.gf_last_error().decode())
         return k
 
+    @classmethod
+    def generate_ctr(cls, ctx, d_sold, d_pka, d_pkb, seed, ns=1, sigma=3.19, stream=None):
+        """gf_key_generate with u, e0, e1 drawn on the device (counter sampler)."""
+        k = cls.__new__(cls)
+        k.ctx = ctx
+        k.h = lib.gf_key_generate_ctr(ctx.h, d_sold.ptr, d_pka.ptr, d_pkb.ptr, ns, sigma, seed,
+                                      stream.s if stream else None)
+        if not k.h:
+            raise GpuFheError(lib.gf_last_error().decode())
+        return k
+
     def download(self, stream=None):
         """(av, bv) in plain form, [nparts][rows][n] flattened."""
         p, r, f = C.c_uint32(), C.c_uint32(), C.c_uint32()
@@ -438,6 +451,15 @@ class RefSequence:
     def outputs(self):
         w = (self.level - 1) * self.ctx.n
         return self.r[0].download(w), self.r[1].download(w)
+
+
+def keygen(ctx, seed, sigma=3.19, ns=1, stream=None):
+    """Public key generated on the device: (s, pkA, pkB) as device buffers of
+    [L+K][n] words, evaluation form."""
+    w = (ctx.L + ctx.K) * ctx.n
+    s, a, b = Dev(w), Dev(w), Dev(w)
+    check(lib.gf_keygen(ctx.h, seed, sigma, ns, s.ptr, a.ptr, b.ptr, stream.s if stream else None))
+    return s, a, b
 
 
 def plan_evalmult_rescale(ctx, key, levels, c0a, c1a, c0b, c1b, o0, o1, ws):

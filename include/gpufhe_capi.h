@@ -98,6 +98,22 @@ gf_key* gf_key_create_uniform(gf_ctx* c, uint32_t nparts, uint64_t seed, hipStre
  * (sigma), so the key equals evalkeygen_host_sold's for the same inputs. */
 gf_key* gf_key_generate(gf_ctx* c, const uint64_t* d_sold, const uint64_t* d_pka, const uint64_t* d_pkb,
                         uint64_t ns, double sigma, uint64_t seed, hipStream_t s);
+/* The same switching key with u, e0, e1 drawn on the device by the counter
+ * sampler: draw(seed, domain, i) = SplitMix64's output function of
+ * seed + (domain 2^32 + i + 1) 0x9E3779B97F4A7C15; ternary = draw mod 3 - 1,
+ * Gaussian = the cumulative table's index of draw >> 1, negated when draw & 1;
+ * part p uses domains 0x20000 + 3p (u), + 1 (e0), + 2 (e1).  Nothing is
+ * sampled or computed on the host. */
+gf_key* gf_key_generate_ctr(gf_ctx* c, const uint64_t* d_sold, const uint64_t* d_pka, const uint64_t* d_pkb,
+                            uint64_t ns, double sigma, uint64_t seed, hipStream_t s);
+/* Public key generation on the device (src/keygen.cpp:55-112, the reference's
+ * keygen_host) with the counter sampler: s ternary (domain 1), e Gaussian
+ * (domain 2), a of tower t uniform mod q_t (domain 0x10000 + t, taken as
+ * evaluation form), b = ns e - a s, over all L + K towers of the context.
+ * d_s, d_pka, d_pkb: device arrays [L+K][n], written in evaluation form;
+ * returns after the key is complete.  Moduli must exceed 2^32. */
+int gf_keygen(gf_ctx* c, uint64_t seed, double sigma, uint64_t ns, uint64_t* d_s, uint64_t* d_pka,
+              uint64_t* d_pkb, hipStream_t s);
 /* rows of a resident key in plain form, h_av/h_bv [nparts][rows][n] */
 int gf_key_download(const gf_key* k, uint64_t* h_av, uint64_t* h_bv, hipStream_t s);
 /* Encryption into resident buffers (encrypt_host of src/encdec.cpp:41-64 with

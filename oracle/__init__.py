@@ -123,6 +123,13 @@ _sig = {
     "or_encrypt": (None, [C.c_uint32, u64p, u64p, C.c_uint32, i64p, u64p, u64p, C.c_uint64,
                           C.c_double, C.c_uint64, u64p, u64p]),
     "or_decrypt": (None, [C.c_uint32, C.c_uint64, C.c_uint64, u64p, u64p, u64p, i64p]),
+    "or_ctr_draw": (C.c_uint64, [C.c_uint64, C.c_uint32, C.c_uint32]),
+    "or_ctr_gauss": (C.c_int64, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_double]),
+    "or_keygen_ctr": (None, [C.c_uint32, u64p, u64p, C.c_uint32, C.c_uint64, C.c_double, C.c_uint64,
+                             u64p, u64p, u64p]),
+    "or_evalkeygen_ctr": (None, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, u64p,
+                                 u64p, u64p, u64p, u64p, C.c_uint64, C.c_double, C.c_uint64, u64p,
+                                 u64p]),
 }
 for _name, (_res, _args) in _sig.items():
     _f = getattr(lib, _name)
@@ -328,6 +335,26 @@ def evalkeygen_sold(n, sizeQ, sizeP, alpha, numPart, modQP, rootQP, s_old, pkA, 
     bv = np.zeros(numPart * W * n, np.uint64)
     lib.or_evalkeygen_sold(n, sizeQ, sizeP, alpha, numPart, P(U(modQP)), P(U(rootQP)),
                            P(U(s_old)), P(U(pkA)), P(U(pkB)), ns, sigma, seed, P(av), P(bv))
+    return av, bv
+
+
+def keygen_ctr(n, mods, roots, ns=1, sigma=3.19, seed=101):
+    """or_keygen with the counter sampler (the library's device key generation)."""
+    mods = U(mods)
+    roots = U(roots)
+    T = len(mods) * n
+    s, a, b = (np.zeros(T, np.uint64) for _ in range(3))
+    lib.or_keygen_ctr(n, P(mods), P(roots), len(mods), ns, sigma, seed, P(s), P(a), P(b))
+    return s, a, b
+
+
+def evalkeygen_ctr(n, sizeQ, sizeP, alpha, numPart, modQP, rootQP, s_old, pkA, pkB, ns=1, sigma=3.19,
+                   seed=202):
+    W = sizeQ + sizeP
+    av = np.zeros(numPart * W * n, np.uint64)
+    bv = np.zeros(numPart * W * n, np.uint64)
+    lib.or_evalkeygen_ctr(n, sizeQ, sizeP, alpha, numPart, P(U(modQP)), P(U(rootQP)), P(U(s_old)),
+                          P(U(pkA)), P(U(pkB)), ns, sigma, seed, P(av), P(bv))
     return av, bv
 
 

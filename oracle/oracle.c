@@ -91,22 +91,27 @@ uint64_t or_mt64_next(or_mt64* g) {
  * with r < T[k] (last index if none); if k > 0 the sign is next() & 1.
  */
 #define GMAX 256
+/* the table of the Gaussian sampler for sigma: K and T[0..K] */
+static void gauss_table(double sigma, uint64_t* T, int* Kout) {
+    int K = (int)ceil(sigma * 12.0);
+    if (K > GMAX) K = GMAX;
+    double w[GMAX + 1], total = 0.0, cum = 0.0;
+    for (int k = 0; k <= K; ++k) {
+        w[k] = (k == 0) ? 1.0 : 2.0 * exp(-(double)k * (double)k / (2.0 * sigma * sigma));
+        total += w[k];
+    }
+    for (int k = 0; k <= K; ++k) {
+        cum += w[k];
+        T[k] = (uint64_t)ldexp(cum / total, 63);
+    }
+    *Kout = K;
+}
 int64_t or_gauss_sample(or_mt64* g, double sigma) {
     static double cached_sigma = -1.0;
     static uint64_t T[GMAX + 1];
     static int K = 0;
     if (sigma != cached_sigma) {
-        K = (int)ceil(sigma * 12.0);
-        if (K > GMAX) K = GMAX;
-        double w[GMAX + 1], total = 0.0, cum = 0.0;
-        for (int k = 0; k <= K; ++k) {
-            w[k] = (k == 0) ? 1.0 : 2.0 * exp(-(double)k * (double)k / (2.0 * sigma * sigma));
-            total += w[k];
-        }
-        for (int k = 0; k <= K; ++k) {
-            cum += w[k];
-            T[k] = (uint64_t)ldexp(cum / total, 63);
-        }
+        gauss_table(sigma, T, &K);
         cached_sigma = sigma;
     }
     uint64_t r = or_mt64_next(g) >> 1;
@@ -906,6 +911,115 @@ void or_evalkeygen_sold(uint32_t n, uint32_t sizeQ, uint32_t sizeP, uint32_t alp
             e0c[k] = or_gauss_sample(&g, sigma);
             e1c[k] = or_gauss_sample(&g, sigma);
         }
+        for (uint32_t t = 0; t < W; ++t) {
+            uint64_t q = modQP[t];
+            for (uint32_t k = 0; k < n; ++k) {
+                uT[k] = embed(uc[k], q); e0T[k] = embed(e0c[k], q); e1T[k] = embed(e1c[k], q);
+            }
+            ntt_fresh(uT, n, q, rootQP[t]);
+            ntt_fresh(e0T, n, q, rootQP[t]);
+            ntt_fresh(e1T, n, q, rootQP[t]);
+            uint64_t Pq = prod_mod(modQP + sizeQ, sizeP, q, -1);
+            size_t o = ((size_t)part * W + t) * n;
+            for (uint32_t k = 0; k < n; ++k) {
+                uint64_t a = addm(or_mulmod(pkA[(size_t)t * n + k], uT[k], q), or_mulmod(ns % q, e1T[k], q), q);
+                uint64_t b = addm(or_mulmod(pkB[(size_t)t * n + k], uT[k], q), or_mulmod(ns % q, e0T[k], q), q);
+                if (t >= st && t < en) b = addm(b, or_mulmod(Pq, sOld[(size_t)t * n + k], q), q);
+                av[o + k] = a;
+                bv[o + k] = b;
+            }
+        }
+    }
+    free(uc); free(e0c); free(e1c); free(uT); free(e0T); free(e1T);
+}
+
+/* ---- counter sampler (the library's device key generation) ----
+ * draw(seed, domain, i) = SplitMix64's output function of
+ * seed + (domain 2^32 + i + 1) 0x9E3779B97F4A7C15 (so draw(seed, 0, i - 1) is
+ * the i-th output of sequential SplitMix64 from `seed`); ternary = draw mod 3
+ * - 1; uniform = draw mod q; Gaussian = #{j < K : (draw >> 1) >= T[j]} of the
+ * or_gauss_sample table, negated when draw & 1.  Domains: secret 1, error 2,
+ * uniform a of tower t 0x10000 + t; switching key part p: u 0x20000 + 3p,
+ * e0 0x20001 + 3p, e1 0x20002 + 3p. */
+uint64_t or_ctr_draw(uint64_t seed, uint32_t domain, uint32_t index) {
+    uint64_t c = ((uint64_t)domain << 32) + (uint64_t)index + 1;
+    uint64_t z = seed + c * 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+int64_t or_ctr_gauss(uint64_t seed, uint32_t domain, uint32_t index, double sigma) {
+    uint64_t T[GMAX + 1];
+    int K;
+    gauss_table(sigma, T, &K);
+    uint64_t r = or_ctr_draw(seed, domain, index);
+    int64_t k = 0;
+    for (int j = 0; j < K; ++j) k += (r >> 1) >= T[j];
+    return (r & 1) ? -k : k;
+}
+static void ctr_gauss_poly(uint64_t seed, uint32_t domain, uint32_t n, double sigma, int64_t* out) {
+    uint64_t T[GMAX + 1];
+    int K;
+    gauss_table(sigma, T, &K);
+    for (uint32_t i = 0; i < n; ++i) {
+        uint64_t r = or_ctr_draw(seed, domain, i);
+        int64_t k = 0;
+        for (int j = 0; j < K; ++j) k += (r >> 1) >= T[j];
+        out[i] = (r & 1) ? -k : k;
+    }
+}
+
+/* or_keygen with the counter sampler: s ternary (domain 1), e Gaussian
+ * (domain 2), a of tower t uniform (domain 0x10000 + t, taken as eval form),
+ * b = ns e - a s: src/keygen.cpp:55-112. */
+void or_keygen_ctr(uint32_t n, const uint64_t* mods, const uint64_t* roots, uint32_t count,
+                   uint64_t ns, double sigma, uint64_t seed, uint64_t* s, uint64_t* pkA,
+                   uint64_t* pkB) {
+    int64_t* sc = (int64_t*)malloc((size_t)n * 8);
+    int64_t* ec = (int64_t*)malloc((size_t)n * 8);
+    uint64_t* eT = (uint64_t*)malloc((size_t)n * 8);
+    for (uint32_t k = 0; k < n; ++k) sc[k] = (int64_t)(or_ctr_draw(seed, 1, k) % 3) - 1;
+    ctr_gauss_poly(seed, 2, n, sigma, ec);
+    for (uint32_t t = 0; t < count; ++t) {
+        uint64_t q = mods[t];
+        uint64_t* sT = s + (size_t)t * n;
+        uint64_t* aT = pkA + (size_t)t * n;
+        uint64_t* bT = pkB + (size_t)t * n;
+        for (uint32_t k = 0; k < n; ++k) {
+            sT[k] = embed(sc[k], q);
+            eT[k] = embed(ec[k], q);
+            aT[k] = or_ctr_draw(seed, 0x10000 + t, k) % q;
+        }
+        ntt_fresh(sT, n, q, roots[t]);
+        ntt_fresh(eT, n, q, roots[t]);
+        for (uint32_t k = 0; k < n; ++k)
+            bT[k] = subm(or_mulmod(ns % q, eT[k], q), or_mulmod(aT[k], sT[k], q), q);
+    }
+    free(eT); free(sc); free(ec);
+}
+
+/* or_evalkeygen_sold with the counter sampler: part p draws u (domain
+ * 0x20000 + 3p, ternary), e0 (+1) and e1 (+2, Gaussian):
+ * src/evalkeygen.cpp:42-96. */
+void or_evalkeygen_ctr(uint32_t n, uint32_t sizeQ, uint32_t sizeP, uint32_t alpha,
+                       uint32_t numPart, const uint64_t* modQP, const uint64_t* rootQP,
+                       const uint64_t* sOld, const uint64_t* pkA, const uint64_t* pkB,
+                       uint64_t ns, double sigma, uint64_t seed, uint64_t* av,
+                       uint64_t* bv) {
+    const uint32_t W = sizeQ + sizeP;
+    int64_t* uc = (int64_t*)malloc((size_t)n * 8);
+    int64_t* e0c = (int64_t*)malloc((size_t)n * 8);
+    int64_t* e1c = (int64_t*)malloc((size_t)n * 8);
+    uint64_t* uT = (uint64_t*)malloc((size_t)n * 8);
+    uint64_t* e0T = (uint64_t*)malloc((size_t)n * 8);
+    uint64_t* e1T = (uint64_t*)malloc((size_t)n * 8);
+    for (uint32_t part = 0; part < numPart; ++part) {
+        uint32_t st = alpha * part;
+        uint32_t sp = (sizeQ > st + alpha) ? alpha : (sizeQ - st);
+        uint32_t en = st + sp;
+        for (uint32_t k = 0; k < n; ++k) uc[k] = (int64_t)(or_ctr_draw(seed, 0x20000 + 3 * part, k) % 3) - 1;
+        ctr_gauss_poly(seed, 0x20001 + 3 * part, n, sigma, e0c);
+        ctr_gauss_poly(seed, 0x20002 + 3 * part, n, sigma, e1c);
         for (uint32_t t = 0; t < W; ++t) {
             uint64_t q = modQP[t];
             for (uint32_t k = 0; k < n; ++k) {

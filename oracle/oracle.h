@@ -38,6 +38,14 @@
  *   or_keygen, or_evalkeygen, or_encrypt, or_decrypt
  *                        RLWE decryption identities (noise bounds), enc/dec
  *                        round trip with maxerr 0.
+ *   or_ctr_draw          published SplitMix64 known answers (sequential
+ *                        outputs from seed 1234567), domain separation.
+ *   or_ctr_gauss         distribution: moments against sigma and the
+ *                        histogram of or_gauss_sample (same table).
+ *   or_keygen_ctr, or_evalkeygen_ctr
+ *                        ternary / uniform sample statistics, the RLWE
+ *                        identity b + a s = ns e with e small, and a switching
+ *                        key's decryption identity under the generated pk.
  * No function in this file is "parity unpinned".
  */
 #ifndef GPUFHE_ORACLE_H
@@ -167,6 +175,18 @@ void or_evalkeygen_sold(uint32_t n, uint32_t sizeQ, uint32_t sizeP, uint32_t alp
                         const uint64_t* sOld, const uint64_t* pkA, const uint64_t* pkB,
                         uint64_t ns, double sigma, uint64_t seed, uint64_t* av,
                         uint64_t* bv);
+/* counter sampler of the device key generation (see oracle.c) and the key
+ * generators that use it */
+uint64_t or_ctr_draw(uint64_t seed, uint32_t domain, uint32_t index);
+int64_t or_ctr_gauss(uint64_t seed, uint32_t domain, uint32_t index, double sigma);
+void or_keygen_ctr(uint32_t n, const uint64_t* mods, const uint64_t* roots, uint32_t count,
+                   uint64_t ns, double sigma, uint64_t seed, uint64_t* s, uint64_t* pkA,
+                   uint64_t* pkB);
+void or_evalkeygen_ctr(uint32_t n, uint32_t sizeQ, uint32_t sizeP, uint32_t alpha,
+                       uint32_t numPart, const uint64_t* modQP, const uint64_t* rootQP,
+                       const uint64_t* sOld, const uint64_t* pkA, const uint64_t* pkB,
+                       uint64_t ns, double sigma, uint64_t seed, uint64_t* av,
+                       uint64_t* bv);
 void or_encrypt(uint32_t n, const uint64_t* mods, const uint64_t* roots, uint32_t count,
                 const int64_t* m, const uint64_t* pkA, const uint64_t* pkB, uint64_t ns,
                 double sigma, uint64_t seed, uint64_t* c0, uint64_t* c1);

@@ -104,6 +104,20 @@ class KeySet:
         self.ek_seed = ek_seed
         self._rlk = None
 
+    @classmethod
+    def counter(cls, P, seed=101, ek_seed=202):
+        """The same key set drawn by the counter sampler (the library's device
+        key generation: O.keygen_ctr / O.evalkeygen_ctr)."""
+        ks = cls.__new__(cls)
+        ks.P = P
+        n, L = P.n, P.L
+        ks.sQP, ks.pkAQP, ks.pkBQP = O.keygen_ctr(n, P.mods, P.roots, seed=seed)
+        ks.sQ, ks.pkA, ks.pkB = ks.sQP[: L * n], ks.pkAQP[: L * n], ks.pkBQP[: L * n]
+        ks.ek_seed = ek_seed
+        ks._rlk = O.evalkeygen_ctr(n, P.L, P.K, P.alpha, P.dnum, P.mods, P.roots, ks.s2(), ks.pkAQP, ks.pkBQP,
+                                   seed=ek_seed)
+        return ks
+
     def s2(self):
         P = self.P
         n = P.n

@@ -590,6 +590,29 @@ def test_device_keygen_matches_oracle(G, n, L, dnum):
     assert np.array_equal(ra, wa) and np.array_equal(rb, wb)
 
 
+@pytest.mark.parametrize("n,L,dnum", [(1 << 12, 4, 2), (1 << 13, 6, 3), (1 << 16, 24, 4)])
+def test_device_keygen_counter_sampler_matches_oracle(G, n, L, dnum):
+    """Keys generated entirely on the device (gf_keygen: s, e, a drawn by the
+    counter sampler, embedded, transformed, b = ns e - a s; then
+    gf_key_generate_ctr: the relinearisation key of s^2 under that public
+    key) equal the oracle's counter-sampler key generation bit for bit, up to
+    n = 2^16, L = 24, dnum = 4."""
+    P = make_params(n, L, dnum)
+    ctx = G.Context(n, P.q, P.p, P.roots, dnum)
+    s = G.Stream()
+    ds, da, db = G.keygen(ctx, 4242, stream=s)
+    ws_, wa, wb = O.keygen_ctr(n, P.mods, P.roots, seed=4242)
+    assert np.array_equal(ds.download(), ws_), "s"
+    assert np.array_equal(da.download(), wa), "pkA"
+    assert np.array_equal(db.download(), wb), "pkB"
+    W = P.L + P.K
+    s2 = np.concatenate([mulmod_vec(ws_[t * n:(t + 1) * n], ws_[t * n:(t + 1) * n], int(P.mods[t])) for t in range(W)])
+    key = G.Key.generate_ctr(ctx, G.Dev.of(s2), da, db, 5151, stream=s)
+    av, bv = key.download(s)
+    ea, eb = O.evalkeygen_ctr(n, P.L, P.K, P.alpha, P.dnum, P.mods, P.roots, s2, wa, wb, seed=5151)
+    assert np.array_equal(av, ea) and np.array_equal(bv, eb), "switching key"
+
+
 @pytest.mark.parametrize("n,L,dnum", [(1 << 12, 3, 3), (1 << 13, 6, 3), (1 << 16, 24, 4)])
 def test_device_encrypt_decrypt_match_oracle(G, n, L, dnum):
     """gf_encrypt equals the oracle's encryption (same sampler and seed) bit

@@ -71,6 +71,77 @@ def test_gauss_sampler_moments():
     assert np.abs(xs).max() <= int(np.ceil(3.19 * 12))
 
 
+def test_ctr_draw_known_answers():
+    """draw(seed, 0, i - 1) is the i-th output of sequential SplitMix64: the
+    published outputs from seed 1234567; other domains are other counters."""
+    want = [6457827717110365317, 3203168211198807973, 9817491932198370423, 4593380528125082431,
+            16408922859458223821]
+    assert [O.lib.or_ctr_draw(1234567, 0, i) for i in range(5)] == want
+    # domain d, index i is counter d 2^32 + i + 1 of the same sequence
+    g = 0x9E3779B97F4A7C15
+    M = (1 << 64) - 1
+
+    def mix(z):
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+        return z ^ (z >> 31)
+    for d, i in ((1, 0), (2, 17), (0x10000 + 3, 65535), (0x20002, 5)):
+        assert O.lib.or_ctr_draw(99, d, i) == mix((99 + ((d << 32) + i + 1) * g) & M)
+    assert len({O.lib.or_ctr_draw(7, d, 0) for d in range(64)}) == 64
+
+
+def test_ctr_gauss_distribution():
+    """The counter Gaussian has the table sampler's law: moments against sigma
+    and a histogram matching or_gauss_sample's within sampling error."""
+    N = 200000
+    xs = np.array([O.lib.or_ctr_gauss(11, 2, i, 3.19) for i in range(N)])
+    assert abs(xs.mean()) < 0.05
+    assert abs(xs.std() - 3.19) < 0.05
+    assert np.abs(xs).max() <= int(np.ceil(3.19 * 12))
+    g = O.MT64()
+    O.lib.or_mt64_seed(C.byref(g), 5)
+    ys = np.array([O.lib.or_gauss_sample(C.byref(g), 3.19) for _ in range(N)])
+    for k in range(-8, 9):
+        p1, p2 = (xs == k).mean(), (ys == k).mean()
+        assert abs(p1 - p2) < 5 * np.sqrt(max(p2, 1e-4) / N) + 1e-4, k
+    # the discrete Gaussian's closed-form mass at 0: 1 / sum_k exp(-k^2 / 2 sigma^2)
+    p0 = 1.0 / sum(np.exp(-k * k / (2 * 3.19 ** 2)) for k in range(-40, 41))
+    assert abs((xs == 0).mean() - p0) < 5 * np.sqrt(p0 / N)
+
+
+def test_keygen_ctr_identity_and_statistics():
+    """Counter-sampled keys: s ternary with equal frequencies, a uniform,
+    b + a s = ns e with e small, on every tower."""
+    P = make_params(1 << 11, 4, 2)
+    n = P.n
+    s, a, b = O.keygen_ctr(n, P.mods, P.roots, ns=1, seed=42)
+    for t in (0, 3, len(P.mods) - 1):
+        q = int(P.mods[t])
+        fr, fp, ir, ip, ni, nip = O.root_tables(n, q, int(P.roots[t]))
+        sl = slice(t * n, (t + 1) * n)
+        sc = np.array([centred(int(x), q) for x in O.intt(s[sl], q, ir, ip, ni, nip)])
+        assert set(sc.tolist()) <= {-1, 0, 1}
+        assert all(abs((sc == v).mean() - 1 / 3) < 0.05 for v in (-1, 0, 1))
+        ph = (b[sl].astype(object) + mulmod_vec(a[sl], s[sl], q).astype(object)) % q
+        e = [centred(int(x), q) for x in O.intt(O.U(list(ph)), q, ir, ip, ni, nip)]
+        assert max(abs(x) for x in e) <= 40 and np.std(e) > 2.5
+        assert abs(a[sl].astype(float).mean() / q - 0.5) < 0.03
+    # the secret is one polynomial: the same coefficients on every tower
+    s0 = [centred(int(x), int(P.mods[0])) for x in O.intt(s[:n], int(P.mods[0]), *O.root_tables(n, int(P.mods[0]), int(P.roots[0]))[2:])]
+    q1 = int(P.mods[1])
+    s1 = [centred(int(x), q1) for x in O.intt(s[n:2 * n], q1, *O.root_tables(n, q1, int(P.roots[1]))[2:])]
+    assert s0 == s1
+
+
+@pytest.mark.parametrize("level", [4, 2])
+def test_evalkeygen_ctr_keyswitch_identity(level):
+    """A relinearisation key from the counter sampler switches s^2 -> s:
+    ba0 + ba1 s == d s^2 + small noise."""
+    P = make_params(1 << 10, 4, 2)
+    ks = KeySet.counter(P)
+    assert _ks_identity(P, ks, level, level + 5) < (1 << 20)
+
+
 # ---------------------------------------------------------- parameters ----
 PRIME_CASES = [(4, 35, 1024), (3, 50, 1 << 15), (24, 50, 1 << 16), (6, 60, 1 << 16), (2, 36, 1024)]
 

@@ -55,7 +55,8 @@ STEP_KERNELS = {"ntt_fwd": ("k_ntt_A", "k_ntt_B"), "ntt_inv": ("k_ntt_B", "k_ntt
                 "tensor": ("k_tensor",), "conv": ("k_conv",), "inner": ("k_inner",),
                 "combine": ("k_combine",), "switch": ("k_switch",), "fuse": ("k_fuse",),
                 "add": ("k_add",), "perm": ("k_perm",), "copy": ("k_copy",),
-                "rsv": ("k_rs",), "ptacc": ("k_ptacc",), "ntt_fwd_a": ("k_ntt_A",), "ksi": ("k_ksi",)}
+                "rsv": ("k_rs",), "ptacc": ("k_ptacc",), "ntt_fwd_a": ("k_ntt_A",), "ksi": ("k_ksi",),
+                "ntt_inv_b": ("k_ntt_B",), "modup": ("k_modup",)}
 
 
 def kernel_bytes(name, kind, nkern, total, epilogue):
@@ -82,6 +83,26 @@ def step_bytes(n, L, K, alpha, np_parts, kinds, batch):
     sp = [min(alpha, L - alpha * p) for p in range(np_parts)]
     if "ksi" in kinds:
         assert kinds.count("ksi") == 2 and len(kinds) == 9, kinds
+    tail = [
+        ("intt_moddown", 2 * T * (2 * K + 2), 0),       # + INTT of res0/res1's last tower
+        # per component: the K P-part sources and the dropped tower's two
+        # coefficient-form operands in, the L - 1 surviving towers out
+        ("moddown_rescale_prologue", T * 2 * (K + 2 + (L - 1)), 0),
+        # transform + the t and res operands its epilogue reads
+        ("ntt_rescale_epilogue", 4 * T * 2 * (L - 1), 2 * T * 2 * (L - 1)),
+    ]
+    ksi = [("rowpass_inner_q", T * L * ((np_parts - 1) + 1 + key + 2), 0),
+           ("rowpass_inner_p", T * K * (np_parts + key + 2), 0)]
+    if "modup" in kinds:
+        # fused ModUp: the inverse transforms' row pass (half of their 16 n B),
+        # the two extra towers' whole inverse transforms, then one kernel with
+        # the inverse column passes (the other half), the conversion (sources
+        # and targets once) and the digits' column passes (half of theirs)
+        assert kinds == ["tensor", "ntt_inv_b", "ntt_inv", "modup", "ksi", "ksi", "ntt_inv", "rsv", "ntt_fwd"], kinds
+        return [("tensor", 7 * T * L, 0),
+                ("intt_modup_rowpass", T * L, 0),
+                ("intt_extra", 2 * T * 2, 0),
+                ("modup_fused", T * (L + (sum(sp) + sum(sc)) + sum(sc)), 0)] + ksi + tail
     if "ksi" in kinds or len(kinds) == 8:   # fused ModDown + rescale plan
         if "ksi" in kinds:
             # the digits' column pass, then their row pass fused with the key
@@ -669,7 +690,7 @@ def main():
         radix, ntt_ms = ctx.ntt_radix(), float("nan")
     else:
         radix, ntt_ms = ctx.autotune(stream, args.batch)   # NTT register radix, then the plan knobs at B
-    knobs = {k_: ctx.knob(k_) for k_ in ("ntt_radix", "ksi_extra", "conv_chunk", "rs_chunk")}
+    knobs = {k_: ctx.knob(k_) for k_ in ("ntt_radix", "ksi_extra", "conv_chunk", "rs_chunk", "modup_fused")}
     log("context ready: K=%d alpha=%d ws=%.1f MiB/op, knobs %s (%s)"
         % (K, alpha, ctx.ws_bytes / 2**20, knobs, "pinned" if pinned else "autotuned"))
     tuned = {"shape": {"n": N, "L": L, "dnum": DNUM}, "chosen": knobs, "pinned": pinned,

@@ -13,7 +13,7 @@ ARCH = os.environ.get("GPUFHE_ARCH", "gfx950")
 FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
          # packed FP32 instructions stay off in every code object of this library
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
-SOURCES = ["kernels_ntt_f.hip", "kernels_ntt_i.hip", "kernels_ksi.hip", "kernels_ntt_ref.hip", "kernels.hip", "engine.hip", "capi.hip", "serve.hip", "params.cpp", "api.cpp", "keys.cpp", "wire.cpp"]
+SOURCES = ["kernels_ntt_f.hip", "kernels_ntt_i.hip", "kernels_ksi.hip", "kernels_modup.hip", "kernels_ntt_ref.hip", "kernels.hip", "engine.hip", "capi.hip", "serve.hip", "params.cpp", "api.cpp", "keys.cpp", "wire.cpp"]
 
 
 def _stale(src, obj):

@@ -173,9 +173,16 @@ Ctx::Ctx(uint32_t n_, const std::vector<uint64_t>& q, const std::vector<uint64_t
                 tq[j] = mods[P.compl_rows[j]];
                 tqn[j] = host_qneg(tq[j]);
             }
+            std::vector<uint64_t> ttw(P.sc), tred(P.sc);
+            for (uint32_t j = 0; j < P.sc; ++j) {
+                ttw[j] = (uint64_t)(uintptr_t)rt[P.compl_rows[j]].fw2;
+                tred[j] = lazy_red_of(tq[j]);
+            }
             put(tab, &P.d_tab);
             put(tq, &P.d_tq);
             put(tqn, &P.d_tqneg);
+            put(ttw, &P.d_ttw);
+            put(tred, &P.d_tred);
         }
         // ModDown P -> Q_l
         std::vector<uint64_t> tab((size_t)K * l), tq(l), tqn(l);
@@ -342,6 +349,11 @@ void Ctx::set_knob(const std::string& name, uint32_t v) {
     } else if (name == "conv_chunk" || name == "rs_chunk") {
         if (v < 1 || v > 64) throw std::runtime_error("gpufhe: chunk knobs must be in [1, 64]");
         (name == "conv_chunk" ? conv_chunk : rs_chunk) = v;
+    } else if (name == "modup_fused") {
+        if (v > 1) throw std::runtime_error("gpufhe: modup_fused must be 0 or 1");
+        modup_fused = v;
+    } else if (name == "modup_min_wgs") {
+        modup_min_wgs = v;
     } else if (name == "rechunk_wgs") {
         rechunk_wgs = v;
     } else {
@@ -353,6 +365,8 @@ uint32_t Ctx::knob(const std::string& name) const {
     if (name == "ksi_extra") return ksi_extra;
     if (name == "conv_chunk") return conv_chunk;
     if (name == "rs_chunk") return rs_chunk;
+    if (name == "modup_fused") return modup_fused;
+    if (name == "modup_min_wgs") return modup_min_wgs;
     if (name == "rechunk_wgs") return rechunk_wgs;
     throw std::runtime_error("gpufhe: unknown tuning knob " + name);
 }
@@ -437,7 +451,14 @@ void Ctx::autotune_plan(hipStream_t s, uint32_t B) {
     };
     rep << ", \"batch\": " << B << ", \"level\": " << l;
     if (fused && !geom.single && n % KSI_TILE == 0) tune("ksi_extra", ksi_extra, {0, 1}, {Step::KSI});
+    // the conversion chunk width matters where the ModUp runs unfused: tuned
+    // with it unfused, then the fused ModUp is weighed against that
+    const uint32_t mf = modup_fused;
+    modup_fused = 0;
     tune("conv_chunk", conv_chunk, {8, 16, 32}, {Step::CONV});
+    modup_fused = mf;
+    if (fused && !geom.single && n % KSI_TILE == 0 && n % MODUP_TILE == 0 && !wide && alpha <= MODUP_MAXSRC)
+        tune("modup_fused", modup_fused, {0, 1}, {Step::NTT_IB, Step::NTT_I, Step::MODUP, Step::CONV, Step::NTT_FA});
     if (fused) tune("rs_chunk", rs_chunk, {8, 16, 32}, {Step::RSV});
     rep << "}";
     (void)hipEventDestroy(e0);
@@ -761,7 +782,7 @@ void Plan::run(hipStream_t s) const {
 
 uint32_t Plan::step_kernels(size_t i) const {
     const Step::Kind k = steps[i].kind;
-    return (k == Step::NTT_F || k == Step::NTT_I) && !geom.single ? 2u : 1u;
+    return (k == Step::NTT_F || k == Step::NTT_I) && !geom.single ? 2u : 1u;   // NTT_IB, NTT_FA: one pass
 }
 
 void Plan::run_step(size_t i, hipStream_t s, uint32_t which) const { run_jobs(i, s, which, 0, steps[i].count); }
@@ -788,6 +809,8 @@ void Plan::run_jobs(size_t i, hipStream_t s, uint32_t which, uint32_t first, uin
         case Step::PTACC: launch_ptacc(GF_J(PtAccJob), cnt, n, s); break;
         case Step::NTT_FA: launch_ntt(GF_J(NttJob), cnt, geom, true, s, 1); break;
         case Step::KSI: launch_ksi(GF_J(KsiJob), cnt, geom, st.arg & 0xFF, st.arg >> 8, s); break;
+        case Step::NTT_IB: launch_ntt(GF_J(NttJob), cnt, geom, false, s, 1); break;
+        case Step::MODUP: launch_modup(GF_J(ModupJob), cnt, geom, st.arg, s); break;
     }
 #undef GF_J
 }
@@ -856,6 +879,10 @@ void PlanBuilder::keyswitch_core(uint32_t l, const uint64_t* d, const DevKey& ke
 bool PlanBuilder::use_ksi(uint32_t l) const {
     return C_.fused && !C_.geom.single && C_.n % KSI_TILE == 0 && C_.lv[l].np <= KSI_MAXP;
 }
+// the fused ModUp: column sums over at most MODUP_MAXSRC sources below 2^60
+bool PlanBuilder::use_modup(uint32_t l) const {
+    return C_.modup_fused && use_ksi(l) && !C_.wide && C_.alpha <= MODUP_MAXSRC && C_.n % MODUP_TILE == 0;
+}
 
 // 3b-5. The digits' row pass fused with the inner product over Q_l P
 // (src/keyswitch_resident.cpp:171-200), into the workspace accumulators;
@@ -916,11 +943,15 @@ void PlanBuilder::modup(uint32_t l, const uint64_t* d, uint64_t* ws, const uint6
     const WsLayout w = WsLayout::make(C);
     uint64_t* x = ws + w.x * n;
     uint64_t* dig = ws + w.dig * n;
+    // fused: the inverse transforms' row pass here; their column pass, the
+    // conversion and the digits' column pass in one ModUp kernel per part
+    const bool fuse = rowpass_in_ksi && use_modup(l);
     // 1. part towers to coefficient form, scaled by n^-1 QHatInv (ModUp's first factor)
     for (uint32_t i = 0; i < l; ++i) {
         const uint32_t pi = i / C.alpha, ii = i - pi * C.alpha;
-        intt_up_.push_back(NttJob{d + (size_t)i * n, x + (size_t)i * n, C.rt[i].iv2, nullptr,
-                                  C.mods[i], lv.parts[pi].post[ii], lv.parts[pi].postp[ii]});
+        (fuse ? intt_upb_ : intt_up_)
+            .push_back(NttJob{d + (size_t)i * n, x + (size_t)i * n, C.rt[i].iv2, nullptr, C.mods[i],
+                              lv.parts[pi].post[ii], lv.parts[pi].postp[ii]});
     }
     const uint64_t* ex[2] = {extra0, extra1};
     uint64_t* dx[2] = {dx0, dx1};
@@ -932,14 +963,46 @@ void PlanBuilder::modup(uint32_t l, const uint64_t* d, uint64_t* ws, const uint6
     for (uint32_t pi = 0; pi < lv.np; ++pi) {
         const PartConst& P = lv.parts[pi];
         uint64_t* dp = dig + (size_t)pi * W * n;
-        for (uint32_t j0 = 0; j0 < P.sc; j0 += C.conv_chunk)
-            conv_up_.push_back(ConvJob{x + (size_t)P.st * n, dp, P.d_tab, P.d_tq, P.d_tqneg, P.sp, P.sc,
-                                       j0, j0 + C.conv_chunk < P.sc ? j0 + C.conv_chunk : P.sc, C.wide});
-        for (uint32_t j = 0; j < P.sc; ++j) {
-            const uint32_t r = P.compl_rows[j];
-            (rowpass_in_ksi ? ntt_diga_ : ntt_dig_)
-                .push_back(NttJob{dp + (size_t)j * n, dp + (size_t)j * n, C.rt[r].fw2, nullptr, C.mods[r], 0, 0});
+        if (fuse) {
+            modup_src_.push_back(ModupSrc{l, pi, x, dp});
+            ModupJob J{};
+            for (uint32_t i = 0; i < P.sp; ++i) {
+                const uint32_t r = P.st + i;
+                J.src[i] = x + (size_t)r * n;
+                J.stw[i] = C.rt[r].iv2;
+                J.sq[i] = C.mods[r];
+                J.post[i] = P.post[i];
+                J.postp[i] = P.postp[i];
+            }
+            J.dst = dp;
+            J.tab = P.d_tab;
+            J.tq = P.d_tq;
+            J.tqneg = P.d_tqneg;
+            J.ttw = P.d_ttw;
+            J.tred = P.d_tred;
+            J.sp = P.sp;
+            J.sc = P.sc;
+            modup_.push_back(J);
+            continue;
         }
+        modup_unfused(l, pi, x, dp, rowpass_in_ksi);
+    }
+}
+
+// The conversion of part pi (sources x, coefficient form) into the digit
+// towers dp, and the digits' forward transforms (column pass only when their
+// row pass runs in the KSI step)
+void PlanBuilder::modup_unfused(uint32_t l, uint32_t pi, const uint64_t* x, uint64_t* dp, bool colpass_only) {
+    const Ctx& C = C_;
+    const uint32_t n = C.n;
+    const PartConst& P = C.lv[l].parts[pi];
+    for (uint32_t j0 = 0; j0 < P.sc; j0 += C.conv_chunk)
+        conv_up_.push_back(ConvJob{x + (size_t)P.st * n, dp, P.d_tab, P.d_tq, P.d_tqneg, P.sp, P.sc, j0,
+                                   j0 + C.conv_chunk < P.sc ? j0 + C.conv_chunk : P.sc, C.wide});
+    for (uint32_t j = 0; j < P.sc; ++j) {
+        const uint32_t r = P.compl_rows[j];
+        (colpass_only ? ntt_diga_ : ntt_dig_)
+            .push_back(NttJob{dp + (size_t)j * n, dp + (size_t)j * n, C.rt[r].fw2, nullptr, C.mods[r], 0, 0});
     }
 }
 
@@ -1342,7 +1405,16 @@ std::unique_ptr<Plan> PlanBuilder::finish_into(void* d_arena, void* h_stage, siz
 }
 
 std::unique_ptr<Plan> PlanBuilder::layout(std::vector<char>& blob) {
-    for (auto* v : {&intt_up_, &ntt_dig_, &ntt_diga_, &intt_p_, &ntt_conv_, &intt_drop_, &ntt_sw_, &ntt_v_}) {
+    // a fused ModUp stage too small to fill the chip runs unfused: the
+    // inverse transforms whole, then the conversion and the column passes
+    if (!modup_.empty() && modup_.size() * (C_.n / MODUP_TILE) < C_.modup_min_wgs) {
+        for (NttJob& j : intt_upb_) intt_up_.push_back(j);
+        intt_upb_.clear();
+        for (const ModupSrc& m : modup_src_) modup_unfused(m.l, m.pi, m.x, m.dp, true);
+        modup_.clear();
+        modup_src_.clear();
+    }
+    for (auto* v : {&intt_up_, &intt_upb_, &ntt_dig_, &ntt_diga_, &intt_p_, &ntt_conv_, &intt_drop_, &ntt_sw_, &ntt_v_}) {
         group_by(*v, [](const NttJob& j) { return (uintptr_t)j.tw; });
         // every transform of a plan uses the context's paired tables; its phase B
         // reads the matching row table of the current radix
@@ -1386,7 +1458,17 @@ std::unique_ptr<Plan> PlanBuilder::layout(std::vector<char>& blob) {
 #define GF_STAGE(k, v) stage(Step::k, v.data(), v.size(), sizeof(v[0]))
     GF_STAGE(TENSOR, tensor_);
     GF_STAGE(PERM, perm_);
+    GF_STAGE(NTT_IB, intt_upb_);
     GF_STAGE(NTT_I, intt_up_);
+    // fused ModUp jobs by source count (one launch each)
+    std::stable_sort(modup_.begin(), modup_.end(), [](const ModupJob& a, const ModupJob& b) { return a.sp < b.sp; });
+    for (size_t i = 0; i < modup_.size();) {
+        size_t e = i;
+        while (e < modup_.size() && modup_[e].sp == modup_[i].sp) ++e;
+        stage(Step::MODUP, modup_.data() + i, e - i, sizeof(ModupJob));
+        P->steps.back().arg = modup_[i].sp;
+        i = e;
+    }
     GF_STAGE(CONV, conv_up_);
     GF_STAGE(NTT_F, ntt_dig_);
     GF_STAGE(NTT_FA, ntt_diga_);

@@ -63,6 +63,8 @@ struct PartConst {
     const uint64_t* d_tab = nullptr;           // QHatModp * R mod target, [sp][sc]
     const uint64_t* d_tq = nullptr;            // target moduli [sc]
     const uint64_t* d_tqneg = nullptr;
+    const uint64_t* d_ttw = nullptr;           // [sc] addresses of the targets' paired forward tables
+    const uint64_t* d_tred = nullptr;          // [sc] lazy_red_of(target modulus)
 };
 struct LevelConst {
     uint32_t l = 0, np = 0;
@@ -112,6 +114,12 @@ public:
     // + inner product workgroup (it only takes part in the inner product), and
     // the targets per job of the ModUp conversion and of the rescale prologue.
     uint32_t ksi_extra = 0, conv_chunk = 32, rs_chunk = 32;
+    // 1: ModUp's inverse column pass, conversion and digit column pass run as
+    // one kernel (modup.hpp) where the shape allows it
+    uint32_t modup_fused = 1;
+    // a plan whose fused ModUp stage would launch fewer workgroups than this
+    // runs it unfused (a single operation's stage cannot fill the chip)
+    uint32_t modup_min_wgs = 2048;
     // conversion and prologue stages that would launch fewer workgroups than
     // this are re-split into 8-target jobs, whatever the chunk knobs say
     uint32_t rechunk_wgs = 8192;
@@ -122,7 +130,7 @@ public:
     void autotune_plan(hipStream_t s, uint32_t batch);
     std::string tune_report;
     // sets or reads a knob by name ("ntt_radix", "ksi_extra", "conv_chunk",
-    // "rs_chunk", "rechunk_wgs")
+    // "rs_chunk", "modup_fused", "modup_min_wgs", "rechunk_wgs")
     void set_knob(const std::string& name, uint32_t v);
     uint32_t knob(const std::string& name) const;
     std::string radix_report;   // the radix search of autotune(), as JSON
@@ -159,8 +167,10 @@ void decrypt_device(const Ctx& C, const uint64_t* c0, const uint64_t* c1, const 
 struct Step {
     // NTT_FA: a forward transform's first (column) pass only; KSI: the fused
     // row pass of the ModUp digits and the key inner product
+    // NTT_IB: an inverse transform's first (row) pass only; MODUP: the fused
+    // ModUp (modup.hpp), arg = the jobs' source count
     enum Kind { NTT_F, NTT_I, TENSOR, CONV, INNER, COMBINE, SWITCH, FUSE, ADD, PERM, COPY, RSV, PTACC,
-                NTT_FA, KSI } kind;
+                NTT_FA, KSI, NTT_IB, MODUP } kind;
     size_t off;     // byte offset of the first job in the plan's device buffer
     uint32_t count;
     // KSI: LDS regions (bits 0-7) and waves per workgroup (bits 8-15); the
@@ -240,6 +250,14 @@ private:
     std::vector<TensorJob> tensor_;
     std::vector<PermJob> perm_;
     std::vector<NttJob> intt_up_;
+    std::vector<NttJob> intt_upb_;   // inverse transforms whose column pass runs in the fused ModUp
+    std::vector<ModupJob> modup_;
+    struct ModupSrc {
+        uint32_t l, pi;
+        const uint64_t* x;
+        uint64_t* dp;
+    };
+    std::vector<ModupSrc> modup_src_;   // what each fused ModUp job converts, to run it unfused
     std::vector<ConvJob> conv_up_;
     std::vector<NttJob> ntt_dig_;
     std::vector<NttJob> ntt_diga_;   // digits whose row pass runs inside the KSI step
@@ -270,6 +288,8 @@ private:
     // the digits' row pass fused with the key inner product (two-phase rings,
     // up to KSI_MAXP parts, fused schedule on)
     bool use_ksi(uint32_t l) const;
+    bool use_modup(uint32_t l) const;
+    void modup_unfused(uint32_t l, uint32_t pi, const uint64_t* x, uint64_t* dp, bool colpass_only);
     void ksi_moddown_src(uint32_t l, const uint64_t* d, const DevKey& key, uint64_t* ws, uint32_t lazy_rows);
     const uint64_t* digit_tower(uint32_t l, const uint64_t* d, const uint64_t* dig, uint32_t pi,
                                 uint32_t i) const;

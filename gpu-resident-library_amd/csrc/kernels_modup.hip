@@ -1,0 +1,57 @@
+// The fused ModUp kernel (modup.hpp) and its launch.
+#include "modup.hpp"
+#include "ntt_kernels.hpp"
+
+namespace gf {
+
+// One workgroup per (part job, column tile).  Source count SP fixed at compile
+// time, so every thread's unit of the sources stays in registers.
+// sources I .. SP - 1, one after the other (compile-time source index, so the
+// unit arrays are indexed by constants and stay in registers)
+template <uint32_t S1, uint32_t SP, uint32_t I>
+__device__ __forceinline__ void modup_sources(const ModupJob& j, const NttGeom& g, uint64_t* lds, uint32_t tid,
+                                              ModupUnit<SP>& x) {
+    if constexpr (I < SP) {
+        modup_source_pass<S1>(j, I, g, blockIdx.x, lds, tid, MODUP_THREADS, DevSync{});
+        __syncthreads();
+        modup_take_unit<SP, S1>(j, I, lds, tid, x);
+        __syncthreads();
+        modup_sources<S1, SP, I + 1>(j, g, lds, tid, x);
+    }
+}
+
+template <uint32_t S1, uint32_t SP>
+__global__ __launch_bounds__(MODUP_THREADS) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_modup(const ModupJob* __restrict__ jobs, uint32_t off, NttGeom g) {
+    __shared__ uint64_t lds[MODUP_TILE];
+    const ModupJob& j = jobs[off + blockIdx.y];
+    const uint32_t tid = threadIdx.x;
+    ModupUnit<SP> x;
+    modup_sources<S1, SP, 0>(j, g, lds, tid, x);
+    for (uint32_t t = 0; t < j.sc; ++t) {
+        modup_convert_unit<SP, S1>(j, t, lds, tid, x);
+        __syncthreads();
+        modup_target_pass<S1>(j, t, g, blockIdx.x, lds, tid, MODUP_THREADS, DevSync{});
+        __syncthreads();
+    }
+}
+
+// jobs of one launch share their source count sp
+void launch_modup(const ModupJob* d, uint32_t nj, const NttGeom& g, uint32_t sp, hipStream_t s) {
+    if (g.single || g.n % MODUP_TILE || sp < 1 || sp > MODUP_MAXSRC)
+        throw std::runtime_error("gpufhe: fused ModUp launch shape out of range");
+    const bool ok = with_split(g.logN1, g.logN2, [&](auto A, auto) {
+        constexpr uint32_t S1 = decltype(A)::value;
+        for (uint32_t off = 0; off < nj; off += MAXY) {
+            const uint32_t cnt = nj - off < MAXY ? nj - off : MAXY;
+            const dim3 grid(g.n / MODUP_TILE, cnt);
+            switch (sp) {
+#define GF_SP(k) case k: k_modup<S1, k><<<grid, MODUP_THREADS, 0, s>>>(d, off, g); break;
+                GF_SP(1) GF_SP(2) GF_SP(3) GF_SP(4) GF_SP(5) GF_SP(6)
+#undef GF_SP
+            }
+        }
+    });
+    if (!ok) throw std::runtime_error("gpufhe: unsupported ring dimension for the fused ModUp");
+}
+
+}  // namespace gf

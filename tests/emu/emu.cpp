@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "ksi.hpp"
+#include "modup.hpp"
 #include "ntt_engine.hpp"
 
 using namespace gf;
@@ -202,4 +203,127 @@ extern "C" uint64_t emu_shoup(uint64_t x, uint64_t w, uint64_t q) {
 extern "C" uint64_t emu_mulhi(uint64_t a, uint64_t b) { return mulhi64(a, b); }
 extern "C" uint64_t emu_redc(uint64_t lo, uint64_t hi, uint64_t q) {
     return redc(U128{lo, hi}, q, host_qneg(q));
+}
+
+namespace {
+std::vector<uint64_t> paired(const uint64_t* tw, const uint64_t* twp, uint32_t n) {
+    std::vector<uint64_t> t2(2 * (std::size_t)n);
+    for (uint32_t i = 0; i < n; ++i) {
+        t2[2 * i] = tw[i];
+        t2[2 * i + 1] = twp[i];
+    }
+    return t2;
+}
+
+template <uint32_t SP, uint32_t S1>
+void emu_modup_t(const ModupJob& j, const NttGeom& g) {
+    std::vector<uint64_t> lds(MODUP_TILE);
+    std::vector<ModupUnit<SP>> units(MODUP_THREADS);
+    for (uint32_t tile = 0; tile < g.n / MODUP_TILE; ++tile) {
+        for (uint32_t i = 0; i < SP; ++i) {
+            modup_source_pass<S1>(j, i, g, tile, lds.data(), 0, 1, NoSync{});
+            for (uint32_t u = 0; u < MODUP_THREADS; ++u) modup_take_unit<SP, S1>(j, i, lds.data(), u, units[u]);
+        }
+        for (uint32_t t = 0; t < j.sc; ++t) {
+            for (uint32_t u = 0; u < MODUP_THREADS; ++u) modup_convert_unit<SP, S1>(j, t, lds.data(), u, units[u]);
+            modup_target_pass<S1>(j, t, g, tile, lds.data(), 0, 1, NoSync{});
+        }
+    }
+}
+}  // namespace
+
+// The fused ModUp over one part: src[i] (i < sp) holds the inverse row-pass
+// output of source i (modulus sq[i], tables tw/twp of sw[i]), post/postp its
+// final factors; targets j < sc with moduli tq[j] (tables ttw/ttwp[j]), tab
+// [sp][sc] in Montgomery form; dst[j] receives target j's column-pass output.
+// ref != 0 runs the unfused kernels' order instead: each source's whole
+// inverse column pass (standard tiles, radix rmax), the conversion with the
+// conversion kernel's arithmetic, each target's forward column pass.
+extern "C" void emu_modup(uint32_t n, uint32_t sp, uint32_t sc, const uint64_t* const* src, const uint64_t* sq,
+                          const uint64_t* const* sw, const uint64_t* const* swp, const uint64_t* post,
+                          const uint64_t* postp, const uint64_t* tq, const uint64_t* const* ttw,
+                          const uint64_t* const* ttwp, const uint64_t* tab, uint64_t* dst, int ref, uint32_t rmax) {
+    const NttGeom g = make_geom(n, rmax);
+    std::vector<std::vector<uint64_t>> stab(sp), ttab(sc);
+    for (uint32_t i = 0; i < sp; ++i) stab[i] = paired(sw[i], swp[i], n);
+    for (uint32_t t = 0; t < sc; ++t) ttab[t] = paired(ttw[t], ttwp[t], n);
+    std::vector<uint64_t> tqn(sc), tred(sc), ttw_addr(sc);
+    for (uint32_t t = 0; t < sc; ++t) {
+        tqn[t] = host_qneg(tq[t]);
+        tred[t] = lazy_red_of(tq[t]);
+        ttw_addr[t] = (uint64_t)(uintptr_t)ttab[t].data();
+    }
+    if (!ref) {
+        ModupJob J{};
+        for (uint32_t i = 0; i < sp; ++i) {
+            J.src[i] = src[i];
+            J.stw[i] = stab[i].data();
+            J.sq[i] = sq[i];
+            J.post[i] = post[i];
+            J.postp[i] = postp[i];
+        }
+        J.dst = dst;
+        J.tab = tab;
+        J.tq = tq;
+        J.tqneg = tqn.data();
+        J.ttw = ttw_addr.data();
+        J.tred = tred.data();
+        J.sp = sp;
+        J.sc = sc;
+        with_split(g.logN1, g.logN2, [&](auto A, auto) {
+            constexpr uint32_t S1 = decltype(A)::value;
+            switch (sp) {
+                case 1: emu_modup_t<1, S1>(J, g); break;
+                case 2: emu_modup_t<2, S1>(J, g); break;
+                case 3: emu_modup_t<3, S1>(J, g); break;
+                case 4: emu_modup_t<4, S1>(J, g); break;
+                case 5: emu_modup_t<5, S1>(J, g); break;
+                case 6: emu_modup_t<6, S1>(J, g); break;
+            }
+        });
+        return;
+    }
+    std::vector<uint64_t> x((size_t)sp * n), lds(NTT_TILE_PADDED);
+    with_radix(rmax, [&](auto Rc) {
+        constexpr uint32_t RM = decltype(Rc)::value;
+        with_split(g.logN1, g.logN2, [&](auto A, auto) {
+            constexpr uint32_t S1 = decltype(A)::value;
+            const uint32_t tilesA = g.n / NTT_TILE;
+            for (uint32_t i = 0; i < sp; ++i) {
+                std::copy(src[i], src[i] + n, x.begin() + (size_t)i * n);
+                NttJob jn{nullptr, x.data() + (size_t)i * n, stab[i].data(), nullptr, sq[i], post[i], postp[i]};
+                for (uint32_t tile = 0; tile < tilesA; ++tile)
+                    phaseA_body<false, false, true, S1, RM>(jn, g, tile, lds.data(), 0, 1, NoSync{});
+            }
+            for (uint32_t t = 0; t < sc; ++t) {
+                uint64_t* y = dst + (size_t)t * n;
+                for (uint32_t k = 0; k < n; ++k) {
+                    if (sp <= 8) {
+                        uint32_t lo[8], hi[8];
+                        for (uint32_t i = 0; i < sp; ++i) {
+                            lo[i] = split30_lo(x[(size_t)i * n + k]);
+                            hi[i] = split30_hi(x[(size_t)i * n + k]);
+                        }
+                        U128 acc{0, 0};
+                        auto b = [&](uint32_t i) { return tab[(size_t)i * sc + t]; };
+                        switch (sp) {
+                            case 1: acc = dot30<1>(lo, hi, b); break;
+                            case 2: acc = dot30<2>(lo, hi, b); break;
+                            case 3: acc = dot30<3>(lo, hi, b); break;
+                            case 4: acc = dot30<4>(lo, hi, b); break;
+                            case 5: acc = dot30<5>(lo, hi, b); break;
+                            case 6: acc = dot30<6>(lo, hi, b); break;
+                        }
+                        y[k] = redc_lazy(acc, tq[t], tqn[t]);
+                    }
+                }
+                NttJob jt{y, y, ttab[t].data(), nullptr, tq[t], 0, 0};
+                jt.red = tred[t];
+                for (uint32_t tile = 0; tile < tilesA; ++tile) {
+                    if (jt.red) phaseA_body<true, true, true, S1, RM>(jt, g, tile, lds.data(), 0, 1, NoSync{});
+                    else phaseA_body<true, false, true, S1, RM>(jt, g, tile, lds.data(), 0, 1, NoSync{});
+                }
+            }
+        });
+    });
 }

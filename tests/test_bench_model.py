@@ -12,6 +12,7 @@ import bench  # noqa: E402
 N, L, K, ALPHA, NP, B = 1 << 16, 24, 6, 6, 4, 64
 T = 8 * N                                   # one tower
 FUSED_KSI = ["tensor", "ntt_inv", "conv", "ntt_fwd_a", "ksi", "ksi", "ntt_inv", "rsv", "ntt_fwd"]
+FUSED_MODUP = ["tensor", "ntt_inv_b", "ntt_inv", "modup", "ksi", "ksi", "ntt_inv", "rsv", "ntt_fwd"]
 
 
 def test_fused_plan_steps():
@@ -57,3 +58,20 @@ def test_unfused_plan_has_the_reference_stage_order():
              "ntt_inv", "switch", "ntt_fwd", "fuse"]
     names = [s[0] for s in bench.step_bytes(N, L, K, ALPHA, NP, kinds, B)]
     assert names[3:5] == ["ntt_digits", "key_inner_product"] and names[-1] == "rescale_fuse"
+
+
+def test_fused_modup_plan_steps():
+    """With the fused ModUp the same algorithmic bytes are charged as the
+    unfused stages they replace: the inverse transforms (24 towers, row pass
+    here, column pass in the fused kernel), the conversion and the digits'
+    column pass; the extra inverse transforms of t0/t1's last tower stand
+    alone."""
+    fused = {s[0]: s[1] for s in bench.step_bytes(N, L, K, ALPHA, NP, FUSED_MODUP, B)}
+    apart = {s[0]: s[1] for s in bench.step_bytes(N, L, K, ALPHA, NP, FUSED_KSI, B)}
+    assert list(fused) == ["tensor", "intt_modup_rowpass", "intt_extra", "modup_fused", "rowpass_inner_q",
+                           "rowpass_inner_p", "intt_moddown", "moddown_rescale_prologue", "ntt_rescale_epilogue"]
+    assert fused["intt_modup_rowpass"] == T * 24
+    assert fused["intt_extra"] == 2 * T * 2
+    # INTT column passes 24 T + conversion (24 + 96) T + digit column passes 96 T
+    assert fused["modup_fused"] == T * (24 + 24 + 96 + 96)
+    assert sum(fused.values()) == sum(apart.values())

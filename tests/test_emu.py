@@ -12,6 +12,7 @@ import numpy as np
 import pytest
 
 import oracle as O
+from common import make_params
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
@@ -48,6 +49,10 @@ def emu():
     lib.emu_inner_red.argtypes = [u64p, u64p, C.c_uint32, C.c_uint64]
     lib.emu_reduce_red.restype = C.c_uint64
     lib.emu_reduce_red.argtypes = [C.c_uint64, C.c_uint64]
+    pp = C.POINTER(u64p)
+    lib.emu_modup.restype = None
+    lib.emu_modup.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, pp, u64p, pp, pp, u64p, u64p, u64p, pp, pp, u64p,
+                              u64p, C.c_int, C.c_uint32]
     yield lib
     shutil.rmtree(d, ignore_errors=True)
 
@@ -267,3 +272,66 @@ def test_fused_rowpass_inner_product(emu, logn, np_, xf):
                     o0, o1 = o0 % np.uint64(q), o1 % np.uint64(q)
                 assert np.array_equal(o0, w0), (bits, rmax, pairs, lazy_out)
                 assert np.array_equal(o1, w1), (bits, rmax, pairs, lazy_out)
+
+
+@pytest.mark.parametrize("logn,L,dnum,part", [(13, 6, 3, 0), (14, 5, 1, 0), (15, 12, 3, 2), (16, 24, 4, 0),
+                                              (16, 24, 4, 3), (17, 3, 1, 0)])
+def test_fused_modup_equals_unfused(emu, logn, L, dnum, part):
+    """The fused ModUp (the sources' inverse column pass, the conversion to
+    the part's complement and the digits' forward column pass in one
+    workgroup per column tile) stores exactly what the unfused kernels store:
+    each source's inverse column pass (standard tiles, radix 2^3 and 2^4), the
+    conversion kernel's sums, each digit's forward column pass, over every
+    ring split (2^13 .. 2^17), 1 to 6 sources, 50-bit (unreduced) and 60-bit
+    (reduced butterflies) targets, inputs up to 4q - 1."""
+    n = 1 << logn
+    P = make_params(n, L, dnum)
+    alpha = -(-L // dnum)
+    st = alpha * part
+    sp = min(alpha, L - st)
+    srcm = [int(m) for m in P.q[st:st + sp]]
+    tgt = [int(m) for i, m in enumerate(P.q) if not st <= i < st + sp] + [int(m) for m in P.p]
+    sc = len(tgt)
+    rng = np.random.default_rng(logn * 31 + part)
+    R = 1 << 64
+    src, sw, swp, post, postp = [], [], [], [], []
+    for i, q in enumerate(srcm):
+        x = rng.integers(0, 4 * q - 1, n, dtype=np.uint64)
+        x[:5] = 4 * q - 1
+        src.append(np.ascontiguousarray(x))
+        _, _, ir, ip, ni, _ = O.root_tables(n, q, int(P.roots[st + i]))
+        sw.append(ir)
+        swp.append(ip)
+        qhat = 1
+        for k, m in enumerate(srcm):
+            if k != i:
+                qhat = qhat * m
+        v = pow(qhat % q, -1, q) * int(ni) % q
+        post.append(v)
+        postp.append((v << 64) // q)
+    tw, twp = [], []
+    roots = [int(r) for r in P.roots]
+    mods = [int(m) for m in P.mods]
+    for t in tgt:
+        fr, fp, _, _, _, _ = O.root_tables(n, t, roots[mods.index(t)])
+        tw.append(fr)
+        twp.append(fp)
+    tab = np.zeros(sp * sc, np.uint64)
+    for i in range(sp):
+        qhat = 1
+        for k, m in enumerate(srcm):
+            if k != i:
+                qhat *= m
+        for j, t in enumerate(tgt):
+            tab[i * sc + j] = (qhat % t) * R % t
+    arr = lambda v: (u64p * len(v))(*[O.P(np.ascontiguousarray(a, dtype=np.uint64)) for a in v])  # noqa: E731
+    keep = [src, sw, swp, tw, twp]
+    outs = {}
+    for ref, rmax in ((0, 3), (1, 3), (1, 4)):
+        dst = np.zeros(sc * n, np.uint64)
+        emu.emu_modup(n, sp, sc, arr(src), O.P(O.U(srcm)), arr(sw), arr(swp), O.P(O.U(post)), O.P(O.U(postp)),
+                      O.P(O.U(tgt)), arr(tw), arr(twp), O.P(tab), O.P(dst), ref, rmax)
+        outs[(ref, rmax)] = dst
+    assert keep
+    assert np.array_equal(outs[(0, 3)], outs[(1, 3)])
+    assert np.array_equal(outs[(0, 3)], outs[(1, 4)])

@@ -338,17 +338,19 @@ def test_batch_plan_uneven_levels_and_graph(G):
         assert np.array_equal(outs[i][1].download(), want[i][1]), ("per job", i)
 
 
-@pytest.mark.parametrize("ksi_extra,conv_chunk,rs_chunk", [(0, 8, 8), (1, 16, 32), (1, 32, 16), (0, 5, 3)])
-def test_plan_knobs_keep_results(G, ksi_extra, conv_chunk, rs_chunk):
+@pytest.mark.parametrize("ksi_extra,conv_chunk,rs_chunk,modup_fused", [(0, 8, 8, 1), (1, 16, 32, 1), (1, 32, 16, 0),
+                                                                      (0, 5, 3, 0)])
+def test_plan_knobs_keep_results(G, ksi_extra, conv_chunk, rs_chunk, modup_fused):
     """Every setting of the autotuner's plan knobs (extra inner-product wave,
     conversion and prologue chunk widths, re-chunking off so the widths take
-    effect) gives the oracle's results, on a batch of uneven levels (parts of
-    3 and 2 towers; ragged chunks with widths 5 and 3); the autotuner's own
-    search leaves a report naming its choices."""
+    effect, the fused or the unfused ModUp) gives the oracle's results, on a
+    batch of uneven levels (parts of 3 and 2 towers; ragged chunks with widths
+    5 and 3); the autotuner's own search leaves a report naming its
+    choices."""
     n, L, dnum = 1 << 13, 6, 3
     P, ctx, k, av, bv = setup(G, n, L, dnum, seed=77)   # a context of its own: the knobs stay here
     for name, v in (("ksi_extra", ksi_extra), ("conv_chunk", conv_chunk), ("rs_chunk", rs_chunk),
-                    ("rechunk_wgs", 0)):
+                    ("modup_fused", modup_fused), ("modup_min_wgs", 0), ("rechunk_wgs", 0)):
         ctx.set_knob(name, v)
         assert ctx.knob(name) == v
     rng = np.random.default_rng(11 + conv_chunk)
@@ -372,7 +374,7 @@ def test_plan_knobs_keep_results(G, ksi_extra, conv_chunk, rs_chunk):
         ctx.set_knob("rechunk_wgs", 8192)
         ctx.autotune(s)
         rep = ctx.tuning_report()
-        for name in ("ntt_radix", "ksi_extra", "conv_chunk", "rs_chunk"):
+        for name in ("ntt_radix", "ksi_extra", "conv_chunk", "rs_chunk", "modup_fused"):
             assert rep[name]["chosen"] == ctx.knob(name), name
 
 

@@ -10,13 +10,13 @@ namespace gf {
 // unit arrays are indexed by constants and stay in registers)
 template <uint32_t S1, uint32_t SP, uint32_t I>
 __device__ __forceinline__ void modup_sources(const ModupJob& j, const NttGeom& g, uint64_t* lds, uint32_t tid,
-                                              ModupUnit<SP>& x) {
+                                              uint32_t tile, ModupUnit<SP>& x) {
     if constexpr (I < SP) {
-        modup_source_pass<S1>(j, I, g, blockIdx.x, lds, tid, MODUP_THREADS, DevSync{});
+        modup_source_pass<S1>(j, I, g, tile, lds, tid, MODUP_THREADS, DevSync{});
         __syncthreads();
         modup_take_unit<SP, S1>(j, I, lds, tid, x);
         __syncthreads();
-        modup_sources<S1, SP, I + 1>(j, g, lds, tid, x);
+        modup_sources<S1, SP, I + 1>(j, g, lds, tid, tile, x);
     }
 }
 
@@ -25,12 +25,19 @@ __global__ __launch_bounds__(MODUP_THREADS) __attribute__((amdgpu_waves_per_eu(4
     __shared__ uint64_t lds[MODUP_TILE];
     const ModupJob& j = jobs[off + blockIdx.y];
     const uint32_t tid = threadIdx.x;
+    // Workgroups go to the 8 XCDs in turn (linear id mod 8, and gridDim.x is
+    // a multiple of 8 when remapped), each with its own L2.  A tile's rows are
+    // 2^(11 - S1) words, shorter than a 128-byte L2 line at n = 2^16 and
+    // above, so neighbouring tiles share lines: give each XCD a run of
+    // consecutive tiles of the tower instead of every eighth one.
+    const uint32_t nt = gridDim.x;
+    const uint32_t tile = nt % 8 ? blockIdx.x : (blockIdx.x % 8) * (nt / 8) + blockIdx.x / 8;
     ModupUnit<SP> x;
-    modup_sources<S1, SP, 0>(j, g, lds, tid, x);
+    modup_sources<S1, SP, 0>(j, g, lds, tid, tile, x);
     for (uint32_t t = 0; t < j.sc; ++t) {
         modup_convert_unit<SP, S1>(j, t, lds, tid, x);
         __syncthreads();
-        modup_target_pass<S1>(j, t, g, blockIdx.x, lds, tid, MODUP_THREADS, DevSync{});
+        modup_target_pass<S1>(j, t, g, tile, lds, tid, MODUP_THREADS, DevSync{});
         __syncthreads();
     }
 }

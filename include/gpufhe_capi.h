@@ -289,10 +289,11 @@ int gf_ctx_autotune(gf_ctx* c, hipStream_t s, uint32_t* rmax_out, float* ms_out)
  * plan and times the stages the knob shapes): "ksi_extra" (one
  * extra wave in the fused row pass + inner product workgroups, 0/1),
  * "conv_chunk" and "rs_chunk" (targets per ModUp-conversion / rescale-prologue
- * job).  The report is JSON with every candidate's time, the radix search of
+ * job), "modup_fused" (ModUp as one kernel per part and column tile, 0/1).
+ * The report is JSON with every candidate's time, the radix search of
  * gf_ctx_autotune included; valid until the next tuning call.  Knobs (and
- * "ntt_radix", "rechunk_wgs") can be set by name; plans built afterwards use
- * them, and results are identical for every setting. */
+ * "ntt_radix", "rechunk_wgs", "modup_min_wgs") can be set by name; plans built
+ * afterwards use them, and results are identical for every setting. */
 int gf_ctx_autotune_plan(gf_ctx* c, uint32_t batch, hipStream_t s);
 const char* gf_ctx_tuning_report(const gf_ctx* c);
 int gf_ctx_set_knob(gf_ctx* c, const char* name, uint32_t value);

@@ -38,6 +38,16 @@ GF_HD uint64_t opaque64(uint64_t v) {
     return v;
 }
 
+// The same for a 32-bit value.  Inside a loop it keeps a loop-invariant
+// 32-bit operand from being widened to 64 bits once, before the loop, and
+// held in a register pair.
+GF_HD uint32_t opaque32(uint32_t v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm("" : "+v"(v));
+#endif
+    return v;
+}
+
 // a + b of two 32-bit words as a 33-bit value (carry from the add itself)
 GF_HD uint64_t add33(uint32_t a, uint32_t b) {
     uint32_t s;

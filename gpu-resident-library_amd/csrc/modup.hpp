@@ -10,8 +10,9 @@
 // column s = unit & (C - 1), rows l + u TL (l = unit >> log C,
 // TL = 2^(S1 - 2), u < 4), LDS slot s + (l + u TL) C.  The inverse
 // transforms' last pass is the same chunk, so after it each thread keeps its
-// unit of every source in registers (reduced, times n^-1 QHatInv: 8 VGPRs
-// per source); for each target it forms the conversion of its
+// unit of every source in registers (reduced, times n^-1 QHatInv, split into
+// the 30-bit halves the conversion sums take: 8 VGPRs per source); for each
+// target it forms the conversion of its
 // 4 elements, stores them at their slots, and the target's forward column
 // pass runs from LDS to HBM.  The coefficient-form sources and digits never
 // travel through HBM.  Every butterfly, product and reduction is the one the
@@ -56,10 +57,11 @@ struct ModupJob {
     uint32_t sp, sc;
 };
 
-// one thread's unit of every source: element u of source i (reduced, below 2^60)
+// one thread's unit of every source: element u of source i (reduced, below
+// 2^60) as its low and high 30 bits
 template <uint32_t SP>
 struct ModupUnit {
-    uint64_t v[MODUP_UNIT][SP];
+    uint32_t lo[MODUP_UNIT][SP], hi[MODUP_UNIT][SP];
 };
 
 template <uint32_t S1>
@@ -86,27 +88,27 @@ GF_HD void modup_take_unit(const ModupJob& j, uint32_t i, const uint64_t* lds, u
 #pragma unroll
     for (uint32_t u = 0; u < MODUP_UNIT; ++u) {
         // the inverse transform's final store: times n^-1 QHatInv, reduced
-        x.v[u][i] = shoup(lds[modup_slot<S1>(unit, u)], post, postp, q);
+        const uint64_t v = shoup(lds[modup_slot<S1>(unit, u)], post, postp, q);
+        x.lo[u][i] = split30_lo(v);
+        x.hi[u][i] = split30_hi(v);
     }
 }
 // target t's conversion of this thread's unit into LDS: the exact column sums
 // over the part's sources, REDC without its correction ([0, 2 q_t)), as the
 // conversion kernel leaves them for the forward transforms
 template <uint32_t SP, uint32_t S1>
-GF_HD void modup_convert_unit(const ModupJob& j, uint32_t t, uint64_t* lds, uint32_t unit, const ModupUnit<SP>& x) {
+GF_HD void modup_convert_unit(const ModupJob& j, uint32_t t, uint64_t* lds, uint32_t unit, ModupUnit<SP>& x) {
     const uint64_t q = uniform_at(j.tq, t), qn = uniform_at(j.tqneg, t);
 #pragma unroll
     for (uint32_t u = 0; u < MODUP_UNIT; ++u) {
-        uint32_t lo[SP], hi[SP];
 #pragma unroll
         for (uint32_t i = 0; i < SP; ++i) {
-            // opaque: the split is redone per target instead of being hoisted
-            // out of the target loop, which would keep it live beside x
-            const uint64_t v = opaque64(x.v[u][i]);
-            lo[i] = split30_lo(v);
-            hi[i] = split30_hi(v);
+            // in place, so that each target's sums take the 32-bit halves as
+            // they are (no 64-bit copies made once before the target loop)
+            x.lo[u][i] = opaque32(x.lo[u][i]);
+            x.hi[u][i] = opaque32(x.hi[u][i]);
         }
-        const U128 acc = dot30<SP>(lo, hi, [&](uint32_t s) { return uniform_at(j.tab, s * j.sc + t); });
+        const U128 acc = dot30<SP>(x.lo[u], x.hi[u], [&](uint32_t s) { return uniform_at(j.tab, s * j.sc + t); });
         lds[modup_slot<S1>(unit, u)] = redc_lazy(acc, q, qn);
     }
 }

@@ -27,9 +27,9 @@ __global__ __launch_bounds__(MODUP_THREADS) __attribute__((amdgpu_waves_per_eu(4
     const uint32_t tid = threadIdx.x;
     // Workgroups go to the 8 XCDs in turn (linear id mod 8, and gridDim.x is
     // a multiple of 8 when remapped), each with its own L2.  A tile's rows are
-    // 2^(11 - S1) words, shorter than a 128-byte L2 line at n = 2^16 and
-    // above, so neighbouring tiles share lines: give each XCD a run of
-    // consecutive tiles of the tower instead of every eighth one.
+    // 2^(10 - S1) words, shorter than a 128-byte L2 line at every ring size,
+    // so neighbouring tiles share lines: give each XCD a run of consecutive
+    // tiles of the tower instead of every eighth one.
     const uint32_t nt = gridDim.x;
     const uint32_t tile = nt % 8 ? blockIdx.x : (blockIdx.x % 8) * (nt / 8) + blockIdx.x / 8;
     ModupUnit<SP> x;

@@ -6,7 +6,7 @@
 //
 // The tile is MODUP_TILE elements: C = MODUP_TILE >> S1 columns of all
 // N1 = 2^S1 rows.  With radix-4 register passes a column transform's first
-// pass (chunk 0, stages 0..1) has exactly one unit per thread (512 threads):
+// pass (chunk 0, stages 0..1) has exactly one unit per thread (256 threads):
 // column s = unit & (C - 1), rows l + u TL (l = unit >> log C,
 // TL = 2^(S1 - 2), u < 4), LDS slot s + (l + u TL) C.  The inverse
 // transforms' last pass is the same chunk, so after it each thread keeps its
@@ -36,7 +36,7 @@ GF_HD T uniform_at(const T* p, uint32_t i) {
 }
 
 enum {
-    MODUP_LOG_TILE = 11,
+    MODUP_LOG_TILE = 10,                           // 1024 elements: 4 workgroups of 4 waves per CU
     MODUP_TILE = 1 << MODUP_LOG_TILE,
     MODUP_RM = 2,                                  // register radix 2^2: 4 elements per unit
     MODUP_UNIT = 1 << MODUP_RM,

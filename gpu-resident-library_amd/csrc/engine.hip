@@ -27,7 +27,9 @@ RootTable root_table(uint32_t n, uint64_t q, uint64_t root) {
     if (it != cache.end()) return it->second;
     uint32_t logn = 0;
     while ((1u << logn) < n) ++logn;
-    std::vector<uint64_t> h((size_t)8 * n), pw(n);
+    // moduli below 2^50 also get the fp64 pairs of both tables (ntt_f64.hpp)
+    const bool f64 = f64_modulus(q);
+    std::vector<uint64_t> h((size_t)(f64 ? 12 : 8) * n), pw(n);
     const uint64_t bases[2] = {root, host_invmod(root, q)};
     for (int s = 0; s < 2; ++s) {
         pw[0] = 1;
@@ -42,6 +44,13 @@ RootTable root_table(uint32_t n, uint64_t q, uint64_t root) {
             uint64_t* pr = h.data() + (size_t)(4 + 2 * s) * n;   // paired copy
             pr[2 * i] = tr[i];
             pr[2 * i + 1] = tp[i];
+            if (f64) {
+                double wd, wq;
+                f64_twiddle(tr[i], q, wd, wq);
+                uint64_t* pd = h.data() + (size_t)(8 + 2 * s) * n;
+                std::memcpy(pd + 2 * i, &wd, 8);
+                std::memcpy(pd + 2 * i + 1, &wq, 8);
+            }
         }
     }
     void* d = nullptr;
@@ -55,6 +64,8 @@ RootTable root_table(uint32_t n, uint64_t q, uint64_t root) {
     t.ip = b + 3 * (size_t)n;
     t.fw2 = b + 4 * (size_t)n;
     t.iv2 = b + 6 * (size_t)n;
+    t.fw2d = f64 ? b + 8 * (size_t)n : nullptr;
+    t.iv2d = f64 ? b + 10 * (size_t)n : nullptr;
     t.ninv = host_invmod(n, q);
     t.ninvp = host_precon(t.ninv, q);
     cache.emplace(key, t);
@@ -173,16 +184,22 @@ Ctx::Ctx(uint32_t n_, const std::vector<uint64_t>& q, const std::vector<uint64_t
                 tq[j] = mods[P.compl_rows[j]];
                 tqn[j] = host_qneg(tq[j]);
             }
-            std::vector<uint64_t> ttw(P.sc), tred(P.sc);
+            std::vector<uint64_t> ttw(P.sc), tred(P.sc), ttwd(P.sc), tqf(2 * (size_t)P.sc);
             for (uint32_t j = 0; j < P.sc; ++j) {
                 ttw[j] = (uint64_t)(uintptr_t)rt[P.compl_rows[j]].fw2;
                 tred[j] = lazy_red_of(tq[j]);
+                ttwd[j] = (uint64_t)(uintptr_t)rt[P.compl_rows[j]].fw2d;
+                const double qd = (double)tq[j], qi = 1.0 / qd;
+                std::memcpy(&tqf[2 * j], &qd, 8);
+                std::memcpy(&tqf[2 * j + 1], &qi, 8);
             }
             put(tab, &P.d_tab);
             put(tq, &P.d_tq);
             put(tqn, &P.d_tqneg);
             put(ttw, &P.d_ttw);
             put(tred, &P.d_tred);
+            put(ttwd, &P.d_ttwd);
+            put(tqf, &P.d_tqf);
         }
         // ModDown P -> Q_l
         std::vector<uint64_t> tab((size_t)K * l), tq(l), tqn(l);
@@ -233,6 +250,7 @@ Ctx::Ctx(uint32_t n_, const std::vector<uint64_t>& q, const std::vector<uint64_t
             put(tg, (const uint64_t**)&c.d_rstg);
         }
     }
+    put(std::vector<uint64_t>(L + K, 0), &d_zeros);
     consts.alloc(blob.size() * 8);
     hip_check(hipMemcpy(consts.p, blob.data(), blob.size() * 8, hipMemcpyHostToDevice), "upload constants");
     for (auto& f : fix) *f.dst = consts.as<uint64_t>() + f.off;
@@ -352,6 +370,9 @@ void Ctx::set_knob(const std::string& name, uint32_t v) {
     } else if (name == "modup_fused") {
         if (v > 1) throw std::runtime_error("gpufhe: modup_fused must be 0 or 1");
         modup_fused = v;
+    } else if (name == "modup_f64") {
+        if (v > 1) throw std::runtime_error("gpufhe: modup_f64 must be 0 or 1");
+        modup_f64 = v;
     } else if (name == "modup_min_wgs") {
         modup_min_wgs = v;
     } else if (name == "rechunk_wgs") {
@@ -366,6 +387,7 @@ uint32_t Ctx::knob(const std::string& name) const {
     if (name == "conv_chunk") return conv_chunk;
     if (name == "rs_chunk") return rs_chunk;
     if (name == "modup_fused") return modup_fused;
+    if (name == "modup_f64") return modup_f64;
     if (name == "modup_min_wgs") return modup_min_wgs;
     if (name == "rechunk_wgs") return rechunk_wgs;
     throw std::runtime_error("gpufhe: unknown tuning knob " + name);
@@ -457,8 +479,13 @@ void Ctx::autotune_plan(hipStream_t s, uint32_t B) {
     modup_fused = 0;
     tune("conv_chunk", conv_chunk, {8, 16, 32}, {Step::CONV});
     modup_fused = mf;
-    if (fused && !geom.single && n % KSI_TILE == 0 && n % MODUP_TILE == 0 && !wide && alpha <= MODUP_MAXSRC)
+    if (fused && !geom.single && n % KSI_TILE == 0 && n % MODUP_TILE == 0 && !wide && alpha <= MODUP_MAXSRC) {
+        // fp64 column passes inside the fused ModUp, timed with it fused
+        modup_fused = 1;
+        tune("modup_f64", modup_f64, {0, 1}, {Step::MODUP});
+        modup_fused = mf;
         tune("modup_fused", modup_fused, {0, 1}, {Step::NTT_IB, Step::NTT_I, Step::MODUP, Step::CONV, Step::NTT_FA});
+    }
     if (fused) tune("rs_chunk", rs_chunk, {8, 16, 32}, {Step::RSV});
     rep << "}";
     (void)hipEventDestroy(e0);
@@ -973,8 +1000,16 @@ void PlanBuilder::modup(uint32_t l, const uint64_t* d, uint64_t* ws, const uint6
                 J.sq[i] = C.mods[r];
                 J.post[i] = P.post[i];
                 J.postp[i] = P.postp[i];
+                if (C.modup_f64 && C.rt[r].iv2d) {
+                    J.stwd[i] = C.rt[r].iv2d;
+                    J.sqd[i] = (double)C.mods[r];
+                    J.sqinv[i] = 1.0 / J.sqd[i];
+                    f64_twiddle(P.post[i], C.mods[r], J.spost[i], J.spostq[i]);
+                }
             }
             J.dst = dp;
+            J.ttwd = C.modup_f64 ? P.d_ttwd : C.d_zeros;
+            J.tqf = P.d_tqf;
             J.tab = P.d_tab;
             J.tq = P.d_tq;
             J.tqneg = P.d_tqneg;

@@ -45,6 +45,9 @@ struct DevBuf {
 struct RootTable {
     const uint64_t *fr, *fp, *ir, *ip;   // reference layout (separate arrays)
     const uint64_t *fw2, *iv2;           // interleaved (w, precon) pairs used by the plans
+    // moduli below 2^50: the same twiddles as fp64 pairs (w centred, fl(w/q);
+    // ntt_f64.hpp), else null
+    const uint64_t *fw2d = nullptr, *iv2d = nullptr;
     uint64_t ninv, ninvp;
 };
 RootTable root_table(uint32_t n, uint64_t q, uint64_t root);
@@ -65,6 +68,8 @@ struct PartConst {
     const uint64_t* d_tqneg = nullptr;
     const uint64_t* d_ttw = nullptr;           // [sc] addresses of the targets' paired forward tables
     const uint64_t* d_tred = nullptr;          // [sc] lazy_red_of(target modulus)
+    const uint64_t* d_ttwd = nullptr;          // [sc] addresses of the targets' fp64 forward pairs, or 0
+    const uint64_t* d_tqf = nullptr;           // [2 sc] target q and fl(1/q) as doubles
 };
 struct LevelConst {
     uint32_t l = 0, np = 0;
@@ -92,6 +97,7 @@ public:
     std::vector<RootTable> rt;
     std::vector<LevelConst> lv;               // index l = 1..L (0 unused)
     DevBuf consts;                            // all per-level device constants
+    const uint64_t* d_zeros = nullptr;        // L + K zero words (a target table with no fp64 target)
     // EvalMult+Rescale with merged ModDown/rescale, and keyswitches whose
     // digit row pass feeds the key inner product directly (KSI)
     bool fused = true;
@@ -117,6 +123,9 @@ public:
     // 1: ModUp's inverse column pass, conversion and digit column pass run as
     // one kernel (modup.hpp) where the shape allows it
     uint32_t modup_fused = 1;
+    // 1: the fused ModUp runs the column passes of moduli below 2^50 in
+    // double precision (ntt_f64.hpp)
+    uint32_t modup_f64 = 1;
     // a plan whose fused ModUp stage would launch fewer workgroups than this
     // runs it unfused (a single operation's stage cannot fill the chip)
     uint32_t modup_min_wgs = 2048;
@@ -130,7 +139,7 @@ public:
     void autotune_plan(hipStream_t s, uint32_t batch);
     std::string tune_report;
     // sets or reads a knob by name ("ntt_radix", "ksi_extra", "conv_chunk",
-    // "rs_chunk", "modup_fused", "modup_min_wgs", "rechunk_wgs")
+    // "rs_chunk", "modup_fused", "modup_f64", "modup_min_wgs", "rechunk_wgs")
     void set_knob(const std::string& name, uint32_t v);
     uint32_t knob(const std::string& name) const;
     std::string radix_report;   // the radix search of autotune(), as JSON

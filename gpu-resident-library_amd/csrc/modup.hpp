@@ -18,8 +18,16 @@
 // travel through HBM.  Every butterfly, product and reduction is the one the
 // unfused kernels apply, so the stored digit column
 // passes equal theirs bit for bit.
+//
+// Sources and targets whose modulus is below 2^50 run their column passes in
+// double precision (ntt_f64.hpp) when the job gives their fp64 tables; the
+// values stored for the digits' row pass are then other representatives of
+// the same residues (in [0, 2q) instead of the integer pass's lazy range),
+// and everything from the sources' reduced values to the conversion sums is
+// identical.
 #pragma once
 #include "ntt_engine.hpp"
+#include "ntt_f64.hpp"
 
 namespace gf {
 
@@ -55,6 +63,34 @@ struct ModupJob {
     const uint64_t* ttw;                  // [sc] addresses of the targets' paired forward tables
     const uint64_t* tred;                 // [sc] lazy_red_of(target modulus)
     uint32_t sp, sc;
+    // fp64 passes (moduli below 2^50).  Source i: its inverse pairs (TwF64
+    // layout; null: the integer passes), q, fl(1/q), and n^-1 QHatInv centred
+    // with its fl(./q).  Target t: ttwd[t] the address of its forward pairs
+    // (0: the integer passes), tqf[2t], tqf[2t+1] the bit patterns of q and
+    // fl(1/q).
+    const uint64_t* stwd[MODUP_MAXSRC];
+    double sqd[MODUP_MAXSRC], sqinv[MODUP_MAXSRC], spost[MODUP_MAXSRC], spostq[MODUP_MAXSRC];
+    const uint64_t* ttwd;
+    const uint64_t* tqf;
+};
+
+// Column access in fp64: the inverse transforms' row-pass output (lazy, below
+// 4q) in, centred; the digits' column-pass output out, in [0, 2q)
+struct ColInF64 {
+    const GF_GLOBAL uint64_t* src;
+    uint32_t logN2, b0;
+    double q, qinv, c2;   // c2 = 2^52 + 2q
+    GF_HD double operator()(uint32_t s, uint32_t k, uint32_t) const {
+        return f64_center(f64_from_u52(gat(src, (k << logN2) + b0 + s), c2), q, qinv);
+    }
+};
+struct ColOutF64 {
+    GF_GLOBAL uint64_t* dst;
+    uint32_t logN2, b0;
+    double q, qinv;
+    GF_HD void operator()(uint32_t s, uint32_t k, uint32_t, double v) const {
+        gat(dst, (k << logN2) + b0 + s) = u52_from_f64(f64_center(v, q, qinv) + q);
+    }
 };
 
 // one thread's unit of every source: element u of source i (reduced, below
@@ -64,11 +100,44 @@ struct ModupUnit {
     uint32_t lo[MODUP_UNIT][SP], hi[MODUP_UNIT][SP];
 };
 
+// LDS word of tile slot `at` (column s, row k at k C + s): bits 4..6 xor-ed
+// into bits 2..4.  In the last forward pass (first inverse pass) a unit is 4
+// consecutive rows of one column, and the 16 lanes that share a bank cycle
+// would otherwise meet on 4 bank pairs at C = 4 (n = 2^16); with the swizzle
+// every pass of that tile is free of bank conflicts for 16- and 32-lane bank
+// cycles alike, and the worst case of the other ring sizes halves.  A
+// bijection on [0, 1024).
+GF_HD uint32_t modup_swz(uint32_t at) { return at ^ (((at >> 4) & 7) << 2); }
+struct ModupLdsIn {
+    const uint64_t* a;
+    GF_HD uint64_t operator()(uint32_t, uint32_t, uint32_t at) const { return a[modup_swz(at)]; }
+};
+struct ModupLdsOut {
+    uint64_t* a;
+    GF_HD void operator()(uint32_t, uint32_t, uint32_t at, uint64_t v) const { a[modup_swz(at)] = v; }
+};
+struct ModupLio {
+    GF_HD ModupLdsIn in(const uint64_t* a) const { return ModupLdsIn{a}; }
+    GF_HD ModupLdsOut out(uint64_t* a) const { return ModupLdsOut{a}; }
+};
+struct ModupLdsInF64 {
+    const uint64_t* a;
+    GF_HD double operator()(uint32_t, uint32_t, uint32_t at) const { return f64_of_bits(a[modup_swz(at)]); }
+};
+struct ModupLdsOutF64 {
+    uint64_t* a;
+    GF_HD void operator()(uint32_t, uint32_t, uint32_t at, double v) const { a[modup_swz(at)] = bits_of_f64(v); }
+};
+struct ModupLioF64 {
+    GF_HD ModupLdsInF64 in(const uint64_t* a) const { return ModupLdsInF64{a}; }
+    GF_HD ModupLdsOutF64 out(uint64_t* a) const { return ModupLdsOutF64{a}; }
+};
+
 template <uint32_t S1>
 GF_HD uint32_t modup_slot(uint32_t unit, uint32_t u) {
     static_assert(chunk_size(S1, MODUP_RM, 0) == MODUP_RM, "chunk 0 must be a full radix-4 pass");
     constexpr uint32_t LOGC = MODUP_LOG_TILE - S1, C = 1u << LOGC, TL = 1u << (S1 - MODUP_RM);
-    return (unit & (C - 1)) + (((unit >> LOGC) + u * TL) << LOGC);
+    return modup_swz((unit & (C - 1)) + (((unit >> LOGC) + u * TL) << LOGC));
 }
 
 // source i's inverse column pass over the tile (all threads), its last pass
@@ -78,12 +147,31 @@ GF_HD void modup_source_pass(const ModupJob& j, uint32_t i, const NttGeom& g, ui
                              uint32_t tid, uint32_t nthr, Sync sync) {
     constexpr uint32_t LOGC = MODUP_LOG_TILE - S1, C = 1u << LOGC;
     const TileDesc d{LOGC, C, 1, 1, 0, true, false};
+    if (j.stwd[i]) {
+        const double q = j.sqd[i], qinv = j.sqinv[i];
+        const ColInF64 gin{gcast(j.src[i]), g.logN2, tile * C, q, qinv, F64_2P52 + 2 * q};
+        tile_chunks_f64<false, S1, MODUP_RM, 0>(lds, d, TwF64{gcast((const double*)j.stwd[i])}, q, qinv, tid, nthr,
+                                                sync, gin, ModupLdsOutF64{lds}, ModupLioF64{});
+        return;
+    }
     const ColIn gin{gcast(j.src[i]), g.logN2, tile * C};
     tile_chunks<false, false, S1, MODUP_RM, 0>(lds, d, TwPaired{gcast(j.stw[i])}, TwPaired{gcast(j.stw[i])}, j.sq[i],
-                                               tid, nthr, sync, gin, LdsOut{lds});
+                                               tid, nthr, sync, gin, ModupLdsOut{lds}, ModupLio{});
 }
 template <uint32_t SP, uint32_t S1>
 GF_HD void modup_take_unit(const ModupJob& j, uint32_t i, const uint64_t* lds, uint32_t unit, ModupUnit<SP>& x) {
+    if (j.stwd[i]) {
+        const double q = j.sqd[i], qinv = j.sqinv[i], w = j.spost[i], wq = j.spostq[i];
+#pragma unroll
+        for (uint32_t u = 0; u < MODUP_UNIT; ++u) {
+            // the last pass left |x| < 3q: times n^-1 QHatInv, into [0, q)
+            const double y = f64_mulmod(f64_of_bits(lds[modup_slot<S1>(unit, u)]), w, wq, q);
+            const uint64_t v = u52_from_f64(f64_reduce(y, q, qinv));
+            x.lo[u][i] = split30_lo(v);
+            x.hi[u][i] = split30_hi(v);
+        }
+        return;
+    }
     const uint64_t post = j.post[i], postp = j.postp[i], q = j.sq[i];
 #pragma unroll
     for (uint32_t u = 0; u < MODUP_UNIT; ++u) {
@@ -95,10 +183,13 @@ GF_HD void modup_take_unit(const ModupJob& j, uint32_t i, const uint64_t* lds, u
 }
 // target t's conversion of this thread's unit into LDS: the exact column sums
 // over the part's sources, REDC without its correction ([0, 2 q_t)), as the
-// conversion kernel leaves them for the forward transforms
+// conversion kernel leaves them for the forward transforms; for an fp64
+// target as the double v - q_t (|.| <= q_t)
 template <uint32_t SP, uint32_t S1>
 GF_HD void modup_convert_unit(const ModupJob& j, uint32_t t, uint64_t* lds, uint32_t unit, ModupUnit<SP>& x) {
     const uint64_t q = uniform_at(j.tq, t), qn = uniform_at(j.tqneg, t);
+    const bool f64 = uniform_at(j.ttwd, t) != 0;
+    const double c2 = f64 ? F64_2P52 + f64_of_bits(uniform_at(j.tqf, 2 * t)) : 0.0;
 #pragma unroll
     for (uint32_t u = 0; u < MODUP_UNIT; ++u) {
 #pragma unroll
@@ -109,7 +200,8 @@ GF_HD void modup_convert_unit(const ModupJob& j, uint32_t t, uint64_t* lds, uint
             x.hi[u][i] = opaque32(x.hi[u][i]);
         }
         const U128 acc = dot30<SP>(x.lo[u], x.hi[u], [&](uint32_t s) { return uniform_at(j.tab, s * j.sc + t); });
-        lds[modup_slot<S1>(unit, u)] = redc_lazy(acc, q, qn);
+        const uint64_t v = redc_lazy(acc, q, qn);
+        lds[modup_slot<S1>(unit, u)] = f64 ? bits_of_f64(f64_from_u52(v, c2)) : v;
     }
 }
 // target t's forward column pass from LDS, stored to HBM unreduced (the
@@ -119,14 +211,24 @@ GF_HD void modup_target_pass(const ModupJob& j, uint32_t t, const NttGeom& g, ui
                              uint32_t tid, uint32_t nthr, Sync sync) {
     constexpr uint32_t LOGC = MODUP_LOG_TILE - S1, C = 1u << LOGC;
     const TileDesc d{LOGC, C, 1, 1, 0, true, false};
+    const uint64_t twd = uniform_at(j.ttwd, t);
+    if (twd) {
+        const double q = f64_of_bits(uniform_at(j.tqf, 2 * t)), qinv = f64_of_bits(uniform_at(j.tqf, 2 * t + 1));
+        const ColOutF64 gout{gcast(j.dst + (size_t)t * g.n), g.logN2, tile * C, q, qinv};
+        tile_chunks_f64<true, S1, MODUP_RM, 0>(lds, d, TwF64{gcast((const double*)(uintptr_t)twd)}, q, qinv, tid, nthr,
+                                               sync, ModupLdsInF64{lds}, gout, ModupLioF64{});
+        return;
+    }
     NttJob jt{};
     jt.q = uniform_at(j.tq, t);
     const ColOut<true> gout{gcast(j.dst + (size_t)t * g.n), g.logN2, tile * C, jt};
     const TwPaired tw{gcast((const uint64_t*)(uintptr_t)uniform_at(j.ttw, t))};
     if (uniform_at(j.tred, t))
-        tile_chunks<true, true, S1, MODUP_RM, 0>(lds, d, tw, tw, jt.q, tid, nthr, sync, LdsIn{lds}, gout);
+        tile_chunks<true, true, S1, MODUP_RM, 0>(lds, d, tw, tw, jt.q, tid, nthr, sync, ModupLdsIn{lds}, gout,
+                                                 ModupLio{});
     else
-        tile_chunks<true, false, S1, MODUP_RM, 0>(lds, d, tw, tw, jt.q, tid, nthr, sync, LdsIn{lds}, gout);
+        tile_chunks<true, false, S1, MODUP_RM, 0>(lds, d, tw, tw, jt.q, tid, nthr, sync, ModupLdsIn{lds}, gout,
+                                                  ModupLio{});
 }
 
 }  // namespace gf

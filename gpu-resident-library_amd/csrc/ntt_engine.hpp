@@ -230,6 +230,11 @@ struct LdsOut {
     uint64_t* a;
     GF_HD void operator()(uint32_t, uint32_t, uint32_t at, uint64_t v) const { a[at] = v; }
 };
+// the LDS accessors of the passes between a phase's first and last
+struct LdsPlain {
+    GF_HD LdsIn in(const uint64_t* a) const { return LdsIn{a}; }
+    GF_HD LdsOut out(uint64_t* a) const { return LdsOut{a}; }
+};
 
 // One radix-2^R pass (stages ST0 .. ST0+R-1 of an S-stage transform) over all
 // units of the tile owned by this thread; fully unrolled, the R-stage unit
@@ -321,10 +326,12 @@ struct PassOf {
 // All passes of one S-stage phase, chunk order fixed at compile time.  The
 // first pass reads through `gin` and the last writes through `gout` (global
 // memory, or LDS when the caller stages that end); the passes in between go
-// through LDS with a barrier after each.  A one-pass phase never touches LDS.
-template <bool FWD, bool LZ, uint32_t S, uint32_t RM, uint32_t I, class TW, class TWL, class Sync, class GIN, class GOUT>
+// through LDS (accessors from lio) with a barrier after each.  A one-pass
+// phase never touches LDS.
+template <bool FWD, bool LZ, uint32_t S, uint32_t RM, uint32_t I, class TW, class TWL, class Sync, class GIN, class GOUT,
+          class LIO = LdsPlain>
 GF_HD void tile_chunks(uint64_t* a, const TileDesc& d, const TW& tw, const TWL& twl, uint64_t q, uint32_t tid,
-                       uint32_t nthr, Sync sync, const GIN& gin, const GOUT& gout) {
+                       uint32_t nthr, Sync sync, const GIN& gin, const GOUT& gout, LIO lio = LIO{}) {
     using P = PassOf<FWD, S, RM, I>;
     if constexpr (I < P::NC) {
         constexpr bool first = I == 0, last = I + 1 == P::NC;
@@ -336,15 +343,15 @@ GF_HD void tile_chunks(uint64_t* a, const TileDesc& d, const TW& tw, const TWL& 
         if constexpr (first && last) {
             run(gin, gout);
         } else if constexpr (first) {
-            run(gin, LdsOut{a});
+            run(gin, lio.out(a));
         } else if constexpr (last) {
-            run(LdsIn{a}, gout);
+            run(lio.in(a), gout);
         } else {
-            run(LdsIn{a}, LdsOut{a});
+            run(lio.in(a), lio.out(a));
         }
         if constexpr (!last) {
             sync();
-            tile_chunks<FWD, LZ, S, RM, I + 1>(a, d, tw, twl, q, tid, nthr, sync, gin, gout);
+            tile_chunks<FWD, LZ, S, RM, I + 1>(a, d, tw, twl, q, tid, nthr, sync, gin, gout, lio);
         }
     }
 }

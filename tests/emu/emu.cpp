@@ -4,6 +4,7 @@
 // on the CPU against the oracle before any GPU run.
 #include <cstddef>
 #include <cstdint>
+#include <cstring>
 #include <vector>
 
 #include "ksi.hpp"
@@ -239,10 +240,12 @@ void emu_modup_t(const ModupJob& j, const NttGeom& g) {
 // ref != 0 runs the unfused kernels' order instead: each source's whole
 // inverse column pass (standard tiles, radix rmax), the conversion with the
 // conversion kernel's arithmetic, each target's forward column pass.
+// f64 != 0 (fused only): sources and targets below 2^50 take the fp64 passes.
 extern "C" void emu_modup(uint32_t n, uint32_t sp, uint32_t sc, const uint64_t* const* src, const uint64_t* sq,
                           const uint64_t* const* sw, const uint64_t* const* swp, const uint64_t* post,
                           const uint64_t* postp, const uint64_t* tq, const uint64_t* const* ttw,
-                          const uint64_t* const* ttwp, const uint64_t* tab, uint64_t* dst, int ref, uint32_t rmax) {
+                          const uint64_t* const* ttwp, const uint64_t* tab, uint64_t* dst, int ref, uint32_t rmax,
+                          int f64) {
     const NttGeom g = make_geom(n, rmax);
     std::vector<std::vector<uint64_t>> stab(sp), ttab(sc);
     for (uint32_t i = 0; i < sp; ++i) stab[i] = paired(sw[i], swp[i], n);
@@ -254,6 +257,19 @@ extern "C" void emu_modup(uint32_t n, uint32_t sp, uint32_t sc, const uint64_t* 
         ttw_addr[t] = (uint64_t)(uintptr_t)ttab[t].data();
     }
     if (!ref) {
+        // fp64 pairs of the same twiddles (the engine's root tables), as words
+        auto f64_pairs = [&](const uint64_t* w, uint64_t q) {
+            std::vector<uint64_t> out(2 * (size_t)n);
+            for (uint32_t k = 0; k < n; ++k) {
+                double wd, wq;
+                f64_twiddle(w[k], q, wd, wq);
+                std::memcpy(&out[2 * k], &wd, 8);
+                std::memcpy(&out[2 * k + 1], &wq, 8);
+            }
+            return out;
+        };
+        std::vector<std::vector<uint64_t>> sd(sp), td(sc);
+        std::vector<uint64_t> ttwd(sc, 0), tqf(2 * (size_t)sc);
         ModupJob J{};
         for (uint32_t i = 0; i < sp; ++i) {
             J.src[i] = src[i];
@@ -261,6 +277,22 @@ extern "C" void emu_modup(uint32_t n, uint32_t sp, uint32_t sc, const uint64_t* 
             J.sq[i] = sq[i];
             J.post[i] = post[i];
             J.postp[i] = postp[i];
+            if (f64 && f64_modulus(sq[i])) {
+                sd[i] = f64_pairs(sw[i], sq[i]);
+                J.stwd[i] = sd[i].data();
+                J.sqd[i] = (double)sq[i];
+                J.sqinv[i] = 1.0 / J.sqd[i];
+                f64_twiddle(post[i], sq[i], J.spost[i], J.spostq[i]);
+            }
+        }
+        for (uint32_t t = 0; t < sc; ++t) {
+            const double qd = (double)tq[t], qi = 1.0 / qd;
+            std::memcpy(&tqf[2 * t], &qd, 8);
+            std::memcpy(&tqf[2 * t + 1], &qi, 8);
+            if (f64 && f64_modulus(tq[t])) {
+                td[t] = f64_pairs(ttw[t], tq[t]);
+                ttwd[t] = (uint64_t)(uintptr_t)td[t].data();
+            }
         }
         J.dst = dst;
         J.tab = tab;
@@ -268,6 +300,8 @@ extern "C" void emu_modup(uint32_t n, uint32_t sp, uint32_t sc, const uint64_t* 
         J.tqneg = tqn.data();
         J.ttw = ttw_addr.data();
         J.tred = tred.data();
+        J.ttwd = ttwd.data();
+        J.tqf = tqf.data();
         J.sp = sp;
         J.sc = sc;
         with_split(g.logN1, g.logN2, [&](auto A, auto) {

@@ -52,7 +52,7 @@ def emu():
     pp = C.POINTER(u64p)
     lib.emu_modup.restype = None
     lib.emu_modup.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, pp, u64p, pp, pp, u64p, u64p, u64p, pp, pp, u64p,
-                              u64p, C.c_int, C.c_uint32]
+                              u64p, C.c_int, C.c_uint32, C.c_int]
     yield lib
     shutil.rmtree(d, ignore_errors=True)
 
@@ -283,7 +283,10 @@ def test_fused_modup_equals_unfused(emu, logn, L, dnum, part):
     each source's inverse column pass (standard tiles, radix 2^3 and 2^4), the
     conversion kernel's sums, each digit's forward column pass, over every
     ring split (2^13 .. 2^17), 1 to 6 sources, 50-bit (unreduced) and 60-bit
-    (reduced butterflies) targets, inputs up to 4q - 1."""
+    (reduced butterflies) targets, inputs up to 4q - 1.  With the fp64 passes
+    for moduli below 2^50 the 60-bit targets are still stored bit for bit
+    (their sources' reduced values and the conversion sums are identical),
+    and the fp64 targets hold the same residues in [0, 2q)."""
     n = 1 << logn
     P = make_params(n, L, dnum)
     alpha = -(-L // dnum)
@@ -327,11 +330,22 @@ def test_fused_modup_equals_unfused(emu, logn, L, dnum, part):
     arr = lambda v: (u64p * len(v))(*[O.P(np.ascontiguousarray(a, dtype=np.uint64)) for a in v])  # noqa: E731
     keep = [src, sw, swp, tw, twp]
     outs = {}
-    for ref, rmax in ((0, 3), (1, 3), (1, 4)):
+    for ref, rmax, f64 in ((0, 3, 0), (1, 3, 0), (1, 4, 0), (0, 3, 1)):
         dst = np.zeros(sc * n, np.uint64)
         emu.emu_modup(n, sp, sc, arr(src), O.P(O.U(srcm)), arr(sw), arr(swp), O.P(O.U(post)), O.P(O.U(postp)),
-                      O.P(O.U(tgt)), arr(tw), arr(twp), O.P(tab), O.P(dst), ref, rmax)
-        outs[(ref, rmax)] = dst
+                      O.P(O.U(tgt)), arr(tw), arr(twp), O.P(tab), O.P(dst), ref, rmax, f64)
+        outs[(ref, rmax, f64)] = dst
     assert keep
-    assert np.array_equal(outs[(0, 3)], outs[(1, 3)])
-    assert np.array_equal(outs[(0, 3)], outs[(1, 4)])
+    assert np.array_equal(outs[(0, 3, 0)], outs[(1, 3, 0)])
+    assert np.array_equal(outs[(0, 3, 0)], outs[(1, 4, 0)])
+    f = outs[(0, 3, 1)].reshape(sc, n)
+    r = outs[(1, 3, 0)].reshape(sc, n)
+    nf = 0
+    for j, t in enumerate(tgt):
+        if t < 1 << 50:
+            nf += 1
+            assert int(f[j].max()) < 2 * t, j
+            assert np.array_equal(f[j] % np.uint64(t), r[j] % np.uint64(t)), j
+        else:
+            assert np.array_equal(f[j], r[j]), j
+    assert nf == sum(t < 1 << 50 for t in tgt)

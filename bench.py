@@ -691,7 +691,7 @@ def main():
     else:
         radix, ntt_ms = ctx.autotune(stream, args.batch)   # NTT register radix, then the plan knobs at B
     knobs = {k_: ctx.knob(k_) for k_ in ("ntt_radix", "ksi_extra", "conv_chunk", "rs_chunk", "modup_fused",
-                                          "modup_f64")}
+                                          "modup_f64", "ksi_f64")}
     log("context ready: K=%d alpha=%d ws=%.1f MiB/op, knobs %s (%s)"
         % (K, alpha, ctx.ws_bytes / 2**20, knobs, "pinned" if pinned else "autotuned"))
     tuned = {"shape": {"n": N, "L": L, "dnum": DNUM}, "chosen": knobs, "pinned": pinned,

@@ -86,15 +86,20 @@ RowTables row_tables(uint32_t n, uint64_t q, uint64_t root, uint32_t rmax) {
     if (it != cache.end()) return it->second;
     const RootTable t = root_table(n, q, root);
     const size_t words = row_table_words(g);
-    std::vector<uint64_t> pair(2 * (size_t)n), out(2 * words);
+    // forward, inverse, and for moduli below 2^50 their fp64 pairs (entries
+    // of two words either way, so the same permutation)
+    const uint64_t* src[4] = {t.fw2, t.iv2, t.fw2d, t.iv2d};
+    const int nt = t.fw2d ? 4 : 2;
+    std::vector<uint64_t> pair(2 * (size_t)n), out(nt * words);
     void* d = nullptr;
     hip_check(hipMalloc(&d, out.size() * 8), "hipMalloc row table");
-    for (int dir = 0; dir < 2; ++dir) {
-        hip_check(hipMemcpy(pair.data(), dir ? t.iv2 : t.fw2, pair.size() * 8, hipMemcpyDeviceToHost), "read table");
-        build_row_table(pair.data(), g, out.data() + dir * words);
+    for (int k = 0; k < nt; ++k) {
+        hip_check(hipMemcpy(pair.data(), src[k], pair.size() * 8, hipMemcpyDeviceToHost), "read table");
+        build_row_table(pair.data(), g, out.data() + k * words);
     }
     hip_check(hipMemcpy(d, out.data(), out.size() * 8, hipMemcpyHostToDevice), "upload row table");
-    RowTables r{(const uint64_t*)d, (const uint64_t*)d + words};
+    const uint64_t* b = (const uint64_t*)d;
+    RowTables r{b, b + words, nt == 4 ? b + 2 * words : nullptr, nt == 4 ? b + 3 * words : nullptr};
     cache.emplace(key, r);
     return r;
 }
@@ -287,9 +292,19 @@ void Ctx::set_radix(uint32_t r) {
 }
 
 const uint64_t* Ctx::row_table_of(const uint64_t* paired) const {
+    if (!paired) return nullptr;
     for (uint32_t i = 0; i < L + K; ++i) {
         if (paired == rt[i].fw2) return rtr[i].fwd;
         if (paired == rt[i].iv2) return rtr[i].inv;
+        if (paired == rt[i].fw2d) return rtr[i].fwd_d;
+        if (paired == rt[i].iv2d) return rtr[i].inv_d;
+    }
+    return nullptr;
+}
+const uint64_t* Ctx::f64_table_of(const uint64_t* paired) const {
+    for (uint32_t i = 0; i < L + K; ++i) {
+        if (paired == rt[i].fw2) return rt[i].fw2d;
+        if (paired == rt[i].iv2) return rt[i].iv2d;
     }
     return nullptr;
 }
@@ -370,9 +385,9 @@ void Ctx::set_knob(const std::string& name, uint32_t v) {
     } else if (name == "modup_fused") {
         if (v > 1) throw std::runtime_error("gpufhe: modup_fused must be 0 or 1");
         modup_fused = v;
-    } else if (name == "modup_f64") {
-        if (v > 1) throw std::runtime_error("gpufhe: modup_f64 must be 0 or 1");
-        modup_f64 = v;
+    } else if (name == "modup_f64" || name == "ksi_f64") {
+        if (v > 1) throw std::runtime_error("gpufhe: " + name + " must be 0 or 1");
+        (name == "modup_f64" ? modup_f64 : ksi_f64) = v;
     } else if (name == "modup_min_wgs") {
         modup_min_wgs = v;
     } else if (name == "rechunk_wgs") {
@@ -388,6 +403,7 @@ uint32_t Ctx::knob(const std::string& name) const {
     if (name == "rs_chunk") return rs_chunk;
     if (name == "modup_fused") return modup_fused;
     if (name == "modup_f64") return modup_f64;
+    if (name == "ksi_f64") return ksi_f64;
     if (name == "modup_min_wgs") return modup_min_wgs;
     if (name == "rechunk_wgs") return rechunk_wgs;
     throw std::runtime_error("gpufhe: unknown tuning knob " + name);
@@ -480,9 +496,12 @@ void Ctx::autotune_plan(hipStream_t s, uint32_t B) {
     tune("conv_chunk", conv_chunk, {8, 16, 32}, {Step::CONV});
     modup_fused = mf;
     if (fused && !geom.single && n % KSI_TILE == 0 && n % MODUP_TILE == 0 && !wide && alpha <= MODUP_MAXSRC) {
-        // fp64 column passes inside the fused ModUp, timed with it fused
+        // fp64 passes (the fused ModUp's column passes and the inverse row
+        // pass before it; then the fused inner product's row passes), timed
+        // with the ModUp fused
         modup_fused = 1;
-        tune("modup_f64", modup_f64, {0, 1}, {Step::MODUP});
+        tune("modup_f64", modup_f64, {0, 1}, {Step::NTT_IB, Step::MODUP});
+        if (modup_f64) tune("ksi_f64", ksi_f64, {0, 1}, {Step::KSI});
         modup_fused = mf;
         tune("modup_fused", modup_fused, {0, 1}, {Step::NTT_IB, Step::NTT_I, Step::MODUP, Step::CONV, Step::NTT_FA});
     }
@@ -1466,6 +1485,24 @@ std::unique_ptr<Plan> PlanBuilder::layout(std::vector<char>& blob) {
     for (KsiJob& j : ksi_) {
         j.twr = C_.row_table_of(j.tw);
         if (!j.twr) throw std::runtime_error("gpufhe: transform table of another context");
+    }
+    // With the fp64 fused ModUp in this plan, the inverse row pass in front
+    // of it runs in fp64 too for moduli below 2^50 (its inputs, the tensor's
+    // t2 or a ciphertext tower, are reduced residues), and with ksi_f64 so do
+    // the row passes of the fused inner product (the ModUp leaves those
+    // digits in [0, 2q)).
+    if (!modup_.empty() && C_.modup_f64 && !C_.geom.single) {
+        auto f64_job = [&](auto& j) {
+            const uint64_t* td = C_.f64_table_of(j.tw);
+            if (!td) return;
+            j.twd = td;
+            j.twrd = C_.row_table_of(td);
+            j.fq = (double)j.q;
+            j.fqinv = 1.0 / j.fq;
+        };
+        if (C_.ksi_f64)
+            for (KsiJob& j : ksi_) f64_job(j);
+        for (NttJob& j : intt_upb_) f64_job(j);
     }
     // KSI jobs by transformed-part count (one step each: that many LDS
     // regions and waves), by key within a step

@@ -55,6 +55,8 @@ RootTable root_table(uint32_t n, uint64_t q, uint64_t root);
 struct RowTables {
     const uint64_t* fwd = nullptr;
     const uint64_t* inv = nullptr;
+    const uint64_t* fwd_d = nullptr;   // of the fp64 pairs (moduli below 2^50), else null
+    const uint64_t* inv_d = nullptr;
 };
 RowTables row_tables(uint32_t n, uint64_t q, uint64_t root, uint32_t rmax);
 
@@ -105,8 +107,12 @@ public:
     std::vector<RowTables> rtr;               // per row: phase-B row tables for geom.rmax
     // register radix of transforms in plans built from now on (2..4)
     void set_radix(uint32_t r);
-    // the row table matching a paired table of this context (fw2 or iv2), else null
+    // the row table matching a paired table of this context (fw2, iv2, or
+    // their fp64 pairs fw2d, iv2d), else null
     const uint64_t* row_table_of(const uint64_t* paired) const;
+    // the fp64 pairs of the same twiddles as a paired table (fw2 -> fw2d,
+    // iv2 -> iv2d), null for moduli of 2^50 and above
+    const uint64_t* f64_table_of(const uint64_t* paired) const;
     uint32_t row_q(uint32_t i) const { return i; }
     uint32_t row_p(uint32_t k) const { return L + k; }
     uint32_t num_parts(uint32_t l) const { return (l + alpha - 1) / alpha; }
@@ -123,9 +129,13 @@ public:
     // 1: ModUp's inverse column pass, conversion and digit column pass run as
     // one kernel (modup.hpp) where the shape allows it
     uint32_t modup_fused = 1;
-    // 1: the fused ModUp runs the column passes of moduli below 2^50 in
-    // double precision (ntt_f64.hpp)
+    // 1: plans with the fused ModUp run the transforms of moduli below 2^50
+    // around it in double precision (ntt_f64.hpp): the inverse row pass
+    // before it and its column passes
     uint32_t modup_f64 = 1;
+    // 1 (with modup_f64): the row passes of the fused inner product after it
+    // as well (fewer instructions, but measured slower on gfx950)
+    uint32_t ksi_f64 = 0;
     // a plan whose fused ModUp stage would launch fewer workgroups than this
     // runs it unfused (a single operation's stage cannot fill the chip)
     uint32_t modup_min_wgs = 2048;
@@ -139,7 +149,8 @@ public:
     void autotune_plan(hipStream_t s, uint32_t batch);
     std::string tune_report;
     // sets or reads a knob by name ("ntt_radix", "ksi_extra", "conv_chunk",
-    // "rs_chunk", "modup_fused", "modup_f64", "modup_min_wgs", "rechunk_wgs")
+    // "rs_chunk", "modup_fused", "modup_f64", "ksi_f64", "modup_min_wgs",
+    // "rechunk_wgs")
     void set_knob(const std::string& name, uint32_t v);
     uint32_t knob(const std::string& name) const;
     std::string radix_report;   // the radix search of autotune(), as JSON

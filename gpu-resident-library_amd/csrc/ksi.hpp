@@ -17,6 +17,7 @@
 // are read from HBM at the inner product.
 #pragma once
 #include "ntt_engine.hpp"
+#include "ntt_f64.hpp"
 
 namespace gf {
 
@@ -49,6 +50,12 @@ struct KsiJob {
     // 1: every digit and key value of the row is below 2^60 (ksi_d30), so the
     // inner product runs as 30-bit column sums
     uint32_t d30;
+    // digits from the fp64 ModUp (below 2q) of a modulus below 2^50: the row
+    // passes in fp64 (ntt_f64.hpp) with these pairs and their row table, q and
+    // fl(1/q); null: the integer passes
+    const uint64_t* twd;
+    const uint64_t* twrd;
+    double fq, fqinv;
 };
 
 // V consecutive words from word i of a tower (V = 2: one 16-byte access; the
@@ -198,9 +205,14 @@ GF_HD void ksi_body(const KsiJob& j, uint32_t N1, uint32_t tile, uint64_t* lds, 
     uint32_t k = 0;
     for (uint32_t p = 0; p < j.np; ++p) {
         if (!((j.xf >> p) & 1)) continue;
-        if (k % nwaves == wave)
-            row_tile<true, LZ, true, S2, RM, ROWS, 2, true, WSync, true>(
-                t, d, gcast(j.src[p]), nullptr, off, lds + k * KSI_PART_WORDS, lane, wlanes, wsync);
+        if (k % nwaves == wave) {
+            if (j.twd)
+                row_tile_f64<true, S2, RM, ROWS, true>(j.twd, j.twrd, j.fq, j.fqinv, d, gcast(j.src[p]), nullptr, off,
+                                                       lds + k * KSI_PART_WORDS, lane, wlanes, wsync);
+            else
+                row_tile<true, LZ, true, S2, RM, ROWS, 2, true, WSync, true>(
+                    t, d, gcast(j.src[p]), nullptr, off, lds + k * KSI_PART_WORDS, lane, wlanes, wsync);
+        }
         ++k;
     }
     bsync();

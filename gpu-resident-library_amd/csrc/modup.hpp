@@ -150,8 +150,9 @@ GF_HD void modup_source_pass(const ModupJob& j, uint32_t i, const NttGeom& g, ui
     if (j.stwd[i]) {
         const double q = j.sqd[i], qinv = j.sqinv[i];
         const ColInF64 gin{gcast(j.src[i]), g.logN2, tile * C, q, qinv, F64_2P52 + 2 * q};
-        tile_chunks_f64<false, S1, MODUP_RM, 0>(lds, d, TwF64{gcast((const double*)j.stwd[i])}, q, qinv, tid, nthr,
-                                                sync, gin, ModupLdsOutF64{lds}, ModupLioF64{});
+        const TwF64 tw{gcast((const double*)j.stwd[i])};
+        tile_chunks_f64<false, S1, MODUP_RM, 0>(lds, d, tw, tw, q, qinv, tid, nthr, sync, gin, ModupLdsOutF64{lds},
+                                                ModupLioF64{});
         return;
     }
     const ColIn gin{gcast(j.src[i]), g.logN2, tile * C};
@@ -215,8 +216,9 @@ GF_HD void modup_target_pass(const ModupJob& j, uint32_t t, const NttGeom& g, ui
     if (twd) {
         const double q = f64_of_bits(uniform_at(j.tqf, 2 * t)), qinv = f64_of_bits(uniform_at(j.tqf, 2 * t + 1));
         const ColOutF64 gout{gcast(j.dst + (size_t)t * g.n), g.logN2, tile * C, q, qinv};
-        tile_chunks_f64<true, S1, MODUP_RM, 0>(lds, d, TwF64{gcast((const double*)(uintptr_t)twd)}, q, qinv, tid, nthr,
-                                               sync, ModupLdsInF64{lds}, gout, ModupLioF64{});
+        const TwF64 tw{gcast((const double*)(uintptr_t)twd)};
+        tile_chunks_f64<true, S1, MODUP_RM, 0>(lds, d, tw, tw, q, qinv, tid, nthr, sync, ModupLdsInF64{lds}, gout,
+                                               ModupLioF64{});
         return;
     }
     NttJob jt{};

@@ -83,6 +83,12 @@ struct NttJob {
     // with red: store those unreduced values, for a consumer that reduces them
     // itself (the key inner product); ignored with an epilogue
     uint32_t lazy_out;
+    // inverse phase B of a modulus below 2^50 with inputs below 4q: its fp64
+    // pairs and their row table (ntt_f64.hpp; null: the integer passes), q
+    // and fl(1/q).  Its output is then in [0, 2q).
+    const uint64_t* twd;
+    const uint64_t* twrd;
+    double fq, fqinv;
 };
 
 // floor(2^64 / q) for the moduli whose forward transforms run without

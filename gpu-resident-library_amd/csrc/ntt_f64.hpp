@@ -78,10 +78,23 @@ GF_HD double f64_reduce(double x, double q, double qinv) {
 
 // (w, fl(w/q)) pairs, w centred, in the index layout of the paired integer tables
 struct TwF64 {
+    static constexpr bool kRows = false;
     const GF_GLOBAL double* t;
     GF_HD void get(uint32_t i, double& w, double& wq) const {
         w = t[2 * i];
         wq = t[2 * i + 1];
+    }
+};
+// the row table of one phase-B tile (TwRows' lane-order layout) in fp64 pairs
+template <uint32_t R, uint32_t LOGU>
+struct TwRowsF64 {
+    static constexpr bool kRows = true;
+    static constexpr uint32_t PR = ((1u << R) - 1) << LOGU;
+    const GF_GLOBAL double* t;
+    GF_HD void get_rp(uint32_t c, uint32_t s, uint32_t g, uint32_t v, double& w, double& wq) const {
+        const uint32_t e = s * PR + (((1u << c) - 1) << LOGU) + (v << LOGU) + g;
+        w = t[2 * e];
+        wq = t[2 * e + 1];
     }
 };
 inline void f64_twiddle(uint64_t w, uint64_t q, double& wd, double& wq) {
@@ -105,9 +118,11 @@ struct LdsPlainF64 {
 };
 
 // One radix-2^R pass over the tile's units, as tile_pass (same unit and
-// twiddle indexing) on doubles.  CENTER: outputs centred before `out`.
-template <bool FWD, uint32_t S, uint32_t ST0, uint32_t R, bool CENTER, class IN, class OUT>
-GF_HD void tile_pass_f64(const TileDesc& d, const TwF64& tw, double q, double qinv, uint32_t tid, uint32_t nthr,
+// twiddle indexing) on doubles.  CENTER: outputs centred before `out`.  An
+// inverse pass of more than two stages centres its unit after every second
+// stage as well.
+template <bool FWD, uint32_t S, uint32_t ST0, uint32_t R, bool CENTER, class TW, class IN, class OUT>
+GF_HD void tile_pass_f64(const TileDesc& d, const TW& tw, double q, double qinv, uint32_t tid, uint32_t nthr,
                          const IN& in, const OUT& out) {
     constexpr uint32_t LOGTL = S - ST0 - R;
     constexpr uint32_t TL = 1u << LOGTL;
@@ -145,7 +160,8 @@ GF_HD void tile_pass_f64(const TileDesc& d, const TwF64& tw, double q, double qi
                 for (uint32_t u = 0; u < R8; ++u) {
                     if (u & h) continue;
                     double w, wq;
-                    tw.get(tb + (u >> (R - c)), w, wq);
+                    if constexpr (TW::kRows) tw.get_rp(c, s, g, u >> (R - c), w, wq);
+                    else tw.get(tb + (u >> (R - c)), w, wq);
                     const double m = f64_mulmod(x[u + h], w, wq, q);
                     const double a = x[u];
                     x[u] = a + m;
@@ -161,10 +177,15 @@ GF_HD void tile_pass_f64(const TileDesc& d, const TwF64& tw, double q, double qi
                 for (uint32_t u = 0; u < R8; ++u) {
                     if (u & h) continue;
                     double w, wq;
-                    tw.get(tb + (u >> (R - c)), w, wq);
+                    if constexpr (TW::kRows) tw.get_rp((uint32_t)c, s, g, u >> (R - c), w, wq);
+                    else tw.get(tb + (u >> (R - c)), w, wq);
                     const double a = x[u], b = x[u + h];
                     x[u] = a + b;
                     x[u + h] = f64_mulmod(a - b, w, wq, q);
+                }
+                if (c > 0 && (R - c) % 2 == 0) {
+#pragma unroll
+                    for (uint32_t u = 0; u < R8; ++u) x[u] = f64_center(x[u], q, qinv);
                 }
             }
         }
@@ -178,10 +199,10 @@ GF_HD void tile_pass_f64(const TileDesc& d, const TwF64& tw, double q, double qi
 
 // Whether fp64 pass I (execution order) of an S-stage phase in radix-2^RM
 // chunks centres its outputs.  The last pass never does (its consumer
-// reduces).  Inverse, inputs centred: every pass (sums double per stage, so
-// a pass of at most 2 stages ends below 3 q).  Forward, inputs |x| <= q: a
-// pass of R stages adds 1.5 R q, and a pass centres when the next one would
-// take the bound past 7.5 q.
+// reduces).  Inverse, inputs centred: every pass (sums double per stage, and
+// with the centring after every second stage inside a pass its values stay
+// below 3 q).  Forward, inputs |x| <= q: a pass of R stages adds 1.5 R q, and
+// a pass centres when the next one would take the bound past 7.5 q.
 template <bool FWD, uint32_t S, uint32_t RM>
 GF_HD constexpr bool f64_center_after(uint32_t I) {
     const uint32_t NC = num_chunks(S, RM);
@@ -209,18 +230,20 @@ GF_HD constexpr double f64_fwd_out_bound() {
 }
 
 // All passes of one phase in fp64, as tile_chunks: the first reads through
-// gin, the last writes through gout, LDS (bit patterns) in between.
-template <bool FWD, uint32_t S, uint32_t RM, uint32_t I, class Sync, class GIN, class GOUT, class LIO = LdsPlainF64>
-GF_HD void tile_chunks_f64(uint64_t* a, const TileDesc& d, const TwF64& tw, double q, double qinv, uint32_t tid,
-                           uint32_t nthr, Sync sync, const GIN& gin, const GOUT& gout, LIO lio = LIO{}) {
+// gin, the last writes through gout, LDS (bit patterns) in between; the last
+// chunk (stages up to S - 1) reads its twiddles through twl.
+template <bool FWD, uint32_t S, uint32_t RM, uint32_t I, class TW, class TWL, class Sync, class GIN, class GOUT,
+          class LIO = LdsPlainF64>
+GF_HD void tile_chunks_f64(uint64_t* a, const TileDesc& d, const TW& tw, const TWL& twl, double q, double qinv,
+                           uint32_t tid, uint32_t nthr, Sync sync, const GIN& gin, const GOUT& gout, LIO lio = LIO{}) {
     using P = PassOf<FWD, S, RM, I>;
     if constexpr (I < P::NC) {
         static_assert(!FWD || f64_fwd_out_bound<S, RM>() <= 7.5, "forward bound");
-        static_assert(FWD || P::R <= 2, "inverse passes of at most two stages keep their sums below 3 q");
         constexpr bool first = I == 0, last = I + 1 == P::NC;
         constexpr bool ctr = f64_center_after<FWD, S, RM>(I);
         auto run = [&](const auto& in, const auto& out) {
-            tile_pass_f64<FWD, S, P::ST0, P::R, ctr>(d, tw, q, qinv, tid, nthr, in, out);
+            if constexpr (P::C + 1 == P::NC) tile_pass_f64<FWD, S, P::ST0, P::R, ctr>(d, twl, q, qinv, tid, nthr, in, out);
+            else tile_pass_f64<FWD, S, P::ST0, P::R, ctr>(d, tw, q, qinv, tid, nthr, in, out);
         };
         if constexpr (first && last) {
             run(gin, gout);
@@ -233,9 +256,96 @@ GF_HD void tile_chunks_f64(uint64_t* a, const TileDesc& d, const TwF64& tw, doub
         }
         if constexpr (!last) {
             sync();
-            tile_chunks_f64<FWD, S, RM, I + 1>(a, d, tw, q, qinv, tid, nthr, sync, gin, gout, lio);
+            tile_chunks_f64<FWD, S, RM, I + 1>(a, d, tw, twl, q, qinv, tid, nthr, sync, gin, gout, lio);
         }
     }
+}
+
+// Row access in fp64.  In: a tower's words (below 4q; CTR: centred after
+// the conversion, else below 2q and shifted by q), from HBM or staged raw in
+// LDS.  Out: the centred value plus q as a word in [0, 2q), to HBM or LDS.
+template <bool CTR>
+struct RowInF64 {
+    const GF_GLOBAL uint64_t* src;   // tile start
+    uint32_t N;
+    double c2, q, qinv;
+    GF_HD double operator()(uint32_t s, uint32_t k, uint32_t) const {
+        const double x = f64_from_u52(gat(src, s * N + k), c2);
+        return CTR ? f64_center(x, q, qinv) : x;
+    }
+};
+template <bool CTR>
+struct LdsConvIn {
+    const uint64_t* a;
+    double c2, q, qinv;
+    GF_HD double operator()(uint32_t, uint32_t, uint32_t at) const {
+        const double x = f64_from_u52(a[at], c2);
+        return CTR ? f64_center(x, q, qinv) : x;
+    }
+};
+struct RowOutU52 {
+    GF_GLOBAL uint64_t* dst;   // tower start
+    uint32_t off, N;
+    double q, qinv;
+    GF_HD void operator()(uint32_t s, uint32_t k, uint32_t, double v) const {
+        gat(dst, off + s * N + k) = u52_from_f64(f64_center(v, q, qinv) + q);
+    }
+};
+struct LdsOutU52 {
+    uint64_t* a;
+    double q, qinv;
+    GF_HD void operator()(uint32_t, uint32_t, uint32_t at, double v) const {
+        a[at] = u52_from_f64(f64_center(v, q, qinv) + q);
+    }
+};
+
+// A row tile in fp64, with row_tile's layout, staging and twiddle tables
+// (twd: fp64 pairs, twrd: their row table), for the two uses that take it:
+//   forward, KEEP: the keyswitch digits' row pass, inputs below 2q, the tile
+//     left in LDS for the inner product (element idx at lds[padded(idx)], in
+//     [0, 2q));
+//   inverse: a phase B, inputs below 4q, its output in [0, 2q) to dst for
+//     phase A.
+template <bool FWD, uint32_t S, uint32_t RM, uint32_t ROWS, bool KEEP, class Sync>
+GF_HD void row_tile_f64(const uint64_t* twd, const uint64_t* twrd, double q, double qinv, const TileDesc& d,
+                        const GF_GLOBAL uint64_t* src, GF_GLOBAL uint64_t* dst, uint32_t off, uint64_t* lds,
+                        uint32_t tid, uint32_t nthr, Sync sync) {
+    constexpr uint32_t N = 1u << S, NC = num_chunks(S, RM), WORDS = ROWS * N;
+    using P0 = PassOf<FWD, S, RM, 0>;
+    using PL = PassOf<FWD, S, RM, NC - 1>;
+    constexpr bool in_direct = P0::wide_runs, out_direct = PL::wide_runs;
+    static_assert(FWD == KEEP, "fp64 row tiles: forward kept in LDS, or inverse to HBM");
+    static_assert(KEEP ? !out_direct : out_direct, "fp64 row tile shape");
+    const double c2 = F64_2P52 + (FWD ? q : 2 * q);
+    if constexpr (!in_direct) {
+        wave_units<P0::R, WORDS>(tid, nthr, [&](uint32_t idx) { lds[padded(idx)] = gat(src, off + idx); });
+        sync.wave();
+    }
+    constexpr uint32_t RL = row_pass_r(S, RM), LOGU = S - RL;
+    using TR = TwRowsF64<RL, LOGU>;
+    const TR twl{gcast((const double*)twrd) + (size_t)2 * (off >> S) * TR::PR};
+    const TwF64 tw{gcast((const double*)twd)};
+    auto outs = [&](const auto& gin) {
+        if constexpr (KEEP) tile_chunks_f64<FWD, S, RM, 0>(lds, d, tw, twl, q, qinv, tid, nthr, sync, gin,
+                                                           LdsOutU52{lds, q, qinv});
+        else tile_chunks_f64<FWD, S, RM, 0>(lds, d, tw, twl, q, qinv, tid, nthr, sync, gin,
+                                            RowOutU52{dst, off, N, q, qinv});
+    };
+    if constexpr (in_direct) outs(RowInF64<!FWD>{src + off, N, c2, q, qinv});
+    else outs(LdsConvIn<!FWD>{lds, c2, q, qinv});
+}
+
+// Inverse phase B in fp64 (tile = rows [a0, a0 + R), as phaseB_body) for a
+// job with fp64 tables: output in [0, 2q) for phase A.
+template <uint32_t S2, uint32_t RM, class Sync>
+GF_HD void phaseB_inv_f64(const NttJob& j, const NttGeom& g, uint32_t tile, uint64_t* lds, uint32_t tid,
+                          uint32_t nthr, Sync sync) {
+    constexpr uint32_t N2 = 1u << S2, logR = NTT_LOG_TILE - S2, R = 1u << logR;
+    const uint32_t N1 = 1u << g.logN1;
+    const uint32_t a0 = tile * R;
+    const TileDesc d{logR, 1, padded(N2), N1 + a0, 1, false, true};
+    row_tile_f64<false, S2, RM, R, false>(j.twd, j.twrd, j.fq, j.fqinv, d, gcast(j.in), gcast(j.out), a0 << S2, lds,
+                                          tid, nthr, sync);
 }
 
 }  // namespace gf

@@ -55,6 +55,13 @@ template <bool FWD, bool PAIRED, uint32_t S2, uint32_t RM, class J>
 __global__ __launch_bounds__(NTT_THREADS) void k_ntt_B(J jobs, NttGeom g) {
     extern __shared__ uint64_t lds[];
     const NttJob j = jobs.get(blockIdx.y);
+    if constexpr (!FWD && PAIRED) {
+        // inverse jobs that carry fp64 tables (moduli below 2^50)
+        if (j.twd) {
+            phaseB_inv_f64<S2, RM>(j, g, blockIdx.x, lds, threadIdx.x, blockDim.x, DevSync{});
+            return;
+        }
+    }
     with_lz<FWD>(j, [&](auto L) {
         phaseB_body<FWD, decltype(L)::value, PAIRED, S2, RM>(j, g, blockIdx.x, lds, threadIdx.x, blockDim.x,
                                                              DevSync{});

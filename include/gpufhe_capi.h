@@ -290,8 +290,9 @@ int gf_ctx_autotune(gf_ctx* c, hipStream_t s, uint32_t* rmax_out, float* ms_out)
  * extra wave in the fused row pass + inner product workgroups, 0/1),
  * "conv_chunk" and "rs_chunk" (targets per ModUp-conversion / rescale-prologue
  * job), "modup_fused" (ModUp as one kernel per part and column tile, 0/1),
- * "modup_f64" (that kernel's column passes in double precision for moduli
- * below 2^50, 0/1).
+ * "modup_f64" (that kernel's column passes and the inverse row pass before
+ * it in double precision for moduli below 2^50, 0/1), "ksi_f64" (with
+ * modup_f64, the fused inner product's row passes as well, 0/1).
  * The report is JSON with every candidate's time, the radix search of
  * gf_ctx_autotune included; valid until the next tuning call.  Knobs (and
  * "ntt_radix", "rechunk_wgs", "modup_min_wgs") can be set by name; plans built

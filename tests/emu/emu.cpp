@@ -4,6 +4,7 @@
 // on the CPU against the oracle before any GPU run.
 #include <cstddef>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -102,14 +103,37 @@ extern "C" void emu_ntt(uint64_t* x, uint32_t n, uint64_t q, const uint64_t* tw,
 extern "C" uint64_t emu_lazy_red_of(uint64_t q) { return lazy_red_of(q); }
 
 namespace {
+// fp64 pairs (ntt_f64.hpp) of a paired integer table's twiddles
+std::vector<uint64_t> f64_pairs_of(const uint64_t* paired, uint32_t n, uint64_t q) {
+    std::vector<uint64_t> out(2 * (size_t)n);
+    for (uint32_t k = 0; k < n; ++k) {
+        double wd, wq;
+        f64_twiddle(paired[2 * k], q, wd, wq);
+        std::memcpy(&out[2 * k], &wd, 8);
+        std::memcpy(&out[2 * k + 1], &wq, 8);
+    }
+    return out;
+}
+
 template <uint32_t RM, bool LZ>
 void emu_ksi_t(uint32_t n, uint64_t q, const uint64_t* t2, uint32_t np, uint32_t xf, uint64_t* src,
                const uint64_t* kb, const uint64_t* ka, uint64_t* o0, uint64_t* o1, uint32_t pairs,
-               uint32_t lazy_out) {
+               uint32_t lazy_out, int f64) {
     const NttGeom g = make_geom(n, RM);
     std::vector<uint64_t> rows(row_table_words(g));
     build_row_table(t2, g, rows.data());
+    std::vector<uint64_t> t2d, rowsd;
     KsiJob J{};
+    if (f64) {
+        // the row passes in fp64, with digits below 2q as the fp64 ModUp leaves them
+        t2d = f64_pairs_of(t2, n, q);
+        rowsd.resize(row_table_words(g));
+        build_row_table(t2d.data(), g, rowsd.data());
+        J.twd = t2d.data();
+        J.twrd = rowsd.data();
+        J.fq = (double)q;
+        J.fqinv = 1.0 / J.fq;
+    }
     J.np = np;
     J.xf = xf;
     J.tw = t2;
@@ -134,6 +158,8 @@ void emu_ksi_t(uint32_t n, uint64_t q, const uint64_t* t2, uint32_t np, uint32_t
                 t.red = J.red;
                 for (uint32_t tile = 0; tile < (1u << S2) / g.C; ++tile)
                     phaseA_body<true, LZ, true, S1, RM>(t, g, tile, lds.data(), 0, 1, NoSync{});
+                if (f64)
+                    for (uint32_t k = 0; k < n; ++k) src[(size_t)p * n + k] %= q;
             }
         }
         for (uint32_t tile = 0; tile < n / KSI_TILE; ++tile)
@@ -145,9 +171,11 @@ void emu_ksi_t(uint32_t n, uint64_t q, const uint64_t* t2, uint32_t np, uint32_t
 // Fused digit row pass + key inner product over np parts: part p of src holds
 // a coefficient-form tower (bit p of xf: transformed here, column pass then the
 // fused row pass) or an evaluation-form digit; o = sum_p NTT?(src_p) k_p R^-1.
+// f64 != 0 (q below 2^50): the column passes' output reduced below q, then
+// the row passes in fp64.
 extern "C" void emu_ksi(uint32_t n, uint64_t q, const uint64_t* tw, const uint64_t* twp, uint32_t rmax,
                         uint32_t np, uint32_t xf, uint64_t* src, const uint64_t* kb, const uint64_t* ka,
-                        uint64_t* o0, uint64_t* o1, uint32_t pairs, uint32_t lazy_out) {
+                        uint64_t* o0, uint64_t* o1, uint32_t pairs, uint32_t lazy_out, int f64) {
     std::vector<uint64_t> t2(2 * (std::size_t)n);
     for (uint32_t i = 0; i < n; ++i) {
         t2[2 * i] = tw[i];
@@ -155,8 +183,41 @@ extern "C" void emu_ksi(uint32_t n, uint64_t q, const uint64_t* tw, const uint64
     }
     with_radix(rmax, [&](auto Rc) {
         constexpr uint32_t RM = decltype(Rc)::value;
-        if (lazy_red_of(q)) emu_ksi_t<RM, true>(n, q, t2.data(), np, xf, src, kb, ka, o0, o1, pairs, lazy_out);
-        else emu_ksi_t<RM, false>(n, q, t2.data(), np, xf, src, kb, ka, o0, o1, pairs, lazy_out);
+        if (lazy_red_of(q)) emu_ksi_t<RM, true>(n, q, t2.data(), np, xf, src, kb, ka, o0, o1, pairs, lazy_out, f64);
+        else emu_ksi_t<RM, false>(n, q, t2.data(), np, xf, src, kb, ka, o0, o1, pairs, lazy_out, f64);
+    });
+}
+
+// Inverse transform of a two-phase ring with phase B in fp64 (q below 2^50,
+// inputs below 4q) and the integer phase A after it, in place; post, postp:
+// the final factor of phase A (n^-1 c).  Output in [0, q).
+extern "C" void emu_intt_f64(uint64_t* x, uint32_t n, uint64_t q, const uint64_t* iw, const uint64_t* iwp,
+                             uint64_t post, uint64_t postp, uint32_t rmax) {
+    std::vector<uint64_t> t2(2 * (std::size_t)n);
+    for (uint32_t i = 0; i < n; ++i) {
+        t2[2 * i] = iw[i];
+        t2[2 * i + 1] = iwp[i];
+    }
+    with_radix(rmax, [&](auto Rc) {
+        constexpr uint32_t RM = decltype(Rc)::value;
+        const NttGeom g = make_geom(n, RM);
+        const std::vector<uint64_t> t2d = f64_pairs_of(t2.data(), n, q);
+        std::vector<uint64_t> rowsd(row_table_words(g)), lds(NTT_TILE_PADDED);
+        build_row_table(t2d.data(), g, rowsd.data());
+        NttJob j{x, x, t2.data(), nullptr, q, post, postp};
+        j.twd = t2d.data();
+        j.twrd = rowsd.data();
+        j.fq = (double)q;
+        j.fqinv = 1.0 / j.fq;
+        with_split(g.logN1, g.logN2, [&](auto A, auto B) {
+            constexpr uint32_t S1 = decltype(A)::value, S2 = decltype(B)::value;
+            for (uint32_t tile = 0; tile < (1u << S1) / g.R; ++tile)
+                phaseB_inv_f64<S2, RM>(j, g, tile, lds.data(), 0, 1, NoSync{});
+            for (uint32_t k = 0; k < n; ++k)
+                if (x[k] >= 2 * q) std::abort();   // phase B hands phase A values in [0, 2q)
+            for (uint32_t tile = 0; tile < (1u << S2) / g.C; ++tile)
+                phaseA_body<false, false, true, S1, RM>(j, g, tile, lds.data(), 0, 1, NoSync{});
+        });
     });
 }
 // the key inner product's output with unreduced digits: sum_s a[s] b[s]

@@ -44,7 +44,9 @@ def emu():
     lib.emu_lazy_red_of.argtypes = [C.c_uint64]
     lib.emu_ksi.restype = None
     lib.emu_ksi.argtypes = [C.c_uint32, C.c_uint64, u64p, u64p, C.c_uint32, C.c_uint32, C.c_uint32, u64p, u64p,
-                            u64p, u64p, u64p, C.c_uint32, C.c_uint32]
+                            u64p, u64p, u64p, C.c_uint32, C.c_uint32, C.c_int]
+    lib.emu_intt_f64.restype = None
+    lib.emu_intt_f64.argtypes = [u64p, C.c_uint32, C.c_uint64, u64p, u64p, C.c_uint64, C.c_uint64, C.c_uint32]
     lib.emu_inner_red.restype = C.c_uint64
     lib.emu_inner_red.argtypes = [u64p, u64p, C.c_uint32, C.c_uint64]
     lib.emu_reduce_red.restype = C.c_uint64
@@ -236,7 +238,8 @@ def test_fused_rowpass_inner_product(emu, logn, np_, xf):
     sum_p digit_p * key_p * 2^-64 mod q with digit_p the oracle's forward
     transform, for moduli below 2^57 (unreduced digits) and above (reduced
     digits), with digits and keys below 2^60 (column sums) and above (128-bit
-    sums), every register radix."""
+    sums), every register radix; and for the 50-bit modulus with the row
+    passes in fp64 (digits below 2q, as the fp64 ModUp leaves them)."""
     n = 1 << logn
     # 50 bits: unreduced digits, column sums; 55: unreduced digits above 2^60
     # (128-bit sums); 59: reduced digits, column sums; 61: 128-bit sums
@@ -262,16 +265,37 @@ def test_fused_rowpass_inner_product(emu, logn, np_, xf):
         for rmax in (2, 3, 4):
             pairs = rmax % 2   # both access widths of the inner product
             for lazy_out in (0, 1):
-                s = src.copy()
-                o0 = np.zeros(n, np.uint64)
-                o1 = np.zeros(n, np.uint64)
-                emu.emu_ksi(n, q, O.P(fr), O.P(fp), rmax, np_, xf, O.P(s), O.P(np.ascontiguousarray(kb)),
-                            O.P(np.ascontiguousarray(ka)), O.P(o0), O.P(o1), pairs, lazy_out)
-                if lazy_out:   # unreduced, for the epilogue's lazy products: congruent, below 2^62
-                    assert int(max(o0.max(), o1.max())) < 1 << 62
-                    o0, o1 = o0 % np.uint64(q), o1 % np.uint64(q)
-                assert np.array_equal(o0, w0), (bits, rmax, pairs, lazy_out)
-                assert np.array_equal(o1, w1), (bits, rmax, pairs, lazy_out)
+                for f64 in ((0, 1) if bits == 50 else (0,)):
+                    s = src.copy()
+                    o0 = np.zeros(n, np.uint64)
+                    o1 = np.zeros(n, np.uint64)
+                    emu.emu_ksi(n, q, O.P(fr), O.P(fp), rmax, np_, xf, O.P(s), O.P(np.ascontiguousarray(kb)),
+                                O.P(np.ascontiguousarray(ka)), O.P(o0), O.P(o1), pairs, lazy_out, f64)
+                    if lazy_out:   # unreduced, for the epilogue's lazy products: congruent, below 2^62
+                        assert int(max(o0.max(), o1.max())) < 1 << 62
+                        o0, o1 = o0 % np.uint64(q), o1 % np.uint64(q)
+                    assert np.array_equal(o0, w0), (bits, rmax, pairs, lazy_out, f64)
+                    assert np.array_equal(o1, w1), (bits, rmax, pairs, lazy_out, f64)
+
+
+@pytest.mark.parametrize("logn", [13, 14, 15, 16, 17])
+def test_inverse_rowpass_f64(emu, logn):
+    """An inverse transform whose phase B (the row pass) runs in fp64 and
+    hands [0, 2q) to the integer phase A equals the oracle's inverse transform
+    times the final factor, for inputs up to 4q - 1 (the lazy range phase B
+    accepts), every register radix, 50-bit moduli."""
+    n = 1 << logn
+    for q in [int(v) for v in O.native_primes(2, 50, n)]:
+        psi = int(O.lib.or_native_root(n, q))
+        _, _, ir, ip, ni, nip = O.root_tables(n, q, psi)
+        rng = np.random.default_rng(logn + q % 1000)
+        x = rng.integers(0, 4 * q, n, dtype=np.uint64)
+        x[:7] = 4 * q - 1
+        want = O.intt(x % np.uint64(q), q, ir, ip, ni, nip)
+        for rmax in (2, 3, 4):
+            y = x.copy()
+            emu.emu_intt_f64(O.P(y), n, q, O.P(ir), O.P(ip), int(ni), int(nip), rmax)
+            assert np.array_equal(y, want), (q, rmax)
 
 
 @pytest.mark.parametrize("logn,L,dnum,part", [(13, 6, 3, 0), (14, 5, 1, 0), (15, 12, 3, 2), (16, 24, 4, 0),

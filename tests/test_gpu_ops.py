@@ -338,19 +338,23 @@ def test_batch_plan_uneven_levels_and_graph(G):
         assert np.array_equal(outs[i][1].download(), want[i][1]), ("per job", i)
 
 
-@pytest.mark.parametrize("ksi_extra,conv_chunk,rs_chunk,modup_fused", [(0, 8, 8, 1), (1, 16, 32, 1), (1, 32, 16, 0),
-                                                                      (0, 5, 3, 0)])
-def test_plan_knobs_keep_results(G, ksi_extra, conv_chunk, rs_chunk, modup_fused):
+@pytest.mark.parametrize("ksi_extra,conv_chunk,rs_chunk,modup_fused,modup_f64,ksi_f64",
+                         [(0, 8, 8, 1, 1, 1), (1, 16, 32, 1, 0, 0), (1, 32, 16, 0, 1, 1), (0, 5, 3, 0, 0, 0),
+                          (0, 8, 16, 1, 1, 0)])
+def test_plan_knobs_keep_results(G, ksi_extra, conv_chunk, rs_chunk, modup_fused, modup_f64, ksi_f64):
     """Every setting of the autotuner's plan knobs (extra inner-product wave,
     conversion and prologue chunk widths, re-chunking off so the widths take
-    effect, the fused or the unfused ModUp) gives the oracle's results, on a
-    batch of uneven levels (parts of 3 and 2 towers; ragged chunks with widths
-    5 and 3); the autotuner's own search leaves a report naming its
-    choices."""
+    effect, the fused or the unfused ModUp, the fp64 passes of the 50-bit
+    towers around the fused ModUp and in the fused inner product) gives the
+    oracle's results, on a batch of uneven levels (parts of 3 and 2 towers;
+    ragged chunks with widths 5 and 3); the autotuner's own search leaves a
+    report naming its choices."""
     n, L, dnum = 1 << 13, 6, 3
     P, ctx, k, av, bv = setup(G, n, L, dnum, seed=77)   # a context of its own: the knobs stay here
+    assert any(int(q) < 1 << 50 for q in P.q)   # the fp64 passes have towers to take
     for name, v in (("ksi_extra", ksi_extra), ("conv_chunk", conv_chunk), ("rs_chunk", rs_chunk),
-                    ("modup_fused", modup_fused), ("modup_min_wgs", 0), ("rechunk_wgs", 0)):
+                    ("modup_fused", modup_fused), ("modup_f64", modup_f64), ("ksi_f64", ksi_f64),
+                    ("modup_min_wgs", 0), ("rechunk_wgs", 0)):
         ctx.set_knob(name, v)
         assert ctx.knob(name) == v
     rng = np.random.default_rng(11 + conv_chunk)
@@ -374,8 +378,10 @@ def test_plan_knobs_keep_results(G, ksi_extra, conv_chunk, rs_chunk, modup_fused
         ctx.set_knob("rechunk_wgs", 8192)
         ctx.autotune(s)
         rep = ctx.tuning_report()
-        for name in ("ntt_radix", "ksi_extra", "conv_chunk", "rs_chunk", "modup_fused"):
+        for name in ("ntt_radix", "ksi_extra", "conv_chunk", "rs_chunk", "modup_fused", "modup_f64"):
             assert rep[name]["chosen"] == ctx.knob(name), name
+        if ctx.knob("modup_f64"):
+            assert rep["ksi_f64"]["chosen"] == ctx.knob("ksi_f64")
 
 
 def test_rotate_plan_multi_keys(G):

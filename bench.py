@@ -56,7 +56,8 @@ STEP_KERNELS = {"ntt_fwd": ("k_ntt_A", "k_ntt_B"), "ntt_inv": ("k_ntt_B", "k_ntt
                 "combine": ("k_combine",), "switch": ("k_switch",), "fuse": ("k_fuse",),
                 "add": ("k_add",), "perm": ("k_perm",), "copy": ("k_copy",),
                 "rsv": ("k_rs",), "ptacc": ("k_ptacc",), "ntt_fwd_a": ("k_ntt_A",), "ksi": ("k_ksi",),
-                "ntt_inv_b": ("k_ntt_B",), "modup": ("k_modup",)}
+                "ntt_inv_b": ("k_ntt_B",), "modup": ("k_modup",), "moddown": ("k_modup",),
+                "ntt_fwd_b": ("k_ntt_B",)}
 
 
 def kernel_bytes(name, kind, nkern, total, epilogue):
@@ -82,7 +83,7 @@ def step_bytes(n, L, K, alpha, np_parts, kinds, batch):
     sc = [L - min(alpha, L - alpha * p) + K for p in range(np_parts)]
     sp = [min(alpha, L - alpha * p) for p in range(np_parts)]
     if "ksi" in kinds:
-        assert kinds.count("ksi") == 2 and len(kinds) == 9, kinds
+        assert kinds.count("ksi") == 2 and len(kinds) in (9, 10), kinds
     tail = [
         ("intt_moddown", 2 * T * (2 * K + 2), 0),       # + INTT of res0/res1's last tower
         # per component: the K P-part sources and the dropped tower's two
@@ -93,12 +94,26 @@ def step_bytes(n, L, K, alpha, np_parts, kinds, batch):
     ]
     ksi = [("rowpass_inner_q", T * L * ((np_parts - 1) + 1 + key + 2), 0),
            ("rowpass_inner_p", T * K * (np_parts + key + 2), 0)]
+    if "moddown" in kinds:
+        # fused ModDown, charged as the unfused stages it replaces: the P
+        # towers' inverse row pass (half of their 16 n B) apart, the whole
+        # inverse transforms of res0/res1's last tower, then one kernel with
+        # the P towers' inverse column pass (the other half), the prologue
+        # (its operands once) and the surviving towers' forward column pass
+        # (half); the epilogue's row pass (the other half and its operands)
+        assert kinds == ["tensor", "ntt_inv_b", "ntt_inv", "modup", "ksi", "ksi", "ntt_inv", "ntt_inv_b",
+                         "moddown", "ntt_fwd_b"], kinds
+        tail = [("intt_moddown_last", 2 * T * 2, 0),
+                ("intt_moddown_rowpass", T * 2 * K, 0),
+                ("moddown_fused", T * 2 * K + T * 2 * (K + 2 + (L - 1)) + T * 2 * (L - 1), 0),
+                ("ntt_rescale_epilogue_rowpass", T * 2 * (L - 1) + 2 * T * 2 * (L - 1), 2 * T * 2 * (L - 1))]
     if "modup" in kinds:
         # fused ModUp: the inverse transforms' row pass (half of their 16 n B),
         # the two extra towers' whole inverse transforms, then one kernel with
         # the inverse column passes (the other half), the conversion (sources
         # and targets once) and the digits' column passes (half of theirs)
-        assert kinds == ["tensor", "ntt_inv_b", "ntt_inv", "modup", "ksi", "ksi", "ntt_inv", "rsv", "ntt_fwd"], kinds
+        assert kinds[:6] == ["tensor", "ntt_inv_b", "ntt_inv", "modup", "ksi", "ksi"], kinds
+        assert "moddown" in kinds or kinds[6:] == ["ntt_inv", "rsv", "ntt_fwd"], kinds
         return [("tensor", 7 * T * L, 0),
                 ("intt_modup_rowpass", T * L, 0),
                 ("intt_extra", 2 * T * 2, 0),
@@ -691,7 +706,7 @@ def main():
     else:
         radix, ntt_ms = ctx.autotune(stream, args.batch)   # NTT register radix, then the plan knobs at B
     knobs = {k_: ctx.knob(k_) for k_ in ("ntt_radix", "ksi_extra", "conv_chunk", "rs_chunk", "modup_fused",
-                                          "modup_f64", "ksi_f64")}
+                                          "modup_f64", "ksi_f64", "moddown_fused")}
     log("context ready: K=%d alpha=%d ws=%.1f MiB/op, knobs %s (%s)"
         % (K, alpha, ctx.ws_bytes / 2**20, knobs, "pinned" if pinned else "autotuned"))
     tuned = {"shape": {"n": N, "L": L, "dnum": DNUM}, "chosen": knobs, "pinned": pinned,

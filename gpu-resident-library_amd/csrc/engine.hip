@@ -253,6 +253,24 @@ Ctx::Ctx(uint32_t n_, const std::vector<uint64_t>& q, const std::vector<uint64_t
             }
             put(tabRS, &c.d_tabRS);
             put(tg, (const uint64_t**)&c.d_rstg);
+            // the surviving towers as the fused ModDown's targets
+            std::vector<uint64_t> rq(l - 1), rqn(l - 1), rtw(l - 1), rred(l - 1), rtwd(l - 1), rqf(2 * (size_t)(l - 1));
+            for (uint32_t i = 0; i + 1 < l; ++i) {
+                rq[i] = mods[i];
+                rqn[i] = host_qneg(mods[i]);
+                rtw[i] = (uint64_t)(uintptr_t)rt[i].fw2;
+                rred[i] = lazy_red_of(mods[i]);
+                rtwd[i] = (uint64_t)(uintptr_t)rt[i].fw2d;
+                const double qd = (double)mods[i], qi = 1.0 / qd;
+                std::memcpy(&rqf[2 * i], &qd, 8);
+                std::memcpy(&rqf[2 * i + 1], &qi, 8);
+            }
+            put(rq, &c.d_rs_tq);
+            put(rqn, &c.d_rs_tqneg);
+            put(rtw, &c.d_rs_ttw);
+            put(rred, &c.d_rs_tred);
+            put(rtwd, &c.d_rs_ttwd);
+            put(rqf, &c.d_rs_tqf);
         }
     }
     put(std::vector<uint64_t>(L + K, 0), &d_zeros);
@@ -385,9 +403,9 @@ void Ctx::set_knob(const std::string& name, uint32_t v) {
     } else if (name == "modup_fused") {
         if (v > 1) throw std::runtime_error("gpufhe: modup_fused must be 0 or 1");
         modup_fused = v;
-    } else if (name == "modup_f64" || name == "ksi_f64") {
+    } else if (name == "modup_f64" || name == "ksi_f64" || name == "moddown_fused") {
         if (v > 1) throw std::runtime_error("gpufhe: " + name + " must be 0 or 1");
-        (name == "modup_f64" ? modup_f64 : ksi_f64) = v;
+        (name == "modup_f64" ? modup_f64 : name == "ksi_f64" ? ksi_f64 : moddown_fused) = v;
     } else if (name == "modup_min_wgs") {
         modup_min_wgs = v;
     } else if (name == "rechunk_wgs") {
@@ -404,6 +422,7 @@ uint32_t Ctx::knob(const std::string& name) const {
     if (name == "modup_fused") return modup_fused;
     if (name == "modup_f64") return modup_f64;
     if (name == "ksi_f64") return ksi_f64;
+    if (name == "moddown_fused") return moddown_fused;
     if (name == "modup_min_wgs") return modup_min_wgs;
     if (name == "rechunk_wgs") return rechunk_wgs;
     throw std::runtime_error("gpufhe: unknown tuning knob " + name);
@@ -505,7 +524,18 @@ void Ctx::autotune_plan(hipStream_t s, uint32_t B) {
         modup_fused = mf;
         tune("modup_fused", modup_fused, {0, 1}, {Step::NTT_IB, Step::NTT_I, Step::MODUP, Step::CONV, Step::NTT_FA});
     }
-    if (fused) tune("rs_chunk", rs_chunk, {8, 16, 32}, {Step::RSV});
+    if (fused) {
+        // the prologue's chunk width matters where the ModDown runs apart:
+        // tuned that way, then the fused ModDown is weighed against it
+        const uint32_t md = moddown_fused;
+        moddown_fused = 0;
+        tune("rs_chunk", rs_chunk, {8, 16, 32}, {Step::RSV});
+        moddown_fused = md;
+        if (modup_fused && !geom.single && n % KSI_TILE == 0 && n % MODUP_TILE == 0 && !wide &&
+            alpha <= MODUP_MAXSRC && K <= MODUP_MAXSRC)
+            tune("moddown_fused", moddown_fused, {0, 1},
+                 {Step::NTT_IB, Step::NTT_I, Step::MODDOWN, Step::RSV, Step::NTT_F, Step::NTT_FB});
+    }
     rep << "}";
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -857,6 +887,8 @@ void Plan::run_jobs(size_t i, hipStream_t s, uint32_t which, uint32_t first, uin
         case Step::KSI: launch_ksi(GF_J(KsiJob), cnt, geom, st.arg & 0xFF, st.arg >> 8, s); break;
         case Step::NTT_IB: launch_ntt(GF_J(NttJob), cnt, geom, false, s, 1); break;
         case Step::MODUP: launch_modup(GF_J(ModupJob), cnt, geom, st.arg, s); break;
+        case Step::MODDOWN: launch_moddown(GF_J(ModupJob), cnt, geom, st.arg, s); break;
+        case Step::NTT_FB: launch_ntt(GF_J(NttJob), cnt, geom, true, s, 2); break;
     }
 #undef GF_J
 }
@@ -908,15 +940,16 @@ static void split_chunks(std::vector<J>& v, uint32_t n, uint32_t min_wgs) {
 // transforms (plain n^-1) ride in the first INTT launch.
 void PlanBuilder::keyswitch_core(uint32_t l, const uint64_t* d, const DevKey& key, uint64_t* ws,
                                  const uint64_t* extra0, const uint64_t* extra1, uint64_t* dx0,
-                                 uint64_t* dx1, uint32_t lazy_rows) {
+                                 uint64_t* dx1, uint32_t lazy_rows, bool p_rowpass) {
     const Ctx& C = C_;
     if (l < 1 || l > C.L) throw std::runtime_error("gpufhe: bad level");
     if (key.nparts < C.lv[l].np) throw std::runtime_error("gpufhe: key has too few parts for this level");
     if (use_ksi(l)) {
         modup(l, d, ws, extra0, extra1, dx0, dx1, true);
-        ksi_moddown_src(l, d, key, ws, lazy_rows);
+        ksi_moddown_src(l, d, key, ws, lazy_rows, p_rowpass);
         return;
     }
+    if (p_rowpass) throw std::runtime_error("gpufhe: the fused ModDown needs the fused inner product");
     modup(l, d, ws, extra0, extra1, dx0, dx1);
     const uint64_t* dig = ws + WsLayout::make(C).dig * C.n;
     inner_moddown_src(l, key, ws, [&](uint32_t pi, uint32_t i) { return digit_tower(l, d, dig, pi, i); });
@@ -929,6 +962,11 @@ bool PlanBuilder::use_ksi(uint32_t l) const {
 bool PlanBuilder::use_modup(uint32_t l) const {
     return C_.modup_fused && use_ksi(l) && !C_.wide && C_.alpha <= MODUP_MAXSRC && C_.n % MODUP_TILE == 0;
 }
+// the fused ModDown + rescale prologue: the fused ModUp's kernel over the K
+// P towers
+bool PlanBuilder::use_moddown(uint32_t l) const {
+    return C_.moddown_fused && use_modup(l) && C_.K <= MODUP_MAXSRC && l >= 2;
+}
 
 // 3b-5. The digits' row pass fused with the inner product over Q_l P
 // (src/keyswitch_resident.cpp:171-200), into the workspace accumulators;
@@ -937,7 +975,7 @@ bool PlanBuilder::use_modup(uint32_t l) const {
 // the column-pass output of their forward transform (modup with
 // rowpass_in_ksi).
 void PlanBuilder::ksi_moddown_src(uint32_t l, const uint64_t* d, const DevKey& key, uint64_t* ws,
-                                  uint32_t lazy_rows) {
+                                  uint32_t lazy_rows, bool p_rowpass) {
     const Ctx& C = C_;
     const uint32_t n = C.n, K = C.K, W = l + K;
     const LevelConst& lv = C.lv[l];
@@ -974,8 +1012,9 @@ void PlanBuilder::ksi_moddown_src(uint32_t l, const uint64_t* d, const DevKey& k
     for (int c = 0; c < 2; ++c)
         for (uint32_t k = 0; k < K; ++k) {
             const uint32_t r = C.row_p(k);
-            intt_p_.push_back(NttJob{res[c] + (size_t)(l + k) * n, xp + (size_t)(c * K + k) * n, C.rt[r].iv2,
-                                     nullptr, C.mods[r], lv.pd_post[k], lv.pd_postp[k]});
+            (p_rowpass ? intt_pb_ : intt_p_)
+                .push_back(NttJob{res[c] + (size_t)(l + k) * n, xp + (size_t)(c * K + k) * n, C.rt[r].iv2, nullptr,
+                                  C.mods[r], lv.pd_post[k], lv.pd_postp[k]});
         }
 }
 
@@ -1158,7 +1197,6 @@ void PlanBuilder::evalmult_rescale_fused(uint32_t l, const uint64_t* c0a, const 
                                          uint64_t* o1, const DevKey& key, uint64_t* ws) {
     const Ctx& C = C_;
     const uint32_t n = C.n, K = C.K, last = l - 1;
-    const LevelConst& lv = C.lv[l];
     const WsLayout w = WsLayout::make(C);
     uint64_t* t[3] = {ws + w.t0 * n, ws + w.t1 * n, ws + w.t2 * n};
     for (uint32_t i = 0; i < l; ++i) {
@@ -1168,8 +1206,9 @@ void PlanBuilder::evalmult_rescale_fused(uint32_t l, const uint64_t* c0a, const 
     }
     uint64_t* dt = ws + w.dt * n;
     uint64_t* dr = ws + w.dr * n;
+    const bool fmd = use_moddown(l);
     // accumulator rows below `last` only feed the epilogue's lazy Shoup products
-    keyswitch_core(l, t[2], key, ws, t[0] + (size_t)last * n, t[1] + (size_t)last * n, dt, dt + n, last);
+    keyswitch_core(l, t[2], key, ws, t[0] + (size_t)last * n, t[1] + (size_t)last * n, dt, dt + n, last, fmd);
     uint64_t* res[2] = {ws + w.res0 * n, ws + w.res1 * n};
     uint64_t* xp = ws + w.xp * n;
     uint64_t* v = ws + w.dig * n;     // digits are dead after the inner product
@@ -1177,13 +1216,64 @@ void PlanBuilder::evalmult_rescale_fused(uint32_t l, const uint64_t* c0a, const 
     for (int c = 0; c < 2; ++c) {
         intt_p_.push_back(NttJob{res[c] + (size_t)last * n, dr + (size_t)c * n, C.rt[last].iv2, nullptr,
                                  C.mods[last], C.rt[last].ninv, C.rt[last].ninvp});
-        uint64_t* vc = v + (size_t)c * last * n;
+        const ModdownSrc m{l, (uint32_t)c, xp + (size_t)c * K * n, dt + (size_t)c * n, dr + (size_t)c * n,
+                           t[c], res[c], v + (size_t)c * last * n, outs[c]};
+        rescale_tail(m, fmd);
+    }
+}
+
+// The rescale tail of one component: from the K P towers (coefficient form,
+// or after their inverse row pass when fused), the two dropped-tower
+// operands and the accumulators, the surviving output towers.
+void PlanBuilder::rescale_tail(const ModdownSrc& m, bool fused) {
+    const Ctx& C = C_;
+    const uint32_t n = C.n, K = C.K, l = m.l, last = l - 1;
+    const LevelConst& lv = C.lv[l];
+    if (fused) {
+        ModupJob J{};
+        for (uint32_t k = 0; k < K; ++k) {
+            const uint32_t r = C.row_p(k);
+            J.src[k] = m.xp + (size_t)k * n;
+            J.stw[k] = C.rt[r].iv2;
+            J.sq[k] = C.mods[r];
+            J.post[k] = lv.pd_post[k];
+            J.postp[k] = lv.pd_postp[k];
+            if (C.modup_f64 && C.rt[r].iv2d) {
+                J.stwd[k] = C.rt[r].iv2d;
+                J.sqd[k] = (double)C.mods[r];
+                J.sqinv[k] = 1.0 / J.sqd[k];
+                f64_twiddle(lv.pd_post[k], C.mods[r], J.spost[k], J.spostq[k]);
+            }
+        }
+        J.dst = m.out;   // surviving tower i's column-pass output at out + i n
+        J.tab = lv.d_tabRS;
+        J.tq = lv.d_rs_tq;
+        J.tqneg = lv.d_rs_tqneg;
+        J.ttw = lv.d_rs_ttw;
+        J.tred = lv.d_rs_tred;
+        J.ttwd = C.modup_f64 ? lv.d_rs_ttwd : C.d_zeros;
+        J.tqf = lv.d_rs_tqf;
+        J.sp = K;
+        J.sc = last;
+        J.dt = m.dt;
+        J.dr = m.dr;
+        J.tabd = lv.d_tabMD + last;   // PHatModq column of q_last, row stride l
+        J.tg = lv.d_rstg;
+        J.q_old = C.mods[last];
+        J.half = C.mods[last] >> 1;
+        J.qd_neg = C.rc[last].qneg;
+        J.pinv = lv.pinv[last];
+        J.pinvp = lv.pinvp[last];
+        J.strided = l;
+        moddown_.push_back(J);
+        moddown_src_.push_back(m);
+    } else {
         for (uint32_t j0 = 0; j0 < last; j0 += C.rs_chunk) {
             RsJob R{};
-            R.xp = xp + (size_t)c * K * n;
-            R.dt = dt + (size_t)c * n;
-            R.dr = dr + (size_t)c * n;
-            R.out = vc;
+            R.xp = m.xp;
+            R.dt = m.dt;
+            R.dr = m.dr;
+            R.out = m.vc;
             R.tab = lv.d_tabRS;
             R.tg = lv.d_rstg;
             R.tabd = lv.d_tabMD + last;   // PHatModq column of q_last, row stride l
@@ -1200,16 +1290,18 @@ void PlanBuilder::evalmult_rescale_fused(uint32_t l, const uint64_t* c0a, const 
             R.wide = C.wide;
             rs_.push_back(R);
         }
-        for (uint32_t i = 0; i < last; ++i) {
-            NttJob J{vc + (size_t)i * n, outs[c] + (size_t)i * n, C.rt[i].fw2, nullptr, C.mods[i], 0, 0};
-            J.e0 = t[c] + (size_t)i * n;
-            J.c0 = lv.s1[i];
-            J.c0p = lv.s1p[i];
-            J.e1 = res[c] + (size_t)i * n;
-            J.c1 = lv.ps1[i];
-            J.c1p = lv.ps1p[i];
-            ntt_v_.push_back(J);
-        }
+    }
+    for (uint32_t i = 0; i < last; ++i) {
+        // fused: the column pass is done, this row pass reads and writes out
+        NttJob J{fused ? m.out + (size_t)i * n : m.vc + (size_t)i * n, m.out + (size_t)i * n, C.rt[i].fw2, nullptr,
+                 C.mods[i], 0, 0};
+        J.e0 = m.t + (size_t)i * n;
+        J.c0 = lv.s1[i];
+        J.c0p = lv.s1p[i];
+        J.e1 = m.res + (size_t)i * n;
+        J.c1 = lv.ps1[i];
+        J.c1p = lv.ps1p[i];
+        (fused ? ntt_vb_ : ntt_v_).push_back(J);
     }
 }
 
@@ -1468,7 +1560,19 @@ std::unique_ptr<Plan> PlanBuilder::layout(std::vector<char>& blob) {
         modup_.clear();
         modup_src_.clear();
     }
-    for (auto* v : {&intt_up_, &intt_upb_, &ntt_dig_, &ntt_diga_, &intt_p_, &ntt_conv_, &intt_drop_, &ntt_sw_, &ntt_v_}) {
+    // likewise a fused ModDown stage: the P towers' whole inverse transforms,
+    // the prologue, the epilogue transforms whole
+    if (!moddown_.empty() && moddown_.size() * (C_.n / MODUP_TILE) < C_.modup_min_wgs) {
+        for (NttJob& j : intt_pb_) intt_p_.push_back(j);
+        intt_pb_.clear();
+        ntt_vb_.clear();
+        moddown_.clear();
+        std::vector<ModdownSrc> ms;
+        ms.swap(moddown_src_);
+        for (const ModdownSrc& m : ms) rescale_tail(m, false);
+    }
+    for (auto* v : {&intt_up_, &intt_upb_, &ntt_dig_, &ntt_diga_, &intt_p_, &intt_pb_, &ntt_conv_, &intt_drop_,
+                    &ntt_sw_, &ntt_v_, &ntt_vb_}) {
         group_by(*v, [](const NttJob& j) { return (uintptr_t)j.tw; });
         // every transform of a plan uses the context's paired tables; its phase B
         // reads the matching row table of the current radix
@@ -1479,7 +1583,7 @@ std::unique_ptr<Plan> PlanBuilder::layout(std::vector<char>& blob) {
     }
     // forward transforms over moduli below 2^57 skip the butterflies'
     // conditional subtractions; the digits stay unreduced for the inner product
-    for (auto* v : {&ntt_dig_, &ntt_diga_, &ntt_conv_, &ntt_sw_, &ntt_v_})
+    for (auto* v : {&ntt_dig_, &ntt_diga_, &ntt_conv_, &ntt_sw_, &ntt_v_, &ntt_vb_})
         for (NttJob& j : *v) j.red = lazy_red_of(j.q);
     for (NttJob& j : ntt_dig_) j.lazy_out = j.red != 0;
     for (KsiJob& j : ksi_) {
@@ -1554,6 +1658,17 @@ std::unique_ptr<Plan> PlanBuilder::layout(std::vector<char>& blob) {
         i = e;
     }
     GF_STAGE(NTT_I, intt_p_);
+    GF_STAGE(NTT_IB, intt_pb_);
+    // fused ModDown jobs by source count (one launch each)
+    std::stable_sort(moddown_.begin(), moddown_.end(),
+                     [](const ModupJob& a, const ModupJob& b) { return a.sp < b.sp; });
+    for (size_t i = 0; i < moddown_.size();) {
+        size_t e = i;
+        while (e < moddown_.size() && moddown_[e].sp == moddown_[i].sp) ++e;
+        stage(Step::MODDOWN, moddown_.data() + i, e - i, sizeof(ModupJob));
+        P->steps.back().arg = moddown_[i].sp;
+        i = e;
+    }
     GF_STAGE(CONV, conv_dn_);
     GF_STAGE(NTT_F, ntt_conv_);
     GF_STAGE(COMBINE, combine_);
@@ -1565,6 +1680,7 @@ std::unique_ptr<Plan> PlanBuilder::layout(std::vector<char>& blob) {
     GF_STAGE(FUSE, fuse_);
     GF_STAGE(RSV, rs_);
     GF_STAGE(NTT_F, ntt_v_);
+    GF_STAGE(NTT_FB, ntt_vb_);
     GF_STAGE(ADD, add_);
     GF_STAGE(COPY, copy_);
     GF_STAGE(PTACC, ptacc_);

@@ -85,6 +85,13 @@ struct LevelConst {
     std::vector<uint64_t> ps1, ps1p;           // P^-1 s1 mod q_i (fused ModDown + rescale)
     const uint64_t* d_tabRS = nullptr;         // PHatModq * (-P^-1 s1) * R mod q_i, [K][l-1]
     const RsTarget* d_rstg = nullptr;          // [l-1]
+    // the surviving towers as targets of the fused ModDown (ModupJob tables, [l-1])
+    const uint64_t* d_rs_tq = nullptr;
+    const uint64_t* d_rs_tqneg = nullptr;
+    const uint64_t* d_rs_ttw = nullptr;
+    const uint64_t* d_rs_tred = nullptr;
+    const uint64_t* d_rs_ttwd = nullptr;
+    const uint64_t* d_rs_tqf = nullptr;        // [2 (l-1)]
 };
 
 // Resident parameter set: Q moduli (L), P moduli (K), part width alpha.
@@ -136,6 +143,11 @@ public:
     // 1 (with modup_f64): the row passes of the fused inner product after it
     // as well (fewer instructions, but measured slower on gfx950)
     uint32_t ksi_f64 = 0;
+    // 1: the fused EvalMult+Rescale runs the P towers' inverse column pass,
+    // the rescale prologue and the surviving towers' forward column pass as
+    // one kernel (the fused ModUp kernel's rescale mode), where the fused
+    // ModUp runs; stages below modup_min_wgs workgroups run them apart
+    uint32_t moddown_fused = 1;
     // a plan whose fused ModUp stage would launch fewer workgroups than this
     // runs it unfused (a single operation's stage cannot fill the chip)
     uint32_t modup_min_wgs = 2048;
@@ -149,8 +161,8 @@ public:
     void autotune_plan(hipStream_t s, uint32_t batch);
     std::string tune_report;
     // sets or reads a knob by name ("ntt_radix", "ksi_extra", "conv_chunk",
-    // "rs_chunk", "modup_fused", "modup_f64", "ksi_f64", "modup_min_wgs",
-    // "rechunk_wgs")
+    // "rs_chunk", "modup_fused", "modup_f64", "ksi_f64", "moddown_fused",
+    // "modup_min_wgs", "rechunk_wgs")
     void set_knob(const std::string& name, uint32_t v);
     uint32_t knob(const std::string& name) const;
     std::string radix_report;   // the radix search of autotune(), as JSON
@@ -190,7 +202,7 @@ struct Step {
     // NTT_IB: an inverse transform's first (row) pass only; MODUP: the fused
     // ModUp (modup.hpp), arg = the jobs' source count
     enum Kind { NTT_F, NTT_I, TENSOR, CONV, INNER, COMBINE, SWITCH, FUSE, ADD, PERM, COPY, RSV, PTACC,
-                NTT_FA, KSI, NTT_IB, MODUP } kind;
+                NTT_FA, KSI, NTT_IB, MODUP, MODDOWN, NTT_FB } kind;
     size_t off;     // byte offset of the first job in the plan's device buffer
     uint32_t count;
     // KSI: LDS regions (bits 0-7) and waves per workgroup (bits 8-15); the
@@ -295,14 +307,30 @@ private:
     std::vector<FuseJob> fuse_;
     std::vector<RsJob> rs_;
     std::vector<NttJob> ntt_v_;
+    std::vector<NttJob> intt_pb_;    // P towers whose inverse column pass runs in the fused ModDown
+    std::vector<ModupJob> moddown_;
+    std::vector<NttJob> ntt_vb_;     // epilogue transforms whose column pass runs in the fused ModDown
+    struct ModdownSrc {
+        uint32_t l, c;
+        const uint64_t *xp, *dt, *dr, *t, *res;
+        uint64_t *vc, *out;
+    };
+    std::vector<ModdownSrc> moddown_src_;   // what each fused ModDown job computes, to run it apart
+    // the rescale tail of one component of the fused EvalMult+Rescale: the
+    // prologue and the epilogue transforms (apart), or the fused ModDown and
+    // the epilogue's row pass
+    void rescale_tail(const ModdownSrc& m, bool fused);
+    bool use_moddown(uint32_t l) const;
     void evalmult_rescale_fused(uint32_t l, const uint64_t* c0a, const uint64_t* c1a,
                                 const uint64_t* c0b, const uint64_t* c1b, uint64_t* o0, uint64_t* o1,
                                 const DevKey& key, uint64_t* ws);
     // lazy_rows: accumulator rows below it only feed the fused rescale
     // epilogue, so the fused inner product may leave them unreduced
+    // p_rowpass: the P towers' inverse transforms stop after their row pass
+    // (their column pass runs in the fused ModDown)
     void keyswitch_core(uint32_t l, const uint64_t* d, const DevKey& key, uint64_t* ws,
                         const uint64_t* extra0, const uint64_t* extra1, uint64_t* dx0, uint64_t* dx1,
-                        uint32_t lazy_rows = 0);
+                        uint32_t lazy_rows = 0, bool p_rowpass = false);
     void modup(uint32_t l, const uint64_t* d, uint64_t* ws, const uint64_t* extra0, const uint64_t* extra1,
                uint64_t* dx0, uint64_t* dx1, bool rowpass_in_ksi = false);
     // the digits' row pass fused with the key inner product (two-phase rings,
@@ -310,7 +338,8 @@ private:
     bool use_ksi(uint32_t l) const;
     bool use_modup(uint32_t l) const;
     void modup_unfused(uint32_t l, uint32_t pi, const uint64_t* x, uint64_t* dp, bool colpass_only);
-    void ksi_moddown_src(uint32_t l, const uint64_t* d, const DevKey& key, uint64_t* ws, uint32_t lazy_rows);
+    void ksi_moddown_src(uint32_t l, const uint64_t* d, const DevKey& key, uint64_t* ws, uint32_t lazy_rows,
+                         bool p_rowpass = false);
     const uint64_t* digit_tower(uint32_t l, const uint64_t* d, const uint64_t* dig, uint32_t pi,
                                 uint32_t i) const;
     // gk != 0: the digits are read through the automorphism of index gk

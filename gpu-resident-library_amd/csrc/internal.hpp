@@ -104,11 +104,6 @@ struct PermJob {
     uint64_t k;
 };
 
-// per surviving tower constants of the fused ModDown + rescale prologue
-struct RsTarget {
-    uint64_t q, qneg, diff, new_larger, s2, s2p;
-    uint64_t red;   // floor(2^64 / q): Shoup reduction x mod q = shoup(x, 1, red, q)
-};
 // Fused ModDown + rescale prologue of one component.  First the dropped tower
 // of r = t + ba in coefficient form (q_old = q_last):
 //   drop = dt + (dr - REDC(sum_k xp[k] * tabd[k*strided])) * pinv
@@ -242,6 +237,7 @@ void launch_ptacc(const PtAccJob* d, uint32_t nj, uint32_t n, hipStream_t s);
 void launch_ksi(const KsiJob* d, uint32_t nj, const NttGeom& g, uint32_t regions, uint32_t waves, hipStream_t s);
 // the fused ModUp over jobs of sp sources each
 void launch_modup(const ModupJob* d, uint32_t nj, const NttGeom& g, uint32_t sp, hipStream_t s);
+void launch_moddown(const ModupJob* d, uint32_t nj, const NttGeom& g, uint32_t sp, hipStream_t s);
 
 inline SwitchJob make_switch(const uint64_t* in, uint64_t* out, uint64_t q_old, uint64_t q_new) {
     SwitchJob j;

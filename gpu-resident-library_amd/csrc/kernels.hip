@@ -277,18 +277,8 @@ __device__ __forceinline__ void rs_one(const RsJob& j, uint32_t n, uint32_t i) {
         const U128 acc = dot_src<K>(j.xp, n, i, j.K, tab, j.ntgt, t, x);
         if constexpr (K > 0) {
             // the output only feeds the epilogue's forward transform, which
-            // takes inputs below 4q: the lazy Shoup product is below 2q and
-            // REDC without its correction below 1.5q + 1 (at most 8 sources
-            // below 2^60)
-            uint64_t sw;
-            if (tg.new_larger) {
-                sw = dv + (hi ? tg.diff : 0);
-            } else {
-                const uint64_t d = hi ? tg.diff : 0;
-                const uint64_t xm = shoup(dv, 1, tg.red, tg.q);
-                sw = xm >= d ? xm - d : xm + tg.q - d;
-            }
-            ga(j.out, t * n + i) = shoup_lazy(sw, tg.s2, tg.s2p, tg.q) + redc_lazy(acc, tg.q, tg.qneg);
+            // takes inputs below 4q (the fused ModDown computes the same value)
+            ga(j.out, t * n + i) = rs_lazy_value(dv, hi, tg, acc);
             continue;
         }
         const uint64_t conv = redc(acc, tg.q, tg.qneg);

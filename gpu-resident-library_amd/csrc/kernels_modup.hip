@@ -20,7 +20,8 @@ __device__ __forceinline__ void modup_sources(const ModupJob& j, const NttGeom& 
     }
 }
 
-template <uint32_t S1, uint32_t SP>
+// RS: the rescale mode (the fused ModDown + rescale prologue; ModupJob)
+template <uint32_t S1, uint32_t SP, bool RS>
 __global__ __launch_bounds__(MODUP_THREADS) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_modup(const ModupJob* __restrict__ jobs, uint32_t off, NttGeom g) {
     __shared__ uint64_t lds[MODUP_TILE];
     const ModupJob& j = jobs[off + blockIdx.y];
@@ -34,31 +35,49 @@ __global__ __launch_bounds__(MODUP_THREADS) __attribute__((amdgpu_waves_per_eu(4
     const uint32_t tile = nt % 8 ? blockIdx.x : (blockIdx.x % 8) * (nt / 8) + blockIdx.x / 8;
     ModupUnit<SP> x;
     modup_sources<S1, SP, 0>(j, g, lds, tid, tile, x);
-    for (uint32_t t = 0; t < j.sc; ++t) {
-        modup_convert_unit<SP, S1>(j, t, lds, tid, x);
-        __syncthreads();
-        modup_target_pass<S1>(j, t, g, tile, lds, tid, MODUP_THREADS, DevSync{});
-        __syncthreads();
+    if constexpr (RS) {
+        uint64_t dv[MODUP_UNIT];
+        modup_drop_unit<SP, S1>(j, g, tile, tid, x, dv);
+        for (uint32_t t = 0; t < j.sc; ++t) {
+            modup_rs_unit<SP, S1>(j, t, lds, tid, x, dv);
+            __syncthreads();
+            modup_target_pass<S1>(j, t, g, tile, lds, tid, MODUP_THREADS, DevSync{});
+            __syncthreads();
+        }
+    } else {
+        for (uint32_t t = 0; t < j.sc; ++t) {
+            modup_convert_unit<SP, S1>(j, t, lds, tid, x);
+            __syncthreads();
+            modup_target_pass<S1>(j, t, g, tile, lds, tid, MODUP_THREADS, DevSync{});
+            __syncthreads();
+        }
     }
 }
 
 // jobs of one launch share their source count sp
-void launch_modup(const ModupJob* d, uint32_t nj, const NttGeom& g, uint32_t sp, hipStream_t s) {
+template <bool RS>
+static void launch_modup_t(const ModupJob* d, uint32_t nj, const NttGeom& g, uint32_t sp, hipStream_t s) {
     if (g.single || g.n % MODUP_TILE || sp < 1 || sp > MODUP_MAXSRC)
-        throw std::runtime_error("gpufhe: fused ModUp launch shape out of range");
+        throw std::runtime_error("gpufhe: fused ModUp / ModDown launch shape out of range");
     const bool ok = with_split(g.logN1, g.logN2, [&](auto A, auto) {
         constexpr uint32_t S1 = decltype(A)::value;
         for (uint32_t off = 0; off < nj; off += MAXY) {
             const uint32_t cnt = nj - off < MAXY ? nj - off : MAXY;
             const dim3 grid(g.n / MODUP_TILE, cnt);
             switch (sp) {
-#define GF_SP(k) case k: k_modup<S1, k><<<grid, MODUP_THREADS, 0, s>>>(d, off, g); break;
+#define GF_SP(k) case k: k_modup<S1, k, RS><<<grid, MODUP_THREADS, 0, s>>>(d, off, g); break;
                 GF_SP(1) GF_SP(2) GF_SP(3) GF_SP(4) GF_SP(5) GF_SP(6)
 #undef GF_SP
             }
         }
     });
     if (!ok) throw std::runtime_error("gpufhe: unsupported ring dimension for the fused ModUp");
+}
+void launch_modup(const ModupJob* d, uint32_t nj, const NttGeom& g, uint32_t sp, hipStream_t s) {
+    launch_modup_t<false>(d, nj, g, sp, s);
+}
+void launch_moddown(const ModupJob* d, uint32_t nj, const NttGeom& g, uint32_t sp, hipStream_t s) {
+    launch_modup_t<true>(d, nj, g, sp, s);
 }
 
 }  // namespace gf

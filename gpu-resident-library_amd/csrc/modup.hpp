@@ -52,6 +52,28 @@ enum {
     MODUP_MAXSRC = 6
 };
 
+// per surviving tower constants of the fused ModDown + rescale prologue
+struct RsTarget {
+    uint64_t q, qneg, diff, new_larger, s2, s2p;
+    uint64_t red;   // floor(2^64 / q): Shoup reduction x mod q = shoup(x, 1, red, q)
+};
+// The rescale prologue's value for one surviving tower t and one position:
+// the centred switch of the dropped tower's value dv (hi: dv above q_old / 2)
+// to q_t, times s2 (lazy Shoup, below 2q), plus the ModDown conversion sum acc
+// after REDC without its correction (below 1.5q + 1 for at most 8 sources
+// below 2^60): below 4q, which the forward transform after it takes.
+GF_HD uint64_t rs_lazy_value(uint64_t dv, bool hi, const RsTarget& tg, const U128& acc) {
+    uint64_t sw;
+    if (tg.new_larger) {
+        sw = dv + (hi ? tg.diff : 0);
+    } else {
+        const uint64_t d = hi ? tg.diff : 0;
+        const uint64_t xm = shoup(dv, 1, tg.red, tg.q);   // dv mod q_t, exact for any dv < 2^64
+        sw = xm >= d ? xm - d : xm + tg.q - d;
+    }
+    return shoup_lazy(sw, tg.s2, tg.s2p, tg.q) + redc_lazy(acc, tg.q, tg.qneg);
+}
+
 struct ModupJob {
     const uint64_t* src[MODUP_MAXSRC];   // the inverse transforms' row-pass output, tower st + i
     const uint64_t* stw[MODUP_MAXSRC];   // their paired inverse twiddle tables
@@ -72,6 +94,18 @@ struct ModupJob {
     double sqd[MODUP_MAXSRC], sqinv[MODUP_MAXSRC], spost[MODUP_MAXSRC], spostq[MODUP_MAXSRC];
     const uint64_t* ttwd;
     const uint64_t* tqf;
+    // Rescale mode (the fused ModDown + rescale prologue of one component):
+    // the sources are the P towers (times n^-1 PHatInv), the targets the
+    // surviving Q towers.  Per position the dropped tower
+    //   dv = dt + (dr - REDC(sum_k x_k tabd[k strided])) pinv   (mod q_old)
+    // then per target rs_lazy_value(dv, dv > half, tg[t], sum_k x_k tab[k sc + t]),
+    // as the prologue kernel computes them (RsJob)
+    const uint64_t* dt;
+    const uint64_t* dr;
+    const uint64_t* tabd;
+    const RsTarget* tg;
+    uint64_t q_old, half, qd_neg, pinv, pinvp;
+    uint32_t strided;
 };
 
 // Column access in fp64: the inverse transforms' row-pass output (lazy, below
@@ -205,6 +239,55 @@ GF_HD void modup_convert_unit(const ModupJob& j, uint32_t t, uint64_t* lds, uint
         lds[modup_slot<S1>(unit, u)] = f64 ? bits_of_f64(f64_from_u52(v, c2)) : v;
     }
 }
+// Rescale mode: the dropped tower's value at this thread's 4 positions
+// (column s, rows k0 + u TL of the tile), from the sources already in
+// registers and dt, dr (coefficient form) read at those positions
+template <uint32_t SP, uint32_t S1>
+GF_HD void modup_drop_unit(const ModupJob& j, const NttGeom& g, uint32_t tile, uint32_t unit, ModupUnit<SP>& x,
+                           uint64_t* dv) {
+    constexpr uint32_t LOGC = MODUP_LOG_TILE - S1, C = 1u << LOGC, TL = 1u << (S1 - MODUP_RM);
+    const uint32_t s = unit & (C - 1), k0 = unit >> LOGC;
+    const uint64_t q = j.q_old;
+#pragma unroll
+    for (uint32_t u = 0; u < MODUP_UNIT; ++u) {
+        const uint32_t gi = ((k0 + u * TL) << g.logN2) + tile * C + s;
+        const U128 acc = dot30<SP>(x.lo[u], x.hi[u], [&](uint32_t k) { return uniform_at(j.tabd, k * j.strided); });
+        const uint64_t cd = redc(acc, q, j.qd_neg);
+        dv[u] = addmod(gat(gcast(j.dt), gi), shoup(submod(gat(gcast(j.dr), gi), cd, q), j.pinv, j.pinvp, q), q);
+    }
+}
+// Rescale mode: target t's prologue value of this thread's unit into LDS
+// (below 4 q_t; for an fp64 target as the centred double)
+template <uint32_t SP, uint32_t S1>
+GF_HD void modup_rs_unit(const ModupJob& j, uint32_t t, uint64_t* lds, uint32_t unit, ModupUnit<SP>& x,
+                         const uint64_t* dv) {
+    static_assert(sizeof(RsTarget) == 7 * 8, "RsTarget layout");
+    RsTarget tg;
+    const uint64_t* w = (const uint64_t*)j.tg + 7 * t;
+    tg.q = uniform_at(w, 0);
+    tg.qneg = uniform_at(w, 1);
+    tg.diff = uniform_at(w, 2);
+    tg.new_larger = uniform_at(w, 3);
+    tg.s2 = uniform_at(w, 4);
+    tg.s2p = uniform_at(w, 5);
+    tg.red = uniform_at(w, 6);
+    const bool f64 = uniform_at(j.ttwd, t) != 0;
+    const double qd = f64 ? f64_of_bits(uniform_at(j.tqf, 2 * t)) : 0.0;
+    const double qinv = f64 ? f64_of_bits(uniform_at(j.tqf, 2 * t + 1)) : 0.0;
+#pragma unroll
+    for (uint32_t u = 0; u < MODUP_UNIT; ++u) {
+#pragma unroll
+        for (uint32_t i = 0; i < SP; ++i) {
+            x.lo[u][i] = opaque32(x.lo[u][i]);
+            x.hi[u][i] = opaque32(x.hi[u][i]);
+        }
+        const U128 acc = dot30<SP>(x.lo[u], x.hi[u], [&](uint32_t s) { return uniform_at(j.tab, s * j.sc + t); });
+        const uint64_t v = rs_lazy_value(dv[u], dv[u] > j.half, tg, acc);
+        lds[modup_slot<S1>(unit, u)] =
+            f64 ? bits_of_f64(f64_center(f64_from_u52(v, F64_2P52 + 2 * qd), qd, qinv)) : v;
+    }
+}
+
 // target t's forward column pass from LDS, stored to HBM unreduced (the
 // digits' row pass continues from these values)
 template <uint32_t S1, class Sync>

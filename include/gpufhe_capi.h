@@ -292,7 +292,9 @@ int gf_ctx_autotune(gf_ctx* c, hipStream_t s, uint32_t* rmax_out, float* ms_out)
  * job), "modup_fused" (ModUp as one kernel per part and column tile, 0/1),
  * "modup_f64" (that kernel's column passes and the inverse row pass before
  * it in double precision for moduli below 2^50, 0/1), "ksi_f64" (with
- * modup_f64, the fused inner product's row passes as well, 0/1).
+ * modup_f64, the fused inner product's row passes as well, 0/1),
+ * "moddown_fused" (EvalMult+Rescale's P-tower inverse column pass, rescale
+ * prologue and surviving towers' forward column pass as one kernel, 0/1).
  * The report is JSON with every candidate's time, the radix search of
  * gf_ctx_autotune included; valid until the next tuning call.  Knobs (and
  * "ntt_radix", "rechunk_wgs", "modup_min_wgs") can be set by name; plans built

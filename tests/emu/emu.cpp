@@ -292,7 +292,149 @@ void emu_modup_t(const ModupJob& j, const NttGeom& g) {
         }
     }
 }
+template <uint32_t SP, uint32_t S1>
+void emu_moddown_t(const ModupJob& j, const NttGeom& g) {
+    std::vector<uint64_t> lds(MODUP_TILE);
+    std::vector<ModupUnit<SP>> units(MODUP_THREADS);
+    std::vector<uint64_t> dv((size_t)MODUP_THREADS * MODUP_UNIT);
+    for (uint32_t tile = 0; tile < g.n / MODUP_TILE; ++tile) {
+        for (uint32_t i = 0; i < SP; ++i) {
+            modup_source_pass<S1>(j, i, g, tile, lds.data(), 0, 1, NoSync{});
+            for (uint32_t u = 0; u < MODUP_THREADS; ++u) modup_take_unit<SP, S1>(j, i, lds.data(), u, units[u]);
+        }
+        for (uint32_t u = 0; u < MODUP_THREADS; ++u)
+            modup_drop_unit<SP, S1>(j, g, tile, u, units[u], dv.data() + (size_t)u * MODUP_UNIT);
+        for (uint32_t t = 0; t < j.sc; ++t) {
+            for (uint32_t u = 0; u < MODUP_THREADS; ++u)
+                modup_rs_unit<SP, S1>(j, t, lds.data(), u, units[u], dv.data() + (size_t)u * MODUP_UNIT);
+            modup_target_pass<S1>(j, t, g, tile, lds.data(), 0, 1, NoSync{});
+        }
+    }
+}
 }  // namespace
+
+// The fused ModDown + rescale prologue of one component (the fused ModUp
+// kernel's rescale mode): src[k] (k < K) the inverse row-pass output of P
+// tower k (modulus pm[k], inverse tables sw/swp, final factor post/postp);
+// the dropped tower q_old with its coefficient-form operands dt, dr, the
+// conversion column tabd[K] and P^-1 mod q_old pinv; surviving towers t < sc
+// with moduli tq[t], forward tables ttw/ttwp, constants tab[K][sc] and s2[t];
+// dst[t] receives target t's forward column-pass output.  ref != 0: the
+// unfused order (each P tower's inverse column pass, the prologue kernel's
+// arithmetic per position, each target's forward column pass).  f64 != 0
+// (fused only): targets below 2^50 take the fp64 passes.
+extern "C" void emu_moddown(uint32_t n, uint32_t K, uint32_t sc, const uint64_t* const* src, const uint64_t* pm,
+                            const uint64_t* const* sw, const uint64_t* const* swp, const uint64_t* post,
+                            const uint64_t* postp, const uint64_t* tq, const uint64_t* const* ttw,
+                            const uint64_t* const* ttwp, const uint64_t* tab, const uint64_t* tabd,
+                            const uint64_t* dt, const uint64_t* dr, uint64_t q_old, uint64_t pinv,
+                            const uint64_t* s2, uint64_t* dst, int ref, int f64) {
+    const NttGeom g = make_geom(n, 3);
+    std::vector<std::vector<uint64_t>> stab(K), ttab(sc), td(sc);
+    for (uint32_t k = 0; k < K; ++k) stab[k] = paired(sw[k], swp[k], n);
+    for (uint32_t t = 0; t < sc; ++t) ttab[t] = paired(ttw[t], ttwp[t], n);
+    std::vector<RsTarget> tg(sc);
+    std::vector<uint64_t> tqn(sc), tred(sc), ttw_addr(sc), ttwd(sc, 0), tqf(2 * (size_t)sc);
+    for (uint32_t t = 0; t < sc; ++t) {
+        const uint64_t q = tq[t];
+        tg[t] = RsTarget{q, host_qneg(q), q > q_old ? q - q_old : (q_old - q) % q, q > q_old ? 1u : 0u, s2[t],
+                         host_precon(s2[t], q), host_precon(1, q)};
+        tqn[t] = host_qneg(q);
+        tred[t] = lazy_red_of(q);
+        ttw_addr[t] = (uint64_t)(uintptr_t)ttab[t].data();
+        const double qd = (double)q, qi = 1.0 / qd;
+        std::memcpy(&tqf[2 * t], &qd, 8);
+        std::memcpy(&tqf[2 * t + 1], &qi, 8);
+        if (f64 && f64_modulus(q)) {
+            td[t] = f64_pairs_of(ttab[t].data(), n, q);
+            ttwd[t] = (uint64_t)(uintptr_t)td[t].data();
+        }
+    }
+    const uint64_t pinvp = host_precon(pinv, q_old), qd_neg = host_qneg(q_old), half = q_old >> 1;
+    if (!ref) {
+        ModupJob J{};
+        for (uint32_t k = 0; k < K; ++k) {
+            J.src[k] = src[k];
+            J.stw[k] = stab[k].data();
+            J.sq[k] = pm[k];
+            J.post[k] = post[k];
+            J.postp[k] = postp[k];
+        }
+        J.dst = dst;
+        J.tab = tab;
+        J.tq = tq;
+        J.tqneg = tqn.data();
+        J.ttw = ttw_addr.data();
+        J.tred = tred.data();
+        J.ttwd = ttwd.data();
+        J.tqf = tqf.data();
+        J.sp = K;
+        J.sc = sc;
+        J.dt = dt;
+        J.dr = dr;
+        J.tabd = tabd;
+        J.strided = 1;
+        J.tg = tg.data();
+        J.q_old = q_old;
+        J.half = half;
+        J.qd_neg = qd_neg;
+        J.pinv = pinv;
+        J.pinvp = pinvp;
+        with_split(g.logN1, g.logN2, [&](auto A, auto) {
+            constexpr uint32_t S1 = decltype(A)::value;
+            switch (K) {
+                case 1: emu_moddown_t<1, S1>(J, g); break;
+                case 2: emu_moddown_t<2, S1>(J, g); break;
+                case 3: emu_moddown_t<3, S1>(J, g); break;
+                case 4: emu_moddown_t<4, S1>(J, g); break;
+                case 5: emu_moddown_t<5, S1>(J, g); break;
+                case 6: emu_moddown_t<6, S1>(J, g); break;
+            }
+        });
+        return;
+    }
+    std::vector<uint64_t> x((size_t)K * n), lds(NTT_TILE_PADDED);
+    with_split(g.logN1, g.logN2, [&](auto A, auto) {
+        constexpr uint32_t S1 = decltype(A)::value;
+        const uint32_t tilesA = g.n / NTT_TILE;
+        for (uint32_t k = 0; k < K; ++k) {
+            std::copy(src[k], src[k] + n, x.begin() + (size_t)k * n);
+            NttJob jn{nullptr, x.data() + (size_t)k * n, stab[k].data(), nullptr, pm[k], post[k], postp[k]};
+            for (uint32_t tile = 0; tile < tilesA; ++tile)
+                phaseA_body<false, false, true, S1, 3>(jn, g, tile, lds.data(), 0, 1, NoSync{});
+        }
+        for (uint32_t i = 0; i < n; ++i) {
+            uint32_t lo[8], hi[8];
+            for (uint32_t k = 0; k < K; ++k) {
+                lo[k] = split30_lo(x[(size_t)k * n + i]);
+                hi[k] = split30_hi(x[(size_t)k * n + i]);
+            }
+            auto dot = [&](auto b) {
+                switch (K) {
+                    case 1: return dot30<1>(lo, hi, b);
+                    case 2: return dot30<2>(lo, hi, b);
+                    case 3: return dot30<3>(lo, hi, b);
+                    case 4: return dot30<4>(lo, hi, b);
+                    case 5: return dot30<5>(lo, hi, b);
+                    default: return dot30<6>(lo, hi, b);
+                }
+            };
+            const uint64_t cd = redc(dot([&](uint32_t k) { return tabd[k]; }), q_old, qd_neg);
+            const uint64_t dv = addmod(dt[i], shoup(submod(dr[i], cd, q_old), pinv, pinvp, q_old), q_old);
+            for (uint32_t t = 0; t < sc; ++t)
+                dst[(size_t)t * n + i] = rs_lazy_value(dv, dv > half, tg[t], dot([&](uint32_t k) { return tab[k * sc + t]; }));
+        }
+        for (uint32_t t = 0; t < sc; ++t) {
+            uint64_t* y = dst + (size_t)t * n;
+            NttJob jt{y, y, ttab[t].data(), nullptr, tq[t], 0, 0};
+            jt.red = tred[t];
+            for (uint32_t tile = 0; tile < tilesA; ++tile) {
+                if (jt.red) phaseA_body<true, true, true, S1, 3>(jt, g, tile, lds.data(), 0, 1, NoSync{});
+                else phaseA_body<true, false, true, S1, 3>(jt, g, tile, lds.data(), 0, 1, NoSync{});
+            }
+        }
+    });
+}
 
 // The fused ModUp over one part: src[i] (i < sp) holds the inverse row-pass
 // output of source i (modulus sq[i], tables tw/twp of sw[i]), post/postp its

@@ -13,6 +13,8 @@ N, L, K, ALPHA, NP, B = 1 << 16, 24, 6, 6, 4, 64
 T = 8 * N                                   # one tower
 FUSED_KSI = ["tensor", "ntt_inv", "conv", "ntt_fwd_a", "ksi", "ksi", "ntt_inv", "rsv", "ntt_fwd"]
 FUSED_MODUP = ["tensor", "ntt_inv_b", "ntt_inv", "modup", "ksi", "ksi", "ntt_inv", "rsv", "ntt_fwd"]
+FUSED_MODDOWN = ["tensor", "ntt_inv_b", "ntt_inv", "modup", "ksi", "ksi", "ntt_inv", "ntt_inv_b", "moddown",
+                 "ntt_fwd_b"]
 
 
 def test_fused_plan_steps():
@@ -74,4 +76,21 @@ def test_fused_modup_plan_steps():
     assert fused["intt_extra"] == 2 * T * 2
     # INTT column passes 24 T + conversion (24 + 96) T + digit column passes 96 T
     assert fused["modup_fused"] == T * (24 + 24 + 96 + 96)
+    assert sum(fused.values()) == sum(apart.values())
+
+
+def test_fused_moddown_plan_steps():
+    """With the fused ModDown the same algorithmic bytes are charged as the
+    stages it replaces: the P towers' inverse transforms (row pass apart,
+    column pass in the fused kernel), the prologue and the epilogue
+    transforms' column pass (row pass and its operands apart)."""
+    fused = {s[0]: s[1] for s in bench.step_bytes(N, L, K, ALPHA, NP, FUSED_MODDOWN, B)}
+    apart = {s[0]: s[1] for s in bench.step_bytes(N, L, K, ALPHA, NP, FUSED_MODUP, B)}
+    assert list(fused)[-4:] == ["intt_moddown_last", "intt_moddown_rowpass", "moddown_fused",
+                                "ntt_rescale_epilogue_rowpass"]
+    assert fused["intt_moddown_last"] == 2 * T * 2
+    assert fused["intt_moddown_rowpass"] == T * 2 * K
+    # P column passes 2K T + prologue 2 (K + 2 + 23) T + surviving towers' column passes 2 x 23 T
+    assert fused["moddown_fused"] == T * (2 * K + 2 * (K + 2 + 23) + 2 * 23)
+    assert fused["ntt_rescale_epilogue_rowpass"] == T * 2 * 23 * 3
     assert sum(fused.values()) == sum(apart.values())

@@ -52,6 +52,9 @@ def emu():
     lib.emu_reduce_red.restype = C.c_uint64
     lib.emu_reduce_red.argtypes = [C.c_uint64, C.c_uint64]
     pp = C.POINTER(u64p)
+    lib.emu_moddown.restype = None
+    lib.emu_moddown.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, pp, u64p, pp, pp, u64p, u64p, u64p, pp, pp,
+                                u64p, u64p, u64p, u64p, C.c_uint64, C.c_uint64, u64p, u64p, C.c_int, C.c_int]
     lib.emu_modup.restype = None
     lib.emu_modup.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, pp, u64p, pp, pp, u64p, u64p, u64p, pp, pp, u64p,
                               u64p, C.c_int, C.c_uint32, C.c_int]
@@ -373,3 +376,67 @@ def test_fused_modup_equals_unfused(emu, logn, L, dnum, part):
         else:
             assert np.array_equal(f[j], r[j]), j
     assert nf == sum(t < 1 << 50 for t in tgt)
+
+
+@pytest.mark.parametrize("logn,L,dnum,level", [(13, 6, 3, 6), (14, 5, 1, 3), (15, 12, 3, 12), (16, 24, 4, 24),
+                                               (16, 24, 4, 9), (17, 3, 1, 3)])
+def test_fused_moddown_equals_unfused(emu, logn, L, dnum, level):
+    """The fused ModDown + rescale prologue (the fused ModUp kernel's rescale
+    mode: the P towers' inverse column pass, the dropped tower, the prologue
+    value of every surviving tower and its forward column pass, in one
+    workgroup per column tile) stores what the unfused order stores: every
+    surviving tower bit for bit with the integer passes, and the same residues
+    in [0, 2q) where the towers below 2^50 take the fp64 passes; every ring
+    split, 1 to 6 P towers, the dropped tower larger and smaller than the
+    surviving ones."""
+    n = 1 << logn
+    P = make_params(n, L, dnum)
+    K = len(P.p)
+    if K > 6:
+        pytest.skip("more P towers than the fused kernel takes")
+    mods = [int(m) for m in P.mods]
+    roots = [int(r) for r in P.roots]
+    pm = [int(m) for m in P.p]
+    tgt = [int(m) for m in P.q[:level - 1]]
+    q_old = int(P.q[level - 1])
+    sc = len(tgt)
+    rng = np.random.default_rng(logn * 13 + level)
+    src, sw, swp, post, postp = [], [], [], [], []
+    for p in pm:
+        x = rng.integers(0, 4 * p - 1, n, dtype=np.uint64)
+        x[:3] = 4 * p - 1
+        src.append(np.ascontiguousarray(x))
+        _, _, ir, ip, _, _ = O.root_tables(n, p, roots[mods.index(p)])
+        sw.append(ir)
+        swp.append(ip)
+        v = int(rng.integers(1, p))
+        post.append(v)
+        postp.append((v << 64) // p)
+    tw, twp = [], []
+    for t in tgt:
+        fr, fp, _, _, _, _ = O.root_tables(n, t, roots[mods.index(t)])
+        tw.append(fr)
+        twp.append(fp)
+    tab = np.array([int(rng.integers(0, t)) for _ in range(K) for t in tgt], np.uint64)
+    tabd = np.array([int(rng.integers(0, q_old)) for _ in range(K)], np.uint64)
+    s2 = np.array([int(rng.integers(0, t)) for t in tgt], np.uint64)
+    dt = rng.integers(0, q_old, n, dtype=np.uint64)
+    dr = rng.integers(0, q_old, n, dtype=np.uint64)
+    dt[:2] = q_old - 1
+    pinv = int(rng.integers(1, q_old))
+    arr = lambda v: (u64p * len(v))(*[O.P(np.ascontiguousarray(a, dtype=np.uint64)) for a in v])  # noqa: E731
+    outs = {}
+    for ref, f64 in ((1, 0), (0, 0), (0, 1)):
+        dst = np.zeros(sc * n, np.uint64)
+        emu.emu_moddown(n, K, sc, arr(src), O.P(O.U(pm)), arr(sw), arr(swp), O.P(O.U(post)), O.P(O.U(postp)),
+                        O.P(O.U(tgt)), arr(tw), arr(twp), O.P(tab), O.P(tabd), O.P(dt), O.P(dr), q_old, pinv,
+                        O.P(s2), O.P(dst), ref, f64)
+        outs[(ref, f64)] = dst.reshape(sc, n)
+    r, a, f = outs[(1, 0)], outs[(0, 0)], outs[(0, 1)]
+    assert np.array_equal(a, r)
+    for j, t in enumerate(tgt):
+        if t < 1 << 50:
+            assert int(f[j].max()) < 2 * t, j
+            assert np.array_equal(f[j] % np.uint64(t), r[j] % np.uint64(t)), j
+        else:
+            assert np.array_equal(f[j], r[j]), j

@@ -338,14 +338,16 @@ def test_batch_plan_uneven_levels_and_graph(G):
         assert np.array_equal(outs[i][1].download(), want[i][1]), ("per job", i)
 
 
-@pytest.mark.parametrize("ksi_extra,conv_chunk,rs_chunk,modup_fused,modup_f64,ksi_f64",
-                         [(0, 8, 8, 1, 1, 1), (1, 16, 32, 1, 0, 0), (1, 32, 16, 0, 1, 1), (0, 5, 3, 0, 0, 0),
-                          (0, 8, 16, 1, 1, 0)])
-def test_plan_knobs_keep_results(G, ksi_extra, conv_chunk, rs_chunk, modup_fused, modup_f64, ksi_f64):
+@pytest.mark.parametrize("ksi_extra,conv_chunk,rs_chunk,modup_fused,modup_f64,ksi_f64,moddown_fused",
+                         [(0, 8, 8, 1, 1, 1, 1), (1, 16, 32, 1, 0, 0, 1), (1, 32, 16, 0, 1, 1, 1),
+                          (0, 5, 3, 0, 0, 0, 0), (0, 8, 16, 1, 1, 0, 0)])
+def test_plan_knobs_keep_results(G, ksi_extra, conv_chunk, rs_chunk, modup_fused, modup_f64, ksi_f64,
+                                 moddown_fused):
     """Every setting of the autotuner's plan knobs (extra inner-product wave,
     conversion and prologue chunk widths, re-chunking off so the widths take
     effect, the fused or the unfused ModUp, the fp64 passes of the 50-bit
-    towers around the fused ModUp and in the fused inner product) gives the
+    towers around the fused ModUp and in the fused inner product, the fused
+    or the unfused ModDown + rescale prologue) gives the
     oracle's results, on a batch of uneven levels (parts of 3 and 2 towers;
     ragged chunks with widths 5 and 3); the autotuner's own search leaves a
     report naming its choices."""
@@ -354,6 +356,7 @@ def test_plan_knobs_keep_results(G, ksi_extra, conv_chunk, rs_chunk, modup_fused
     assert any(int(q) < 1 << 50 for q in P.q)   # the fp64 passes have towers to take
     for name, v in (("ksi_extra", ksi_extra), ("conv_chunk", conv_chunk), ("rs_chunk", rs_chunk),
                     ("modup_fused", modup_fused), ("modup_f64", modup_f64), ("ksi_f64", ksi_f64),
+                    ("moddown_fused", moddown_fused),
                     ("modup_min_wgs", 0), ("rechunk_wgs", 0)):
         ctx.set_knob(name, v)
         assert ctx.knob(name) == v
@@ -380,6 +383,8 @@ def test_plan_knobs_keep_results(G, ksi_extra, conv_chunk, rs_chunk, modup_fused
         rep = ctx.tuning_report()
         for name in ("ntt_radix", "ksi_extra", "conv_chunk", "rs_chunk", "modup_fused", "modup_f64"):
             assert rep[name]["chosen"] == ctx.knob(name), name
+        if ctx.knob("modup_fused"):   # the fused ModDown is weighed where the fused ModUp runs
+            assert rep["moddown_fused"]["chosen"] == ctx.knob("moddown_fused")
         if ctx.knob("modup_f64"):
             assert rep["ksi_f64"]["chosen"] == ctx.knob("ksi_f64")
 

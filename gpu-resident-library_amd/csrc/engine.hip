@@ -172,7 +172,12 @@ Ctx::Ctx(uint32_t n_, const std::vector<uint64_t>& q, const std::vector<uint64_t
                 if (i < P.st || i >= P.st + P.sp) P.compl_rows.push_back(row_q(i));
             for (uint32_t k = 0; k < K; ++k) P.compl_rows.push_back(row_p(k));
             P.sc = (uint32_t)P.compl_rows.size();
-            std::vector<uint64_t> tab((size_t)P.sp * P.sc), tq(P.sc), tqn(P.sc);
+            std::vector<uint64_t> tab((size_t)P.sp * P.sc), tab60((size_t)P.sp * P.sc), tq(P.sc), tqn(P.sc);
+            unsigned __int128 ssum = 0;
+            for (uint32_t i = 0; i < P.sp; ++i) ssum += mods[P.st + i];
+            // the R = 2^60 column reduction keeps the targets below 2.2 q when
+            // the sources (reduced residues) sum to at most 1.2 2^60
+            P.c60 = (5 * ssum <= 6 * ((unsigned __int128)1 << 60) && P.sp <= 6 && qmax < (1ull << 60)) ? 1u : 0u;
             for (uint32_t i = 0; i < P.sp; ++i) {
                 const uint64_t qi = mods[P.st + i];
                 const uint64_t qhat = prod_mod(mods, P.st, P.st + P.sp, qi, P.st + i);
@@ -181,8 +186,9 @@ Ctx::Ctx(uint32_t n_, const std::vector<uint64_t>& q, const std::vector<uint64_t
                 P.postp.push_back(host_precon(v, qi));
                 for (uint32_t j = 0; j < P.sc; ++j) {
                     const uint64_t t = mods[P.compl_rows[j]];
-                    tab[(size_t)i * P.sc + j] =
-                        host_mulmod(prod_mod(mods, P.st, P.st + P.sp, t, P.st + i), host_r1(t), t);
+                    const uint64_t qh = prod_mod(mods, P.st, P.st + P.sp, t, P.st + i);
+                    tab[(size_t)i * P.sc + j] = host_mulmod(qh, host_r1(t), t);
+                    tab60[(size_t)i * P.sc + j] = host_mulmod(qh, (1ull << 60) % t, t);
                 }
             }
             for (uint32_t j = 0; j < P.sc; ++j) {
@@ -199,6 +205,7 @@ Ctx::Ctx(uint32_t n_, const std::vector<uint64_t>& q, const std::vector<uint64_t
                 std::memcpy(&tqf[2 * j + 1], &qi, 8);
             }
             put(tab, &P.d_tab);
+            put(tab60, &P.d_tab60);
             put(tq, &P.d_tq);
             put(tqn, &P.d_tqneg);
             put(ttw, &P.d_ttw);
@@ -1068,7 +1075,8 @@ void PlanBuilder::modup(uint32_t l, const uint64_t* d, uint64_t* ws, const uint6
             J.dst = dp;
             J.ttwd = C.modup_f64 ? P.d_ttwd : C.d_zeros;
             J.tqf = P.d_tqf;
-            J.tab = P.d_tab;
+            J.c60 = P.c60;
+            J.tab = P.c60 ? P.d_tab60 : P.d_tab;
             J.tq = P.d_tq;
             J.tqneg = P.d_tqneg;
             J.ttw = P.d_ttw;
@@ -1636,13 +1644,15 @@ std::unique_ptr<Plan> PlanBuilder::layout(std::vector<char>& blob) {
     GF_STAGE(PERM, perm_);
     GF_STAGE(NTT_IB, intt_upb_);
     GF_STAGE(NTT_I, intt_up_);
-    // fused ModUp jobs by source count (one launch each)
-    std::stable_sort(modup_.begin(), modup_.end(), [](const ModupJob& a, const ModupJob& b) { return a.sp < b.sp; });
+    // fused ModUp jobs by source count and conversion reduction (one launch each)
+    auto mkey = [](const ModupJob& a) { return a.sp | a.c60 << 8; };
+    std::stable_sort(modup_.begin(), modup_.end(),
+                     [&](const ModupJob& a, const ModupJob& b) { return mkey(a) < mkey(b); });
     for (size_t i = 0; i < modup_.size();) {
         size_t e = i;
-        while (e < modup_.size() && modup_[e].sp == modup_[i].sp) ++e;
+        while (e < modup_.size() && mkey(modup_[e]) == mkey(modup_[i])) ++e;
         stage(Step::MODUP, modup_.data() + i, e - i, sizeof(ModupJob));
-        P->steps.back().arg = modup_[i].sp;
+        P->steps.back().arg = mkey(modup_[i]);
         i = e;
     }
     GF_STAGE(CONV, conv_up_);

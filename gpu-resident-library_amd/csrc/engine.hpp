@@ -66,6 +66,8 @@ struct PartConst {
     std::vector<uint32_t> compl_rows;          // context rows of the complement towers
     std::vector<uint64_t> post, postp;         // n^-1 * QHatInv_i (INTT folding), per source
     const uint64_t* d_tab = nullptr;           // QHatModp * R mod target, [sp][sc]
+    const uint64_t* d_tab60 = nullptr;         // the same times 2^60 instead of R = 2^64
+    uint32_t c60 = 0;                          // 1: the sources sum to at most 1.2 2^60 (ModupJob::c60)
     const uint64_t* d_tq = nullptr;            // target moduli [sc]
     const uint64_t* d_tqneg = nullptr;
     const uint64_t* d_ttw = nullptr;           // [sc] addresses of the targets' paired forward tables

@@ -236,7 +236,7 @@ void launch_ptacc(const PtAccJob* d, uint32_t nj, uint32_t n, hipStream_t s);
 // job's transformed-part count), waves = workgroup size in waves (1..KSI_WAVES)
 void launch_ksi(const KsiJob* d, uint32_t nj, const NttGeom& g, uint32_t regions, uint32_t waves, hipStream_t s);
 // the fused ModUp over jobs of sp sources each
-void launch_modup(const ModupJob* d, uint32_t nj, const NttGeom& g, uint32_t sp, hipStream_t s);
+void launch_modup(const ModupJob* d, uint32_t nj, const NttGeom& g, uint32_t arg, hipStream_t s);
 void launch_moddown(const ModupJob* d, uint32_t nj, const NttGeom& g, uint32_t sp, hipStream_t s);
 
 inline SwitchJob make_switch(const uint64_t* in, uint64_t* out, uint64_t q_old, uint64_t q_new) {

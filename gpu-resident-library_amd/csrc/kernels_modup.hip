@@ -20,8 +20,9 @@ __device__ __forceinline__ void modup_sources(const ModupJob& j, const NttGeom& 
     }
 }
 
-// RS: the rescale mode (the fused ModDown + rescale prologue; ModupJob)
-template <uint32_t S1, uint32_t SP, bool RS>
+// RS: the rescale mode (the fused ModDown + rescale prologue; ModupJob);
+// C60: the conversion's R = 2^60 column reduction (ModupJob::c60)
+template <uint32_t S1, uint32_t SP, bool RS, bool C60>
 __global__ __launch_bounds__(MODUP_THREADS) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_modup(const ModupJob* __restrict__ jobs, uint32_t off, NttGeom g) {
     __shared__ uint64_t lds[MODUP_TILE];
     const ModupJob& j = jobs[off + blockIdx.y];
@@ -46,7 +47,7 @@ __global__ __launch_bounds__(MODUP_THREADS) __attribute__((amdgpu_waves_per_eu(4
         }
     } else {
         for (uint32_t t = 0; t < j.sc; ++t) {
-            modup_convert_unit<SP, S1>(j, t, lds, tid, x);
+            modup_convert_unit<SP, S1, C60>(j, t, lds, tid, x);
             __syncthreads();
             modup_target_pass<S1>(j, t, g, tile, lds, tid, MODUP_THREADS, DevSync{});
             __syncthreads();
@@ -54,8 +55,8 @@ __global__ __launch_bounds__(MODUP_THREADS) __attribute__((amdgpu_waves_per_eu(4
     }
 }
 
-// jobs of one launch share their source count sp
-template <bool RS>
+// jobs of one launch share their source count sp (and c60)
+template <bool RS, bool C60>
 static void launch_modup_t(const ModupJob* d, uint32_t nj, const NttGeom& g, uint32_t sp, hipStream_t s) {
     if (g.single || g.n % MODUP_TILE || sp < 1 || sp > MODUP_MAXSRC)
         throw std::runtime_error("gpufhe: fused ModUp / ModDown launch shape out of range");
@@ -65,7 +66,7 @@ static void launch_modup_t(const ModupJob* d, uint32_t nj, const NttGeom& g, uin
             const uint32_t cnt = nj - off < MAXY ? nj - off : MAXY;
             const dim3 grid(g.n / MODUP_TILE, cnt);
             switch (sp) {
-#define GF_SP(k) case k: k_modup<S1, k, RS><<<grid, MODUP_THREADS, 0, s>>>(d, off, g); break;
+#define GF_SP(k) case k: k_modup<S1, k, RS, C60><<<grid, MODUP_THREADS, 0, s>>>(d, off, g); break;
                 GF_SP(1) GF_SP(2) GF_SP(3) GF_SP(4) GF_SP(5) GF_SP(6)
 #undef GF_SP
             }
@@ -73,11 +74,13 @@ static void launch_modup_t(const ModupJob* d, uint32_t nj, const NttGeom& g, uin
     });
     if (!ok) throw std::runtime_error("gpufhe: unsupported ring dimension for the fused ModUp");
 }
-void launch_modup(const ModupJob* d, uint32_t nj, const NttGeom& g, uint32_t sp, hipStream_t s) {
-    launch_modup_t<false>(d, nj, g, sp, s);
+// arg: sp | c60 << 8
+void launch_modup(const ModupJob* d, uint32_t nj, const NttGeom& g, uint32_t arg, hipStream_t s) {
+    if (arg >> 8) launch_modup_t<false, true>(d, nj, g, arg & 0xff, s);
+    else launch_modup_t<false, false>(d, nj, g, arg & 0xff, s);
 }
 void launch_moddown(const ModupJob* d, uint32_t nj, const NttGeom& g, uint32_t sp, hipStream_t s) {
-    launch_modup_t<true>(d, nj, g, sp, s);
+    launch_modup_t<true, false>(d, nj, g, sp, s);
 }
 
 }  // namespace gf

@@ -170,6 +170,45 @@ GF_HD U128 dot30(const uint32_t* alo, const uint32_t* ahi, B b) {
     return T;
 }
 
+// The column sums of dot30 themselves (lo + mid 2^30 + hi 2^60), for a
+// reduction that takes them as they are (redc60_lazy)
+struct Cols30 {
+    uint64_t lo, mid, hi;
+};
+template <uint32_t NS, class B>
+GF_HD Cols30 dot30_cols(const uint32_t* alo, const uint32_t* ahi, B b) {
+    static_assert(NS >= 1 && NS <= 8, "column sums are exact for at most 8 terms");
+    Cols30 c{0, 0, 0};
+#pragma unroll
+    for (uint32_t s = 0; s < NS; ++s) {
+        const uint64_t v = b(s);
+        const uint32_t b0 = split30_lo(v), b1 = split30_hi(v);
+        c.lo = mad32(alo[s], b0, c.lo);
+        c.mid = mad32(alo[s], b1, c.mid);
+        c.mid = mad32(ahi[s], b0, c.mid);
+        c.hi = mad32(ahi[s], b1, c.hi);
+    }
+    return c;
+}
+// Montgomery reduction with R = 2^60 straight from the column sums
+// T = lo + mid 2^30 + hi 2^60 of dot30_cols over at most 6 terms (two
+// values below 2^60 each): two steps of one 30-bit digit each,
+// m = (low 30 bits) (-q^-1 mod 2^30) mod 2^30, adding m q split at bit 30
+// into the two columns it touches and carrying the (now zero) low 30 bits
+// away.  Returns (T + M q) / 2^60 for some M < 2^60: congruent to T 2^-60,
+// below T / 2^60 + q.  Needs q < 2^60 (qneg = -q^-1 mod 2^64).  With NS
+// terms the columns stay below (NS + 1) 2^60 (lo) and (2 NS + 2) 2^60 + 2^35
+// (mid after both steps), so below 2^64 for NS <= 6.
+GF_HD uint64_t redc60_lazy(Cols30 c, uint64_t q, uint64_t qneg) {
+    const uint32_t qi = (uint32_t)qneg & SPLIT30_MASK, q0 = (uint32_t)q & SPLIT30_MASK, q1 = (uint32_t)(q >> 30);
+    const uint32_t m1 = ((uint32_t)c.lo * qi) & SPLIT30_MASK;
+    const uint64_t lo = mad32(m1, q0, c.lo);
+    const uint64_t mid = mad32(m1, q1, c.mid) + (lo >> 30);
+    const uint32_t m2 = ((uint32_t)mid * qi) & SPLIT30_MASK;
+    const uint64_t mid2 = mad32(m2, q0, mid);
+    return mad32(m2, q1, c.hi) + (mid2 >> 30);
+}
+
 // Montgomery reduction, R = 2^64, qneg = -q^{-1} mod 2^64, without the final
 // correction: T R^{-1} mod q plus a multiple of q, below T / 2^64 + q + 1
 // (any T with T / 2^64 + q + 1 <= 2^64).

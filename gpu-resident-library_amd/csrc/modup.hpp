@@ -26,6 +26,8 @@
 // and everything from the sources' reduced values to the conversion sums is
 // identical.
 #pragma once
+#include <type_traits>
+
 #include "ntt_engine.hpp"
 #include "ntt_f64.hpp"
 
@@ -94,6 +96,12 @@ struct ModupJob {
     double sqd[MODUP_MAXSRC], sqinv[MODUP_MAXSRC], spost[MODUP_MAXSRC], spostq[MODUP_MAXSRC];
     const uint64_t* ttwd;
     const uint64_t* tqf;
+    // 1: tab holds the constants times 2^60 (not 2^64) and the conversion
+    // reduces its column sums with redc60_lazy; the host sets it where the
+    // part's source moduli sum to at most 1.2 2^60, so the targets stay
+    // below 2.2 q_t (the forward passes take below 4q; the fp64 ones below
+    // 1.25q after the shift by q).  Jobs of one launch share it.
+    uint32_t c60;
     // Rescale mode (the fused ModDown + rescale prologue of one component):
     // the sources are the P towers (times n^-1 PHatInv), the targets the
     // surviving Q towers.  Per position the dropped tower
@@ -134,44 +142,11 @@ struct ModupUnit {
     uint32_t lo[MODUP_UNIT][SP], hi[MODUP_UNIT][SP];
 };
 
-// LDS word of tile slot `at` (column s, row k at k C + s): bits 4..6 xor-ed
-// into bits 2..4.  In the last forward pass (first inverse pass) a unit is 4
-// consecutive rows of one column, and the 16 lanes that share a bank cycle
-// would otherwise meet on 4 bank pairs at C = 4 (n = 2^16); with the swizzle
-// every pass of that tile is free of bank conflicts for 16- and 32-lane bank
-// cycles alike, and the worst case of the other ring sizes halves.  A
-// bijection on [0, 1024).
-GF_HD uint32_t modup_swz(uint32_t at) { return at ^ (((at >> 4) & 7) << 2); }
-struct ModupLdsIn {
-    const uint64_t* a;
-    GF_HD uint64_t operator()(uint32_t, uint32_t, uint32_t at) const { return a[modup_swz(at)]; }
-};
-struct ModupLdsOut {
-    uint64_t* a;
-    GF_HD void operator()(uint32_t, uint32_t, uint32_t at, uint64_t v) const { a[modup_swz(at)] = v; }
-};
-struct ModupLio {
-    GF_HD ModupLdsIn in(const uint64_t* a) const { return ModupLdsIn{a}; }
-    GF_HD ModupLdsOut out(uint64_t* a) const { return ModupLdsOut{a}; }
-};
-struct ModupLdsInF64 {
-    const uint64_t* a;
-    GF_HD double operator()(uint32_t, uint32_t, uint32_t at) const { return f64_of_bits(a[modup_swz(at)]); }
-};
-struct ModupLdsOutF64 {
-    uint64_t* a;
-    GF_HD void operator()(uint32_t, uint32_t, uint32_t at, double v) const { a[modup_swz(at)] = bits_of_f64(v); }
-};
-struct ModupLioF64 {
-    GF_HD ModupLdsInF64 in(const uint64_t* a) const { return ModupLdsInF64{a}; }
-    GF_HD ModupLdsOutF64 out(uint64_t* a) const { return ModupLdsOutF64{a}; }
-};
-
 template <uint32_t S1>
 GF_HD uint32_t modup_slot(uint32_t unit, uint32_t u) {
     static_assert(chunk_size(S1, MODUP_RM, 0) == MODUP_RM, "chunk 0 must be a full radix-4 pass");
     constexpr uint32_t LOGC = MODUP_LOG_TILE - S1, C = 1u << LOGC, TL = 1u << (S1 - MODUP_RM);
-    return modup_swz((unit & (C - 1)) + (((unit >> LOGC) + u * TL) << LOGC));
+    return (unit & (C - 1)) + (((unit >> LOGC) + u * TL) << LOGC);
 }
 
 // source i's inverse column pass over the tile (all threads), its last pass
@@ -185,13 +160,12 @@ GF_HD void modup_source_pass(const ModupJob& j, uint32_t i, const NttGeom& g, ui
         const double q = j.sqd[i], qinv = j.sqinv[i];
         const ColInF64 gin{gcast(j.src[i]), g.logN2, tile * C, q, qinv, F64_2P52 + 2 * q};
         const TwF64 tw{gcast((const double*)j.stwd[i])};
-        tile_chunks_f64<false, S1, MODUP_RM, 0>(lds, d, tw, tw, q, qinv, tid, nthr, sync, gin, ModupLdsOutF64{lds},
-                                                ModupLioF64{});
+        tile_chunks_f64<false, S1, MODUP_RM, 0>(lds, d, tw, tw, q, qinv, tid, nthr, sync, gin, LdsOutF64{lds});
         return;
     }
     const ColIn gin{gcast(j.src[i]), g.logN2, tile * C};
     tile_chunks<false, false, S1, MODUP_RM, 0>(lds, d, TwPaired{gcast(j.stw[i])}, TwPaired{gcast(j.stw[i])}, j.sq[i],
-                                               tid, nthr, sync, gin, ModupLdsOut{lds}, ModupLio{});
+                                               tid, nthr, sync, gin, LdsOut{lds});
 }
 template <uint32_t SP, uint32_t S1>
 GF_HD void modup_take_unit(const ModupJob& j, uint32_t i, const uint64_t* lds, uint32_t unit, ModupUnit<SP>& x) {
@@ -218,13 +192,15 @@ GF_HD void modup_take_unit(const ModupJob& j, uint32_t i, const uint64_t* lds, u
 }
 // target t's conversion of this thread's unit into LDS: the exact column sums
 // over the part's sources, REDC without its correction ([0, 2 q_t)), as the
-// conversion kernel leaves them for the forward transforms; for an fp64
-// target as the double v - q_t (|.| <= q_t)
-template <uint32_t SP, uint32_t S1>
+// conversion kernel leaves them for the forward transforms (with c60 the
+// R = 2^60 reduction of the columns, below 2.2 q_t); for an fp64 target as
+// the double v - q_t (|.| <= 1.25 q_t)
+template <uint32_t SP, uint32_t S1, bool C60>
 GF_HD void modup_convert_unit(const ModupJob& j, uint32_t t, uint64_t* lds, uint32_t unit, ModupUnit<SP>& x) {
     const uint64_t q = uniform_at(j.tq, t), qn = uniform_at(j.tqneg, t);
     const bool f64 = uniform_at(j.ttwd, t) != 0;
     const double c2 = f64 ? F64_2P52 + f64_of_bits(uniform_at(j.tqf, 2 * t)) : 0.0;
+    static_assert(!C60 || SP <= 6, "redc60_lazy takes at most 6 terms");
 #pragma unroll
     for (uint32_t u = 0; u < MODUP_UNIT; ++u) {
 #pragma unroll
@@ -234,8 +210,10 @@ GF_HD void modup_convert_unit(const ModupJob& j, uint32_t t, uint64_t* lds, uint
             x.lo[u][i] = opaque32(x.lo[u][i]);
             x.hi[u][i] = opaque32(x.hi[u][i]);
         }
-        const U128 acc = dot30<SP>(x.lo[u], x.hi[u], [&](uint32_t s) { return uniform_at(j.tab, s * j.sc + t); });
-        const uint64_t v = redc_lazy(acc, q, qn);
+        auto cst = [&](uint32_t s) { return uniform_at(j.tab, s * j.sc + t); };
+        uint64_t v;
+        if constexpr (C60) v = redc60_lazy(dot30_cols<SP>(x.lo[u], x.hi[u], cst), q, qn);
+        else v = redc_lazy(dot30<SP>(x.lo[u], x.hi[u], cst), q, qn);
         lds[modup_slot<S1>(unit, u)] = f64 ? bits_of_f64(f64_from_u52(v, c2)) : v;
     }
 }
@@ -300,8 +278,7 @@ GF_HD void modup_target_pass(const ModupJob& j, uint32_t t, const NttGeom& g, ui
         const double q = f64_of_bits(uniform_at(j.tqf, 2 * t)), qinv = f64_of_bits(uniform_at(j.tqf, 2 * t + 1));
         const ColOutF64 gout{gcast(j.dst + (size_t)t * g.n), g.logN2, tile * C, q, qinv};
         const TwF64 tw{gcast((const double*)(uintptr_t)twd)};
-        tile_chunks_f64<true, S1, MODUP_RM, 0>(lds, d, tw, tw, q, qinv, tid, nthr, sync, ModupLdsInF64{lds}, gout,
-                                               ModupLioF64{});
+        tile_chunks_f64<true, S1, MODUP_RM, 0>(lds, d, tw, tw, q, qinv, tid, nthr, sync, LdsInF64{lds}, gout);
         return;
     }
     NttJob jt{};
@@ -309,11 +286,9 @@ GF_HD void modup_target_pass(const ModupJob& j, uint32_t t, const NttGeom& g, ui
     const ColOut<true> gout{gcast(j.dst + (size_t)t * g.n), g.logN2, tile * C, jt};
     const TwPaired tw{gcast((const uint64_t*)(uintptr_t)uniform_at(j.ttw, t))};
     if (uniform_at(j.tred, t))
-        tile_chunks<true, true, S1, MODUP_RM, 0>(lds, d, tw, tw, jt.q, tid, nthr, sync, ModupLdsIn{lds}, gout,
-                                                 ModupLio{});
+        tile_chunks<true, true, S1, MODUP_RM, 0>(lds, d, tw, tw, jt.q, tid, nthr, sync, LdsIn{lds}, gout);
     else
-        tile_chunks<true, false, S1, MODUP_RM, 0>(lds, d, tw, tw, jt.q, tid, nthr, sync, ModupLdsIn{lds}, gout,
-                                                  ModupLio{});
+        tile_chunks<true, false, S1, MODUP_RM, 0>(lds, d, tw, tw, jt.q, tid, nthr, sync, LdsIn{lds}, gout);
 }
 
 }  // namespace gf

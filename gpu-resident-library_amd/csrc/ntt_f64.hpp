@@ -201,14 +201,15 @@ GF_HD void tile_pass_f64(const TileDesc& d, const TW& tw, double q, double qinv,
 // chunks centres its outputs.  The last pass never does (its consumer
 // reduces).  Inverse, inputs centred: every pass (sums double per stage, and
 // with the centring after every second stage inside a pass its values stay
-// below 3 q).  Forward, inputs |x| <= q: a pass of R stages adds 1.5 R q, and
-// a pass centres when the next one would take the bound past 7.5 q.
+// below 3 q).  Forward, inputs |x| <= 1.25 q: a pass of R stages adds
+// 1.5 R q, and a pass centres when the next one would take the bound past
+// 7.5 q.
 template <bool FWD, uint32_t S, uint32_t RM>
 GF_HD constexpr bool f64_center_after(uint32_t I) {
     const uint32_t NC = num_chunks(S, RM);
     if (I + 1 >= NC) return false;
     if (!FWD) return true;
-    double b = 1.0;
+    double b = 1.25;
     for (uint32_t c = 0; c + 1 < NC; ++c) {
         b += 1.5 * chunk_size(S, RM, c);
         const bool ctr = b + 1.5 * chunk_size(S, RM, c + 1) > 7.5;
@@ -221,7 +222,7 @@ GF_HD constexpr bool f64_center_after(uint32_t I) {
 template <uint32_t S, uint32_t RM>
 GF_HD constexpr double f64_fwd_out_bound() {
     const uint32_t NC = num_chunks(S, RM);
-    double b = 1.0;
+    double b = 1.25;
     for (uint32_t c = 0; c < NC; ++c) {
         b += 1.5 * chunk_size(S, RM, c);
         if (c + 1 < NC && b + 1.5 * chunk_size(S, RM, c + 1) > 7.5) b = 0.51;

@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "ksi.hpp"
@@ -277,7 +278,7 @@ std::vector<uint64_t> paired(const uint64_t* tw, const uint64_t* twp, uint32_t n
     return t2;
 }
 
-template <uint32_t SP, uint32_t S1>
+template <uint32_t SP, uint32_t S1, bool C60>
 void emu_modup_t(const ModupJob& j, const NttGeom& g) {
     std::vector<uint64_t> lds(MODUP_TILE);
     std::vector<ModupUnit<SP>> units(MODUP_THREADS);
@@ -287,7 +288,8 @@ void emu_modup_t(const ModupJob& j, const NttGeom& g) {
             for (uint32_t u = 0; u < MODUP_THREADS; ++u) modup_take_unit<SP, S1>(j, i, lds.data(), u, units[u]);
         }
         for (uint32_t t = 0; t < j.sc; ++t) {
-            for (uint32_t u = 0; u < MODUP_THREADS; ++u) modup_convert_unit<SP, S1>(j, t, lds.data(), u, units[u]);
+            for (uint32_t u = 0; u < MODUP_THREADS; ++u)
+                modup_convert_unit<SP, S1, C60>(j, t, lds.data(), u, units[u]);
             modup_target_pass<S1>(j, t, g, tile, lds.data(), 0, 1, NoSync{});
         }
     }
@@ -444,11 +446,13 @@ extern "C" void emu_moddown(uint32_t n, uint32_t K, uint32_t sc, const uint64_t*
 // inverse column pass (standard tiles, radix rmax), the conversion with the
 // conversion kernel's arithmetic, each target's forward column pass.
 // f64 != 0 (fused only): sources and targets below 2^50 take the fp64 passes.
+// c60 != 0 (fused only): the conversion's R = 2^60 column reduction, with the
+// constants rescaled from tab (times 2^-4).
 extern "C" void emu_modup(uint32_t n, uint32_t sp, uint32_t sc, const uint64_t* const* src, const uint64_t* sq,
                           const uint64_t* const* sw, const uint64_t* const* swp, const uint64_t* post,
                           const uint64_t* postp, const uint64_t* tq, const uint64_t* const* ttw,
                           const uint64_t* const* ttwp, const uint64_t* tab, uint64_t* dst, int ref, uint32_t rmax,
-                          int f64) {
+                          int f64, int c60) {
     const NttGeom g = make_geom(n, rmax);
     std::vector<std::vector<uint64_t>> stab(sp), ttab(sc);
     for (uint32_t i = 0; i < sp; ++i) stab[i] = paired(sw[i], swp[i], n);
@@ -497,8 +501,13 @@ extern "C" void emu_modup(uint32_t n, uint32_t sp, uint32_t sc, const uint64_t* 
                 ttwd[t] = (uint64_t)(uintptr_t)td[t].data();
             }
         }
+        std::vector<uint64_t> tab60((size_t)sp * sc);
+        for (uint32_t i = 0; i < sp; ++i)
+            for (uint32_t t = 0; t < sc; ++t)
+                tab60[(size_t)i * sc + t] = host_mulmod(tab[(size_t)i * sc + t], host_invmod(16, tq[t]), tq[t]);
         J.dst = dst;
-        J.tab = tab;
+        J.tab = c60 ? tab60.data() : tab;
+        J.c60 = c60 ? 1u : 0u;
         J.tq = tq;
         J.tqneg = tqn.data();
         J.ttw = ttw_addr.data();
@@ -507,17 +516,22 @@ extern "C" void emu_modup(uint32_t n, uint32_t sp, uint32_t sc, const uint64_t* 
         J.tqf = tqf.data();
         J.sp = sp;
         J.sc = sc;
-        with_split(g.logN1, g.logN2, [&](auto A, auto) {
-            constexpr uint32_t S1 = decltype(A)::value;
-            switch (sp) {
-                case 1: emu_modup_t<1, S1>(J, g); break;
-                case 2: emu_modup_t<2, S1>(J, g); break;
-                case 3: emu_modup_t<3, S1>(J, g); break;
-                case 4: emu_modup_t<4, S1>(J, g); break;
-                case 5: emu_modup_t<5, S1>(J, g); break;
-                case 6: emu_modup_t<6, S1>(J, g); break;
-            }
-        });
+        auto go = [&](auto C60) {
+            constexpr bool CC = decltype(C60)::value;
+            with_split(g.logN1, g.logN2, [&](auto A, auto) {
+                constexpr uint32_t S1 = decltype(A)::value;
+                switch (sp) {
+                    case 1: emu_modup_t<1, S1, CC>(J, g); break;
+                    case 2: emu_modup_t<2, S1, CC>(J, g); break;
+                    case 3: emu_modup_t<3, S1, CC>(J, g); break;
+                    case 4: emu_modup_t<4, S1, CC>(J, g); break;
+                    case 5: emu_modup_t<5, S1, CC>(J, g); break;
+                    case 6: emu_modup_t<6, S1, CC>(J, g); break;
+                }
+            });
+        };
+        if (c60) go(std::true_type{});
+        else go(std::false_type{});
         return;
     }
     std::vector<uint64_t> x((size_t)sp * n), lds(NTT_TILE_PADDED);

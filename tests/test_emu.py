@@ -57,7 +57,7 @@ def emu():
                                 u64p, u64p, u64p, u64p, C.c_uint64, C.c_uint64, u64p, u64p, C.c_int, C.c_int]
     lib.emu_modup.restype = None
     lib.emu_modup.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, pp, u64p, pp, pp, u64p, u64p, u64p, pp, pp, u64p,
-                              u64p, C.c_int, C.c_uint32, C.c_int]
+                              u64p, C.c_int, C.c_uint32, C.c_int, C.c_int]
     yield lib
     shutil.rmtree(d, ignore_errors=True)
 
@@ -313,7 +313,10 @@ def test_fused_modup_equals_unfused(emu, logn, L, dnum, part):
     (reduced butterflies) targets, inputs up to 4q - 1.  With the fp64 passes
     for moduli below 2^50 the 60-bit targets are still stored bit for bit
     (their sources' reduced values and the conversion sums are identical),
-    and the fp64 targets hold the same residues in [0, 2q)."""
+    and the fp64 targets hold the same residues in [0, 2q).  With the
+    conversion's R = 2^60 column reduction (where the sources sum to at most
+    1.2 2^60) every target holds the same residues, within the bounds the
+    next passes take."""
     n = 1 << logn
     P = make_params(n, L, dnum)
     alpha = -(-L // dnum)
@@ -357,16 +360,19 @@ def test_fused_modup_equals_unfused(emu, logn, L, dnum, part):
     arr = lambda v: (u64p * len(v))(*[O.P(np.ascontiguousarray(a, dtype=np.uint64)) for a in v])  # noqa: E731
     keep = [src, sw, swp, tw, twp]
     outs = {}
-    for ref, rmax, f64 in ((0, 3, 0), (1, 3, 0), (1, 4, 0), (0, 3, 1)):
+    c60_ok = 5 * sum(srcm) <= 6 << 60
+    for ref, rmax, f64, c60 in ((0, 3, 0, 0), (1, 3, 0, 0), (1, 4, 0, 0), (0, 3, 1, 0), (0, 3, 1, 1), (0, 3, 0, 1)):
+        if c60 and not c60_ok:
+            continue
         dst = np.zeros(sc * n, np.uint64)
         emu.emu_modup(n, sp, sc, arr(src), O.P(O.U(srcm)), arr(sw), arr(swp), O.P(O.U(post)), O.P(O.U(postp)),
-                      O.P(O.U(tgt)), arr(tw), arr(twp), O.P(tab), O.P(dst), ref, rmax, f64)
-        outs[(ref, rmax, f64)] = dst
+                      O.P(O.U(tgt)), arr(tw), arr(twp), O.P(tab), O.P(dst), ref, rmax, f64, c60)
+        outs[(ref, rmax, f64, c60)] = dst
     assert keep
-    assert np.array_equal(outs[(0, 3, 0)], outs[(1, 3, 0)])
-    assert np.array_equal(outs[(0, 3, 0)], outs[(1, 4, 0)])
-    f = outs[(0, 3, 1)].reshape(sc, n)
-    r = outs[(1, 3, 0)].reshape(sc, n)
+    assert np.array_equal(outs[(0, 3, 0, 0)], outs[(1, 3, 0, 0)])
+    assert np.array_equal(outs[(0, 3, 0, 0)], outs[(1, 4, 0, 0)])
+    f = outs[(0, 3, 1, 0)].reshape(sc, n)
+    r = outs[(1, 3, 0, 0)].reshape(sc, n)
     nf = 0
     for j, t in enumerate(tgt):
         if t < 1 << 50:
@@ -376,6 +382,15 @@ def test_fused_modup_equals_unfused(emu, logn, L, dnum, part):
         else:
             assert np.array_equal(f[j], r[j]), j
     assert nf == sum(t < 1 << 50 for t in tgt)
+    if c60_ok:
+        for key in ((0, 3, 1, 1), (0, 3, 0, 1)):
+            c = outs[key].reshape(sc, n)
+            for j, t in enumerate(tgt):
+                assert np.array_equal(c[j] % np.uint64(t), r[j] % np.uint64(t)), (key, j)
+                if key[2] and t < 1 << 50:
+                    assert int(c[j].max()) < 2 * t, (key, j)
+                elif t < 1 << 57:   # unreduced forward column pass from inputs below 2.2 t
+                    assert int(c[j].max()) < 40 * t, (key, j)
 
 
 @pytest.mark.parametrize("logn,L,dnum,level", [(13, 6, 3, 6), (14, 5, 1, 3), (15, 12, 3, 12), (16, 24, 4, 24),

@@ -7,15 +7,15 @@ namespace gf {
 // One workgroup per (part job, column tile).  Source count SP fixed at compile
 // time, so every thread's unit of the sources stays in registers.
 // sources I .. SP - 1, one after the other (compile-time source index, so the
-// unit arrays are indexed by constants and stay in registers)
+// unit arrays are indexed by constants and stay in registers); source I is
+// phase I and takes LDS tile I mod 2
 template <uint32_t S1, uint32_t SP, uint32_t I>
 __device__ __forceinline__ void modup_sources(const ModupJob& j, const NttGeom& g, uint64_t* lds, uint32_t tid,
                                               uint32_t tile, ModupUnit<SP>& x) {
     if constexpr (I < SP) {
-        modup_source_pass<S1>(j, I, g, tile, lds, tid, MODUP_THREADS, DevSync{});
-        __syncthreads();
-        modup_take_unit<SP, S1>(j, I, lds, tid, x);
-        __syncthreads();
+        uint64_t v[MODUP_UNIT];
+        modup_source_pass<S1, false>(j, I, g, tile, lds + (I & 1) * MODUP_TILE, tid, MODUP_THREADS, DevSync{}, v);
+        modup_take_unit<SP, S1>(j, I, v, x);
         modup_sources<S1, SP, I + 1>(j, g, lds, tid, tile, x);
     }
 }
@@ -24,7 +24,7 @@ __device__ __forceinline__ void modup_sources(const ModupJob& j, const NttGeom& 
 // C60: the conversion's R = 2^60 column reduction (ModupJob::c60)
 template <uint32_t S1, uint32_t SP, bool RS, bool C60>
 __global__ __launch_bounds__(MODUP_THREADS) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_modup(const ModupJob* __restrict__ jobs, uint32_t off, NttGeom g) {
-    __shared__ uint64_t lds[MODUP_TILE];
+    __shared__ uint64_t lds[2 * MODUP_TILE];   // phases alternate between the two tiles
     const ModupJob& j = jobs[off + blockIdx.y];
     const uint32_t tid = threadIdx.x;
     // Workgroups go to the 8 XCDs in turn (linear id mod 8, and gridDim.x is
@@ -36,21 +36,21 @@ __global__ __launch_bounds__(MODUP_THREADS) __attribute__((amdgpu_waves_per_eu(4
     const uint32_t tile = nt % 8 ? blockIdx.x : (blockIdx.x % 8) * (nt / 8) + blockIdx.x / 8;
     ModupUnit<SP> x;
     modup_sources<S1, SP, 0>(j, g, lds, tid, tile, x);
+    // target t is phase SP + t
+    uint64_t cv[MODUP_UNIT];
     if constexpr (RS) {
         uint64_t dv[MODUP_UNIT];
         modup_drop_unit<SP, S1>(j, g, tile, tid, x, dv);
         for (uint32_t t = 0; t < j.sc; ++t) {
-            modup_rs_unit<SP, S1>(j, t, lds, tid, x, dv);
-            __syncthreads();
-            modup_target_pass<S1>(j, t, g, tile, lds, tid, MODUP_THREADS, DevSync{});
-            __syncthreads();
+            modup_rs_unit<SP, S1>(j, t, x, dv, cv);
+            modup_target_pass<S1, false>(j, t, g, tile, lds + ((SP + t) & 1) * MODUP_TILE, tid, MODUP_THREADS,
+                                         DevSync{}, cv);
         }
     } else {
         for (uint32_t t = 0; t < j.sc; ++t) {
-            modup_convert_unit<SP, S1, C60>(j, t, lds, tid, x);
-            __syncthreads();
-            modup_target_pass<S1>(j, t, g, tile, lds, tid, MODUP_THREADS, DevSync{});
-            __syncthreads();
+            modup_convert_unit<SP, S1, C60>(j, t, x, cv);
+            modup_target_pass<S1, false>(j, t, g, tile, lds + ((SP + t) & 1) * MODUP_TILE, tid, MODUP_THREADS,
+                                         DevSync{}, cv);
         }
     }
 }

@@ -8,14 +8,20 @@
 // N1 = 2^S1 rows.  With radix-4 register passes a column transform's first
 // pass (chunk 0, stages 0..1) has exactly one unit per thread (256 threads):
 // column s = unit & (C - 1), rows l + u TL (l = unit >> log C,
-// TL = 2^(S1 - 2), u < 4), LDS slot s + (l + u TL) C.  The inverse
-// transforms' last pass is the same chunk, so after it each thread keeps its
-// unit of every source in registers (reduced, times n^-1 QHatInv, split into
-// the 30-bit halves the conversion sums take: 8 VGPRs per source); for each
-// target it forms the conversion of its
-// 4 elements, stores them at their slots, and the target's forward column
-// pass runs from LDS to HBM.  The coefficient-form sources and digits never
-// travel through HBM.  Every butterfly, product and reduction is the one the
+// TL = 2^(S1 - 2), u < 4).  The inverse transforms' last pass is the same
+// chunk, so it leaves each thread's unit of a source in its registers, where
+// the thread keeps it for the whole tile (reduced, times n^-1 QHatInv, split
+// into the 30-bit halves the conversion sums take: 8 VGPRs per source); for
+// each target it forms the conversion of its 4 elements and runs the target's
+// first forward pass on them in registers, the later passes through LDS, the
+// last to HBM.  The coefficient-form sources and digits never travel through
+// HBM, and no pass of a source or target needs LDS at its register end.
+//
+// Each source and target (a phase) takes the next of two LDS tiles in turn:
+// a phase's first LDS write can only overlap the previous phase's reads of
+// the other tile, and every phase of two or more passes has a barrier that
+// all threads reach only after finishing the phase before it, so the
+// phases need no barriers between them.  Every butterfly, product and reduction is the one the
 // unfused kernels apply, so the stored digit column
 // passes equal theirs bit for bit.
 //
@@ -142,39 +148,72 @@ struct ModupUnit {
     uint32_t lo[MODUP_UNIT][SP], hi[MODUP_UNIT][SP];
 };
 
+// A chunk-0 unit in registers: element u at v[u] (a word, or an fp64 pass's
+// bit pattern).  ALL = false: the thread's own unit (one unit per thread, so
+// uid is the thread's and the index is the unrolled constant u); ALL = true:
+// every unit of the tile at v + 4 uid, for a host emulation that runs the
+// tile's units in one thread.
+template <bool ALL>
+struct UnitIn {
+    static constexpr bool kUnit = true;
+    const uint64_t* v;
+    GF_HD uint64_t unit(uint32_t uid, uint32_t u) const { return v[(ALL ? uid * MODUP_UNIT : 0) + u]; }
+};
+template <bool ALL>
+struct UnitInF64 {
+    static constexpr bool kUnit = true;
+    const uint64_t* v;
+    GF_HD double unit(uint32_t uid, uint32_t u) const { return f64_of_bits(v[(ALL ? uid * MODUP_UNIT : 0) + u]); }
+};
+template <bool ALL>
+struct UnitOut {
+    static constexpr bool kUnit = true;
+    uint64_t* v;
+    GF_HD void unit(uint32_t uid, uint32_t u, uint64_t x) const { v[(ALL ? uid * MODUP_UNIT : 0) + u] = x; }
+};
+template <bool ALL>
+struct UnitOutF64 {
+    static constexpr bool kUnit = true;
+    uint64_t* v;
+    GF_HD void unit(uint32_t uid, uint32_t u, double x) const { v[(ALL ? uid * MODUP_UNIT : 0) + u] = bits_of_f64(x); }
+};
+
+// chunk 0 is one full radix-4 pass with one unit per thread
 template <uint32_t S1>
-GF_HD uint32_t modup_slot(uint32_t unit, uint32_t u) {
-    static_assert(chunk_size(S1, MODUP_RM, 0) == MODUP_RM, "chunk 0 must be a full radix-4 pass");
-    constexpr uint32_t LOGC = MODUP_LOG_TILE - S1, C = 1u << LOGC, TL = 1u << (S1 - MODUP_RM);
-    return (unit & (C - 1)) + (((unit >> LOGC) + u * TL) << LOGC);
+GF_HD constexpr bool modup_chunk0_fits() {
+    return chunk_size(S1, MODUP_RM, 0) == MODUP_RM &&
+           ((1u << (MODUP_LOG_TILE - S1)) << (S1 - MODUP_RM)) == MODUP_THREADS;
 }
 
 // source i's inverse column pass over the tile (all threads), its last pass
-// into LDS; then each thread takes its unit (after the block barrier)
-template <uint32_t S1, class Sync>
+// (chunk 0) into the unit registers v (see UnitOut)
+template <uint32_t S1, bool ALL, class Sync>
 GF_HD void modup_source_pass(const ModupJob& j, uint32_t i, const NttGeom& g, uint32_t tile, uint64_t* lds,
-                             uint32_t tid, uint32_t nthr, Sync sync) {
+                             uint32_t tid, uint32_t nthr, Sync sync, uint64_t* v) {
+    static_assert(modup_chunk0_fits<S1>(), "chunk 0 must be a full radix-4 pass, one unit per thread");
     constexpr uint32_t LOGC = MODUP_LOG_TILE - S1, C = 1u << LOGC;
     const TileDesc d{LOGC, C, 1, 1, 0, true, false};
     if (j.stwd[i]) {
         const double q = j.sqd[i], qinv = j.sqinv[i];
         const ColInF64 gin{gcast(j.src[i]), g.logN2, tile * C, q, qinv, F64_2P52 + 2 * q};
         const TwF64 tw{gcast((const double*)j.stwd[i])};
-        tile_chunks_f64<false, S1, MODUP_RM, 0>(lds, d, tw, tw, q, qinv, tid, nthr, sync, gin, LdsOutF64{lds});
+        tile_chunks_f64<false, S1, MODUP_RM, 0>(lds, d, tw, tw, q, qinv, tid, nthr, sync, gin, UnitOutF64<ALL>{v});
         return;
     }
     const ColIn gin{gcast(j.src[i]), g.logN2, tile * C};
     tile_chunks<false, false, S1, MODUP_RM, 0>(lds, d, TwPaired{gcast(j.stw[i])}, TwPaired{gcast(j.stw[i])}, j.sq[i],
-                                               tid, nthr, sync, gin, LdsOut{lds});
+                                               tid, nthr, sync, gin, UnitOut<ALL>{v});
 }
+// source i's unit v (its inverse transform's last pass) into the
+// conversion's registers
 template <uint32_t SP, uint32_t S1>
-GF_HD void modup_take_unit(const ModupJob& j, uint32_t i, const uint64_t* lds, uint32_t unit, ModupUnit<SP>& x) {
+GF_HD void modup_take_unit(const ModupJob& j, uint32_t i, const uint64_t* v, ModupUnit<SP>& x) {
     if (j.stwd[i]) {
         const double q = j.sqd[i], qinv = j.sqinv[i], w = j.spost[i], wq = j.spostq[i];
 #pragma unroll
         for (uint32_t u = 0; u < MODUP_UNIT; ++u) {
             // the last pass left |x| < 3q: times n^-1 QHatInv, into [0, q)
-            const double y = f64_mulmod(f64_of_bits(lds[modup_slot<S1>(unit, u)]), w, wq, q);
+            const double y = f64_mulmod(f64_of_bits(v[u]), w, wq, q);
             const uint64_t v = u52_from_f64(f64_reduce(y, q, qinv));
             x.lo[u][i] = split30_lo(v);
             x.hi[u][i] = split30_hi(v);
@@ -185,18 +224,18 @@ GF_HD void modup_take_unit(const ModupJob& j, uint32_t i, const uint64_t* lds, u
 #pragma unroll
     for (uint32_t u = 0; u < MODUP_UNIT; ++u) {
         // the inverse transform's final store: times n^-1 QHatInv, reduced
-        const uint64_t v = shoup(lds[modup_slot<S1>(unit, u)], post, postp, q);
-        x.lo[u][i] = split30_lo(v);
-        x.hi[u][i] = split30_hi(v);
+        const uint64_t r = shoup(v[u], post, postp, q);
+        x.lo[u][i] = split30_lo(r);
+        x.hi[u][i] = split30_hi(r);
     }
 }
-// target t's conversion of this thread's unit into LDS: the exact column sums
+// target t's conversion of this thread's unit into cv: the exact column sums
 // over the part's sources, REDC without its correction ([0, 2 q_t)), as the
 // conversion kernel leaves them for the forward transforms (with c60 the
 // R = 2^60 reduction of the columns, below 2.2 q_t); for an fp64 target as
 // the double v - q_t (|.| <= 1.25 q_t)
 template <uint32_t SP, uint32_t S1, bool C60>
-GF_HD void modup_convert_unit(const ModupJob& j, uint32_t t, uint64_t* lds, uint32_t unit, ModupUnit<SP>& x) {
+GF_HD void modup_convert_unit(const ModupJob& j, uint32_t t, ModupUnit<SP>& x, uint64_t* cv) {
     const uint64_t q = uniform_at(j.tq, t), qn = uniform_at(j.tqneg, t);
     const bool f64 = uniform_at(j.ttwd, t) != 0;
     const double c2 = f64 ? F64_2P52 + f64_of_bits(uniform_at(j.tqf, 2 * t)) : 0.0;
@@ -214,7 +253,7 @@ GF_HD void modup_convert_unit(const ModupJob& j, uint32_t t, uint64_t* lds, uint
         uint64_t v;
         if constexpr (C60) v = redc60_lazy(dot30_cols<SP>(x.lo[u], x.hi[u], cst), q, qn);
         else v = redc_lazy(dot30<SP>(x.lo[u], x.hi[u], cst), q, qn);
-        lds[modup_slot<S1>(unit, u)] = f64 ? bits_of_f64(f64_from_u52(v, c2)) : v;
+        cv[u] = f64 ? bits_of_f64(f64_from_u52(v, c2)) : v;
     }
 }
 // Rescale mode: the dropped tower's value at this thread's 4 positions
@@ -234,11 +273,10 @@ GF_HD void modup_drop_unit(const ModupJob& j, const NttGeom& g, uint32_t tile, u
         dv[u] = addmod(gat(gcast(j.dt), gi), shoup(submod(gat(gcast(j.dr), gi), cd, q), j.pinv, j.pinvp, q), q);
     }
 }
-// Rescale mode: target t's prologue value of this thread's unit into LDS
+// Rescale mode: target t's prologue value of this thread's unit into cv
 // (below 4 q_t; for an fp64 target as the centred double)
 template <uint32_t SP, uint32_t S1>
-GF_HD void modup_rs_unit(const ModupJob& j, uint32_t t, uint64_t* lds, uint32_t unit, ModupUnit<SP>& x,
-                         const uint64_t* dv) {
+GF_HD void modup_rs_unit(const ModupJob& j, uint32_t t, ModupUnit<SP>& x, const uint64_t* dv, uint64_t* cv) {
     static_assert(sizeof(RsTarget) == 7 * 8, "RsTarget layout");
     RsTarget tg;
     const uint64_t* w = (const uint64_t*)j.tg + 7 * t;
@@ -261,16 +299,16 @@ GF_HD void modup_rs_unit(const ModupJob& j, uint32_t t, uint64_t* lds, uint32_t 
         }
         const U128 acc = dot30<SP>(x.lo[u], x.hi[u], [&](uint32_t s) { return uniform_at(j.tab, s * j.sc + t); });
         const uint64_t v = rs_lazy_value(dv[u], dv[u] > j.half, tg, acc);
-        lds[modup_slot<S1>(unit, u)] =
-            f64 ? bits_of_f64(f64_center(f64_from_u52(v, F64_2P52 + 2 * qd), qd, qinv)) : v;
+        cv[u] = f64 ? bits_of_f64(f64_center(f64_from_u52(v, F64_2P52 + 2 * qd), qd, qinv)) : v;
     }
 }
 
-// target t's forward column pass from LDS, stored to HBM unreduced (the
-// digits' row pass continues from these values)
-template <uint32_t S1, class Sync>
+// target t's forward column pass from the unit registers cv (see UnitIn),
+// stored to HBM unreduced (the digits' row pass continues from these values)
+template <uint32_t S1, bool ALL, class Sync>
 GF_HD void modup_target_pass(const ModupJob& j, uint32_t t, const NttGeom& g, uint32_t tile, uint64_t* lds,
-                             uint32_t tid, uint32_t nthr, Sync sync) {
+                             uint32_t tid, uint32_t nthr, Sync sync, const uint64_t* cv) {
+    static_assert(modup_chunk0_fits<S1>(), "chunk 0 must be a full radix-4 pass, one unit per thread");
     constexpr uint32_t LOGC = MODUP_LOG_TILE - S1, C = 1u << LOGC;
     const TileDesc d{LOGC, C, 1, 1, 0, true, false};
     const uint64_t twd = uniform_at(j.ttwd, t);
@@ -278,7 +316,7 @@ GF_HD void modup_target_pass(const ModupJob& j, uint32_t t, const NttGeom& g, ui
         const double q = f64_of_bits(uniform_at(j.tqf, 2 * t)), qinv = f64_of_bits(uniform_at(j.tqf, 2 * t + 1));
         const ColOutF64 gout{gcast(j.dst + (size_t)t * g.n), g.logN2, tile * C, q, qinv};
         const TwF64 tw{gcast((const double*)(uintptr_t)twd)};
-        tile_chunks_f64<true, S1, MODUP_RM, 0>(lds, d, tw, tw, q, qinv, tid, nthr, sync, LdsInF64{lds}, gout);
+        tile_chunks_f64<true, S1, MODUP_RM, 0>(lds, d, tw, tw, q, qinv, tid, nthr, sync, UnitInF64<ALL>{cv}, gout);
         return;
     }
     NttJob jt{};
@@ -286,9 +324,9 @@ GF_HD void modup_target_pass(const ModupJob& j, uint32_t t, const NttGeom& g, ui
     const ColOut<true> gout{gcast(j.dst + (size_t)t * g.n), g.logN2, tile * C, jt};
     const TwPaired tw{gcast((const uint64_t*)(uintptr_t)uniform_at(j.ttw, t))};
     if (uniform_at(j.tred, t))
-        tile_chunks<true, true, S1, MODUP_RM, 0>(lds, d, tw, tw, jt.q, tid, nthr, sync, LdsIn{lds}, gout);
+        tile_chunks<true, true, S1, MODUP_RM, 0>(lds, d, tw, tw, jt.q, tid, nthr, sync, UnitIn<ALL>{cv}, gout);
     else
-        tile_chunks<true, false, S1, MODUP_RM, 0>(lds, d, tw, tw, jt.q, tid, nthr, sync, LdsIn{lds}, gout);
+        tile_chunks<true, false, S1, MODUP_RM, 0>(lds, d, tw, tw, jt.q, tid, nthr, sync, UnitIn<ALL>{cv}, gout);
 }
 
 }  // namespace gf

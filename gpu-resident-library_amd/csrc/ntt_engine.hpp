@@ -242,6 +242,24 @@ struct LdsPlain {
     GF_HD LdsOut out(uint64_t* a) const { return LdsOut{a}; }
 };
 
+// Accessors that name an element by its unit instead (kUnit): element u of
+// unit uid, for a pass whose one end is the values a thread holds in
+// registers.  pass_in / pass_out pick the form the accessor takes.
+template <class A, class = void>
+struct IsUnitAcc : std::false_type {};
+template <class A>
+struct IsUnitAcc<A, std::void_t<decltype(A::kUnit)>> : std::true_type {};
+template <class IN>
+GF_HD auto pass_in(const IN& in, uint32_t uid, uint32_t u, uint32_t s, uint32_t k, uint32_t at) {
+    if constexpr (IsUnitAcc<IN>::value) return in.unit(uid, u);
+    else return in(s, k, at);
+}
+template <class OUT, class V>
+GF_HD void pass_out(const OUT& out, uint32_t uid, uint32_t u, uint32_t s, uint32_t k, uint32_t at, V v) {
+    if constexpr (IsUnitAcc<OUT>::value) out.unit(uid, u, v);
+    else out(s, k, at, v);
+}
+
 // One radix-2^R pass (stages ST0 .. ST0+R-1 of an S-stage transform) over all
 // units of the tile owned by this thread; fully unrolled, the R-stage unit
 // stays in registers.  `in` supplies the unit's values, `out` takes them.
@@ -275,7 +293,7 @@ GF_HD void tile_pass(const TileDesc& d, const TW& tw, uint64_t q, uint32_t tid, 
             at[u] = sbase + (d.pad ? padded(k0 + u * TL) : (k0 + u * TL) * d.KS);
         uint64_t x[R8];
 #pragma unroll
-        for (uint32_t u = 0; u < R8; ++u) x[u] = in(s, k0 + u * TL, at[u]);
+        for (uint32_t u = 0; u < R8; ++u) x[u] = pass_in(in, uid, u, s, k0 + u * TL, at[u]);
         const uint32_t G = d.g0 + s * d.gstep;
         if (FWD) {
 #pragma unroll
@@ -314,7 +332,7 @@ GF_HD void tile_pass(const TileDesc& d, const TW& tw, uint64_t q, uint32_t tid, 
             }
         }
 #pragma unroll
-        for (uint32_t u = 0; u < R8; ++u) out(s, k0 + u * TL, at[u], x[u]);
+        for (uint32_t u = 0; u < R8; ++u) pass_out(out, uid, u, s, k0 + u * TL, at[u], x[u]);
     }
 }
 

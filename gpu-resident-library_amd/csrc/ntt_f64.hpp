@@ -149,7 +149,7 @@ GF_HD void tile_pass_f64(const TileDesc& d, const TW& tw, double q, double qinv,
             at[u] = sbase + (d.pad ? padded(k0 + u * TL) : (k0 + u * TL) * d.KS);
         double x[R8];
 #pragma unroll
-        for (uint32_t u = 0; u < R8; ++u) x[u] = in(s, k0 + u * TL, at[u]);
+        for (uint32_t u = 0; u < R8; ++u) x[u] = pass_in(in, uid, u, s, k0 + u * TL, at[u]);
         const uint32_t G = d.g0 + s * d.gstep;
         if (FWD) {
 #pragma unroll
@@ -192,7 +192,7 @@ GF_HD void tile_pass_f64(const TileDesc& d, const TW& tw, double q, double qinv,
 #pragma unroll
         for (uint32_t u = 0; u < R8; ++u) {
             if constexpr (CENTER) x[u] = f64_center(x[u], q, qinv);
-            out(s, k0 + u * TL, at[u], x[u]);
+            pass_out(out, uid, u, s, k0 + u * TL, at[u], x[u]);
         }
     }
 }

@@ -148,36 +148,6 @@ struct ModupUnit {
     uint32_t lo[MODUP_UNIT][SP], hi[MODUP_UNIT][SP];
 };
 
-// A chunk-0 unit in registers: element u at v[u] (a word, or an fp64 pass's
-// bit pattern).  ALL = false: the thread's own unit (one unit per thread, so
-// uid is the thread's and the index is the unrolled constant u); ALL = true:
-// every unit of the tile at v + 4 uid, for a host emulation that runs the
-// tile's units in one thread.
-template <bool ALL>
-struct UnitIn {
-    static constexpr bool kUnit = true;
-    const uint64_t* v;
-    GF_HD uint64_t unit(uint32_t uid, uint32_t u) const { return v[(ALL ? uid * MODUP_UNIT : 0) + u]; }
-};
-template <bool ALL>
-struct UnitInF64 {
-    static constexpr bool kUnit = true;
-    const uint64_t* v;
-    GF_HD double unit(uint32_t uid, uint32_t u) const { return f64_of_bits(v[(ALL ? uid * MODUP_UNIT : 0) + u]); }
-};
-template <bool ALL>
-struct UnitOut {
-    static constexpr bool kUnit = true;
-    uint64_t* v;
-    GF_HD void unit(uint32_t uid, uint32_t u, uint64_t x) const { v[(ALL ? uid * MODUP_UNIT : 0) + u] = x; }
-};
-template <bool ALL>
-struct UnitOutF64 {
-    static constexpr bool kUnit = true;
-    uint64_t* v;
-    GF_HD void unit(uint32_t uid, uint32_t u, double x) const { v[(ALL ? uid * MODUP_UNIT : 0) + u] = bits_of_f64(x); }
-};
-
 // chunk 0 is one full radix-4 pass with one unit per thread
 template <uint32_t S1>
 GF_HD constexpr bool modup_chunk0_fits() {
@@ -197,12 +167,12 @@ GF_HD void modup_source_pass(const ModupJob& j, uint32_t i, const NttGeom& g, ui
         const double q = j.sqd[i], qinv = j.sqinv[i];
         const ColInF64 gin{gcast(j.src[i]), g.logN2, tile * C, q, qinv, F64_2P52 + 2 * q};
         const TwF64 tw{gcast((const double*)j.stwd[i])};
-        tile_chunks_f64<false, S1, MODUP_RM, 0>(lds, d, tw, tw, q, qinv, tid, nthr, sync, gin, UnitOutF64<ALL>{v});
+        tile_chunks_f64<false, S1, MODUP_RM, 0>(lds, d, tw, tw, q, qinv, tid, nthr, sync, gin, UnitOutF64<ALL, MODUP_UNIT>{v});
         return;
     }
     const ColIn gin{gcast(j.src[i]), g.logN2, tile * C};
     tile_chunks<false, false, S1, MODUP_RM, 0>(lds, d, TwPaired{gcast(j.stw[i])}, TwPaired{gcast(j.stw[i])}, j.sq[i],
-                                               tid, nthr, sync, gin, UnitOut<ALL>{v});
+                                               tid, nthr, sync, gin, UnitOut<ALL, MODUP_UNIT>{v});
 }
 // source i's unit v (its inverse transform's last pass) into the
 // conversion's registers
@@ -316,7 +286,7 @@ GF_HD void modup_target_pass(const ModupJob& j, uint32_t t, const NttGeom& g, ui
         const double q = f64_of_bits(uniform_at(j.tqf, 2 * t)), qinv = f64_of_bits(uniform_at(j.tqf, 2 * t + 1));
         const ColOutF64 gout{gcast(j.dst + (size_t)t * g.n), g.logN2, tile * C, q, qinv};
         const TwF64 tw{gcast((const double*)(uintptr_t)twd)};
-        tile_chunks_f64<true, S1, MODUP_RM, 0>(lds, d, tw, tw, q, qinv, tid, nthr, sync, UnitInF64<ALL>{cv}, gout);
+        tile_chunks_f64<true, S1, MODUP_RM, 0>(lds, d, tw, tw, q, qinv, tid, nthr, sync, UnitInF64<ALL, MODUP_UNIT>{cv}, gout);
         return;
     }
     NttJob jt{};
@@ -324,9 +294,9 @@ GF_HD void modup_target_pass(const ModupJob& j, uint32_t t, const NttGeom& g, ui
     const ColOut<true> gout{gcast(j.dst + (size_t)t * g.n), g.logN2, tile * C, jt};
     const TwPaired tw{gcast((const uint64_t*)(uintptr_t)uniform_at(j.ttw, t))};
     if (uniform_at(j.tred, t))
-        tile_chunks<true, true, S1, MODUP_RM, 0>(lds, d, tw, tw, jt.q, tid, nthr, sync, UnitIn<ALL>{cv}, gout);
+        tile_chunks<true, true, S1, MODUP_RM, 0>(lds, d, tw, tw, jt.q, tid, nthr, sync, UnitIn<ALL, MODUP_UNIT>{cv}, gout);
     else
-        tile_chunks<true, false, S1, MODUP_RM, 0>(lds, d, tw, tw, jt.q, tid, nthr, sync, UnitIn<ALL>{cv}, gout);
+        tile_chunks<true, false, S1, MODUP_RM, 0>(lds, d, tw, tw, jt.q, tid, nthr, sync, UnitIn<ALL, MODUP_UNIT>{cv}, gout);
 }
 
 }  // namespace gf

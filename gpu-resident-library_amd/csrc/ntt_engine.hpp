@@ -259,6 +259,23 @@ GF_HD void pass_out(const OUT& out, uint32_t uid, uint32_t u, uint32_t s, uint32
     if constexpr (IsUnitAcc<OUT>::value) out.unit(uid, u, v);
     else out(s, k, at, v);
 }
+// A pass's unit of U elements in registers: element u at v[u].  ALL = false:
+// the thread's own unit (a pass with one unit per thread, so uid is the
+// thread's and the index is the unrolled constant u); ALL = true: every unit
+// of the tile at v + U uid, for a host emulation that runs the tile's units
+// in one thread.
+template <bool ALL, uint32_t U = 4>
+struct UnitIn {
+    static constexpr bool kUnit = true;
+    const uint64_t* v;
+    GF_HD uint64_t unit(uint32_t uid, uint32_t u) const { return v[(ALL ? uid * U : 0) + u]; }
+};
+template <bool ALL, uint32_t U = 4>
+struct UnitOut {
+    static constexpr bool kUnit = true;
+    uint64_t* v;
+    GF_HD void unit(uint32_t uid, uint32_t u, uint64_t x) const { v[(ALL ? uid * U : 0) + u] = x; }
+};
 
 // One radix-2^R pass (stages ST0 .. ST0+R-1 of an S-stage transform) over all
 // units of the tile owned by this thread; fully unrolled, the R-stage unit

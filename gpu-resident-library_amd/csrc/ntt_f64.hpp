@@ -116,6 +116,19 @@ struct LdsPlainF64 {
     GF_HD LdsInF64 in(const uint64_t* a) const { return LdsInF64{a}; }
     GF_HD LdsOutF64 out(uint64_t* a) const { return LdsOutF64{a}; }
 };
+// a unit in registers (UnitIn, UnitOut) holding the doubles' bit patterns
+template <bool ALL, uint32_t U = 4>
+struct UnitInF64 {
+    static constexpr bool kUnit = true;
+    const uint64_t* v;
+    GF_HD double unit(uint32_t uid, uint32_t u) const { return f64_of_bits(v[(ALL ? uid * U : 0) + u]); }
+};
+template <bool ALL, uint32_t U = 4>
+struct UnitOutF64 {
+    static constexpr bool kUnit = true;
+    uint64_t* v;
+    GF_HD void unit(uint32_t uid, uint32_t u, double x) const { v[(ALL ? uid * U : 0) + u] = bits_of_f64(x); }
+};
 
 // One radix-2^R pass over the tile's units, as tile_pass (same unit and
 // twiddle indexing) on doubles.  CENTER: outputs centred before `out`.  An

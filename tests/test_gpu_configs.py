@@ -136,6 +136,29 @@ def test_cfg3_uneven_levels_run_and_graph(G):
             assert np.array_equal(outs[i][1].download(), w1), (mode, levels[i])
 
 
+def test_cfg3_inner_product_row_passes(G):
+    """The fused row pass + inner product with integer and with fp64 row
+    passes gives the oracle's results at n = 2^16, L = 24 over uneven levels
+    (Q rows with one own part, P rows with none, lazy accumulator rows)."""
+    c = cfg3(G)
+    levels = [24, 13, 2]
+    saved = c.ctx.knob("ksi_f64")
+    want = None
+    try:
+        for f64 in (0, 1):
+            c.ctx.set_knob("ksi_f64", f64)
+            plan, ins, outs, ws = emr_batch(G, c, levels, 31)
+            if want is None:
+                want = oracle_emr(c, levels, ins, range(len(levels)))
+            plan.run_graph(c.s)
+            c.s.sync()
+            for i, (w0, w1) in enumerate(want):
+                assert np.array_equal(outs[i][0].download(), w0), (f64, levels[i])
+                assert np.array_equal(outs[i][1].download(), w1), (f64, levels[i])
+    finally:
+        c.ctx.set_knob("ksi_f64", saved)
+
+
 def test_cfg2_chain_16x10_matches_oracle(G):
     """16 ciphertexts, each multiplied (EvalMult+Rescale) 10 times in a row by
     a resident operand of the matching level, n = 2^15, L = 12 -> 2, dnum = 3:

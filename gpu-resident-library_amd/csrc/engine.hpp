@@ -143,8 +143,9 @@ public:
     // before it and its column passes
     uint32_t modup_f64 = 1;
     // 1 (with modup_f64): the row passes of the fused inner product after it
-    // as well (fewer instructions, but measured slower on gfx950)
-    uint32_t ksi_f64 = 0;
+    // as well (fp64 butterflies issue in about half the cycles of the integer
+    // Shoup butterflies)
+    uint32_t ksi_f64 = 1;
     // 1: the fused EvalMult+Rescale runs the P towers' inverse column pass,
     // the rescale prologue and the surviving towers' forward column pass as
     // one kernel (the fused ModUp kernel's rescale mode), where the fused

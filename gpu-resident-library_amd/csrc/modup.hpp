@@ -128,9 +128,8 @@ struct ColInF64 {
     const GF_GLOBAL uint64_t* src;
     uint32_t logN2, b0;
     double q, qinv, c2;   // c2 = 2^52 + 2q
-    GF_HD double operator()(uint32_t s, uint32_t k, uint32_t) const {
-        return f64_center(f64_from_u52(gat(src, (k << logN2) + b0 + s), c2), q, qinv);
-    }
+    GF_HD uint64_t raw(uint32_t s, uint32_t k, uint32_t) const { return gat(src, (k << logN2) + b0 + s); }
+    GF_HD double conv(uint64_t w) const { return f64_center(f64_from_u52(w, c2), q, qinv); }
 };
 struct ColOutF64 {
     GF_GLOBAL uint64_t* dst;

@@ -35,6 +35,24 @@ namespace gf {
 #define GF_NO_CONTRACT
 #endif
 
+// The device scheduler may not move instructions across this point (a
+// scheduling hint only).  The fp64 passes put it between a unit's global loads
+// and their conversions: left to itself, the scheduler interleaves each load
+// with its conversion under register pressure and waits for the last loads
+// one at a time, a memory round trip each.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define GF_SCHED_FENCE() __builtin_amdgcn_sched_barrier(0)
+#else
+#define GF_SCHED_FENCE() ((void)0)
+#endif
+
+// An input accessor with a raw form (HasRaw): raw(s, k, at) fetches the word,
+// conv(w) turns it into the pass's double
+template <class A, class = void>
+struct HasRaw : std::false_type {};
+template <class A>
+struct HasRaw<A, std::void_t<decltype(&A::raw)>> : std::true_type {};
+
 // moduli whose transforms may run in double precision
 GF_HD constexpr bool f64_modulus(uint64_t q) { return q < (1ull << 50); }
 
@@ -160,10 +178,25 @@ GF_HD void tile_pass_f64(const TileDesc& d, const TW& tw, double q, double qinv,
 #pragma unroll
         for (uint32_t u = 0; u < R8; ++u)
             at[u] = sbase + (d.pad ? padded(k0 + u * TL) : (k0 + u * TL) * d.KS);
-        double x[R8];
-#pragma unroll
-        for (uint32_t u = 0; u < R8; ++u) x[u] = pass_in(in, uid, u, s, k0 + u * TL, at[u]);
         const uint32_t G = d.g0 + s * d.gstep;
+        // a forward pass's first stage takes one twiddle: requested with the data
+        double w0 = 0, wq0 = 0;
+        if constexpr (FWD) {
+            if constexpr (TW::kRows) tw.get_rp(0, s, g, 0, w0, wq0);
+            else tw.get((G << ST0) + g, w0, wq0);
+        }
+        double x[R8];
+        if constexpr (HasRaw<IN>::value) {
+            uint64_t r[R8];
+#pragma unroll
+            for (uint32_t u = 0; u < R8; ++u) r[u] = in.raw(s, k0 + u * TL, at[u]);
+            GF_SCHED_FENCE();
+#pragma unroll
+            for (uint32_t u = 0; u < R8; ++u) x[u] = in.conv(r[u]);
+        } else {
+#pragma unroll
+            for (uint32_t u = 0; u < R8; ++u) x[u] = pass_in(in, uid, u, s, k0 + u * TL, at[u]);
+        }
         if (FWD) {
 #pragma unroll
             for (uint32_t c = 0; c < R; ++c) {
@@ -172,9 +205,11 @@ GF_HD void tile_pass_f64(const TileDesc& d, const TW& tw, double q, double qinv,
 #pragma unroll
                 for (uint32_t u = 0; u < R8; ++u) {
                     if (u & h) continue;
-                    double w, wq;
-                    if constexpr (TW::kRows) tw.get_rp(c, s, g, u >> (R - c), w, wq);
-                    else tw.get(tb + (u >> (R - c)), w, wq);
+                    double w = w0, wq = wq0;
+                    if (c > 0) {
+                        if constexpr (TW::kRows) tw.get_rp(c, s, g, u >> (R - c), w, wq);
+                        else tw.get(tb + (u >> (R - c)), w, wq);
+                    }
                     const double m = f64_mulmod(x[u + h], w, wq, q);
                     const double a = x[u];
                     x[u] = a + m;
@@ -283,8 +318,9 @@ struct RowInF64 {
     const GF_GLOBAL uint64_t* src;   // tile start
     uint32_t N;
     double c2, q, qinv;
-    GF_HD double operator()(uint32_t s, uint32_t k, uint32_t) const {
-        const double x = f64_from_u52(gat(src, s * N + k), c2);
+    GF_HD uint64_t raw(uint32_t s, uint32_t k, uint32_t) const { return gat(src, s * N + k); }
+    GF_HD double conv(uint64_t w) const {
+        const double x = f64_from_u52(w, c2);
         return CTR ? f64_center(x, q, qinv) : x;
     }
 };

@@ -25,6 +25,11 @@ __device__ __forceinline__ void modup_sources(const ModupJob& j, const NttGeom& 
 template <uint32_t S1, uint32_t SP, bool RS, bool C60>
 __global__ __launch_bounds__(MODUP_THREADS) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_modup(const ModupJob* __restrict__ jobs, uint32_t off, NttGeom g) {
     __shared__ uint64_t lds[2 * MODUP_TILE];   // phases alternate between the two tiles
+    // the targets' column-pass twiddles (modup_tw_pair), by target parity as
+    // the tiles: target t + 2 rewrites target t's copy only after a barrier of
+    // target t + 1, which every thread reaches after its last read of it
+    constexpr uint32_t TWP = 1u << S1, TWPER = TWP > MODUP_THREADS ? TWP / MODUP_THREADS : 1;
+    __shared__ uint64_t twl[2][2 * TWP];
     const ModupJob& j = jobs[off + blockIdx.y];
     const uint32_t tid = threadIdx.x;
     // Workgroups go to the 8 XCDs in turn (linear id mod 8, and gridDim.x is
@@ -36,22 +41,28 @@ __global__ __launch_bounds__(MODUP_THREADS) __attribute__((amdgpu_waves_per_eu(4
     const uint32_t tile = nt % 8 ? blockIdx.x : (blockIdx.x % 8) * (nt / 8) + blockIdx.x / 8;
     ModupUnit<SP> x;
     modup_sources<S1, SP, 0>(j, g, lds, tid, tile, x);
-    // target t is phase SP + t
+    // target t is phase SP + t; its twiddles are requested before its
+    // conversion and stored to LDS after it
     uint64_t cv[MODUP_UNIT];
+    auto target = [&](uint32_t t, auto&& convert) {
+        Words2 tp[TWPER];
+#pragma unroll
+        for (uint32_t k = 0; k < TWPER; ++k)
+            if (tid + k * MODUP_THREADS < TWP) tp[k] = modup_tw_pair(j, t, tid + k * MODUP_THREADS);
+        convert();
+        uint64_t* tl = twl[(SP + t) & 1];
+#pragma unroll
+        for (uint32_t k = 0; k < TWPER; ++k)
+            if (tid + k * MODUP_THREADS < TWP) modup_tw_put(tl, tid + k * MODUP_THREADS, tp[k]);
+        modup_target_pass<S1, false>(j, t, g, tile, lds + ((SP + t) & 1) * MODUP_TILE, tid, MODUP_THREADS, DevSync{},
+                                     cv, tl);
+    };
     if constexpr (RS) {
         uint64_t dv[MODUP_UNIT];
         modup_drop_unit<SP, S1>(j, g, tile, tid, x, dv);
-        for (uint32_t t = 0; t < j.sc; ++t) {
-            modup_rs_unit<SP, S1>(j, t, x, dv, cv);
-            modup_target_pass<S1, false>(j, t, g, tile, lds + ((SP + t) & 1) * MODUP_TILE, tid, MODUP_THREADS,
-                                         DevSync{}, cv);
-        }
+        for (uint32_t t = 0; t < j.sc; ++t) target(t, [&] { modup_rs_unit<SP, S1>(j, t, x, dv, cv); });
     } else {
-        for (uint32_t t = 0; t < j.sc; ++t) {
-            modup_convert_unit<SP, S1, C60>(j, t, x, cv);
-            modup_target_pass<S1, false>(j, t, g, tile, lds + ((SP + t) & 1) * MODUP_TILE, tid, MODUP_THREADS,
-                                         DevSync{}, cv);
-        }
+        for (uint32_t t = 0; t < j.sc; ++t) target(t, [&] { modup_convert_unit<SP, S1, C60>(j, t, x, cv); });
     }
 }
 

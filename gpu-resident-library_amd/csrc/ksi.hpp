@@ -60,9 +60,6 @@ struct KsiJob {
 
 // V consecutive words from word i of a tower (V = 2: one 16-byte access; the
 // address is 16-byte aligned), addressed as base + 32-bit byte offset (gat)
-struct alignas(16) Words2 {
-    uint64_t w[2];
-};
 template <uint32_t V>
 GF_HD void load_words(const GF_GLOBAL uint64_t* base, uint32_t i, uint64_t* out) {
     if constexpr (V == 2) {

@@ -272,30 +272,73 @@ GF_HD void modup_rs_unit(const ModupJob& j, uint32_t t, ModupUnit<SP>& x, const 
     }
 }
 
+// A target's column-pass twiddles: every column of a column pass takes the
+// same ones, the first 2^S1 pairs (16 bytes each, fp64 or integer) of the
+// target's forward table.  The kernel requests its share of them before the
+// target's conversion and stores them into an LDS tile (tl) before the first
+// pass; the passes behind the first block barrier read them from there.
+GF_HD Words2 modup_tw_pair(const ModupJob& j, uint32_t t, uint32_t i) {
+    const uint64_t twd = uniform_at(j.ttwd, t);
+    const uint64_t* tab = twd ? (const uint64_t*)(uintptr_t)twd : (const uint64_t*)(uintptr_t)uniform_at(j.ttw, t);
+    return gat((const GF_GLOBAL Words2*)gcast(tab), i);
+}
+GF_HD void modup_tw_put(uint64_t* tl, uint32_t i, const Words2& p) {
+    tl[2 * i] = p.w[0];
+    tl[2 * i + 1] = p.w[1];
+}
+struct TwLds {
+    static constexpr bool kRows = false;
+    const uint64_t* t;
+    GF_HD void get(uint32_t i, uint64_t& w, uint64_t& wp) const {
+        w = t[2 * i];
+        wp = t[2 * i + 1];
+    }
+};
+struct TwLdsF64 {
+    static constexpr bool kRows = false;
+    const uint64_t* t;
+    GF_HD void get(uint32_t i, double& w, double& wq) const {
+        w = f64_of_bits(t[2 * i]);
+        wq = f64_of_bits(t[2 * i + 1]);
+    }
+};
+
 // target t's forward column pass from the unit registers cv (see UnitIn),
-// stored to HBM unreduced (the digits' row pass continues from these values)
+// stored to HBM unreduced (the digits' row pass continues from these
+// values): the first pass with the table's own twiddles, the others with
+// their copy in tl (visible after the first pass's barrier)
 template <uint32_t S1, bool ALL, class Sync>
 GF_HD void modup_target_pass(const ModupJob& j, uint32_t t, const NttGeom& g, uint32_t tile, uint64_t* lds,
-                             uint32_t tid, uint32_t nthr, Sync sync, const uint64_t* cv) {
+                             uint32_t tid, uint32_t nthr, Sync sync, const uint64_t* cv, const uint64_t* tl) {
     static_assert(modup_chunk0_fits<S1>(), "chunk 0 must be a full radix-4 pass, one unit per thread");
+    using P = PassOf<true, S1, MODUP_RM, 0>;
+    static_assert(P::NC >= 2, "a column pass of two or more passes");
     constexpr uint32_t LOGC = MODUP_LOG_TILE - S1, C = 1u << LOGC;
     const TileDesc d{LOGC, C, 1, 1, 0, true, false};
     const uint64_t twd = uniform_at(j.ttwd, t);
     if (twd) {
         const double q = f64_of_bits(uniform_at(j.tqf, 2 * t)), qinv = f64_of_bits(uniform_at(j.tqf, 2 * t + 1));
         const ColOutF64 gout{gcast(j.dst + (size_t)t * g.n), g.logN2, tile * C, q, qinv};
-        const TwF64 tw{gcast((const double*)(uintptr_t)twd)};
-        tile_chunks_f64<true, S1, MODUP_RM, 0>(lds, d, tw, tw, q, qinv, tid, nthr, sync, UnitInF64<ALL, MODUP_UNIT>{cv}, gout);
+        const UnitInF64<ALL, MODUP_UNIT> in{cv};
+        tile_pass_f64<true, S1, P::ST0, P::R, f64_center_after<true, S1, MODUP_RM>(0)>(
+            d, TwF64{gcast((const double*)(uintptr_t)twd)}, q, qinv, tid, nthr, in, LdsOutF64{lds});
+        sync();
+        tile_chunks_f64<true, S1, MODUP_RM, 1>(lds, d, TwLdsF64{tl}, TwLdsF64{tl}, q, qinv, tid, nthr, sync, in, gout);
         return;
     }
     NttJob jt{};
     jt.q = uniform_at(j.tq, t);
     const ColOut<true> gout{gcast(j.dst + (size_t)t * g.n), g.logN2, tile * C, jt};
     const TwPaired tw{gcast((const uint64_t*)(uintptr_t)uniform_at(j.ttw, t))};
-    if (uniform_at(j.tred, t))
-        tile_chunks<true, true, S1, MODUP_RM, 0>(lds, d, tw, tw, jt.q, tid, nthr, sync, UnitIn<ALL, MODUP_UNIT>{cv}, gout);
-    else
-        tile_chunks<true, false, S1, MODUP_RM, 0>(lds, d, tw, tw, jt.q, tid, nthr, sync, UnitIn<ALL, MODUP_UNIT>{cv}, gout);
+    const UnitIn<ALL, MODUP_UNIT> in{cv};
+    auto run = [&](auto LZc) {
+        constexpr bool LZ = decltype(LZc)::value;
+        tile_pass<true, LZ, S1, P::ST0, P::R>(d, tw, jt.q, tid, nthr, in, LdsOut{lds});
+        sync();
+        tile_chunks<true, LZ, S1, MODUP_RM, 1>(lds, d, TwLds{tl}, TwLds{tl}, jt.q, tid, nthr, sync, in, gout);
+    };
+    if (uniform_at(j.tred, t)) run(std::true_type{});
+    else run(std::false_type{});
 }
 
 }  // namespace gf

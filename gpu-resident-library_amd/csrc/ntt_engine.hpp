@@ -137,6 +137,11 @@ inline NttGeom make_geom(uint32_t n, uint32_t rmax = 3) {
     return g;
 }
 
+// two consecutive words as one 16-byte access
+struct alignas(16) Words2 {
+    uint64_t w[2];
+};
+
 // twiddle accessors: paired (w, wp) table -> one 16-byte load; or two arrays
 struct TwPaired {
     static constexpr bool kRows = false;

@@ -281,7 +281,7 @@ std::vector<uint64_t> paired(const uint64_t* tw, const uint64_t* twp, uint32_t n
 template <uint32_t SP, uint32_t S1, bool C60>
 void emu_modup_t(const ModupJob& j, const NttGeom& g) {
     // one thread runs the tile's 256 units: their registers side by side
-    std::vector<uint64_t> lds(MODUP_TILE), v((size_t)MODUP_THREADS * MODUP_UNIT);
+    std::vector<uint64_t> lds(MODUP_TILE), v((size_t)MODUP_THREADS * MODUP_UNIT), tl(2u << S1);
     std::vector<ModupUnit<SP>> units(MODUP_THREADS);
     for (uint32_t tile = 0; tile < g.n / MODUP_TILE; ++tile) {
         for (uint32_t i = 0; i < SP; ++i) {
@@ -292,13 +292,14 @@ void emu_modup_t(const ModupJob& j, const NttGeom& g) {
         for (uint32_t t = 0; t < j.sc; ++t) {
             for (uint32_t u = 0; u < MODUP_THREADS; ++u)
                 modup_convert_unit<SP, S1, C60>(j, t, units[u], v.data() + (size_t)u * MODUP_UNIT);
-            modup_target_pass<S1, true>(j, t, g, tile, lds.data(), 0, 1, NoSync{}, v.data());
+            for (uint32_t i = 0; i < (1u << S1); ++i) modup_tw_put(tl.data(), i, modup_tw_pair(j, t, i));
+            modup_target_pass<S1, true>(j, t, g, tile, lds.data(), 0, 1, NoSync{}, v.data(), tl.data());
         }
     }
 }
 template <uint32_t SP, uint32_t S1>
 void emu_moddown_t(const ModupJob& j, const NttGeom& g) {
-    std::vector<uint64_t> lds(MODUP_TILE), v((size_t)MODUP_THREADS * MODUP_UNIT);
+    std::vector<uint64_t> lds(MODUP_TILE), v((size_t)MODUP_THREADS * MODUP_UNIT), tl(2u << S1);
     std::vector<ModupUnit<SP>> units(MODUP_THREADS);
     std::vector<uint64_t> dv((size_t)MODUP_THREADS * MODUP_UNIT);
     for (uint32_t tile = 0; tile < g.n / MODUP_TILE; ++tile) {
@@ -313,7 +314,8 @@ void emu_moddown_t(const ModupJob& j, const NttGeom& g) {
             for (uint32_t u = 0; u < MODUP_THREADS; ++u)
                 modup_rs_unit<SP, S1>(j, t, units[u], dv.data() + (size_t)u * MODUP_UNIT,
                                       v.data() + (size_t)u * MODUP_UNIT);
-            modup_target_pass<S1, true>(j, t, g, tile, lds.data(), 0, 1, NoSync{}, v.data());
+            for (uint32_t i = 0; i < (1u << S1); ++i) modup_tw_put(tl.data(), i, modup_tw_pair(j, t, i));
+            modup_target_pass<S1, true>(j, t, g, tile, lds.data(), 0, 1, NoSync{}, v.data(), tl.data());
         }
     }
 }

@@ -137,6 +137,17 @@ inline NttGeom make_geom(uint32_t n, uint32_t rmax = 3) {
     return g;
 }
 
+// The device scheduler may not move instructions across this point (a
+// scheduling hint only).  Put between a batch of global loads and the work on
+// them: left to itself, the scheduler interleaves each load with its use
+// under register pressure and waits for the last loads one at a time, a
+// memory round trip each.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define GF_SCHED_FENCE() __builtin_amdgcn_sched_barrier(0)
+#else
+#define GF_SCHED_FENCE() ((void)0)
+#endif
+
 // two consecutive words as one 16-byte access
 struct alignas(16) Words2 {
     uint64_t w[2];
@@ -411,24 +422,29 @@ GF_HD uint64_t ntt_final(uint64_t v, const NttJob& j) {
 // forward final store with the optional linear epilogue at global index gi.
 // LZ: v is below 72q; with the epilogue's two lazy products below 80q < 2^64,
 // reduced once (or stored as is for a lazy_out job without epilogue).
+// the epilogue's value for a forward transform's output v and the operands
+// x0 = e0[gi], x1 = e1[gi] (j.e0 set)
 template <bool LZ>
-GF_HD uint64_t ntt_final_ep(uint64_t v, const NttJob& j, uint64_t gi) {
+GF_HD uint64_t ntt_ep_value(uint64_t v, const NttJob& j, uint64_t x0, uint64_t x1) {
     if constexpr (LZ) {
-        if (j.e0) {
-            const uint64_t nq = 0 - j.q;
-            v += shoup_lazy4(gat(gcast(j.e0), gi), j.c0, j.c0p, nq);
-            v += shoup_lazy4(gat(gcast(j.e1), gi), j.c1, j.c1p, nq);
-        } else if (j.lazy_out) {
-            return v;
-        }
+        const uint64_t nq = 0 - j.q;
+        v += shoup_lazy4(x0, j.c0, j.c0p, nq);
+        v += shoup_lazy4(x1, j.c1, j.c1p, nq);
         return reduce_red(v, j.q, (uint32_t)j.red);
     } else {
         uint64_t r = csub(csub(csub(v, 4 * j.q), 2 * j.q), j.q);
-        if (j.e0) {
-            r = addmod(r, shoup(gat(gcast(j.e0), gi), j.c0, j.c0p, j.q), j.q);
-            r = addmod(r, shoup(gat(gcast(j.e1), gi), j.c1, j.c1p, j.q), j.q);
-        }
-        return r;
+        r = addmod(r, shoup(x0, j.c0, j.c0p, j.q), j.q);
+        return addmod(r, shoup(x1, j.c1, j.c1p, j.q), j.q);
+    }
+}
+template <bool LZ>
+GF_HD uint64_t ntt_final_ep(uint64_t v, const NttJob& j, uint64_t gi) {
+    if (j.e0) return ntt_ep_value<LZ>(v, j, gat(gcast(j.e0), gi), gat(gcast(j.e1), gi));
+    if constexpr (LZ) {
+        if (j.lazy_out) return v;
+        return reduce_red(v, j.q, (uint32_t)j.red);
+    } else {
+        return csub(csub(csub(v, 4 * j.q), 2 * j.q), j.q);
     }
 }
 
@@ -512,6 +528,43 @@ GF_HD void wave_units(uint32_t tid, uint32_t nthr, F f) {
     }
 }
 
+// The stores of a forward row tile with an epilogue (j.e0 set), staged in LDS
+// as wave_units visits them: the epilogue operands of 2 of a lane's
+// words are requested before any is used (per word, the loads would
+// otherwise wait in turn behind the previous word's arithmetic, a memory
+// round trip each; 2 at a time keep the kernel's register count).
+template <uint32_t R, uint32_t WORDS, bool LZ>
+GF_HD void row_out_ep(const NttJob& j, GF_GLOBAL uint64_t* dst, uint32_t off, const uint64_t* lds, uint32_t tid,
+                      uint32_t nthr) {
+    const uint32_t W = nthr < 64 ? nthr : 64;
+    const uint32_t lane = tid & (W - 1);
+    constexpr uint32_t U = WORDS >> R, M = 1u << R, MB = M < 2 ? M : 2;
+    const GF_GLOBAL uint64_t* e0 = gcast(j.e0);
+    const GF_GLOBAL uint64_t* e1 = gcast(j.e1);
+    for (uint32_t u0 = tid - lane; u0 < U; u0 += nthr) {
+#pragma unroll
+        for (uint32_t mb = 0; mb < M; mb += MB) {
+            uint64_t x0[MB], x1[MB];
+            GF_SCHED_FENCE();   // this batch's loads stay behind the previous batch's stores
+#pragma unroll
+            for (uint32_t m = 0; m < MB; ++m) {
+                const uint32_t idx = (u0 << R) + (mb + m) * W + lane;
+                if (U % 64 == 0 || idx < WORDS) {
+                    x0[m] = gat(e0, off + idx);
+                    x1[m] = gat(e1, off + idx);
+                }
+            }
+            GF_SCHED_FENCE();
+#pragma unroll
+            for (uint32_t m = 0; m < MB; ++m) {
+                const uint32_t idx = (u0 << R) + (mb + m) * W + lane;
+                if (U % 64 == 0 || idx < WORDS)
+                    gat(dst, off + idx) = ntt_ep_value<LZ>(lds[padded(idx)], j, x0[m], x1[m]);
+            }
+        }
+    }
+}
+
 // A row tile whose first or last pass holds short runs (TL < 8) stages that
 // end through LDS, wave by wave (no block barrier).
 // ROWTAB: the job's row table (j.twr) serves the last chunk; off = a0 << S for
@@ -554,6 +607,12 @@ GF_HD void row_tile(const NttJob& j, const TileDesc& d, const GF_GLOBAL uint64_t
     }
     if constexpr (!out_direct && !KEEP) {
         sync.wave();
+        if constexpr (FIN == 2) {
+            if (j.e0) {
+                row_out_ep<PL::R, WORDS, LZ>(j, dst, off, lds, tid, nthr);
+                return;
+            }
+        }
         wave_units<PL::R, WORDS>(tid, nthr, [&](uint32_t idx) { gout(0, idx, 0, lds[padded(idx)]); });
     }
 }

@@ -35,17 +35,6 @@ namespace gf {
 #define GF_NO_CONTRACT
 #endif
 
-// The device scheduler may not move instructions across this point (a
-// scheduling hint only).  The fp64 passes put it between a unit's global loads
-// and their conversions: left to itself, the scheduler interleaves each load
-// with its conversion under register pressure and waits for the last loads
-// one at a time, a memory round trip each.
-#if defined(__HIP_DEVICE_COMPILE__)
-#define GF_SCHED_FENCE() __builtin_amdgcn_sched_barrier(0)
-#else
-#define GF_SCHED_FENCE() ((void)0)
-#endif
-
 // An input accessor with a raw form (HasRaw): raw(s, k, at) fetches the word,
 // conv(w) turns it into the pass's double
 template <class A, class = void>

@@ -286,6 +286,25 @@ GF_HD void modup_tw_put(uint64_t* tl, uint32_t i, const Words2& p) {
     tl[2 * i] = p.w[0];
     tl[2 * i + 1] = p.w[1];
 }
+// A column pass's first forward pass (chunk 0) takes twiddles 1, 2, 3 in
+// every thread: read through the constant address space (scalar loads, no
+// VGPRs, requested early)
+struct TwUniform {
+    static constexpr bool kRows = false;
+    const uint64_t* t;
+    GF_HD void get(uint32_t i, uint64_t& w, uint64_t& wp) const {
+        w = uniform_at(t, 2 * i);
+        wp = uniform_at(t, 2 * i + 1);
+    }
+};
+struct TwUniformF64 {
+    static constexpr bool kRows = false;
+    const uint64_t* t;
+    GF_HD void get(uint32_t i, double& w, double& wq) const {
+        w = f64_of_bits(uniform_at(t, 2 * i));
+        wq = f64_of_bits(uniform_at(t, 2 * i + 1));
+    }
+};
 struct TwLds {
     static constexpr bool kRows = false;
     const uint64_t* t;
@@ -321,7 +340,7 @@ GF_HD void modup_target_pass(const ModupJob& j, uint32_t t, const NttGeom& g, ui
         const ColOutF64 gout{gcast(j.dst + (size_t)t * g.n), g.logN2, tile * C, q, qinv};
         const UnitInF64<ALL, MODUP_UNIT> in{cv};
         tile_pass_f64<true, S1, P::ST0, P::R, f64_center_after<true, S1, MODUP_RM>(0)>(
-            d, TwF64{gcast((const double*)(uintptr_t)twd)}, q, qinv, tid, nthr, in, LdsOutF64{lds});
+            d, TwUniformF64{(const uint64_t*)(uintptr_t)twd}, q, qinv, tid, nthr, in, LdsOutF64{lds});
         sync();
         tile_chunks_f64<true, S1, MODUP_RM, 1>(lds, d, TwLdsF64{tl}, TwLdsF64{tl}, q, qinv, tid, nthr, sync, in, gout);
         return;
@@ -329,7 +348,7 @@ GF_HD void modup_target_pass(const ModupJob& j, uint32_t t, const NttGeom& g, ui
     NttJob jt{};
     jt.q = uniform_at(j.tq, t);
     const ColOut<true> gout{gcast(j.dst + (size_t)t * g.n), g.logN2, tile * C, jt};
-    const TwPaired tw{gcast((const uint64_t*)(uintptr_t)uniform_at(j.ttw, t))};
+    const TwUniform tw{(const uint64_t*)(uintptr_t)uniform_at(j.ttw, t)};
     const UnitIn<ALL, MODUP_UNIT> in{cv};
     auto run = [&](auto LZc) {
         constexpr bool LZ = decltype(LZc)::value;

@@ -8,15 +8,29 @@ namespace gf {
 // time, so every thread's unit of the sources stays in registers.
 // sources I .. SP - 1, one after the other (compile-time source index, so the
 // unit arrays are indexed by constants and stay in registers); source I is
-// phase I and takes LDS tile I mod 2
+// phase I and takes LDS tile I mod 2 and twiddle tile I mod 2; its twiddles
+// are requested with its first pass's data
 template <uint32_t S1, uint32_t SP, uint32_t I>
-__device__ __forceinline__ void modup_sources(const ModupJob& j, const NttGeom& g, uint64_t* lds, uint32_t tid,
-                                              uint32_t tile, ModupUnit<SP>& x) {
+__device__ __forceinline__ void modup_sources(const ModupJob& j, const NttGeom& g, uint64_t* lds,
+                                              uint64_t (*twl)[2u << S1], uint32_t tid, uint32_t tile,
+                                              ModupUnit<SP>& x) {
     if constexpr (I < SP) {
+        constexpr uint32_t TWP = 1u << S1, TWPER = TWP > MODUP_THREADS ? TWP / MODUP_THREADS : 1;
+        Words2 tp[TWPER];
+#pragma unroll
+        for (uint32_t k = 0; k < TWPER; ++k)
+            if (tid + k * MODUP_THREADS < TWP) tp[k] = modup_src_tw_pair(j, I, tid + k * MODUP_THREADS);
+        uint64_t* tl = twl[I & 1];
         uint64_t v[MODUP_UNIT];
-        modup_source_pass<S1, false>(j, I, g, tile, lds + (I & 1) * MODUP_TILE, tid, MODUP_THREADS, DevSync{}, v);
+        modup_source_pass<S1, false>(j, I, g, tile, lds + (I & 1) * MODUP_TILE, tid, MODUP_THREADS, DevSync{}, v, tl,
+                                     [&] {
+#pragma unroll
+                                         for (uint32_t k = 0; k < TWPER; ++k)
+                                             if (tid + k * MODUP_THREADS < TWP)
+                                                 modup_tw_put(tl, tid + k * MODUP_THREADS, tp[k]);
+                                     });
         modup_take_unit<SP, S1>(j, I, v, x);
-        modup_sources<S1, SP, I + 1>(j, g, lds, tid, tile, x);
+        modup_sources<S1, SP, I + 1>(j, g, lds, twl, tid, tile, x);
     }
 }
 
@@ -25,9 +39,10 @@ __device__ __forceinline__ void modup_sources(const ModupJob& j, const NttGeom& 
 template <uint32_t S1, uint32_t SP, bool RS, bool C60>
 __global__ __launch_bounds__(MODUP_THREADS) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_modup(const ModupJob* __restrict__ jobs, uint32_t off, NttGeom g) {
     __shared__ uint64_t lds[2 * MODUP_TILE];   // phases alternate between the two tiles
-    // the targets' column-pass twiddles (modup_tw_pair), by target parity as
-    // the tiles: target t + 2 rewrites target t's copy only after a barrier of
-    // target t + 1, which every thread reaches after its last read of it
+    // the phases' column-pass twiddles (modup_src_tw_pair, modup_tw_pair), by
+    // phase parity as the tiles: phase p + 2 rewrites phase p's copy only
+    // after a barrier of phase p + 1, which every thread reaches after its
+    // last read of it
     constexpr uint32_t TWP = 1u << S1, TWPER = TWP > MODUP_THREADS ? TWP / MODUP_THREADS : 1;
     __shared__ uint64_t twl[2][2 * TWP];
     const ModupJob& j = jobs[off + blockIdx.y];
@@ -40,7 +55,7 @@ __global__ __launch_bounds__(MODUP_THREADS) __attribute__((amdgpu_waves_per_eu(4
     const uint32_t nt = gridDim.x;
     const uint32_t tile = nt % 8 ? blockIdx.x : (blockIdx.x % 8) * (nt / 8) + blockIdx.x / 8;
     ModupUnit<SP> x;
-    modup_sources<S1, SP, 0>(j, g, lds, tid, tile, x);
+    modup_sources<S1, SP, 0>(j, g, lds, twl, tid, tile, x);
     // target t is phase SP + t; its twiddles are requested before its
     // conversion and stored to LDS after it
     uint64_t cv[MODUP_UNIT];

@@ -154,24 +154,93 @@ GF_HD constexpr bool modup_chunk0_fits() {
            ((1u << (MODUP_LOG_TILE - S1)) << (S1 - MODUP_RM)) == MODUP_THREADS;
 }
 
+// A target's column-pass twiddles: every column of a column pass takes the
+// same ones, the first 2^S1 pairs (16 bytes each, fp64 or integer) of the
+// target's forward table.  The kernel requests its share of them before the
+// target's conversion and stores them into an LDS tile (tl) before the first
+// pass; the passes behind the first block barrier read them from there.
+GF_HD Words2 modup_tw_pair(const ModupJob& j, uint32_t t, uint32_t i) {
+    const uint64_t twd = uniform_at(j.ttwd, t);
+    const uint64_t* tab = twd ? (const uint64_t*)(uintptr_t)twd : (const uint64_t*)(uintptr_t)uniform_at(j.ttw, t);
+    return gat((const GF_GLOBAL Words2*)gcast(tab), i);
+}
+GF_HD void modup_tw_put(uint64_t* tl, uint32_t i, const Words2& p) {
+    tl[2 * i] = p.w[0];
+    tl[2 * i + 1] = p.w[1];
+}
+// A column pass's first forward pass (chunk 0) takes twiddles 1, 2, 3 in
+// every thread: read through the constant address space (scalar loads, no
+// VGPRs, requested early)
+struct TwUniform {
+    static constexpr bool kRows = false;
+    const uint64_t* t;
+    GF_HD void get(uint32_t i, uint64_t& w, uint64_t& wp) const {
+        w = uniform_at(t, 2 * i);
+        wp = uniform_at(t, 2 * i + 1);
+    }
+};
+struct TwUniformF64 {
+    static constexpr bool kRows = false;
+    const uint64_t* t;
+    GF_HD void get(uint32_t i, double& w, double& wq) const {
+        w = f64_of_bits(uniform_at(t, 2 * i));
+        wq = f64_of_bits(uniform_at(t, 2 * i + 1));
+    }
+};
+struct TwLds {
+    static constexpr bool kRows = false;
+    const uint64_t* t;
+    GF_HD void get(uint32_t i, uint64_t& w, uint64_t& wp) const {
+        w = t[2 * i];
+        wp = t[2 * i + 1];
+    }
+};
+struct TwLdsF64 {
+    static constexpr bool kRows = false;
+    const uint64_t* t;
+    GF_HD void get(uint32_t i, double& w, double& wq) const {
+        w = f64_of_bits(t[2 * i]);
+        wq = f64_of_bits(t[2 * i + 1]);
+    }
+};
+
+// Source i's column-pass twiddles: the first 2^S1 pairs of its inverse table
+// (as modup_tw_pair for the targets)
+GF_HD Words2 modup_src_tw_pair(const ModupJob& j, uint32_t i, uint32_t k) {
+    const uint64_t* tab = j.stwd[i] ? j.stwd[i] : j.stw[i];
+    return gat((const GF_GLOBAL Words2*)gcast(tab), k);
+}
+
 // source i's inverse column pass over the tile (all threads), its last pass
-// (chunk 0) into the unit registers v (see UnitOut)
-template <uint32_t S1, bool ALL, class Sync>
+// (chunk 0) into the unit registers v (see UnitOut).  The first pass reads
+// the table's own twiddles; stage_tw() then stores the source's twiddles
+// into the LDS tile tl, which the passes behind the first barrier read.
+template <uint32_t S1, bool ALL, class Sync, class StageTw>
 GF_HD void modup_source_pass(const ModupJob& j, uint32_t i, const NttGeom& g, uint32_t tile, uint64_t* lds,
-                             uint32_t tid, uint32_t nthr, Sync sync, uint64_t* v) {
+                             uint32_t tid, uint32_t nthr, Sync sync, uint64_t* v, const uint64_t* tl,
+                             StageTw stage_tw) {
     static_assert(modup_chunk0_fits<S1>(), "chunk 0 must be a full radix-4 pass, one unit per thread");
+    using P = PassOf<false, S1, MODUP_RM, 0>;
+    static_assert(P::NC >= 2, "a column pass of two or more passes");
     constexpr uint32_t LOGC = MODUP_LOG_TILE - S1, C = 1u << LOGC;
     const TileDesc d{LOGC, C, 1, 1, 0, true, false};
     if (j.stwd[i]) {
         const double q = j.sqd[i], qinv = j.sqinv[i];
         const ColInF64 gin{gcast(j.src[i]), g.logN2, tile * C, q, qinv, F64_2P52 + 2 * q};
-        const TwF64 tw{gcast((const double*)j.stwd[i])};
-        tile_chunks_f64<false, S1, MODUP_RM, 0>(lds, d, tw, tw, q, qinv, tid, nthr, sync, gin, UnitOutF64<ALL, MODUP_UNIT>{v});
+        tile_pass_f64<false, S1, P::ST0, P::R, f64_center_after<false, S1, MODUP_RM>(0)>(
+            d, TwF64{gcast((const double*)j.stwd[i])}, q, qinv, tid, nthr, gin, LdsOutF64{lds});
+        stage_tw();
+        sync();
+        tile_chunks_f64<false, S1, MODUP_RM, 1>(lds, d, TwLdsF64{tl}, TwLdsF64{tl}, q, qinv, tid, nthr, sync, gin,
+                                                UnitOutF64<ALL, MODUP_UNIT>{v});
         return;
     }
     const ColIn gin{gcast(j.src[i]), g.logN2, tile * C};
-    tile_chunks<false, false, S1, MODUP_RM, 0>(lds, d, TwPaired{gcast(j.stw[i])}, TwPaired{gcast(j.stw[i])}, j.sq[i],
-                                               tid, nthr, sync, gin, UnitOut<ALL, MODUP_UNIT>{v});
+    tile_pass<false, false, S1, P::ST0, P::R>(d, TwPaired{gcast(j.stw[i])}, j.sq[i], tid, nthr, gin, LdsOut{lds});
+    stage_tw();
+    sync();
+    tile_chunks<false, false, S1, MODUP_RM, 1>(lds, d, TwLds{tl}, TwLds{tl}, j.sq[i], tid, nthr, sync, gin,
+                                               UnitOut<ALL, MODUP_UNIT>{v});
 }
 // source i's unit v (its inverse transform's last pass) into the
 // conversion's registers
@@ -271,56 +340,6 @@ GF_HD void modup_rs_unit(const ModupJob& j, uint32_t t, ModupUnit<SP>& x, const 
         cv[u] = f64 ? bits_of_f64(f64_center(f64_from_u52(v, F64_2P52 + 2 * qd), qd, qinv)) : v;
     }
 }
-
-// A target's column-pass twiddles: every column of a column pass takes the
-// same ones, the first 2^S1 pairs (16 bytes each, fp64 or integer) of the
-// target's forward table.  The kernel requests its share of them before the
-// target's conversion and stores them into an LDS tile (tl) before the first
-// pass; the passes behind the first block barrier read them from there.
-GF_HD Words2 modup_tw_pair(const ModupJob& j, uint32_t t, uint32_t i) {
-    const uint64_t twd = uniform_at(j.ttwd, t);
-    const uint64_t* tab = twd ? (const uint64_t*)(uintptr_t)twd : (const uint64_t*)(uintptr_t)uniform_at(j.ttw, t);
-    return gat((const GF_GLOBAL Words2*)gcast(tab), i);
-}
-GF_HD void modup_tw_put(uint64_t* tl, uint32_t i, const Words2& p) {
-    tl[2 * i] = p.w[0];
-    tl[2 * i + 1] = p.w[1];
-}
-// A column pass's first forward pass (chunk 0) takes twiddles 1, 2, 3 in
-// every thread: read through the constant address space (scalar loads, no
-// VGPRs, requested early)
-struct TwUniform {
-    static constexpr bool kRows = false;
-    const uint64_t* t;
-    GF_HD void get(uint32_t i, uint64_t& w, uint64_t& wp) const {
-        w = uniform_at(t, 2 * i);
-        wp = uniform_at(t, 2 * i + 1);
-    }
-};
-struct TwUniformF64 {
-    static constexpr bool kRows = false;
-    const uint64_t* t;
-    GF_HD void get(uint32_t i, double& w, double& wq) const {
-        w = f64_of_bits(uniform_at(t, 2 * i));
-        wq = f64_of_bits(uniform_at(t, 2 * i + 1));
-    }
-};
-struct TwLds {
-    static constexpr bool kRows = false;
-    const uint64_t* t;
-    GF_HD void get(uint32_t i, uint64_t& w, uint64_t& wp) const {
-        w = t[2 * i];
-        wp = t[2 * i + 1];
-    }
-};
-struct TwLdsF64 {
-    static constexpr bool kRows = false;
-    const uint64_t* t;
-    GF_HD void get(uint32_t i, double& w, double& wq) const {
-        w = f64_of_bits(t[2 * i]);
-        wq = f64_of_bits(t[2 * i + 1]);
-    }
-};
 
 // target t's forward column pass from the unit registers cv (see UnitIn),
 // stored to HBM unreduced (the digits' row pass continues from these

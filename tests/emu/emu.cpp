@@ -285,7 +285,9 @@ void emu_modup_t(const ModupJob& j, const NttGeom& g) {
     std::vector<ModupUnit<SP>> units(MODUP_THREADS);
     for (uint32_t tile = 0; tile < g.n / MODUP_TILE; ++tile) {
         for (uint32_t i = 0; i < SP; ++i) {
-            modup_source_pass<S1, true>(j, i, g, tile, lds.data(), 0, 1, NoSync{}, v.data());
+            modup_source_pass<S1, true>(j, i, g, tile, lds.data(), 0, 1, NoSync{}, v.data(), tl.data(), [&] {
+                for (uint32_t k = 0; k < (1u << S1); ++k) modup_tw_put(tl.data(), k, modup_src_tw_pair(j, i, k));
+            });
             for (uint32_t u = 0; u < MODUP_THREADS; ++u)
                 modup_take_unit<SP, S1>(j, i, v.data() + (size_t)u * MODUP_UNIT, units[u]);
         }
@@ -304,7 +306,9 @@ void emu_moddown_t(const ModupJob& j, const NttGeom& g) {
     std::vector<uint64_t> dv((size_t)MODUP_THREADS * MODUP_UNIT);
     for (uint32_t tile = 0; tile < g.n / MODUP_TILE; ++tile) {
         for (uint32_t i = 0; i < SP; ++i) {
-            modup_source_pass<S1, true>(j, i, g, tile, lds.data(), 0, 1, NoSync{}, v.data());
+            modup_source_pass<S1, true>(j, i, g, tile, lds.data(), 0, 1, NoSync{}, v.data(), tl.data(), [&] {
+                for (uint32_t k = 0; k < (1u << S1); ++k) modup_tw_put(tl.data(), k, modup_src_tw_pair(j, i, k));
+            });
             for (uint32_t u = 0; u < MODUP_THREADS; ++u)
                 modup_take_unit<SP, S1>(j, i, v.data() + (size_t)u * MODUP_UNIT, units[u]);
         }

@@ -389,6 +389,35 @@ def test_plan_knobs_keep_results(G, ksi_extra, conv_chunk, rs_chunk, modup_fused
             assert rep["ksi_f64"]["chosen"] == ctx.knob("ksi_f64")
 
 
+def test_fused_kernels_largest_ring(G):
+    """The fused ModUp, the fused ModDown and the fused row pass + inner
+    product (fp64 passes on) at the largest two-phase ring, n = 2^17 (a
+    512-row column pass: each thread stages two twiddle pairs per phase),
+    forced on for a small batch of uneven levels: the oracle's results."""
+    n, L, dnum = 1 << 17, 4, 2
+    P, ctx, k, av, bv = setup(G, n, L, dnum, seed=91)
+    for name, v in (("modup_fused", 1), ("modup_f64", 1), ("ksi_f64", 1), ("moddown_fused", 1),
+                    ("modup_min_wgs", 0), ("rechunk_wgs", 0)):
+        ctx.set_knob(name, v)
+    rng = np.random.default_rng(17)
+    levels = [4, 3]
+    ins = [[G.Dev.of(rand_ct(P, lv, rng)) for _ in range(4)] for lv in levels]
+    outs = [[G.Dev((lv - 1) * n), G.Dev((lv - 1) * n)] for lv in levels]
+    wss = [ctx.workspace() for _ in levels]
+    col = lambda j: [x[j].ptr for x in ins]  # noqa: E731
+    plan = G.plan_evalmult_rescale(ctx, k, levels, col(0), col(1), col(2), col(3),
+                                   [o[0].ptr for o in outs], [o[1].ptr for o in outs], [w.ptr for w in wss])
+    kinds = [kd for kd, _ in plan.steps()]
+    assert "modup" in kinds and "moddown" in kinds and "ksi" in kinds, kinds
+    s = G.Stream()
+    plan.run_graph(s)
+    s.sync()
+    for i, lv in enumerate(levels):
+        w0, w1 = P.evalmult_rescale(lv, *[x.download() for x in ins[i]], av, bv)
+        assert np.array_equal(outs[i][0].download(), w0), (i, lv)
+        assert np.array_equal(outs[i][1].download(), w1), (i, lv)
+
+
 def test_rotate_plan_multi_keys(G):
     n, L, dnum = 1 << 12, 5, 5
     P = make_params(n, L, dnum)

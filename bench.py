@@ -57,7 +57,7 @@ STEP_KERNELS = {"ntt_fwd": ("k_ntt_A", "k_ntt_B"), "ntt_inv": ("k_ntt_B", "k_ntt
                 "add": ("k_add",), "perm": ("k_perm",), "copy": ("k_copy",),
                 "rsv": ("k_rs",), "ptacc": ("k_ptacc",), "ntt_fwd_a": ("k_ntt_A",), "ksi": ("k_ksi",),
                 "ntt_inv_b": ("k_ntt_B",), "modup": ("k_modup",), "moddown": ("k_modup",),
-                "ntt_fwd_b": ("k_ntt_B",)}
+                "ntt_fwd_b": ("k_ntt_B",), "tensor_ib": ("k_tensor_ib",)}
 
 
 def kernel_bytes(name, kind, nkern, total, epilogue):
@@ -80,6 +80,11 @@ def step_bytes(n, L, K, alpha, np_parts, kinds, batch):
     by a fused transform epilogue)."""
     T = 8 * n
     key = 2 * np_parts / batch   # key rows read per tower row, per op
+    # the tensor product fused with the inverse row pass of its t2 is charged
+    # as the two steps it replaces
+    tensor_ib = kinds[:1] == ["tensor_ib"]
+    if tensor_ib:
+        kinds = ["tensor", "ntt_inv_b"] + list(kinds[1:])
     sc = [L - min(alpha, L - alpha * p) + K for p in range(np_parts)]
     sp = [min(alpha, L - alpha * p) for p in range(np_parts)]
     if "ksi" in kinds:
@@ -114,9 +119,9 @@ def step_bytes(n, L, K, alpha, np_parts, kinds, batch):
         # and targets once) and the digits' column passes (half of theirs)
         assert kinds[:6] == ["tensor", "ntt_inv_b", "ntt_inv", "modup", "ksi", "ksi"], kinds
         assert "moddown" in kinds or kinds[6:] == ["ntt_inv", "rsv", "ntt_fwd"], kinds
-        return [("tensor", 7 * T * L, 0),
-                ("intt_modup_rowpass", T * L, 0),
-                ("intt_extra", 2 * T * 2, 0),
+        head = [("tensor_intt_rowpass", 7 * T * L + T * L, 0)] if tensor_ib else \
+            [("tensor", 7 * T * L, 0), ("intt_modup_rowpass", T * L, 0)]
+        return head + [("intt_extra", 2 * T * 2, 0),
                 ("modup_fused", T * (L + (sum(sp) + sum(sc)) + sum(sc)), 0)] + ksi + tail
     if "ksi" in kinds or len(kinds) == 8:   # fused ModDown + rescale plan
         if "ksi" in kinds:
@@ -706,7 +711,7 @@ def main():
     else:
         radix, ntt_ms = ctx.autotune(stream, args.batch)   # NTT register radix, then the plan knobs at B
     knobs = {k_: ctx.knob(k_) for k_ in ("ntt_radix", "ksi_extra", "conv_chunk", "rs_chunk", "modup_fused",
-                                          "modup_f64", "ksi_f64", "moddown_fused")}
+                                          "modup_f64", "ksi_f64", "moddown_fused", "tensor_ib")}
     log("context ready: K=%d alpha=%d ws=%.1f MiB/op, knobs %s (%s)"
         % (K, alpha, ctx.ws_bytes / 2**20, knobs, "pinned" if pinned else "autotuned"))
     tuned = {"shape": {"n": N, "L": L, "dnum": DNUM}, "chosen": knobs, "pinned": pinned,

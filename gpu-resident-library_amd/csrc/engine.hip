@@ -410,9 +410,9 @@ void Ctx::set_knob(const std::string& name, uint32_t v) {
     } else if (name == "modup_fused") {
         if (v > 1) throw std::runtime_error("gpufhe: modup_fused must be 0 or 1");
         modup_fused = v;
-    } else if (name == "modup_f64" || name == "ksi_f64" || name == "moddown_fused") {
+    } else if (name == "modup_f64" || name == "ksi_f64" || name == "moddown_fused" || name == "tensor_ib") {
         if (v > 1) throw std::runtime_error("gpufhe: " + name + " must be 0 or 1");
-        (name == "modup_f64" ? modup_f64 : name == "ksi_f64" ? ksi_f64 : moddown_fused) = v;
+        (name == "modup_f64" ? modup_f64 : name == "ksi_f64" ? ksi_f64 : name == "tensor_ib" ? tensor_ib : moddown_fused) = v;
     } else if (name == "modup_min_wgs") {
         modup_min_wgs = v;
     } else if (name == "rechunk_wgs") {
@@ -429,6 +429,7 @@ uint32_t Ctx::knob(const std::string& name) const {
     if (name == "modup_fused") return modup_fused;
     if (name == "modup_f64") return modup_f64;
     if (name == "ksi_f64") return ksi_f64;
+    if (name == "tensor_ib") return tensor_ib;
     if (name == "moddown_fused") return moddown_fused;
     if (name == "modup_min_wgs") return modup_min_wgs;
     if (name == "rechunk_wgs") return rechunk_wgs;
@@ -530,6 +531,8 @@ void Ctx::autotune_plan(hipStream_t s, uint32_t B) {
         if (modup_f64) tune("ksi_f64", ksi_f64, {0, 1}, {Step::KSI});
         modup_fused = mf;
         tune("modup_fused", modup_fused, {0, 1}, {Step::NTT_IB, Step::NTT_I, Step::MODUP, Step::CONV, Step::NTT_FA});
+        // the tensor product fused with the row pass in front of the fused ModUp
+        if (modup_fused) tune("tensor_ib", tensor_ib, {0, 1}, {Step::TENSOR, Step::NTT_IB, Step::TENSOR_IB});
     }
     if (fused) {
         // the prologue's chunk width matters where the ModDown runs apart:
@@ -880,6 +883,7 @@ void Plan::run_jobs(size_t i, hipStream_t s, uint32_t which, uint32_t first, uin
         case Step::NTT_F: launch_ntt(GF_J(NttJob), cnt, geom, true, s, which); break;
         case Step::NTT_I: launch_ntt(GF_J(NttJob), cnt, geom, false, s, which); break;
         case Step::TENSOR: launch_tensor(GF_J(TensorJob), cnt, n, s); break;
+        case Step::TENSOR_IB: launch_tensor_ib(GF_J(TensorIbJob), cnt, geom, s); break;
         case Step::CONV: launch_conv(GF_J(ConvJob), cnt, n, s); break;
         case Step::INNER: launch_inner(GF_J(InnerJob), cnt, n, s); break;
         case Step::COMBINE: launch_combine(GF_J(CombineJob), cnt, n, s); break;
@@ -1640,6 +1644,22 @@ std::unique_ptr<Plan> PlanBuilder::layout(std::vector<char>& blob) {
         P->steps.push_back(Step{kind, off, (uint32_t)cnt});
     };
 #define GF_STAGE(k, v) stage(Step::k, v.data(), v.size(), sizeof(v[0]))
+    // the tensor product fused with the inverse row pass of its t2 where every
+    // tensor job has that row pass (the fused EvalMult+Rescale with the fused
+    // ModUp): t2 goes to the row pass in LDS instead of through HBM
+    if (C_.tensor_ib && !C_.geom.single && C_.geom.R * (1u << C_.geom.logN2) == NTT_TILE && !tensor_.empty() &&
+        tensor_.size() == intt_upb_.size()) {
+        std::map<const uint64_t*, size_t> by_in;
+        for (size_t i = 0; i < intt_upb_.size(); ++i) by_in[intt_upb_[i].in] = i;
+        bool all = true;
+        for (const TensorJob& t : tensor_) all &= t.pairs && by_in.count(t.t2) == 1;
+        if (all && by_in.size() == tensor_.size()) {
+            for (const TensorJob& t : tensor_) tensor_ib_.push_back(TensorIbJob{t, intt_upb_[by_in[t.t2]]});
+            tensor_.clear();
+            intt_upb_.clear();
+        }
+    }
+    GF_STAGE(TENSOR_IB, tensor_ib_);
     GF_STAGE(TENSOR, tensor_);
     GF_STAGE(PERM, perm_);
     GF_STAGE(NTT_IB, intt_upb_);

@@ -146,6 +146,9 @@ public:
     // as well (fp64 butterflies issue in about half the cycles of the integer
     // Shoup butterflies)
     uint32_t ksi_f64 = 1;
+    // 1: the tensor product and the inverse row pass of its t2 (in front of
+    // the fused ModUp) run as one kernel per row tile, t2 handed over in LDS
+    uint32_t tensor_ib = 1;
     // 1: the fused EvalMult+Rescale runs the P towers' inverse column pass,
     // the rescale prologue and the surviving towers' forward column pass as
     // one kernel (the fused ModUp kernel's rescale mode), where the fused
@@ -164,7 +167,7 @@ public:
     void autotune_plan(hipStream_t s, uint32_t batch);
     std::string tune_report;
     // sets or reads a knob by name ("ntt_radix", "ksi_extra", "conv_chunk",
-    // "rs_chunk", "modup_fused", "modup_f64", "ksi_f64", "moddown_fused",
+    // "rs_chunk", "modup_fused", "modup_f64", "ksi_f64", "moddown_fused", "tensor_ib",
     // "modup_min_wgs", "rechunk_wgs")
     void set_knob(const std::string& name, uint32_t v);
     uint32_t knob(const std::string& name) const;
@@ -204,8 +207,9 @@ struct Step {
     // row pass of the ModUp digits and the key inner product
     // NTT_IB: an inverse transform's first (row) pass only; MODUP: the fused
     // ModUp (modup.hpp), arg = the jobs' source count
+    // TENSOR_IB: the tensor product fused with the inverse row pass of its t2
     enum Kind { NTT_F, NTT_I, TENSOR, CONV, INNER, COMBINE, SWITCH, FUSE, ADD, PERM, COPY, RSV, PTACC,
-                NTT_FA, KSI, NTT_IB, MODUP, MODDOWN, NTT_FB } kind;
+                NTT_FA, KSI, NTT_IB, MODUP, MODDOWN, NTT_FB, TENSOR_IB } kind;
     size_t off;     // byte offset of the first job in the plan's device buffer
     uint32_t count;
     // KSI: LDS regions (bits 0-7) and waves per workgroup (bits 8-15); the
@@ -283,6 +287,7 @@ private:
     const Ctx& C_;
     // stage lists in execution order
     std::vector<TensorJob> tensor_;
+    std::vector<TensorIbJob> tensor_ib_;
     std::vector<PermJob> perm_;
     std::vector<NttJob> intt_up_;
     std::vector<NttJob> intt_upb_;   // inverse transforms whose column pass runs in the fused ModUp

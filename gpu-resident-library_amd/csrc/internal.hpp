@@ -14,6 +14,7 @@
 #include "modarith.hpp"
 #include "modup.hpp"
 #include "ntt_engine.hpp"
+#include "tensor.hpp"
 
 namespace gf {
 
@@ -22,14 +23,6 @@ enum { MAXP = 16, PW_THREADS = 256, PW_EPT = 4, CONV_MAX_SRC = 16 };
 // per-modulus constants (row = index in the Q|P modulus list of a context)
 struct RowConst {
     uint64_t q, qneg, r2, r1, ninv, ninvp;
-};
-
-// t0 = a0 b0, t1 = a0 b1 + a1 b0, t2 = a1 b1 (one tower)
-struct TensorJob {
-    const uint64_t *c0a, *c1a, *c0b, *c1b;
-    uint64_t *t0, *t1, *t2;
-    uint64_t q, qneg, r2;
-    uint32_t pairs;   // 1: every pointer 16-byte aligned (two residues per lane and access)
 };
 
 // dst[j] = (sum_i src[i] * tab[i*ntgt + j]) * R^-1 mod p_j for targets j in
@@ -235,6 +228,7 @@ void launch_ptacc(const PtAccJob* d, uint32_t nj, uint32_t n, hipStream_t s);
 // fused digit row pass + key inner product: regions = LDS regions (at least every
 // job's transformed-part count), waves = workgroup size in waves (1..KSI_WAVES)
 void launch_ksi(const KsiJob* d, uint32_t nj, const NttGeom& g, uint32_t regions, uint32_t waves, hipStream_t s);
+void launch_tensor_ib(const TensorIbJob* d, uint32_t nj, const NttGeom& g, hipStream_t s);
 // the fused ModUp over jobs of sp sources each
 void launch_modup(const ModupJob* d, uint32_t nj, const NttGeom& g, uint32_t arg, hipStream_t s);
 void launch_moddown(const ModupJob* d, uint32_t nj, const NttGeom& g, uint32_t sp, hipStream_t s);

@@ -43,14 +43,6 @@ struct alignas(16) Pair {
 __device__ __forceinline__ Pair ld2(const uint64_t* p, uint32_t i) { return ga((const Pair*)p, i / 2); }
 __device__ __forceinline__ void st2(uint64_t* p, uint32_t i, const Pair& x) { ga((Pair*)p, i / 2) = x; }
 
-__device__ __forceinline__ void tensor_one(const TensorJob& j, uint64_t a0, uint64_t a1, uint64_t b0,
-                                           uint64_t b1, uint64_t& t0, uint64_t& t1, uint64_t& t2) {
-    t0 = mulmod(a0, b0, j.q, j.qneg, j.r2);
-    const U128 m = add128(mul128(a0, b1), mul128(a1, b0));
-    t1 = redc(mul128(redc(m, j.q, j.qneg), j.r2), j.q, j.qneg);
-    t2 = mulmod(a1, b1, j.q, j.qneg, j.r2);
-}
-
 __device__ __forceinline__ void tensor_body(const TensorJob& j, uint32_t n) {
     if (j.pairs) {
         PW2_FOR(n) {

@@ -570,9 +570,11 @@ GF_HD void row_out_ep(const NttJob& j, GF_GLOBAL uint64_t* dst, uint32_t off, co
 // ROWTAB: the job's row table (j.twr) serves the last chunk; off = a0 << S for
 // the tile's first row a0.  KEEP: the tile's values stay in LDS after the last
 // pass (unreduced, element idx of the tile at lds[padded(idx)]) for a consumer
-// in the same workgroup, and dst is unused.
+// in the same workgroup, and dst is unused.  IN_LDS: the tile's input is
+// already in LDS in that layout (written by the workgroup before a block
+// barrier), and src is unused.
 template <bool FWD, bool LZ, bool PAIRED, uint32_t S, uint32_t RM, uint32_t ROWS, int FIN, bool ROWTAB,
-          class Sync, bool KEEP = false>
+          class Sync, bool KEEP = false, bool IN_LDS = false>
 GF_HD void row_tile(const NttJob& j, const TileDesc& d, const GF_GLOBAL uint64_t* src, GF_GLOBAL uint64_t* dst,
                     uint32_t off, uint64_t* lds, uint32_t tid, uint32_t nthr, Sync sync) {
     constexpr uint32_t N = 1u << S, NC = num_chunks(S, RM), WORDS = ROWS * N;
@@ -581,7 +583,8 @@ GF_HD void row_tile(const NttJob& j, const TileDesc& d, const GF_GLOBAL uint64_t
     constexpr bool in_direct = P0::wide_runs;
     constexpr bool out_direct = PL::wide_runs;
     static_assert(!KEEP || !out_direct, "a kept tile ends in LDS");
-    if constexpr (!in_direct) {
+    static_assert(!IN_LDS || !in_direct, "a tile given in LDS starts with a staged pass");
+    if constexpr (!in_direct && !IN_LDS) {
         wave_units<P0::R, WORDS>(tid, nthr, [&](uint32_t idx) { lds[padded(idx)] = gat(src, off + idx); });
         sync.wave();
     }
@@ -618,7 +621,8 @@ GF_HD void row_tile(const NttJob& j, const TileDesc& d, const GF_GLOBAL uint64_t
 }
 
 // Phase B: tile = rows [a0, a0+R), each a contiguous run of N2 = 2^S2.
-template <bool FWD, bool LZ, bool PAIRED, uint32_t S2, uint32_t RM, class Sync>
+// IN_LDS (inverse): the tile's input is already in LDS (row_tile).
+template <bool FWD, bool LZ, bool PAIRED, uint32_t S2, uint32_t RM, class Sync, bool IN_LDS = false>
 GF_HD void phaseB_body(const NttJob& j, const NttGeom& g, uint32_t tile, uint64_t* lds,
                        uint32_t tid, uint32_t nthr, Sync sync) {
     constexpr uint32_t N2 = 1u << S2, logR = NTT_LOG_TILE - S2, R = 1u << logR;
@@ -627,8 +631,8 @@ GF_HD void phaseB_body(const NttJob& j, const NttGeom& g, uint32_t tile, uint64_
     const TileDesc d{logR, 1, padded(N2), N1 + a0, 1, false, true};
     // forward: phase B is the second phase (reads out); inverse: the first.
     // Inverse phase B hands lazy values in [0,4q) to phase A.
-    row_tile<FWD, LZ, PAIRED, S2, RM, R, FWD ? 2 : 0, PAIRED>(j, d, gcast(FWD ? j.out : j.in), gcast(j.out), a0 << S2,
-                                                          lds, tid, nthr, sync);
+    row_tile<FWD, LZ, PAIRED, S2, RM, R, FWD ? 2 : 0, PAIRED, Sync, false, IN_LDS>(
+        j, d, gcast(FWD ? j.out : j.in), gcast(j.out), a0 << S2, lds, tid, nthr, sync);
 }
 
 // Single phase: the whole tower (n = 2^S <= NTT_SINGLE_MAX) is one tile.

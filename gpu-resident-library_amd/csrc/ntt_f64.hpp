@@ -345,7 +345,7 @@ struct LdsOutU52 {
 //     [0, 2q));
 //   inverse: a phase B, inputs below 4q, its output in [0, 2q) to dst for
 //     phase A.
-template <bool FWD, uint32_t S, uint32_t RM, uint32_t ROWS, bool KEEP, class Sync>
+template <bool FWD, uint32_t S, uint32_t RM, uint32_t ROWS, bool KEEP, class Sync, bool IN_LDS = false>
 GF_HD void row_tile_f64(const uint64_t* twd, const uint64_t* twrd, double q, double qinv, const TileDesc& d,
                         const GF_GLOBAL uint64_t* src, GF_GLOBAL uint64_t* dst, uint32_t off, uint64_t* lds,
                         uint32_t tid, uint32_t nthr, Sync sync) {
@@ -356,7 +356,8 @@ GF_HD void row_tile_f64(const uint64_t* twd, const uint64_t* twrd, double q, dou
     static_assert(FWD == KEEP, "fp64 row tiles: forward kept in LDS, or inverse to HBM");
     static_assert(KEEP ? !out_direct : out_direct, "fp64 row tile shape");
     const double c2 = F64_2P52 + (FWD ? q : 2 * q);
-    if constexpr (!in_direct) {
+    static_assert(!IN_LDS || !in_direct, "a tile given in LDS starts with a staged pass");
+    if constexpr (!in_direct && !IN_LDS) {
         wave_units<P0::R, WORDS>(tid, nthr, [&](uint32_t idx) { lds[padded(idx)] = gat(src, off + idx); });
         sync.wave();
     }
@@ -375,16 +376,17 @@ GF_HD void row_tile_f64(const uint64_t* twd, const uint64_t* twrd, double q, dou
 }
 
 // Inverse phase B in fp64 (tile = rows [a0, a0 + R), as phaseB_body) for a
-// job with fp64 tables: output in [0, 2q) for phase A.
-template <uint32_t S2, uint32_t RM, class Sync>
+// job with fp64 tables: output in [0, 2q) for phase A.  IN_LDS: the tile's
+// input is already in LDS (row_tile).
+template <uint32_t S2, uint32_t RM, class Sync, bool IN_LDS = false>
 GF_HD void phaseB_inv_f64(const NttJob& j, const NttGeom& g, uint32_t tile, uint64_t* lds, uint32_t tid,
                           uint32_t nthr, Sync sync) {
     constexpr uint32_t N2 = 1u << S2, logR = NTT_LOG_TILE - S2, R = 1u << logR;
     const uint32_t N1 = 1u << g.logN1;
     const uint32_t a0 = tile * R;
     const TileDesc d{logR, 1, padded(N2), N1 + a0, 1, false, true};
-    row_tile_f64<false, S2, RM, R, false>(j.twd, j.twrd, j.fq, j.fqinv, d, gcast(j.in), gcast(j.out), a0 << S2, lds,
-                                          tid, nthr, sync);
+    row_tile_f64<false, S2, RM, R, false, Sync, IN_LDS>(j.twd, j.twrd, j.fq, j.fqinv, d, gcast(j.in), gcast(j.out),
+                                                         a0 << S2, lds, tid, nthr, sync);
 }
 
 }  // namespace gf

@@ -365,7 +365,8 @@ class Plan:
 
 
 STEP_KINDS = ["ntt_fwd", "ntt_inv", "tensor", "conv", "inner", "combine", "switch", "fuse", "add",
-              "perm", "copy", "rsv", "ptacc", "ntt_fwd_a", "ksi", "ntt_inv_b", "modup", "moddown", "ntt_fwd_b"]
+              "perm", "copy", "rsv", "ptacc", "ntt_fwd_a", "ksi", "ntt_inv_b", "modup", "moddown", "ntt_fwd_b",
+              "tensor_ib"]
 
 
 class Event:

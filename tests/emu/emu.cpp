@@ -12,6 +12,7 @@
 #include "ksi.hpp"
 #include "modup.hpp"
 #include "ntt_engine.hpp"
+#include "tensor.hpp"
 
 using namespace gf;
 
@@ -186,6 +187,51 @@ extern "C" void emu_ksi(uint32_t n, uint64_t q, const uint64_t* tw, const uint64
         constexpr uint32_t RM = decltype(Rc)::value;
         if (lazy_red_of(q)) emu_ksi_t<RM, true>(n, q, t2.data(), np, xf, src, kb, ka, o0, o1, pairs, lazy_out, f64);
         else emu_ksi_t<RM, false>(n, q, t2.data(), np, xf, src, kb, ka, o0, o1, pairs, lazy_out, f64);
+    });
+}
+
+// The tensor product fused with the inverse row pass of its t2, as the
+// fused kernel runs it per row tile (tensor_tile_lds, then phase B from LDS,
+// fp64 when f64 != 0), then the integer phase A over x: t0, t1, t2 and
+// x = INTT(t2) times the final factor post (post, postp).
+extern "C" void emu_tensor_ib(uint32_t n, uint64_t q, const uint64_t* iw, const uint64_t* iwp, uint64_t post,
+                              uint64_t postp, uint32_t rmax, int f64, const uint64_t* a0, const uint64_t* a1,
+                              const uint64_t* b0, const uint64_t* b1, uint64_t* t0, uint64_t* t1, uint64_t* t2,
+                              uint64_t* x) {
+    std::vector<uint64_t> tw2(2 * (std::size_t)n);
+    for (uint32_t i = 0; i < n; ++i) {
+        tw2[2 * i] = iw[i];
+        tw2[2 * i + 1] = iwp[i];
+    }
+    with_radix(rmax, [&](auto Rc) {
+        constexpr uint32_t RM = decltype(Rc)::value;
+        const NttGeom g = make_geom(n, RM);
+        const std::vector<uint64_t> t2d = f64_pairs_of(tw2.data(), n, q);
+        std::vector<uint64_t> rows(row_table_words(g)), rowsd(row_table_words(g)), lds(NTT_TILE_PADDED);
+        build_row_table(tw2.data(), g, rows.data());
+        build_row_table(t2d.data(), g, rowsd.data());
+        const TensorJob tj{a0, a1, b0, b1, t0, t1, t2, q, host_qneg(q), host_r2(q), 1};
+        NttJob j{t2, x, tw2.data(), nullptr, q, post, postp};
+        j.twr = rows.data();
+        if (f64) {
+            j.twd = t2d.data();
+            j.twrd = rowsd.data();
+            j.fq = (double)q;
+            j.fqinv = 1.0 / j.fq;
+        }
+        with_split(g.logN1, g.logN2, [&](auto A, auto B) {
+            constexpr uint32_t S1 = decltype(A)::value, S2 = decltype(B)::value;
+            if constexpr ((NTT_TILE >> S2) << S2 == NTT_TILE) {
+                for (uint32_t tile = 0; tile < (1u << S1) / g.R; ++tile) {
+                    std::fill(lds.begin(), lds.end(), ~0ull);   // the row pass sees only what the tensor left
+                    tensor_tile_lds(tj, tile * NTT_TILE, lds.data(), 0, 1);
+                    if (f64) phaseB_inv_f64<S2, RM, NoSync, true>(j, g, tile, lds.data(), 0, 1, NoSync{});
+                    else phaseB_body<false, false, true, S2, RM, NoSync, true>(j, g, tile, lds.data(), 0, 1, NoSync{});
+                }
+                for (uint32_t tile = 0; tile < (1u << S2) / g.C; ++tile)
+                    phaseA_body<false, false, true, S1, RM>(j, g, tile, lds.data(), 0, 1, NoSync{});
+            }
+        });
     });
 }
 

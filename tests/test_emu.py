@@ -45,6 +45,9 @@ def emu():
     lib.emu_ksi.restype = None
     lib.emu_ksi.argtypes = [C.c_uint32, C.c_uint64, u64p, u64p, C.c_uint32, C.c_uint32, C.c_uint32, u64p, u64p,
                             u64p, u64p, u64p, C.c_uint32, C.c_uint32, C.c_int]
+    lib.emu_tensor_ib.restype = None
+    lib.emu_tensor_ib.argtypes = [C.c_uint32, C.c_uint64, u64p, u64p, C.c_uint64, C.c_uint64, C.c_uint32, C.c_int] + \
+        [u64p] * 8
     lib.emu_intt_f64.restype = None
     lib.emu_intt_f64.argtypes = [u64p, C.c_uint32, C.c_uint64, u64p, u64p, C.c_uint64, C.c_uint64, C.c_uint32]
     lib.emu_inner_red.restype = C.c_uint64
@@ -279,6 +282,38 @@ def test_fused_rowpass_inner_product(emu, logn, np_, xf):
                         o0, o1 = o0 % np.uint64(q), o1 % np.uint64(q)
                     assert np.array_equal(o0, w0), (bits, rmax, pairs, lazy_out, f64)
                     assert np.array_equal(o1, w1), (bits, rmax, pairs, lazy_out, f64)
+
+
+@pytest.mark.parametrize("logn", [13, 15, 16, 17])
+def test_tensor_fused_with_rowpass(emu, logn):
+    """The tensor product fused with the inverse row pass of its t2 (t2 handed
+    over in LDS per row tile, the row pass in fp64 or in integers, then the
+    ordinary phase A) gives the oracle's t0 = a0 b0, t1 = a0 b1 + a1 b0,
+    t2 = a1 b1 and its inverse transform of t2 times the final factor, for a
+    50-bit modulus (both row-pass kinds) and a 60-bit one (integer), every
+    register radix."""
+    n = 1 << logn
+    for bits, f64s in ((50, (0, 1)), (60, (0,))):
+        q = int(O.native_primes(1, bits, n)[0])
+        psi = int(O.lib.or_native_root(n, q))
+        _, _, ir, ip, ni, nip = O.root_tables(n, q, psi)
+        rng = np.random.default_rng(logn * 3 + bits)
+        a = [rng.integers(0, q, n, dtype=np.uint64) for _ in range(4)]
+        a[0][:3] = q - 1
+        a[3][:3] = q - 1
+        A = [x.astype(object) for x in a]
+        w0 = np.array((A[0] * A[2]) % q, np.uint64)
+        w1 = np.array((A[0] * A[3] + A[1] * A[2]) % q, np.uint64)
+        w2 = np.array((A[1] * A[3]) % q, np.uint64)
+        wx = O.intt(w2, q, ir, ip, ni, nip)
+        for rmax in (2, 3, 4):
+            for f64 in f64s:
+                t = [np.zeros(n, np.uint64) for _ in range(3)]
+                x = np.zeros(n, np.uint64)
+                emu.emu_tensor_ib(n, q, O.P(ir), O.P(ip), int(ni), int(nip), rmax, f64, O.P(a[0]), O.P(a[1]),
+                                  O.P(a[2]), O.P(a[3]), *[O.P(v) for v in t], O.P(x))
+                assert np.array_equal(t[0], w0) and np.array_equal(t[1], w1) and np.array_equal(t[2], w2), (bits, rmax)
+                assert np.array_equal(x % np.uint64(q), wx), (bits, rmax, f64)
 
 
 @pytest.mark.parametrize("logn", [13, 14, 15, 16, 17])

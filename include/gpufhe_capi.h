@@ -294,7 +294,9 @@ int gf_ctx_autotune(gf_ctx* c, hipStream_t s, uint32_t* rmax_out, float* ms_out)
  * it in double precision for moduli below 2^50, 0/1), "ksi_f64" (with
  * modup_f64, the fused inner product's row passes as well, 0/1),
  * "moddown_fused" (EvalMult+Rescale's P-tower inverse column pass, rescale
- * prologue and surviving towers' forward column pass as one kernel, 0/1).
+ * prologue and surviving towers' forward column pass as one kernel, 0/1),
+ * "tensor_ib" (with modup_fused, the tensor product and the inverse row pass
+ * of its t2 as one kernel per row tile, 0/1).
  * The report is JSON with every candidate's time, the radix search of
  * gf_ctx_autotune included; valid until the next tuning call.  Knobs (and
  * "ntt_radix", "rechunk_wgs", "modup_min_wgs") can be set by name; plans built

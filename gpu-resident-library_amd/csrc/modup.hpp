@@ -17,13 +17,14 @@
 // last to HBM.  The coefficient-form sources and digits never travel through
 // HBM, and no pass of a source or target needs LDS at its register end.
 //
-// Each source and target (a phase) takes the next of two LDS tiles in turn:
-// a phase's first LDS write can only overlap the previous phase's reads of
-// the other tile, and every phase of two or more passes has a barrier that
-// all threads reach only after finishing the phase before it, so the
-// phases need no barriers between them.  Every butterfly, product and reduction is the one the
-// unfused kernels apply, so the stored digit column
-// passes equal theirs bit for bit.
+// Each source and target (a phase) takes the next of two LDS tiles in turn,
+// and likewise the next of two LDS twiddle tiles (the phase's column-pass
+// twiddles, requested when the phase starts): a phase's first LDS write can
+// only overlap the previous phase's reads of the other tile, and every phase
+// of two or more passes has a barrier that all threads reach only after
+// finishing the phase before it, so the phases need no barriers between
+// them.  Every butterfly, product and reduction is the one the unfused
+// kernels apply, so the stored digit column passes equal theirs bit for bit.
 //
 // Sources and targets whose modulus is below 2^50 run their column passes in
 // double precision (ntt_f64.hpp) when the job gives their fp64 tables; the

@@ -85,6 +85,12 @@ def step_bytes(n, L, K, alpha, np_parts, kinds, batch):
     tensor_ib = kinds[:1] == ["tensor_ib"]
     if tensor_ib:
         kinds = ["tensor", "ntt_inv_b"] + list(kinds[1:])
+    # likewise the P rows' inner product with the P part's inverse row pass
+    # inside it (no row-pass step between the ModDown's two inverse steps)
+    ksi_rp = "moddown" in kinds and kinds[kinds.index("moddown") - 1] == "ntt_inv"
+    if ksi_rp:
+        i = kinds.index("moddown")
+        kinds = list(kinds[:i]) + ["ntt_inv_b"] + list(kinds[i:])
     sc = [L - min(alpha, L - alpha * p) + K for p in range(np_parts)]
     sp = [min(alpha, L - alpha * p) for p in range(np_parts)]
     if "ksi" in kinds:
@@ -121,6 +127,10 @@ def step_bytes(n, L, K, alpha, np_parts, kinds, batch):
         assert "moddown" in kinds or kinds[6:] == ["ntt_inv", "rsv", "ntt_fwd"], kinds
         head = [("tensor_intt_rowpass", 7 * T * L + T * L, 0)] if tensor_ib else \
             [("tensor", 7 * T * L, 0), ("intt_modup_rowpass", T * L, 0)]
+        if ksi_rp:
+            rp = [t for t in tail if t[0] == "intt_moddown_rowpass"][0]
+            ksi = [ksi[0], ("rowpass_inner_p_intt_rowpass", ksi[1][1] + rp[1], 0)]
+            tail = [t for t in tail if t[0] != "intt_moddown_rowpass"]
         return head + [("intt_extra", 2 * T * 2, 0),
                 ("modup_fused", T * (L + (sum(sp) + sum(sc)) + sum(sc)), 0)] + ksi + tail
     if "ksi" in kinds or len(kinds) == 8:   # fused ModDown + rescale plan
@@ -711,7 +721,7 @@ def main():
     else:
         radix, ntt_ms = ctx.autotune(stream, args.batch)   # NTT register radix, then the plan knobs at B
     knobs = {k_: ctx.knob(k_) for k_ in ("ntt_radix", "ksi_extra", "conv_chunk", "rs_chunk", "modup_fused",
-                                          "modup_f64", "ksi_f64", "moddown_fused", "tensor_ib")}
+                                          "modup_f64", "ksi_f64", "moddown_fused", "tensor_ib", "ksi_rowpass")}
     log("context ready: K=%d alpha=%d ws=%.1f MiB/op, knobs %s (%s)"
         % (K, alpha, ctx.ws_bytes / 2**20, knobs, "pinned" if pinned else "autotuned"))
     tuned = {"shape": {"n": N, "L": L, "dnum": DNUM}, "chosen": knobs, "pinned": pinned,

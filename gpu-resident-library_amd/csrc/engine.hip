@@ -410,9 +410,14 @@ void Ctx::set_knob(const std::string& name, uint32_t v) {
     } else if (name == "modup_fused") {
         if (v > 1) throw std::runtime_error("gpufhe: modup_fused must be 0 or 1");
         modup_fused = v;
-    } else if (name == "modup_f64" || name == "ksi_f64" || name == "moddown_fused" || name == "tensor_ib") {
+    } else if (name == "modup_f64" || name == "ksi_f64" || name == "moddown_fused" || name == "tensor_ib" ||
+               name == "ksi_rowpass") {
         if (v > 1) throw std::runtime_error("gpufhe: " + name + " must be 0 or 1");
-        (name == "modup_f64" ? modup_f64 : name == "ksi_f64" ? ksi_f64 : name == "tensor_ib" ? tensor_ib : moddown_fused) = v;
+        (name == "modup_f64"     ? modup_f64
+         : name == "ksi_f64"     ? ksi_f64
+         : name == "tensor_ib"   ? tensor_ib
+         : name == "ksi_rowpass" ? ksi_rowpass
+                                 : moddown_fused) = v;
     } else if (name == "modup_min_wgs") {
         modup_min_wgs = v;
     } else if (name == "rechunk_wgs") {
@@ -430,6 +435,7 @@ uint32_t Ctx::knob(const std::string& name) const {
     if (name == "modup_f64") return modup_f64;
     if (name == "ksi_f64") return ksi_f64;
     if (name == "tensor_ib") return tensor_ib;
+    if (name == "ksi_rowpass") return ksi_rowpass;
     if (name == "moddown_fused") return moddown_fused;
     if (name == "modup_min_wgs") return modup_min_wgs;
     if (name == "rechunk_wgs") return rechunk_wgs;
@@ -545,6 +551,9 @@ void Ctx::autotune_plan(hipStream_t s, uint32_t B) {
             alpha <= MODUP_MAXSRC && K <= MODUP_MAXSRC)
             tune("moddown_fused", moddown_fused, {0, 1},
                  {Step::NTT_IB, Step::NTT_I, Step::MODDOWN, Step::RSV, Step::NTT_F, Step::NTT_FB});
+        // the P part's inverse row pass inside the P rows' inner product
+        if (moddown_fused && !geom.single && n % KSI_TILE == 0)
+            tune("ksi_rowpass", ksi_rowpass, {0, 1}, {Step::KSI, Step::NTT_IB});
     }
     rep << "}";
     (void)hipEventDestroy(e0);
@@ -1601,6 +1610,28 @@ std::unique_ptr<Plan> PlanBuilder::layout(std::vector<char>& blob) {
     for (KsiJob& j : ksi_) {
         j.twr = C_.row_table_of(j.tw);
         if (!j.twr) throw std::runtime_error("gpufhe: transform table of another context");
+    }
+    // with the fused ModDown, a P row's inner product runs the inverse row
+    // pass of its two output rows itself (jobs with at least two transformed
+    // parts: two LDS regions to hand the rows over in), and those row-pass
+    // jobs leave their stage
+    if (C_.ksi_rowpass && !moddown_.empty() && !intt_pb_.empty()) {
+        std::map<const uint64_t*, size_t> by_in;
+        for (size_t i = 0; i < intt_pb_.size(); ++i) by_in[intt_pb_[i].in] = i;
+        std::vector<char> used(intt_pb_.size(), 0);
+        for (KsiJob& j : ksi_) {
+            const auto a = by_in.find(j.o0), b = by_in.find(j.o1);
+            if (a == by_in.end() || b == by_in.end() || __builtin_popcount(j.xf) < 2 || j.lazy_out) continue;
+            if (used[a->second] || used[b->second]) continue;
+            j.rp_on = 1;
+            j.rp[0] = intt_pb_[a->second];
+            j.rp[1] = intt_pb_[b->second];
+            used[a->second] = used[b->second] = 1;
+        }
+        std::vector<NttJob> rest;
+        for (size_t i = 0; i < intt_pb_.size(); ++i)
+            if (!used[i]) rest.push_back(intt_pb_[i]);
+        intt_pb_.swap(rest);
     }
     // With the fp64 fused ModUp in this plan, the inverse row pass in front
     // of it runs in fp64 too for moduli below 2^50 (its inputs, the tensor's

@@ -149,6 +149,9 @@ public:
     // 1: the tensor product and the inverse row pass of its t2 (in front of
     // the fused ModUp) run as one kernel per row tile, t2 handed over in LDS
     uint32_t tensor_ib = 1;
+    // 1: with the fused ModDown, the P rows' inner product runs the inverse
+    // row pass of its two output rows itself (the rows handed over in LDS)
+    uint32_t ksi_rowpass = 1;
     // 1: the fused EvalMult+Rescale runs the P towers' inverse column pass,
     // the rescale prologue and the surviving towers' forward column pass as
     // one kernel (the fused ModUp kernel's rescale mode), where the fused
@@ -167,7 +170,7 @@ public:
     void autotune_plan(hipStream_t s, uint32_t batch);
     std::string tune_report;
     // sets or reads a knob by name ("ntt_radix", "ksi_extra", "conv_chunk",
-    // "rs_chunk", "modup_fused", "modup_f64", "ksi_f64", "moddown_fused", "tensor_ib",
+    // "rs_chunk", "modup_fused", "modup_f64", "ksi_f64", "moddown_fused", "tensor_ib", "ksi_rowpass",
     // "modup_min_wgs", "rechunk_wgs")
     void set_knob(const std::string& name, uint32_t v);
     uint32_t knob(const std::string& name) const;

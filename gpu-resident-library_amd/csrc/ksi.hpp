@@ -56,6 +56,12 @@ struct KsiJob {
     const uint64_t* twd;
     const uint64_t* twrd;
     double fq, fqinv;
+    // rp_on: the row's two outputs are not stored; instead the inverse
+    // transforms' row pass rp[c] (its input tower o0 / o1) runs on them after
+    // the inner product, from LDS (at least two transformed parts, so two LDS
+    // regions)
+    uint32_t rp_on;
+    NttJob rp[2];
 };
 
 // V consecutive words from word i of a tower (V = 2: one 16-byte access; the
@@ -87,6 +93,25 @@ inline uint32_t ksi_d30(uint64_t q, uint64_t red) {
     return dmax <= ((unsigned __int128)1 << 60) ? 1u : 0u;
 }
 
+// The inner product's outputs at positions idx .. idx + V - 1 (word gi of the
+// row): to HBM, or (j.rp_on) into LDS regions 0 and 1 at those positions'
+// slots for the row pass after it; each thread writes only its own
+// positions, whose digits it has already read
+template <uint32_t V>
+GF_HD void ksi_store(const KsiJob& j, uint64_t* lds, uint32_t idx, uint32_t gi, const uint64_t* r0,
+                     const uint64_t* r1) {
+    if (j.rp_on) {
+#pragma unroll
+        for (uint32_t v = 0; v < V; ++v) {
+            lds[padded(idx + v)] = r0[v];
+            lds[KSI_PART_WORDS + padded(idx + v)] = r1[v];
+        }
+        return;
+    }
+    store_words<V>(gcast(j.o0), gi, r0);
+    store_words<V>(gcast(j.o1), gi, r1);
+}
+
 // hi:lo of lo + mid 2^30 + hi 2^60 (the column sums of dot30's layout)
 GF_HD U128 cols_to_u128(uint64_t lo, uint64_t mid, uint64_t hi) {
     U128 T;
@@ -106,7 +131,7 @@ GF_HD U128 cols_to_u128(uint64_t lo, uint64_t mid, uint64_t hi) {
 // multiply-adds with no carries.  NP <= 7 keeps every column below 2^64
 // (lo, hi < 7 2^60; mid < 14 2^60).
 template <uint32_t NP, uint32_t V, bool LZ>
-GF_HD void ksi_inner_cols(const KsiJob& j, const uint64_t* lds, uint32_t off, uint32_t tid, uint32_t nthr,
+GF_HD void ksi_inner_cols(const KsiJob& j, uint64_t* lds, uint32_t off, uint32_t tid, uint32_t nthr,
                           const NttJob& t) {
     const GF_GLOBAL uint64_t *src[NP], *kb[NP], *ka[NP];
     uint32_t reg[NP];
@@ -167,14 +192,13 @@ GF_HD void ksi_inner_cols(const KsiJob& j, const uint64_t* lds, uint32_t off, ui
                 r1[v] = redc(s1, q, qneg);
             }
         }
-        store_words<V>(gcast(j.o0), gi, r0);
-        store_words<V>(gcast(j.o1), gi, r1);
+        ksi_store<V>(j, lds, idx, gi, r0, r1);
     }
 }
 
 // ksi_inner_cols for the job's part count, when it has an instantiation
 template <uint32_t V, bool LZ>
-GF_HD bool ksi_inner_cols_np(const KsiJob& j, const uint64_t* lds, uint32_t off, uint32_t tid, uint32_t nthr,
+GF_HD bool ksi_inner_cols_np(const KsiJob& j, uint64_t* lds, uint32_t off, uint32_t tid, uint32_t nthr,
                              const NttJob& t) {
     switch (j.np) {
         case 1: ksi_inner_cols<1, V, LZ>(j, lds, off, tid, nthr, t); return true;
@@ -214,10 +238,10 @@ GF_HD void ksi_body(const KsiJob& j, uint32_t N1, uint32_t tile, uint64_t* lds, 
     }
     bsync();
     // up to 4 parts with every value below 2^60: column sums, loads first
-    if (j.d30) {
-        if (j.pairs ? ksi_inner_cols_np<2, LZ>(j, lds, off, tid, nthr, t) : ksi_inner_cols_np<1, LZ>(j, lds, off, tid, nthr, t))
-            return;
-    }
+    bool done = false;
+    if (j.d30)
+        done = j.pairs ? ksi_inner_cols_np<2, LZ>(j, lds, off, tid, nthr, t)
+                       : ksi_inner_cols_np<1, LZ>(j, lds, off, tid, nthr, t);
     // positions idx .. idx + V - 1 per step (V = 2: one 16-byte access per operand)
     auto inner = [&](auto Vc) {
         constexpr uint32_t V = decltype(Vc)::value;
@@ -258,12 +282,24 @@ GF_HD void ksi_body(const KsiJob& j, uint32_t N1, uint32_t tile, uint64_t* lds, 
                     r1[v] = redc(s1[v], q, qneg);
                 }
             }
-            store_words<V>(gcast(j.o0), gi, r0);
-            store_words<V>(gcast(j.o1), gi, r1);
+            ksi_store<V>(j, lds, idx, gi, r0, r1);
         }
     };
-    if (j.pairs) inner(std::integral_constant<uint32_t, 2>{});
-    else inner(std::integral_constant<uint32_t, 1>{});
+    if (!done) {
+        if (j.pairs) inner(std::integral_constant<uint32_t, 2>{});
+        else inner(std::integral_constant<uint32_t, 1>{});
+    }
+    // the inverse transforms' row pass of the two output rows (inputs in
+    // [0, q), lazy outputs for the column pass after it), component c by
+    // wave c
+    if (j.rp_on) {
+        bsync();
+        for (uint32_t c = wave; c < 2; c += nwaves) {
+            const NttJob& r = j.rp[c];
+            row_tile<false, false, true, S2, RM, ROWS, 0, true, WSync, false, true>(
+                r, d, nullptr, gcast(r.out), off, lds + c * KSI_PART_WORDS, lane, wlanes, wsync);
+        }
+    }
 }
 
 }  // namespace gf

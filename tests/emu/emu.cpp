@@ -120,7 +120,8 @@ std::vector<uint64_t> f64_pairs_of(const uint64_t* paired, uint32_t n, uint64_t 
 template <uint32_t RM, bool LZ>
 void emu_ksi_t(uint32_t n, uint64_t q, const uint64_t* t2, uint32_t np, uint32_t xf, uint64_t* src,
                const uint64_t* kb, const uint64_t* ka, uint64_t* o0, uint64_t* o1, uint32_t pairs,
-               uint32_t lazy_out, int f64) {
+               uint32_t lazy_out, int f64, const uint64_t* it2, uint64_t post, uint64_t postp, uint64_t* x0,
+               uint64_t* x1) {
     const NttGeom g = make_geom(n, RM);
     std::vector<uint64_t> rows(row_table_words(g));
     build_row_table(t2, g, rows.data());
@@ -148,6 +149,17 @@ void emu_ksi_t(uint32_t n, uint64_t q, const uint64_t* t2, uint32_t np, uint32_t
     J.o1 = o1;
     J.pairs = pairs;
     J.lazy_out = lazy_out;
+    std::vector<uint64_t> irows;
+    if (it2) {   // the inverse row pass of both outputs inside the fused step, into x0 / x1
+        irows.resize(row_table_words(g));
+        build_row_table(it2, g, irows.data());
+        uint64_t* xs[2] = {x0, x1};
+        for (int c = 0; c < 2; ++c) {
+            J.rp[c] = NttJob{c ? o1 : o0, xs[c], it2, nullptr, q, post, postp};
+            J.rp[c].twr = irows.data();
+        }
+        J.rp_on = 1;
+    }
     std::vector<uint64_t> lds(NTT_TILE_PADDED > (size_t)np * KSI_PART_WORDS ? NTT_TILE_PADDED : (size_t)np * KSI_PART_WORDS);
     with_split(g.logN1, g.logN2, [&](auto A, auto B) {
         constexpr uint32_t S1 = decltype(A)::value, S2 = decltype(B)::value;
@@ -166,6 +178,10 @@ void emu_ksi_t(uint32_t n, uint64_t q, const uint64_t* t2, uint32_t np, uint32_t
         }
         for (uint32_t tile = 0; tile < n / KSI_TILE; ++tile)
             ksi_body<S2, RM, LZ>(J, 1u << S1, tile, lds.data(), 0, 1, 0, 1, 0, 1, NoSync{}, NoSync{});
+        if (it2)   // the inverse transforms' column pass after it, as the fused ModDown runs it
+            for (int c = 0; c < 2; ++c)
+                for (uint32_t tile = 0; tile < (1u << S2) / g.C; ++tile)
+                    phaseA_body<false, false, true, S1, RM>(J.rp[c], g, tile, lds.data(), 0, 1, NoSync{});
     });
 }
 }  // namespace
@@ -174,19 +190,35 @@ void emu_ksi_t(uint32_t n, uint64_t q, const uint64_t* t2, uint32_t np, uint32_t
 // a coefficient-form tower (bit p of xf: transformed here, column pass then the
 // fused row pass) or an evaluation-form digit; o = sum_p NTT?(src_p) k_p R^-1.
 // f64 != 0 (q below 2^50): the column passes' output reduced below q, then
-// the row passes in fp64.
+// the row passes in fp64.  iw != null: the step also runs the inverse
+// transforms' row pass of both outputs (tables iw/iwp, final factor
+// post/postp) into x0 / x1 instead of storing o0 / o1, and the column pass
+// follows here: x_c = INTT(o_c) times post.
 extern "C" void emu_ksi(uint32_t n, uint64_t q, const uint64_t* tw, const uint64_t* twp, uint32_t rmax,
                         uint32_t np, uint32_t xf, uint64_t* src, const uint64_t* kb, const uint64_t* ka,
-                        uint64_t* o0, uint64_t* o1, uint32_t pairs, uint32_t lazy_out, int f64) {
-    std::vector<uint64_t> t2(2 * (std::size_t)n);
+                        uint64_t* o0, uint64_t* o1, uint32_t pairs, uint32_t lazy_out, int f64, const uint64_t* iw,
+                        const uint64_t* iwp, uint64_t post, uint64_t postp, uint64_t* x0, uint64_t* x1) {
+    std::vector<uint64_t> t2(2 * (std::size_t)n), it2;
     for (uint32_t i = 0; i < n; ++i) {
         t2[2 * i] = tw[i];
         t2[2 * i + 1] = twp[i];
     }
+    if (iw) {
+        it2.resize(2 * (std::size_t)n);
+        for (uint32_t i = 0; i < n; ++i) {
+            it2[2 * i] = iw[i];
+            it2[2 * i + 1] = iwp[i];
+        }
+    }
+    const uint64_t* it = iw ? it2.data() : nullptr;
     with_radix(rmax, [&](auto Rc) {
         constexpr uint32_t RM = decltype(Rc)::value;
-        if (lazy_red_of(q)) emu_ksi_t<RM, true>(n, q, t2.data(), np, xf, src, kb, ka, o0, o1, pairs, lazy_out, f64);
-        else emu_ksi_t<RM, false>(n, q, t2.data(), np, xf, src, kb, ka, o0, o1, pairs, lazy_out, f64);
+        if (lazy_red_of(q))
+            emu_ksi_t<RM, true>(n, q, t2.data(), np, xf, src, kb, ka, o0, o1, pairs, lazy_out, f64, it, post, postp,
+                                x0, x1);
+        else
+            emu_ksi_t<RM, false>(n, q, t2.data(), np, xf, src, kb, ka, o0, o1, pairs, lazy_out, f64, it, post, postp,
+                                 x0, x1);
     });
 }
 

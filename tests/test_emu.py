@@ -44,7 +44,8 @@ def emu():
     lib.emu_lazy_red_of.argtypes = [C.c_uint64]
     lib.emu_ksi.restype = None
     lib.emu_ksi.argtypes = [C.c_uint32, C.c_uint64, u64p, u64p, C.c_uint32, C.c_uint32, C.c_uint32, u64p, u64p,
-                            u64p, u64p, u64p, C.c_uint32, C.c_uint32, C.c_int]
+                            u64p, u64p, u64p, C.c_uint32, C.c_uint32, C.c_int, u64p, u64p, C.c_uint64, C.c_uint64,
+                            u64p, u64p]
     lib.emu_tensor_ib.restype = None
     lib.emu_tensor_ib.argtypes = [C.c_uint32, C.c_uint64, u64p, u64p, C.c_uint64, C.c_uint64, C.c_uint32, C.c_int] + \
         [u64p] * 8
@@ -245,14 +246,18 @@ def test_fused_rowpass_inner_product(emu, logn, np_, xf):
     transform, for moduli below 2^57 (unreduced digits) and above (reduced
     digits), with digits and keys below 2^60 (column sums) and above (128-bit
     sums), every register radix; and for the 50-bit modulus with the row
-    passes in fp64 (digits below 2q, as the fp64 ModUp leaves them)."""
+    passes in fp64 (digits below 2q, as the fp64 ModUp leaves them).  With at
+    least two transformed parts, the step also runs as the P rows run it with
+    the fused ModDown: the inverse row pass of both outputs inside it (the rows
+    handed over in LDS), after which the column pass gives the oracle's
+    inverse transform of each output."""
     n = 1 << logn
     # 50 bits: unreduced digits, column sums; 55: unreduced digits above 2^60
     # (128-bit sums); 59: reduced digits, column sums; 61: 128-bit sums
     for bits in (50, 55, 59, 61):
         q = int(O.native_primes(1, bits, n)[0])
         psi = int(O.lib.or_native_root(n, q))
-        fr, fp, _, _, _, _ = O.root_tables(n, q, psi)
+        fr, fp, ir, ip, ni, nip = O.root_tables(n, q, psi)
         rng = np.random.default_rng(logn * 7 + bits + np_)
         src = rng.integers(0, q, (np_, n), dtype=np.uint64)
         src[0, :3] = q - 1
@@ -276,12 +281,22 @@ def test_fused_rowpass_inner_product(emu, logn, np_, xf):
                     o0 = np.zeros(n, np.uint64)
                     o1 = np.zeros(n, np.uint64)
                     emu.emu_ksi(n, q, O.P(fr), O.P(fp), rmax, np_, xf, O.P(s), O.P(np.ascontiguousarray(kb)),
-                                O.P(np.ascontiguousarray(ka)), O.P(o0), O.P(o1), pairs, lazy_out, f64)
+                                O.P(np.ascontiguousarray(ka)), O.P(o0), O.P(o1), pairs, lazy_out, f64,
+                                None, None, 0, 0, None, None)
                     if lazy_out:   # unreduced, for the epilogue's lazy products: congruent, below 2^62
                         assert int(max(o0.max(), o1.max())) < 1 << 62
                         o0, o1 = o0 % np.uint64(q), o1 % np.uint64(q)
                     assert np.array_equal(o0, w0), (bits, rmax, pairs, lazy_out, f64)
                     assert np.array_equal(o1, w1), (bits, rmax, pairs, lazy_out, f64)
+                    if lazy_out or f64 or bin(xf).count("1") < 2:
+                        continue
+                    s = src.copy()
+                    x0, x1 = np.zeros(n, np.uint64), np.zeros(n, np.uint64)
+                    emu.emu_ksi(n, q, O.P(fr), O.P(fp), rmax, np_, xf, O.P(s), O.P(np.ascontiguousarray(kb)),
+                                O.P(np.ascontiguousarray(ka)), O.P(o0), O.P(o1), pairs, 0, 0,
+                                O.P(ir), O.P(ip), int(ni), int(nip), O.P(x0), O.P(x1))
+                    assert np.array_equal(x0 % np.uint64(q), O.intt(w0, q, ir, ip, ni, nip)), ("rp", bits, rmax)
+                    assert np.array_equal(x1 % np.uint64(q), O.intt(w1, q, ir, ip, ni, nip)), ("rp", bits, rmax)
 
 
 @pytest.mark.parametrize("logn", [13, 15, 16, 17])

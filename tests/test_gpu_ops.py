@@ -348,7 +348,8 @@ def test_plan_knobs_keep_results(G, ksi_extra, conv_chunk, rs_chunk, modup_fused
     effect, the fused or the unfused ModUp, the fp64 passes of the 50-bit
     towers around the fused ModUp and in the fused inner product, the fused
     or the unfused ModDown + rescale prologue, the tensor product fused with
-    the row pass after it or apart) gives the
+    the row pass after it or apart, the P part's inverse row pass inside the
+    P rows' inner product or apart) gives the
     oracle's results, on a batch of uneven levels (parts of 3 and 2 towers;
     ragged chunks with widths 5 and 3); the autotuner's own search leaves a
     report naming its choices."""
@@ -358,6 +359,7 @@ def test_plan_knobs_keep_results(G, ksi_extra, conv_chunk, rs_chunk, modup_fused
     for name, v in (("ksi_extra", ksi_extra), ("conv_chunk", conv_chunk), ("rs_chunk", rs_chunk),
                     ("modup_fused", modup_fused), ("modup_f64", modup_f64), ("ksi_f64", ksi_f64),
                     ("moddown_fused", moddown_fused), ("tensor_ib", 0 if conv_chunk == 16 else 1),
+                    ("ksi_rowpass", 0 if conv_chunk == 32 else 1),
                     ("modup_min_wgs", 0), ("rechunk_wgs", 0)):
         ctx.set_knob(name, v)
         assert ctx.knob(name) == v

@@ -552,8 +552,10 @@ void Ctx::autotune_plan(hipStream_t s, uint32_t B) {
             tune("moddown_fused", moddown_fused, {0, 1},
                  {Step::NTT_IB, Step::NTT_I, Step::MODDOWN, Step::RSV, Step::NTT_F, Step::NTT_FB});
         // the P part's inverse row pass inside the P rows' inner product
+        // (listed first: on a tie the tuner keeps the first candidate, and
+        // this one moves fewer bytes in one launch fewer)
         if (moddown_fused && !geom.single && n % KSI_TILE == 0)
-            tune("ksi_rowpass", ksi_rowpass, {0, 1}, {Step::KSI, Step::NTT_IB});
+            tune("ksi_rowpass", ksi_rowpass, {1, 0}, {Step::KSI, Step::NTT_IB});
     }
     rep << "}";
     (void)hipEventDestroy(e0);

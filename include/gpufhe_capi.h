@@ -296,7 +296,9 @@ int gf_ctx_autotune(gf_ctx* c, hipStream_t s, uint32_t* rmax_out, float* ms_out)
  * "moddown_fused" (EvalMult+Rescale's P-tower inverse column pass, rescale
  * prologue and surviving towers' forward column pass as one kernel, 0/1),
  * "tensor_ib" (with modup_fused, the tensor product and the inverse row pass
- * of its t2 as one kernel per row tile, 0/1).
+ * of its t2 as one kernel per row tile, 0/1), "ksi_rowpass" (with
+ * moddown_fused, the P part's inverse row pass inside the P rows' inner
+ * product launch, from its LDS, 0/1).
  * The report is JSON with every candidate's time, the radix search of
  * gf_ctx_autotune included; valid until the next tuning call.  Knobs (and
  * "ntt_radix", "rechunk_wgs", "modup_min_wgs") can be set by name; plans built

@@ -780,6 +780,23 @@ def main():
             del r
     sweep[str(B)] = round(value, 1)
 
+    # the same batch with every transform and conversion in 64-bit integer
+    # instructions only (the fp64 knobs off): the exact-integer datapath
+    # without double-precision arithmetic, same results bit for bit
+    int_only = None
+    if not args.no_extra:
+        saved = {k_: ctx.knob(k_) for k_ in ("modup_f64", "ksi_f64")}
+        ctx.set_knob("modup_f64", 0)
+        ctx.set_knob("ksi_f64", 0)
+        r = run_batched(G, ctx, key, B, args.steps, args.warmup, stream)
+        for k_, v_ in saved.items():
+            ctx.set_knob(k_, v_)
+        int_only = dict(knobs={"modup_f64": 0, "ksi_f64": 0}, ops_s=round(B * args.steps / (r["ms_total"] / 1e3), 1),
+                        ms_per_step=round(r["ms_total"] / args.steps, 4))
+        log("batched B=%d, integer instructions only: %.3f ms/step, %.1f ops/s"
+            % (B, int_only["ms_per_step"], int_only["ops_s"]))
+        del r
+
     del res
     chain = rotations = cfg1 = refseq = None
     if not args.no_extra:
@@ -862,6 +879,7 @@ def main():
         "per_op_per_tower_launch_ops_s": round(perjob_ops, 2),
         "batched_over_per_tower_launch": round(value / perjob_ops, 3),
         "batch_sweep_ops_s": sweep,
+        "integer_only": int_only,
         "stages": stage_report,
         "roofline": {"bound": "hbm", "kernel": dom["stage"] + ":" + dom["kernel"],
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

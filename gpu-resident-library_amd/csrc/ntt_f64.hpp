@@ -84,8 +84,11 @@ GF_HD double f64_reduce(double x, double q, double qinv) {
 }
 
 // (w, fl(w/q)) pairs, w centred, in the index layout of the paired integer tables
+// (kAhead: a table in global memory, whose forward pass twiddles of the first
+// stages are requested together with the pass's data)
 struct TwF64 {
     static constexpr bool kRows = false;
+    static constexpr bool kAhead = true;
     const GF_GLOBAL double* t;
     GF_HD void get(uint32_t i, double& w, double& wq) const {
         w = t[2 * i];
@@ -96,6 +99,7 @@ struct TwF64 {
 template <uint32_t R, uint32_t LOGU>
 struct TwRowsF64 {
     static constexpr bool kRows = true;
+    static constexpr bool kAhead = true;
     static constexpr uint32_t PR = ((1u << R) - 1) << LOGU;
     const GF_GLOBAL double* t;
     GF_HD void get_rp(uint32_t c, uint32_t s, uint32_t g, uint32_t v, double& w, double& wq) const {
@@ -137,6 +141,11 @@ struct UnitOutF64 {
     GF_HD void unit(uint32_t uid, uint32_t u, double x) const { v[(ALL ? uid * U : 0) + u] = bits_of_f64(x); }
 };
 
+template <class TW, class = void>
+struct TwAhead : std::false_type {};
+template <class TW>
+struct TwAhead<TW, std::void_t<decltype(TW::kAhead)>> : std::integral_constant<bool, TW::kAhead> {};
+
 // One radix-2^R pass over the tile's units, as tile_pass (same unit and
 // twiddle indexing) on doubles.  CENTER: outputs centred before `out`.  An
 // inverse pass of more than two stages centres its unit after every second
@@ -168,11 +177,25 @@ GF_HD void tile_pass_f64(const TileDesc& d, const TW& tw, double q, double qinv,
         for (uint32_t u = 0; u < R8; ++u)
             at[u] = sbase + (d.pad ? padded(k0 + u * TL) : (k0 + u * TL) * d.KS);
         const uint32_t G = d.g0 + s * d.gstep;
-        // a forward pass's first stage takes one twiddle: requested with the data
+        // a forward pass's first stage takes one twiddle: requested with the
+        // data; from a global table, so are the 2 and 4 of stages 1 and 2
+        // (one memory round trip instead of three before stage 3)
+        constexpr uint32_t AH = FWD && TwAhead<TW>::value ? (R < 3 ? R : 3) : 1;
         double w0 = 0, wq0 = 0;
+        double wa[6], wqa[6];   // stage c < AH: its pair v at (1 << c) - 2 + v
         if constexpr (FWD) {
             if constexpr (TW::kRows) tw.get_rp(0, s, g, 0, w0, wq0);
             else tw.get((G << ST0) + g, w0, wq0);
+#pragma unroll
+            for (uint32_t c = 1; c < AH; ++c) {
+#pragma unroll
+                for (uint32_t v = 0; v < (1u << c); ++v) {
+                    double& w = wa[(1u << c) - 2 + v];
+                    double& wq = wqa[(1u << c) - 2 + v];
+                    if constexpr (TW::kRows) tw.get_rp(c, s, g, v, w, wq);
+                    else tw.get((G << (ST0 + c)) + (g << c) + v, w, wq);
+                }
+            }
         }
         double x[R8];
         if constexpr (HasRaw<IN>::value) {
@@ -195,7 +218,10 @@ GF_HD void tile_pass_f64(const TileDesc& d, const TW& tw, double q, double qinv,
                 for (uint32_t u = 0; u < R8; ++u) {
                     if (u & h) continue;
                     double w = w0, wq = wq0;
-                    if (c > 0) {
+                    if (c > 0 && c < AH) {
+                        w = wa[(1u << c) - 2 + (u >> (R - c))];
+                        wq = wqa[(1u << c) - 2 + (u >> (R - c))];
+                    } else if (c > 0) {
                         if constexpr (TW::kRows) tw.get_rp(c, s, g, u >> (R - c), w, wq);
                         else tw.get(tb + (u >> (R - c)), w, wq);
                     }

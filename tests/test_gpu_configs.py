@@ -136,27 +136,32 @@ def test_cfg3_uneven_levels_run_and_graph(G):
             assert np.array_equal(outs[i][1].download(), w1), (mode, levels[i])
 
 
-def test_cfg3_inner_product_row_passes(G):
-    """The fused row pass + inner product with integer and with fp64 row
-    passes gives the oracle's results at n = 2^16, L = 24 over uneven levels
-    (Q rows with one own part, P rows with none, lazy accumulator rows)."""
+def test_cfg3_integer_and_fp64_passes(G):
+    """The plan with every transform and conversion in integer instructions
+    (modup_f64 = 0, the bench's integer_only line), with fp64 passes around
+    the fused ModUp and ModDown only, and with the fused inner product's row
+    passes in fp64 as well, gives the oracle's results at n = 2^16, L = 24
+    over uneven levels (Q rows with one own part, P rows with none, lazy
+    accumulator rows)."""
     c = cfg3(G)
     levels = [24, 13, 2]
-    saved = c.ctx.knob("ksi_f64")
+    saved = {k: c.ctx.knob(k) for k in ("modup_f64", "ksi_f64")}
     want = None
     try:
-        for f64 in (0, 1):
-            c.ctx.set_knob("ksi_f64", f64)
+        for modup_f64, ksi_f64 in ((0, 0), (1, 0), (1, 1)):
+            c.ctx.set_knob("modup_f64", modup_f64)
+            c.ctx.set_knob("ksi_f64", ksi_f64)
             plan, ins, outs, ws = emr_batch(G, c, levels, 31)
             if want is None:
                 want = oracle_emr(c, levels, ins, range(len(levels)))
             plan.run_graph(c.s)
             c.s.sync()
             for i, (w0, w1) in enumerate(want):
-                assert np.array_equal(outs[i][0].download(), w0), (f64, levels[i])
-                assert np.array_equal(outs[i][1].download(), w1), (f64, levels[i])
+                assert np.array_equal(outs[i][0].download(), w0), (modup_f64, ksi_f64, levels[i])
+                assert np.array_equal(outs[i][1].download(), w1), (modup_f64, ksi_f64, levels[i])
     finally:
-        c.ctx.set_knob("ksi_f64", saved)
+        for k, v in saved.items():
+            c.ctx.set_knob(k, v)
 
 
 def test_cfg2_chain_16x10_matches_oracle(G):

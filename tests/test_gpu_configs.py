@@ -142,23 +142,33 @@ def test_cfg3_integer_and_fp64_passes(G):
     the fused ModUp and ModDown only, and with the fused inner product's row
     passes in fp64 as well, gives the oracle's results at n = 2^16, L = 24
     over uneven levels (Q rows with one own part, P rows with none, lazy
-    accumulator rows)."""
+    accumulator rows).  The fused ModUp and ModDown, whose column passes the
+    fp64 setting covers, are forced on for this small batch (modup_min_wgs =
+    0), and each plan runs directly and as a graph."""
     c = cfg3(G)
     levels = [24, 13, 2]
-    saved = {k: c.ctx.knob(k) for k in ("modup_f64", "ksi_f64")}
+    saved = {k: c.ctx.knob(k) for k in ("modup_f64", "ksi_f64", "modup_min_wgs")}
     want = None
     try:
+        c.ctx.set_knob("modup_min_wgs", 0)
         for modup_f64, ksi_f64 in ((0, 0), (1, 0), (1, 1)):
             c.ctx.set_knob("modup_f64", modup_f64)
             c.ctx.set_knob("ksi_f64", ksi_f64)
             plan, ins, outs, ws = emr_batch(G, c, levels, 31)
             if want is None:
                 want = oracle_emr(c, levels, ins, range(len(levels)))
-            plan.run_graph(c.s)
-            c.s.sync()
-            for i, (w0, w1) in enumerate(want):
-                assert np.array_equal(outs[i][0].download(), w0), (modup_f64, ksi_f64, levels[i])
-                assert np.array_equal(outs[i][1].download(), w1), (modup_f64, ksi_f64, levels[i])
+            for mode in ("run", "graph"):
+                for lv, o in zip(levels, outs):
+                    G.check(G.lib.gf_fill_uniform(c.ctx.h, o[0].ptr, lv - 1, 5, c.s.s))
+                    G.check(G.lib.gf_fill_uniform(c.ctx.h, o[1].ptr, lv - 1, 6, c.s.s))
+                if mode == "run":
+                    plan.run(c.s)
+                else:
+                    plan.run_graph(c.s)
+                c.s.sync()
+                for i, (w0, w1) in enumerate(want):
+                    assert np.array_equal(outs[i][0].download(), w0), (modup_f64, ksi_f64, mode, levels[i])
+                    assert np.array_equal(outs[i][1].download(), w1), (modup_f64, ksi_f64, mode, levels[i])
     finally:
         for k, v in saved.items():
             c.ctx.set_knob(k, v)

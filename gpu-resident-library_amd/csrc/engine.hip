@@ -237,7 +237,7 @@ Ctx::Ctx(uint32_t n_, const std::vector<uint64_t>& q, const std::vector<uint64_t
         // (s2 = floor(Q' [Q'^-1]_{q_l} / q_l) mod q_i is congruent to -s1, src/params.cpp:50-75)
         if (l >= 2) {
             const uint64_t ql = mods[l - 1];
-            std::vector<uint64_t> tabRS((size_t)K * (l - 1)), tg;
+            std::vector<uint64_t> tabRS((size_t)K * (l - 1)), tg, rsw(2 * (size_t)(l - 1));
             for (uint32_t i = 0; i + 1 < l; ++i) {
                 const uint64_t qi = mods[i];
                 const uint64_t s1 = host_invmod(ql % qi, qi);
@@ -257,9 +257,11 @@ Ctx::Ctx(uint32_t n_, const std::vector<uint64_t>& q, const std::vector<uint64_t
                 const RsTarget t{qi, host_qneg(qi), sj.diff, sj.new_larger, s2, host_precon(s2, qi), sj.red};
                 const uint64_t* tw = (const uint64_t*)&t;
                 tg.insert(tg.end(), tw, tw + sizeof(RsTarget) / 8);
+                rs_sw_consts(qi, ql, s2, rsw[2 * i], rsw[2 * i + 1]);
             }
             put(tabRS, &c.d_tabRS);
             put(tg, (const uint64_t**)&c.d_rstg);
+            put(rsw, &c.d_rs_sw);
             // the surviving towers as the fused ModDown's targets
             std::vector<uint64_t> rq(l - 1), rqn(l - 1), rtw(l - 1), rred(l - 1), rtwd(l - 1), rqf(2 * (size_t)(l - 1));
             for (uint32_t i = 0; i + 1 < l; ++i) {
@@ -1282,6 +1284,7 @@ void PlanBuilder::rescale_tail(const ModdownSrc& m, bool fused) {
         J.dr = m.dr;
         J.tabd = lv.d_tabMD + last;   // PHatModq column of q_last, row stride l
         J.tg = lv.d_rstg;
+        J.tsw = lv.d_rs_sw;
         J.q_old = C.mods[last];
         J.half = C.mods[last] >> 1;
         J.qd_neg = C.rc[last].qneg;

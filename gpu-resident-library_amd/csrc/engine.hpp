@@ -87,6 +87,7 @@ struct LevelConst {
     std::vector<uint64_t> ps1, ps1p;           // P^-1 s1 mod q_i (fused ModDown + rescale)
     const uint64_t* d_tabRS = nullptr;         // PHatModq * (-P^-1 s1) * R mod q_i, [K][l-1]
     const RsTarget* d_rstg = nullptr;          // [l-1]
+    const uint64_t* d_rs_sw = nullptr;         // [2 (l-1)] rs_sw_consts of each surviving tower
     // the surviving towers as targets of the fused ModDown (ModupJob tables, [l-1])
     const uint64_t* d_rs_tq = nullptr;
     const uint64_t* d_rs_tqneg = nullptr;

@@ -83,6 +83,16 @@ GF_HD uint64_t rs_lazy_value(uint64_t dv, bool hi, const RsTarget& tg, const U12
     return shoup_lazy(sw, tg.s2, tg.s2p, tg.q) + redc_lazy(acc, tg.q, tg.qneg);
 }
 
+// The fused ModDown's prologue folds the switch term into the conversion's
+// Montgomery sum: with x~ = dv - q_old [dv > q_old / 2] the centred dropped
+// value, x~ s2 = dv s2 + [dv > q_old / 2] (-q_old s2) mod q_t, so dv and the
+// 0/1 flag join the sources as two more terms with these constants
+// (Montgomery form, R = 2^64): cs = s2 2^64, ch = -q_old s2 2^64 mod q.
+inline void rs_sw_consts(uint64_t q, uint64_t q_old, uint64_t s2, uint64_t& cs, uint64_t& ch) {
+    cs = (uint64_t)(((unsigned __int128)(s2 % q) << 64) % q);
+    ch = (uint64_t)(((unsigned __int128)((q - q_old % q) % q) * cs) % q);
+}
+
 struct ModupJob {
     const uint64_t* src[MODUP_MAXSRC];   // the inverse transforms' row-pass output, tower st + i
     const uint64_t* stw[MODUP_MAXSRC];   // their paired inverse twiddle tables
@@ -119,6 +129,8 @@ struct ModupJob {
     const uint64_t* dr;
     const uint64_t* tabd;
     const RsTarget* tg;
+    // [2 sc] per target: s2 2^64 and -q_old s2 2^64 mod q_t (rs_sw_consts)
+    const uint64_t* tsw;
     uint64_t q_old, half, qd_neg, pinv, pinvp;
     uint32_t strided;
 };
@@ -312,20 +324,15 @@ GF_HD void modup_drop_unit(const ModupJob& j, const NttGeom& g, uint32_t tile, u
         dv[u] = addmod(gat(gcast(j.dt), gi), shoup(submod(gat(gcast(j.dr), gi), cd, q), j.pinv, j.pinvp, q), q);
     }
 }
-// Rescale mode: target t's prologue value of this thread's unit into cv
-// (below 4 q_t; for an fp64 target as the centred double)
+// Rescale mode: target t's prologue value of this thread's unit into cv,
+// congruent to rs_lazy_value's (below 1.4 q_t + 1; for an fp64 target as the
+// centred double)
 template <uint32_t SP, uint32_t S1>
 GF_HD void modup_rs_unit(const ModupJob& j, uint32_t t, ModupUnit<SP>& x, const uint64_t* dv, uint64_t* cv) {
     static_assert(sizeof(RsTarget) == 7 * 8, "RsTarget layout");
-    RsTarget tg;
+    static_assert(SP + 2 <= 8, "the column sums take at most 8 terms");
     const uint64_t* w = (const uint64_t*)j.tg + 7 * t;
-    tg.q = uniform_at(w, 0);
-    tg.qneg = uniform_at(w, 1);
-    tg.diff = uniform_at(w, 2);
-    tg.new_larger = uniform_at(w, 3);
-    tg.s2 = uniform_at(w, 4);
-    tg.s2p = uniform_at(w, 5);
-    tg.red = uniform_at(w, 6);
+    const uint64_t q = uniform_at(w, 0), qneg = uniform_at(w, 1);
     const bool f64 = uniform_at(j.ttwd, t) != 0;
     const double qd = f64 ? f64_of_bits(uniform_at(j.tqf, 2 * t)) : 0.0;
     const double qinv = f64 ? f64_of_bits(uniform_at(j.tqf, 2 * t + 1)) : 0.0;
@@ -336,8 +343,22 @@ GF_HD void modup_rs_unit(const ModupJob& j, uint32_t t, ModupUnit<SP>& x, const 
             x.lo[u][i] = opaque32(x.lo[u][i]);
             x.hi[u][i] = opaque32(x.hi[u][i]);
         }
-        const U128 acc = dot30<SP>(x.lo[u], x.hi[u], [&](uint32_t s) { return uniform_at(j.tab, s * j.sc + t); });
-        const uint64_t v = rs_lazy_value(dv[u], dv[u] > j.half, tg, acc);
+        // the sources, dv (below q_old < 2^60) and the centring flag as one
+        // sum of SP + 2 terms: REDC leaves it below 1.4 q + 1
+        uint32_t al[SP + 2], ah[SP + 2];
+#pragma unroll
+        for (uint32_t i = 0; i < SP; ++i) {
+            al[i] = x.lo[u][i];
+            ah[i] = x.hi[u][i];
+        }
+        al[SP] = split30_lo(dv[u]);
+        ah[SP] = split30_hi(dv[u]);
+        al[SP + 1] = dv[u] > j.half ? 1u : 0u;
+        ah[SP + 1] = 0;
+        const U128 acc = dot30<SP + 2>(al, ah, [&](uint32_t s) {
+            return s < SP ? uniform_at(j.tab, s * j.sc + t) : uniform_at(j.tsw, 2 * t + (s - SP));
+        });
+        const uint64_t v = redc_lazy(acc, q, qneg);
         cv[u] = f64 ? bits_of_f64(f64_center(f64_from_u52(v, F64_2P52 + 2 * qd), qd, qinv)) : v;
     }
 }

@@ -424,11 +424,12 @@ extern "C" void emu_moddown(uint32_t n, uint32_t K, uint32_t sc, const uint64_t*
     for (uint32_t k = 0; k < K; ++k) stab[k] = paired(sw[k], swp[k], n);
     for (uint32_t t = 0; t < sc; ++t) ttab[t] = paired(ttw[t], ttwp[t], n);
     std::vector<RsTarget> tg(sc);
-    std::vector<uint64_t> tqn(sc), tred(sc), ttw_addr(sc), ttwd(sc, 0), tqf(2 * (size_t)sc);
+    std::vector<uint64_t> tqn(sc), tred(sc), ttw_addr(sc), ttwd(sc, 0), tqf(2 * (size_t)sc), tsw(2 * (size_t)sc);
     for (uint32_t t = 0; t < sc; ++t) {
         const uint64_t q = tq[t];
         tg[t] = RsTarget{q, host_qneg(q), q > q_old ? q - q_old : (q_old - q) % q, q > q_old ? 1u : 0u, s2[t],
                          host_precon(s2[t], q), host_precon(1, q)};
+        rs_sw_consts(q, q_old, s2[t], tsw[2 * t], tsw[2 * t + 1]);
         tqn[t] = host_qneg(q);
         tred[t] = lazy_red_of(q);
         ttw_addr[t] = (uint64_t)(uintptr_t)ttab[t].data();
@@ -465,6 +466,7 @@ extern "C" void emu_moddown(uint32_t n, uint32_t K, uint32_t sc, const uint64_t*
         J.tabd = tabd;
         J.strided = 1;
         J.tg = tg.data();
+        J.tsw = tsw.data();
         J.q_old = q_old;
         J.half = half;
         J.qd_neg = qd_neg;

@@ -325,7 +325,7 @@ GF_HD void modup_drop_unit(const ModupJob& j, const NttGeom& g, uint32_t tile, u
     }
 }
 // Rescale mode: target t's prologue value of this thread's unit into cv,
-// congruent to rs_lazy_value's (below 1.4 q_t + 1; for an fp64 target as the
+// congruent to rs_lazy_value's (below 1.44 q_t + 1; for an fp64 target as the
 // centred double)
 template <uint32_t SP, uint32_t S1>
 GF_HD void modup_rs_unit(const ModupJob& j, uint32_t t, ModupUnit<SP>& x, const uint64_t* dv, uint64_t* cv) {
@@ -344,7 +344,7 @@ GF_HD void modup_rs_unit(const ModupJob& j, uint32_t t, ModupUnit<SP>& x, const 
             x.hi[u][i] = opaque32(x.hi[u][i]);
         }
         // the sources, dv (below q_old < 2^60) and the centring flag as one
-        // sum of SP + 2 terms: REDC leaves it below 1.4 q + 1
+        // sum of SP + 2 terms: REDC leaves it below 1.44 q + 1
         uint32_t al[SP + 2], ah[SP + 2];
 #pragma unroll
         for (uint32_t i = 0; i < SP; ++i) {

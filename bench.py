@@ -757,6 +757,16 @@ def main():
     dom = max(kernels, key=lambda r: r["ms"])
     achieved = dom["bytes"] / (dom["ms"] / 1e3) / 1e9
     traffic = read_traffic(dom["stage"] + ":" + dom["kernel"], B)
+
+    def roofline_of(k):
+        a = k["bytes"] / (k["ms"] / 1e3) / 1e9
+        return {"bound": "hbm", "kernel": k["stage"] + ":" + k["kernel"], "achieved": round(a, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(a / HBM_PEAK_GBS, 4),
+                "traffic": read_traffic(k["stage"] + ":" + k["kernel"], B), "algorithmic_bytes": k["bytes"],
+                "launch_ms": round(k["ms"], 4)}
+    # the keyswitch kernel on its own: the Q rows' fused row pass + inner product
+    ksi_q = [k for k in kernels if k["stage"] == "rowpass_inner_q"]
+    roofline_ksi = roofline_of(ksi_q[0]) if ksi_q else None
     # the whole operation against HBM: compulsory bytes per EvalMult+Rescale
     # (inputs 4L and outputs 2(L - 1) towers once, the key's 2 dnum (L + K)
     # towers once per batch) at the measured rate, and the counter bytes the
@@ -885,6 +895,7 @@ def main():
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algorithmic_bytes": dom["bytes"], "launch_ms": round(dom["ms"], 4)},
+        "roofline_keyswitch": roofline_ksi,
         "cfg1": cfg1,
         "reference_sequence": refseq,
         "chain": chain,
